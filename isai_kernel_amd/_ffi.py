@@ -1,0 +1,177 @@
+"""ctypes binding of libisai_amd.so (the C ABI declared in include/*.h).
+
+The library is the product; this module only declares argument types, turns
+non-zero status codes into exceptions and moves numpy arrays to and from device
+memory.  There is no Python or CPU fallback: importing fails loudly when the
+shared library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libisai_amd.so"
+
+if not LIB_PATH.exists():
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+    )
+
+lib = C.CDLL(str(LIB_PATH))
+
+P = C.c_void_p
+I = C.c_int
+SZ = C.c_size_t
+U64 = C.c_uint64
+D = C.c_double
+F = C.c_float
+
+# name -> argument types (every function returns int status unless listed in _RESTYPE)
+SIGNATURES: dict[str, list] = {
+    # isai_types.h
+    "isai_last_error": [],
+    "isai_status_string": [I],
+    # isai_runtime.h
+    "isai_device_count": [P],
+    "isai_device_info": [P, I, P, P],
+    "isai_set_device": [I],
+    "isai_device_malloc": [P, SZ],
+    "isai_device_free": [P],
+    "isai_host_free": [P],
+    "isai_memcpy_htod": [P, P, SZ],
+    "isai_memcpy_dtoh": [P, P, SZ],
+    "isai_memcpy_dtod_async": [P, P, SZ, P],
+    "isai_fill_u32_async": [P, C.c_uint32, SZ, P],
+    "isai_fill_f64_async": [P, D, SZ, P],
+    "isai_device_synchronize": [],
+    "isai_stream_create": [P],
+    "isai_stream_destroy": [P],
+    "isai_stream_synchronize": [P],
+    "isai_event_create": [P],
+    "isai_event_destroy": [P],
+    "isai_event_record": [P, P],
+    "isai_event_synchronize": [P],
+    "isai_event_elapsed_ms": [P, P, P],
+    # isai_gen.h
+    "isai_gen_problem_count": [I, I, P, P],
+    "isai_gen_problem_fill": [I, I, U64, P, P, P],
+    "isai_gen_normal": [U64, SZ, P],
+    # csr2bsr.h
+    "isai_csr_to_bsr_count": [I, I, P, P, P, P],
+    "isai_csr_to_bsr_fill": [I, I, P, P, P, P, P, P],
+    # bsr2csr.h
+    "isai_bsr_to_csr": [I, I, I, P, P, P, P, P, P, P],
+    # bcsc2bcsr.h
+    "isai_tri_bcsr_pattern_count": [I, P, P, I, P],
+    "isai_tri_bcsr_pattern": [I, P, P, I, P, P],
+    "isai_map_bcsc2bcsr": [I, P, P, P, P, P],
+    "isai_reorder_bcsc_to_bcsr": [I, I, P, P, P, P],
+    # isai_bsr.h
+    "isai_bcsc_pattern_count": [I, P, P, I, P],
+    "isai_bcsc_pattern": [I, P, P, I, P, P],
+    # bsr_tensor_SpMV.h
+    "isai_bsr_spmv": [I, I, I, P, P, P, P, P, P],
+    "isai_bsr_spmv_prec": [I, I, I, I, P, P, P, P, P, P],
+    "isai_convert_values": [SZ, I, P, P, P],
+    "isai_apply": [I, I, I, P, P, P, I, P, P, P, I, P, P, P, P],
+    "isai_bsr_spmv_bench": [P, P, P, I, I, I, P, P, I, I, P],
+}
+_RESTYPE = {"isai_last_error": C.c_char_p, "isai_status_string": C.c_char_p}
+
+for _name, _args in SIGNATURES.items():
+    _fn = getattr(lib, _name)
+    _fn.argtypes = _args
+    _fn.restype = _RESTYPE.get(_name, I)
+
+
+class IsaiError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        msg = lib.isai_last_error().decode(errors="replace")
+        name = lib.isai_status_string(status).decode()
+        super().__init__(f"{where}: {name} ({status}) {msg}")
+
+
+def check(status: int, where: str = "isai") -> int:
+    if status != 0:
+        raise IsaiError(status, where)
+    return status
+
+
+def call(name: str, *args) -> int:
+    """Call a C-ABI function and raise IsaiError on a non-zero status."""
+    return check(getattr(lib, name)(*args), name)
+
+
+def ptr(a) -> int | None:
+    """Address of a numpy array (host) or DeviceArray (device); None for None."""
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    return int(a)
+
+
+class DeviceArray:
+    """A device allocation owned from Python (hipMalloc through the C ABI)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = P()
+        call("isai_device_malloc", C.byref(p), max(self.nbytes, 1))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if a.nbytes:
+            call("isai_memcpy_htod", d.ptr, a.ctypes.data, a.nbytes)
+        return d
+
+    @classmethod
+    def zeros(cls, shape, dtype) -> "DeviceArray":
+        return cls.from_numpy(np.zeros(shape, dtype))
+
+    def to_numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            call("isai_memcpy_dtoh", out.ctypes.data, self.ptr, self.nbytes)
+        return out
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        call("isai_memcpy_htod", self.ptr, a.ctypes.data, a.nbytes)
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            lib.isai_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def declared_symbols(include_dir: Path) -> list[str]:
+    """Every isai_* function declared in the public headers."""
+    import re
+
+    names = set()
+    for h in sorted(Path(include_dir).glob("*.h")):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(isai_\w+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return sorted(names)

@@ -1,0 +1,354 @@
+// spmv.hip -- BSR SpMV, ISAI apply, reduced-precision factor storage, BCSC->BCSR
+// value permutation and BSR->CSR expansion for gfx950.
+//
+// bs = 4 layout: one 4x4 FP64 block = 128 B, row-major.  The kernels stream the
+// value array with 16-byte lane loads so that the lanes serving one block row
+// read whole 128-byte lines:
+//   FP64  8 lanes per block row, lane s holds row s/2, columns 2*(s%2)..+1
+//   FP32  4 lanes per block row, lane s holds row s (4 floats = 16 B)
+//   BF16  2 lanes per block row, lane s holds rows 2s, 2s+1 (8 bf16 = 16 B)
+// Values are read with non-temporal loads (streamed once per product); the x
+// gathers use the normal cached path, since each x block is reused by every
+// block row that references it.
+#include <hip/hip_bf16.h>
+
+#include "bsr2csr.h"
+#include "bcsc2bcsr.h"
+#include "bsr_tensor_SpMV.h"
+#include "common.hpp"
+#include "isai_runtime.h"
+
+namespace isai {
+
+constexpr int kSpmvBlock = 256;
+
+// ---------------------------------------------------------------- FP64, bs = 4
+template <int UNROLL>
+__global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f64_kernel(
+    int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+    const v2d *__restrict__ val2, const v2d *__restrict__ x2, double *__restrict__ y) {
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const int br = wg * (kSpmvBlock / 8) + (threadIdx.x >> 3);
+    const int s = threadIdx.x & 7;
+    const int h = s & 1;
+    double acc0 = 0.0, acc1 = 0.0;
+    if (br < nb) {
+        const int rs = rp[br], re = rp[br + 1];
+        int k = rs;
+        for (; k + UNROLL <= re; k += UNROLL) {
+            int c[UNROLL];
+            v2d a[UNROLL], xv[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) c[u] = ci[k + u];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+                a[u] = __builtin_nontemporal_load(&val2[(size_t)(k + u) * 8 + s]);
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) xv[u] = x2[(size_t)c[u] * 2 + h];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                acc0 = fma(a[u].x, xv[u].x, acc0);
+                acc1 = fma(a[u].y, xv[u].y, acc1);
+            }
+        }
+        for (; k < re; ++k) {
+            const int c = ci[k];
+            const v2d a = __builtin_nontemporal_load(&val2[(size_t)k * 8 + s]);
+            const v2d xv = x2[(size_t)c * 2 + h];
+            acc0 = fma(a.x, xv.x, acc0);
+            acc1 = fma(a.y, xv.y, acc1);
+        }
+    }
+    double sum = acc0 + acc1;
+    sum += __shfl_xor(sum, 1);
+    if (br < nb && h == 0) y[(size_t)br * 4 + (s >> 1)] = sum;
+}
+
+// ---------------------------------------------------------------- FP32, bs = 4
+template <int UNROLL>
+__global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f32_kernel(
+    int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+    const v4f *__restrict__ val4, const v2d *__restrict__ x2, double *__restrict__ y) {
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const int br = wg * (kSpmvBlock / 4) + (threadIdx.x >> 2);
+    const int r = threadIdx.x & 3;
+    if (br >= nb) return;
+    float acc = 0.0f;
+    const int rs = rp[br], re = rp[br + 1];
+    for (int k = rs; k < re; ++k) {
+        const int c = ci[k];
+        const v4f a = __builtin_nontemporal_load(&val4[(size_t)k * 4 + r]);
+        const v2d x01 = x2[(size_t)c * 2 + 0];
+        const v2d x23 = x2[(size_t)c * 2 + 1];
+        acc = fmaf(a.x, (float)x01.x, acc);
+        acc = fmaf(a.y, (float)x01.y, acc);
+        acc = fmaf(a.z, (float)x23.x, acc);
+        acc = fmaf(a.w, (float)x23.y, acc);
+    }
+    y[(size_t)br * 4 + r] = (double)acc;
+}
+
+// ---------------------------------------------------------------- BF16, bs = 4
+__device__ __forceinline__ float bf16_lo(unsigned int w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(unsigned int w) { return __uint_as_float(w & 0xffff0000u); }
+
+__global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_bf16_kernel(
+    int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+    const v4u *__restrict__ val8, const v2d *__restrict__ x2, double *__restrict__ y) {
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const int br = wg * (kSpmvBlock / 2) + (threadIdx.x >> 1);
+    const int h = threadIdx.x & 1;  // rows 2h, 2h+1
+    if (br >= nb) return;
+    float acc0 = 0.0f, acc1 = 0.0f;
+    const int rs = rp[br], re = rp[br + 1];
+    for (int k = rs; k < re; ++k) {
+        const int c = ci[k];
+        const v4u w = __builtin_nontemporal_load(&val8[(size_t)k * 2 + h]);
+        const v2d x01 = x2[(size_t)c * 2 + 0];
+        const v2d x23 = x2[(size_t)c * 2 + 1];
+        const float f0 = (float)x01.x, f1 = (float)x01.y, f2 = (float)x23.x, f3 = (float)x23.y;
+        acc0 = fmaf(bf16_lo(w.x), f0, acc0);
+        acc0 = fmaf(bf16_hi(w.x), f1, acc0);
+        acc0 = fmaf(bf16_lo(w.y), f2, acc0);
+        acc0 = fmaf(bf16_hi(w.y), f3, acc0);
+        acc1 = fmaf(bf16_lo(w.z), f0, acc1);
+        acc1 = fmaf(bf16_hi(w.z), f1, acc1);
+        acc1 = fmaf(bf16_lo(w.w), f2, acc1);
+        acc1 = fmaf(bf16_hi(w.w), f3, acc1);
+    }
+    y[(size_t)br * 4 + 2 * h] = (double)acc0;
+    y[(size_t)br * 4 + 2 * h + 1] = (double)acc1;
+}
+
+// ---------------------------------------------------- generic bs (1..8), FP64
+__global__ void bsr_spmv_generic_kernel(int nb, int bs, const int *__restrict__ rp,
+                                        const int *__restrict__ ci, const double *__restrict__ val,
+                                        const double *__restrict__ x, double *__restrict__ y) {
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= (long long)nb * bs) return;
+    const int br = (int)(row / bs), r = (int)(row % bs), bs2 = bs * bs;
+    double sum = 0.0;
+    for (int k = rp[br]; k < rp[br + 1]; ++k) {
+        const double *blk = val + (size_t)k * bs2 + r * bs;
+        const double *xb = x + (size_t)ci[k] * bs;
+        for (int c = 0; c < bs; ++c) sum = fma(blk[c], xb[c], sum);
+    }
+    y[row] = sum;
+}
+
+// ------------------------------------------------ value conversion (RNE)
+__device__ __forceinline__ unsigned short f32_to_bf16_rne(float f) {
+    unsigned int u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (unsigned short)((u | 0x00400000u) >> 16);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__global__ void convert_f32_kernel(size_t n, const double *__restrict__ src, float *__restrict__ dst) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
+__global__ void convert_bf16_kernel(size_t n, const double *__restrict__ src, unsigned short *__restrict__ dst) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = f32_to_bf16_rne((float)src[i]);
+}
+
+// ------------------------------------------------ BCSC -> BCSR value permutation
+// One thread per (entry, element): reads are fully coalesced, each block's
+// writes form one contiguous bs*bs run.
+__global__ void reorder_kernel(long long total, int bs2, const int *__restrict__ map,
+                               const double *__restrict__ src, double *__restrict__ dst) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const long long k = i / bs2;
+    const int e = (int)(i - k * bs2);
+    dst[(size_t)map[k] * bs2 + e] = src[i];
+}
+
+// ------------------------------------------------ BSR -> CSR (all block entries)
+// One thread per scalar row: row r = br*bs + lr holds its blocks in block-column
+// order and lc = 0..bs-1 inside each block; nnz = nnzb*bs*bs.
+__global__ void bsr2csr_kernel(int nb, int bs, const int *__restrict__ brp, const int *__restrict__ bci,
+                               const double *__restrict__ bv, int *__restrict__ rp,
+                               int *__restrict__ ci, double *__restrict__ v) {
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= (long long)nb * bs) return;
+    const int br = (int)(row / bs), lr = (int)(row % bs), bs2 = bs * bs;
+    const int len = brp[br + 1] - brp[br];
+    long long out = (long long)brp[br] * bs2 + (long long)lr * len * bs;
+    rp[row + 1] = (int)(out + (long long)len * bs);
+    if (row == 0) rp[0] = 0;
+    for (int q = brp[br]; q < brp[br + 1]; ++q) {
+        const int c0 = bci[q] * bs;
+        const double *blk = bv + (size_t)q * bs2 + lr * bs;
+        for (int lc = 0; lc < bs; ++lc, ++out) {
+            ci[out] = c0 + lc;
+            v[out] = blk[lc];
+        }
+    }
+}
+
+int spmv_launch(int mb, int nnzb, int bs, isai_precision_t prec, const int *rp, const int *ci,
+                const void *val, const double *x, double *y, hipStream_t st) {
+    ISAI_REQUIRE(mb >= 0 && nnzb >= 0, "bsr_spmv: negative size (mb=%d nnzb=%d)", mb, nnzb);
+    ISAI_REQUIRE(bs >= 1 && bs <= 8, "bsr_spmv: bs=%d outside 1..8", bs);
+    if (mb == 0) return ISAI_OK;
+    ISAI_REQUIRE(rp && ci && val && x && y, "bsr_spmv: null device pointer");
+    if (bs == 4) {
+        if (prec == ISAI_PREC_FP64) {
+            const unsigned g = grid_for(mb, kSpmvBlock / 8);
+            bsr4_spmv_f64_kernel<2><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v2d *)val,
+                                                              (const v2d *)x, y);
+        } else if (prec == ISAI_PREC_FP32) {
+            const unsigned g = grid_for(mb, kSpmvBlock / 4);
+            bsr4_spmv_f32_kernel<1><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v4f *)val,
+                                                              (const v2d *)x, y);
+        } else if (prec == ISAI_PREC_BF16) {
+            const unsigned g = grid_for(mb, kSpmvBlock / 2);
+            bsr4_spmv_bf16_kernel<<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v4u *)val,
+                                                            (const v2d *)x, y);
+        } else {
+            set_error("bsr_spmv: unknown precision %d", (int)prec);
+            return ISAI_ERR_INVALID_ARG;
+        }
+    } else {
+        if (prec != ISAI_PREC_FP64) {
+            set_error("bsr_spmv: reduced precision needs bs = 4 (got %d)", bs);
+            return ISAI_ERR_UNSUPPORTED;
+        }
+        const unsigned g = grid_for((long long)mb * bs, 256);
+        bsr_spmv_generic_kernel<<<g, 256, 0, st>>>(mb, bs, rp, ci, (const double *)val, x, y);
+    }
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+}  // namespace isai
+
+using namespace isai;
+
+extern "C" {
+
+int isai_bsr_spmv(int mb, int nnzb, int bs, const int *d_rowptr, const int *d_colind,
+                  const double *d_val, const double *d_x, double *d_y, isai_stream_t stream) {
+    return spmv_launch(mb, nnzb, bs, ISAI_PREC_FP64, d_rowptr, d_colind, d_val, d_x, d_y,
+                       (hipStream_t)stream);
+}
+
+int isai_bsr_spmv_prec(int mb, int nnzb, int bs, isai_precision_t prec, const int *d_rowptr,
+                       const int *d_colind, const void *d_val, const double *d_x, double *d_y,
+                       isai_stream_t stream) {
+    return spmv_launch(mb, nnzb, bs, prec, d_rowptr, d_colind, d_val, d_x, d_y, (hipStream_t)stream);
+}
+
+int isai_convert_values(size_t count, isai_precision_t prec, const double *d_src, void *d_dst,
+                        isai_stream_t stream) {
+    if (count == 0) return ISAI_OK;
+    ISAI_REQUIRE(d_src && d_dst, "isai_convert_values: null pointer");
+    size_t g = (count + 255) / 256;
+    if (g > 65536) g = 65536;
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == ISAI_PREC_FP32) {
+        convert_f32_kernel<<<(unsigned)g, 256, 0, st>>>(count, d_src, (float *)d_dst);
+    } else if (prec == ISAI_PREC_BF16) {
+        convert_bf16_kernel<<<(unsigned)g, 256, 0, st>>>(count, d_src, (unsigned short *)d_dst);
+    } else if (prec == ISAI_PREC_FP64) {
+        ISAI_HIP(hipMemcpyAsync(d_dst, d_src, count * sizeof(double), hipMemcpyDeviceToDevice, st));
+        return ISAI_OK;
+    } else {
+        set_error("isai_convert_values: unknown precision %d", (int)prec);
+        return ISAI_ERR_INVALID_ARG;
+    }
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+int isai_apply(int mb, int bs, isai_precision_t prec, const int *d_Lrow, const int *d_Lcol,
+               const void *d_Lval, int nnzbL, const int *d_Urow, const int *d_Ucol,
+               const void *d_Uval, int nnzbU, const double *d_b, double *d_y, double *d_x,
+               isai_stream_t stream) {
+    hipStream_t st = (hipStream_t)stream;
+    int s = spmv_launch(mb, nnzbL, bs, prec, d_Lrow, d_Lcol, d_Lval, d_b, d_y, st);
+    if (s != ISAI_OK) return s;
+    return spmv_launch(mb, nnzbU, bs, prec, d_Urow, d_Ucol, d_Uval, d_y, d_x, st);
+}
+
+int isai_reorder_bcsc_to_bcsr(int nnzb, int bs, const int *d_map, const double *d_src,
+                              double *d_dst, isai_stream_t stream) {
+    ISAI_REQUIRE(nnzb >= 0 && bs >= 1 && bs <= 8, "isai_reorder_bcsc_to_bcsr: bad size");
+    if (nnzb == 0) return ISAI_OK;
+    ISAI_REQUIRE(d_map && d_src && d_dst, "isai_reorder_bcsc_to_bcsr: null pointer");
+    const long long total = (long long)nnzb * bs * bs;
+    reorder_kernel<<<grid_for(total, 256), 256, 0, (hipStream_t)stream>>>(total, bs * bs, d_map, d_src, d_dst);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+int isai_bsr_to_csr(int mb, int nnzb, int bs, const int *d_brp, const int *d_bci, const double *d_bv,
+                    int *d_rp, int *d_ci, double *d_v, isai_stream_t stream) {
+    ISAI_REQUIRE(mb >= 0 && nnzb >= 0 && bs >= 1 && bs <= 8, "isai_bsr_to_csr: bad size");
+    ISAI_REQUIRE((long long)nnzb * bs * bs < (1LL << 31), "isai_bsr_to_csr: nnz overflows int32");
+    if (mb == 0) return ISAI_OK;
+    const long long rows = (long long)mb * bs;
+    bsr2csr_kernel<<<grid_for(rows, 256), 256, 0, (hipStream_t)stream>>>(mb, bs, d_brp, d_bci, d_bv,
+                                                                          d_rp, d_ci, d_v);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+int isai_bsr_spmv_bench(const int *bsrRowPtr, const int *bsrColIdx, const double *bsrVal, int mb,
+                        int nnzb, int bdim, const double *Val_B, double *Val_C, int warmup,
+                        int iters, double *avg_ms) {
+    ISAI_REQUIRE(bsrRowPtr && bsrColIdx && bsrVal && Val_B && Val_C && avg_ms,
+                 "isai_bsr_spmv_bench: null pointer");
+    ISAI_REQUIRE(mb > 0 && nnzb >= 0 && bdim >= 1 && bdim <= 8 && iters > 0 && warmup >= 0,
+                 "isai_bsr_spmv_bench: bad size");
+    const size_t bs2 = (size_t)bdim * bdim;
+    int *d_rp = nullptr, *d_ci = nullptr;
+    double *d_v = nullptr, *d_x = nullptr, *d_y = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int status = ISAI_OK;
+    float ms = 0.0f;
+#define BENCH_TRY(call)                                                                \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess) { status = hip_fail(_e, #call, __FILE__, __LINE__); goto done; } \
+    } while (0)
+    BENCH_TRY(hipMalloc(&d_rp, sizeof(int) * (mb + 1)));
+    BENCH_TRY(hipMalloc(&d_ci, sizeof(int) * (nnzb ? nnzb : 1)));
+    BENCH_TRY(hipMalloc(&d_v, sizeof(double) * (nnzb ? nnzb : 1) * bs2));
+    BENCH_TRY(hipMalloc(&d_x, sizeof(double) * mb * bdim));
+    BENCH_TRY(hipMalloc(&d_y, sizeof(double) * mb * bdim));
+    BENCH_TRY(hipMemcpy(d_rp, bsrRowPtr, sizeof(int) * (mb + 1), hipMemcpyHostToDevice));
+    BENCH_TRY(hipMemcpy(d_ci, bsrColIdx, sizeof(int) * nnzb, hipMemcpyHostToDevice));
+    BENCH_TRY(hipMemcpy(d_v, bsrVal, sizeof(double) * nnzb * bs2, hipMemcpyHostToDevice));
+    BENCH_TRY(hipMemcpy(d_x, Val_B, sizeof(double) * mb * bdim, hipMemcpyHostToDevice));
+    BENCH_TRY(hipEventCreate(&e0));
+    BENCH_TRY(hipEventCreate(&e1));
+    for (int i = 0; i < warmup && status == ISAI_OK; ++i)
+        status = spmv_launch(mb, nnzb, bdim, ISAI_PREC_FP64, d_rp, d_ci, d_v, d_x, d_y, nullptr);
+    if (status != ISAI_OK) goto done;
+    BENCH_TRY(hipDeviceSynchronize());
+    BENCH_TRY(hipEventRecord(e0, nullptr));
+    for (int i = 0; i < iters && status == ISAI_OK; ++i)
+        status = spmv_launch(mb, nnzb, bdim, ISAI_PREC_FP64, d_rp, d_ci, d_v, d_x, d_y, nullptr);
+    if (status != ISAI_OK) goto done;
+    BENCH_TRY(hipEventRecord(e1, nullptr));
+    BENCH_TRY(hipEventSynchronize(e1));
+    BENCH_TRY(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = (double)ms / iters;
+    BENCH_TRY(hipMemcpy(Val_C, d_y, sizeof(double) * mb * bdim, hipMemcpyDeviceToHost));
+#undef BENCH_TRY
+done:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d_rp);
+    (void)hipFree(d_ci);
+    (void)hipFree(d_v);
+    (void)hipFree(d_x);
+    (void)hipFree(d_y);
+    return status;
+}
+
+}  // extern "C"

@@ -1,0 +1,787 @@
+/*
+ * oracle.c -- FP64 CPU oracle (TEST INFRASTRUCTURE ONLY; see oracle.h).
+ *
+ * How each function is pinned (the reference ships no tests, golden vectors or
+ * matrices, so every check below is against mathematics or brute force, run by
+ * tests/test_oracle.py):
+ *   ora_csr2bsr_*        hand-computed 8x8 fixture (tests/golden/csr2bsr_small.json),
+ *                        duplicate summing, -0.0 first-entry rule, round trip with
+ *                        ora_bsr2csr on random patterns (brute-force dense compare).
+ *   ora_bsr2csr          dense expansion equality (closed form).
+ *   ora_bcsc_pattern     = transpose of the tri pattern with forced diagonal,
+ *                        checked against a brute-force dense mask.
+ *   ora_tri_bcsr_pattern dense mask; counts equal bcsc counts (main.cu:537-546).
+ *   ora_map_bcsc2bcsr    permutation property + (row,col) equality; mismatch error.
+ *   ora_pairs_*          brute-force enumeration over dense block masks.
+ *   ora_invert_block     A * Ainv = I; numpy-free closed form on triangular blocks.
+ *   ora_build_dinv       D * Dinv = I for the unit-lower / upper parts.
+ *   ora_isai_lower/upper defining invariant (L M)(i,j) = delta_ij I on the
+ *                        pattern; closed form on block-bidiagonal L; equality with
+ *                        the exact inverse when the pattern is full (dense).
+ *   ora_bsrilu0          (L U)_IJ = A_IJ on every stored block; dense scalar
+ *                        Gaussian elimination with pattern dropping (n <= 64);
+ *                        exact LU on a full pattern; block-tridiagonal closed form.
+ *   ora_levels           closed forms i+j (2-D 5-pt), i+j+k (3-D 7-pt),
+ *                        i+2j (2-D 9-pt), i+2j+4k (3-D 27-pt).
+ *   ora_bsr_spmv         dense mat-vec.
+ *   ora_bsr_spmv_f32acc  float64 dense mat-vec of the rounded values (tolerance).
+ *   ora_bsr_sptrsv_lu    L (U x) = b residual with dense factors.
+ *   ora_csr_sptrsv_lu    equals ora_bsr_sptrsv_lu through ora_bsr2csr.
+ *   ora_bicgstab/fgmres  true residual <= tol; equality with a direct dense solve.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int ora_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+static int cmp_int(const void *a, const void *b) {
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+/* ------------------------------------------------------------------ csr2bsr */
+int ora_csr2bsr_count(int n, int bs, const int *rp, const int *ci, int *nb, int *nnzb) {
+    if (bs <= 0 || n < 0 || n % bs != 0) return ORA_ERR_INVALID;
+    int nbl = n / bs;
+    int *tmp = (int *)malloc(sizeof(int) * (size_t)(bs > 0 ? 1 : 1));
+    long total = 0;
+    int cap = 0;
+    for (int br = 0; br < nbl; ++br) {
+        int cnt = rp[(br + 1) * bs] - rp[br * bs];
+        if (cnt > cap) { cap = cnt; tmp = (int *)realloc(tmp, sizeof(int) * (size_t)cap); }
+        int k = 0;
+        for (int p = rp[br * bs]; p < rp[(br + 1) * bs]; ++p) tmp[k++] = ci[p] / bs;
+        qsort(tmp, (size_t)k, sizeof(int), cmp_int);
+        int u = 0;
+        for (int q = 0; q < k; ++q) if (q == 0 || tmp[q] != tmp[q - 1]) ++u;
+        total += u;
+    }
+    free(tmp);
+    *nb = nbl;
+    *nnzb = (int)total;
+    return ORA_OK;
+}
+
+int ora_csr2bsr_fill(int n, int bs, const int *rp, const int *ci, const double *v,
+                     int *brp, int *bci, double *bv) {
+    if (bs <= 0 || n < 0 || n % bs != 0) return ORA_ERR_INVALID;
+    int nbl = n / bs, bs2 = bs * bs;
+    int *tmp = NULL, *seen = NULL;
+    int cap = 0;
+    brp[0] = 0;
+    for (int br = 0; br < nbl; ++br) {
+        int cnt = rp[(br + 1) * bs] - rp[br * bs];
+        if (cnt > cap) {
+            cap = cnt;
+            tmp = (int *)realloc(tmp, sizeof(int) * (size_t)cap);
+            seen = (int *)realloc(seen, sizeof(int) * (size_t)cap);
+        }
+        int k = 0;
+        for (int p = rp[br * bs]; p < rp[(br + 1) * bs]; ++p) tmp[k++] = ci[p] / bs;
+        qsort(tmp, (size_t)k, sizeof(int), cmp_int);
+        int u = 0;
+        for (int q = 0; q < k; ++q) if (q == 0 || tmp[q] != tmp[u - 1]) tmp[u++] = tmp[q];
+        int base = brp[br];
+        for (int q = 0; q < u; ++q) {
+            bci[base + q] = tmp[q];
+            seen[q] = 0;
+            for (int e = 0; e < bs2; ++e) bv[(size_t)(base + q) * bs2 + e] = 0.0;
+        }
+        brp[br + 1] = base + u;
+        /* csr2bsr.h:41-62: walk local rows then CSR order; a block's first entry
+         * is assigned into a zero-filled block, later entries are added. */
+        for (int lr = 0; lr < bs; ++lr) {
+            int row = br * bs + lr;
+            for (int p = rp[row]; p < rp[row + 1]; ++p) {
+                int bc = ci[p] / bs, lc = ci[p] % bs;
+                int *hit = (int *)bsearch(&bc, tmp, (size_t)u, sizeof(int), cmp_int);
+                int q = (int)(hit - tmp);
+                double *blk = bv + (size_t)(base + q) * bs2;
+                if (!seen[q]) { blk[lr * bs + lc] = v[p]; seen[q] = 1; }
+                else blk[lr * bs + lc] += v[p];
+            }
+        }
+    }
+    free(tmp);
+    free(seen);
+    return ORA_OK;
+}
+
+/* ------------------------------------------------------------------ bsr2csr */
+int ora_bsr2csr(int nb, int bs, const int *brp, const int *bci, const double *bv,
+                int *rp, int *ci, double *v) {
+    int bs2 = bs * bs;
+    rp[0] = 0;
+    for (int br = 0; br < nb; ++br) {
+        int len = brp[br + 1] - brp[br];
+        for (int lr = 0; lr < bs; ++lr) {
+            int row = br * bs + lr;
+            int out = brp[br] * bs2 + lr * len * bs;
+            rp[row + 1] = out + len * bs;
+            for (int q = brp[br]; q < brp[br + 1]; ++q) {
+                for (int lc = 0; lc < bs; ++lc) {
+                    ci[out] = bci[q] * bs + lc;
+                    v[out] = bv[(size_t)q * bs2 + lr * bs + lc];
+                    ++out;
+                }
+            }
+        }
+    }
+    return ORA_OK;
+}
+
+/* ---------------------------------------------------------------- patterns */
+int ora_bcsc_pattern(int nb, const int *Arow, const int *Acol, int lower,
+                     int *Mbrow, int *Mcol, int cap, int *nnz) {
+    /* Column j holds the rows br with (br, j) kept, plus j itself
+     * (isai_bsr.h:257-269), sorted and deduplicated (isai_bsr.h:272-276). */
+    int *cnt = (int *)calloc((size_t)nb + 1, sizeof(int));
+    for (int br = 0; br < nb; ++br) {
+        int has_diag = 0;
+        for (int p = Arow[br]; p < Arow[br + 1]; ++p) {
+            int bc = Acol[p];
+            if (lower ? (bc > br) : (bc < br)) continue;
+            if (bc == br) has_diag = 1;
+            cnt[bc]++;
+        }
+        if (!has_diag) cnt[br]++;
+    }
+    Mbrow[0] = 0;
+    for (int j = 0; j < nb; ++j) Mbrow[j + 1] = Mbrow[j] + cnt[j];
+    *nnz = Mbrow[nb];
+    if (Mbrow[nb] > cap) { free(cnt); return ORA_ERR_CAPACITY; }
+    int *fill = (int *)malloc(sizeof(int) * (size_t)(nb + 1));
+    memcpy(fill, Mbrow, sizeof(int) * (size_t)(nb + 1));
+    /* Rows are visited in ascending order, so every column receives its rows in
+     * ascending order; the forced diagonal is inserted at its sorted place. */
+    for (int br = 0; br < nb; ++br) {
+        int has_diag = 0;
+        for (int p = Arow[br]; p < Arow[br + 1]; ++p) {
+            int bc = Acol[p];
+            if (lower ? (bc > br) : (bc < br)) continue;
+            if (bc == br) has_diag = 1;
+            Mcol[fill[bc]++] = br;
+        }
+        if (!has_diag) Mcol[fill[br]++] = br;
+    }
+    /* Columns of A within a row may be unsorted in a general BSR input; sort
+     * each column to keep the reference's ordering guarantee. */
+    for (int j = 0; j < nb; ++j)
+        qsort(Mcol + Mbrow[j], (size_t)(Mbrow[j + 1] - Mbrow[j]), sizeof(int), cmp_int);
+    free(fill);
+    free(cnt);
+    return ORA_OK;
+}
+
+int ora_tri_bcsr_pattern(int nb, const int *Arow, const int *Acol, int lower,
+                         int *Trow, int *Tcol, int cap, int *nnz) {
+    int k = 0;
+    Trow[0] = 0;
+    for (int r = 0; r < nb; ++r) {
+        for (int p = Arow[r]; p < Arow[r + 1]; ++p) {
+            int c = Acol[p];
+            int keep = lower ? (c <= r) : (c >= r);
+            if (keep) {
+                if (k < cap) Tcol[k] = c;
+                ++k;
+            }
+        }
+        Trow[r + 1] = k;
+    }
+    *nnz = k;
+    return k > cap ? ORA_ERR_CAPACITY : ORA_OK;
+}
+
+int ora_map_bcsc2bcsr(int nb, const int *colptr, const int *rowind,
+                      const int *rowptr, const int *colind, int *map) {
+    for (int c = 0; c < nb; ++c) {
+        for (int k = colptr[c]; k < colptr[c + 1]; ++k) {
+            int r = rowind[k], found = -1;
+            for (int p = rowptr[r]; p < rowptr[r + 1]; ++p)
+                if (colind[p] == c) { found = p; break; }
+            if (found < 0) return ORA_ERR_PATTERN;
+            map[k] = found;
+        }
+    }
+    return ORA_OK;
+}
+
+void ora_reorder_bcsc_to_bcsr(int nnzb, int bs, const int *map, const double *src, double *dst) {
+    int bs2 = bs * bs;
+    for (int k = 0; k < nnzb; ++k)
+        memcpy(dst + (size_t)map[k] * bs2, src + (size_t)k * bs2, sizeof(double) * (size_t)bs2);
+}
+
+/* ------------------------------------------------------------------- pairs */
+static int pairs_common(int nb, const int *Arow, const int *Acol, const int *Mbrow,
+                        const int *Mcol, int *PairPtr, int *PairPLocal, int *PairSrc,
+                        int cap, int *total, int upper) {
+    int Mnnz = Mbrow[nb];
+    /* Merge of A's row t with S_j (isai_bsr.h:1403-1418): count pass. */
+    long q = 0;
+    for (int j = 0; j < nb; ++j) {
+        int start = Mbrow[j], end = Mbrow[j + 1];
+        for (int idx = 0; idx < end - start; ++idx) {
+            int t = Mcol[start + idx];
+            int iA = Arow[t], a1 = Arow[t + 1], iS = start, cnt = 0;
+            while (iA < a1 && iS < end) {
+                int colA = Acol[iA], colS = Mcol[iS];
+                if (colA == colS) {
+                    int pl = iS - start;
+                    if (upper ? (pl > idx) : (pl < idx)) ++cnt;
+                    ++iA; ++iS;
+                } else if (colA < colS) ++iA;
+                else ++iS;
+            }
+            PairPtr[start + idx] = (int)q;
+            q += cnt;
+        }
+    }
+    PairPtr[Mnnz] = (int)q;
+    *total = (int)q;
+    if (q > cap) return ORA_ERR_CAPACITY;
+    /* Fill pass (isai_bsr.h:1448-1472). */
+    for (int j = 0; j < nb; ++j) {
+        int start = Mbrow[j], end = Mbrow[j + 1];
+        for (int idx = 0; idx < end - start; ++idx) {
+            int t = Mcol[start + idx];
+            int iA = Arow[t], a1 = Arow[t + 1], iS = start;
+            int out = PairPtr[start + idx];
+            int n_here = PairPtr[start + idx + 1] - out;
+            int w = 0;
+            while (iA < a1 && iS < end) {
+                int colA = Acol[iA], colS = Mcol[iS];
+                if (colA == colS) {
+                    int pl = iS - start;
+                    if (upper ? (pl > idx) : (pl < idx)) {
+                        /* upper: store in descending p_local order */
+                        int slot = upper ? (out + n_here - 1 - w) : (out + w);
+                        PairPLocal[slot] = pl;
+                        PairSrc[slot] = iA;
+                        ++w;
+                    }
+                    ++iA; ++iS;
+                } else if (colA < colS) ++iA;
+                else ++iS;
+            }
+        }
+    }
+    return ORA_OK;
+}
+
+int ora_pairs_lower(int nb, const int *Arow, const int *Acol, const int *Mbrow, const int *Mcol,
+                    int *PairPtr, int *PairPLocal, int *PairSrc, int cap, int *total) {
+    return pairs_common(nb, Arow, Acol, Mbrow, Mcol, PairPtr, PairPLocal, PairSrc, cap, total, 0);
+}
+
+int ora_pairs_upper(int nb, const int *Arow, const int *Acol, const int *Mbrow, const int *Mcol,
+                    int *PairPtr, int *PairPLocal, int *PairSrc, int cap, int *total) {
+    return pairs_common(nb, Arow, Acol, Mbrow, Mcol, PairPtr, PairPLocal, PairSrc, cap, total, 1);
+}
+
+/* -------------------------------------------------------------- block inverse */
+int ora_invert_block(int bs, const double *A, double *Ainv, double eps) {
+    double M[64], I[64];
+    int bs2 = bs * bs;
+    for (int i = 0; i < bs2; ++i) { M[i] = A[i]; I[i] = 0.0; }
+    for (int d = 0; d < bs; ++d) I[d * bs + d] = 1.0;
+    for (int k = 0; k < bs; ++k) {
+        double piv = M[k * bs + k];
+        if (fabs(piv) < eps) return 0;
+        double invp = 1.0 / piv;
+        for (int j = 0; j < bs; ++j) { M[k * bs + j] *= invp; I[k * bs + j] *= invp; }
+        for (int i = 0; i < bs; ++i) {
+            if (i == k) continue;
+            double f = M[i * bs + k];
+            for (int j = 0; j < bs; ++j) {
+                M[i * bs + j] -= f * M[k * bs + j];
+                I[i * bs + j] -= f * I[k * bs + j];
+            }
+        }
+    }
+    for (int i = 0; i < bs2; ++i) Ainv[i] = I[i];
+    return 1;
+}
+
+static const double *find_block(const int *rp, const int *ci, const double *val, int bs2,
+                                int br, int bc) {
+    int s = rp[br], e = rp[br + 1] - 1;
+    while (s <= e) {
+        int m = (s + e) >> 1, c = ci[m];
+        if (c == bc) return val + (size_t)m * bs2;
+        if (c < bc) s = m + 1; else e = m - 1;
+    }
+    return NULL;
+}
+
+void ora_build_dinv(int nb, int bs, const int *Arow, const int *Acol, const double *Aval,
+                    int lower, double *Dinv) {
+    int bs2 = bs * bs;
+    for (int k = 0; k < nb; ++k) {
+        double D[64], Dinv_k[64];
+        const double *src = find_block(Arow, Acol, Aval, bs2, k, k);
+        for (int e = 0; e < bs2; ++e) D[e] = src ? src[e] : 0.0;
+        for (int i = 0; i < bs; ++i) {
+            for (int j = 0; j < bs; ++j) {
+                if (lower && j > i) D[i * bs + j] = 0.0;
+                if (!lower && j < i) D[i * bs + j] = 0.0;
+            }
+            if (lower) D[i * bs + i] = 1.0;
+        }
+        if (!ora_invert_block(bs, D, Dinv_k, 1e-14)) {
+            for (int d = 0; d < bs; ++d) D[d * bs + d] += 1e-12;
+            ora_invert_block(bs, D, Dinv_k, 1e-14);
+        }
+        memcpy(Dinv + (size_t)k * bs2, Dinv_k, sizeof(double) * (size_t)bs2);
+    }
+}
+
+/* -------------------------------------------------------------------- ISAI */
+void ora_isai_lower(int nb, int bs, const double *Aval, const int *Mbrow, const int *Mcol,
+                    const int *PairPtr, const int *PairPLocal, const int *PairSrc,
+                    const double *DinvL, double *Mval) {
+    int bs2 = bs * bs;
+    for (int j = 0; j < nb; ++j) {
+        int start = Mbrow[j], N = Mbrow[j + 1] - start;
+        for (int idx = 0; idx < N; ++idx) {
+            int t = Mcol[start + idx];
+            double rhs[64];
+            for (int e = 0; e < bs2; ++e) {
+                int r = e / bs, c = e % bs;
+                rhs[e] = (t == j && r == c) ? 1.0 : 0.0;
+            }
+            for (int q = PairPtr[start + idx]; q < PairPtr[start + idx + 1]; ++q) {
+                const double *A = Aval + (size_t)PairSrc[q] * bs2;
+                const double *Yp = Mval + (size_t)(start + PairPLocal[q]) * bs2;
+                for (int e = 0; e < bs2; ++e) {
+                    int rr = e / bs, cc = e % bs;
+                    double sum = 0.0;
+                    for (int k = 0; k < bs; ++k) sum += A[rr * bs + k] * Yp[k * bs + cc];
+                    rhs[e] -= sum;
+                }
+            }
+            const double *inv = DinvL + (size_t)t * bs2;
+            double *Y = Mval + (size_t)(start + idx) * bs2;
+            for (int e = 0; e < bs2; ++e) {
+                int rr = e / bs, cc = e % bs;
+                double acc = 0.0;
+                for (int k = 0; k < bs; ++k) acc += inv[rr * bs + k] * rhs[k * bs + cc];
+                Y[e] = acc;
+            }
+        }
+    }
+}
+
+void ora_isai_upper(int nb, int bs, const int *Arow, const int *Acol, const double *Aval,
+                    const int *Mbrow, const int *Mcol, const double *DinvU, double *Mval) {
+    int bs2 = bs * bs;
+    for (int j = 0; j < nb; ++j) {
+        int start = Mbrow[j], N = Mbrow[j + 1] - start;
+        for (int idx = N - 1; idx >= 0; --idx) {
+            int t = Mcol[start + idx];
+            double rhs[64];
+            for (int e = 0; e < bs2; ++e) {
+                int r = e / bs, c = e % bs;
+                rhs[e] = (t == j && r == c) ? 1.0 : 0.0;
+            }
+            for (int p = N - 1; p > idx; --p) {
+                const double *U = find_block(Arow, Acol, Aval, bs2, t, Mcol[start + p]);
+                if (!U) continue; /* absent block: the reference subtracts a zero product */
+                const double *Yp = Mval + (size_t)(start + p) * bs2;
+                for (int e = 0; e < bs2; ++e) {
+                    int i = e / bs, jj = e % bs;
+                    double sum = 0.0;
+                    for (int k = 0; k < bs; ++k) sum += U[i * bs + k] * Yp[k * bs + jj];
+                    rhs[e] -= sum;
+                }
+            }
+            const double *inv = DinvU + (size_t)t * bs2;
+            double *Y = Mval + (size_t)(start + idx) * bs2;
+            for (int e = 0; e < bs2; ++e) {
+                int i = e / bs, jj = e % bs;
+                double sum = 0.0;
+                for (int k = 0; k < bs; ++k) sum += inv[i * bs + k] * rhs[k * bs + jj];
+                Y[e] = sum;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ ILU(0) */
+static int find_pos(const int *rp, const int *ci, int br, int bc) {
+    int s = rp[br], e = rp[br + 1] - 1;
+    while (s <= e) {
+        int m = (s + e) >> 1, c = ci[m];
+        if (c == bc) return m;
+        if (c < bc) s = m + 1; else e = m - 1;
+    }
+    return -1;
+}
+
+int ora_bsrilu0(int nb, int bs, const int *Arow, const int *Acol, double *Aval, int *zero_pivot) {
+    int bs2 = bs * bs;
+    *zero_pivot = -1;
+    int *diag = (int *)malloc(sizeof(int) * (size_t)(nb > 0 ? nb : 1));
+    for (int i = 0; i < nb; ++i) {
+        diag[i] = find_pos(Arow, Acol, i, i);
+        if (diag[i] < 0) { *zero_pivot = i; free(diag); return ORA_ERR_ZERO_PIVOT; }
+    }
+    for (int i = 0; i < nb; ++i) {
+        for (int pk = Arow[i]; pk < Arow[i + 1]; ++pk) {
+            int k = Acol[pk];
+            if (k >= i) break;
+            /* L_ik = A'_ik * U_kk^{-1}: right solve with the upper part of the
+             * packed diagonal block of row k. */
+            double *Lik = Aval + (size_t)pk * bs2;
+            const double *Dk = Aval + (size_t)diag[k] * bs2;
+            for (int r = 0; r < bs; ++r) {
+                for (int c = 0; c < bs; ++c) {
+                    double s = Lik[r * bs + c];
+                    for (int m = 0; m < c; ++m) s -= Lik[r * bs + m] * Dk[m * bs + c];
+                    Lik[r * bs + c] = s / Dk[c * bs + c];
+                }
+            }
+            /* A_ij -= L_ik * U_kj for every stored (i,j), j > k, with (k,j) stored. */
+            for (int pj = pk + 1; pj < Arow[i + 1]; ++pj) {
+                int j = Acol[pj];
+                int pkj = find_pos(Arow, Acol, k, j);
+                if (pkj < 0) continue;
+                const double *Ukj = Aval + (size_t)pkj * bs2;
+                double *Aij = Aval + (size_t)pj * bs2;
+                for (int r = 0; r < bs; ++r)
+                    for (int c = 0; c < bs; ++c) {
+                        double s = 0.0;
+                        for (int m = 0; m < bs; ++m) s += Lik[r * bs + m] * Ukj[m * bs + c];
+                        Aij[r * bs + c] -= s;
+                    }
+            }
+        }
+        /* Diagonal block: scalar LU without pivoting, packed L\U. */
+        double *D = Aval + (size_t)diag[i] * bs2;
+        for (int c = 0; c < bs; ++c) {
+            if (D[c * bs + c] == 0.0 && *zero_pivot < 0) *zero_pivot = i;
+            for (int r = c + 1; r < bs; ++r) {
+                D[r * bs + c] /= D[c * bs + c];
+                for (int m = c + 1; m < bs; ++m) D[r * bs + m] -= D[r * bs + c] * D[c * bs + m];
+            }
+        }
+        /* U_ij = L_ii^{-1} A'_ij for j > i: unit-lower forward substitution. */
+        for (int pj = diag[i] + 1; pj < Arow[i + 1]; ++pj) {
+            double *X = Aval + (size_t)pj * bs2;
+            for (int r = 1; r < bs; ++r)
+                for (int m = 0; m < r; ++m)
+                    for (int c = 0; c < bs; ++c) X[r * bs + c] -= D[r * bs + m] * X[m * bs + c];
+        }
+    }
+    free(diag);
+    return *zero_pivot >= 0 ? ORA_ERR_ZERO_PIVOT : ORA_OK;
+}
+
+int ora_levels(int nb, const int *Arow, const int *Acol, int lower, int *level) {
+    int nlev = 0;
+    if (lower) {
+        for (int i = 0; i < nb; ++i) {
+            int l = 0;
+            for (int p = Arow[i]; p < Arow[i + 1]; ++p)
+                if (Acol[p] < i && level[Acol[p]] + 1 > l) l = level[Acol[p]] + 1;
+            level[i] = l;
+            if (l + 1 > nlev) nlev = l + 1;
+        }
+    } else {
+        for (int i = nb - 1; i >= 0; --i) {
+            int l = 0;
+            for (int p = Arow[i]; p < Arow[i + 1]; ++p)
+                if (Acol[p] > i && level[Acol[p]] + 1 > l) l = level[Acol[p]] + 1;
+            level[i] = l;
+            if (l + 1 > nlev) nlev = l + 1;
+        }
+    }
+    return nlev;
+}
+
+/* -------------------------------------------------------------------- SpMV */
+void ora_bsr_spmv(int nb, int bs, const int *rp, const int *ci, const double *val,
+                  const double *x, double *y) {
+    int bs2 = bs * bs;
+#pragma omp parallel for schedule(static)
+    for (int br = 0; br < nb; ++br) {
+        for (int r = 0; r < bs; ++r) {
+            double sum = 0.0;
+            for (int idx = rp[br]; idx < rp[br + 1]; ++idx) {
+                const double *blk = val + (size_t)idx * bs2;
+                int bc = ci[idx];
+                for (int c = 0; c < bs; ++c) sum += blk[r * bs + c] * x[(size_t)bc * bs + c];
+            }
+            y[(size_t)br * bs + r] = sum;
+        }
+    }
+}
+
+float ora_round_f32(double v) { return (float)v; }
+
+float ora_round_bf16(double v) {
+    /* float64 -> float32 (RNE) -> bfloat16 (RNE on the upper 16 bits). */
+    float f = (float)v;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) {
+        u |= 0x00400000u; /* quiet NaN */
+    } else {
+        uint32_t lsb = (u >> 16) & 1u;
+        u += 0x7fffu + lsb;
+    }
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+void ora_bsr_spmv_f32acc(int nb, int bs, const int *rp, const int *ci, const double *val,
+                         int bf16_values, const double *x, double *y) {
+    int bs2 = bs * bs;
+#pragma omp parallel for schedule(static)
+    for (int br = 0; br < nb; ++br) {
+        for (int r = 0; r < bs; ++r) {
+            float sum = 0.0f;
+            for (int idx = rp[br]; idx < rp[br + 1]; ++idx) {
+                const double *blk = val + (size_t)idx * bs2;
+                int bc = ci[idx];
+                for (int c = 0; c < bs; ++c) {
+                    float a = bf16_values ? ora_round_bf16(blk[r * bs + c]) : ora_round_f32(blk[r * bs + c]);
+                    float xv = (float)x[(size_t)bc * bs + c];
+                    sum = fmaf(a, xv, sum);
+                }
+            }
+            y[(size_t)br * bs + r] = (double)sum;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ SpTRSV */
+void ora_bsr_sptrsv_lu(int nb, int bs, const int *rp, const int *ci, const double *LU,
+                       const double *b, double *x) {
+    int bs2 = bs * bs;
+    double z[8];
+    /* forward: L y = b, L unit lower (strict-lower blocks + unit-lower diagonal part) */
+    for (int i = 0; i < nb; ++i) {
+        for (int r = 0; r < bs; ++r) z[r] = b[(size_t)i * bs + r];
+        const double *D = NULL;
+        for (int p = rp[i]; p < rp[i + 1]; ++p) {
+            int k = ci[p];
+            if (k == i) { D = LU + (size_t)p * bs2; continue; }
+            if (k > i) continue;
+            const double *Lk = LU + (size_t)p * bs2;
+            for (int r = 0; r < bs; ++r)
+                for (int c = 0; c < bs; ++c) z[r] -= Lk[r * bs + c] * x[(size_t)k * bs + c];
+        }
+        for (int r = 0; r < bs; ++r) {
+            if (D) for (int c = 0; c < r; ++c) z[r] -= D[r * bs + c] * z[c];
+            x[(size_t)i * bs + r] = z[r];
+        }
+    }
+    /* backward: U x = y */
+    for (int i = nb - 1; i >= 0; --i) {
+        for (int r = 0; r < bs; ++r) z[r] = x[(size_t)i * bs + r];
+        const double *D = NULL;
+        for (int p = rp[i + 1] - 1; p >= rp[i]; --p) {
+            int k = ci[p];
+            if (k == i) { D = LU + (size_t)p * bs2; continue; }
+            if (k < i) continue;
+            const double *Uk = LU + (size_t)p * bs2;
+            for (int r = 0; r < bs; ++r)
+                for (int c = 0; c < bs; ++c) z[r] -= Uk[r * bs + c] * x[(size_t)k * bs + c];
+        }
+        for (int r = bs - 1; r >= 0; --r) {
+            for (int c = r + 1; c < bs; ++c) z[r] -= D[r * bs + c] * z[c];
+            z[r] /= D[r * bs + r];
+        }
+        for (int r = 0; r < bs; ++r) x[(size_t)i * bs + r] = z[r];
+    }
+}
+
+int ora_csr_sptrsv_lu(int m, const int *rp, const int *ci, const double *v,
+                      const double *b, double *x) {
+    double *y = (double *)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    /* cape_sptrsv.h:8-53: unit L, columns ascending, diagonal skipped */
+    for (int i = 0; i < m; ++i) {
+        double left = 0.0;
+        for (int p = rp[i]; p < rp[i + 1] && ci[p] < i; ++p) left += v[p] * y[ci[p]];
+        y[i] = b[i] - left;
+    }
+    /* cape_sptrsv.h:55-103: columns descending, divide by the diagonal entry */
+    for (int i = m - 1; i >= 0; --i) {
+        double right = 0.0;
+        int p = rp[i + 1] - 1;
+        for (; p >= rp[i] && ci[p] > i; --p) right += v[p] * x[ci[p]];
+        if (p < rp[i] || ci[p] != i) { free(y); return ORA_ERR_ZERO_PIVOT; }
+        x[i] = (y[i] - right) / v[p];
+    }
+    free(y);
+    return ORA_OK;
+}
+
+double ora_rel_residual_bsr(int nb, int bs, const int *rp, const int *ci, const double *val,
+                            const double *b, const double *x) {
+    size_t n = (size_t)nb * bs;
+    double *Ax = (double *)malloc(sizeof(double) * (n ? n : 1));
+    ora_bsr_spmv(nb, bs, rp, ci, val, x, Ax);
+    double nr = 0.0, nbv = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        double r = b[i] - Ax[i];
+        nr += r * r;
+        nbv += b[i] * b[i];
+    }
+    free(Ax);
+    return nbv > 0.0 ? sqrt(nr) / sqrt(nbv) : INFINITY;
+}
+
+/* ------------------------------------------------------------------ Krylov */
+static double dot(size_t n, const double *a, const double *b) {
+    double s = 0.0;
+    for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+void ora_precond_apply(const ora_precond_t *P, const double *v, double *z) {
+    size_t n = (size_t)P->nb * P->bs;
+    if (P->precond == 1) {
+        double *t = (double *)malloc(sizeof(double) * n);
+        if (P->isai_f32 || P->isai_bf16) {
+            ora_bsr_spmv_f32acc(P->nb, P->bs, P->Lrp, P->Lci, P->Lval, P->isai_bf16, v, t);
+            ora_bsr_spmv_f32acc(P->nb, P->bs, P->Urp, P->Uci, P->Uval, P->isai_bf16, t, z);
+        } else {
+            ora_bsr_spmv(P->nb, P->bs, P->Lrp, P->Lci, P->Lval, v, t);
+            ora_bsr_spmv(P->nb, P->bs, P->Urp, P->Uci, P->Uval, t, z);
+        }
+        free(t);
+    } else if (P->precond == 2) {
+        ora_bsr_sptrsv_lu(P->nb, P->bs, P->LUrp, P->LUci, P->LUval, v, z);
+    } else {
+        memcpy(z, v, sizeof(double) * n);
+    }
+}
+
+int ora_bicgstab(int nb, int bs, const int *rp, const int *ci, const double *val,
+                 const ora_precond_t *P, const double *b, double *x,
+                 double tol, int maxit, int *iters, double *relres) {
+    size_t n = (size_t)nb * bs;
+    double *r = malloc(sizeof(double) * n), *rh = malloc(sizeof(double) * n);
+    double *p = calloc(n, sizeof(double)), *v = calloc(n, sizeof(double));
+    double *ph = malloc(sizeof(double) * n), *s = malloc(sizeof(double) * n);
+    double *sh = malloc(sizeof(double) * n), *t = malloc(sizeof(double) * n);
+    ora_bsr_spmv(nb, bs, rp, ci, val, x, r);
+    for (size_t i = 0; i < n; ++i) { r[i] = b[i] - r[i]; rh[i] = r[i]; }
+    double bnorm = sqrt(dot(n, b, b));
+    double rho_old = 1.0, alpha = 1.0, omega = 1.0;
+    int it = 0, conv = 0;
+    if (bnorm == 0.0) bnorm = 1.0;
+    if (sqrt(dot(n, r, r)) / bnorm <= tol) conv = 1;
+    while (!conv && it < maxit) {
+        ++it;
+        double rho = dot(n, rh, r);
+        if (rho == 0.0) break;
+        if (it == 1) memcpy(p, r, sizeof(double) * n);
+        else {
+            double beta = (rho / rho_old) * (alpha / omega);
+            for (size_t i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+        }
+        ora_precond_apply(P, p, ph);
+        ora_bsr_spmv(nb, bs, rp, ci, val, ph, v);
+        alpha = rho / dot(n, rh, v);
+        for (size_t i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
+        if (sqrt(dot(n, s, s)) / bnorm <= tol) {
+            for (size_t i = 0; i < n; ++i) x[i] += alpha * ph[i];
+            conv = 1;
+            break;
+        }
+        ora_precond_apply(P, s, sh);
+        ora_bsr_spmv(nb, bs, rp, ci, val, sh, t);
+        omega = dot(n, t, s) / dot(n, t, t);
+        for (size_t i = 0; i < n; ++i) {
+            x[i] += alpha * ph[i] + omega * sh[i];
+            r[i] = s[i] - omega * t[i];
+        }
+        rho_old = rho;
+        if (sqrt(dot(n, r, r)) / bnorm <= tol) conv = 1;
+    }
+    *iters = it;
+    *relres = ora_rel_residual_bsr(nb, bs, rp, ci, val, b, x);
+    free(r); free(rh); free(p); free(v); free(ph); free(s); free(sh); free(t);
+    return conv ? ORA_OK : ORA_ERR_NOT_CONVERGED;
+}
+
+int ora_fgmres(int nb, int bs, const int *rp, const int *ci, const double *val,
+               const ora_precond_t *P, const double *b, double *x, int m,
+               double tol, int maxit, int *iters, double *relres) {
+    size_t n = (size_t)nb * bs;
+    double *V = malloc(sizeof(double) * n * (size_t)(m + 1));
+    double *Z = malloc(sizeof(double) * n * (size_t)m);
+    double *w = malloc(sizeof(double) * n);
+    double *H = calloc((size_t)(m + 1) * m, sizeof(double));
+    double *cs = malloc(sizeof(double) * m), *sn = malloc(sizeof(double) * m);
+    double *g = malloc(sizeof(double) * (m + 1)), *yv = malloc(sizeof(double) * m);
+    double bnorm = sqrt(dot(n, b, b));
+    if (bnorm == 0.0) bnorm = 1.0;
+    int it = 0, conv = 0;
+    while (it < maxit && !conv) {
+        ora_bsr_spmv(nb, bs, rp, ci, val, x, w);
+        for (size_t i = 0; i < n; ++i) w[i] = b[i] - w[i];
+        double beta = sqrt(dot(n, w, w));
+        if (beta / bnorm <= tol) { conv = 1; break; }
+        for (size_t i = 0; i < n; ++i) V[i] = w[i] / beta;
+        for (int k = 0; k <= m; ++k) g[k] = 0.0;
+        g[0] = beta;
+        int jend = 0;
+        for (int j = 0; j < m && it < maxit; ++j) {
+            ++it;
+            ora_precond_apply(P, V + (size_t)j * n, Z + (size_t)j * n);
+            ora_bsr_spmv(nb, bs, rp, ci, val, Z + (size_t)j * n, w);
+            for (int i = 0; i <= j; ++i) {
+                double h = dot(n, w, V + (size_t)i * n);
+                H[i * m + j] = h;
+                for (size_t q = 0; q < n; ++q) w[q] -= h * V[(size_t)i * n + q];
+            }
+            double hn = sqrt(dot(n, w, w));
+            H[(j + 1) * m + j] = hn;
+            if (hn != 0.0) for (size_t q = 0; q < n; ++q) V[(size_t)(j + 1) * n + q] = w[q] / hn;
+            for (int i = 0; i < j; ++i) {
+                double a = H[i * m + j], c = H[(i + 1) * m + j];
+                H[i * m + j] = cs[i] * a + sn[i] * c;
+                H[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+            }
+            double a = H[j * m + j], c = H[(j + 1) * m + j];
+            double den = sqrt(a * a + c * c);
+            cs[j] = den == 0.0 ? 1.0 : a / den;
+            sn[j] = den == 0.0 ? 0.0 : c / den;
+            H[j * m + j] = cs[j] * a + sn[j] * c;
+            H[(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            jend = j + 1;
+            if (fabs(g[j + 1]) / bnorm <= tol || hn == 0.0) break;
+        }
+        for (int i = jend - 1; i >= 0; --i) {
+            double s = g[i];
+            for (int k = i + 1; k < jend; ++k) s -= H[i * m + k] * yv[k];
+            yv[i] = s / H[i * m + i];
+        }
+        for (int k = 0; k < jend; ++k)
+            for (size_t q = 0; q < n; ++q) x[q] += yv[k] * Z[(size_t)k * n + q];
+        if (ora_rel_residual_bsr(nb, bs, rp, ci, val, b, x) <= tol) conv = 1;
+    }
+    *iters = it;
+    *relres = ora_rel_residual_bsr(nb, bs, rp, ci, val, b, x);
+    free(V); free(Z); free(w); free(H); free(cs); free(sn); free(g); free(yv);
+    return conv ? ORA_OK : ORA_ERR_NOT_CONVERGED;
+}

@@ -1,0 +1,373 @@
+"""Pins the CPU oracle against mathematics, brute force and hand-worked fixtures.
+
+The reference ships no tests or golden vectors, so each oracle function is
+checked here against something that does not reuse its own formula: dense
+numpy algebra, textbook dense elimination with pattern dropping, closed forms,
+and a hand-worked conversion fixture.  CPU only.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from isai_kernel_amd import problems
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+RNG = np.random.default_rng(1234)
+
+
+def _f(v):
+    return float(v) if not isinstance(v, str) else float(v)
+
+
+def load_csr2bsr_fixture():
+    d = json.loads((GOLDEN / "csr2bsr_small.json").read_text())
+    v = np.array([_f(x) for x in d["csr_val"]])
+    bv = np.array([[[_f(x) for x in row] for row in blk] for blk in d["bsr_val"]])
+    return d, v, bv
+
+
+def random_bsr(nb, bs, density, rng, diag=True, dominant=True):
+    """Random block pattern (diagonal always present when diag), random values."""
+    mask = rng.random((nb, nb)) < density
+    if diag:
+        np.fill_diagonal(mask, True)
+    rp = np.zeros(nb + 1, np.int32)
+    cols, vals = [], []
+    for i in range(nb):
+        js = np.nonzero(mask[i])[0]
+        rp[i + 1] = rp[i] + len(js)
+        for j in js:
+            cols.append(j)
+            b = rng.uniform(-1, 1, (bs, bs))
+            if i == j and dominant:
+                b += np.eye(bs) * (2.0 * bs * (1 + mask[i].sum()))
+            vals.append(b)
+    return rp, np.array(cols, np.int32), np.array(vals).reshape(-1, bs, bs)
+
+
+def dense_mask(rp, ci, nb):
+    m = np.zeros((nb, nb), bool)
+    for i in range(nb):
+        m[i, ci[rp[i]:rp[i + 1]]] = True
+    return m
+
+
+def dense_ilu0(A, mask):
+    """Textbook IKJ ILU(0) on a scalar pattern (Saad, Alg. 10.4), packed L\\U."""
+    W = A.copy()
+    n = A.shape[0]
+    for i in range(1, n):
+        for k in range(i):
+            if not mask[i, k]:
+                continue
+            W[i, k] /= W[k, k]
+            sel = mask[i, k + 1:]
+            W[i, k + 1:] -= W[i, k] * W[k, k + 1:] * sel
+    return W
+
+
+# ------------------------------------------------------------------ csr2bsr
+def test_csr2bsr_hand_fixture():
+    d, v, bv = load_csr2bsr_fixture()
+    brp, bci, out = O.csr2bsr(d["n"], d["bs"], d["csr_rowptr"], d["csr_colind"], v)
+    assert brp.tolist() == d["bsr_rowptr"] and bci.tolist() == d["bsr_colind"]
+    # bitwise, so the -0.0 of the block-opening entry is checked too
+    assert out.tobytes() == bv.tobytes()
+
+
+def test_csr2bsr_random_vs_dense_and_roundtrip():
+    for trial in range(5):
+        nb, bs = 7, [1, 2, 3, 4, 5][trial]
+        rp, ci, val = random_bsr(nb, bs, 0.4, RNG)
+        crp, cci, cv = O.bsr2csr(rp, ci, val)
+        A = O.bsr_to_dense(rp, ci, val)
+        # bsr2csr closed form: dense equality and nnz = nnzb*bs^2
+        D = np.zeros_like(A)
+        for r in range(nb * bs):
+            D[r, cci[crp[r]:crp[r + 1]]] += cv[crp[r]:crp[r + 1]]
+        assert np.array_equal(A, D) and len(cci) == len(ci) * bs * bs
+        brp, bci, bv = O.csr2bsr(nb * bs, bs, crp, cci, cv)
+        assert np.array_equal(brp, rp) and np.array_equal(bci, ci) and np.array_equal(bv, val)
+
+
+def test_csr2bsr_duplicates_summed_and_bad_n():
+    rp = np.array([0, 3, 3, 3, 3], np.int32)
+    ci = np.array([1, 1, 1], np.int32)
+    v = np.array([0.25, 0.5, 1.0])
+    _, _, bv = O.csr2bsr(4, 2, rp, ci, v)
+    assert bv[0, 0, 1] == 1.75
+    with pytest.raises(ValueError):
+        O.csr2bsr(5, 2, np.zeros(6, np.int32), np.zeros(0, np.int32), np.zeros(0))
+
+
+# ----------------------------------------------------------------- patterns
+@pytest.mark.parametrize("lower", [True, False])
+def test_patterns_vs_dense_mask(lower):
+    rp, ci, _ = random_bsr(23, 1, 0.2, RNG, diag=False)
+    nb = 23
+    m = dense_mask(rp, ci, nb)
+    tri = np.tril(m) if lower else np.triu(m)
+    mb, mc = O.bcsc_pattern(rp, ci, lower)
+    for j in range(nb):
+        expect = set(np.nonzero(tri[:, j])[0]) | {j}
+        got = mc[mb[j]:mb[j + 1]]
+        assert list(got) == sorted(expect)
+    trp, tci = O.tri_bcsr_pattern(rp, ci, lower)
+    for i in range(nb):
+        assert list(tci[trp[i]:trp[i + 1]]) == list(np.nonzero(tri[i])[0])
+
+
+def test_map_is_a_permutation_and_detects_mismatch():
+    rp, ci, _ = random_bsr(30, 1, 0.15, RNG)
+    mb, mc = O.bcsc_pattern(rp, ci, True)
+    trp, tci = O.tri_bcsr_pattern(rp, ci, True)
+    st, m = O.map_bcsc2bcsr(mb, mc, trp, tci)
+    assert st == 0
+    assert sorted(m.tolist()) == list(range(len(tci)))
+    for j in range(30):
+        for k in range(mb[j], mb[j + 1]):
+            p = m[k]
+            assert tci[p] == j and trp[mc[k]] <= p < trp[mc[k] + 1]
+    # drop one row's diagonal from the BCSR pattern -> "Pattern mismatch"
+    trp2, tci2 = trp.copy(), tci.copy()
+    r = 5
+    keep = np.ones(len(tci2), bool)
+    keep[trp2[r]:trp2[r + 1]][tci2[trp2[r]:trp2[r + 1]] == r] = False
+    tci2 = tci2[keep]
+    trp2[r + 1:] -= 1
+    st, _ = O.map_bcsc2bcsr(mb, mc, trp2, tci2)
+    assert st == -4
+
+
+def test_reorder_is_map_scatter():
+    src = RNG.standard_normal((9, 4, 4))
+    mp = RNG.permutation(9).astype(np.int32)
+    dst = O.reorder(mp, src)
+    for k in range(9):
+        assert np.array_equal(dst[mp[k]], src[k])
+
+
+# -------------------------------------------------------------------- pairs
+@pytest.mark.parametrize("upper", [False, True])
+def test_pairs_brute_force(upper):
+    nb = 25
+    rp, ci, _ = random_bsr(nb, 1, 0.2, RNG)
+    mask = dense_mask(rp, ci, nb)
+    mb, mc = O.bcsc_pattern(rp, ci, not upper)
+    pp, pl, ps = O.pairs(rp, ci, mb, mc, upper=upper)
+    for j in range(nb):
+        S = mc[mb[j]:mb[j + 1]]
+        for idx, t in enumerate(S):
+            got = list(zip(pl[pp[mb[j] + idx]:pp[mb[j] + idx + 1]], ps[pp[mb[j] + idx]:pp[mb[j] + idx + 1]]))
+            exp = []
+            rng_p = range(idx) if not upper else range(len(S) - 1, idx, -1)
+            for p in rng_p:
+                if mask[t, S[p]]:
+                    pos = rp[t] + int(np.nonzero(ci[rp[t]:rp[t + 1]] == S[p])[0][0])
+                    exp.append((p, pos))
+            assert got == exp
+
+
+# ------------------------------------------------------------ block inverse
+def test_invert_block_and_guard():
+    for bs in (1, 3, 4, 5, 7):
+        A = RNG.uniform(-1, 1, (bs, bs)) + 3 * bs * np.eye(bs)
+        ok, Ai = O.invert_block(A)
+        assert ok and np.allclose(A @ Ai, np.eye(bs), atol=1e-13)
+    ok, _ = O.invert_block(np.array([[0.0, 1.0], [1.0, 0.0]]))
+    assert not ok  # zero pivot without pivoting (isai_bsr.h:69)
+    # triangular closed form: inverse of unit lower [[1,0],[a,1]] is [[1,0],[-a,1]]
+    ok, Ai = O.invert_block(np.array([[1.0, 0.0], [2.5, 1.0]]))
+    assert ok and np.array_equal(Ai, np.array([[1.0, 0.0], [-2.5, 1.0]]))
+
+
+def test_dinv_parts():
+    rp, ci, val = problems.generate(1, 4, seed=3)
+    lu, st, _ = O.bsrilu0(rp, ci, val)
+    assert st == 0
+    for lower in (True, False):
+        dinv = O.build_dinv(rp, ci, lu, lower)
+        for k in range(len(rp) - 1):
+            pos = rp[k] + int(np.nonzero(ci[rp[k]:rp[k + 1]] == k)[0][0])
+            D = lu[pos]
+            part = np.tril(D, -1) + np.eye(4) if lower else np.triu(D)
+            assert np.allclose(part @ dinv[k], np.eye(4), atol=1e-13)
+
+
+# ------------------------------------------------------------------ ILU(0)
+@pytest.mark.parametrize("density", [0.25, 1.0])
+def test_ilu0_vs_dense_masked_elimination(density):
+    nb, bs = 12, 4
+    rp, ci, val = random_bsr(nb, bs, density, RNG)
+    lu, st, zp = O.bsrilu0(rp, ci, val)
+    assert st == 0 and zp == -1
+    A = O.bsr_to_dense(rp, ci, val)
+    mask = np.kron(dense_mask(rp, ci, nb), np.ones((bs, bs), bool))
+    W = dense_ilu0(A, mask)
+    LUd = O.bsr_to_dense(rp, ci, lu)
+    assert np.allclose(LUd[mask], W[mask], rtol=1e-12, atol=1e-12)
+    # defining invariant: (L U)_IJ = A_IJ on every stored block
+    L = np.tril(LUd, -1) + np.eye(nb * bs)
+    U = np.triu(LUd)
+    assert np.allclose((L @ U)[mask], A[mask], rtol=1e-12, atol=1e-12)
+    if density == 1.0:  # full pattern: exact LU
+        assert np.allclose(L @ U, A, rtol=1e-12, atol=1e-12)
+
+
+def test_ilu0_block_tridiagonal_closed_form():
+    nb, bs = 6, 4
+    rp = np.array([0] + [min(i + 2, nb) - max(i - 1, 0) for i in range(nb)], np.int32).cumsum().astype(np.int32)
+    ci = np.array([j for i in range(nb) for j in range(max(i - 1, 0), min(i + 2, nb))], np.int32)
+    val = RNG.uniform(-1, 1, (len(ci), bs, bs))
+    for i in range(nb):
+        val[rp[i] + (1 if i > 0 else 0)] += 10 * np.eye(bs)
+    lu, st, _ = O.bsrilu0(rp, ci, val)
+    assert st == 0
+    A = O.bsr_to_dense(rp, ci, val)
+    # D_i = A_ii - A_{i,i-1} D_{i-1}^{-1} A_{i-1,i}; L_{i,i-1} = A_{i,i-1} U_{i-1}^{-1}
+    D_prev = None
+    for i in range(nb):
+        Aii = A[i*bs:(i+1)*bs, i*bs:(i+1)*bs]
+        D = Aii if i == 0 else Aii - A[i*bs:(i+1)*bs, (i-1)*bs:i*bs] @ np.linalg.solve(
+            D_prev, A[(i-1)*bs:i*bs, i*bs:(i+1)*bs])
+        packed = lu[rp[i] + (1 if i > 0 else 0)]
+        Lii, Uii = np.tril(packed, -1) + np.eye(bs), np.triu(packed)
+        assert np.allclose(Lii @ Uii, D, rtol=1e-12, atol=1e-12)
+        D_prev = D
+
+
+def test_ilu0_zero_pivot_reporting():
+    rp = np.array([0, 1, 2], np.int32)
+    ci = np.array([0, 0], np.int32)  # row 1 has no diagonal block: structural zero
+    val = np.stack([np.eye(4), np.eye(4)])
+    _, st, zp = O.bsrilu0(rp, ci, val)
+    assert st == -5 and zp == 1
+    rp = np.array([0, 1], np.int32)
+    ci = np.array([0], np.int32)
+    v = np.eye(4)
+    v[2, 2] = 0.0  # numerical zero pivot in U
+    _, st, zp = O.bsrilu0(rp, ci, v[None])
+    assert st == -5 and zp == 0
+
+
+# ------------------------------------------------------------------- ISAI
+def _isai_factors(rp, ci, lu):
+    nb = len(rp) - 1
+    mbL, mcL = O.bcsc_pattern(rp, ci, True)
+    mbU, mcU = O.bcsc_pattern(rp, ci, False)
+    pp, pl, ps = O.pairs(rp, ci, mbL, mcL)
+    ML = O.isai_lower(lu, mbL, mcL, pp, pl, ps, O.build_dinv(rp, ci, lu, True))
+    MU = O.isai_upper(rp, ci, lu, mbU, mcU, O.build_dinv(rp, ci, lu, False))
+    return (mbL, mcL, ML), (mbU, mcU, MU)
+
+
+@pytest.mark.parametrize("density", [0.3, 1.0])
+def test_isai_defining_invariant(density):
+    nb, bs = 14, 4
+    rp, ci, val = random_bsr(nb, bs, density, RNG)
+    lu, st, _ = O.bsrilu0(rp, ci, val)
+    (mbL, mcL, ML), (mbU, mcU, MU) = _isai_factors(rp, ci, lu)
+    LUd = O.bsr_to_dense(rp, ci, lu)
+    Ld = np.tril(LUd, -1) + np.eye(nb * bs)
+    Ud = np.triu(LUd)
+    for (mb, mc, M, F) in ((mbL, mcL, ML, Ld), (mbU, mcU, MU, Ud)):
+        Md = O.bcsc_to_dense(mb, mc, M, nb)
+        P = F @ Md
+        for j in range(nb):
+            for k in range(mb[j], mb[j + 1]):
+                i = mc[k]
+                blk = P[i*bs:(i+1)*bs, j*bs:(j+1)*bs]
+                assert np.allclose(blk, np.eye(bs) if i == j else 0, atol=1e-12)
+        if density == 1.0:  # full pattern: ISAI is the exact inverse
+            assert np.allclose(Md, np.linalg.inv(F), atol=1e-12)
+
+
+def test_isai_lower_bidiagonal_closed_form():
+    nb, bs = 5, 4
+    rp = np.array([0, 1, 3, 5, 7, 9], np.int32)
+    ci = np.array([0, 0, 1, 1, 2, 2, 3, 3, 4], np.int32)
+    val = RNG.uniform(-1, 1, (9, bs, bs))
+    lu = val.copy()  # use values directly as a packed factor (L unit-lower part)
+    (mbL, mcL, ML), _ = _isai_factors(rp, ci, lu)
+    def Ldiag(i):
+        return np.tril(lu[rp[i + 1] - 1], -1) + np.eye(bs)
+    for j in range(nb):
+        k0 = mbL[j]
+        assert np.allclose(ML[k0], np.linalg.inv(Ldiag(j)), atol=1e-13)
+        if j + 1 < nb:
+            Lsub = lu[rp[j + 1]]  # block (j+1, j)
+            exp = -np.linalg.inv(Ldiag(j + 1)) @ Lsub @ np.linalg.inv(Ldiag(j))
+            assert np.allclose(ML[k0 + 1], exp, atol=1e-12)
+
+
+# ------------------------------------------------------------------ levels
+@pytest.mark.parametrize("problem,L,formula", [
+    (0, 9, lambda x, y, z: x + y),
+    (1, 6, lambda x, y, z: x + y + z),
+    (3, 8, lambda x, y, z: x + 2 * y),
+    (2, 5, lambda x, y, z: x + 2 * y + 4 * z),
+])
+def test_levels_closed_forms(problem, L, formula):
+    rp, ci, _ = problems.generate(problem, L, seed=0)
+    lev, n = O.levels(rp, ci, True)
+    Lz = L if problem in (1, 2) else 1
+    exp = np.array([formula(x, y, z) for z in range(Lz) for y in range(L) for x in range(L)])
+    assert np.array_equal(lev, exp) and n == exp.max() + 1
+    levU, nU = O.levels(rp, ci, False)
+    assert nU == n and np.array_equal(levU, exp.max() - exp)
+
+
+# --------------------------------------------------------------- SpMV/solve
+def test_spmv_dense_and_reduced_precision():
+    rp, ci, val = problems.generate(1, 5, seed=1)
+    x = RNG.standard_normal((len(rp) - 1) * 4)
+    A = O.bsr_to_dense(rp, ci, val)
+    assert np.allclose(O.spmv(rp, ci, val, x), A @ x, rtol=1e-14, atol=1e-13)
+    A32 = O.bsr_to_dense(rp, ci, val.astype(np.float32).astype(np.float64))
+    y32 = O.spmv_f32acc(rp, ci, val, x)
+    ref = A32 @ x.astype(np.float32).astype(np.float64)
+    assert np.max(np.abs(y32 - ref)) <= 8 * 2**-24 * np.max(np.abs(A32) @ np.abs(x))
+    # BF16 rounding: round-to-nearest-even on the top 16 bits of the float
+    for v, exp in ((1.0 + 2**-8, 1.0), (1.0 + 3 * 2**-8, 1.0 + 2**-6), (-2.0 - 2**-7, -2.0), (1.00390625, 1.0)):
+        assert O.ora.ora_round_bf16(v) == exp
+
+
+def test_sptrsv_block_and_csr():
+    rp, ci, val = problems.generate(1, 4, seed=2)
+    lu, st, _ = O.bsrilu0(rp, ci, val)
+    n = (len(rp) - 1) * 4
+    b = RNG.standard_normal(n)
+    x = O.sptrsv_lu(rp, ci, lu, b)
+    LUd = O.bsr_to_dense(rp, ci, lu)
+    L = np.tril(LUd, -1) + np.eye(n)
+    U = np.triu(LUd)
+    assert np.allclose(L @ (U @ x), b, rtol=1e-12, atol=1e-12)
+    crp, cci, cv = O.bsr2csr(rp, ci, lu)
+    st, xc = O.csr_sptrsv_lu(crp, cci, cv, b)
+    assert st == 0 and np.allclose(xc, x, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "fgmres"])
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_krylov_solvers_converge_to_direct_solution(solver, kind):
+    rp, ci, val = problems.generate(0, 8, seed=0)
+    nb, n = len(rp) - 1, (len(rp) - 1) * 4
+    b = O.spmv(rp, ci, val, np.ones(n))
+    lu, _, _ = O.bsrilu0(rp, ci, val)
+    (mbL, mcL, ML), (mbU, mcU, MU) = _isai_factors(rp, ci, lu)
+    trpL, tciL = O.tri_bcsr_pattern(rp, ci, True)
+    trpU, tciU = O.tri_bcsr_pattern(rp, ci, False)
+    _, mapL = O.map_bcsc2bcsr(mbL, mcL, trpL, tciL)
+    _, mapU = O.map_bcsc2bcsr(mbU, mcU, trpU, tciU)
+    L = (trpL, tciL, O.reorder(mapL, ML))
+    U = (trpU, tciU, O.reorder(mapU, MU))
+    P = O.make_precond(kind, nb, 4, L=L, U=U, LU=(rp, ci, lu))
+    fn = O.bicgstab if solver == "bicgstab" else O.fgmres
+    x, st, it, rr = fn(rp, ci, val, P, b, tol=1e-10, maxit=500)
+    assert st == 0 and rr <= 1.5e-10
+    xd = np.linalg.solve(O.bsr_to_dense(rp, ci, val), b)
+    assert np.allclose(x, xd, rtol=1e-8, atol=1e-8)
+    assert np.allclose(xd, 1.0, atol=1e-10)
