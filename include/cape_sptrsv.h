@@ -1,0 +1,60 @@
+/*
+ * cape_sptrsv.h -- sync-free triangular solves with the packed ILU(0) factor:
+ * L y = b (unit lower), then U x = y.
+ *
+ * Replaces spTrSolveL / spTrSolveU / csr_cape_SpTRSV (reference
+ * cape_sptrsv.h:8-179).  Semantics kept: L's diagonal is taken as 1 and never
+ * read (cape_sptrsv.h:36-47); U is divided by its diagonal (cape_sptrsv.h:95-96).
+ * The reference's host wrapper copies b into d_x and then zeroes d_x
+ * (cape_sptrsv.h:131,155), so it solves with a zero right-hand side, and never
+ * copies the result back despite its comment (cape_sptrsv.h:111); here the
+ * intended behaviour is implemented: x = (LU)^{-1} b written back into b.
+ *
+ * No host synchronisation and no global barrier inside a solve; waits are
+ * bounded and report ISAI_ERR_TIMEOUT instead of hanging.
+ */
+#ifndef ISAI_CAPE_SPTRSV_H
+#define ISAI_CAPE_SPTRSV_H
+
+#include "isai_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Host-pointer form with the reference's signature (csr_cape_SpTRSV,
+ * cape_sptrsv.h:105-112): CSR of the packed factor (e.g. from isai_bsr_to_csr),
+ * b overwritten with the solution.  Synchronous. */
+int isai_csr_sptrsv_lu(int m, const int *csrRowPtr, const int *csrColIdx, const double *csrVal,
+                       int nnz, double *b);
+/* Device form of the same scalar CSR solve; d_work holds
+ * isai_csr_sptrsv_work_size(m) bytes; chunk <= 0 picks the default.  Synchronizes
+ * to report a timed-out wait. */
+int isai_csr_sptrsv_work_size(int m, size_t *bytes);
+int isai_csr_sptrsv_lu_device(int m, int nnz, const int *d_rowptr, const int *d_colind,
+                              const double *d_val, const double *d_b, double *d_x, void *d_work,
+                              int chunk, int spin_limit, isai_stream_t stream);
+
+/* BSR-native form (bs = 4): one wave per chunk of consecutive block rows.  The
+ * analysis records the diagonal positions and inverts the diagonal parts of the
+ * packed factor; isai_bsr_sptrsv_update_values redoes the inversion for new
+ * values on the same pattern.  chunk <= 0 picks the default (64 block rows). */
+typedef struct isai_sptrsv_plan *isai_sptrsv_plan_t;
+int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const int *d_col,
+                             const double *d_LU, int chunk, isai_sptrsv_plan_t *plan,
+                             isai_stream_t stream);
+int isai_bsr_sptrsv_update_values(isai_sptrsv_plan_t plan, const double *d_LU, isai_stream_t stream);
+/* d_x = U^{-1} L^{-1} d_b (d_x may alias d_b).  Asynchronous. */
+int isai_bsr_sptrsv_solve(isai_sptrsv_plan_t plan, const double *d_LU, const double *d_b, double *d_x,
+                          isai_stream_t stream);
+/* Bound on the polls of one dependency before the solve gives up on it. */
+int isai_bsr_sptrsv_set_spin_limit(isai_sptrsv_plan_t plan, int limit);
+/* Synchronizes the stream; ISAI_ERR_TIMEOUT if a wait of the previous solves
+ * ran out (the error word is then cleared). */
+int isai_sptrsv_status(isai_sptrsv_plan_t plan, isai_stream_t stream);
+int isai_bsr_sptrsv_destroy(isai_sptrsv_plan_t plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -23,6 +23,59 @@ extern "C" {
 int isai_bcsc_pattern_count(int nb, const int *Arow, const int *Acol, int lower, int *nnz);
 int isai_bcsc_pattern(int nb, const int *Arow, const int *Acol, int lower, int *Mbrow, int *Mcol);
 
+/* Replaces the kernels build_DinvL_from_BSR5_kernel_fix / build_DinvU_...
+ * (reference isai_bsr.h:91-158, launched at main.cu:659-663 and 154-159):
+ * d_Dinv[k] = inverse of the unit-lower (lower != 0) or upper part of the packed
+ * diagonal block of row k, Gauss-Jordan without pivoting with the reference's
+ * 1e-14 pivot guard and +1e-12 diagonal retry.  Device pointers, bs in 1..8. */
+int isai_build_dinv(int nb, int bs, const int *d_Arow, const int *d_Acol, const double *d_Aval,
+                    int lower, double *d_Dinv, isai_stream_t stream);
+
+/* Replaces count_pairs_lower_kernel_v2 + thrust::exclusive_scan (reference
+ * isai_bsr.h:1378-1424, main.cu:665-690): d_PairPtr[Mnnz+1] becomes the
+ * exclusive prefix of the per-entry pair counts and *total_pairs = d_PairPtr[Mnnz]
+ * (this call synchronizes the stream to return it, as main.cu:690 does).
+ * upper != 0 builds the ISAI(U) lists (p_local > idx), else ISAI(L) (p_local < idx). */
+int isai_pairs_count(int nb, int Mnnz, const int *d_Arow, const int *d_Acol, const int *d_Mbrow,
+                     const int *d_Mcol, int upper, int *d_PairPtr, int *total_pairs,
+                     isai_stream_t stream);
+/* Replaces fill_pairs_lower_kernel_v2 (reference isai_bsr.h:1427-1476): PairPLocal
+ * (index within S_j) and PairSrc (BSR position of A(t, S_j[p])).  Lower lists are
+ * ascending in p_local, upper lists descending (the reference's ISAI(U) loop order,
+ * isai_bsr.h:196). */
+int isai_pairs_fill(int nb, const int *d_Arow, const int *d_Acol, const int *d_Mbrow,
+                    const int *d_Mcol, int upper, const int *d_PairPtr, int *d_PairPLocal,
+                    int *d_PairSrc, isai_stream_t stream);
+
+/* Replaces the ISAI(L) kernels (reference isai_bsr.h:288-399 cachedY, 869-974
+ * tensor cachedY, 1189-1353 globalY, cacheY_isai.h:159-341 hybrid; launched at
+ * main.cu:51-152): column-wise forward substitution over S_j with the pair lists
+ * and DinvL, d_Mval in BCSC order.  Nmax = longest column (main.cu:699-703) sizes
+ * the LDS Y cache; beyond the cache budget Y is read back from d_Mval. */
+int isai_build_lower(int nb, int bs, const double *d_Aval, const int *d_Mbrow, const int *d_Mcol,
+                     const int *d_PairPtr, const int *d_PairPLocal, const int *d_PairSrc,
+                     const double *d_DinvL, double *d_Mval, int Nmax, isai_stream_t stream);
+/* ISAI(U) with precomputed upper pair lists (same recurrence as
+ * isai_upper_bsr5_warpcol_with_Dinv_kernel, reference isai_bsr.h:161-242). */
+int isai_build_upper(int nb, int bs, const double *d_Aval, const int *d_Mbrow, const int *d_Mcol,
+                     const int *d_PairPtr, const int *d_PairPLocal, const int *d_PairSrc,
+                     const double *d_DinvU, double *d_Mval, int Nmax, isai_stream_t stream);
+/* Drop-in form of the reference's ISAI(U) kernel arguments (isai_bsr.h:161-170 plus
+ * Mnnz and Nmax): builds the upper pair lists internally (library-owned scratch,
+ * freed before returning; synchronizes the stream). */
+int isai_build_upper_lookup(int nb, int bs, const int *d_Arow, const int *d_Acol,
+                            const double *d_Aval, const int *d_Mbrow, const int *d_Mcol, int Mnnz,
+                            const double *d_DinvU, double *d_Mval, int Nmax, isai_stream_t stream);
+/* Both factors in ONE launch: every column's local triangular system of ISAI(L)
+ * and ISAI(U) is solved by the same batched kernel. */
+int isai_build_lu(int nb, int bs, const double *d_Aval,
+                  const int *d_MbrowL, const int *d_McolL, const int *d_PairPtrL,
+                  const int *d_PairPLocalL, const int *d_PairSrcL, const double *d_DinvL,
+                  double *d_MvalL,
+                  const int *d_MbrowU, const int *d_McolU, const int *d_PairPtrU,
+                  const int *d_PairPLocalU, const int *d_PairSrcU, const double *d_DinvU,
+                  double *d_MvalU, int Nmax, isai_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

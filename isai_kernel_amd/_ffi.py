@@ -71,6 +71,30 @@ SIGNATURES: dict[str, list] = {
     # isai_bsr.h
     "isai_bcsc_pattern_count": [I, P, P, I, P],
     "isai_bcsc_pattern": [I, P, P, I, P, P],
+    "isai_build_dinv": [I, I, P, P, P, I, P, P],
+    "isai_pairs_count": [I, I, P, P, P, P, I, P, P, P],
+    "isai_pairs_fill": [I, P, P, P, P, I, P, P, P, P],
+    "isai_build_lower": [I, I, P, P, P, P, P, P, P, P, I, P],
+    "isai_build_upper": [I, I, P, P, P, P, P, P, P, P, I, P],
+    "isai_build_upper_lookup": [I, I, P, P, P, P, P, I, P, P, I, P],
+    "isai_build_lu": [I, I, P] + [P] * 7 + [P] * 7 + [I, P],
+    # bsrilu.h
+    "isai_bsrilu0_inplace": [I, I, I, P, P, P, P, P],
+    "isai_bsrilu0_analysis": [I, I, I, P, P, P, P, P],
+    "isai_bsrilu0_factor": [P, P, P, P],
+    "isai_bsrilu0_plan_info": [P, P, P],
+    "isai_bsrilu0_destroy": [P],
+    "isai_bsr_levels": [I, P, P, I, P, P],
+    # cape_sptrsv.h
+    "isai_csr_sptrsv_lu": [I, P, P, P, I, P],
+    "isai_csr_sptrsv_work_size": [I, P],
+    "isai_csr_sptrsv_lu_device": [I, I, P, P, P, P, P, P, I, I, P],
+    "isai_bsr_sptrsv_analysis": [I, I, I, P, P, P, I, P, P],
+    "isai_bsr_sptrsv_update_values": [P, P, P],
+    "isai_bsr_sptrsv_solve": [P, P, P, P, P],
+    "isai_bsr_sptrsv_set_spin_limit": [P, I],
+    "isai_sptrsv_status": [P, P],
+    "isai_bsr_sptrsv_destroy": [P],
     # bsr_tensor_SpMV.h
     "isai_bsr_spmv": [I, I, I, P, P, P, P, P, P],
     "isai_bsr_spmv_prec": [I, I, I, I, P, P, P, P, P, P],

@@ -126,3 +126,14 @@ def test_generator_coefficients():
     a = problems.generate(2, 4, seed=5)[2]
     assert np.array_equal(a, problems.generate(2, 4, seed=5)[2])
     assert not np.array_equal(a, problems.generate(2, 4, seed=6)[2])
+
+
+@pytest.mark.parametrize("lower", [True, False])
+def test_levels_match_oracle(lower):
+    rp, ci, _ = problems.generate(2, 7, seed=0)
+    lev = np.empty(len(rp) - 1, np.int32)
+    n = ctypes.c_int()
+    assert lib.isai_bsr_levels(len(rp) - 1, rp.ctypes.data, ci.ctypes.data, int(lower), lev.ctypes.data,
+                               ctypes.byref(n)) == 0
+    olev, on = O.levels(rp, ci, lower)
+    assert np.array_equal(lev, olev) and n.value == on
