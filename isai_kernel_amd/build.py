@@ -31,9 +31,10 @@ HIP_FLAGS = [
     "-O3",
     "-fPIC",
     "-std=c++17",
-    "-munsafe-fp-atomics",
-    # Packed FP32 ops are left to the compiler's own scalar/float2 lowering off:
-    # the feature is disabled for every device compile.
+    # Arithmetic is exactly what the source writes: fused multiply-adds only where
+    # fma() is called, so factor kernels can reproduce a sequential order bit for bit.
+    "-ffp-contract=off",
+    # No packed FP32 instructions in any device code.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
     f"-I{INCLUDE}",

@@ -13,11 +13,13 @@
 // and publishes its flag with release semantics.  Because tickets follow level
 // order, a row only ever waits on rows whose workgroups already hold a ticket.
 //
-// Per row i (lanes = one element (r,c) of a block, G lanes per block):
-//   for each stored k < i, ascending:   L_ik = A'_ik * inv(U_kk)
-//       for each stored j > k with (k,j) stored:  A'_ij -= L_ik * U_kj
-//   A'_ii = L_ii U_ii (scalar LU, no pivoting, FP64) ; inv(U_ii), inv(L_ii)
-//   for each stored j > i:   U_ij = inv(L_ii) * A'_ij
+// Per row i:
+//   for each stored k < i, ascending:   L_ik = A'_ik inv(U_kk)  (right substitution)
+//       for each stored j > k with (k,j) stored:  A'_ij -= L_ik U_kj
+//   A'_ii = L_ii U_ii (scalar LU, no pivoting, FP64)
+//   for each stored j > i:   U_ij = inv(L_ii) A'_ij            (forward substitution)
+// Every scalar operation is the oracle's, in the same order, without contraction,
+// so the factor is bit-identical to the sequential restatement.
 #include <climits>
 #include <vector>
 
@@ -57,14 +59,13 @@ __device__ __forceinline__ int ilu_find(const int *__restrict__ rp, const int *_
 template <int BS>
 __global__ __launch_bounds__(64) void ilu0_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
                                                   double *__restrict__ val, const int *__restrict__ order,
-                                                  const int *__restrict__ diagpos, double *__restrict__ uinv,
-                                                  int *__restrict__ flags, unsigned int *__restrict__ ticket,
-                                                  int *__restrict__ zero_pivot, int *__restrict__ err) {
+                                                  const int *__restrict__ diagpos, int *__restrict__ flags,
+                                                  unsigned int *__restrict__ ticket, int *__restrict__ zero_pivot,
+                                                  int *__restrict__ err) {
     using S = IluShape<BS>;
     constexpr int B2 = S::B2;
+    constexpr int CB = 64 / BS;  // blocks per pass when one lane owns one row or column
     __shared__ double rowv[kIluMaxRow * B2];
-    __shared__ double Lt[B2];
-    __shared__ double Linv[B2];
     __shared__ int s_row;
     const int lane = threadIdx.x;
     if (lane == 0) s_row = order[atomicAdd(ticket, 1u)];
@@ -95,88 +96,70 @@ __global__ __launch_bounds__(64) void ilu0_kernel(int nb, const int *__restrict_
     for (int p = rs; p < dp; ++p) {
         const int k = ci[p];
         const int lp = p - rs;
-        if (lane < B2) {
-            const double *U = uinv + (size_t)k * B2;
-            double s = 0.0;
+        // L_ik = A'_ik inv(U_kk) by right substitution with the upper part of the
+        // packed diagonal block of row k; lane r owns row r.
+        if (lane < BS) {
+            const double *Dk = val + (size_t)diagpos[k] * B2;
+            double *X = rowv + lp * B2 + lane * BS;
 #pragma unroll
-            for (int m = 0; m < BS; ++m) s = fma(rowv[lp * B2 + r * BS + m], U[m * BS + c], s);
-            Lt[lane] = s;
+            for (int cc = 0; cc < BS; ++cc) {
+                double sacc = X[cc];
+#pragma unroll
+                for (int m = 0; m < cc; ++m) sacc = sacc - X[m] * Dk[m * BS + cc];
+                X[cc] = sacc / Dk[cc * BS + cc];
+            }
         }
         __syncthreads();
-        if (lane < B2) rowv[lp * B2 + lane] = Lt[lane];
+        // A'_ij -= L_ik U_kj for the stored j > k with (k, j) stored; one lane per
+        // element (r, c), G lanes per block.
         for (int q0 = p + 1; q0 < re; q0 += S::C) {
             const int q = q0 + g;
             if (q < re && ok) {
                 const int pkj = ilu_find(rp, ci, k, ci[q]);
                 if (pkj >= 0) {
                     const double *Ukj = val + (size_t)pkj * B2;
-                    double s = 0.0;
+                    const double *Lr = rowv + lp * B2 + r * BS;
+                    double sacc = 0.0;
 #pragma unroll
-                    for (int m = 0; m < BS; ++m) s = fma(Lt[r * BS + m], Ukj[m * BS + c], s);
-                    rowv[(q - rs) * B2 + e] -= s;
+                    for (int m = 0; m < BS; ++m) sacc = sacc + Lr[m] * Ukj[m * BS + c];
+                    rowv[(q - rs) * B2 + e] = rowv[(q - rs) * B2 + e] - sacc;
                 }
             }
         }
         __syncthreads();
     }
 
-    // Diagonal block: scalar LU without pivoting, then inv(U_ii) and inv(L_ii).
+    // Diagonal block: scalar LU without pivoting, packed L_ii\U_ii.
     if (lane == 0) {
-        double D[B2], Ui[B2], Li[B2];
-#pragma unroll
-        for (int x = 0; x < B2; ++x) D[x] = rowv[(dp - rs) * B2 + x];
+        double *D = rowv + (dp - rs) * B2;
         bool zero = false;
-#pragma unroll
         for (int cc = 0; cc < BS; ++cc) {
             if (D[cc * BS + cc] == 0.0) zero = true;
-#pragma unroll
             for (int rr = cc + 1; rr < BS; ++rr) {
-                D[rr * BS + cc] /= D[cc * BS + cc];
-#pragma unroll
-                for (int m = cc + 1; m < BS; ++m) D[rr * BS + m] -= D[rr * BS + cc] * D[cc * BS + m];
+                D[rr * BS + cc] = D[rr * BS + cc] / D[cc * BS + cc];
+                for (int m = cc + 1; m < BS; ++m) D[rr * BS + m] = D[rr * BS + m] - D[rr * BS + cc] * D[cc * BS + m];
             }
         }
         if (zero) atomicMin(zero_pivot, i);
-        // inv(U): column by column back substitution; inv(L): forward, unit diagonal
-#pragma unroll
-        for (int x = 0; x < B2; ++x) { Ui[x] = 0.0; Li[x] = 0.0; }
-#pragma unroll
-        for (int col = 0; col < BS; ++col) {
-#pragma unroll
-            for (int rr = BS - 1; rr >= 0; --rr) {
-                double s = (rr == col) ? 1.0 : 0.0;
-#pragma unroll
-                for (int m = rr + 1; m < BS; ++m) s -= D[rr * BS + m] * Ui[m * BS + col];
-                Ui[rr * BS + col] = s / D[rr * BS + rr];
-            }
-#pragma unroll
-            for (int rr = 0; rr < BS; ++rr) {
-                double s = (rr == col) ? 1.0 : 0.0;
-#pragma unroll
-                for (int m = 0; m < rr; ++m) s -= D[rr * BS + m] * Li[m * BS + col];
-                Li[rr * BS + col] = s;
-            }
-        }
-#pragma unroll
-        for (int x = 0; x < B2; ++x) {
-            rowv[(dp - rs) * B2 + x] = D[x];
-            Lt[x] = Ui[x];
-            Linv[x] = Li[x];
-        }
     }
     __syncthreads();
-    if (lane < B2) uinv[(size_t)i * B2 + lane] = Lt[lane];
 
-    // U_ij = inv(L_ii) * A'_ij for the stored j > i.
-    for (int q0 = dp + 1; q0 < re; q0 += S::C) {
-        const int q = q0 + g;
-        double s = 0.0;
-        if (q < re && ok) {
-#pragma unroll
-            for (int m = 0; m < BS; ++m) s = fma(Linv[r * BS + m], rowv[(q - rs) * B2 + m * BS + c], s);
+    // U_ij = inv(L_ii) A'_ij for the stored j > i by forward substitution with the
+    // unit-lower part of the diagonal block; lane (block, column c).
+    {
+        const double *D = rowv + (dp - rs) * B2;
+        const int gb = lane / BS, col = lane % BS;
+        for (int q0 = dp + 1; q0 < re; q0 += CB) {
+            const int q = q0 + gb;
+            if (gb < CB && q < re) {
+                double *X = rowv + (q - rs) * B2 + col;
+                for (int rr = 1; rr < BS; ++rr) {
+                    double xv = X[rr * BS];
+                    for (int m = 0; m < rr; ++m) xv = xv - D[rr * BS + m] * X[m * BS];
+                    X[rr * BS] = xv;
+                }
+            }
         }
-        __syncthreads();
-        if (q < re && ok) rowv[(q - rs) * B2 + e] = s;
     }
     __syncthreads();
     for (int x = lane; x < len * B2; x += 64) val[(size_t)rs * B2 + x] = rowv[x];
@@ -185,10 +168,10 @@ __global__ __launch_bounds__(64) void ilu0_kernel(int nb, const int *__restrict_
 }
 
 template <int BS>
-void launch_ilu(const isai_ilu_plan *P, double *val, double *uinv, int *flags, unsigned *ticket, int *zp,
-                int *err, hipStream_t st) {
-    ilu0_kernel<BS><<<P->nb, 64, 0, st>>>(P->nb, P->d_row, P->d_col, val, P->d_order, P->d_diag, uinv, flags,
-                                          ticket, zp, err);
+void launch_ilu(const isai_ilu_plan *P, double *val, int *flags, unsigned *ticket, int *zp, int *err,
+                hipStream_t st) {
+    ilu0_kernel<BS><<<P->nb, 64, 0, st>>>(P->nb, P->d_row, P->d_col, val, P->d_order, P->d_diag, flags, ticket,
+                                          zp, err);
 }
 
 }  // namespace isai
@@ -268,28 +251,26 @@ int isai_bsrilu0_factor(isai_ilu_plan_t P, double *d_val, int *zero_pivot, isai_
     *zero_pivot = -1;
     if (P->nb == 0) return ISAI_OK;
     hipStream_t st = (hipStream_t)stream;
-    const size_t b2 = (size_t)P->bs * P->bs;
-    // scratch: inv(U_kk) per row, ready flags, ticket, zero-pivot word, error word
+    // scratch: ready flags, then ticket, zero-pivot word, error word
     char *scratch = nullptr;
-    const size_t off_flags = sizeof(double) * b2 * P->nb;
-    const size_t off_words = off_flags + sizeof(int) * ((P->nb + 63) / 64 * 64);
+    const size_t off_flags = 0;
+    const size_t off_words = sizeof(int) * ((P->nb + 63) / 64 * 64);
     const size_t bytes = off_words + 4 * sizeof(int);
     ISAI_HIP(hipMallocAsync((void **)&scratch, bytes, st));
-    double *uinv = (double *)scratch;
     int *flags = (int *)(scratch + off_flags);
     int *words = (int *)(scratch + off_words);  // [0] ticket, [1] zero pivot, [2] error
     ISAI_HIP(hipMemsetAsync(flags, 0, bytes - off_flags, st));
     ISAI_HIP(hipMemsetD32Async((hipDeviceptr_t)&words[1], INT_MAX, 1, st));
     unsigned *ticket = (unsigned *)&words[0];
     switch (P->bs) {
-        case 1: launch_ilu<1>(P, d_val, uinv, flags, ticket, &words[1], &words[2], st); break;
-        case 2: launch_ilu<2>(P, d_val, uinv, flags, ticket, &words[1], &words[2], st); break;
-        case 3: launch_ilu<3>(P, d_val, uinv, flags, ticket, &words[1], &words[2], st); break;
-        case 4: launch_ilu<4>(P, d_val, uinv, flags, ticket, &words[1], &words[2], st); break;
-        case 5: launch_ilu<5>(P, d_val, uinv, flags, ticket, &words[1], &words[2], st); break;
-        case 6: launch_ilu<6>(P, d_val, uinv, flags, ticket, &words[1], &words[2], st); break;
-        case 7: launch_ilu<7>(P, d_val, uinv, flags, ticket, &words[1], &words[2], st); break;
-        case 8: launch_ilu<8>(P, d_val, uinv, flags, ticket, &words[1], &words[2], st); break;
+        case 1: launch_ilu<1>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
+        case 2: launch_ilu<2>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
+        case 3: launch_ilu<3>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
+        case 4: launch_ilu<4>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
+        case 5: launch_ilu<5>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
+        case 6: launch_ilu<6>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
+        case 7: launch_ilu<7>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
+        case 8: launch_ilu<8>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
     }
     ISAI_LAUNCH_CHECK();
     int h[4];

@@ -203,19 +203,21 @@ __global__ __launch_bounds__(64) void isai_build_kernel(int nb, const double *__
                 const int src = F.PairSrc[qb + q];
                 const double *Arow = Aval + (size_t)src * B2 + i * BS;
                 const double *Y = gy ? F.Mval + (size_t)(start + pl) * B2 + jj : Yc + pl * B2 + jj;
+                // Same operations and order as the sequential restatement (no
+                // contraction): s = sum_k A(i,k) Y(k,jj); r = r - s.
                 double s = 0.0;
                 if constexpr (BS == 4) {
                     const v2d a01 = *(const v2d *)Arow;
                     const v2d a23 = *(const v2d *)(Arow + 2);
-                    s = a01.x * Y[0];
-                    s = fma(a01.y, Y[4], s);
-                    s = fma(a23.x, Y[8], s);
-                    s = fma(a23.y, Y[12], s);
+                    s = s + a01.x * Y[0];
+                    s = s + a01.y * Y[4];
+                    s = s + a23.x * Y[8];
+                    s = s + a23.y * Y[12];
                 } else {
 #pragma unroll
-                    for (int k = 0; k < BS; ++k) s = fma(Arow[k], Y[k * BS], s);
+                    for (int k = 0; k < BS; ++k) s = s + Arow[k] * Y[k * BS];
                 }
-                r -= s;
+                r = r - s;
             }
         }
         if (lane_ok) rhsS[e] = r;
@@ -226,13 +228,13 @@ __global__ __launch_bounds__(64) void isai_build_kernel(int nb, const double *__
             if constexpr (BS == 4) {
                 const v2d d01 = *(const v2d *)D;
                 const v2d d23 = *(const v2d *)(D + 2);
-                y = d01.x * rhsS[jj];
-                y = fma(d01.y, rhsS[4 + jj], y);
-                y = fma(d23.x, rhsS[8 + jj], y);
-                y = fma(d23.y, rhsS[12 + jj], y);
+                y = y + d01.x * rhsS[jj];
+                y = y + d01.y * rhsS[4 + jj];
+                y = y + d23.x * rhsS[8 + jj];
+                y = y + d23.y * rhsS[12 + jj];
             } else {
 #pragma unroll
-                for (int k = 0; k < BS; ++k) y = fma(D[k], rhsS[k * BS + jj], y);
+                for (int k = 0; k < BS; ++k) y = y + D[k] * rhsS[k * BS + jj];
             }
             if (!gy) Yc[idx * B2 + e] = y;
             F.Mval[(size_t)(start + idx) * B2 + e] = y;
