@@ -1,22 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark of the block-sparse ILU(0)/ISAI preconditioning path on one MI355X.
 
-Workload: 3-D 7-point stencil on a 96^3 grid, 4 dof per node (bs = 4, 884,736
-block rows, 6,137,856 blocks), FP64, generated in-process (synthetic, seed 0).
+Workload (N=1): 3-D 7-point stencil on a 96^3 grid, 4 dof per node (bs = 4,
+884,736 block rows, 6,137,856 blocks), FP64, synthetic (generated in-process,
+seed 0), b = A*1.  A "step" is one preconditioned solve to 1e-10 relative
+residual: ISAI-preconditioned BiCGSTAB with the ISAI factors stored in reduced
+precision (the mixed form), FP64 everywhere else.  The plain FP64 forms (FP64
+ISAI, and exact ILU(0) applied with the sync-free triangular solves) are timed
+the same way for the speed-up.
 
-A "step" is one pass of the hot path over that matrix.  Kernel figures are
-hipEvent timings on the stream the kernels run on; the roofline object is for
-the dominant kernel (the BSR SpMV) with algorithmic bytes per launch
-(16*8+4)*nnzb + 4*(nb+1) + 2*4*8*nb.
+Kernel figures are hipEvent timings on the stream the kernels run on; the
+roofline object is for the dominant kernel of the solve (the FP64 BSR SpMV with
+A), with algorithmic bytes per launch (16*8+4)*nnzb + 4*(nb+1) + 2*4*8*nb.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
 from __future__ import annotations
 
 import argparse
 import ctypes
 import json
-import os
 import sys
 import time
 from pathlib import Path
@@ -28,6 +31,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "solve time to 1e-10 rel-res, mixed vs FP64 ISAI; BSR SpMV/SpTRSV HBM GB/s"
 HBM_PEAK_GBPS = 8000.0
+TOL = 1e-10
 
 
 def log(msg: str) -> None:
@@ -36,6 +40,10 @@ def log(msg: str) -> None:
 
 def spmv_bytes(nb: int, nnzb: int, value_bytes: int = 8) -> int:
     return (16 * value_bytes + 4) * nnzb + 4 * (nb + 1) + 2 * 4 * 8 * nb
+
+
+def sptrsv_bytes(nb: int, nnzb_l: int, nnzb_u: int) -> int:
+    return (16 * 8 + 4) * (nnzb_l + nnzb_u) + 2 * 4 * (nb + 1) + 4 * 4 * 8 * nb + 2 * 4 * nb
 
 
 class Timer:
@@ -48,8 +56,9 @@ class Timer:
         lib.isai_event_create(ctypes.byref(self.e1))
 
     def time(self, fn, iters: int) -> float:
-        """Average milliseconds of fn() over iters launches."""
+        """Average milliseconds of fn() over iters calls."""
         from isai_kernel_amd import check
+        check(self.lib.isai_stream_synchronize(self.stream), "sync")
         check(self.lib.isai_event_record(self.e0, self.stream), "event_record")
         for _ in range(iters):
             fn()
@@ -60,29 +69,42 @@ class Timer:
         return ms.value / iters
 
 
-def cpu_spmv_baseline(rp, ci, val, x, budget_s: float = 12.0):
-    """Oracle SpMV on the host cores over the full matrix, repeated within a budget."""
+def cpu_baseline(rp, ci, val, b, budget_s: float = 40.0):
+    """The oracle's plain FP64 method (ILU(0), ISAI(L/U), ISAI-preconditioned
+    BiCGSTAB to 1e-10) on the full matrix, on the host cores."""
     sys.path.insert(0, str(ROOT / "tests"))
-    import _oracle as O  # the checker, used here only as the CPU baseline
+    import _oracle as O  # the checker, run here only as the CPU baseline
 
     nb = len(rp) - 1
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        O.spmv(rp, ci, val, x)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or reps >= 50:
-            break
-    ms = el / reps * 1e3
-    return ms, reps, O.num_threads(), nb
+    t0 = time.perf_counter()
+    lu, st, _ = O.bsrilu0(rp, ci, val)
+    mbL, mcL = O.bcsc_pattern(rp, ci, True)
+    mbU, mcU = O.bcsc_pattern(rp, ci, False)
+    pp, pl, ps = O.pairs(rp, ci, mbL, mcL)
+    ML = O.isai_lower(lu, mbL, mcL, pp, pl, ps, O.build_dinv(rp, ci, lu, True))
+    MU = O.isai_upper(rp, ci, lu, mbU, mcU, O.build_dinv(rp, ci, lu, False))
+    tL = O.tri_bcsr_pattern(rp, ci, True)
+    tU = O.tri_bcsr_pattern(rp, ci, False)
+    _, mL = O.map_bcsc2bcsr(mbL, mcL, *tL)
+    _, mU = O.map_bcsc2bcsr(mbU, mcU, *tU)
+    L = (tL[0], tL[1], O.reorder(mL, ML))
+    U = (tU[0], tU[1], O.reorder(mU, MU))
+    t_setup = time.perf_counter() - t0
+    P = O.make_precond(1, nb, 4, L=L, U=U)
+    t1 = time.perf_counter()
+    x, st, it, rr = O.bicgstab(rp, ci, val, P, b, tol=TOL, maxit=1000)
+    t_solve = time.perf_counter() - t1
+    return dict(setup_s=t_setup, solve_s=t_solve, iters=it, relres=rr, cores=O.num_threads(), status=st)
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10, help="timed solves per form")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--kernel-iters", type=int, default=100)
     ap.add_argument("--problem", type=int, default=1)
+    ap.add_argument("--chunk", type=int, default=0, help="rows per wave of the sync-free solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -91,6 +113,7 @@ def main() -> int:
         return 2
 
     from isai_kernel_amd import DeviceArray, call, lib, problems
+    from isai_kernel_amd.solver import Handle, Matrix
 
     name = ctypes.create_string_buffer(256)
     cus = ctypes.c_int()
@@ -106,52 +129,126 @@ def main() -> int:
     stream = ctypes.c_void_p()
     call("isai_stream_create", ctypes.byref(stream))
     timer = Timer(lib, stream)
-
-    d_rp = DeviceArray.from_numpy(rp)
-    d_ci = DeviceArray.from_numpy(ci)
-    d_val = DeviceArray.from_numpy(val)
-    x = np.ones(n)
-    d_x = DeviceArray.from_numpy(x)
+    A = Matrix(rp, ci, val)
+    d_one = DeviceArray.from_numpy(np.ones(n))
+    d_b = DeviceArray((n,), np.float64)
+    d_x = DeviceArray((n,), np.float64)
     d_y = DeviceArray((n,), np.float64)
-
-    def spmv():
-        lib.isai_bsr_spmv(nb, nnzb, 4, d_rp.ptr, d_ci.ptr, d_val.ptr, d_x.ptr, d_y.ptr, stream)
-
-    for _ in range(args.warmup):
-        spmv()
+    A.spmv(d_one, d_b, stream)
     call("isai_stream_synchronize", stream)
-    spmv_ms = timer.time(spmv, args.steps)
-    bytes_spmv = spmv_bytes(nb, nnzb)
-    spmv_gbps = bytes_spmv / (spmv_ms * 1e-3) / 1e9
-    log(f"SpMV fp64: {spmv_ms*1e3:.1f} us/launch, {spmv_gbps:.0f} GB/s "
-        f"({100*spmv_gbps/HBM_PEAK_GBPS:.1f}% of 8 TB/s)")
 
+    # ---- setup: ILU(0) + ISAI build (device time from the library's events)
+    t0 = time.perf_counter()
+    h = Handle(A, precond="isai", precision="fp64", chunk=args.chunk, stream=stream)
+    setup_wall = time.perf_counter() - t0
+    ms_ilu, ms_isai, ms_setup = h.times()
+    nnzbL, nnzbU, nlevels = h.info()
+    log(f"setup: ILU(0) {ms_ilu:.2f} ms, ISAI build {ms_isai:.2f} ms (levels {nlevels}, "
+        f"nnzbL {nnzbL}, host wall incl. analysis {setup_wall:.2f} s)")
+
+    kernels = {}
+    K = args.kernel_iters
+
+    def record(name, ms, nbytes):
+        gbps = nbytes / (ms * 1e-3) / 1e9
+        kernels[name] = {"us": round(ms * 1e3, 2), "GBps": round(gbps, 1),
+                         "frac_hbm": round(gbps / HBM_PEAK_GBPS, 4), "algorithmic_bytes": nbytes}
+        log(f"{name}: {ms*1e3:.1f} us, {gbps:.0f} GB/s ({100*gbps/HBM_PEAK_GBPS:.1f}% of 8 TB/s)")
+
+    # ---- BSR SpMV with A (FP64)
+    def spmv():
+        lib.isai_bsr_spmv(nb, nnzb, 4, A.d_rp.ptr, A.d_ci.ptr, A.d_val.ptr, d_one.ptr, d_y.ptr, stream)
+    for _ in range(10):
+        spmv()
+    spmv_ms = timer.time(spmv, K)
+    record("bsr_spmv_fp64", spmv_ms, spmv_bytes(nb, nnzb))
+
+    # ---- ISAI apply (M_U (M_L v)) in each storage precision
+    for prec, vb in (("fp64", 8), ("fp32", 4), ("bf16", 2)):
+        h.set_preconditioner("isai", prec)
+
+        def apply():
+            lib.isai_precond_apply(h.h, d_b.ptr, d_y.ptr, stream)
+        for _ in range(5):
+            apply()
+        ms = timer.time(apply, K)
+        record(f"isai_apply_{prec}", ms, 2 * ((16 * vb + 4) * nnzbL + 4 * (nb + 1) + 2 * 4 * 8 * nb))
+
+    # ---- sync-free SpTRSV (L then U with the packed ILU(0) factor)
+    h.set_preconditioner("ilu", "fp64")
+
+    def trsv():
+        lib.isai_precond_apply(h.h, d_b.ptr, d_y.ptr, stream)
+    for _ in range(3):
+        trsv()
+    tr_ms = timer.time(trsv, max(K // 4, 10))
+    record("sptrsv_lu_fp64", tr_ms, sptrsv_bytes(nb, nnzbL, nnzbU))
+
+    # ---- solves (the steps)
+    solves = {}
+    for form, (pc, prec) in {"isai_bf16": ("isai", "bf16"), "isai_fp32": ("isai", "fp32"),
+                             "isai_fp64": ("isai", "fp64"), "ilu_sptrsv_fp64": ("ilu", "fp64")}.items():
+        h.set_preconditioner(pc, prec)
+        state = {}
+
+        def one_solve():
+            call("isai_fill_f64_async", d_x.ptr, 0.0, n, stream)
+            st, it, rr = h.solve(d_b, d_x, "bicgstab", tol=TOL, maxit=1000, check=False)
+            state.update(st=st, it=it, rr=rr)
+        for _ in range(args.warmup):
+            one_solve()
+        call("isai_stream_synchronize", stream)
+        t0 = time.perf_counter()
+        ms = timer.time(one_solve, args.steps)
+        wall = (time.perf_counter() - t0) / args.steps * 1e3
+        x = d_x.to_numpy()
+        solves[form] = {"ms": round(ms, 3), "wall_ms": round(wall, 3), "iters": state["it"],
+                        "relres": state["rr"], "status": state["st"],
+                        "max_err_vs_ones": float(np.max(np.abs(x - 1.0)))}
+        log(f"solve {form}: {ms:.2f} ms ({state['it']} it, relres {state['rr']:.2e}, status {state['st']})")
+
+    ok = {k: v for k, v in solves.items() if v["status"] == 0 and v["relres"] <= TOL}
+    mixed = min((k for k in ("isai_bf16", "isai_fp32") if k in ok), key=lambda k: ok[k]["ms"], default=None)
+    plain = [k for k in ("isai_fp64", "ilu_sptrsv_fp64") if k in ok]
+    value = ok[mixed]["ms"] if mixed else None
+    speedups = {f"{mixed}_vs_{p}": round(ok[p]["ms"] / ok[mixed]["ms"], 3) for p in plain} if mixed else {}
+
+    gbps = kernels["bsr_spmv_fp64"]["GBps"]
     result = {
         "metric": METRIC,
-        "value": round(spmv_gbps, 2),
-        "unit": "GB/s",
+        "value": value,
+        "unit": "ms",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(spmv_ms, 5),
-        "higher_is_better": True,
+        "ms_per_step": value,
+        "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "bsr_spmv 3-D 7-pt 96^3 bs=4 (884736 block rows, 6137856 blocks)",
-                   "problem": args.problem, "nb": nb, "nnzb": nnzb},
-        "roofline": {"bound": "hbm", "achieved": round(spmv_gbps, 2), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(spmv_gbps / HBM_PEAK_GBPS, 4), "traffic": None,
-                     "kernel": "bsr4_spmv_f64_kernel", "algorithmic_bytes": bytes_spmv},
+        "config": {"workload": "ISAI-preconditioned BiCGSTAB to 1e-10 (mixed-precision ISAI, FP64 Krylov), "
+                               "3-D 7-pt 96^3, bs=4 (884736 block rows, 6137856 blocks), b = A*1",
+                   "problem": args.problem, "nb": nb, "nnzb": nnzb, "mixed_form": mixed},
+        "roofline": {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(gbps / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "bsr4_spmv_f64_kernel", "algorithmic_bytes": spmv_bytes(nb, nnzb)},
+        "speedup": speedups,
+        "solves": solves,
+        "kernels": kernels,
+        "setup_ms": {"ilu0": round(ms_ilu, 3), "isai_build": round(ms_isai, 3), "levels": nlevels},
     }
+    h.close()
     if not args.no_cpu:
-        ms, reps, cores, _ = cpu_spmv_baseline(rp, ci, val, x)
+        c = cpu_baseline(rp, ci, val, d_b.to_numpy())
         result["cpu_baseline"] = {
-            "value": round(bytes_spmv / (ms * 1e-3) / 1e9, 3), "unit": "GB/s", "cores": cores,
-            "kind": "port",
-            "sample": f"oracle BSR SpMV over the full 96^3 matrix, {reps} repetitions, {ms:.1f} ms each",
+            "value": round(c["solve_s"] * 1e3, 1), "unit": "ms", "cores": c["cores"], "kind": "port",
+            "sample": (f"oracle FP64 ISAI-preconditioned BiCGSTAB on the full 96^3 matrix to 1e-10: "
+                       f"{c['iters']} iterations, relres {c['relres']:.2e}; its ILU(0)+ISAI setup took "
+                       f"{c['setup_s']:.1f} s (not included)"),
         }
+        log(f"cpu baseline: solve {c['solve_s']*1e3:.0f} ms ({c['iters']} it), setup {c['setup_s']:.1f} s, "
+            f"{c['cores']} threads")
     print(json.dumps(result), flush=True)
     return 0
 

@@ -1,0 +1,80 @@
+/*
+ * isai_solve.h -- the preconditioned solve that the reference's driver stops short
+ * of: main.cu:449-866 factors, builds ISAI(L/U) and applies x = M_U (M_L b) once;
+ * it has no Krylov method (no BiCGSTAB / GMRES anywhere in the reference).
+ *
+ * A handle owns everything the setup produces from a BSR matrix A on the device:
+ * the packed ILU(0) factor (A is not modified), the ISAI factors in BCSR (FP64 and
+ * the selected storage precision), the sync-free triangular-solve plan, and the
+ * Krylov work vectors.  The Krylov methods run in FP64 (matrix products with A,
+ * residuals, dot products, orthogonalisation); only the preconditioner values may
+ * be stored in FP32 / BF16 (applied with FP32 accumulation).
+ *
+ *   BiCGSTAB: van der Vorst (1992), right preconditioning (Saad, Iterative
+ *             Methods for Sparse Linear Systems, 2nd ed., Alg. 7.7).
+ *   FGMRES(m): flexible GMRES with modified Gram-Schmidt (Saad Alg. 9.6).
+ * Scalars (alpha, beta, omega, Hessenberg entries, Givens rotations) live on the
+ * device; the host only reads a convergence word once per iteration.
+ */
+#ifndef ISAI_SOLVE_H
+#define ISAI_SOLVE_H
+
+#include "isai_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ISAI_PRECOND_NONE = 0,
+    ISAI_PRECOND_ISAI = 1,        /* z = M_U (M_L v) */
+    ISAI_PRECOND_ILU_SPTRSV = 2   /* z = U^{-1} L^{-1} v, sync-free solves */
+} isai_precond_kind_t;
+
+typedef enum { ISAI_SOLVER_BICGSTAB = 0, ISAI_SOLVER_FGMRES = 1 } isai_solver_kind_t;
+
+typedef struct {
+    int precond;      /* isai_precond_kind_t */
+    int precision;    /* isai_precision_t of the ISAI factor values */
+    int sptrsv_chunk; /* rows per wave of the sync-free solve, 0 = default */
+    int reserved[5];
+} isai_setup_opts_t;
+
+typedef struct isai_handle *isai_handle_t;
+
+/* Runs ILU(0) on a copy of A, the ISAI patterns / pair lists / Dinv / batched
+ * ISAI build / BCSR permutation, the precision conversion and (for
+ * ISAI_PRECOND_ILU_SPTRSV) the triangular-solve analysis.  Synchronous.
+ * A's device arrays must stay valid for the life of the handle. */
+int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, const double *d_val,
+               const isai_setup_opts_t *opts, isai_handle_t *handle, isai_stream_t stream);
+/* Device time of the setup phases (hipEvents): ILU(0) factorization, ISAI
+ * construction (pairs + Dinv + build + permutation + conversion), total. */
+int isai_setup_times(isai_handle_t handle, double *ms_ilu, double *ms_isai, double *ms_total);
+/* Switch the preconditioner (kind, storage precision) without refactoring. */
+int isai_set_preconditioner(isai_handle_t handle, int precond, int precision, isai_stream_t stream);
+/* Sizes of the ISAI factors (block counts). */
+int isai_handle_info(isai_handle_t handle, int *nnzbL, int *nnzbU, int *nlevels);
+/* Device pointers of the factors (BCSR of M_L / M_U, values in FP64, and the
+ * packed ILU(0) factor with A's pattern); for tests and custom pipelines. */
+int isai_handle_factors(isai_handle_t handle, const int **Lrow, const int **Lcol, const double **Lval,
+                        const int **Urow, const int **Ucol, const double **Uval, const double **LU);
+
+/* d_z = M d_v with the handle's current preconditioner.  Asynchronous. */
+int isai_precond_apply(isai_handle_t handle, const double *d_v, double *d_z, isai_stream_t stream);
+
+/* Solve A x = b.  d_x holds the initial guess on entry.  Stops when the
+ * recursive residual estimate reaches tol * ||b|| (then confirmed with the true
+ * FP64 residual, iterating on if it is not met) or after maxit iterations.
+ * restart = FGMRES cycle length (ignored by BiCGSTAB).  *iters = iterations
+ * (matrix products with A), *relres = ||b - A x|| / ||b|| computed in FP64.
+ * Returns ISAI_ERR_NOT_CONVERGED when maxit is reached. */
+int isai_solve(isai_handle_t handle, int solver, int restart, double tol, int maxit,
+               const double *d_b, double *d_x, int *iters, double *relres, isai_stream_t stream);
+
+int isai_destroy(isai_handle_t handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -95,6 +95,15 @@ SIGNATURES: dict[str, list] = {
     "isai_bsr_sptrsv_set_spin_limit": [P, I],
     "isai_sptrsv_status": [P, P],
     "isai_bsr_sptrsv_destroy": [P],
+    # isai_solve.h
+    "isai_setup": [I, I, I, P, P, P, P, P, P],
+    "isai_setup_times": [P, P, P, P],
+    "isai_set_preconditioner": [P, I, I, P],
+    "isai_handle_info": [P, P, P, P],
+    "isai_handle_factors": [P, P, P, P, P, P, P, P],
+    "isai_precond_apply": [P, P, P, P],
+    "isai_solve": [P, I, I, D, I, P, P, P, P, P],
+    "isai_destroy": [P],
     # bsr_tensor_SpMV.h
     "isai_bsr_spmv": [I, I, I, P, P, P, P, P, P],
     "isai_bsr_spmv_prec": [I, I, I, I, P, P, P, P, P, P],
@@ -108,6 +117,11 @@ for _name, _args in SIGNATURES.items():
     _fn = getattr(lib, _name)
     _fn.argtypes = _args
     _fn.restype = _RESTYPE.get(_name, I)
+
+
+class SetupOpts(C.Structure):
+    """isai_setup_opts_t"""
+    _fields_ = [("precond", I), ("precision", I), ("sptrsv_chunk", I), ("reserved", I * 5)]
 
 
 class IsaiError(RuntimeError):

@@ -1,0 +1,830 @@
+// solve.hip -- setup handle and the FP64 Krylov methods (BiCGSTAB, FGMRES(m))
+// around the ILU(0)/ISAI preconditioner, for gfx950.
+//
+// The reference's driver ends with one application x = M_U (M_L b) and a host
+// residual print (main.cu:817-866); the Krylov loop here is new.  All scalar
+// recurrences stay on the device: every reduction is a fixed-shape two-stage
+// sum (deterministic), and the small scalar updates run in the final stage of the
+// reduction that produces them.  A device word `done` makes every later kernel of
+// an iteration return at once after convergence, so the host only reads that
+// word once per iteration (BiCGSTAB) or per Arnoldi step (FGMRES).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bcsc2bcsr.h"
+#include "bsr_tensor_SpMV.h"
+#include "bsrilu.h"
+#include "cape_sptrsv.h"
+#include "common.hpp"
+#include "internal.hpp"
+#include "isai_bsr.h"
+#include "isai_solve.h"
+
+namespace isai {
+
+constexpr int kRedBlocks = 512;
+constexpr int kRedThreads = 256;
+constexpr int kMaxRestart = 64;
+
+// Device scalar record of a solve.
+struct Scalars {
+    double rho, rho_old, alpha, omega, beta;
+    double bnorm2, rnorm2, snorm2, tol2;
+    double H[(kMaxRestart + 1) * kMaxRestart];
+    double cs[kMaxRestart], sn[kMaxRestart], g[kMaxRestart + 1], y[kMaxRestart];
+    int done;   // 0 running, 1 converged, 2 converged at s (BiCGSTAB), 3 breakdown
+    int first;
+    double partial[2][kRedBlocks];
+};
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double ws[kRedThreads / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) ws[w] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < kRedThreads / 64; ++k) t += ws[k];
+    return t;
+}
+
+// Sum of partial[slot][0..kRedBlocks) in a fixed order (one block).
+__device__ __forceinline__ double final_sum(const Scalars *S, int slot) {
+    double v = 0.0;
+    for (int k = threadIdx.x; k < kRedBlocks; k += kRedThreads) v += S->partial[slot][k];
+    return block_sum(v);
+}
+
+__device__ __forceinline__ bool skipped(const Scalars *S) { return S->done != 0; }
+
+// partial[0] += a.b ; partial[1] += a.c (when c != null)
+__global__ __launch_bounds__(kRedThreads) void dot_kernel(size_t n, const double *__restrict__ a,
+                                                          const double *__restrict__ b, const double *__restrict__ c,
+                                                          Scalars *S, int honor_done) {
+    if (honor_done && skipped(S)) return;
+    double s0 = 0.0, s1 = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads) {
+        const double av = a[i];
+        s0 += av * b[i];
+        if (c) s1 += av * c[i];
+    }
+    s0 = block_sum(s0);
+    if (c) s1 = block_sum(s1);
+    if (threadIdx.x == 0) {
+        S->partial[0][blockIdx.x] = s0;
+        S->partial[1][blockIdx.x] = c ? s1 : 0.0;
+    }
+}
+
+// ------------------------------------------------------------------ BiCGSTAB
+// init: r = b - Ax (r holds Ax), rhat = r, p = v = 0; partial ||b||^2, ||r||^2
+__global__ __launch_bounds__(kRedThreads) void bicg_init_kernel(size_t n, const double *__restrict__ b, double *r,
+                                                                double *rhat, double *p, double *v, Scalars *S) {
+    double sb = 0.0, sr = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads) {
+        const double bi = b[i], ri = bi - r[i];
+        r[i] = ri;
+        rhat[i] = ri;
+        p[i] = 0.0;
+        v[i] = 0.0;
+        sb += bi * bi;
+        sr += ri * ri;
+    }
+    sb = block_sum(sb);
+    sr = block_sum(sr);
+    if (threadIdx.x == 0) {
+        S->partial[0][blockIdx.x] = sb;
+        S->partial[1][blockIdx.x] = sr;
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void bicg_init_final(Scalars *S, double tol) {
+    const double b2 = final_sum(S, 0), r2 = final_sum(S, 1);
+    if (threadIdx.x == 0) {
+        S->bnorm2 = b2 > 0.0 ? b2 : 1.0;
+        S->rnorm2 = r2;
+        S->tol2 = tol * tol * S->bnorm2;
+        S->rho_old = S->alpha = S->omega = 1.0;
+        S->first = 1;
+        S->done = (r2 <= S->tol2) ? 1 : 0;
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void bicg_rho_final(Scalars *S) {
+    if (skipped(S)) return;
+    const double rho = final_sum(S, 0);
+    if (threadIdx.x == 0) {
+        S->rho = rho;
+        if (rho == 0.0) {
+            S->done = 3;
+        } else {
+            S->beta = S->first ? 0.0 : (rho / S->rho_old) * (S->alpha / S->omega);
+        }
+    }
+}
+
+__global__ void bicg_p_kernel(size_t n, const double *__restrict__ r, double *p, const double *__restrict__ v,
+                              const Scalars *S) {
+    if (skipped(S)) return;
+    const bool first = S->first;
+    const double beta = S->beta, omega = S->omega;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = first ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
+}
+
+__global__ __launch_bounds__(kRedThreads) void bicg_alpha_final(Scalars *S) {
+    if (skipped(S)) return;
+    const double d = final_sum(S, 0);
+    if (threadIdx.x == 0) {
+        if (d == 0.0) S->done = 3;
+        else S->alpha = S->rho / d;
+        S->first = 0;
+    }
+}
+
+// s = r - alpha v; partial ||s||^2
+__global__ __launch_bounds__(kRedThreads) void bicg_s_kernel(size_t n, const double *__restrict__ r,
+                                                             const double *__restrict__ v, double *__restrict__ s,
+                                                             Scalars *S) {
+    if (skipped(S)) return;
+    const double alpha = S->alpha;
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads) {
+        const double si = r[i] - alpha * v[i];
+        s[i] = si;
+        acc += si * si;
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) S->partial[0][blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(kRedThreads) void bicg_snorm_final(Scalars *S) {
+    if (skipped(S)) return;
+    const double s2 = final_sum(S, 0);
+    if (threadIdx.x == 0) {
+        S->snorm2 = s2;
+        if (s2 <= S->tol2) S->done = 2;
+    }
+}
+
+// x += alpha phat, only when the solve converged on s.
+__global__ void bicg_xs_kernel(size_t n, double *x, const double *__restrict__ phat, const Scalars *S) {
+    if (S->done != 2) return;
+    const double alpha = S->alpha;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        x[i] += alpha * phat[i];
+}
+
+__global__ __launch_bounds__(kRedThreads) void bicg_omega_final(Scalars *S) {
+    if (skipped(S)) return;
+    const double ts = final_sum(S, 0), tt = final_sum(S, 1);
+    if (threadIdx.x == 0) {
+        if (tt == 0.0) S->done = 3;
+        else S->omega = ts / tt;
+    }
+}
+
+// x += alpha phat + omega shat; r = s - omega t; partial ||r||^2
+__global__ __launch_bounds__(kRedThreads) void bicg_xr_kernel(size_t n, double *x, double *r,
+                                                              const double *__restrict__ phat,
+                                                              const double *__restrict__ shat,
+                                                              const double *__restrict__ s,
+                                                              const double *__restrict__ t, Scalars *S) {
+    if (skipped(S)) return;
+    const double alpha = S->alpha, omega = S->omega;
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads) {
+        x[i] += alpha * phat[i] + omega * shat[i];
+        const double ri = s[i] - omega * t[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) S->partial[0][blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(kRedThreads) void bicg_rnorm_final(Scalars *S) {
+    if (skipped(S)) return;
+    const double r2 = final_sum(S, 0);
+    if (threadIdx.x == 0) {
+        S->rnorm2 = r2;
+        S->rho_old = S->rho;
+        if (r2 <= S->tol2) S->done = 1;
+    }
+}
+
+// ------------------------------------------------------------- residual
+// r = b - r (r holds A x); partial ||r||^2, ||b||^2
+__global__ __launch_bounds__(kRedThreads) void residual_kernel(size_t n, const double *__restrict__ b, double *r,
+                                                               Scalars *S) {
+    double sr = 0.0, sb = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads) {
+        const double bi = b[i], ri = bi - r[i];
+        r[i] = ri;
+        sr += ri * ri;
+        sb += bi * bi;
+    }
+    sr = block_sum(sr);
+    sb = block_sum(sb);
+    if (threadIdx.x == 0) {
+        S->partial[0][blockIdx.x] = sr;
+        S->partial[1][blockIdx.x] = sb;
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void residual_final(Scalars *S, double *out2) {
+    const double r2 = final_sum(S, 0), b2 = final_sum(S, 1);
+    if (threadIdx.x == 0) {
+        out2[0] = r2;
+        out2[1] = b2;
+    }
+}
+
+// ------------------------------------------------------------------- FGMRES
+// cycle start: w holds A x -> w = b - Ax, partial ||w||^2, ||b||^2
+__global__ __launch_bounds__(kRedThreads) void fg_beta_final(Scalars *S, double tol) {
+    const double w2 = final_sum(S, 0), b2 = final_sum(S, 1);
+    if (threadIdx.x == 0) {
+        S->bnorm2 = b2 > 0.0 ? b2 : 1.0;
+        S->tol2 = tol * tol * S->bnorm2;
+        const double beta = sqrt(w2);
+        S->beta = beta;
+        for (int k = 0; k <= kMaxRestart; ++k) S->g[k] = 0.0;
+        S->g[0] = beta;
+        S->done = (w2 <= S->tol2) ? 1 : 0;
+    }
+}
+
+__global__ void fg_scale_kernel(size_t n, const double *__restrict__ w, double *__restrict__ v, const Scalars *S,
+                                int hrow, int hcol, int use_beta) {
+    if (S->done) return;
+    const double d = use_beta ? S->beta : S->H[hrow * kMaxRestart + hcol];
+    if (d == 0.0) return;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        v[i] = w[i] / d;
+}
+
+__global__ __launch_bounds__(kRedThreads) void fg_h_final(Scalars *S, int i, int j) {
+    if (skipped(S)) return;
+    const double h = final_sum(S, 0);
+    if (threadIdx.x == 0) S->H[i * kMaxRestart + j] = h;
+}
+
+__global__ void fg_axpy_h_kernel(size_t n, double *w, const double *__restrict__ vi, const Scalars *S, int i, int j) {
+    if (skipped(S)) return;
+    const double h = S->H[i * kMaxRestart + j];
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        w[k] -= h * vi[k];
+}
+
+// ||w|| -> H[j+1][j]; Givens rotations on column j; residual estimate |g[j+1]|.
+__global__ __launch_bounds__(kRedThreads) void fg_givens_final(Scalars *S, int j) {
+    if (skipped(S)) return;
+    const double w2 = final_sum(S, 0);
+    if (threadIdx.x != 0) return;
+    constexpr int m = kMaxRestart;
+    const double hn = sqrt(w2);
+    S->H[(j + 1) * m + j] = hn;
+    for (int i = 0; i < j; ++i) {
+        const double a = S->H[i * m + j], c = S->H[(i + 1) * m + j];
+        S->H[i * m + j] = S->cs[i] * a + S->sn[i] * c;
+        S->H[(i + 1) * m + j] = -S->sn[i] * a + S->cs[i] * c;
+    }
+    const double a = S->H[j * m + j], c = hn;
+    const double den = sqrt(a * a + c * c);
+    S->cs[j] = den == 0.0 ? 1.0 : a / den;
+    S->sn[j] = den == 0.0 ? 0.0 : c / den;
+    S->H[j * m + j] = S->cs[j] * a + S->sn[j] * c;
+    S->g[j + 1] = -S->sn[j] * S->g[j];
+    S->g[j] = S->cs[j] * S->g[j];
+    // keep the unrotated norm for the basis scaling
+    S->rnorm2 = hn;
+    if (S->g[j + 1] * S->g[j + 1] <= S->tol2 || hn == 0.0) S->done = 1;
+}
+
+__global__ void fg_scale_next_kernel(size_t n, const double *__restrict__ w, double *__restrict__ v,
+                                     const Scalars *S) {
+    if (S->done) return;
+    const double d = S->rnorm2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        v[i] = w[i] / d;
+}
+
+// Back substitution H(0:jend,0:jend) y = g.
+__global__ void fg_solve_y_kernel(Scalars *S, int jend) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    constexpr int m = kMaxRestart;
+    for (int i = jend - 1; i >= 0; --i) {
+        double s = S->g[i];
+        for (int k = i + 1; k < jend; ++k) s -= S->H[i * m + k] * S->y[k];
+        S->y[i] = s / S->H[i * m + i];
+    }
+}
+
+// x += sum_k y_k Z_k
+__global__ void fg_update_x_kernel(size_t n, double *x, const double *__restrict__ Z, const Scalars *S, int jend) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double acc = x[i];
+        for (int k = 0; k < jend; ++k) acc += S->y[k] * Z[(size_t)k * n + i];
+        x[i] = acc;
+    }
+}
+
+static unsigned vec_grid(size_t n) {
+    size_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    return (unsigned)(g ? g : 1);
+}
+
+}  // namespace isai
+
+using namespace isai;
+
+struct isai_handle {
+    int nb = 0, nnzb = 0, bs = 0;
+    size_t n = 0;
+    const int *d_row = nullptr, *d_col = nullptr;
+    const double *d_val = nullptr;
+    double *d_LU = nullptr;
+    int nnzbL = 0, nnzbU = 0, nlevels = 0;
+    int *d_Lrow = nullptr, *d_Lcol = nullptr, *d_Urow = nullptr, *d_Ucol = nullptr;
+    double *d_Lval = nullptr, *d_Uval = nullptr;
+    void *d_Llp = nullptr, *d_Ulp = nullptr;  // reduced-precision copies
+    int lp_prec = -1;
+    int precond = ISAI_PRECOND_ISAI, precision = ISAI_PREC_FP64, chunk = 0;
+    isai_sptrsv_plan_t sv = nullptr;
+    double ms_ilu = 0, ms_isai = 0, ms_total = 0;
+    double *work = nullptr;
+    size_t work_vecs = 0;
+    Scalars *S = nullptr;
+    double *d_res2 = nullptr;  // residual output pair
+    int *h_done = nullptr;      // pinned
+    double *h_res2 = nullptr;   // pinned
+};
+
+namespace {
+
+int ensure_work(isai_handle *h, size_t vecs) {
+    if (h->work_vecs >= vecs) return ISAI_OK;
+    if (h->work) (void)hipFree(h->work);
+    h->work = nullptr;
+    h->work_vecs = 0;
+    if (hipMalloc(&h->work, sizeof(double) * h->n * vecs) != hipSuccess) {
+        set_error("isai_solve: cannot allocate %zu work vectors of %zu doubles", vecs, h->n);
+        return ISAI_ERR_ALLOC;
+    }
+    h->work_vecs = vecs;
+    return ISAI_OK;
+}
+
+int convert_lp(isai_handle *h, int prec, hipStream_t st) {
+    if (prec == ISAI_PREC_FP64 || h->lp_prec == prec) return ISAI_OK;
+    const size_t esz = prec == ISAI_PREC_FP32 ? 4 : 2;
+    if (!h->d_Llp) {
+        if (hipMalloc(&h->d_Llp, 4 * 16 * (size_t)h->nnzbL) != hipSuccess ||
+            hipMalloc(&h->d_Ulp, 4 * 16 * (size_t)h->nnzbU) != hipSuccess) {
+            set_error("isai: cannot allocate reduced-precision factors");
+            return ISAI_ERR_ALLOC;
+        }
+    }
+    (void)esz;
+    int s = isai_convert_values((size_t)h->nnzbL * 16, (isai_precision_t)prec, h->d_Lval, h->d_Llp, (isai_stream_t)st);
+    if (s == ISAI_OK) s = isai_convert_values((size_t)h->nnzbU * 16, (isai_precision_t)prec, h->d_Uval, h->d_Ulp, (isai_stream_t)st);
+    if (s == ISAI_OK) h->lp_prec = prec;
+    return s;
+}
+
+// z = M v; tmp is a work vector (ISAI form)
+int precond_apply(isai_handle *h, const double *v, double *z, double *tmp, hipStream_t st, const int *skip) {
+    switch (h->precond) {
+        case ISAI_PRECOND_NONE:
+            ISAI_HIP(hipMemcpyAsync(z, v, sizeof(double) * h->n, hipMemcpyDeviceToDevice, st));
+            return ISAI_OK;
+        case ISAI_PRECOND_ISAI: {
+            const isai_precision_t p = (isai_precision_t)h->precision;
+            const void *L = p == ISAI_PREC_FP64 ? (const void *)h->d_Lval : h->d_Llp;
+            const void *U = p == ISAI_PREC_FP64 ? (const void *)h->d_Uval : h->d_Ulp;
+            int s = spmv_launch(h->nb, h->nnzbL, h->bs, p, h->d_Lrow, h->d_Lcol, L, v, tmp, st, skip);
+            if (s == ISAI_OK) s = spmv_launch(h->nb, h->nnzbU, h->bs, p, h->d_Urow, h->d_Ucol, U, tmp, z, st, skip);
+            return s;
+        }
+        case ISAI_PRECOND_ILU_SPTRSV:
+            return sptrsv_solve_internal(h->sv, h->d_LU, v, z, st, skip);
+        default:
+            set_error("isai: unknown preconditioner %d", h->precond);
+            return ISAI_ERR_INVALID_ARG;
+    }
+}
+
+int true_residual(isai_handle *h, const double *b, const double *x, double *r, double *relres, hipStream_t st) {
+    int s = spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, x, r, st, nullptr);
+    if (s != ISAI_OK) return s;
+    residual_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(h->n, b, r, h->S);
+    residual_final<<<1, kRedThreads, 0, st>>>(h->S, h->d_res2);
+    ISAI_LAUNCH_CHECK();
+    ISAI_HIP(hipMemcpyAsync(h->h_res2, h->d_res2, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipStreamSynchronize(st));
+    const double b2 = h->h_res2[1] > 0.0 ? h->h_res2[1] : 1.0;
+    *relres = std::sqrt(h->h_res2[0]) / std::sqrt(b2);
+    return ISAI_OK;
+}
+
+int read_done(isai_handle *h, hipStream_t st, int *done) {
+    ISAI_HIP(hipMemcpyAsync(h->h_done, &h->S->done, sizeof(int), hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipStreamSynchronize(st));
+    *done = *h->h_done;
+    return ISAI_OK;
+}
+
+#define TRY(x)                          \
+    do {                                \
+        int _s = (x);                   \
+        if (_s != ISAI_OK) return _s;   \
+    } while (0)
+
+int bicgstab(isai_handle *h, double tol, int maxit, const double *b, double *x, int *iters, double *relres,
+             hipStream_t st) {
+    TRY(ensure_work(h, 9));
+    const size_t n = h->n;
+    double *r = h->work, *rh = r + n, *p = rh + n, *v = p + n, *ph = v + n, *s = ph + n, *sh = s + n, *t = sh + n,
+           *tmp = t + n;
+    const int *skip = &h->S->done;
+    int it = 0;
+    for (int restart = 0; restart < 8; ++restart) {
+        TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, x, r, st, nullptr));
+        bicg_init_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, b, r, rh, p, v, h->S);
+        bicg_init_final<<<1, kRedThreads, 0, st>>>(h->S, tol);
+        ISAI_LAUNCH_CHECK();
+        int done = 0;
+        TRY(read_done(h, st, &done));
+        while (!done && it < maxit) {
+            ++it;
+            dot_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, rh, r, nullptr, h->S, 1);
+            bicg_rho_final<<<1, kRedThreads, 0, st>>>(h->S);
+            bicg_p_kernel<<<vec_grid(n), 256, 0, st>>>(n, r, p, v, h->S);
+            TRY(precond_apply(h, p, ph, tmp, st, skip));
+            TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, ph, v, st, skip));
+            dot_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, rh, v, nullptr, h->S, 1);
+            bicg_alpha_final<<<1, kRedThreads, 0, st>>>(h->S);
+            bicg_s_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, r, v, s, h->S);
+            bicg_snorm_final<<<1, kRedThreads, 0, st>>>(h->S);
+            bicg_xs_kernel<<<vec_grid(n), 256, 0, st>>>(n, x, ph, h->S);
+            TRY(precond_apply(h, s, sh, tmp, st, skip));
+            TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, sh, t, st, skip));
+            dot_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, t, s, t, h->S, 1);
+            bicg_omega_final<<<1, kRedThreads, 0, st>>>(h->S);
+            bicg_xr_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, x, r, ph, sh, s, t, h->S);
+            bicg_rnorm_final<<<1, kRedThreads, 0, st>>>(h->S);
+            ISAI_LAUNCH_CHECK();
+            TRY(read_done(h, st, &done));
+        }
+        TRY(true_residual(h, b, x, r, relres, st));
+        *iters = it;
+        if (*relres <= tol) return ISAI_OK;
+        if (it >= maxit) break;
+        // recursive residual drifted from the true one, or a breakdown: restart
+    }
+    set_error("bicgstab: not converged after %d iterations (relres %.3e)", it, *relres);
+    return ISAI_ERR_NOT_CONVERGED;
+}
+
+int fgmres(isai_handle *h, int m, double tol, int maxit, const double *b, double *x, int *iters, double *relres,
+           hipStream_t st) {
+    if (m <= 0) m = 30;
+    if (m > kMaxRestart) {
+        set_error("fgmres: restart %d above %d", m, kMaxRestart);
+        return ISAI_ERR_INVALID_ARG;
+    }
+    TRY(ensure_work(h, (size_t)2 * m + 3));
+    const size_t n = h->n;
+    double *V = h->work, *Z = V + n * (m + 1), *w = Z + n * m, *tmp = w + n;
+    const int *skip = &h->S->done;
+    int it = 0;
+    *relres = INFINITY;
+    while (it < maxit) {
+        TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, x, w, st, nullptr));
+        residual_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, b, w, h->S);
+        fg_beta_final<<<1, kRedThreads, 0, st>>>(h->S, tol);
+        fg_scale_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V, h->S, 0, 0, 1);
+        ISAI_LAUNCH_CHECK();
+        int done = 0;
+        TRY(read_done(h, st, &done));
+        if (done) break;
+        int jend = 0;
+        for (int j = 0; j < m && it < maxit; ++j) {
+            ++it;
+            double *vj = V + n * j, *zj = Z + n * j;
+            TRY(precond_apply(h, vj, zj, tmp, st, skip));
+            TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, zj, w, st, skip));
+            for (int i = 0; i <= j; ++i) {
+                dot_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, V + n * i, nullptr, h->S, 1);
+                fg_h_final<<<1, kRedThreads, 0, st>>>(h->S, i, j);
+                fg_axpy_h_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V + n * i, h->S, i, j);
+            }
+            dot_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, w, nullptr, h->S, 1);
+            fg_givens_final<<<1, kRedThreads, 0, st>>>(h->S, j);
+            fg_scale_next_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V + n * (j + 1), h->S);
+            ISAI_LAUNCH_CHECK();
+            jend = j + 1;
+            TRY(read_done(h, st, &done));
+            if (done) break;
+        }
+        fg_solve_y_kernel<<<1, 64, 0, st>>>(h->S, jend);
+        fg_update_x_kernel<<<vec_grid(n), 256, 0, st>>>(n, x, Z, h->S, jend);
+        ISAI_LAUNCH_CHECK();
+        TRY(true_residual(h, b, x, w, relres, st));
+        if (*relres <= tol) break;
+    }
+    *iters = it;
+    if (*relres <= tol) return ISAI_OK;
+    set_error("fgmres: not converged after %d iterations (relres %.3e)", it, *relres);
+    return ISAI_ERR_NOT_CONVERGED;
+}
+
+}  // namespace
+
+extern "C" {
+
+int isai_destroy(isai_handle_t h) {
+    if (!h) return ISAI_OK;
+    (void)hipFree(h->d_LU);
+    (void)hipFree(h->d_Lrow);
+    (void)hipFree(h->d_Lcol);
+    (void)hipFree(h->d_Urow);
+    (void)hipFree(h->d_Ucol);
+    (void)hipFree(h->d_Lval);
+    (void)hipFree(h->d_Uval);
+    (void)hipFree(h->d_Llp);
+    (void)hipFree(h->d_Ulp);
+    (void)hipFree(h->work);
+    (void)hipFree(h->S);
+    (void)hipFree(h->d_res2);
+    if (h->h_done) (void)hipHostFree(h->h_done);
+    if (h->h_res2) (void)hipHostFree(h->h_res2);
+    if (h->sv) isai_bsr_sptrsv_destroy(h->sv);
+    delete h;
+    return ISAI_OK;
+}
+
+int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, const double *d_val,
+               const isai_setup_opts_t *opts, isai_handle_t *out, isai_stream_t stream) {
+    ISAI_REQUIRE(out && d_row && d_col && d_val, "isai_setup: null argument");
+    ISAI_REQUIRE(nb > 0 && nnzb >= nb, "isai_setup: bad size nb=%d nnzb=%d", nb, nnzb);
+    if (bs != 4) {
+        set_error("isai_setup: the solve path is built for bs = 4 (got %d)", bs);
+        return ISAI_ERR_UNSUPPORTED;
+    }
+    *out = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    auto *h = new isai_handle;
+    h->nb = nb;
+    h->nnzb = nnzb;
+    h->bs = bs;
+    h->n = (size_t)nb * bs;
+    h->d_row = d_row;
+    h->d_col = d_col;
+    h->d_val = d_val;
+    if (opts) {
+        h->precond = opts->precond;
+        h->precision = opts->precision;
+        h->chunk = opts->sptrsv_chunk;
+    }
+    int s = ISAI_OK;
+    auto fail = [&](int code) {
+        isai_destroy(h);
+        return code;
+    };
+    const size_t b2 = (size_t)bs * bs;
+    if (hipMalloc(&h->d_LU, sizeof(double) * b2 * nnzb) != hipSuccess || hipMalloc(&h->S, sizeof(Scalars)) != hipSuccess ||
+        hipMalloc(&h->d_res2, 2 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&h->h_done, sizeof(int)) != hipSuccess || hipHostMalloc(&h->h_res2, 2 * sizeof(double)) != hipSuccess) {
+        set_error("isai_setup: allocation failed");
+        return fail(ISAI_ERR_ALLOC);
+    }
+    hipEvent_t e0, e1, e1b, e2;
+    ISAI_HIP(hipEventCreate(&e0));
+    ISAI_HIP(hipEventCreate(&e1));
+    ISAI_HIP(hipEventCreate(&e1b));
+    ISAI_HIP(hipEventCreate(&e2));
+
+    // --- host pattern work (analysis): BCSC / BCSR patterns and maps
+    std::vector<int> rp(nb + 1), ci(nnzb);
+    ISAI_HIP(hipMemcpyAsync(rp.data(), d_row, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipMemcpyAsync(ci.data(), d_col, sizeof(int) * nnzb, hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipStreamSynchronize(st));
+    struct Fac {
+        std::vector<int> mb, mc, trp, tci, map;
+        int nmax = 0;
+    } F[2];
+    for (int f = 0; f < 2; ++f) {
+        const int lower = f == 0;
+        int nnz = 0;
+        if ((s = isai_bcsc_pattern_count(nb, rp.data(), ci.data(), lower, &nnz)) != ISAI_OK) return fail(s);
+        F[f].mb.resize(nb + 1);
+        F[f].mc.resize(nnz);
+        if ((s = isai_bcsc_pattern(nb, rp.data(), ci.data(), lower, F[f].mb.data(), F[f].mc.data())) != ISAI_OK)
+            return fail(s);
+        int tn = 0;
+        isai_tri_bcsr_pattern_count(nb, rp.data(), ci.data(), lower, &tn);
+        if (tn != nnz) {
+            set_error("isai_setup: %s pattern mismatch: BCSR nnzb = %d, BCSC nnzb = %d (missing diagonal block?)",
+                      lower ? "L" : "U", tn, nnz);
+            return fail(ISAI_ERR_PATTERN);
+        }
+        F[f].trp.resize(nb + 1);
+        F[f].tci.resize(nnz);
+        isai_tri_bcsr_pattern(nb, rp.data(), ci.data(), lower, F[f].trp.data(), F[f].tci.data());
+        F[f].map.resize(nnz);
+        if ((s = isai_map_bcsc2bcsr(nb, F[f].mb.data(), F[f].mc.data(), F[f].trp.data(), F[f].tci.data(),
+                                    F[f].map.data())) != ISAI_OK)
+            return fail(s);
+        for (int j = 0; j < nb; ++j) F[f].nmax = std::max(F[f].nmax, F[f].mb[j + 1] - F[f].mb[j]);
+    }
+    h->nnzbL = (int)F[0].mc.size();
+    h->nnzbU = (int)F[1].mc.size();
+
+    // --- ILU(0) on a copy of A
+    isai_ilu_plan_t ilu = nullptr;
+    int zp = -1;
+    ISAI_HIP(hipMemcpyAsync(h->d_LU, d_val, sizeof(double) * b2 * nnzb, hipMemcpyDeviceToDevice, st));
+    if ((s = isai_bsrilu0_analysis(nb, nnzb, bs, d_row, d_col, &ilu, &zp, stream)) != ISAI_OK) return fail(s);
+    isai_bsrilu0_plan_info(ilu, &h->nlevels, nullptr);
+    ISAI_HIP(hipEventRecord(e0, st));
+    s = isai_bsrilu0_factor(ilu, h->d_LU, &zp, stream);
+    ISAI_HIP(hipEventRecord(e1, st));
+    isai_bsrilu0_destroy(ilu);
+    if (s != ISAI_OK) return fail(s);
+
+    // --- ISAI construction
+    int *d_buf[2][5] = {};  // mb, mc, pp, pl, ps
+    double *d_dinv[2] = {}, *d_mval[2] = {};
+    int *d_map[2] = {};
+    int totals[2] = {0, 0};
+    auto cleanup = [&]() {
+        for (int f = 0; f < 2; ++f) {
+            for (int k = 0; k < 5; ++k) (void)hipFree(d_buf[f][k]);
+            (void)hipFree(d_dinv[f]);
+            (void)hipFree(d_mval[f]);
+            (void)hipFree(d_map[f]);
+        }
+    };
+    for (int f = 0; f < 2; ++f) {
+        const size_t nnz = F[f].mc.size();
+        bool ok = hipMalloc(&d_buf[f][0], sizeof(int) * (nb + 1)) == hipSuccess &&
+                  hipMalloc(&d_buf[f][1], sizeof(int) * nnz) == hipSuccess &&
+                  hipMalloc(&d_buf[f][2], sizeof(int) * (nnz + 1)) == hipSuccess &&
+                  hipMalloc(&d_dinv[f], sizeof(double) * b2 * nb) == hipSuccess &&
+                  hipMalloc(&d_mval[f], sizeof(double) * b2 * nnz) == hipSuccess &&
+                  hipMalloc(&d_map[f], sizeof(int) * nnz) == hipSuccess;
+        if (!ok) {
+            cleanup();
+            set_error("isai_setup: allocation failed");
+            return fail(ISAI_ERR_ALLOC);
+        }
+        ISAI_HIP(hipMemcpyAsync(d_buf[f][0], F[f].mb.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st));
+        ISAI_HIP(hipMemcpyAsync(d_buf[f][1], F[f].mc.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, st));
+        ISAI_HIP(hipMemcpyAsync(d_map[f], F[f].map.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, st));
+    }
+    int *factor_rows[2] = {nullptr, nullptr}, *factor_cols[2] = {nullptr, nullptr};
+    double *factor_vals[2] = {nullptr, nullptr};
+    for (int f = 0; f < 2; ++f) {
+        const size_t nnz = F[f].mc.size();
+        if (hipMalloc(&factor_rows[f], sizeof(int) * (nb + 1)) != hipSuccess ||
+            hipMalloc(&factor_cols[f], sizeof(int) * nnz) != hipSuccess ||
+            hipMalloc(&factor_vals[f], sizeof(double) * b2 * nnz) != hipSuccess) {
+            cleanup();
+            set_error("isai_setup: allocation failed");
+            return fail(ISAI_ERR_ALLOC);
+        }
+        ISAI_HIP(hipMemcpyAsync(factor_rows[f], F[f].trp.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st));
+        ISAI_HIP(hipMemcpyAsync(factor_cols[f], F[f].tci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, st));
+    }
+    h->d_Lrow = factor_rows[0];
+    h->d_Lcol = factor_cols[0];
+    h->d_Lval = factor_vals[0];
+    h->d_Urow = factor_rows[1];
+    h->d_Ucol = factor_cols[1];
+    h->d_Uval = factor_vals[1];
+    ISAI_HIP(hipStreamSynchronize(st));
+    ISAI_HIP(hipEventRecord(e1b, st));
+    // pair lists: the count pass returns the totals (one sync each, as main.cu:690)
+    for (int f = 0; f < 2; ++f) {
+        const int nnz = (int)F[f].mc.size();
+        s = isai_pairs_count(nb, nnz, d_row, d_col, d_buf[f][0], d_buf[f][1], f == 1, d_buf[f][2], &totals[f], stream);
+        if (s != ISAI_OK) { cleanup(); return fail(s); }
+        if (hipMalloc(&d_buf[f][3], sizeof(int) * std::max(totals[f], 1)) != hipSuccess ||
+            hipMalloc(&d_buf[f][4], sizeof(int) * std::max(totals[f], 1)) != hipSuccess) {
+            cleanup();
+            set_error("isai_setup: allocation failed");
+            return fail(ISAI_ERR_ALLOC);
+        }
+    }
+    for (int f = 0; f < 2 && s == ISAI_OK; ++f) {
+        s = isai_pairs_fill(nb, d_row, d_col, d_buf[f][0], d_buf[f][1], f == 1, d_buf[f][2], d_buf[f][3], d_buf[f][4], stream);
+        if (s == ISAI_OK) s = isai_build_dinv(nb, bs, d_row, d_col, h->d_LU, f == 0, d_dinv[f], stream);
+    }
+    if (s == ISAI_OK)
+        s = isai_build_lu(nb, bs, h->d_LU, d_buf[0][0], d_buf[0][1], d_buf[0][2], d_buf[0][3], d_buf[0][4], d_dinv[0],
+                          d_mval[0], d_buf[1][0], d_buf[1][1], d_buf[1][2], d_buf[1][3], d_buf[1][4], d_dinv[1],
+                          d_mval[1], std::max(F[0].nmax, F[1].nmax), stream);
+    for (int f = 0; f < 2 && s == ISAI_OK; ++f)
+        s = isai_reorder_bcsc_to_bcsr((int)F[f].mc.size(), bs, d_map[f], d_mval[f], factor_vals[f], stream);
+    if (s == ISAI_OK && h->precision != ISAI_PREC_FP64) s = convert_lp(h, h->precision, st);
+    ISAI_HIP(hipEventRecord(e2, st));
+    ISAI_HIP(hipStreamSynchronize(st));
+    cleanup();
+    if (s != ISAI_OK) return fail(s);
+    float t_ilu = 0.f, t_isai = 0.f;
+    ISAI_HIP(hipEventElapsedTime(&t_ilu, e0, e1));
+    ISAI_HIP(hipEventElapsedTime(&t_isai, e1b, e2));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e1b);
+    (void)hipEventDestroy(e2);
+    h->ms_ilu = t_ilu;
+    h->ms_isai = t_isai;
+    h->ms_total = t_ilu + t_isai;
+    if (h->precond == ISAI_PRECOND_ILU_SPTRSV) {
+        s = isai_bsr_sptrsv_analysis(nb, nnzb, bs, d_row, d_col, h->d_LU, h->chunk, &h->sv, stream);
+        if (s != ISAI_OK) return fail(s);
+    }
+    *out = h;
+    return ISAI_OK;
+}
+
+int isai_setup_times(isai_handle_t h, double *ms_ilu, double *ms_isai, double *ms_total) {
+    ISAI_REQUIRE(h, "isai_setup_times: null handle");
+    if (ms_ilu) *ms_ilu = h->ms_ilu;
+    if (ms_isai) *ms_isai = h->ms_isai;
+    if (ms_total) *ms_total = h->ms_total;
+    return ISAI_OK;
+}
+
+int isai_handle_info(isai_handle_t h, int *nnzbL, int *nnzbU, int *nlevels) {
+    ISAI_REQUIRE(h, "isai_handle_info: null handle");
+    if (nnzbL) *nnzbL = h->nnzbL;
+    if (nnzbU) *nnzbU = h->nnzbU;
+    if (nlevels) *nlevels = h->nlevels;
+    return ISAI_OK;
+}
+
+int isai_handle_factors(isai_handle_t h, const int **Lrow, const int **Lcol, const double **Lval, const int **Urow,
+                        const int **Ucol, const double **Uval, const double **LU) {
+    ISAI_REQUIRE(h, "isai_handle_factors: null handle");
+    if (Lrow) *Lrow = h->d_Lrow;
+    if (Lcol) *Lcol = h->d_Lcol;
+    if (Lval) *Lval = h->d_Lval;
+    if (Urow) *Urow = h->d_Urow;
+    if (Ucol) *Ucol = h->d_Ucol;
+    if (Uval) *Uval = h->d_Uval;
+    if (LU) *LU = h->d_LU;
+    return ISAI_OK;
+}
+
+int isai_set_preconditioner(isai_handle_t h, int precond, int precision, isai_stream_t stream) {
+    ISAI_REQUIRE(h, "isai_set_preconditioner: null handle");
+    ISAI_REQUIRE(precond >= 0 && precond <= 2 && precision >= 0 && precision <= 2,
+                 "isai_set_preconditioner: bad kind %d / precision %d", precond, precision);
+    hipStream_t st = (hipStream_t)stream;
+    if (precond == ISAI_PRECOND_ILU_SPTRSV && !h->sv) {
+        int s = isai_bsr_sptrsv_analysis(h->nb, h->nnzb, h->bs, h->d_row, h->d_col, h->d_LU, h->chunk, &h->sv, stream);
+        if (s != ISAI_OK) return s;
+    }
+    if (precond == ISAI_PRECOND_ISAI && precision != ISAI_PREC_FP64) {
+        int s = convert_lp(h, precision, st);
+        if (s != ISAI_OK) return s;
+    }
+    h->precond = precond;
+    h->precision = precision;
+    return ISAI_OK;
+}
+
+int isai_precond_apply(isai_handle_t h, const double *d_v, double *d_z, isai_stream_t stream) {
+    ISAI_REQUIRE(h && d_v && d_z, "isai_precond_apply: null argument");
+    int s = ensure_work(h, 9);
+    if (s != ISAI_OK) return s;
+    double *tmp = h->work + 8 * h->n;
+    return precond_apply(h, d_v, d_z, tmp, (hipStream_t)stream, nullptr);
+}
+
+int isai_solve(isai_handle_t h, int solver, int restart, double tol, int maxit, const double *d_b, double *d_x,
+               int *iters, double *relres, isai_stream_t stream) {
+    ISAI_REQUIRE(h && d_b && d_x && iters && relres, "isai_solve: null argument");
+    ISAI_REQUIRE(tol > 0.0 && maxit >= 0, "isai_solve: bad tol / maxit");
+    *iters = 0;
+    *relres = INFINITY;
+    hipStream_t st = (hipStream_t)stream;
+    if (solver == ISAI_SOLVER_BICGSTAB) return bicgstab(h, tol, maxit, d_b, d_x, iters, relres, st);
+    if (solver == ISAI_SOLVER_FGMRES) return fgmres(h, restart, tol, maxit, d_b, d_x, iters, relres, st);
+    set_error("isai_solve: unknown solver %d", solver);
+    return ISAI_ERR_INVALID_ARG;
+}
+
+}  // extern "C"

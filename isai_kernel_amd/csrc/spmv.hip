@@ -16,6 +16,7 @@
 #include "bcsc2bcsr.h"
 #include "bsr_tensor_SpMV.h"
 #include "common.hpp"
+#include "internal.hpp"
 #include "isai_runtime.h"
 
 namespace isai {
@@ -26,7 +27,9 @@ constexpr int kSpmvBlock = 256;
 template <int UNROLL>
 __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f64_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci,
-    const v2d *__restrict__ val2, const v2d *__restrict__ x2, double *__restrict__ y) {
+    const v2d *__restrict__ val2, const v2d *__restrict__ x2, double *__restrict__ y,
+    const int *__restrict__ skip) {
+    if (skip && *skip) return;
     const int wg = xcd_remap(blockIdx.x, gridDim.x);
     const int br = wg * (kSpmvBlock / 8) + (threadIdx.x >> 3);
     const int s = threadIdx.x & 7;
@@ -68,7 +71,9 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f64_kernel(
 template <int UNROLL>
 __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f32_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci,
-    const v4f *__restrict__ val4, const v2d *__restrict__ x2, double *__restrict__ y) {
+    const v4f *__restrict__ val4, const v2d *__restrict__ x2, double *__restrict__ y,
+    const int *__restrict__ skip) {
+    if (skip && *skip) return;
     const int wg = xcd_remap(blockIdx.x, gridDim.x);
     const int br = wg * (kSpmvBlock / 4) + (threadIdx.x >> 2);
     const int r = threadIdx.x & 3;
@@ -94,7 +99,9 @@ __device__ __forceinline__ float bf16_hi(unsigned int w) { return __uint_as_floa
 
 __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_bf16_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci,
-    const v4u *__restrict__ val8, const v2d *__restrict__ x2, double *__restrict__ y) {
+    const v4u *__restrict__ val8, const v2d *__restrict__ x2, double *__restrict__ y,
+    const int *__restrict__ skip) {
+    if (skip && *skip) return;
     const int wg = xcd_remap(blockIdx.x, gridDim.x);
     const int br = wg * (kSpmvBlock / 2) + (threadIdx.x >> 1);
     const int h = threadIdx.x & 1;  // rows 2h, 2h+1
@@ -123,7 +130,9 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_bf16_kernel(
 // ---------------------------------------------------- generic bs (1..8), FP64
 __global__ void bsr_spmv_generic_kernel(int nb, int bs, const int *__restrict__ rp,
                                         const int *__restrict__ ci, const double *__restrict__ val,
-                                        const double *__restrict__ x, double *__restrict__ y) {
+                                        const double *__restrict__ x, double *__restrict__ y,
+                                        const int *__restrict__ skip) {
+    if (skip && *skip) return;
     const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= (long long)nb * bs) return;
     const int br = (int)(row / bs), r = (int)(row % bs), bs2 = bs * bs;
@@ -190,7 +199,7 @@ __global__ void bsr2csr_kernel(int nb, int bs, const int *__restrict__ brp, cons
 }
 
 int spmv_launch(int mb, int nnzb, int bs, isai_precision_t prec, const int *rp, const int *ci,
-                const void *val, const double *x, double *y, hipStream_t st) {
+                const void *val, const double *x, double *y, hipStream_t st, const int *skip) {
     ISAI_REQUIRE(mb >= 0 && nnzb >= 0, "bsr_spmv: negative size (mb=%d nnzb=%d)", mb, nnzb);
     ISAI_REQUIRE(bs >= 1 && bs <= 8, "bsr_spmv: bs=%d outside 1..8", bs);
     if (mb == 0) return ISAI_OK;
@@ -199,15 +208,15 @@ int spmv_launch(int mb, int nnzb, int bs, isai_precision_t prec, const int *rp, 
         if (prec == ISAI_PREC_FP64) {
             const unsigned g = grid_for(mb, kSpmvBlock / 8);
             bsr4_spmv_f64_kernel<2><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v2d *)val,
-                                                              (const v2d *)x, y);
+                                                              (const v2d *)x, y, skip);
         } else if (prec == ISAI_PREC_FP32) {
             const unsigned g = grid_for(mb, kSpmvBlock / 4);
             bsr4_spmv_f32_kernel<1><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v4f *)val,
-                                                              (const v2d *)x, y);
+                                                              (const v2d *)x, y, skip);
         } else if (prec == ISAI_PREC_BF16) {
             const unsigned g = grid_for(mb, kSpmvBlock / 2);
             bsr4_spmv_bf16_kernel<<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v4u *)val,
-                                                            (const v2d *)x, y);
+                                                            (const v2d *)x, y, skip);
         } else {
             set_error("bsr_spmv: unknown precision %d", (int)prec);
             return ISAI_ERR_INVALID_ARG;
@@ -218,7 +227,7 @@ int spmv_launch(int mb, int nnzb, int bs, isai_precision_t prec, const int *rp, 
             return ISAI_ERR_UNSUPPORTED;
         }
         const unsigned g = grid_for((long long)mb * bs, 256);
-        bsr_spmv_generic_kernel<<<g, 256, 0, st>>>(mb, bs, rp, ci, (const double *)val, x, y);
+        bsr_spmv_generic_kernel<<<g, 256, 0, st>>>(mb, bs, rp, ci, (const double *)val, x, y, skip);
     }
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
@@ -233,13 +242,13 @@ extern "C" {
 int isai_bsr_spmv(int mb, int nnzb, int bs, const int *d_rowptr, const int *d_colind,
                   const double *d_val, const double *d_x, double *d_y, isai_stream_t stream) {
     return spmv_launch(mb, nnzb, bs, ISAI_PREC_FP64, d_rowptr, d_colind, d_val, d_x, d_y,
-                       (hipStream_t)stream);
+                       (hipStream_t)stream, nullptr);
 }
 
 int isai_bsr_spmv_prec(int mb, int nnzb, int bs, isai_precision_t prec, const int *d_rowptr,
                        const int *d_colind, const void *d_val, const double *d_x, double *d_y,
                        isai_stream_t stream) {
-    return spmv_launch(mb, nnzb, bs, prec, d_rowptr, d_colind, d_val, d_x, d_y, (hipStream_t)stream);
+    return spmv_launch(mb, nnzb, bs, prec, d_rowptr, d_colind, d_val, d_x, d_y, (hipStream_t)stream, nullptr);
 }
 
 int isai_convert_values(size_t count, isai_precision_t prec, const double *d_src, void *d_dst,
@@ -269,9 +278,9 @@ int isai_apply(int mb, int bs, isai_precision_t prec, const int *d_Lrow, const i
                const void *d_Uval, int nnzbU, const double *d_b, double *d_y, double *d_x,
                isai_stream_t stream) {
     hipStream_t st = (hipStream_t)stream;
-    int s = spmv_launch(mb, nnzbL, bs, prec, d_Lrow, d_Lcol, d_Lval, d_b, d_y, st);
+    int s = spmv_launch(mb, nnzbL, bs, prec, d_Lrow, d_Lcol, d_Lval, d_b, d_y, st, nullptr);
     if (s != ISAI_OK) return s;
-    return spmv_launch(mb, nnzbU, bs, prec, d_Urow, d_Ucol, d_Uval, d_y, d_x, st);
+    return spmv_launch(mb, nnzbU, bs, prec, d_Urow, d_Ucol, d_Uval, d_y, d_x, st, nullptr);
 }
 
 int isai_reorder_bcsc_to_bcsr(int nnzb, int bs, const int *d_map, const double *d_src,
@@ -327,12 +336,12 @@ int isai_bsr_spmv_bench(const int *bsrRowPtr, const int *bsrColIdx, const double
     BENCH_TRY(hipEventCreate(&e0));
     BENCH_TRY(hipEventCreate(&e1));
     for (int i = 0; i < warmup && status == ISAI_OK; ++i)
-        status = spmv_launch(mb, nnzb, bdim, ISAI_PREC_FP64, d_rp, d_ci, d_v, d_x, d_y, nullptr);
+        status = spmv_launch(mb, nnzb, bdim, ISAI_PREC_FP64, d_rp, d_ci, d_v, d_x, d_y, nullptr, nullptr);
     if (status != ISAI_OK) goto done;
     BENCH_TRY(hipDeviceSynchronize());
     BENCH_TRY(hipEventRecord(e0, nullptr));
     for (int i = 0; i < iters && status == ISAI_OK; ++i)
-        status = spmv_launch(mb, nnzb, bdim, ISAI_PREC_FP64, d_rp, d_ci, d_v, d_x, d_y, nullptr);
+        status = spmv_launch(mb, nnzb, bdim, ISAI_PREC_FP64, d_rp, d_ci, d_v, d_x, d_y, nullptr, nullptr);
     if (status != ISAI_OK) goto done;
     BENCH_TRY(hipEventRecord(e1, nullptr));
     BENCH_TRY(hipEventSynchronize(e1));

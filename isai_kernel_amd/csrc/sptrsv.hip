@@ -26,6 +26,7 @@
 
 #include "cape_sptrsv.h"
 #include "common.hpp"
+#include "internal.hpp"
 #include "isai_bsr.h"
 
 namespace isai {
@@ -73,7 +74,9 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_kernel(int nb, int chunk, int 
                                                          const int *__restrict__ diagpos,
                                                          const double *__restrict__ dinv,
                                                          const double *rhs, double *out,
-                                                         double *sentinel_fill, Counters *cnt) {
+                                                         double *sentinel_fill, Counters *cnt,
+                                                         const int *__restrict__ skip) {
+    if (skip && *skip) return;
     __shared__ double xs[kMaxChunk * 4];
     __shared__ int s_c;
     const int lane = threadIdx.x;
@@ -228,7 +231,8 @@ __global__ void diag_pos_kernel(int nb, const int *__restrict__ rp, const int *_
     if (pos < 0) atomicMin(missing, i);
 }
 
-__global__ void fill_sentinel_kernel(double *p, size_t n) {
+__global__ void fill_sentinel_kernel(double *p, size_t n, const int *__restrict__ skip) {
+    if (skip && *skip) return;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
         p[k] = __longlong_as_double((long long)kSentinel);
 }
@@ -249,6 +253,22 @@ struct isai_sptrsv_plan {
     double *d_dinvL = nullptr, *d_dinvU = nullptr, *d_y = nullptr;
     Counters *d_cnt = nullptr;
 };
+
+namespace isai {
+
+int sptrsv_solve_internal(isai_sptrsv_plan_t P, const double *d_LU, const double *d_b, double *d_x,
+                          hipStream_t st, const int *skip) {
+    const size_t n = (size_t)P->nb * 4;
+    fill_sentinel_kernel<<<fill_blocks(n), 256, 0, st>>>(P->d_y, n, skip);
+    bsr4_sptrsv_kernel<false><<<P->nchunks, 64, 0, st>>>(P->nb, P->chunk, P->nchunks, P->d_row, P->d_col, d_LU,
+                                                         P->d_diag, P->d_dinvL, d_b, P->d_y, d_x, P->d_cnt, skip);
+    bsr4_sptrsv_kernel<true><<<P->nchunks, 64, 0, st>>>(P->nb, P->chunk, P->nchunks, P->d_row, P->d_col, d_LU,
+                                                        P->d_diag, P->d_dinvU, P->d_y, d_x, nullptr, P->d_cnt, skip);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+}  // namespace isai
 
 extern "C" {
 
@@ -312,15 +332,7 @@ int isai_bsr_sptrsv_update_values(isai_sptrsv_plan_t P, const double *d_LU, isai
 int isai_bsr_sptrsv_solve(isai_sptrsv_plan_t P, const double *d_LU, const double *d_b, double *d_x,
                           isai_stream_t stream) {
     ISAI_REQUIRE(P && d_LU && d_b && d_x, "isai_bsr_sptrsv_solve: null argument");
-    hipStream_t st = (hipStream_t)stream;
-    const size_t n = (size_t)P->nb * 4;
-    fill_sentinel_kernel<<<fill_blocks(n), 256, 0, st>>>(P->d_y, n);
-    bsr4_sptrsv_kernel<false><<<P->nchunks, 64, 0, st>>>(P->nb, P->chunk, P->nchunks, P->d_row, P->d_col, d_LU,
-                                                         P->d_diag, P->d_dinvL, d_b, P->d_y, d_x, P->d_cnt);
-    bsr4_sptrsv_kernel<true><<<P->nchunks, 64, 0, st>>>(P->nb, P->chunk, P->nchunks, P->d_row, P->d_col, d_LU,
-                                                        P->d_diag, P->d_dinvU, P->d_y, d_x, nullptr, P->d_cnt);
-    ISAI_LAUNCH_CHECK();
-    return ISAI_OK;
+    return isai::sptrsv_solve_internal(P, d_LU, d_b, d_x, (hipStream_t)stream, nullptr);
 }
 
 int isai_bsr_sptrsv_set_spin_limit(isai_sptrsv_plan_t P, int limit) {
@@ -370,7 +382,7 @@ int isai_csr_sptrsv_lu_device(int m, int nnz, const int *d_rp, const int *d_ci, 
     Counters h{};
     h.spin_limit = spin_limit > 0 ? spin_limit : (1 << 20);
     ISAI_HIP(hipMemcpyAsync(cnt, &h, sizeof(h), hipMemcpyHostToDevice, st));
-    fill_sentinel_kernel<<<fill_blocks(m), 256, 0, st>>>(y, (size_t)m);
+    fill_sentinel_kernel<<<fill_blocks(m), 256, 0, st>>>(y, (size_t)m, nullptr);
     csr_sptrsv_kernel<false><<<nchunks, 64, 0, st>>>(m, chunk, nchunks, d_rp, d_ci, d_v, d_b, y, d_x, cnt);
     csr_sptrsv_kernel<true><<<nchunks, 64, 0, st>>>(m, chunk, nchunks, d_rp, d_ci, d_v, y, d_x, nullptr, cnt);
     ISAI_LAUNCH_CHECK();

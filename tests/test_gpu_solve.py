@@ -1,0 +1,131 @@
+"""GPU parity of the setup handle and the preconditioned Krylov solves."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from gpu_pipeline import rel_err
+from isai_kernel_amd import DeviceArray, call, problems
+from isai_kernel_amd.solver import Handle, Matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_factors(rp, ci, val):
+    lu, st, _ = O.bsrilu0(rp, ci, val)
+    mbL, mcL = O.bcsc_pattern(rp, ci, True)
+    mbU, mcU = O.bcsc_pattern(rp, ci, False)
+    pp, pl, ps = O.pairs(rp, ci, mbL, mcL)
+    ML = O.isai_lower(lu, mbL, mcL, pp, pl, ps, O.build_dinv(rp, ci, lu, True))
+    MU = O.isai_upper(rp, ci, lu, mbU, mcU, O.build_dinv(rp, ci, lu, False))
+    trpL, tciL = O.tri_bcsr_pattern(rp, ci, True)
+    trpU, tciU = O.tri_bcsr_pattern(rp, ci, False)
+    _, mapL = O.map_bcsc2bcsr(mbL, mcL, trpL, tciL)
+    _, mapU = O.map_bcsc2bcsr(mbU, mcU, trpU, tciU)
+    return lu, (trpL, tciL, O.reorder(mapL, ML)), (trpU, tciU, O.reorder(mapU, MU))
+
+
+def _download(ptr, shape, dtype):
+    out = np.empty(shape, dtype)
+    call("isai_memcpy_dtoh", out.ctypes.data, ptr, out.nbytes)
+    return out
+
+
+def test_setup_factors_bitexact_and_apply(gpu):
+    print("setup factors problem 0 L=20", flush=True)
+    rp, ci, val = problems.generate(0, 20, seed=1)
+    A = Matrix(rp, ci, val)
+    h = Handle(A)
+    lu, L, U = _oracle_factors(rp, ci, val)
+    Lrow, Lcol, Lval, Urow, Ucol, Uval, LU = h.factors()
+    nnzbL, nnzbU, nlev = h.info()
+    assert nnzbL == len(L[1]) and nnzbU == len(U[1]) and nlev == O.levels(rp, ci, True)[1]
+    assert np.array_equal(_download(LU, lu.shape, np.float64), lu)
+    assert np.array_equal(_download(Lval, L[2].shape, np.float64), L[2])
+    assert np.array_equal(_download(Uval, U[2].shape, np.float64), U[2])
+    assert np.array_equal(_download(Lrow, L[0].shape, np.int32), L[0])
+    assert np.array_equal(_download(Ucol, U[1].shape, np.int32), U[1])
+    v = np.random.default_rng(0).standard_normal(A.n)
+    d_v, d_z = DeviceArray.from_numpy(v), DeviceArray((A.n,), np.float64)
+    for kind, prec in (("isai", "fp64"), ("isai", "fp32"), ("isai", "bf16"), ("ilu", "fp64")):
+        h.set_preconditioner(kind, prec)
+        h.apply(d_v, d_z)
+        call("isai_device_synchronize")
+        P = O.make_precond(1 if kind == "isai" else 2, A.nb, 4, L=L, U=U, LU=(rp, ci, lu),
+                           f32=prec == "fp32", bf16=prec == "bf16")
+        ref = O.precond_apply(P, v)
+        z = d_z.to_numpy()
+        if kind == "isai" and prec != "fp64":
+            # identical rounded operands and fmaf order as the oracle's reduced path
+            assert np.array_equal(z, ref), (prec, rel_err(z, ref))
+        else:
+            assert rel_err(z, ref) <= 1e-12, (kind, rel_err(z, ref))
+    h.close()
+
+
+@pytest.mark.parametrize("precond", ["isai", "ilu", "none"])
+def test_bicgstab_plain_small(gpu, precond):
+    """Problem 0 at its own size (2-D 5-pt 64^2, 4096 block rows), b = A*1."""
+    print(f"bicgstab problem 0 precond={precond}", flush=True)
+    rp, ci, val = problems.generate(0, 0, seed=0)
+    A = Matrix(rp, ci, val)
+    ones = np.ones(A.n)
+    b = O.spmv(rp, ci, val, ones)
+    h = Handle(A, precond=precond)
+    d_b, d_x = DeviceArray.from_numpy(b), DeviceArray.zeros((A.n,), np.float64)
+    st, it, rr = h.solve(d_b, d_x, "bicgstab", tol=1e-10, maxit=2000)
+    x = d_x.to_numpy()
+    assert st == 0 and rr <= 1e-10
+    assert O.rel_residual(rp, ci, val, b, x) <= 1e-10  # independent FP64 check
+    assert np.max(np.abs(x - 1.0)) <= 1e-7
+    # iteration count within a couple of the oracle's textbook BiCGSTAB
+    lu, L, U = _oracle_factors(rp, ci, val)
+    P = O.make_precond({"isai": 1, "ilu": 2, "none": 0}[precond], A.nb, 4, L=L, U=U, LU=(rp, ci, lu))
+    _, ost, oit, orr = O.bicgstab(rp, ci, val, P, b, tol=1e-10, maxit=2000)
+    assert ost == 0 and abs(it - oit) <= 2, (it, oit)
+    h.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
+def test_fgmres_mixed_precision(gpu, precision):
+    """27-pt convection-diffusion (small grid): FGMRES(30) with ISAI stored in
+    FP64 / FP32 / BF16 reaches 1e-10 in FP64."""
+    print(f"fgmres problem 2 L=12 precision={precision}", flush=True)
+    rp, ci, val = problems.generate(2, 12, seed=0)
+    A = Matrix(rp, ci, val)
+    b = O.spmv(rp, ci, val, np.ones(A.n))
+    h = Handle(A, precond="isai", precision=precision)
+    d_b, d_x = DeviceArray.from_numpy(b), DeviceArray.zeros((A.n,), np.float64)
+    st, it, rr = h.solve(d_b, d_x, "fgmres", tol=1e-10, maxit=3000, restart=30)
+    x = d_x.to_numpy()
+    assert st == 0 and rr <= 1e-10
+    assert O.rel_residual(rp, ci, val, b, x) <= 1.5e-10
+    lu, L, U = _oracle_factors(rp, ci, val)
+    P = O.make_precond(1, A.nb, 4, L=L, U=U, f32=precision == "fp32", bf16=precision == "bf16")
+    _, ost, oit, orr = O.fgmres(rp, ci, val, P, b, m=30, tol=1e-10, maxit=3000)
+    assert ost == 0 and abs(it - oit) <= max(3, oit // 10), (it, oit)
+    print(f"  gpu iters {it} oracle iters {oit}", flush=True)
+    h.close()
+
+
+def test_bicgstab_full_size_fp64_and_mixed(gpu):
+    """96^3 7-point, b = A*1: FP64 ISAI and FP32 ISAI BiCGSTAB reach 1e-10."""
+    print("bicgstab full size 96^3", flush=True)
+    rp, ci, val = problems.generate(1, 96, seed=0)
+    A = Matrix(rp, ci, val)
+    d_one = DeviceArray.from_numpy(np.ones(A.n))
+    d_b = DeviceArray((A.n,), np.float64)
+    A.spmv(d_one, d_b)
+    h = Handle(A)
+    res = {}
+    for prec in ("fp64", "fp32", "bf16"):
+        h.set_preconditioner("isai", prec)
+        d_x = DeviceArray.zeros((A.n,), np.float64)
+        st, it, rr = h.solve(d_b, d_x, "bicgstab", tol=1e-10, maxit=500)
+        x = d_x.to_numpy()
+        assert st == 0 and rr <= 1e-10, (prec, st, rr)
+        assert np.max(np.abs(x - 1.0)) <= 1e-7
+        res[prec] = it
+    print(f"  iterations {res}", flush=True)
+    h.close()
