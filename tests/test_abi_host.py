@@ -137,3 +137,36 @@ def test_levels_match_oracle(lower):
                                ctypes.byref(n)) == 0
     olev, on = O.levels(rp, ci, lower)
     assert np.array_equal(lev, olev) and n.value == on
+
+
+def test_matrix_market_reader_and_residual():
+    """Reader semantics (1-based -> 0-based, sorted by (row, col), duplicates kept
+    in file order) on a hand-written file; csr2bsr of it sums the duplicate."""
+    path = str(ROOT / "tests/golden/small.mtx").encode()
+    nr, nc, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    prp, pci, pv = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    assert lib.isai_read_matrix_market(path, ctypes.byref(nr), ctypes.byref(nc), ctypes.byref(nnz),
+                                       ctypes.byref(prp), ctypes.byref(pci), ctypes.byref(pv)) == 0
+    assert (nr.value, nc.value, nnz.value) == (8, 8, 13)
+    rp = np.ctypeslib.as_array(ctypes.cast(prp, ctypes.POINTER(ctypes.c_int)), (9,)).copy()
+    ci = np.ctypeslib.as_array(ctypes.cast(pci, ctypes.POINTER(ctypes.c_int)), (13,)).copy()
+    v = np.ctypeslib.as_array(ctypes.cast(pv, ctypes.POINTER(ctypes.c_double)), (13,)).copy()
+    for p_ in (prp, pci, pv):
+        lib.isai_host_free(p_)
+    assert rp.tolist() == [0, 2, 4, 5, 6, 8, 9, 11, 13]
+    assert ci.tolist() == [0, 5, 1, 1, 6, 0, 3, 4, 5, 2, 6, 0, 7]
+    assert v[2:4].tolist() == [3.0, 0.5] and np.signbit(v[6]) and v[6] == 0.0
+    brp, bci, bv = problems.csr_to_bsr(8, 4, rp, ci, v)
+    assert bv[0, 1, 1] == 3.5 and np.signbit(bv[2, 0, 3])
+    # residual check: exact solution of a diagonal system
+    drp = np.arange(5, dtype=np.int32)
+    dci = np.arange(4, dtype=np.int32)
+    dv = np.array([2.0, 4.0, 8.0, 16.0])
+    b = np.ones(4)
+    x = b / dv
+    ex, rr = ctypes.c_double(), ctypes.c_double()
+    assert lib.isai_rel_errors_csr(4, drp.ctypes.data, dci.ctypes.data, dv.ctypes.data, b.ctypes.data,
+                                   x.ctypes.data, x.ctypes.data, ctypes.byref(ex), ctypes.byref(rr)) == 0
+    assert ex.value == 0.0 and rr.value == 0.0
+    assert lib.isai_read_matrix_market(b"/nonexistent.mtx", ctypes.byref(nr), ctypes.byref(nc), ctypes.byref(nnz),
+                                       ctypes.byref(prp), ctypes.byref(pci), ctypes.byref(pv)) == -10
