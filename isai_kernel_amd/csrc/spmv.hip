@@ -5,8 +5,8 @@
 // value array with 16-byte lane loads so that the lanes serving one block row
 // read whole 128-byte lines:
 //   FP64  8 lanes per block row, lane s holds row s/2, columns 2*(s%2)..+1
-//   FP32  4 lanes per block row, lane s holds row s (4 floats = 16 B)
-//   BF16  2 lanes per block row, lane s holds rows 2s, 2s+1 (8 bf16 = 16 B)
+//   FP32  8 lanes per block row = 2 blocks per line, lane s holds row s%4 of one
+//   BF16  8 lanes per block row = 4 blocks per line, lane s holds rows 2(s%2), +1
 // Values are read with non-temporal loads (streamed once per product); the x
 // gathers use the normal cached path, since each x block is reused by every
 // block row that references it.
@@ -67,64 +67,71 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f64_kernel(
     if (br < nb && h == 0) y[(size_t)br * 4 + (s >> 1)] = sum;
 }
 
-// ---------------------------------------------------------------- FP32, bs = 4
-template <int UNROLL>
-__global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f32_kernel(
-    int nb, const int *__restrict__ rp, const int *__restrict__ ci,
-    const v4f *__restrict__ val4, const v2d *__restrict__ x2, double *__restrict__ y,
-    const int *__restrict__ skip) {
-    if (skip && *skip) return;
-    const int wg = xcd_remap(blockIdx.x, gridDim.x);
-    const int br = wg * (kSpmvBlock / 4) + (threadIdx.x >> 2);
-    const int r = threadIdx.x & 3;
-    if (br >= nb) return;
-    float acc = 0.0f;
-    const int rs = rp[br], re = rp[br + 1];
-    for (int k = rs; k < re; ++k) {
-        const int c = ci[k];
-        const v4f a = __builtin_nontemporal_load(&val4[(size_t)k * 4 + r]);
-        const v2d x01 = x2[(size_t)c * 2 + 0];
-        const v2d x23 = x2[(size_t)c * 2 + 1];
-        acc = fmaf(a.x, (float)x01.x, acc);
-        acc = fmaf(a.y, (float)x01.y, acc);
-        acc = fmaf(a.z, (float)x23.x, acc);
-        acc = fmaf(a.w, (float)x23.y, acc);
-    }
-    y[(size_t)br * 4 + r] = (double)acc;
-}
-
-// ---------------------------------------------------------------- BF16, bs = 4
+// ------------------------------------------------- FP32 / BF16 values, bs = 4
+// Eight lanes per block row, each lane 16 B of the row's value array, so one load
+// instruction of the eight lanes covers one whole 128-byte line: BPB = 2 (FP32)
+// or 4 (BF16) consecutive blocks of the row.  Lane s holds block k + s / LPB and,
+// within it, element run (s % LPB) * VPL .. + VPL - 1 (row-major).  Products
+// accumulate in FP32; lanes that share a row are summed with cross-lane adds at
+// the end (x is FP64 in memory and rounded to FP32 on load).
 __device__ __forceinline__ float bf16_lo(unsigned int w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf16_hi(unsigned int w) { return __uint_as_float(w & 0xffff0000u); }
 
-__global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_bf16_kernel(
-    int nb, const int *__restrict__ rp, const int *__restrict__ ci,
-    const v4u *__restrict__ val8, const v2d *__restrict__ x2, double *__restrict__ y,
-    const int *__restrict__ skip) {
+template <bool BF16>
+__global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
+    int nb, const int *__restrict__ rp, const int *__restrict__ ci, const v4u *__restrict__ val16,
+    const v2d *__restrict__ x2, double *__restrict__ y, const int *__restrict__ skip) {
     if (skip && *skip) return;
+    constexpr int BPB = BF16 ? 4 : 2;  // blocks per 128-byte line
+    constexpr int LPB = 8 / BPB;       // lanes per block
     const int wg = xcd_remap(blockIdx.x, gridDim.x);
-    const int br = wg * (kSpmvBlock / 2) + (threadIdx.x >> 1);
-    const int h = threadIdx.x & 1;  // rows 2h, 2h+1
-    if (br >= nb) return;
+    const int br = wg * (kSpmvBlock / 8) + (threadIdx.x >> 3);
+    const int s = threadIdx.x & 7;
+    const int b = s / LPB, sub = s % LPB;
     float acc0 = 0.0f, acc1 = 0.0f;
-    const int rs = rp[br], re = rp[br + 1];
-    for (int k = rs; k < re; ++k) {
-        const int c = ci[k];
-        const v4u w = __builtin_nontemporal_load(&val8[(size_t)k * 2 + h]);
-        const v2d x01 = x2[(size_t)c * 2 + 0];
-        const v2d x23 = x2[(size_t)c * 2 + 1];
-        const float f0 = (float)x01.x, f1 = (float)x01.y, f2 = (float)x23.x, f3 = (float)x23.y;
-        acc0 = fmaf(bf16_lo(w.x), f0, acc0);
-        acc0 = fmaf(bf16_hi(w.x), f1, acc0);
-        acc0 = fmaf(bf16_lo(w.y), f2, acc0);
-        acc0 = fmaf(bf16_hi(w.y), f3, acc0);
-        acc1 = fmaf(bf16_lo(w.z), f0, acc1);
-        acc1 = fmaf(bf16_hi(w.z), f1, acc1);
-        acc1 = fmaf(bf16_lo(w.w), f2, acc1);
-        acc1 = fmaf(bf16_hi(w.w), f3, acc1);
+    if (br < nb) {
+        const int rs = rp[br], re = rp[br + 1];
+        for (int k = rs; k < re; k += BPB) {
+            const int kb = k + b;
+            if (kb < re) {
+                const int c = ci[kb];
+                // 16 B per lane: the row's value array viewed as 16-byte words
+                const v4u w = __builtin_nontemporal_load(&val16[(size_t)kb * LPB + sub]);
+                const v2d x01 = x2[(size_t)c * 2 + 0];
+                const v2d x23 = x2[(size_t)c * 2 + 1];
+                const float f0 = (float)x01.x, f1 = (float)x01.y, f2 = (float)x23.x, f3 = (float)x23.y;
+                if (BF16) {  // rows 2*sub, 2*sub+1
+                    acc0 = fmaf(bf16_lo(w.x), f0, acc0);
+                    acc0 = fmaf(bf16_hi(w.x), f1, acc0);
+                    acc0 = fmaf(bf16_lo(w.y), f2, acc0);
+                    acc0 = fmaf(bf16_hi(w.y), f3, acc0);
+                    acc1 = fmaf(bf16_lo(w.z), f0, acc1);
+                    acc1 = fmaf(bf16_hi(w.z), f1, acc1);
+                    acc1 = fmaf(bf16_lo(w.w), f2, acc1);
+                    acc1 = fmaf(bf16_hi(w.w), f3, acc1);
+                } else {  // row sub
+                    acc0 = fmaf(__uint_as_float(w.x), f0, acc0);
+                    acc0 = fmaf(__uint_as_float(w.y), f1, acc0);
+                    acc0 = fmaf(__uint_as_float(w.z), f2, acc0);
+                    acc0 = fmaf(__uint_as_float(w.w), f3, acc0);
+                }
+            }
+        }
     }
-    y[(size_t)br * 4 + 2 * h] = (double)acc0;
-    y[(size_t)br * 4 + 2 * h + 1] = (double)acc1;
+    // sum the lanes holding the same row(s): same sub, different b
+#pragma unroll
+    for (int off = LPB; off < 8; off <<= 1) {
+        acc0 += __shfl_xor(acc0, off);
+        if (BF16) acc1 += __shfl_xor(acc1, off);
+    }
+    if (br < nb && b == 0) {
+        if (BF16) {
+            y[(size_t)br * 4 + 2 * sub] = (double)acc0;
+            y[(size_t)br * 4 + 2 * sub + 1] = (double)acc1;
+        } else {
+            y[(size_t)br * 4 + sub] = (double)acc0;
+        }
+    }
 }
 
 // ---------------------------------------------------- generic bs (1..8), FP64
@@ -210,13 +217,13 @@ int spmv_launch(int mb, int nnzb, int bs, isai_precision_t prec, const int *rp, 
             bsr4_spmv_f64_kernel<2><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v2d *)val,
                                                               (const v2d *)x, y, skip);
         } else if (prec == ISAI_PREC_FP32) {
-            const unsigned g = grid_for(mb, kSpmvBlock / 4);
-            bsr4_spmv_f32_kernel<1><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v4f *)val,
-                                                              (const v2d *)x, y, skip);
+            const unsigned g = grid_for(mb, kSpmvBlock / 8);
+            bsr4_spmv_lowp_kernel<false><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v4u *)val,
+                                                                   (const v2d *)x, y, skip);
         } else if (prec == ISAI_PREC_BF16) {
-            const unsigned g = grid_for(mb, kSpmvBlock / 2);
-            bsr4_spmv_bf16_kernel<<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v4u *)val,
-                                                            (const v2d *)x, y, skip);
+            const unsigned g = grid_for(mb, kSpmvBlock / 8);
+            bsr4_spmv_lowp_kernel<true><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v4u *)val,
+                                                                  (const v2d *)x, y, skip);
         } else {
             set_error("bsr_spmv: unknown precision %d", (int)prec);
             return ISAI_ERR_INVALID_ARG;

@@ -68,7 +68,7 @@ __device__ __forceinline__ void publish(double *p, double v) {
 // cross-lane adds, then multiplied by the precomputed inverse of the diagonal
 // part (unit-lower for L, upper for U).
 template <bool UPPER>
-__global__ __launch_bounds__(64) void bsr4_sptrsv_kernel(int nb, int chunk, int nchunks,
+__global__ __launch_bounds__(64) void bsr4_sptrsv_kernel(int nchunks, const int2 *__restrict__ chunks,
                                                          const int *__restrict__ rp, const int *__restrict__ ci,
                                                          const double *__restrict__ LU,
                                                          const int *__restrict__ diagpos,
@@ -86,9 +86,8 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_kernel(int nb, int chunk, int 
     const int limit = cnt->spin_limit;
     if (lane == 0) s_c = (int)atomicAdd(ticket, 1u);
     __syncthreads();
-    const int t = s_c;
-    const int c = UPPER ? nchunks - 1 - t : t;
-    const int lo = c * chunk, hi = min(lo + chunk, nb);
+    const int2 range = chunks[s_c];  // chunks are listed in processing order
+    const int lo = range.x, hi = range.y;
     bool timed_out = false;
 
     for (int step = 0; step < hi - lo; ++step) {
@@ -247,9 +246,11 @@ static unsigned fill_blocks(size_t n) {
 using namespace isai;
 
 struct isai_sptrsv_plan {
-    int nb = 0, nnzb = 0, bs = 0, chunk = 0, nchunks = 0;
+    int nb = 0, nnzb = 0, bs = 0, chunk = 0;
+    int nchunksL = 0, nchunksU = 0;
     const int *d_row = nullptr, *d_col = nullptr;
     int *d_diag = nullptr;
+    int2 *d_chunksL = nullptr, *d_chunksU = nullptr;
     double *d_dinvL = nullptr, *d_dinvU = nullptr, *d_y = nullptr;
     Counters *d_cnt = nullptr;
 };
@@ -260,10 +261,10 @@ int sptrsv_solve_internal(isai_sptrsv_plan_t P, const double *d_LU, const double
                           hipStream_t st, const int *skip) {
     const size_t n = (size_t)P->nb * 4;
     fill_sentinel_kernel<<<fill_blocks(n), 256, 0, st>>>(P->d_y, n, skip);
-    bsr4_sptrsv_kernel<false><<<P->nchunks, 64, 0, st>>>(P->nb, P->chunk, P->nchunks, P->d_row, P->d_col, d_LU,
-                                                         P->d_diag, P->d_dinvL, d_b, P->d_y, d_x, P->d_cnt, skip);
-    bsr4_sptrsv_kernel<true><<<P->nchunks, 64, 0, st>>>(P->nb, P->chunk, P->nchunks, P->d_row, P->d_col, d_LU,
-                                                        P->d_diag, P->d_dinvU, P->d_y, d_x, nullptr, P->d_cnt, skip);
+    bsr4_sptrsv_kernel<false><<<P->nchunksL, 64, 0, st>>>(P->nchunksL, P->d_chunksL, P->d_row, P->d_col, d_LU,
+                                                          P->d_diag, P->d_dinvL, d_b, P->d_y, d_x, P->d_cnt, skip);
+    bsr4_sptrsv_kernel<true><<<P->nchunksU, 64, 0, st>>>(P->nchunksU, P->d_chunksU, P->d_row, P->d_col, d_LU,
+                                                         P->d_diag, P->d_dinvU, P->d_y, d_x, nullptr, P->d_cnt, skip);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
@@ -281,27 +282,58 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         set_error("isai_bsr_sptrsv: the BSR-native solve is built for bs = 4 (got %d); use the CSR form", bs);
         return ISAI_ERR_UNSUPPORTED;
     }
-    if (chunk <= 0) chunk = 64;
+    if (chunk <= 0) chunk = kMaxChunk;
     ISAI_REQUIRE(chunk <= kMaxChunk, "isai_bsr_sptrsv_analysis: chunk %d above %d", chunk, kMaxChunk);
     hipStream_t st = (hipStream_t)stream;
+    // Chunks follow the dependency chains: a new L chunk starts at row i when row i
+    // has no block in column i-1 (so it does not wait for the previous chunk's last
+    // row); a new U chunk starts (walking down from the last row) when row i has
+    // no block in column i+1.  Chunks are capped at `chunk` rows.
+    std::vector<int> rp(nb + 1), ci(nnzb);
+    ISAI_HIP(hipMemcpyAsync(rp.data(), d_row, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipMemcpyAsync(ci.data(), d_col, sizeof(int) * nnzb, hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipStreamSynchronize(st));
+    auto has = [&](int i, int j) {
+        for (int p = rp[i]; p < rp[i + 1]; ++p)
+            if (ci[p] == j) return true;
+        return false;
+    };
+    std::vector<int2> cl, cu;
+    for (int i = 0; i < nb;) {
+        int j = i + 1;
+        while (j < nb && j - i < chunk && has(j, j - 1)) ++j;
+        cl.push_back(make_int2(i, j));
+        i = j;
+    }
+    for (int i = nb - 1; i >= 0;) {
+        int j = i - 1;  // chunk is (j, i]
+        while (j >= 0 && i - j < chunk && has(j, j + 1)) --j;
+        cu.push_back(make_int2(j + 1, i + 1));
+        i = j;
+    }
     auto *P = new isai_sptrsv_plan;
     P->nb = nb;
     P->nnzb = nnzb;
     P->bs = bs;
     P->chunk = chunk;
-    P->nchunks = (nb + chunk - 1) / chunk;
+    P->nchunksL = (int)cl.size();
+    P->nchunksU = (int)cu.size();
     P->d_row = d_row;
     P->d_col = d_col;
     bool ok = hipMalloc(&P->d_diag, sizeof(int) * nb) == hipSuccess &&
               hipMalloc(&P->d_dinvL, sizeof(double) * 16 * nb) == hipSuccess &&
               hipMalloc(&P->d_dinvU, sizeof(double) * 16 * nb) == hipSuccess &&
               hipMalloc(&P->d_y, sizeof(double) * 4 * nb) == hipSuccess &&
+              hipMalloc(&P->d_chunksL, sizeof(int2) * cl.size()) == hipSuccess &&
+              hipMalloc(&P->d_chunksU, sizeof(int2) * cu.size()) == hipSuccess &&
               hipMalloc(&P->d_cnt, sizeof(Counters) + sizeof(int)) == hipSuccess;
     if (!ok) {
         isai_bsr_sptrsv_destroy(P);
         set_error("isai_bsr_sptrsv_analysis: hipMalloc failed");
         return ISAI_ERR_ALLOC;
     }
+    ISAI_HIP(hipMemcpy(P->d_chunksL, cl.data(), sizeof(int2) * cl.size(), hipMemcpyHostToDevice));
+    ISAI_HIP(hipMemcpy(P->d_chunksU, cu.data(), sizeof(int2) * cu.size(), hipMemcpyHostToDevice));
     Counters h{};
     h.spin_limit = 1 << 20;
     int *missing = (int *)(P->d_cnt + 1);
@@ -362,6 +394,8 @@ int isai_bsr_sptrsv_destroy(isai_sptrsv_plan_t P) {
     (void)hipFree(P->d_dinvL);
     (void)hipFree(P->d_dinvU);
     (void)hipFree(P->d_y);
+    (void)hipFree(P->d_chunksL);
+    (void)hipFree(P->d_chunksU);
     (void)hipFree(P->d_cnt);
     delete P;
     return ISAI_OK;
