@@ -106,6 +106,8 @@ def main() -> int:
     ap.add_argument("--problem", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=0, help="rows per wave of the sync-free solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="time the kernels only (no solves, no CPU leg): for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1 "
@@ -149,6 +151,19 @@ def main() -> int:
     kernels = {}
     K = args.kernel_iters
 
+    # ---- plain copy kernel: empirical bandwidth ceiling and byte-counter reference
+    ncopy = 64 * 1024 * 1024  # doubles: 512 MiB read + 512 MiB written per launch
+    d_c0, d_c1 = DeviceArray((ncopy,), np.float64), DeviceArray((ncopy,), np.float64)
+
+    def copy():
+        lib.isai_copy_f64_async(d_c1.ptr, d_c0.ptr, ncopy, stream)
+    for _ in range(3):
+        copy()
+    copy_ms = timer.time(copy, max(K // 5, 5))
+    copy_gbps = 2 * 8 * ncopy / (copy_ms * 1e-3) / 1e9
+    log(f"copy kernel: {copy_ms*1e3:.1f} us for {2*8*ncopy/2**20:.0f} MiB moved, {copy_gbps:.0f} GB/s")
+    del d_c0, d_c1
+
     def record(name, ms, nbytes):
         gbps = nbytes / (ms * 1e-3) / 1e9
         kernels[name] = {"us": round(ms * 1e3, 2), "GBps": round(gbps, 1),
@@ -183,6 +198,11 @@ def main() -> int:
         trsv()
     tr_ms = timer.time(trsv, max(K // 4, 10))
     record("sptrsv_lu_fp64", tr_ms, sptrsv_bytes(nb, nnzbL, nnzbU))
+
+    if args.kernels_only:
+        h.close()
+        print(json.dumps({"kernels": kernels, "copy_GBps": round(copy_gbps, 1)}), flush=True)
+        return 0
 
     # ---- solves (the steps)
     solves = {}
@@ -237,6 +257,7 @@ def main() -> int:
         "solves": solves,
         "kernels": kernels,
         "setup_ms": {"ilu0": round(ms_ilu, 3), "isai_build": round(ms_isai, 3), "levels": nlevels},
+        "copy_kernel_GBps": round(copy_gbps, 1),
     }
     h.close()
     if not args.no_cpu:

@@ -27,6 +27,8 @@ int isai_memcpy_dtod_async(void *dst, const void *src, size_t bytes, isai_stream
 /* Fills with a kernel (not hipMemset), so it is safe inside a captured graph. */
 int isai_fill_u32_async(void *dst, uint32_t value, size_t count, isai_stream_t stream);
 int isai_fill_f64_async(double *dst, double value, size_t count, isai_stream_t stream);
+/* dst[0..count) = src[0..count) with a plain streaming kernel (count even). */
+int isai_copy_f64_async(double *dst, const double *src, size_t count, isai_stream_t stream);
 
 int isai_device_synchronize(void);
 int isai_stream_create(isai_stream_t *stream);

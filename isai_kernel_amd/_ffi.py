@@ -45,6 +45,7 @@ SIGNATURES: dict[str, list] = {
     "isai_memcpy_dtod_async": [P, P, SZ, P],
     "isai_fill_u32_async": [P, C.c_uint32, SZ, P],
     "isai_fill_f64_async": [P, D, SZ, P],
+    "isai_copy_f64_async": [P, P, SZ, P],
     "isai_device_synchronize": [],
     "isai_stream_create": [P],
     "isai_stream_destroy": [P],

@@ -30,6 +30,14 @@ __global__ void fill_f64_kernel(double *dst, double v, size_t n) {
     for (; i < n; i += stride) dst[i] = v;
 }
 
+// Plain streaming copy, 16 B per lane: the reference point for byte counters
+// and the empirical bandwidth ceiling.
+__global__ void copy_f64_kernel(v2d *__restrict__ dst, const v2d *__restrict__ src, size_t n2) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n2; i += stride) dst[i] = src[i];
+}
+
 static unsigned fill_grid(size_t n) {
     size_t g = (n + 255) / 256;
     if (g > 65536) g = 65536;
@@ -134,6 +142,17 @@ int isai_fill_f64_async(double *dst, double value, size_t count, isai_stream_t s
     if (!count) return ISAI_OK;
     ISAI_REQUIRE(dst, "isai_fill_f64_async: null destination");
     fill_f64_kernel<<<fill_grid(count), 256, 0, (hipStream_t)stream>>>(dst, value, count);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+int isai_copy_f64_async(double *dst, const double *src, size_t count, isai_stream_t stream) {
+    if (!count) return ISAI_OK;
+    ISAI_REQUIRE(dst && src && count % 2 == 0, "isai_copy_f64_async: null pointer or odd count");
+    const size_t n2 = count / 2;
+    size_t g = (n2 + 255) / 256;
+    if (g > 16384) g = 16384;
+    copy_f64_kernel<<<(unsigned)g, 256, 0, (hipStream_t)stream>>>((v2d *)dst, (const v2d *)src, n2);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
