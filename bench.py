@@ -32,6 +32,10 @@ sys.path.insert(0, str(ROOT))
 METRIC = "solve time to 1e-10 rel-res, mixed vs FP64 ISAI; BSR SpMV/SpTRSV HBM GB/s"
 HBM_PEAK_GBPS = 8000.0
 TOL = 1e-10
+# HBM bytes per launch of the A SpMV at 96^3, from the FETCH_SIZE / WRITE_SIZE counter
+# passes summarised in profiles/hbm_traffic_96cube_7pt_pmc.md (FETCH_SIZE x2 by the
+# copy-kernel calibration): 413,194.25 KiB x 2 + 27,648 KiB.
+SPMV_TRAFFIC_BYTES_96CUBE = 413194.25 * 2 * 1024 + 27648 * 1024
 
 
 def log(msg: str) -> None:
@@ -251,7 +255,8 @@ def main() -> int:
                                "3-D 7-pt 96^3, bs=4 (884736 block rows, 6137856 blocks), b = A*1",
                    "problem": args.problem, "nb": nb, "nnzb": nnzb, "mixed_form": mixed},
         "roofline": {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(gbps / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(gbps / HBM_PEAK_GBPS, 4),
+                     "traffic": int(SPMV_TRAFFIC_BYTES_96CUBE) if args.problem == 1 else None,
                      "kernel": "bsr4_spmv_f64_kernel", "algorithmic_bytes": spmv_bytes(nb, nnzb)},
         "speedup": speedups,
         "solves": solves,

@@ -96,7 +96,10 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
             if (kb < re) {
                 const int c = ci[kb];
                 // 16 B per lane: the row's value array viewed as 16-byte words
-                const v4u w = __builtin_nontemporal_load(&val16[(size_t)kb * LPB + sub]);
+                // BF16 rows fill whole lines; a 64-B FP32 block shares its line with a
+                // neighbour that another wave may read, so FP32 keeps the line in L2.
+                const v4u w = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kb * LPB + sub])
+                                   : val16[(size_t)kb * LPB + sub];
                 const v2d x01 = x2[(size_t)c * 2 + 0];
                 const v2d x23 = x2[(size_t)c * 2 + 1];
                 const float f0 = (float)x01.x, f1 = (float)x01.y, f2 = (float)x23.x, f3 = (float)x23.y;
