@@ -63,91 +63,179 @@ __device__ __forceinline__ void publish(double *p, double v) {
 }
 
 // -------------------------------------------------------------- BSR, bs = 4
-// Lane layout per row: 4 lanes per off-diagonal block (lane r holds row r of the
-// block), 16 blocks per pass; the per-row sums are reduced over the blocks with
-// cross-lane adds, then multiplied by the precomputed inverse of the diagonal
-// part (unit-lower for L, upper for U).
+// Rows are handled in "processing positions" q: q = i for L (ascending), q = nb-1-i
+// for U (descending), so both triangles share one kernel.  A chunk is a run of
+// positions [qlo, qhi) whose boundaries follow the dependency chains.  The factor
+// values used here are pre-scaled: S_ik = inv(D_i) T_ik for every off-diagonal
+// block (T = L or U, D_i its diagonal part), so row i of the solve is
+//     x_i = inv(D_i) rhs_i - sum_k S_ik x_k .
+// A dependency k of row i is "local" when it lies in the same chunk (its x is in
+// LDS) and "remote" otherwise (an item polled from the solution vector).
+//
+// One wave per chunk:
+//   prologue  stage zb = inv(D_i) rhs_i, the local blocks S_ik and the per-row
+//             item tables of the chunk in LDS;
+//   loop      (a) remote phase: each of the 64 lanes takes one not-yet-done remote
+//                 item from a window starting at the chain's current row, reads
+//                 its x_k with ONE agent-scope atomic poll (no spinning) and, if
+//                 published, stores S_ik x_k in the item's LDS slot;
+//             (b) chain phase: four lanes (one per row of the 4x4 block) advance
+//                 the chunk's rows in order while all remote items of the next row
+//                 are in, touching only LDS, and publish each x_i.
+// Every step either makes progress or counts toward the stall limit, so the
+// kernel always terminates; remote items only ever point at chunks with lower
+// tickets, which are already running.
+constexpr int kPipeRows = 128;   // rows per chunk
+constexpr int kPipeRemote = 512; // remote items per chunk
+constexpr int kPipeLocal = 128;  // local items per chunk
+
 template <bool UPPER>
-__global__ __launch_bounds__(64) void bsr4_sptrsv_kernel(int nchunks, const int2 *__restrict__ chunks,
-                                                         const int *__restrict__ rp, const int *__restrict__ ci,
-                                                         const double *__restrict__ LU,
-                                                         const int *__restrict__ diagpos,
-                                                         const double *__restrict__ dinv,
-                                                         const double *rhs, double *out,
-                                                         double *sentinel_fill, Counters *cnt,
-                                                         const int *__restrict__ skip) {
+__global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
+    int nb, const int4 *__restrict__ chunks, const int *__restrict__ ci, const double *__restrict__ S,
+    const double *__restrict__ dinv, const int *__restrict__ rem_ptr, const int *__restrict__ rem_items,
+    const int *__restrict__ loc_ptr, const int *__restrict__ loc_items, const double *rhs, double *out,
+    double *sentinel_fill, Counters *cnt, const int *__restrict__ skip) {
     if (skip && *skip) return;
-    __shared__ double xs[kMaxChunk * 4];
-    __shared__ int s_c;
+    __shared__ double zb[kPipeRows * 4];
+    __shared__ double xs[kPipeRows * 4];
+    __shared__ double rem[kPipeRemote * 4];
+    __shared__ double locv[kPipeLocal * 16];
+    __shared__ int done[kPipeRemote];
+    __shared__ int lk[kPipeLocal];
+    __shared__ int rrp[kPipeRows + 1];
+    __shared__ int lrp[kPipeRows + 1];
+    __shared__ int s_c, s_chain;
     const int lane = threadIdx.x;
-    const int blk = lane >> 2, r = lane & 3;
     unsigned int *ticket = UPPER ? &cnt->ticket_u : &cnt->ticket_l;
-    unsigned int *done = UPPER ? &cnt->done_u : &cnt->done_l;
+    unsigned int *donec = UPPER ? &cnt->done_u : &cnt->done_l;
     const int limit = cnt->spin_limit;
     if (lane == 0) s_c = (int)atomicAdd(ticket, 1u);
     __syncthreads();
-    const int2 range = chunks[s_c];  // chunks are listed in processing order
-    const int lo = range.x, hi = range.y;
-    bool timed_out = false;
+    const int4 ch = chunks[s_c];
+    const int4 nx = chunks[s_c + 1];
+    const int qlo = ch.x, qhi = ch.y, r0 = ch.z, l0 = ch.w;
+    const int nrows = qhi - qlo, nrem = nx.z - r0, nloc = nx.w - l0;
+    auto row_of = [&](int q) { return UPPER ? nb - 1 - q : q; };
 
-    for (int step = 0; step < hi - lo; ++step) {
-        const int i = UPPER ? hi - 1 - step : lo + step;
-        const int dp = diagpos[i];
-        const int pbeg = UPPER ? dp + 1 : rp[i];
-        const int pend = UPPER ? rp[i + 1] : dp;
-        double acc = 0.0;
-        for (int base = pbeg; base < pend; base += 16) {
-            const int p = base + blk;
-            if (p < pend) {
-                const int k = ci[p];
-                double x0, x1, x2, x3;
-                const bool local = UPPER ? (k < hi) : (k >= lo);
-                if (local) {
-                    const double *xl = xs + (k - lo) * 4;
-                    x0 = xl[0]; x1 = xl[1]; x2 = xl[2]; x3 = xl[3];
-                } else {
-                    const double *xg = out + (size_t)k * 4;
-                    x0 = poll(xg + 0, limit, timed_out);
-                    x1 = poll(xg + 1, limit, timed_out);
-                    x2 = poll(xg + 2, limit, timed_out);
-                    x3 = poll(xg + 3, limit, timed_out);
-                }
-                const v2d a01 = *(const v2d *)(LU + (size_t)p * 16 + r * 4);
-                const v2d a23 = *(const v2d *)(LU + (size_t)p * 16 + r * 4 + 2);
-                acc = fma(a01.x, x0, acc);
-                acc = fma(a01.y, x1, acc);
-                acc = fma(a23.x, x2, acc);
-                acc = fma(a23.y, x3, acc);
+    // ---- prologue
+    for (int t = lane; t < nrows * 4; t += 64) {
+        const int rr = t >> 2, m = t & 3;
+        const int i = row_of(qlo + rr);
+        const double *D = dinv + (size_t)i * 16 + m * 4;
+        const double *b = rhs + (size_t)i * 4;
+        zb[t] = D[0] * b[0] + D[1] * b[1] + D[2] * b[2] + D[3] * b[3];
+    }
+    for (int t = lane; t <= nrows; t += 64) {
+        rrp[t] = rem_ptr[qlo + t] - r0;
+        lrp[t] = loc_ptr[qlo + t] - l0;
+    }
+    for (int t = lane; t < nloc; t += 64) {
+        const int p = loc_items[l0 + t];
+        const int k = ci[p];
+        lk[t] = (UPPER ? nb - 1 - k : k) - qlo;
+    }
+    for (int t = lane; t < nloc * 16; t += 64) locv[t] = S[(size_t)loc_items[l0 + (t >> 4)] * 16 + (t & 15)];
+    for (int t = lane; t < nrem; t += 64) done[t] = 0;
+    __syncthreads();
+
+    int chain = 0, stall = 0;
+    bool timed_out = false;
+    while (chain < nrows) {
+        // ---- (a) remote phase: one poll per pending item of the window
+        int got = 0;
+        const int w = rrp[chain] + lane;
+        if (w < nrem && !done[w]) {
+            const int p = rem_items[r0 + w];
+            const int k = ci[p];
+            unsigned long long *xq = (unsigned long long *)(out + (size_t)k * 4);
+            const unsigned long long u0 = __hip_atomic_load(xq + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long u1 = __hip_atomic_load(xq + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long u2 = __hip_atomic_load(xq + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long u3 = __hip_atomic_load(xq + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const v2d *B = (const v2d *)(S + (size_t)p * 16);
+            const v2d b0 = B[0], b1 = B[1], b2 = B[2], b3 = B[3], b4 = B[4], b5 = B[5], b6 = B[6], b7 = B[7];
+            if (u0 != kSentinel && u1 != kSentinel && u2 != kSentinel && u3 != kSentinel) {
+                const double x0 = __longlong_as_double((long long)u0), x1 = __longlong_as_double((long long)u1);
+                const double x2 = __longlong_as_double((long long)u2), x3 = __longlong_as_double((long long)u3);
+                rem[w * 4 + 0] = b0.x * x0 + b0.y * x1 + b1.x * x2 + b1.y * x3;
+                rem[w * 4 + 1] = b2.x * x0 + b2.y * x1 + b3.x * x2 + b3.y * x3;
+                rem[w * 4 + 2] = b4.x * x0 + b4.y * x1 + b5.x * x2 + b5.y * x3;
+                rem[w * 4 + 3] = b6.x * x0 + b6.y * x1 + b7.x * x2 + b7.y * x3;
+                done[w] = 1;
+                got = 1;
             }
         }
-        acc += __shfl_xor(acc, 4);
-        acc += __shfl_xor(acc, 8);
-        acc += __shfl_xor(acc, 16);
-        acc += __shfl_xor(acc, 32);
-        const double *bi = rhs + (size_t)i * 4;
-        const double z0 = bi[0] - __shfl(acc, 0);
-        const double z1 = bi[1] - __shfl(acc, 1);
-        const double z2 = bi[2] - __shfl(acc, 2);
-        const double z3 = bi[3] - __shfl(acc, 3);
-        const double *D = dinv + (size_t)i * 16 + r * 4;
-        double v = D[0] * z0;
-        v = fma(D[1], z1, v);
-        v = fma(D[2], z2, v);
-        v = fma(D[3], z3, v);
+        __syncthreads();
+        // ---- (b) chain phase: lanes 0..3 advance while the next row is complete
         if (lane < 4) {
-            xs[(i - lo) * 4 + lane] = v;
-            publish(out + (size_t)i * 4 + lane, v);
-            if (sentinel_fill) sentinel_fill[(size_t)i * 4 + lane] = __longlong_as_double((long long)kSentinel);
+            int adv = 0;
+            while (chain < nrows && adv < 64) {
+                const int ra = rrp[chain], rb = rrp[chain + 1];
+                bool ready = true;
+                for (int t = ra; t < rb; ++t) ready = ready && done[t];
+                if (!ready) break;
+                double z = zb[chain * 4 + lane];
+                for (int t = ra; t < rb; ++t) z -= rem[t * 4 + lane];
+                for (int t = lrp[chain]; t < lrp[chain + 1]; ++t) {
+                    const double *Sv = locv + t * 16 + lane * 4;
+                    const double *xk = xs + lk[t] * 4;
+                    z -= Sv[0] * xk[0] + Sv[1] * xk[1] + Sv[2] * xk[2] + Sv[3] * xk[3];
+                }
+                xs[chain * 4 + lane] = z;
+                const int i = row_of(qlo + chain);
+                publish(out + (size_t)i * 4 + lane, z);
+                if (sentinel_fill) sentinel_fill[(size_t)i * 4 + lane] = __longlong_as_double((long long)kSentinel);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                ++chain;
+                ++adv;
+            }
+            if (lane == 0) s_chain = chain | (adv > 0 ? 0x40000000 : 0);
         }
         __syncthreads();
-    }
-    if (timed_out) atomicOr(&cnt->err, 1);
-    if (lane == 0) {
-        // The last wave of this solve resets the counters for the next one.
-        if (atomicAdd(done, 1u) == (unsigned)nchunks - 1) {
-            atomicExch(ticket, 0u);
-            atomicExch(done, 0u);
+        const int sc = s_chain;
+        const bool progressed = (sc & 0x40000000) || __any(got);
+        chain = sc & 0x3fffffff;
+        stall = progressed ? 0 : stall + 1;
+        if (stall > limit) {
+            // give up on the missing items: their contributions read as NaN
+            timed_out = true;
+            for (int t = lane; t < nrem; t += 64)
+                if (!done[t]) {
+                    rem[t * 4 + 0] = rem[t * 4 + 1] = rem[t * 4 + 2] = rem[t * 4 + 3] = __longlong_as_double((long long)kSentinel);
+                    done[t] = 1;
+                }
+            stall = 0;
+            __syncthreads();
         }
+        __builtin_amdgcn_s_sleep(0);
+    }
+    if (timed_out && lane == 0) atomicOr(&cnt->err, 1);
+    if (lane == 0) {
+        if (atomicAdd(donec, 1u) == (unsigned)gridDim.x - 1) {
+            atomicExch(ticket, 0u);
+            atomicExch(donec, 0u);
+        }
+    }
+}
+
+// S_p = inv(D_i) T_p for every off-diagonal block p of row i (T = L below the
+// diagonal, U above), one thread per block row.
+__global__ void scale_offdiag_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ diagpos,
+                                     const double *__restrict__ LU, const double *__restrict__ dinvL,
+                                     const double *__restrict__ dinvU, double *__restrict__ S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const int dp = diagpos[i];
+    for (int p = rp[i]; p < rp[i + 1]; ++p) {
+        if (p == dp) continue;
+        const double *D = (p < dp ? dinvL : dinvU) + (size_t)i * 16;
+        const double *T = LU + (size_t)p * 16;
+        double *o = S + (size_t)p * 16;
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c)
+                o[r * 4 + c] = D[r * 4 + 0] * T[0 * 4 + c] + D[r * 4 + 1] * T[1 * 4 + c] + D[r * 4 + 2] * T[2 * 4 + c] +
+                               D[r * 4 + 3] * T[3 * 4 + c];
     }
 }
 
@@ -216,20 +304,6 @@ __global__ __launch_bounds__(64) void csr_sptrsv_kernel(int m, int chunk, int nc
     }
 }
 
-__global__ void diag_pos_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ ci, int *__restrict__ dp,
-                                int *__restrict__ missing) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    int s = rp[i], e = rp[i + 1] - 1, pos = -1;
-    while (s <= e) {
-        const int mid = (s + e) >> 1, cc = ci[mid];
-        if (cc == i) { pos = mid; break; }
-        if (cc < i) s = mid + 1; else e = mid - 1;
-    }
-    dp[i] = pos;
-    if (pos < 0) atomicMin(missing, i);
-}
-
 __global__ void fill_sentinel_kernel(double *p, size_t n, const int *__restrict__ skip) {
     if (skip && *skip) return;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
@@ -247,11 +321,13 @@ using namespace isai;
 
 struct isai_sptrsv_plan {
     int nb = 0, nnzb = 0, bs = 0, chunk = 0;
-    int nchunksL = 0, nchunksU = 0;
+    int nchunks[2] = {0, 0};
     const int *d_row = nullptr, *d_col = nullptr;
     int *d_diag = nullptr;
-    int2 *d_chunksL = nullptr, *d_chunksU = nullptr;
-    double *d_dinvL = nullptr, *d_dinvU = nullptr, *d_y = nullptr;
+    double *d_dinvL = nullptr, *d_dinvU = nullptr, *d_y = nullptr, *d_S = nullptr;
+    int4 *d_chunks[2] = {nullptr, nullptr};
+    int *d_rem_ptr[2] = {nullptr, nullptr}, *d_rem_items[2] = {nullptr, nullptr};
+    int *d_loc_ptr[2] = {nullptr, nullptr}, *d_loc_items[2] = {nullptr, nullptr};
     Counters *d_cnt = nullptr;
 };
 
@@ -259,13 +335,117 @@ namespace isai {
 
 int sptrsv_solve_internal(isai_sptrsv_plan_t P, const double *d_LU, const double *d_b, double *d_x,
                           hipStream_t st, const int *skip) {
+    (void)d_LU;  // the solve reads the pre-scaled copy made from these values
     const size_t n = (size_t)P->nb * 4;
     fill_sentinel_kernel<<<fill_blocks(n), 256, 0, st>>>(P->d_y, n, skip);
-    bsr4_sptrsv_kernel<false><<<P->nchunksL, 64, 0, st>>>(P->nchunksL, P->d_chunksL, P->d_row, P->d_col, d_LU,
-                                                          P->d_diag, P->d_dinvL, d_b, P->d_y, d_x, P->d_cnt, skip);
-    bsr4_sptrsv_kernel<true><<<P->nchunksU, 64, 0, st>>>(P->nchunksU, P->d_chunksU, P->d_row, P->d_col, d_LU,
-                                                         P->d_diag, P->d_dinvU, P->d_y, d_x, nullptr, P->d_cnt, skip);
+    bsr4_sptrsv_pipe_kernel<false><<<P->nchunks[0], 64, 0, st>>>(
+        P->nb, P->d_chunks[0], P->d_col, P->d_S, P->d_dinvL, P->d_rem_ptr[0], P->d_rem_items[0], P->d_loc_ptr[0],
+        P->d_loc_items[0], d_b, P->d_y, d_x, P->d_cnt, skip);
+    bsr4_sptrsv_pipe_kernel<true><<<P->nchunks[1], 64, 0, st>>>(
+        P->nb, P->d_chunks[1], P->d_col, P->d_S, P->d_dinvU, P->d_rem_ptr[1], P->d_rem_items[1], P->d_loc_ptr[1],
+        P->d_loc_items[1], P->d_y, d_x, nullptr, P->d_cnt, skip);
     ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+// Chunk tables of one triangle (upper = false: L, ascending rows; true: U,
+// descending rows), in processing positions q.
+static int build_tables(int nb, const std::vector<int> &rp, const std::vector<int> &ci,
+                        const std::vector<int> &diag, bool upper, int cap_rows, std::vector<int4> &chunks,
+                        std::vector<int> &rem_ptr, std::vector<int> &rem_items, std::vector<int> &loc_ptr,
+                        std::vector<int> &loc_items) {
+    auto row_of = [&](int q) { return upper ? nb - 1 - q : q; };
+    auto pos_of = [&](int i) { return upper ? nb - 1 - i : i; };
+    auto tri = [&](int i, int &pb, int &pe) {
+        pb = upper ? diag[i] + 1 : rp[i];
+        pe = upper ? rp[i + 1] : diag[i];
+    };
+    // a row starts a chain when it does not depend on the row processed just before
+    auto starts_chain = [&](int q) {
+        if (q == 0) return true;
+        const int i = row_of(q), prev = row_of(q - 1);
+        int pb, pe;
+        tri(i, pb, pe);
+        for (int p = pb; p < pe; ++p)
+            if (ci[p] == prev) return false;
+        return true;
+    };
+    // remote / local item counts of row q inside a chunk starting at qlo
+    auto counts = [&](int q, int qlo, int &nr, int &nl) {
+        const int i = row_of(q);
+        int pb, pe;
+        tri(i, pb, pe);
+        nr = nl = 0;
+        for (int p = pb; p < pe; ++p) (pos_of(ci[p]) >= qlo ? nl : nr)++;
+    };
+    chunks.clear();
+    rem_ptr.assign(nb + 1, 0);
+    loc_ptr.assign(nb + 1, 0);
+    rem_items.clear();
+    loc_items.clear();
+    int q = 0;
+    while (q < nb) {
+        const int qlo = q;
+        int nrem = 0, nloc = 0;
+        // grow the chunk row by row; stop at a chain start once the next chain
+        // would not fit, or at the caps
+        while (q < nb) {
+            if (q > qlo && starts_chain(q)) {
+                // does the whole next chain fit?
+                int e = q + 1, r2 = nrem, l2 = nloc;
+                bool fits = true;
+                for (int t = q; t < nb && (t == q || !starts_chain(t)); ++t, e = t) {
+                    int a, b;
+                    counts(t, qlo, a, b);
+                    r2 += a;
+                    l2 += b;
+                    if (t - qlo + 1 > cap_rows || r2 > kPipeRemote || l2 > kPipeLocal) { fits = false; break; }
+                }
+                (void)e;
+                if (!fits) break;
+            }
+            int a, b;
+            counts(q, qlo, a, b);
+            if (a > 64) {
+                set_error("sptrsv analysis: block row %d depends on %d blocks outside its chunk (limit 64)", row_of(q), a);
+                return ISAI_ERR_UNSUPPORTED;
+            }
+            if (q > qlo && (q - qlo + 1 > cap_rows || nrem + a > kPipeRemote || nloc + b > kPipeLocal)) break;
+            nrem += a;
+            nloc += b;
+            ++q;
+        }
+        chunks.push_back(make_int4(qlo, q, (int)rem_items.size(), (int)loc_items.size()));
+        for (int t = qlo; t < q; ++t) {
+            rem_ptr[t] = (int)rem_items.size();
+            loc_ptr[t] = (int)loc_items.size();
+            const int i = row_of(t);
+            int pb, pe;
+            tri(i, pb, pe);
+            for (int p = pb; p < pe; ++p) (pos_of(ci[p]) >= qlo ? loc_items : rem_items).push_back(p);
+        }
+        // keep loc_ptr ordered: locals of row t come after remotes bookkeeping above
+    }
+    // the per-position pointers must list each row's own items: rebuild them in order
+    rem_ptr.assign(nb + 1, 0);
+    loc_ptr.assign(nb + 1, 0);
+    {
+        int r = 0, l = 0;
+        size_t c = 0;
+        for (int t = 0; t < nb; ++t) {
+            while (c + 1 < chunks.size() && chunks[c + 1].x <= t) ++c;
+            const int qlo = chunks[c].x;
+            rem_ptr[t] = r;
+            loc_ptr[t] = l;
+            int a, b;
+            counts(t, qlo, a, b);
+            r += a;
+            l += b;
+        }
+        rem_ptr[nb] = r;
+        loc_ptr[nb] = l;
+    }
+    chunks.push_back(make_int4(nb, nb, (int)rem_items.size(), (int)loc_items.size()));
     return ISAI_OK;
 }
 
@@ -282,74 +462,74 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         set_error("isai_bsr_sptrsv: the BSR-native solve is built for bs = 4 (got %d); use the CSR form", bs);
         return ISAI_ERR_UNSUPPORTED;
     }
-    if (chunk <= 0) chunk = kMaxChunk;
-    ISAI_REQUIRE(chunk <= kMaxChunk, "isai_bsr_sptrsv_analysis: chunk %d above %d", chunk, kMaxChunk);
+    if (chunk <= 0) chunk = kPipeRows;
+    ISAI_REQUIRE(chunk <= kPipeRows, "isai_bsr_sptrsv_analysis: chunk %d above %d", chunk, kPipeRows);
     hipStream_t st = (hipStream_t)stream;
-    // Chunks follow the dependency chains: a new L chunk starts at row i when row i
-    // has no block in column i-1 (so it does not wait for the previous chunk's last
-    // row); a new U chunk starts (walking down from the last row) when row i has
-    // no block in column i+1.  Chunks are capped at `chunk` rows.
-    std::vector<int> rp(nb + 1), ci(nnzb);
+    std::vector<int> rp(nb + 1), ci(nnzb), diag(nb, -1);
     ISAI_HIP(hipMemcpyAsync(rp.data(), d_row, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, st));
     ISAI_HIP(hipMemcpyAsync(ci.data(), d_col, sizeof(int) * nnzb, hipMemcpyDeviceToHost, st));
     ISAI_HIP(hipStreamSynchronize(st));
-    auto has = [&](int i, int j) {
-        for (int p = rp[i]; p < rp[i + 1]; ++p)
-            if (ci[p] == j) return true;
-        return false;
-    };
-    std::vector<int2> cl, cu;
-    for (int i = 0; i < nb;) {
-        int j = i + 1;
-        while (j < nb && j - i < chunk && has(j, j - 1)) ++j;
-        cl.push_back(make_int2(i, j));
-        i = j;
+    for (int i = 0; i < nb; ++i) {
+        for (int p = rp[i]; p < rp[i + 1]; ++p) {
+            if (p > rp[i] && ci[p] <= ci[p - 1]) {
+                set_error("isai_bsr_sptrsv_analysis: block columns of row %d are not strictly ascending", i);
+                return ISAI_ERR_INVALID_ARG;
+            }
+            if (ci[p] == i) diag[i] = p;
+        }
+        if (diag[i] < 0) {
+            set_error("isai_bsr_sptrsv_analysis: block row %d has no diagonal block", i);
+            return ISAI_ERR_ZERO_PIVOT;
+        }
     }
-    for (int i = nb - 1; i >= 0;) {
-        int j = i - 1;  // chunk is (j, i]
-        while (j >= 0 && i - j < chunk && has(j, j + 1)) --j;
-        cu.push_back(make_int2(j + 1, i + 1));
-        i = j;
+    std::vector<int4> chunks[2];
+    std::vector<int> rptr[2], ritems[2], lptr[2], litems[2];
+    for (int u = 0; u < 2; ++u) {
+        int s = build_tables(nb, rp, ci, diag, u == 1, chunk, chunks[u], rptr[u], ritems[u], lptr[u], litems[u]);
+        if (s != ISAI_OK) return s;
     }
     auto *P = new isai_sptrsv_plan;
     P->nb = nb;
     P->nnzb = nnzb;
     P->bs = bs;
     P->chunk = chunk;
-    P->nchunksL = (int)cl.size();
-    P->nchunksU = (int)cu.size();
     P->d_row = d_row;
     P->d_col = d_col;
     bool ok = hipMalloc(&P->d_diag, sizeof(int) * nb) == hipSuccess &&
               hipMalloc(&P->d_dinvL, sizeof(double) * 16 * nb) == hipSuccess &&
               hipMalloc(&P->d_dinvU, sizeof(double) * 16 * nb) == hipSuccess &&
               hipMalloc(&P->d_y, sizeof(double) * 4 * nb) == hipSuccess &&
-              hipMalloc(&P->d_chunksL, sizeof(int2) * cl.size()) == hipSuccess &&
-              hipMalloc(&P->d_chunksU, sizeof(int2) * cu.size()) == hipSuccess &&
+              hipMalloc(&P->d_S, sizeof(double) * 16 * (size_t)nnzb) == hipSuccess &&
               hipMalloc(&P->d_cnt, sizeof(Counters) + sizeof(int)) == hipSuccess;
+    for (int u = 0; u < 2 && ok; ++u) {
+        P->nchunks[u] = (int)chunks[u].size() - 1;
+        ok = hipMalloc(&P->d_chunks[u], sizeof(int4) * chunks[u].size()) == hipSuccess &&
+             hipMalloc(&P->d_rem_ptr[u], sizeof(int) * (nb + 1)) == hipSuccess &&
+             hipMalloc(&P->d_loc_ptr[u], sizeof(int) * (nb + 1)) == hipSuccess &&
+             hipMalloc(&P->d_rem_items[u], sizeof(int) * std::max<size_t>(ritems[u].size(), 1)) == hipSuccess &&
+             hipMalloc(&P->d_loc_items[u], sizeof(int) * std::max<size_t>(litems[u].size(), 1)) == hipSuccess;
+    }
     if (!ok) {
         isai_bsr_sptrsv_destroy(P);
         set_error("isai_bsr_sptrsv_analysis: hipMalloc failed");
         return ISAI_ERR_ALLOC;
     }
-    ISAI_HIP(hipMemcpy(P->d_chunksL, cl.data(), sizeof(int2) * cl.size(), hipMemcpyHostToDevice));
-    ISAI_HIP(hipMemcpy(P->d_chunksU, cu.data(), sizeof(int2) * cu.size(), hipMemcpyHostToDevice));
-    Counters h{};
-    h.spin_limit = 1 << 20;
-    int *missing = (int *)(P->d_cnt + 1);
-    const int big = INT_MAX;
-    ISAI_HIP(hipMemcpy(P->d_cnt, &h, sizeof(h), hipMemcpyHostToDevice));
-    ISAI_HIP(hipMemcpy(missing, &big, sizeof(int), hipMemcpyHostToDevice));
-    diag_pos_kernel<<<grid_for(nb, 256), 256, 0, st>>>(nb, d_row, d_col, P->d_diag, missing);
-    ISAI_LAUNCH_CHECK();
-    int miss = INT_MAX;
-    ISAI_HIP(hipMemcpyAsync(&miss, missing, sizeof(int), hipMemcpyDeviceToHost, st));
-    ISAI_HIP(hipStreamSynchronize(st));
-    if (miss != INT_MAX) {
-        isai_bsr_sptrsv_destroy(P);
-        set_error("isai_bsr_sptrsv_analysis: block row %d has no diagonal block", miss);
-        return ISAI_ERR_ZERO_PIVOT;
+    ISAI_HIP(hipMemcpy(P->d_diag, diag.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+    for (int u = 0; u < 2; ++u) {
+        ISAI_HIP(hipMemcpy(P->d_chunks[u], chunks[u].data(), sizeof(int4) * chunks[u].size(), hipMemcpyHostToDevice));
+        ISAI_HIP(hipMemcpy(P->d_rem_ptr[u], rptr[u].data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+        ISAI_HIP(hipMemcpy(P->d_loc_ptr[u], lptr[u].data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+        if (!ritems[u].empty())
+            ISAI_HIP(hipMemcpy(P->d_rem_items[u], ritems[u].data(), sizeof(int) * ritems[u].size(), hipMemcpyHostToDevice));
+        if (!litems[u].empty())
+            ISAI_HIP(hipMemcpy(P->d_loc_items[u], litems[u].data(), sizeof(int) * litems[u].size(), hipMemcpyHostToDevice));
     }
+    Counters h{};
+    // steps without progress before a chunk gives up on its missing inputs: a
+    // step is one agent-scope poll round (about a microsecond), so this bound is
+    // tens of milliseconds, far above any legitimate wait inside one solve.
+    h.spin_limit = 1 << 16;
+    ISAI_HIP(hipMemcpy(P->d_cnt, &h, sizeof(h), hipMemcpyHostToDevice));
     *plan = P;
     return isai_bsr_sptrsv_update_values(P, d_LU, stream);
 }
@@ -358,7 +538,11 @@ int isai_bsr_sptrsv_update_values(isai_sptrsv_plan_t P, const double *d_LU, isai
     ISAI_REQUIRE(P && d_LU, "isai_bsr_sptrsv_update_values: null argument");
     int s = isai_build_dinv(P->nb, 4, P->d_row, P->d_col, d_LU, 1, P->d_dinvL, stream);
     if (s == ISAI_OK) s = isai_build_dinv(P->nb, 4, P->d_row, P->d_col, d_LU, 0, P->d_dinvU, stream);
-    return s;
+    if (s != ISAI_OK) return s;
+    scale_offdiag_kernel<<<grid_for(P->nb, 128), 128, 0, (hipStream_t)stream>>>(P->nb, P->d_row, P->d_diag, d_LU,
+                                                                               P->d_dinvL, P->d_dinvU, P->d_S);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
 }
 
 int isai_bsr_sptrsv_solve(isai_sptrsv_plan_t P, const double *d_LU, const double *d_b, double *d_x,
@@ -394,9 +578,15 @@ int isai_bsr_sptrsv_destroy(isai_sptrsv_plan_t P) {
     (void)hipFree(P->d_dinvL);
     (void)hipFree(P->d_dinvU);
     (void)hipFree(P->d_y);
-    (void)hipFree(P->d_chunksL);
-    (void)hipFree(P->d_chunksU);
+    (void)hipFree(P->d_S);
     (void)hipFree(P->d_cnt);
+    for (int u = 0; u < 2; ++u) {
+        (void)hipFree(P->d_chunks[u]);
+        (void)hipFree(P->d_rem_ptr[u]);
+        (void)hipFree(P->d_rem_items[u]);
+        (void)hipFree(P->d_loc_ptr[u]);
+        (void)hipFree(P->d_loc_items[u]);
+    }
     delete P;
     return ISAI_OK;
 }

@@ -21,7 +21,7 @@ def _plan(rp, ci, d_rp, d_ci, d_lu, chunk):
 
 
 @pytest.mark.parametrize("problem,L", [(0, 16), (1, 8), (2, 5), (3, 12)])
-@pytest.mark.parametrize("chunk", [1, 5, 64, 256])
+@pytest.mark.parametrize("chunk", [1, 5, 64, 128])
 def test_bsr_sptrsv_matches_oracle(gpu, problem, L, chunk):
     print(f"bsr sptrsv problem={problem} L={L} chunk={chunk}", flush=True)
     rp, ci, val = problems.generate(problem, L, seed=2)
