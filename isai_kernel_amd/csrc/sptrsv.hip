@@ -21,6 +21,7 @@
 //     hanging, which the host reads with isai_sptrsv_status().
 // The last wave of a solve resets the ticket counters, so consecutive solves
 // (or graph replays) need no memset.
+#include <algorithm>
 #include <climits>
 #include <vector>
 
@@ -89,21 +90,30 @@ constexpr int kPipeRows = 128;   // rows per chunk
 constexpr int kPipeRemote = 512; // remote items per chunk
 constexpr int kPipeLocal = 128;  // local items per chunk
 
+static size_t pipe_lds_bytes(int rows, int rem, int loc) {
+    return sizeof(double) * ((size_t)rows * 8 + (size_t)rem * 4 + (size_t)loc * 16) +
+           sizeof(int) * ((size_t)rem * 3 + loc + 2 * ((size_t)rows + 1));
+}
+
 template <bool UPPER>
 __global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
     int nb, const int4 *__restrict__ chunks, const int *__restrict__ ci, const double *__restrict__ S,
     const double *__restrict__ dinv, const int *__restrict__ rem_ptr, const int *__restrict__ rem_items,
     const int *__restrict__ loc_ptr, const int *__restrict__ loc_items, const double *rhs, double *out,
-    double *sentinel_fill, Counters *cnt, const int *__restrict__ skip) {
+    double *sentinel_fill, Counters *cnt, const int *__restrict__ skip, int max_rows, int max_rem, int max_loc) {
     if (skip && *skip) return;
-    __shared__ double zb[kPipeRows * 4];
-    __shared__ double xs[kPipeRows * 4];
-    __shared__ double rem[kPipeRemote * 4];
-    __shared__ double locv[kPipeLocal * 16];
-    __shared__ int done[kPipeRemote];
-    __shared__ int lk[kPipeLocal];
-    __shared__ int rrp[kPipeRows + 1];
-    __shared__ int lrp[kPipeRows + 1];
+    // LDS carved to the largest chunk of this triangle
+    extern __shared__ double dyn[];
+    double *zb = dyn;                         // max_rows * 4
+    double *xs = zb + max_rows * 4;           // max_rows * 4
+    double *rem = xs + max_rows * 4;          // max_rem * 4
+    double *locv = rem + max_rem * 4;         // max_loc * 16
+    int *done = (int *)(locv + max_loc * 16); // max_rem
+    int *ik = done + max_rem;                 // max_rem: column position of the item's block
+    int *ip = ik + max_rem;                   // max_rem: block position of the item
+    int *lk = ip + max_rem;                   // max_loc
+    int *rrp = lk + max_loc;                  // max_rows + 1
+    int *lrp = rrp + max_rows + 1;            // max_rows + 1
     __shared__ int s_c, s_chain;
     const int lane = threadIdx.x;
     unsigned int *ticket = UPPER ? &cnt->ticket_u : &cnt->ticket_l;
@@ -111,10 +121,11 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
     const int limit = cnt->spin_limit;
     if (lane == 0) s_c = (int)atomicAdd(ticket, 1u);
     __syncthreads();
-    const int4 ch = chunks[s_c];
-    const int4 nx = chunks[s_c + 1];
+    // chunk c: [2c] = (qlo, qhi, first remote item, first local item), [2c+1] = (end remote, end local, level, 0)
+    const int4 ch = chunks[2 * s_c];
+    const int4 ce = chunks[2 * s_c + 1];
     const int qlo = ch.x, qhi = ch.y, r0 = ch.z, l0 = ch.w;
-    const int nrows = qhi - qlo, nrem = nx.z - r0, nloc = nx.w - l0;
+    const int nrows = qhi - qlo, nrem = ce.x - r0, nloc = ce.y - l0;
     auto row_of = [&](int q) { return UPPER ? nb - 1 - q : q; };
 
     // ---- prologue
@@ -135,7 +146,12 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
         lk[t] = (UPPER ? nb - 1 - k : k) - qlo;
     }
     for (int t = lane; t < nloc * 16; t += 64) locv[t] = S[(size_t)loc_items[l0 + (t >> 4)] * 16 + (t & 15)];
-    for (int t = lane; t < nrem; t += 64) done[t] = 0;
+    for (int t = lane; t < nrem; t += 64) {
+        const int p = rem_items[r0 + t];
+        ip[t] = p;
+        ik[t] = ci[p];
+        done[t] = 0;
+    }
     __syncthreads();
 
     int chain = 0, stall = 0;
@@ -145,8 +161,8 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
         int got = 0;
         const int w = rrp[chain] + lane;
         if (w < nrem && !done[w]) {
-            const int p = rem_items[r0 + w];
-            const int k = ci[p];
+            const int p = ip[w];
+            const int k = ik[w];
             unsigned long long *xq = (unsigned long long *)(out + (size_t)k * 4);
             const unsigned long long u0 = __hip_atomic_load(xq + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const unsigned long long u1 = __hip_atomic_load(xq + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -322,6 +338,7 @@ using namespace isai;
 struct isai_sptrsv_plan {
     int nb = 0, nnzb = 0, bs = 0, chunk = 0;
     int nchunks[2] = {0, 0};
+    int maxr[2] = {0, 0}, maxrem[2] = {0, 0}, maxloc[2] = {0, 0};
     const int *d_row = nullptr, *d_col = nullptr;
     int *d_diag = nullptr;
     double *d_dinvL = nullptr, *d_dinvU = nullptr, *d_y = nullptr, *d_S = nullptr;
@@ -338,12 +355,12 @@ int sptrsv_solve_internal(isai_sptrsv_plan_t P, const double *d_LU, const double
     (void)d_LU;  // the solve reads the pre-scaled copy made from these values
     const size_t n = (size_t)P->nb * 4;
     fill_sentinel_kernel<<<fill_blocks(n), 256, 0, st>>>(P->d_y, n, skip);
-    bsr4_sptrsv_pipe_kernel<false><<<P->nchunks[0], 64, 0, st>>>(
+    bsr4_sptrsv_pipe_kernel<false><<<P->nchunks[0], 64, pipe_lds_bytes(P->maxr[0], P->maxrem[0], P->maxloc[0]), st>>>(
         P->nb, P->d_chunks[0], P->d_col, P->d_S, P->d_dinvL, P->d_rem_ptr[0], P->d_rem_items[0], P->d_loc_ptr[0],
-        P->d_loc_items[0], d_b, P->d_y, d_x, P->d_cnt, skip);
-    bsr4_sptrsv_pipe_kernel<true><<<P->nchunks[1], 64, 0, st>>>(
+        P->d_loc_items[0], d_b, P->d_y, d_x, P->d_cnt, skip, P->maxr[0], P->maxrem[0], P->maxloc[0]);
+    bsr4_sptrsv_pipe_kernel<true><<<P->nchunks[1], 64, pipe_lds_bytes(P->maxr[1], P->maxrem[1], P->maxloc[1]), st>>>(
         P->nb, P->d_chunks[1], P->d_col, P->d_S, P->d_dinvU, P->d_rem_ptr[1], P->d_rem_items[1], P->d_loc_ptr[1],
-        P->d_loc_items[1], P->d_y, d_x, nullptr, P->d_cnt, skip);
+        P->d_loc_items[1], P->d_y, d_x, nullptr, P->d_cnt, skip, P->maxr[1], P->maxrem[1], P->maxloc[1]);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
@@ -445,7 +462,38 @@ static int build_tables(int nb, const std::vector<int> &rp, const std::vector<in
         rem_ptr[nb] = r;
         loc_ptr[nb] = l;
     }
-    chunks.push_back(make_int4(nb, nb, (int)rem_items.size(), (int)loc_items.size()));
+    // Dispatch order: chunks sorted by their level in the chunk dependency graph
+    // (1 + the deepest chunk any of their remote items lives in), rows order inside
+    // a level.  Resident waves then hold the active wavefront instead of a block of
+    // consecutive rows, and a chunk only waits on chunks with lower tickets.
+    const int nch = (int)chunks.size();
+    std::vector<int> chunk_of(nb), level(nch, 0);
+    for (int c = 0; c < nch; ++c)
+        for (int t = chunks[c].x; t < chunks[c].y; ++t) chunk_of[t] = c;
+    int nlev = 0;
+    for (int c = 0; c < nch; ++c) {
+        const int rb = chunks[c].z, re = c + 1 < nch ? chunks[c + 1].z : (int)rem_items.size();
+        int l = 0;
+        for (int w = rb; w < re; ++w) l = std::max(l, level[chunk_of[pos_of(ci[rem_items[w]])]] + 1);
+        level[c] = l;
+        nlev = std::max(nlev, l + 1);
+    }
+    std::vector<int> order(nch);
+    {
+        std::vector<int> cnt(nlev + 1, 0);
+        for (int c = 0; c < nch; ++c) cnt[level[c] + 1]++;
+        for (int l = 0; l < nlev; ++l) cnt[l + 1] += cnt[l];
+        for (int c = 0; c < nch; ++c) order[cnt[level[c]]++] = c;
+    }
+    std::vector<int4> out;
+    out.reserve(2 * nch);
+    for (int c : order) {
+        const int re = c + 1 < nch ? chunks[c + 1].z : (int)rem_items.size();
+        const int le = c + 1 < nch ? chunks[c + 1].w : (int)loc_items.size();
+        out.push_back(chunks[c]);
+        out.push_back(make_int4(re, le, level[c], 0));
+    }
+    chunks.swap(out);
     return ISAI_OK;
 }
 
@@ -501,8 +549,16 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
               hipMalloc(&P->d_y, sizeof(double) * 4 * nb) == hipSuccess &&
               hipMalloc(&P->d_S, sizeof(double) * 16 * (size_t)nnzb) == hipSuccess &&
               hipMalloc(&P->d_cnt, sizeof(Counters) + sizeof(int)) == hipSuccess;
+    for (int u = 0; u < 2; ++u) {
+        P->nchunks[u] = (int)chunks[u].size() / 2;
+        for (int c = 0; c < P->nchunks[u]; ++c) {
+            const int4 a = chunks[u][2 * c], b = chunks[u][2 * c + 1];
+            P->maxr[u] = std::max(P->maxr[u], a.y - a.x);
+            P->maxrem[u] = std::max(P->maxrem[u], b.x - a.z);
+            P->maxloc[u] = std::max(P->maxloc[u], b.y - a.w);
+        }
+    }
     for (int u = 0; u < 2 && ok; ++u) {
-        P->nchunks[u] = (int)chunks[u].size() - 1;
         ok = hipMalloc(&P->d_chunks[u], sizeof(int4) * chunks[u].size()) == hipSuccess &&
              hipMalloc(&P->d_rem_ptr[u], sizeof(int) * (nb + 1)) == hipSuccess &&
              hipMalloc(&P->d_loc_ptr[u], sizeof(int) * (nb + 1)) == hipSuccess &&
