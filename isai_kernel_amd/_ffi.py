@@ -108,6 +108,16 @@ SIGNATURES: dict[str, list] = {
     "isai_precond_apply": [P, P, P, P],
     "isai_solve": [P, I, I, D, I, P, P, P, P, P],
     "isai_destroy": [P],
+    # isai_batch.h
+    "isai_batch_setup": [I, P, P, I, P, P, P, I, P, P],
+    "isai_batch_vector_length": [P, P],
+    "isai_batch_vector_offset": [P, I, I, P],
+    "isai_batch_pairs": [P, P],
+    "isai_batch_tuning": [P, P, P, P],
+    "isai_batch_setup_times": [P, P, P],
+    "isai_batch_spmm": [P, P, P, P],
+    "isai_batch_solve": [P, D, I, P, P, I, P, P, P, P],
+    "isai_batch_destroy": [P],
     # bsr_tensor_SpMV.h
     "isai_bsr_spmv": [I, I, I, P, P, P, P, P, P],
     "isai_bsr_spmv_prec": [I, I, I, I, P, P, P, P, P, P],

@@ -1,0 +1,877 @@
+// batch.hip -- the batching form: uneven block-diagonal systems with 1..16
+// right-hand sides each, one batched setup, one batched BiCGSTAB whose
+// iterations are captured as a hipGraph and replayed.
+//
+// Layout: pair p = (system s, right-hand side j) owns a contiguous vector segment
+// of 4*nb_s doubles, padded to a multiple of kTile = 1024; the segments of one
+// system follow each other with a fixed stride, and tile t of the multi-vector
+// belongs to one pair (tile_pair[t]).  Reductions are per tile (fixed shape), then
+// per pair over its contiguous tile range in order, so every result is
+// deterministic and independent of how many iterations one replay holds.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "internal.hpp"
+#include "isai_batch.h"
+#include "isai_solve.h"
+
+namespace isai {
+
+constexpr int kTile = 1024;
+constexpr int kMaxRhs = 16;
+constexpr int kItersPerReplay = 4;  // iterations in one captured graph
+constexpr int kMaxRestarts = 8;
+
+// status: 0 running, 1 converged (recursive residual), 2 converged on s with the
+// x update pending in this iteration, 3 breakdown, 4 done (true residual met),
+// 5 iteration limit reached
+struct PairState {
+    double rho, rho_old, alpha, omega, beta, bnorm2, tol2, pad;
+    int status, iters, first, pad2;
+};
+
+struct Ctl {
+    int nactive, maxit, pad[2];
+};
+
+struct BatchDev {
+    const int *sys_of_row;      // [nb]
+    const int *sys_ptr;         // [nsys+1] block rows
+    const int *sys_nrhs;        // [nsys]
+    const int *sys_pair;        // [nsys] first pair of the system
+    const long long *sys_vec;   // [nsys] offset of the system's first segment
+    const long long *sys_step;  // [nsys] padded segment length
+    const int *tile_pair;       // [ntiles]
+    const int *ptile;           // [npairs+1]
+    PairState *ps;              // [npairs]
+    double *part0, *part1;      // [ntiles]
+    Ctl *ctl;
+};
+
+__device__ __forceinline__ unsigned active_mask(const BatchDev &B, int p0, int r, int all) {
+    unsigned m = 0;
+    for (int j = 0; j < r; ++j)
+        if (all || B.ps[p0 + j].status == 0) m |= 1u << j;
+    return m;
+}
+
+// --------------------------------------------------------------- multi-RHS SpMM
+// Eight lanes per block row (one 128-byte line of FP64 block values per
+// instruction, as in the single-system product); every block is loaded once and
+// applied to all right-hand sides of its system that are still running.
+// FP64: lane l holds row l/2, columns 2(l%2)..+1; FMA chain per lane, then the
+// two lanes of a row are added.
+__global__ void bspmm_f64_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+                                 const v2d *__restrict__ val, BatchDev B, const double *__restrict__ X,
+                                 double *__restrict__ Y, int remap, int all) {
+    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int br = wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3);
+    if (br >= nb) return;  // whole 8-lane groups leave together
+    const int sys = B.sys_of_row[br];
+    const unsigned mask = active_mask(B, B.sys_pair[sys], B.sys_nrhs[sys], all);
+    if (!mask) return;
+    const int base = B.sys_ptr[sys];
+    const long long v0 = B.sys_vec[sys], step = B.sys_step[sys];
+    const int lane = threadIdx.x & 7, h = lane & 1;
+    double acc0[kMaxRhs], acc1[kMaxRhs];
+#pragma unroll
+    for (int j = 0; j < kMaxRhs; ++j) acc0[j] = acc1[j] = 0.0;
+    const int rs = rp[br], re = rp[br + 1];
+    for (int k = rs; k < re; ++k) {
+        const int c = ci[k] - base;
+        const v2d a = __builtin_nontemporal_load(&val[(size_t)k * 8 + lane]);
+        const double *xc = X + v0 + (size_t)c * 4 + 2 * h;
+#pragma unroll
+        for (int j = 0; j < kMaxRhs; ++j) {
+            if (mask & (1u << j)) {
+                const v2d x = *(const v2d *)(xc + j * step);
+                acc0[j] = fma(a.x, x.x, acc0[j]);
+                acc1[j] = fma(a.y, x.y, acc1[j]);
+            }
+        }
+    }
+    double *yr = Y + v0 + (size_t)(br - base) * 4 + (lane >> 1);
+#pragma unroll
+    for (int j = 0; j < kMaxRhs; ++j) {
+        double s = acc0[j] + acc1[j];
+        s += __shfl_xor(s, 1);
+        if ((mask & (1u << j)) && h == 0) yr[j * step] = s;
+    }
+}
+
+// FP32 / BF16 factor values (the single-system lane layouts): FP32 lane l holds row
+// l%4 of block l/4 (two blocks per line); BF16 lane l holds rows 2(l%2), 2(l%2)+1 of
+// block l/2 (four blocks per line).  FP32 accumulation, x rounded to FP32 on load.
+__device__ __forceinline__ float b16_lo(unsigned int w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float b16_hi(unsigned int w) { return __uint_as_float(w & 0xffff0000u); }
+
+template <bool BF16>
+__global__ void bspmm_lowp_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+                                  const v4u *__restrict__ val16, BatchDev B, const double *__restrict__ X,
+                                  double *__restrict__ Y, int remap, int all) {
+    constexpr int BPB = BF16 ? 4 : 2;
+    constexpr int LPB = 8 / BPB;
+    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int br = wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3);
+    if (br >= nb) return;
+    const int sys = B.sys_of_row[br];
+    const unsigned mask = active_mask(B, B.sys_pair[sys], B.sys_nrhs[sys], all);
+    if (!mask) return;
+    const int base = B.sys_ptr[sys];
+    const long long v0 = B.sys_vec[sys], step = B.sys_step[sys];
+    const int lane = threadIdx.x & 7, b = lane / LPB, sub = lane % LPB;
+    float acc0[kMaxRhs], acc1[kMaxRhs];
+#pragma unroll
+    for (int j = 0; j < kMaxRhs; ++j) acc0[j] = acc1[j] = 0.0f;
+    const int rs = rp[br], re = rp[br + 1];
+    for (int k = rs; k < re; k += BPB) {
+        const int kb = k + b;
+        if (kb >= re) continue;
+        const int c = ci[kb] - base;
+        const v4u w = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kb * LPB + sub]) : val16[(size_t)kb * LPB + sub];
+        const double *xc = X + v0 + (size_t)c * 4;
+#pragma unroll
+        for (int j = 0; j < kMaxRhs; ++j) {
+            if (mask & (1u << j)) {
+                const v2d x01 = *(const v2d *)(xc + j * step);
+                const v2d x23 = *(const v2d *)(xc + j * step + 2);
+                const float f0 = (float)x01.x, f1 = (float)x01.y, f2 = (float)x23.x, f3 = (float)x23.y;
+                if (BF16) {
+                    acc0[j] = fmaf(b16_lo(w.x), f0, acc0[j]);
+                    acc0[j] = fmaf(b16_hi(w.x), f1, acc0[j]);
+                    acc0[j] = fmaf(b16_lo(w.y), f2, acc0[j]);
+                    acc0[j] = fmaf(b16_hi(w.y), f3, acc0[j]);
+                    acc1[j] = fmaf(b16_lo(w.z), f0, acc1[j]);
+                    acc1[j] = fmaf(b16_hi(w.z), f1, acc1[j]);
+                    acc1[j] = fmaf(b16_lo(w.w), f2, acc1[j]);
+                    acc1[j] = fmaf(b16_hi(w.w), f3, acc1[j]);
+                } else {
+                    acc0[j] = fmaf(__uint_as_float(w.x), f0, acc0[j]);
+                    acc0[j] = fmaf(__uint_as_float(w.y), f1, acc0[j]);
+                    acc0[j] = fmaf(__uint_as_float(w.z), f2, acc0[j]);
+                    acc0[j] = fmaf(__uint_as_float(w.w), f3, acc0[j]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kMaxRhs; ++j) {
+#pragma unroll
+        for (int off = LPB; off < 8; off <<= 1) {
+            acc0[j] += __shfl_xor(acc0[j], off);
+            if (BF16) acc1[j] += __shfl_xor(acc1[j], off);
+        }
+    }
+    if (b != 0) return;
+    double *yr = Y + v0 + (size_t)(br - base) * 4;
+#pragma unroll
+    for (int j = 0; j < kMaxRhs; ++j) {
+        if (!(mask & (1u << j))) continue;
+        if (BF16) {
+            yr[j * step + 2 * sub] = (double)acc0[j];
+            yr[j * step + 2 * sub + 1] = (double)acc1[j];
+        } else {
+            yr[j * step + sub] = (double)acc0[j];
+        }
+    }
+}
+
+// --------------------------------------------------------------- reductions
+constexpr int kRed = 256;  // threads of the per-tile kernels
+
+__device__ __forceinline__ double wg_sum(double v) {
+    __shared__ double ws[kRed / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
+__device__ __forceinline__ void pair_sums(const BatchDev &B, int p, double &s0, double &s1) {
+    s0 = s1 = 0.0;
+    for (int t = B.ptile[p]; t < B.ptile[p + 1]; ++t) {
+        s0 += B.part0[t];
+        s1 += B.part1[t];
+    }
+}
+
+// part0[t] = a.b and part1[t] = a.c (c optional) over tile t, for running pairs.
+__global__ __launch_bounds__(kRed) void tile_dot_kernel(const double *__restrict__ a, const double *__restrict__ b,
+                                                        const double *__restrict__ c, BatchDev B) {
+    const int t = blockIdx.x;
+    if (B.ps[B.tile_pair[t]].status != 0) return;  // the pair's finals skip it too
+    const size_t base = (size_t)t * kTile;
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < kTile / kRed; ++k) {
+        const size_t i = base + k * kRed + threadIdx.x;
+        s0 = fma(a[i], b[i], s0);
+        if (c) s1 = fma(a[i], c[i], s1);
+    }
+    s0 = wg_sum(s0);
+    if (c) s1 = wg_sum(s1);
+    if (threadIdx.x == 0) {
+        B.part0[t] = s0;
+        B.part1[t] = s1;
+    }
+}
+
+// ------------------------------------------------------------ BiCGSTAB kernels
+// Right-preconditioned BiCGSTAB (Saad, Iterative Methods, Alg. 7.7 with M = M_U M_L),
+// the same recurrences as the single-system solver, with one set of scalars per pair.
+__global__ void zero_active_kernel(Ctl *ctl) { ctl->nactive = 0; }
+
+// r = b - r (r holds A x), rhat = r, p = v = 0 for running pairs; partials ||b||^2, ||r||^2
+__global__ __launch_bounds__(kRed) void binit_kernel(const double *__restrict__ b, double *r, double *rh, double *p,
+                                                     double *v, BatchDev B) {
+    const int t = blockIdx.x;
+    if (B.ps[B.tile_pair[t]].status != 0) return;
+    const size_t base = (size_t)t * kTile;
+    double sb = 0.0, sr = 0.0;
+#pragma unroll
+    for (int k = 0; k < kTile / kRed; ++k) {
+        const size_t i = base + k * kRed + threadIdx.x;
+        const double bi = b[i], ri = bi - r[i];
+        r[i] = ri;
+        rh[i] = ri;
+        p[i] = 0.0;
+        v[i] = 0.0;
+        sb = fma(bi, bi, sb);
+        sr = fma(ri, ri, sr);
+    }
+    sb = wg_sum(sb);
+    sr = wg_sum(sr);
+    if (threadIdx.x == 0) {
+        B.part0[t] = sb;
+        B.part1[t] = sr;
+    }
+}
+
+__global__ void binit_final(BatchDev B, int npairs, double tol) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    PairState &S = B.ps[p];
+    if (S.status != 0) return;
+    double b2, r2;
+    pair_sums(B, p, b2, r2);
+    S.bnorm2 = b2 > 0.0 ? b2 : 1.0;
+    S.tol2 = tol * tol * S.bnorm2;
+    S.rho_old = S.alpha = S.omega = 1.0;
+    S.first = 1;
+    if (r2 <= S.tol2) S.status = 1;
+}
+
+__global__ void brho_final(BatchDev B, int npairs) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    PairState &S = B.ps[p];
+    if (S.status != 0) return;
+    if (S.iters >= B.ctl->maxit) {
+        S.status = 5;
+        return;
+    }
+    double rho, unused;
+    pair_sums(B, p, rho, unused);
+    S.iters++;
+    S.rho = rho;
+    if (rho == 0.0) S.status = 3;
+    else S.beta = S.first ? 0.0 : (rho / S.rho_old) * (S.alpha / S.omega);
+}
+
+// p = r + beta (p - omega v)  (p = r on a pair's first iteration)
+__global__ __launch_bounds__(kRed) void bp_kernel(const double *__restrict__ r, double *p, const double *__restrict__ v,
+                                                  BatchDev B) {
+    const int t = blockIdx.x;
+    const PairState &S = B.ps[B.tile_pair[t]];
+    if (S.status != 0) return;
+    const size_t base = (size_t)t * kTile;
+    const double beta = S.beta, omega = S.omega;
+    const bool first = S.first;
+#pragma unroll
+    for (int k = 0; k < kTile / kRed; ++k) {
+        const size_t i = base + k * kRed + threadIdx.x;
+        p[i] = first ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
+    }
+}
+
+__global__ void balpha_final(BatchDev B, int npairs) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    PairState &S = B.ps[p];
+    if (S.status != 0) return;
+    double d, unused;
+    pair_sums(B, p, d, unused);
+    if (d == 0.0) S.status = 3;
+    else S.alpha = S.rho / d;
+    S.first = 0;
+}
+
+// s = r - alpha v; partial ||s||^2
+__global__ __launch_bounds__(kRed) void bs_kernel(const double *__restrict__ r, const double *__restrict__ v, double *s,
+                                                  BatchDev B) {
+    const int t = blockIdx.x;
+    const PairState &S = B.ps[B.tile_pair[t]];
+    if (S.status != 0) return;
+    const double alpha = S.alpha;
+    const size_t base = (size_t)t * kTile;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kTile / kRed; ++k) {
+        const size_t i = base + k * kRed + threadIdx.x;
+        const double si = r[i] - alpha * v[i];
+        s[i] = si;
+        acc = fma(si, si, acc);
+    }
+    acc = wg_sum(acc);
+    if (threadIdx.x == 0) B.part0[t] = acc;
+}
+
+__global__ void bsnorm_final(BatchDev B, int npairs) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    PairState &S = B.ps[p];
+    if (S.status != 0) return;
+    double s2, unused;
+    pair_sums(B, p, s2, unused);
+    if (s2 <= S.tol2) S.status = 2;
+}
+
+// pairs converged on s: x += alpha ph
+__global__ __launch_bounds__(kRed) void bxs_kernel(double *x, const double *__restrict__ ph, BatchDev B) {
+    const int t = blockIdx.x;
+    const PairState &S = B.ps[B.tile_pair[t]];
+    if (S.status != 2) return;
+    const double alpha = S.alpha;
+    const size_t base = (size_t)t * kTile;
+#pragma unroll
+    for (int k = 0; k < kTile / kRed; ++k) {
+        const size_t i = base + k * kRed + threadIdx.x;
+        x[i] += alpha * ph[i];
+    }
+}
+
+__global__ void bomega_final(BatchDev B, int npairs) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    PairState &S = B.ps[p];
+    if (S.status != 0) return;
+    double ts, tt;
+    pair_sums(B, p, ts, tt);  // part0 = t.s, part1 = t.t
+    if (tt == 0.0) S.status = 3;
+    else S.omega = ts / tt;
+}
+
+// x += alpha ph + omega sh; r = s - omega t; partial ||r||^2
+__global__ __launch_bounds__(kRed) void bxr_kernel(double *x, double *r, const double *__restrict__ ph,
+                                                   const double *__restrict__ sh, const double *__restrict__ s,
+                                                   const double *__restrict__ tv, BatchDev B) {
+    const int t = blockIdx.x;
+    const PairState &S = B.ps[B.tile_pair[t]];
+    if (S.status != 0) return;
+    const double alpha = S.alpha, omega = S.omega;
+    const size_t base = (size_t)t * kTile;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kTile / kRed; ++k) {
+        const size_t i = base + k * kRed + threadIdx.x;
+        x[i] += alpha * ph[i] + omega * sh[i];
+        const double ri = s[i] - omega * tv[i];
+        r[i] = ri;
+        acc = fma(ri, ri, acc);
+    }
+    acc = wg_sum(acc);
+    if (threadIdx.x == 0) B.part0[t] = acc;
+}
+
+__global__ void brnorm_final(BatchDev B, int npairs) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    PairState &S = B.ps[p];
+    if (S.status == 2) S.status = 1;
+    if (S.status != 0) return;
+    double r2, unused;
+    pair_sums(B, p, r2, unused);
+    S.rho_old = S.rho;
+    if (r2 <= S.tol2) S.status = 1;
+    else atomicAdd(&B.ctl->nactive, 1);
+}
+
+// true residual of every pair: r = b - r (r holds A x); part0 = ||r||^2, part1 = ||b||^2
+__global__ __launch_bounds__(kRed) void bres_kernel(const double *__restrict__ b, double *r, BatchDev B) {
+    const int t = blockIdx.x;
+    const size_t base = (size_t)t * kTile;
+    double sr = 0.0, sb = 0.0;
+#pragma unroll
+    for (int k = 0; k < kTile / kRed; ++k) {
+        const size_t i = base + k * kRed + threadIdx.x;
+        const double bi = b[i], ri = bi - r[i];
+        r[i] = ri;
+        sr = fma(ri, ri, sr);
+        sb = fma(bi, bi, sb);
+    }
+    sr = wg_sum(sr);
+    sb = wg_sum(sb);
+    if (threadIdx.x == 0) {
+        B.part0[t] = sr;
+        B.part1[t] = sb;
+    }
+}
+
+// relres per pair; pairs whose true residual misses tol restart while iterations remain
+__global__ void bres_final(BatchDev B, int npairs, double tol, double *relres) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    double r2, b2;
+    pair_sums(B, p, r2, b2);
+    const double rr = sqrt(r2) / sqrt(b2 > 0.0 ? b2 : 1.0);
+    relres[p] = rr;
+    PairState &S = B.ps[p];
+    if (rr <= tol) S.status = 4;
+    else S.status = S.iters < B.ctl->maxit ? 0 : 5;
+}
+
+__global__ void breset_kernel(BatchDev B, int npairs) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    B.ps[p].status = 0;
+    B.ps[p].iters = 0;
+}
+
+}  // namespace isai
+
+using namespace isai;
+
+struct isai_batch {
+    int nsys = 0, nb = 0, nnzb = 0, npairs = 0, ntiles = 0, precision = 0;
+    size_t len = 0;
+    const int *d_row = nullptr, *d_col = nullptr;
+    const double *d_val = nullptr;
+    isai_handle_t pre = nullptr;
+    std::vector<int> sys_ptr, nrhs, sys_pair;
+    std::vector<long long> sys_vec, sys_step;
+    int *d_sys_of_row = nullptr, *d_sys_ptr = nullptr, *d_sys_nrhs = nullptr, *d_sys_pair = nullptr;
+    long long *d_sys_vec = nullptr, *d_sys_step = nullptr;
+    int *d_tile_pair = nullptr, *d_ptile = nullptr;
+    PairState *d_ps = nullptr;
+    double *d_part = nullptr, *d_relres = nullptr;
+    Ctl *d_ctl = nullptr, *h_ctl = nullptr;
+    double *work = nullptr;  // kWork multi-vectors
+    static constexpr int kWork = 9;
+    // launch shapes picked by the autotuner: [0] product with A, [1] with the ISAI factors
+    int wg[2] = {256, 256}, remap[2] = {1, 1};
+    int variant = 0, ncand = 0;
+    double tuned_us = 0;
+    double ms_ilu = 0, ms_isai = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    double *g_x = nullptr;
+
+    BatchDev dev() const {
+        BatchDev B;
+        B.sys_of_row = d_sys_of_row;
+        B.sys_ptr = d_sys_ptr;
+        B.sys_nrhs = d_sys_nrhs;
+        B.sys_pair = d_sys_pair;
+        B.sys_vec = d_sys_vec;
+        B.sys_step = d_sys_step;
+        B.tile_pair = d_tile_pair;
+        B.ptile = d_ptile;
+        B.ps = d_ps;
+        B.part0 = d_part;
+        B.part1 = d_part + ntiles;
+        B.ctl = d_ctl;
+        return B;
+    }
+};
+
+namespace {
+
+enum Operand { kOpA = 0, kOpML = 1, kOpMU = 2 };
+
+// Y = op X over the pairs still running (all = 1: over every pair)
+int spmm(isai_batch *b, Operand op, const double *X, double *Y, int all, hipStream_t st) {
+    const BatchDev B = b->dev();
+    const int slot = op == kOpA ? 0 : 1;
+    const unsigned grid = grid_for(b->nb, b->wg[slot] / 8);
+    const unsigned threads = (unsigned)b->wg[slot];
+    if (op == kOpA) {
+        bspmm_f64_kernel<<<grid, threads, 0, st>>>(b->nb, b->d_row, b->d_col, (const v2d *)b->d_val, B, X, Y,
+                                                   b->remap[0], all);
+    } else {
+        HandleFactors f;
+        const int s = handle_factors_internal(b->pre, &f);
+        if (s != ISAI_OK) return s;
+        const int *rp = op == kOpML ? f.Lrow : f.Urow;
+        const int *ci = op == kOpML ? f.Lcol : f.Ucol;
+        const void *v = op == kOpML ? f.Lval : f.Uval;
+        if (f.precision == ISAI_PREC_FP64)
+            bspmm_f64_kernel<<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v2d *)v, B, X, Y, b->remap[1], all);
+        else if (f.precision == ISAI_PREC_FP32)
+            bspmm_lowp_kernel<false><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, b->remap[1], all);
+        else
+            bspmm_lowp_kernel<true><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, b->remap[1], all);
+    }
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+#define BTRY(x)                       \
+    do {                              \
+        const int _s = (x);           \
+        if (_s != ISAI_OK) return _s; \
+    } while (0)
+
+// One BiCGSTAB iteration of every running pair: kernels only, so it can be captured.
+int batch_iteration(isai_batch *b, double *x, hipStream_t st) {
+    const size_t n = b->len;
+    double *r = b->work, *rh = r + n, *p = rh + n, *v = p + n, *ph = v + n, *s = ph + n, *sh = s + n, *t = sh + n,
+           *tmp = t + n;
+    const BatchDev B = b->dev();
+    const int np = b->npairs, pg = (np + 255) / 256, T = b->ntiles;
+    zero_active_kernel<<<1, 1, 0, st>>>(b->d_ctl);
+    tile_dot_kernel<<<T, kRed, 0, st>>>(rh, r, nullptr, B);
+    brho_final<<<pg, 256, 0, st>>>(B, np);
+    bp_kernel<<<T, kRed, 0, st>>>(r, p, v, B);
+    BTRY(spmm(b, kOpML, p, tmp, 0, st));
+    BTRY(spmm(b, kOpMU, tmp, ph, 0, st));
+    BTRY(spmm(b, kOpA, ph, v, 0, st));
+    tile_dot_kernel<<<T, kRed, 0, st>>>(rh, v, nullptr, B);
+    balpha_final<<<pg, 256, 0, st>>>(B, np);
+    bs_kernel<<<T, kRed, 0, st>>>(r, v, s, B);
+    bsnorm_final<<<pg, 256, 0, st>>>(B, np);
+    bxs_kernel<<<T, kRed, 0, st>>>(x, ph, B);
+    BTRY(spmm(b, kOpML, s, tmp, 0, st));
+    BTRY(spmm(b, kOpMU, tmp, sh, 0, st));
+    BTRY(spmm(b, kOpA, sh, t, 0, st));
+    tile_dot_kernel<<<T, kRed, 0, st>>>(t, s, t, B);
+    bomega_final<<<pg, 256, 0, st>>>(B, np);
+    bxr_kernel<<<T, kRed, 0, st>>>(x, r, ph, sh, s, t, B);
+    brnorm_final<<<pg, 256, 0, st>>>(B, np);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+// Launch-shape autotuner: for the product with A (FP64) and with the ISAI factors
+// (storage precision) time every (workgroup size, XCD remap) candidate on the batch
+// itself, all pairs active, and keep the fastest of each.
+constexpr int kWgCand[4] = {64, 128, 256, 512};
+constexpr int kCandPerOp = 8;
+
+int autotune(isai_batch *b, hipStream_t st) {
+    double *X = b->work, *Y = b->work + b->len;
+    hipEvent_t e0, e1;
+    ISAI_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        set_error("isai_batch_setup: hipEventCreate failed");
+        return ISAI_ERR_HIP;
+    }
+    int status = ISAI_OK;
+    double best_us[2] = {1e300, 1e300};
+    int best[2] = {0, 0};
+    for (int slot = 0; slot < 2 && status == ISAI_OK; ++slot) {
+        const Operand op = slot == 0 ? kOpA : kOpML;
+        for (int c = 0; c < kCandPerOp && status == ISAI_OK; ++c) {
+            b->wg[slot] = kWgCand[c / 2];
+            b->remap[slot] = c % 2;
+            status = spmm(b, op, X, Y, 1, st);  // warm-up
+            if (status != ISAI_OK) break;
+            if (hipEventRecord(e0, st) != hipSuccess) status = ISAI_ERR_HIP;
+            for (int k = 0; k < 5 && status == ISAI_OK; ++k) status = spmm(b, op, X, Y, 1, st);
+            float ms = 0.f;
+            if (status == ISAI_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                      hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                status = ISAI_ERR_HIP;
+            if (status == ISAI_ERR_HIP) set_error("isai_batch_setup: autotuner timing failed");
+            const double us = ms * 1e3 / 5;
+            if (status == ISAI_OK && us < best_us[slot]) {
+                best_us[slot] = us;
+                best[slot] = c;
+            }
+        }
+        b->wg[slot] = kWgCand[best[slot] / 2];
+        b->remap[slot] = best[slot] % 2;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->ncand = 2 * kCandPerOp;
+    b->variant = best[0];
+    b->tuned_us = best_us[0];
+    return status;
+}
+
+int capture_graph(isai_batch *b, double *d_X, hipStream_t st) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->g_x = nullptr;
+    ISAI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int s = ISAI_OK;
+    for (int k = 0; k < kItersPerReplay && s == ISAI_OK; ++k) s = batch_iteration(b, d_X, st);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (s != ISAI_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return s;
+    }
+    ISAI_HIP(e);
+    b->graph = g;
+    ISAI_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->g_x = d_X;
+    return ISAI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int isai_batch_destroy(isai_batch_t b) {
+    if (!b) return ISAI_OK;
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->pre) isai_destroy(b->pre);
+    (void)hipFree(b->d_sys_of_row);
+    (void)hipFree(b->d_sys_ptr);
+    (void)hipFree(b->d_sys_nrhs);
+    (void)hipFree(b->d_sys_pair);
+    (void)hipFree(b->d_sys_vec);
+    (void)hipFree(b->d_sys_step);
+    (void)hipFree(b->d_tile_pair);
+    (void)hipFree(b->d_ptile);
+    (void)hipFree(b->d_ps);
+    (void)hipFree(b->d_part);
+    (void)hipFree(b->d_relres);
+    (void)hipFree(b->d_ctl);
+    (void)hipFree(b->work);
+    if (b->h_ctl) (void)hipHostFree(b->h_ctl);
+    delete b;
+    return ISAI_OK;
+}
+
+int isai_batch_setup(int nsys, const int *sys_ptr, const int *nrhs, int nnzb, const int *d_row, const int *d_col,
+                     const double *d_val, int precision, isai_batch_t *out, isai_stream_t stream) {
+    ISAI_REQUIRE(out && sys_ptr && nrhs && d_row && d_col && d_val && nsys > 0 && nnzb > 0,
+                 "isai_batch_setup: bad argument");
+    ISAI_REQUIRE(precision == ISAI_PREC_FP64 || precision == ISAI_PREC_FP32 || precision == ISAI_PREC_BF16,
+                 "isai_batch_setup: unknown precision %d", precision);
+    *out = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    auto *b = new isai_batch;
+    auto fail = [&](int code) {
+        isai_batch_destroy(b);
+        return code;
+    };
+    b->nsys = nsys;
+    b->precision = precision;
+    b->sys_ptr.assign(sys_ptr, sys_ptr + nsys + 1);
+    b->nrhs.assign(nrhs, nrhs + nsys);
+    if (b->sys_ptr[0] != 0) {
+        set_error("isai_batch_setup: sys_ptr[0] must be 0 (got %d)", b->sys_ptr[0]);
+        return fail(ISAI_ERR_INVALID_ARG);
+    }
+    for (int s = 0; s < nsys; ++s) {
+        if (b->sys_ptr[s + 1] <= b->sys_ptr[s] || b->nrhs[s] < 1 || b->nrhs[s] > kMaxRhs) {
+            set_error("isai_batch_setup: system %d has %d block rows and %d right-hand sides (need >= 1 and 1..%d)",
+                      s, b->sys_ptr[s + 1] - b->sys_ptr[s], b->nrhs[s], kMaxRhs);
+            return fail(ISAI_ERR_INVALID_ARG);
+        }
+    }
+    b->nb = b->sys_ptr[nsys];
+    b->nnzb = nnzb;
+    b->d_row = d_row;
+    b->d_col = d_col;
+    b->d_val = d_val;
+    // every block must stay inside its system's diagonal block
+    {
+        std::vector<int> rp(b->nb + 1), ci(nnzb);
+        if (hipMemcpy(rp.data(), d_row, sizeof(int) * (b->nb + 1), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(ci.data(), d_col, sizeof(int) * nnzb, hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("isai_batch_setup: reading the block pattern failed");
+            return fail(ISAI_ERR_HIP);
+        }
+        if (rp[0] != 0 || rp[b->nb] != nnzb) {
+            set_error("isai_batch_setup: row pointer spans [%d, %d), expected [0, %d)", rp[0], rp[b->nb], nnzb);
+            return fail(ISAI_ERR_INVALID_ARG);
+        }
+        for (int s = 0; s < nsys; ++s)
+            for (int i = b->sys_ptr[s]; i < b->sys_ptr[s + 1]; ++i)
+                for (int q = rp[i]; q < rp[i + 1]; ++q)
+                    if (ci[q] < b->sys_ptr[s] || ci[q] >= b->sys_ptr[s + 1]) {
+                        set_error("isai_batch_setup: block (%d,%d) couples system %d to another system", i, ci[q], s);
+                        return fail(ISAI_ERR_PATTERN);
+                    }
+    }
+    // pairs, padded segments, tiles
+    std::vector<int> sys_of_row(b->nb), tile_pair, ptile{0};
+    long long off = 0;
+    for (int s = 0; s < nsys; ++s) {
+        for (int i = b->sys_ptr[s]; i < b->sys_ptr[s + 1]; ++i) sys_of_row[i] = s;
+        const long long seg = 4LL * (b->sys_ptr[s + 1] - b->sys_ptr[s]);
+        const long long step = (seg + kTile - 1) / kTile * kTile;
+        b->sys_pair.push_back(b->npairs);
+        b->sys_vec.push_back(off);
+        b->sys_step.push_back(step);
+        for (int j = 0; j < b->nrhs[s]; ++j) {
+            for (long long t = 0; t < step / kTile; ++t) tile_pair.push_back(b->npairs);
+            ptile.push_back((int)tile_pair.size());
+            ++b->npairs;
+            off += step;
+        }
+    }
+    b->len = (size_t)off;
+    b->ntiles = (int)tile_pair.size();
+    auto up = [](void **d, const void *h, size_t bytes) -> bool {
+        if (hipMalloc(d, bytes) != hipSuccess) return false;
+        return hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    const bool ok = up((void **)&b->d_sys_of_row, sys_of_row.data(), sizeof(int) * b->nb) &&
+                    up((void **)&b->d_sys_ptr, b->sys_ptr.data(), sizeof(int) * (nsys + 1)) &&
+                    up((void **)&b->d_sys_nrhs, b->nrhs.data(), sizeof(int) * nsys) &&
+                    up((void **)&b->d_sys_pair, b->sys_pair.data(), sizeof(int) * nsys) &&
+                    up((void **)&b->d_sys_vec, b->sys_vec.data(), sizeof(long long) * nsys) &&
+                    up((void **)&b->d_sys_step, b->sys_step.data(), sizeof(long long) * nsys) &&
+                    up((void **)&b->d_tile_pair, tile_pair.data(), sizeof(int) * b->ntiles) &&
+                    up((void **)&b->d_ptile, ptile.data(), sizeof(int) * (b->npairs + 1)) &&
+                    hipMalloc(&b->d_ps, sizeof(PairState) * b->npairs) == hipSuccess &&
+                    hipMalloc(&b->d_part, sizeof(double) * 2 * b->ntiles) == hipSuccess &&
+                    hipMalloc(&b->d_relres, sizeof(double) * b->npairs) == hipSuccess &&
+                    hipMalloc(&b->d_ctl, sizeof(Ctl)) == hipSuccess &&
+                    hipHostMalloc(&b->h_ctl, sizeof(Ctl)) == hipSuccess &&
+                    hipMalloc(&b->work, sizeof(double) * b->len * isai_batch::kWork) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        set_error("isai_batch_setup: allocation failed (%d pairs, %zu doubles per multi-vector)", b->npairs, b->len);
+        return fail(ISAI_ERR_ALLOC);
+    }
+    if (hipMemsetAsync(b->work, 0, sizeof(double) * b->len * isai_batch::kWork, st) != hipSuccess ||
+        hipMemsetAsync(b->d_ps, 0, sizeof(PairState) * b->npairs, st) != hipSuccess ||
+        hipMemsetAsync(b->d_ctl, 0, sizeof(Ctl), st) != hipSuccess) {
+        set_error("isai_batch_setup: clearing the work vectors failed");
+        return fail(ISAI_ERR_HIP);
+    }
+    // preconditioner of the whole block-diagonal matrix: the ILU(0), the pair lists
+    // and the ISAI build each run as one launch over every system
+    isai_setup_opts_t opts{};
+    opts.precond = ISAI_PRECOND_ISAI;
+    opts.precision = precision;
+    int s = isai_setup(b->nb, nnzb, 4, d_row, d_col, d_val, &opts, &b->pre, stream);
+    if (s != ISAI_OK) return fail(s);
+    double total = 0;
+    s = isai_setup_times(b->pre, &b->ms_ilu, &b->ms_isai, &total);
+    if (s != ISAI_OK) return fail(s);
+    s = autotune(b, st);
+    if (s != ISAI_OK) return fail(s);
+    *out = b;
+    return ISAI_OK;
+}
+
+int isai_batch_vector_length(isai_batch_t b, size_t *length) {
+    ISAI_REQUIRE(b && length, "isai_batch_vector_length: null argument");
+    *length = b->len;
+    return ISAI_OK;
+}
+
+int isai_batch_vector_offset(isai_batch_t b, int sys, int rhs, size_t *offset) {
+    ISAI_REQUIRE(b && offset, "isai_batch_vector_offset: null argument");
+    ISAI_REQUIRE(sys >= 0 && sys < b->nsys && rhs >= 0 && rhs < b->nrhs[sys],
+                 "isai_batch_vector_offset: no pair (system %d, right-hand side %d)", sys, rhs);
+    *offset = (size_t)(b->sys_vec[sys] + rhs * b->sys_step[sys]);
+    return ISAI_OK;
+}
+
+int isai_batch_pairs(isai_batch_t b, int *npairs) {
+    ISAI_REQUIRE(b && npairs, "isai_batch_pairs: null argument");
+    *npairs = b->npairs;
+    return ISAI_OK;
+}
+
+int isai_batch_tuning(isai_batch_t b, int *variant, double *us, int *ncand) {
+    ISAI_REQUIRE(b, "isai_batch_tuning: null batch");
+    if (variant) *variant = b->variant;
+    if (us) *us = b->tuned_us;
+    if (ncand) *ncand = b->ncand;
+    return ISAI_OK;
+}
+
+int isai_batch_setup_times(isai_batch_t b, double *ms_ilu, double *ms_isai) {
+    ISAI_REQUIRE(b, "isai_batch_setup_times: null batch");
+    if (ms_ilu) *ms_ilu = b->ms_ilu;
+    if (ms_isai) *ms_isai = b->ms_isai;
+    return ISAI_OK;
+}
+
+int isai_batch_spmm(isai_batch_t b, const double *d_X, double *d_Y, isai_stream_t stream) {
+    ISAI_REQUIRE(b && d_X && d_Y, "isai_batch_spmm: null argument");
+    return spmm(b, kOpA, d_X, d_Y, 1, (hipStream_t)stream);
+}
+
+int isai_batch_solve(isai_batch_t b, double tol, int maxit, const double *d_B, double *d_X, int use_graph, int *iters,
+                     double *relres, int *max_iters, isai_stream_t stream) {
+    ISAI_REQUIRE(b && d_B && d_X && tol > 0.0 && maxit >= 0, "isai_batch_solve: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    ISAI_REQUIRE(!use_graph || st != nullptr, "isai_batch_solve: graph capture needs a created (non-null) stream");
+    const size_t n = b->len;
+    double *r = b->work, *rh = r + n, *p = rh + n, *v = p + n;
+    const BatchDev B = b->dev();
+    const int np = b->npairs, pg = (np + 255) / 256, T = b->ntiles;
+    if (use_graph && (b->g_x != d_X || !b->gexec)) BTRY(capture_graph(b, d_X, st));
+    b->h_ctl->maxit = maxit;
+    b->h_ctl->nactive = 0;
+    ISAI_HIP(hipMemcpyAsync(b->d_ctl, b->h_ctl, sizeof(Ctl), hipMemcpyHostToDevice, st));
+    breset_kernel<<<pg, 256, 0, st>>>(B, np);
+    ISAI_LAUNCH_CHECK();
+    const int max_replays = maxit / kItersPerReplay + 2;
+    std::vector<PairState> hs(np);
+    for (int round = 0; round <= kMaxRestarts; ++round) {
+        // (re)start the running pairs from their current x
+        BTRY(spmm(b, kOpA, d_X, r, 0, st));
+        binit_kernel<<<T, kRed, 0, st>>>(d_B, r, rh, p, v, B);
+        binit_final<<<pg, 256, 0, st>>>(B, np, tol);
+        ISAI_LAUNCH_CHECK();
+        for (int rep = 0; rep < max_replays; ++rep) {
+            if (use_graph) {
+                ISAI_HIP(hipGraphLaunch(b->gexec, st));
+            } else {
+                for (int k = 0; k < kItersPerReplay; ++k) BTRY(batch_iteration(b, d_X, st));
+            }
+            ISAI_HIP(hipMemcpyAsync(&b->h_ctl->nactive, &b->d_ctl->nactive, sizeof(int), hipMemcpyDeviceToHost, st));
+            ISAI_HIP(hipStreamSynchronize(st));
+            if (b->h_ctl->nactive == 0) break;
+        }
+        // true FP64 residual of every pair; pairs that miss tol restart
+        BTRY(spmm(b, kOpA, d_X, r, 1, st));
+        bres_kernel<<<T, kRed, 0, st>>>(d_B, r, B);
+        bres_final<<<pg, 256, 0, st>>>(B, np, tol, b->d_relres);
+        ISAI_LAUNCH_CHECK();
+        ISAI_HIP(hipMemcpyAsync(hs.data(), b->d_ps, sizeof(PairState) * np, hipMemcpyDeviceToHost, st));
+        ISAI_HIP(hipStreamSynchronize(st));
+        bool any = false;
+        for (const PairState &q : hs) any = any || q.status == 0;
+        if (!any) break;
+    }
+    std::vector<double> rr(np);
+    ISAI_HIP(hipMemcpy(rr.data(), b->d_relres, sizeof(double) * np, hipMemcpyDeviceToHost));
+    int mx = 0;
+    bool all = true;
+    for (int q = 0; q < np; ++q) {
+        if (iters) iters[q] = hs[q].iters;
+        if (relres) relres[q] = rr[q];
+        mx = std::max(mx, hs[q].iters);
+        all = all && rr[q] <= tol;
+    }
+    if (max_iters) *max_iters = mx;
+    if (!all) {
+        set_error("isai_batch_solve: some pairs did not reach %.1e within %d iterations", tol, maxit);
+        return ISAI_ERR_NOT_CONVERGED;
+    }
+    return ISAI_OK;
+}
+
+}  // extern "C"

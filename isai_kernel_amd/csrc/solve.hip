@@ -550,6 +550,26 @@ int fgmres(isai_handle *h, int m, double tol, int maxit, const double *b, double
 
 }  // namespace
 
+namespace isai {
+
+int handle_factors_internal(isai_handle_t h, HandleFactors *o) {
+    ISAI_REQUIRE(h && o, "handle_factors_internal: null argument");
+    o->nb = h->nb;
+    o->nnzb = h->nnzb;
+    o->nnzbL = h->nnzbL;
+    o->nnzbU = h->nnzbU;
+    o->Lrow = h->d_Lrow;
+    o->Lcol = h->d_Lcol;
+    o->Urow = h->d_Urow;
+    o->Ucol = h->d_Ucol;
+    o->precision = h->precision;
+    o->Lval = h->precision == ISAI_PREC_FP64 ? (const void *)h->d_Lval : h->d_Llp;
+    o->Uval = h->precision == ISAI_PREC_FP64 ? (const void *)h->d_Uval : h->d_Ulp;
+    return ISAI_OK;
+}
+
+}  // namespace isai
+
 extern "C" {
 
 int isai_destroy(isai_handle_t h) {

@@ -31,6 +31,37 @@ def generate(problem: int, L: int = 0, seed: int = 0):
     return rp, ci, val
 
 
+def batch(nsys: int, lmin: int, lmax: int, seed: int = 0, rhs_max: int = 16, problem: int = 3):
+    """Block-diagonal BSR of nsys problem-`problem` systems (2-D 9-pt by default)
+    with grid side L_s uniform in [lmin, lmax] and 1..rhs_max right-hand sides each.
+
+    L_s and the right-hand-side counts come from numpy's default_rng(seed); system s
+    is generated with seed + s.  Returns (rp, ci, val, sys_ptr, Ls, nrhs)."""
+    rng = np.random.default_rng(seed)
+    Ls = rng.integers(lmin, lmax + 1, nsys)
+    nrhs = rng.integers(1, rhs_max + 1, nsys).astype(np.int32)
+    counts = []
+    for L in Ls:
+        nb, nnzb = C.c_int(), C.c_int()
+        call("isai_gen_problem_count", problem, int(L), C.byref(nb), C.byref(nnzb))
+        counts.append((nb.value, nnzb.value))
+    sys_ptr = np.zeros(nsys + 1, np.int32)
+    sys_ptr[1:] = np.cumsum([c[0] for c in counts])
+    blk_ptr = np.zeros(nsys + 1, np.int64)
+    blk_ptr[1:] = np.cumsum([c[1] for c in counts])
+    rp = np.empty(sys_ptr[-1] + 1, np.int32)
+    ci = np.empty(blk_ptr[-1], np.int32)
+    val = np.empty((blk_ptr[-1], BS, BS), np.float64)
+    for s, L in enumerate(Ls):
+        r0, r1, k0, k1 = sys_ptr[s], sys_ptr[s + 1], blk_ptr[s], blk_ptr[s + 1]
+        srp, sci, sval = generate(problem, int(L), seed + s)
+        rp[r0:r1] = srp[:-1] + k0
+        ci[k0:k1] = sci + r0
+        val[k0:k1] = sval
+    rp[-1] = blk_ptr[-1]
+    return rp, ci, val, sys_ptr, Ls, nrhs
+
+
 def normal(seed: int, n: int) -> np.ndarray:
     out = np.empty(n, np.float64)
     call("isai_gen_normal", seed, n, ptr(out))

@@ -80,3 +80,78 @@ class Handle:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """isai_batch.h: many block-diagonal systems with several right-hand sides each."""
+
+    def __init__(self, A: Matrix, sys_ptr: np.ndarray, nrhs: np.ndarray, precision: str = "fp32", stream=None):
+        self.A = A
+        self.stream = stream
+        self.sys_ptr = np.ascontiguousarray(sys_ptr, np.int32)
+        self.nrhs = np.ascontiguousarray(nrhs, np.int32)
+        h = C.c_void_p()
+        call("isai_batch_setup", len(self.nrhs), self.sys_ptr.ctypes.data, self.nrhs.ctypes.data, A.nnzb,
+             A.d_rp.ptr, A.d_ci.ptr, A.d_val.ptr, PRECISION[precision], C.byref(h), stream)
+        self.h = h
+        n, p = C.c_size_t(), C.c_int()
+        call("isai_batch_vector_length", self.h, C.byref(n))
+        call("isai_batch_pairs", self.h, C.byref(p))
+        self.length, self.npairs = n.value, p.value
+
+    def offset(self, sys: int, rhs: int) -> int:
+        o = C.c_size_t()
+        call("isai_batch_vector_offset", self.h, sys, rhs, C.byref(o))
+        return o.value
+
+    def pack(self, vectors) -> np.ndarray:
+        """vectors[s][j] (length 4*nb_s) -> one zero-padded multi-vector."""
+        out = np.zeros(self.length)
+        for s, vs in enumerate(vectors):
+            for j, v in enumerate(vs):
+                o = self.offset(s, j)
+                out[o:o + len(v)] = v
+        return out
+
+    def unpack(self, multi: np.ndarray):
+        out = []
+        for s in range(len(self.nrhs)):
+            n = 4 * int(self.sys_ptr[s + 1] - self.sys_ptr[s])
+            out.append([multi[self.offset(s, j):self.offset(s, j) + n].copy() for j in range(self.nrhs[s])])
+        return out
+
+    def tuning(self):
+        v, us, nc = C.c_int(), C.c_double(), C.c_int()
+        call("isai_batch_tuning", self.h, C.byref(v), C.byref(us), C.byref(nc))
+        return v.value, us.value, nc.value
+
+    def setup_times(self):
+        a, b = C.c_double(), C.c_double()
+        call("isai_batch_setup_times", self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def spmm(self, d_X: DeviceArray, d_Y: DeviceArray) -> None:
+        call("isai_batch_spmm", self.h, d_X.ptr, d_Y.ptr, self.stream)
+
+    def solve(self, d_B: DeviceArray, d_X: DeviceArray, tol: float = 1e-10, maxit: int = 1000,
+              use_graph: bool = True, check: bool = True):
+        it = np.zeros(self.npairs, np.int32)
+        rr = np.zeros(self.npairs)
+        mx = C.c_int()
+        st = lib.isai_batch_solve(self.h, tol, maxit, d_B.ptr, d_X.ptr, int(use_graph), it.ctypes.data,
+                                  rr.ctypes.data, C.byref(mx), self.stream)
+        if check and st != 0:
+            from ._ffi import check as _check
+            _check(st, "isai_batch_solve")
+        return st, it, rr
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib.isai_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

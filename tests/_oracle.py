@@ -259,6 +259,22 @@ def fgmres(rp, ci, val, P_, b, m=30, x0=None, tol=1e-10, maxit=1000):
     return x, st, it.value, rr.value
 
 
+def isai_factors(rp, ci, val):
+    """The oracle's whole setup path: packed ILU(0), then ISAI(L) and ISAI(U) in BCSC
+    reordered to BCSR.  Returns (lu, (rpL, ciL, valL), (rpU, ciU, valU))."""
+    lu, _, _ = bsrilu0(rp, ci, val)
+    mbL, mcL = bcsc_pattern(rp, ci, True)
+    mbU, mcU = bcsc_pattern(rp, ci, False)
+    pp, pl, ps = pairs(rp, ci, mbL, mcL)
+    ML = isai_lower(lu, mbL, mcL, pp, pl, ps, build_dinv(rp, ci, lu, True))
+    MU = isai_upper(rp, ci, lu, mbU, mcU, build_dinv(rp, ci, lu, False))
+    trpL, tciL = tri_bcsr_pattern(rp, ci, True)
+    trpU, tciU = tri_bcsr_pattern(rp, ci, False)
+    _, mapL = map_bcsc2bcsr(mbL, mcL, trpL, tciL)
+    _, mapU = map_bcsc2bcsr(mbU, mcU, trpU, tciU)
+    return lu, (trpL, tciL, reorder(mapL, ML)), (trpU, tciU, reorder(mapU, MU))
+
+
 def precond_apply(P_, v):
     z = np.empty(P_.nb * P_.bs)
     ora.ora_precond_apply(C.byref(P_), _p(_f64(v)), _p(z))

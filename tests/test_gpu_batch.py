@@ -1,0 +1,154 @@
+"""GPU parity of the batching form (isai_batch.h): block-diagonal systems with
+1..16 right-hand sides each, batched setup, batched BiCGSTAB replayed as a graph."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from gpu_pipeline import rel_err
+from isai_kernel_amd import DeviceArray, IsaiError, call, lib, problems
+from isai_kernel_amd.solver import Batch, Matrix
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def stream(gpu):
+    s = ctypes.c_void_p()
+    call("isai_stream_create", ctypes.byref(s))
+    yield s
+    call("isai_stream_destroy", s)
+
+
+def _system(rp, ci, val, sys_ptr, s):
+    r0, r1 = int(sys_ptr[s]), int(sys_ptr[s + 1])
+    k0, k1 = int(rp[r0]), int(rp[r1])
+    return (rp[r0:r1 + 1] - k0).astype(np.int32), (ci[k0:k1] - r0).astype(np.int32), val[k0:k1]
+
+
+def _rhs(sys_ptr, nrhs, seed):
+    rng = np.random.default_rng(seed)
+    return [[rng.standard_normal(4 * int(sys_ptr[s + 1] - sys_ptr[s])) for _ in range(nrhs[s])]
+            for s in range(len(nrhs))]
+
+
+def _make(nsys, lmin, lmax, seed, rhs_max, precision, stream):
+    rp, ci, val, sys_ptr, Ls, nrhs = problems.batch(nsys, lmin, lmax, seed=seed, rhs_max=rhs_max)
+    A = Matrix(rp, ci, val)
+    return (rp, ci, val, sys_ptr, nrhs), Batch(A, sys_ptr, nrhs, precision=precision, stream=stream)
+
+
+def test_batch_layout_and_spmm(stream):
+    print("batch spmm 7 systems L 4..13", flush=True)
+    (rp, ci, val, sys_ptr, nrhs), bt = _make(7, 4, 13, 5, 16, "fp64", stream)
+    try:
+        assert bt.npairs == int(nrhs.sum())
+        # segments: 1024-aligned, in (system, rhs) order, non-overlapping
+        offs = [bt.offset(s, j) for s in range(len(nrhs)) for j in range(nrhs[s])]
+        assert all(o % 1024 == 0 for o in offs) and offs == sorted(offs) and bt.length % 1024 == 0
+        X = _rhs(sys_ptr, nrhs, 1)
+        d_X = DeviceArray.from_numpy(bt.pack(X))
+        d_Y = DeviceArray.from_numpy(np.full(bt.length, 0.0))
+        bt.spmm(d_X, d_Y)
+        call("isai_stream_synchronize", stream)
+        Y = d_Y.to_numpy()
+        got = bt.unpack(Y)
+        for s in range(len(nrhs)):
+            srp, sci, sval = _system(rp, ci, val, sys_ptr, s)
+            for j in range(nrhs[s]):
+                assert rel_err(got[s][j], O.spmv(srp, sci, sval, X[s][j])) <= 1e-14, (s, j)
+        # padding untouched
+        mask = np.ones(bt.length, bool)
+        for s in range(len(nrhs)):
+            for j in range(nrhs[s]):
+                o = bt.offset(s, j)
+                mask[o:o + 4 * int(sys_ptr[s + 1] - sys_ptr[s])] = False
+        assert not Y[mask].any()
+        v, us, nc = bt.tuning()
+        assert nc == 16 and 0 <= v < 8 and us > 0
+    finally:
+        bt.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
+def test_batch_solve_matches_oracle(stream, precision):
+    """Every pair reaches 1e-10 true FP64 residual; per-pair iteration counts and
+    solutions agree with the oracle's single-system BiCGSTAB on the same system."""
+    print(f"batch solve 6 systems L 5..16 precision={precision}", flush=True)
+    (rp, ci, val, sys_ptr, nrhs), bt = _make(6, 5, 16, 11, 4, precision, stream)
+    try:
+        B = _rhs(sys_ptr, nrhs, 2)
+        d_B = DeviceArray.from_numpy(bt.pack(B))
+        d_X = DeviceArray.zeros((bt.length,), np.float64)
+        st, iters, rr = bt.solve(d_B, d_X, tol=1e-10, maxit=500)
+        assert st == 0 and (rr <= 1e-10).all()
+        X = bt.unpack(d_X.to_numpy())
+        q = 0
+        for s in range(len(nrhs)):
+            srp, sci, sval = _system(rp, ci, val, sys_ptr, s)
+            lu, L, U = O.isai_factors(srp, sci, sval)
+            P = O.make_precond(1, len(srp) - 1, 4, L=L, U=U, f32=precision == "fp32", bf16=precision == "bf16")
+            for j in range(nrhs[s]):
+                assert O.rel_residual(srp, sci, sval, B[s][j], X[s][j]) <= 1e-10, (s, j)
+                xo, sto, ito, _ = O.bicgstab(srp, sci, sval, P, B[s][j], tol=1e-10, maxit=500)
+                assert sto == 0
+                assert abs(int(iters[q]) - ito) <= 2, (s, j, iters[q], ito)
+                assert rel_err(X[s][j], xo) <= 1e-8, (s, j)
+                q += 1
+    finally:
+        bt.close()
+
+
+def test_batch_graph_replay_equals_eager(stream):
+    """The captured graph (replayed with host reads and synchronisations between
+    replays) gives bit-identical iterates to the same kernels launched eagerly."""
+    print("batch graph vs eager 9 systems L 6..24", flush=True)
+    (rp, ci, val, sys_ptr, nrhs), bt = _make(9, 6, 24, 3, 16, "fp32", stream)
+    try:
+        B = DeviceArray.from_numpy(bt.pack(_rhs(sys_ptr, nrhs, 9)))
+        xs, its = [], []
+        for use_graph in (True, False, True):
+            d_X = DeviceArray.zeros((bt.length,), np.float64)
+            st, iters, rr = bt.solve(B, d_X, tol=1e-10, maxit=500, use_graph=use_graph)
+            assert st == 0 and (rr <= 1e-10).all()
+            xs.append(d_X.to_numpy())
+            its.append(iters)
+        assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
+        assert np.array_equal(its[0], its[1]) and np.array_equal(its[0], its[2])
+        # restarting from the solution: every pair stops before its first iteration
+        d_X = DeviceArray.from_numpy(xs[0])
+        st, iters, rr = bt.solve(B, d_X, tol=1e-10, maxit=500)
+        assert st == 0 and not iters.any() and np.array_equal(d_X.to_numpy(), xs[0])
+    finally:
+        bt.close()
+
+
+def test_batch_iteration_limit_and_errors(stream):
+    print("batch limits and argument errors", flush=True)
+    (rp, ci, val, sys_ptr, nrhs), bt = _make(4, 8, 14, 4, 3, "fp64", stream)
+    try:
+        d_B = DeviceArray.from_numpy(bt.pack(_rhs(sys_ptr, nrhs, 3)))
+        d_X = DeviceArray.zeros((bt.length,), np.float64)
+        st, iters, rr = bt.solve(d_B, d_X, tol=1e-10, maxit=3, check=False)
+        assert st == -6 and iters.max() <= 3 and (rr > 1e-10).any()
+        with pytest.raises(IsaiError):
+            bt.offset(0, int(nrhs[0]))
+    finally:
+        bt.close()
+    A = Matrix(rp, ci, val)
+    with pytest.raises(IsaiError):  # 17 right-hand sides
+        Batch(A, sys_ptr, np.full(len(nrhs), 17), stream=stream)
+    # a block that couples two systems
+    merged = np.array([0, sys_ptr[2], sys_ptr[4]], np.int32)
+    bad = np.array([0, sys_ptr[1], sys_ptr[4]], np.int32)
+    Batch(A, merged, np.array([1, 2]), stream=stream).close()  # merging systems is still block-diagonal
+    call("isai_device_synchronize")
+    ci2 = ci.copy()
+    ci2[int(rp[sys_ptr[1]]) - 1] = sys_ptr[1]  # last block of system 0 now points into system 1
+    A2 = Matrix(rp, ci2, val)
+    h = ctypes.c_void_p()
+    nr = np.ones(2, np.int32)
+    st = lib.isai_batch_setup(2, bad.ctypes.data, nr.ctypes.data, len(ci2), A2.d_rp.ptr, A2.d_ci.ptr,
+                              A2.d_val.ptr, 0, ctypes.byref(h), stream)
+    assert st == -4 and not h.value

@@ -12,20 +12,6 @@ from isai_kernel_amd.solver import Handle, Matrix
 pytestmark = pytest.mark.gpu
 
 
-def _oracle_factors(rp, ci, val):
-    lu, st, _ = O.bsrilu0(rp, ci, val)
-    mbL, mcL = O.bcsc_pattern(rp, ci, True)
-    mbU, mcU = O.bcsc_pattern(rp, ci, False)
-    pp, pl, ps = O.pairs(rp, ci, mbL, mcL)
-    ML = O.isai_lower(lu, mbL, mcL, pp, pl, ps, O.build_dinv(rp, ci, lu, True))
-    MU = O.isai_upper(rp, ci, lu, mbU, mcU, O.build_dinv(rp, ci, lu, False))
-    trpL, tciL = O.tri_bcsr_pattern(rp, ci, True)
-    trpU, tciU = O.tri_bcsr_pattern(rp, ci, False)
-    _, mapL = O.map_bcsc2bcsr(mbL, mcL, trpL, tciL)
-    _, mapU = O.map_bcsc2bcsr(mbU, mcU, trpU, tciU)
-    return lu, (trpL, tciL, O.reorder(mapL, ML)), (trpU, tciU, O.reorder(mapU, MU))
-
-
 def _download(ptr, shape, dtype):
     out = np.empty(shape, dtype)
     call("isai_memcpy_dtoh", out.ctypes.data, ptr, out.nbytes)
@@ -37,7 +23,7 @@ def test_setup_factors_bitexact_and_apply(gpu):
     rp, ci, val = problems.generate(0, 20, seed=1)
     A = Matrix(rp, ci, val)
     h = Handle(A)
-    lu, L, U = _oracle_factors(rp, ci, val)
+    lu, L, U = O.isai_factors(rp, ci, val)
     Lrow, Lcol, Lval, Urow, Ucol, Uval, LU = h.factors()
     nnzbL, nnzbU, nlev = h.info()
     assert nnzbL == len(L[1]) and nnzbU == len(U[1]) and nlev == O.levels(rp, ci, True)[1]
@@ -80,7 +66,7 @@ def test_bicgstab_plain_small(gpu, precond):
     assert O.rel_residual(rp, ci, val, b, x) <= 1e-10  # independent FP64 check
     assert np.max(np.abs(x - 1.0)) <= 1e-7
     # iteration count within a couple of the oracle's textbook BiCGSTAB
-    lu, L, U = _oracle_factors(rp, ci, val)
+    lu, L, U = O.isai_factors(rp, ci, val)
     P = O.make_precond({"isai": 1, "ilu": 2, "none": 0}[precond], A.nb, 4, L=L, U=U, LU=(rp, ci, lu))
     _, ost, oit, orr = O.bicgstab(rp, ci, val, P, b, tol=1e-10, maxit=2000)
     assert ost == 0 and abs(it - oit) <= 2, (it, oit)
@@ -101,7 +87,7 @@ def test_fgmres_mixed_precision(gpu, precision):
     x = d_x.to_numpy()
     assert st == 0 and rr <= 1e-10
     assert O.rel_residual(rp, ci, val, b, x) <= 1.5e-10
-    lu, L, U = _oracle_factors(rp, ci, val)
+    lu, L, U = O.isai_factors(rp, ci, val)
     P = O.make_precond(1, A.nb, 4, L=L, U=U, f32=precision == "fp32", bf16=precision == "bf16")
     _, ost, oit, orr = O.fgmres(rp, ci, val, P, b, m=30, tol=1e-10, maxit=3000)
     assert ost == 0 and abs(it - oit) <= max(3, oit // 10), (it, oit)
