@@ -25,6 +25,7 @@ constexpr int kTile = 1024;
 constexpr int kMaxRhs = 16;
 constexpr int kItersPerReplay = 4;  // iterations in one captured graph
 constexpr int kMaxRestarts = 8;
+constexpr int kSpmmMaxWg = 256;  // largest SpMM workgroup the autotuner tries
 
 // status: 0 running, 1 converged (recursive residual), 2 converged on s with the
 // x update pending in this iteration, 3 breakdown, 4 done (true residual met),
@@ -62,44 +63,64 @@ __device__ __forceinline__ unsigned active_mask(const BatchDev &B, int p0, int r
 // --------------------------------------------------------------- multi-RHS SpMM
 // Eight lanes per block row (one 128-byte line of FP64 block values per
 // instruction, as in the single-system product); every block is loaded once and
-// applied to all right-hand sides of its system that are still running.
-// FP64: lane l holds row l/2, columns 2(l%2)..+1; FMA chain per lane, then the
-// two lanes of a row are added.
-__global__ void bspmm_f64_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
-                                 const v2d *__restrict__ val, BatchDev B, const double *__restrict__ X,
-                                 double *__restrict__ Y, int remap, int all) {
+// applied to all R right-hand sides of its system.  R is a template parameter so
+// the R vector loads of a block are issued back to back before the FMAs; pairs
+// that have stopped are skipped by the caller only when the whole row is idle,
+// and their outputs are never written.
+struct RowCtx {
+    int br, base, rs, re, lane;
+    unsigned mask;
+    long long v0, step;
+};
+
+__device__ __forceinline__ bool row_context(const BatchDev &B, const int *rp, int nb, int remap, int all, RowCtx &q,
+                                            int &r) {
     const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    const int br = wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3);
-    if (br >= nb) return;  // whole 8-lane groups leave together
-    const int sys = B.sys_of_row[br];
-    const unsigned mask = active_mask(B, B.sys_pair[sys], B.sys_nrhs[sys], all);
-    if (!mask) return;
-    const int base = B.sys_ptr[sys];
-    const long long v0 = B.sys_vec[sys], step = B.sys_step[sys];
-    const int lane = threadIdx.x & 7, h = lane & 1;
-    double acc0[kMaxRhs], acc1[kMaxRhs];
+    q.br = wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3);
+    if (q.br >= nb) return false;  // whole 8-lane groups leave together
+    const int sys = B.sys_of_row[q.br];
+    r = B.sys_nrhs[sys];
+    q.mask = active_mask(B, B.sys_pair[sys], r, all);
+    if (!q.mask) return false;
+    q.base = B.sys_ptr[sys];
+    q.v0 = B.sys_vec[sys];
+    q.step = B.sys_step[sys];
+    q.rs = rp[q.br];
+    q.re = rp[q.br + 1];
+    q.lane = threadIdx.x & 7;
+    return true;
+}
+
+// FP64 values: lane l holds row l/2, columns 2(l%2)..+1; FMA chain per lane, then
+// the two lanes of a row are added.
+template <int R>
+__device__ __forceinline__ void f64_row(const RowCtx &q, const int *__restrict__ ci, const v2d *__restrict__ val,
+                                        const double *__restrict__ X, double *__restrict__ Y) {
+    constexpr int KU = R <= 2 ? 4 : (R <= 6 ? 2 : 1);
+    const int h = q.lane & 1;
+    double acc0[R], acc1[R];
 #pragma unroll
-    for (int j = 0; j < kMaxRhs; ++j) acc0[j] = acc1[j] = 0.0;
-    const int rs = rp[br], re = rp[br + 1];
-    for (int k = rs; k < re; ++k) {
-        const int c = ci[k] - base;
-        const v2d a = __builtin_nontemporal_load(&val[(size_t)k * 8 + lane]);
-        const double *xc = X + v0 + (size_t)c * 4 + 2 * h;
+    for (int j = 0; j < R; ++j) acc0[j] = acc1[j] = 0.0;
+    const double *xb = X + q.v0 + 2 * h;
+#pragma unroll KU
+    for (int k = q.rs; k < q.re; ++k) {
+        const int c = ci[k] - q.base;
+        const v2d a = __builtin_nontemporal_load(&val[(size_t)k * 8 + q.lane]);
+        v2d x[R];
 #pragma unroll
-        for (int j = 0; j < kMaxRhs; ++j) {
-            if (mask & (1u << j)) {
-                const v2d x = *(const v2d *)(xc + j * step);
-                acc0[j] = fma(a.x, x.x, acc0[j]);
-                acc1[j] = fma(a.y, x.y, acc1[j]);
-            }
+        for (int j = 0; j < R; ++j) x[j] = *(const v2d *)(xb + (size_t)c * 4 + j * q.step);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            acc0[j] = fma(a.x, x[j].x, acc0[j]);
+            acc1[j] = fma(a.y, x[j].y, acc1[j]);
         }
     }
-    double *yr = Y + v0 + (size_t)(br - base) * 4 + (lane >> 1);
+    double *yr = Y + q.v0 + (size_t)(q.br - q.base) * 4 + (q.lane >> 1);
 #pragma unroll
-    for (int j = 0; j < kMaxRhs; ++j) {
+    for (int j = 0; j < R; ++j) {
         double s = acc0[j] + acc1[j];
         s += __shfl_xor(s, 1);
-        if ((mask & (1u << j)) && h == 0) yr[j * step] = s;
+        if ((q.mask & (1u << j)) && h == 0) yr[j * q.step] = s;
     }
 }
 
@@ -109,37 +130,33 @@ __global__ void bspmm_f64_kernel(int nb, const int *__restrict__ rp, const int *
 __device__ __forceinline__ float b16_lo(unsigned int w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float b16_hi(unsigned int w) { return __uint_as_float(w & 0xffff0000u); }
 
-template <bool BF16>
-__global__ void bspmm_lowp_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
-                                  const v4u *__restrict__ val16, BatchDev B, const double *__restrict__ X,
-                                  double *__restrict__ Y, int remap, int all) {
+template <bool BF16, int R>
+__device__ __forceinline__ void lowp_row(const RowCtx &q, const int *__restrict__ ci, const v4u *__restrict__ val16,
+                                         const double *__restrict__ X, double *__restrict__ Y) {
     constexpr int BPB = BF16 ? 4 : 2;
     constexpr int LPB = 8 / BPB;
-    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    const int br = wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3);
-    if (br >= nb) return;
-    const int sys = B.sys_of_row[br];
-    const unsigned mask = active_mask(B, B.sys_pair[sys], B.sys_nrhs[sys], all);
-    if (!mask) return;
-    const int base = B.sys_ptr[sys];
-    const long long v0 = B.sys_vec[sys], step = B.sys_step[sys];
-    const int lane = threadIdx.x & 7, b = lane / LPB, sub = lane % LPB;
-    float acc0[kMaxRhs], acc1[kMaxRhs];
+    constexpr int KU = R <= 4 ? 2 : 1;
+    const int b = q.lane / LPB, sub = q.lane % LPB;
+    float acc0[R], acc1[R];
 #pragma unroll
-    for (int j = 0; j < kMaxRhs; ++j) acc0[j] = acc1[j] = 0.0f;
-    const int rs = rp[br], re = rp[br + 1];
-    for (int k = rs; k < re; k += BPB) {
+    for (int j = 0; j < R; ++j) acc0[j] = acc1[j] = 0.0f;
+#pragma unroll KU
+    for (int k = q.rs; k < q.re; k += BPB) {
         const int kb = k + b;
-        if (kb >= re) continue;
-        const int c = ci[kb] - base;
-        const v4u w = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kb * LPB + sub]) : val16[(size_t)kb * LPB + sub];
-        const double *xc = X + v0 + (size_t)c * 4;
+        if (kb < q.re) {
+            const int c = ci[kb] - q.base;
+            const v4u w = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kb * LPB + sub])
+                               : val16[(size_t)kb * LPB + sub];
+            const double *xc = X + q.v0 + (size_t)c * 4;
+            v2d x01[R], x23[R];
 #pragma unroll
-        for (int j = 0; j < kMaxRhs; ++j) {
-            if (mask & (1u << j)) {
-                const v2d x01 = *(const v2d *)(xc + j * step);
-                const v2d x23 = *(const v2d *)(xc + j * step + 2);
-                const float f0 = (float)x01.x, f1 = (float)x01.y, f2 = (float)x23.x, f3 = (float)x23.y;
+            for (int j = 0; j < R; ++j) {
+                x01[j] = *(const v2d *)(xc + j * q.step);
+                x23[j] = *(const v2d *)(xc + j * q.step + 2);
+            }
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const float f0 = (float)x01[j].x, f1 = (float)x01[j].y, f2 = (float)x23[j].x, f3 = (float)x23[j].y;
                 if (BF16) {
                     acc0[j] = fmaf(b16_lo(w.x), f0, acc0[j]);
                     acc0[j] = fmaf(b16_hi(w.x), f1, acc0[j]);
@@ -159,7 +176,7 @@ __global__ void bspmm_lowp_kernel(int nb, const int *__restrict__ rp, const int 
         }
     }
 #pragma unroll
-    for (int j = 0; j < kMaxRhs; ++j) {
+    for (int j = 0; j < R; ++j) {
 #pragma unroll
         for (int off = LPB; off < 8; off <<= 1) {
             acc0[j] += __shfl_xor(acc0[j], off);
@@ -167,16 +184,57 @@ __global__ void bspmm_lowp_kernel(int nb, const int *__restrict__ rp, const int 
         }
     }
     if (b != 0) return;
-    double *yr = Y + v0 + (size_t)(br - base) * 4;
+    double *yr = Y + q.v0 + (size_t)(q.br - q.base) * 4;
 #pragma unroll
-    for (int j = 0; j < kMaxRhs; ++j) {
-        if (!(mask & (1u << j))) continue;
+    for (int j = 0; j < R; ++j) {
+        if (!(q.mask & (1u << j))) continue;
         if (BF16) {
-            yr[j * step + 2 * sub] = (double)acc0[j];
-            yr[j * step + 2 * sub + 1] = (double)acc1[j];
+            yr[j * q.step + 2 * sub] = (double)acc0[j];
+            yr[j * q.step + 2 * sub + 1] = (double)acc1[j];
         } else {
-            yr[j * step + sub] = (double)acc0[j];
+            yr[j * q.step + sub] = (double)acc0[j];
         }
+    }
+}
+
+#define ISAI_RHS_CASES(CALL) \
+    CALL(1) CALL(2) CALL(3) CALL(4) CALL(5) CALL(6) CALL(7) CALL(8) \
+    CALL(9) CALL(10) CALL(11) CALL(12) CALL(13) CALL(14) CALL(15) CALL(16)
+
+__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+                                 const v2d *__restrict__ val, BatchDev B, const double *__restrict__ X,
+                                 double *__restrict__ Y, int remap, int all) {
+    RowCtx q;
+    int r;
+    if (!row_context(B, rp, nb, remap, all, q, r)) return;
+    switch (r) {
+#define ISAI_F64_CASE(N) \
+    case N:              \
+        f64_row<N>(q, ci, val, X, Y); \
+        break;
+        ISAI_RHS_CASES(ISAI_F64_CASE)
+#undef ISAI_F64_CASE
+        default:
+            break;
+    }
+}
+
+template <bool BF16>
+__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_lowp_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+                                  const v4u *__restrict__ val16, BatchDev B, const double *__restrict__ X,
+                                  double *__restrict__ Y, int remap, int all) {
+    RowCtx q;
+    int r;
+    if (!row_context(B, rp, nb, remap, all, q, r)) return;
+    switch (r) {
+#define ISAI_LOWP_CASE(N) \
+    case N:               \
+        lowp_row<BF16, N>(q, ci, val16, X, Y); \
+        break;
+        ISAI_RHS_CASES(ISAI_LOWP_CASE)
+#undef ISAI_LOWP_CASE
+        default:
+            break;
     }
 }
 
@@ -560,8 +618,8 @@ int batch_iteration(isai_batch *b, double *x, hipStream_t st) {
 // Launch-shape autotuner: for the product with A (FP64) and with the ISAI factors
 // (storage precision) time every (workgroup size, XCD remap) candidate on the batch
 // itself, all pairs active, and keep the fastest of each.
-constexpr int kWgCand[4] = {64, 128, 256, 512};
-constexpr int kCandPerOp = 8;
+constexpr int kWgCand[3] = {64, 128, kSpmmMaxWg};
+constexpr int kCandPerOp = 6;
 
 int autotune(isai_batch *b, hipStream_t st) {
     double *X = b->work, *Y = b->work + b->len;

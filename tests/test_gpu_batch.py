@@ -66,7 +66,7 @@ def test_batch_layout_and_spmm(stream):
                 mask[o:o + 4 * int(sys_ptr[s + 1] - sys_ptr[s])] = False
         assert not Y[mask].any()
         v, us, nc = bt.tuning()
-        assert nc == 16 and 0 <= v < 8 and us > 0
+        assert nc == 12 and 0 <= v < 6 and us > 0
     finally:
         bt.close()
 
