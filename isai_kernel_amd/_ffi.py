@@ -19,6 +19,16 @@ if not LIB_PATH.exists():
         f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
     )
 
+# One HIP runtime per process.  PyTorch-ROCm ships its own libamdhip64 (soname
+# libamdhip64.so.7, loaded by the name libamdhip64.so from torch/lib); if this
+# library were loaded first, its dependency would pull /opt/rocm's copy and torch
+# would then load a second runtime that finds no device.  Loading torch first makes
+# the dynamic linker resolve libamdhip64.so.7 to the copy already in the process.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 lib = C.CDLL(str(LIB_PATH))
 
 P = C.c_void_p
@@ -108,6 +118,10 @@ SIGNATURES: dict[str, list] = {
     "isai_precond_apply": [P, P, P, P],
     "isai_solve": [P, I, I, D, I, P, P, P, P, P],
     "isai_destroy": [P],
+    # isai_adjoint.h
+    "isai_bsr_transpose": [I, I, I, P, P, P, P, P, P, P],
+    "isai_adjoint_handle": [P, P, P],
+    "isai_bsr_outer_grad": [I, I, I, P, P, P, P, D, P, P],
     # isai_batch.h
     "isai_batch_setup": [I, P, P, I, P, P, P, I, P, P],
     "isai_batch_vector_length": [P, P],

@@ -28,4 +28,13 @@ struct HandleFactors {
 // Factors of a setup handle (isai_handle_t) in its current storage precision.
 int handle_factors_internal(isai_handle_t h, HandleFactors *out);
 
+// Transposes (adjoint.hip).  transpose_pattern_host: pattern of A^T and perm[d] =
+// block of A stored at position d of A^T.  transpose_blocks: dst[d] = src[perm[d]]^T.
+// transpose_packed_lu: packed ILU(0) factor of A^T from the factor of A (bs = 4);
+// trow_of[d] = block row of position d in A^T, diag[i] = position of A's block (i, i).
+int transpose_pattern_host(int nb, const int *rp, const int *ci, int *trp, int *tci, int *perm);
+int transpose_blocks(long long nnzb, int bs, const int *d_perm, const double *src, double *dst, hipStream_t st);
+int transpose_packed_lu(int nnzb, const int *d_trow_of, const int *d_tcol, const int *d_perm, const int *d_diag,
+                        const double *d_lu, double *d_lut, hipStream_t st);
+
 }  // namespace isai

@@ -21,6 +21,8 @@
 #include "cape_sptrsv.h"
 #include "common.hpp"
 #include "internal.hpp"
+#include "levels.hpp"
+#include "isai_adjoint.h"
 #include "isai_bsr.h"
 #include "isai_solve.h"
 
@@ -367,6 +369,9 @@ struct isai_handle {
     double *d_res2 = nullptr;  // residual output pair
     int *h_done = nullptr;      // pinned
     double *h_res2 = nullptr;   // pinned
+    // matrix arrays owned by the handle (adjoint handles own A^T)
+    int *own_row = nullptr, *own_col = nullptr;
+    double *own_val = nullptr;
 };
 
 namespace {
@@ -589,6 +594,9 @@ int isai_destroy(isai_handle_t h) {
     if (h->h_done) (void)hipHostFree(h->h_done);
     if (h->h_res2) (void)hipHostFree(h->h_res2);
     if (h->sv) isai_bsr_sptrsv_destroy(h->sv);
+    (void)hipFree(h->own_row);
+    (void)hipFree(h->own_col);
+    (void)hipFree(h->own_val);
     delete h;
     return ISAI_OK;
 }
@@ -776,6 +784,115 @@ int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, con
         if (s != ISAI_OK) return fail(s);
     }
     *out = h;
+    return ISAI_OK;
+}
+
+int isai_adjoint_handle(isai_handle_t h, isai_handle_t *out, isai_stream_t stream) {
+    ISAI_REQUIRE(h && out, "isai_adjoint_handle: null argument");
+    ISAI_REQUIRE(h->bs == 4, "isai_adjoint_handle: bs = 4 only");
+    *out = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = h->nb, nnzb = h->nnzb;
+    auto *g = new isai_handle;
+    g->nb = nb;
+    g->nnzb = nnzb;
+    g->bs = h->bs;
+    g->n = h->n;
+    g->precond = h->precond;
+    g->precision = h->precision;
+    g->chunk = h->chunk;
+    g->nnzbL = h->nnzbU;  // M_U^T is lower triangular and is applied first
+    g->nnzbU = h->nnzbL;
+    std::vector<void *> tmp;
+    auto fail = [&](int code) {
+        for (void *p : tmp) (void)hipFree(p);
+        isai_destroy(g);
+        return code;
+    };
+    auto dmalloc = [&](void **p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 1) == hipSuccess; };
+    const size_t b16 = 16 * sizeof(double);
+    if (!dmalloc((void **)&g->S, sizeof(Scalars)) || !dmalloc((void **)&g->d_res2, 2 * sizeof(double)) ||
+        hipHostMalloc(&g->h_done, sizeof(int)) != hipSuccess || hipHostMalloc(&g->h_res2, 2 * sizeof(double)) != hipSuccess ||
+        !dmalloc((void **)&g->own_row, sizeof(int) * (nb + 1)) || !dmalloc((void **)&g->own_col, sizeof(int) * nnzb) ||
+        !dmalloc((void **)&g->own_val, b16 * nnzb) || !dmalloc((void **)&g->d_LU, b16 * nnzb)) {
+        set_error("isai_adjoint_handle: allocation failed");
+        return fail(ISAI_ERR_ALLOC);
+    }
+    g->d_row = g->own_row;
+    g->d_col = g->own_col;
+    g->d_val = g->own_val;
+    // upload a host int array into a temporary device buffer
+    auto upload = [&](const std::vector<int> &v, int **d) -> bool {
+        if (!dmalloc((void **)d, sizeof(int) * v.size())) return false;
+        tmp.push_back(*d);
+        return v.empty() || hipMemcpyAsync(*d, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, st) == hipSuccess;
+    };
+    // --- A^T and the packed factor of A^T
+    std::vector<int> rp(nb + 1), ci(nnzb), trp(nb + 1), tci(nnzb), perm(nnzb), trow_of(nnzb), diag(nb, -1);
+    ISAI_HIP(hipMemcpyAsync(rp.data(), h->d_row, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipMemcpyAsync(ci.data(), h->d_col, sizeof(int) * nnzb, hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipStreamSynchronize(st));
+    int s = transpose_pattern_host(nb, rp.data(), ci.data(), trp.data(), tci.data(), perm.data());
+    if (s != ISAI_OK) return fail(s);
+    for (int i = 0; i < nb; ++i) {
+        for (int d = trp[i]; d < trp[i + 1]; ++d) trow_of[d] = i;
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] == i) diag[i] = k;
+        if (diag[i] < 0) {
+            set_error("isai_adjoint_handle: block row %d has no diagonal block", i);
+            return fail(ISAI_ERR_PATTERN);
+        }
+    }
+    int *d_perm = nullptr, *d_trow_of = nullptr, *d_diag = nullptr;
+    if (!upload(perm, &d_perm) || !upload(trow_of, &d_trow_of) || !upload(diag, &d_diag)) {
+        set_error("isai_adjoint_handle: allocation / copy failed");
+        return fail(ISAI_ERR_ALLOC);
+    }
+    ISAI_HIP(hipMemcpyAsync(g->own_row, trp.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st));
+    ISAI_HIP(hipMemcpyAsync(g->own_col, tci.data(), sizeof(int) * nnzb, hipMemcpyHostToDevice, st));
+    if ((s = transpose_blocks(nnzb, 4, d_perm, h->d_val, g->own_val, st)) != ISAI_OK) return fail(s);
+    if ((s = transpose_packed_lu(nnzb, d_trow_of, g->own_col, d_perm, d_diag, h->d_LU, g->d_LU, st)) != ISAI_OK)
+        return fail(s);
+    // --- transposed ISAI factors: slot 0 (applied first) = M_U^T, slot 1 = M_L^T
+    struct Src {
+        const int *row, *col;
+        const double *val;
+        int nnz;
+    } src[2] = {{h->d_Urow, h->d_Ucol, h->d_Uval, h->nnzbU}, {h->d_Lrow, h->d_Lcol, h->d_Lval, h->nnzbL}};
+    int **dst_row[2] = {&g->d_Lrow, &g->d_Urow}, **dst_col[2] = {&g->d_Lcol, &g->d_Ucol};
+    double **dst_val[2] = {&g->d_Lval, &g->d_Uval};
+    for (int f = 0; f < 2; ++f) {
+        const int nz = src[f].nnz;
+        std::vector<int> frp(nb + 1), fci(nz), ftrp(nb + 1), ftci(nz), fperm(nz);
+        ISAI_HIP(hipMemcpyAsync(frp.data(), src[f].row, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, st));
+        ISAI_HIP(hipMemcpyAsync(fci.data(), src[f].col, sizeof(int) * nz, hipMemcpyDeviceToHost, st));
+        ISAI_HIP(hipStreamSynchronize(st));
+        if ((s = transpose_pattern_host(nb, frp.data(), fci.data(), ftrp.data(), ftci.data(), fperm.data())) != ISAI_OK)
+            return fail(s);
+        int *d_fperm = nullptr;
+        if (!dmalloc((void **)dst_row[f], sizeof(int) * (nb + 1)) || !dmalloc((void **)dst_col[f], sizeof(int) * nz) ||
+            !dmalloc((void **)dst_val[f], b16 * nz) || !upload(fperm, &d_fperm)) {
+            set_error("isai_adjoint_handle: allocation failed");
+            return fail(ISAI_ERR_ALLOC);
+        }
+        ISAI_HIP(hipMemcpyAsync(*dst_row[f], ftrp.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st));
+        ISAI_HIP(hipMemcpyAsync(*dst_col[f], ftci.data(), sizeof(int) * nz, hipMemcpyHostToDevice, st));
+        if ((s = transpose_blocks(nz, 4, d_fperm, src[f].val, *dst_val[f], st)) != ISAI_OK) return fail(s);
+    }
+    if (g->precision != ISAI_PREC_FP64 && (s = convert_lp(g, g->precision, st)) != ISAI_OK) return fail(s);
+    ISAI_HIP(hipStreamSynchronize(st));
+    for (void *p : tmp) (void)hipFree(p);
+    tmp.clear();
+    int nlev = 0;
+    {
+        std::vector<int> lev(nb);
+        if (compute_levels(nb, trp.data(), tci.data(), true, lev.data(), &nlev) == ISAI_OK) g->nlevels = nlev;
+    }
+    if (g->precond == ISAI_PRECOND_ILU_SPTRSV) {
+        s = isai_bsr_sptrsv_analysis(nb, nnzb, 4, g->d_row, g->d_col, g->d_LU, g->chunk, &g->sv, stream);
+        if (s != ISAI_OK) return fail(s);
+    }
+    *out = g;
     return ISAI_OK;
 }
 

@@ -28,6 +28,11 @@
  *   ora_bsr_sptrsv_lu    L (U x) = b residual with dense factors.
  *   ora_csr_sptrsv_lu    equals ora_bsr_sptrsv_lu through ora_bsr2csr.
  *   ora_bicgstab/fgmres  true residual <= tol; equality with a direct dense solve.
+ *   ora_bsr_transpose    dense transpose equality (scipy-free, brute force).
+ *   ora_transpose_packed_lu  dense identity L'U' = (L U)^T with L' unit lower and
+ *                        U' upper; equality with ora_bsrilu0 of A^T on patterns
+ *                        where the block ILU(0) is exact (full pattern).
+ *   ora_bsr_outer_grad   dense -lam x^T masked by the block pattern.
  */
 #include "oracle.h"
 
@@ -784,4 +789,77 @@ int ora_fgmres(int nb, int bs, const int *rp, const int *ci, const double *val,
     *relres = ora_rel_residual_bsr(nb, bs, rp, ci, val, b, x);
     free(V); free(Z); free(w); free(H); free(cs); free(sn); free(g); free(yv);
     return conv ? ORA_OK : ORA_ERR_NOT_CONVERGED;
+}
+
+/* ------------------------------------------------------- differentiation form */
+void ora_bsr_transpose(int nb, int bs, const int *rp, const int *ci, const double *val,
+                       int *trp, int *tci, double *tval) {
+    const int b2 = bs * bs;
+    int *cnt = (int *)calloc((size_t)nb + 1, sizeof(int));
+    for (int i = 0; i < nb; ++i)
+        for (int k = rp[i]; k < rp[i + 1]; ++k) cnt[ci[k] + 1]++;
+    trp[0] = 0;
+    for (int j = 0; j < nb; ++j) trp[j + 1] = trp[j] + cnt[j + 1];
+    for (int j = 0; j < nb; ++j) cnt[j] = trp[j];
+    for (int i = 0; i < nb; ++i)       /* rows ascending => columns of A^T ascending */
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const int d = cnt[ci[k]]++;
+            tci[d] = i;
+            for (int r = 0; r < bs; ++r)
+                for (int c = 0; c < bs; ++c) tval[(size_t)d * b2 + r * bs + c] = val[(size_t)k * b2 + c * bs + r];
+        }
+    free(cnt);
+}
+
+static int block_pos(const int *rp, const int *ci, int i, int j) {
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+        if (ci[k] == j) return k;
+    return -1;
+}
+
+int ora_transpose_packed_lu(int nb, int bs, const int *rp, const int *ci, const double *lu,
+                            const int *trp, const int *tci, double *lut) {
+    const int b2 = bs * bs;
+    for (int I = 0; I < nb; ++I)
+        for (int d = trp[I]; d < trp[I + 1]; ++d) {
+            const int J = tci[d];
+            const int k = block_pos(rp, ci, J, I); /* block (J, I) of A */
+            if (k < 0) return ORA_ERR_PATTERN;
+            double *o = lut + (size_t)d * b2;
+            const double *s = lu + (size_t)k * b2;
+            if (I > J) {        /* L'_IJ = U_JI^T D_J^{-1} */
+                const int kd = block_pos(rp, ci, J, J);
+                if (kd < 0) return ORA_ERR_PATTERN;
+                const double *D = lu + (size_t)kd * b2;
+                for (int r = 0; r < bs; ++r)
+                    for (int c = 0; c < bs; ++c) o[r * bs + c] = s[c * bs + r] / D[c * bs + c];
+            } else if (I < J) { /* U'_IJ = D_I L_JI^T */
+                const int kd = block_pos(rp, ci, I, I);
+                if (kd < 0) return ORA_ERR_PATTERN;
+                const double *D = lu + (size_t)kd * b2;
+                for (int r = 0; r < bs; ++r)
+                    for (int c = 0; c < bs; ++c) o[r * bs + c] = D[r * bs + r] * s[c * bs + r];
+            } else {            /* diagonal: strict lower from U^T D^{-1}, upper from D L^T */
+                for (int r = 0; r < bs; ++r)
+                    for (int c = 0; c < bs; ++c) {
+                        if (r > c) o[r * bs + c] = s[c * bs + r] / s[c * bs + c];
+                        else if (r == c) o[r * bs + c] = s[r * bs + r];
+                        else o[r * bs + c] = s[r * bs + r] * s[c * bs + r];
+                    }
+            }
+        }
+    return ORA_OK;
+}
+
+void ora_bsr_outer_grad(int nb, int bs, const int *rp, const int *ci, const double *lam,
+                        const double *x, double alpha, double *g) {
+    const int b2 = bs * bs;
+#pragma omp parallel for schedule(static)
+    for (int I = 0; I < nb; ++I)
+        for (int k = rp[I]; k < rp[I + 1]; ++k) {
+            const int J = ci[k];
+            for (int r = 0; r < bs; ++r)
+                for (int c = 0; c < bs; ++c)
+                    g[(size_t)k * b2 + r * bs + c] = alpha * lam[(size_t)I * bs + r] * x[(size_t)J * bs + c];
+        }
 }

@@ -167,6 +167,23 @@ int ora_fgmres(int nb, int bs, const int *rp, const int *ci, const double *val,
 /* Apply the preconditioner once: z = P v. */
 void ora_precond_apply(const ora_precond_t *P, const double *v, double *z);
 
+/* ---- differentiation form (no reference counterpart: the reference has no
+ * gradients; these restate the adjoint of x = A^{-1} b, L = L(x)). */
+
+/* BSR of A^T: block (J, I) of the result = (block (I, J) of A)^T; columns
+ * ascending per block row.  Sizes: trp[nb+1], tci[nnzb], tval[nnzb*bs*bs]. */
+void ora_bsr_transpose(int nb, int bs, const int *rp, const int *ci, const double *val,
+                       int *trp, int *tci, double *tval);
+/* Packed ILU(0) factor of A^T derived from the packed factor of A (bs = 4 layout
+ * of ora_bsrilu0, any bs): with A ~ L U (scalar unit-lower L, upper U, diagonal
+ * D = diag(U)), A^T ~ L' U' with L' = U^T D^{-1} (unit lower) and U' = D L^T.
+ * Output on the transposed pattern (trp, tci from ora_bsr_transpose). */
+int ora_transpose_packed_lu(int nb, int bs, const int *rp, const int *ci, const double *lu,
+                            const int *trp, const int *tci, double *lut);
+/* Pattern-restricted outer product: g[k] = alpha * lam_I x_J^T for block k = (I, J). */
+void ora_bsr_outer_grad(int nb, int bs, const int *rp, const int *ci, const double *lam,
+                        const double *x, double alpha, double *g);
+
 /* Number of OpenMP threads the oracle runs with (1 when built without OpenMP). */
 int ora_num_threads(void);
 

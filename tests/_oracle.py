@@ -46,6 +46,9 @@ _SIGS = {
     "ora_fgmres": ([I, I, P, P, P, P, P, P, I, D, I, P, P], I),
     "ora_precond_apply": ([P, P, P], None),
     "ora_num_threads": ([], I),
+    "ora_bsr_transpose": ([I, I, P, P, P, P, P, P], None),
+    "ora_transpose_packed_lu": ([I, I, P, P, P, P, P, P], I),
+    "ora_bsr_outer_grad": ([I, I, P, P, P, P, D, P], None),
 }
 for _n, (_a, _r) in _SIGS.items():
     _f = getattr(ora, _n)
@@ -279,6 +282,34 @@ def precond_apply(P_, v):
     z = np.empty(P_.nb * P_.bs)
     ora.ora_precond_apply(C.byref(P_), _p(_f64(v)), _p(z))
     return z
+
+
+def bsr_transpose(rp, ci, val):
+    nb, bs = len(rp) - 1, val.shape[1]
+    rp, ci = _i32(rp), _i32(ci)
+    trp = np.empty(nb + 1, np.int32)
+    tci = np.empty(len(ci), np.int32)
+    tval = np.empty((len(ci), bs, bs))
+    ora.ora_bsr_transpose(nb, bs, _p(rp), _p(ci), _p(_f64(val)), _p(trp), _p(tci), _p(tval))
+    return trp, tci, tval
+
+
+def transpose_packed_lu(rp, ci, lu):
+    """(trp, tci, packed ILU(0) factor of A^T) from the packed factor of A."""
+    nb, bs = len(rp) - 1, lu.shape[1]
+    trp, tci, _ = bsr_transpose(rp, ci, lu)
+    lut = np.empty_like(lu)
+    st = ora.ora_transpose_packed_lu(nb, bs, _p(_i32(rp)), _p(_i32(ci)), _p(_f64(lu)), _p(trp), _p(tci), _p(lut))
+    if st:
+        raise ValueError(f"ora_transpose_packed_lu failed {st}")
+    return trp, tci, lut
+
+
+def outer_grad(rp, ci, lam, x, alpha, bs=4):
+    nb = len(rp) - 1
+    g = np.empty((len(ci), bs, bs))
+    ora.ora_bsr_outer_grad(nb, bs, _p(_i32(rp)), _p(_i32(ci)), _p(_f64(lam)), _p(_f64(x)), alpha, _p(g))
+    return g
 
 
 def num_threads():

@@ -371,3 +371,49 @@ def test_krylov_solvers_converge_to_direct_solution(solver, kind):
     xd = np.linalg.solve(O.bsr_to_dense(rp, ci, val), b)
     assert np.allclose(x, xd, rtol=1e-8, atol=1e-8)
     assert np.allclose(xd, 1.0, atol=1e-10)
+
+
+# ---------------------------------------------------------- differentiation form
+def test_bsr_transpose_dense():
+    nb, bs = 14, 4
+    rp, ci, val = random_bsr(nb, bs, 0.3, RNG)
+    trp, tci, tval = O.bsr_transpose(rp, ci, val)
+    assert np.array_equal(O.bsr_to_dense(trp, tci, tval), O.bsr_to_dense(rp, ci, val).T)
+    for i in range(nb):
+        assert np.all(np.diff(tci[trp[i]:trp[i + 1]]) > 0)
+
+
+@pytest.mark.parametrize("density", [0.3, 1.0])
+def test_transpose_packed_lu_identity(density):
+    """L'U' = (LU)^T with L' unit lower, U' upper (dense algebra)."""
+    nb, bs = 10, 4
+    rp, ci, val = random_bsr(nb, bs, density, RNG)
+    lu, st, _ = O.bsrilu0(rp, ci, val)
+    assert st == 0
+    trp, tci, lut = O.transpose_packed_lu(rp, ci, lu)
+    LUd, LUt = O.bsr_to_dense(rp, ci, lu), O.bsr_to_dense(trp, tci, lut)
+    L, U = np.tril(LUd, -1) + np.eye(nb * bs), np.triu(LUd)
+    L2, U2 = np.tril(LUt, -1) + np.eye(nb * bs), np.triu(LUt)
+    assert np.allclose(L2 @ U2, (L @ U).T, rtol=1e-12, atol=1e-12)
+
+
+def test_transpose_packed_lu_equals_ilu0_of_transpose():
+    """On a structurally symmetric pattern ILU(0) is unique, so refactoring A^T
+    gives the derived factor."""
+    rp, ci, val = problems.generate(3, 7, seed=5)
+    lu, st, _ = O.bsrilu0(rp, ci, val)
+    trp, tci, lut = O.transpose_packed_lu(rp, ci, lu)
+    _, _, tval = O.bsr_transpose(rp, ci, val)
+    lu_t, st2, _ = O.bsrilu0(trp, tci, tval)
+    assert st == 0 and st2 == 0
+    assert np.allclose(lut, lu_t, rtol=1e-11, atol=1e-12)
+
+
+def test_outer_grad_dense():
+    nb, bs = 9, 4
+    rp, ci, val = random_bsr(nb, bs, 0.4, RNG)
+    lam, x = RNG.standard_normal(nb * bs), RNG.standard_normal(nb * bs)
+    g = O.outer_grad(rp, ci, lam, x, -1.0)
+    mask = np.kron(dense_mask(rp, ci, nb), np.ones((bs, bs), bool))
+    G = O.bsr_to_dense(rp, ci, g)
+    assert np.array_equal(G[mask], (-np.outer(lam, x))[mask]) and not G[~mask].any()
