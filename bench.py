@@ -13,7 +13,12 @@ Kernel figures are hipEvent timings on the stream the kernels run on; the
 roofline object is for the dominant kernel of the solve (the FP64 BSR SpMV with
 A), with algorithmic bytes per launch (16*8+4)*nnzb + 4*(nb+1) + 2*4*8*nb.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-forms]
+
+The "forms" object adds the precision form (27-pt convection-diffusion 64^3,
+FGMRES(30), FP64/FP32/BF16 ISAI), the batching form (256 uneven 9-pt systems with
+1..16 right-hand sides, graph-replayed batched BiCGSTAB) and the differentiation
+form (forward + adjoint solve and dL/dA on the 9-pt 256^2 stencil).
 """
 from __future__ import annotations
 
@@ -101,6 +106,150 @@ def cpu_baseline(rp, ci, val, b, budget_s: float = 40.0):
     return dict(setup_s=t_setup, solve_s=t_solve, iters=it, relres=rr, cores=O.num_threads(), status=st)
 
 
+def bench_precision(timer, stream, steps: int) -> dict:
+    """Precision form: 3-D 27-pt convection-diffusion 64^3 (bs = 4, 262,144 block rows),
+    FGMRES(30) to 1e-10 with the ISAI factors stored in FP64 / FP32 / BF16."""
+    from isai_kernel_amd import DeviceArray, call, problems
+    from isai_kernel_amd.solver import Handle, Matrix
+
+    rp, ci, val = problems.generate(2, 64, seed=0)
+    nb, nnzb = len(rp) - 1, len(ci)
+    n = 4 * nb
+    A = Matrix(rp, ci, val)
+    d_b = DeviceArray.from_numpy(problems.normal(1, n))
+    d_x = DeviceArray((n,), np.float64)
+    h = Handle(A, precond="isai", precision="fp64", stream=stream)
+    out = {"workload": "3-D 27-pt convection-diffusion 64^3, bs=4, FGMRES(30) to 1e-10, b ~ N(0,1)",
+           "nb": nb, "nnzb": nnzb}
+    xs = {}
+    for prec in ("fp64", "fp32", "bf16"):
+        h.set_preconditioner("isai", prec)
+        state = {}
+
+        def one():
+            call("isai_fill_f64_async", d_x.ptr, 0.0, n, stream)
+            st, it, rr = h.solve(d_b, d_x, "fgmres", tol=TOL, maxit=3000, restart=30, check=False)
+            state.update(st=st, it=it, rr=rr)
+        one()
+        ms = timer.time(one, steps)
+        xs[prec] = d_x.to_numpy()
+        out[prec] = {"ms": round(ms, 3), "iters": state["it"], "relres": state["rr"], "status": state["st"]}
+        log(f"precision form {prec}: {ms:.2f} ms ({state['it']} it, relres {state['rr']:.2e})")
+    for prec in ("fp32", "bf16"):
+        out[prec]["max_rel_diff_vs_fp64"] = float(np.max(np.abs(xs[prec] - xs["fp64"])) / np.max(np.abs(xs["fp64"])))
+        out[prec]["speedup_vs_fp64"] = round(out["fp64"]["ms"] / out[prec]["ms"], 3)
+    h.close()
+    return out
+
+
+def bench_batching(stream, steps: int) -> dict:
+    """Batching form: 256 independent 2-D 9-pt systems, L uniform in [24, 200], 1..16
+    right-hand sides each (numpy default_rng(0)), FP64 Krylov with FP32 ISAI, one
+    batched setup and one batched BiCGSTAB replayed as a hipGraph."""
+    from isai_kernel_amd import DeviceArray, call, problems
+    from isai_kernel_amd.solver import Batch, Matrix
+
+    t0 = time.perf_counter()
+    rp, ci, val, sys_ptr, Ls, nrhs = problems.batch(256, 24, 200, seed=0, rhs_max=16)
+    gen_s = time.perf_counter() - t0
+    A = Matrix(rp, ci, val)
+    t0 = time.perf_counter()
+    bt = Batch(A, sys_ptr, nrhs, precision="fp32", stream=stream)
+    setup_s = time.perf_counter() - t0
+    ms_ilu, ms_isai = bt.setup_times()
+    variant, tuned_us, ncand = bt.tuning()
+    rng = np.random.default_rng(1)
+    B = np.zeros(bt.length)
+    for s in range(len(nrhs)):
+        m = 4 * int(sys_ptr[s + 1] - sys_ptr[s])
+        for j in range(nrhs[s]):
+            o = bt.offset(s, j)
+            B[o:o + m] = rng.standard_normal(m)
+    d_B = DeviceArray.from_numpy(B)
+    d_X = DeviceArray((bt.length,), np.float64)
+    times = []
+    for rep in range(1 + steps):
+        call("isai_fill_f64_async", d_X.ptr, 0.0, bt.length, stream)
+        call("isai_stream_synchronize", stream)
+        t0 = time.perf_counter()
+        st, iters, rr = bt.solve(d_B, d_X, tol=TOL, maxit=1000, use_graph=True, check=False)
+        if rep:
+            times.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.median(times))
+    out = {"workload": "256 x 2-D 9-pt LxL (L in [24,200]), bs=4, 1..16 RHS each, BiCGSTAB to 1e-10, FP32 ISAI",
+           "systems": len(nrhs), "pairs": bt.npairs, "block_rows": int(sys_ptr[-1]), "blocks": len(ci),
+           "ms": round(ms, 3), "status": st, "iters_min": int(iters.min()), "iters_mean": round(float(iters.mean()), 2),
+           "iters_max": int(iters.max()), "max_relres": float(rr.max()),
+           "setup_ms": {"ilu0": round(ms_ilu, 3), "isai_build": round(ms_isai, 3), "host_wall_s": round(setup_s, 2)},
+           "spmm_autotune": {"variant": variant, "us": round(tuned_us, 1), "candidates": ncand},
+           "generate_s": round(gen_s, 2)}
+    log(f"batching form: {ms:.1f} ms for {bt.npairs} pairs ({int(iters.min())}-{int(iters.max())} it, "
+        f"max relres {rr.max():.2e}); setup ILU {ms_ilu:.1f} ms ISAI {ms_isai:.1f} ms; SpMM {tuned_us:.0f} us")
+    bt.close()
+    return out
+
+
+def bench_differentiation(timer, stream, steps: int) -> dict:
+    """Differentiation form on the 2-D 9-pt 256^2 stencil (bs = 4, 65,536 block rows)
+    with the well-posed coefficients of generator problem 3 (the specified
+    coefficients make the matrix indefinite and the forward solve diverge):
+    forward ISAI-BiCGSTAB to 1e-12, adjoint handle from the forward factors,
+    adjoint solve to 1e-12, pattern-restricted dL/dA."""
+    from isai_kernel_amd import DeviceArray, call, lib, problems
+    from isai_kernel_amd.solver import Handle, Matrix
+
+    rp, ci, val = problems.generate(3, 256, seed=0)
+    nb, nnzb = len(rp) - 1, len(ci)
+    n = 4 * nb
+    A = Matrix(rp, ci, val)
+    b, t = problems.normal(1, n), problems.normal(2, n)
+    d_b = DeviceArray.from_numpy(b)
+    d_x, d_lam = DeviceArray((n,), np.float64), DeviceArray((n,), np.float64)
+    d_g = DeviceArray((nnzb, 4, 4), np.float64)
+    h = Handle(A, precond="isai", precision="fp64", stream=stream)
+    st_f = {}
+
+    def forward():
+        call("isai_fill_f64_async", d_x.ptr, 0.0, n, stream)
+        st_f.update(zip(("st", "it", "rr"), h.solve(d_b, d_x, "bicgstab", tol=1e-12, maxit=2000, check=False)))
+    forward()
+    fwd_ms = timer.time(forward, steps)
+    x = d_x.to_numpy()
+    d_gx = DeviceArray.from_numpy(2.0 * (x - t))
+    adj = ctypes.c_void_p()
+    t0 = time.perf_counter()
+    call("isai_adjoint_handle", h.h, ctypes.byref(adj), stream)
+    adj_setup_ms = (time.perf_counter() - t0) * 1e3
+    ah = Handle.__new__(Handle)
+    ah.h, ah.stream, ah.A = adj, stream, None
+    st_a = {}
+
+    def adjoint():
+        call("isai_fill_f64_async", d_lam.ptr, 0.0, n, stream)
+        st_a.update(zip(("st", "it", "rr"), ah.solve(d_gx, d_lam, "bicgstab", tol=1e-12, maxit=2000, check=False)))
+    adjoint()
+    adj_ms = timer.time(adjoint, steps)
+
+    def grad():
+        lib.isai_bsr_outer_grad(nb, nnzb, 4, A.d_rp.ptr, A.d_ci.ptr, d_lam.ptr, d_x.ptr, -1.0, d_g.ptr, stream)
+    for _ in range(5):
+        grad()
+    g_ms = timer.time(grad, 50)
+    g_bytes = (16 * 8 + 4) * nnzb + 4 * (nb + 1) + 2 * 4 * 8 * nb
+    out = {"workload": "2-D 9-pt 256^2, bs=4 (problem-3 coefficients), L = ||x - t||^2, tol 1e-12",
+           "nb": nb, "nnzb": nnzb,
+           "forward_ms": round(fwd_ms, 3), "forward_iters": st_f["it"], "forward_relres": st_f["rr"],
+           "adjoint_handle_ms_wall": round(adj_setup_ms, 2),
+           "adjoint_ms": round(adj_ms, 3), "adjoint_iters": st_a["it"], "adjoint_relres": st_a["rr"],
+           "outer_grad_us": round(g_ms * 1e3, 2),
+           "outer_grad_GBps": round(g_bytes / (g_ms * 1e-3) / 1e9, 1)}
+    log(f"differentiation form: forward {fwd_ms:.2f} ms ({st_f['it']} it), adjoint {adj_ms:.2f} ms "
+        f"({st_a['it']} it), adjoint handle {adj_setup_ms:.1f} ms wall, dL/dA {g_ms*1e3:.1f} us")
+    ah.close()
+    h.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +259,7 @@ def main() -> int:
     ap.add_argument("--problem", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=0, help="rows per wave of the sync-free solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-forms", action="store_true", help="skip the precision / batching / differentiation legs")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the kernels only (no solves, no CPU leg): for profiler runs")
     args = ap.parse_args()
@@ -265,6 +415,12 @@ def main() -> int:
         "copy_kernel_GBps": round(copy_gbps, 1),
     }
     h.close()
+    del A
+    if not args.no_forms:
+        fsteps = max(1, min(args.steps, 3))
+        result["forms"] = {"precision": bench_precision(timer, stream, fsteps),
+                           "batching": bench_batching(stream, fsteps),
+                           "differentiation": bench_differentiation(timer, stream, fsteps)}
     if not args.no_cpu:
         c = cpu_baseline(rp, ci, val, d_b.to_numpy())
         result["cpu_baseline"] = {
