@@ -6,12 +6,15 @@
 // blocks hold U_ij, with A = L U on every stored block.
 //
 // Schedule: the analysis (host) computes the level of every block row and the
-// rows in (level, row) order.  The factor kernel gives each block row one wave64
-// workgroup; a workgroup takes the next row of that order from an atomic ticket,
-// waits on the per-row ready flags of the rows it depends on (no global barrier,
-// no host round trip between levels), factors its row with the row staged in LDS,
-// and publishes its flag with release semantics.  Because tickets follow level
-// order, a row only ever waits on rows whose workgroups already hold a ticket.
+// rows in (level, row) order.  The factor kernel gives each block row one wave64;
+// a workgroup takes the next W rows of that order from an atomic ticket, so a row
+// only ever waits on rows whose waves already hold a ticket (no global barrier, no
+// host round trip between levels).  Finished rows are published into a buffer
+// pre-filled with a NaN sentinel, with relaxed agent-scope atomic stores, and
+// readers poll each value they need with relaxed agent-scope atomic loads until
+// it is no longer the sentinel: every value is its own ready flag.  Agent-scope
+// release/acquire fences are avoided on purpose -- on gfx950 they write back or
+// invalidate a whole XCD L2 per use, which serialises one such operation per row.
 //
 // Per row i:
 //   for each stored k < i, ascending:   L_ik = A'_ik inv(U_kk)  (right substitution)
@@ -20,6 +23,7 @@
 //   for each stored j > i:   U_ij = inv(L_ii) A'_ij            (forward substitution)
 // Every scalar operation is the oracle's, in the same order, without contraction,
 // so the factor is bit-identical to the sequential restatement.
+#include <algorithm>
 #include <climits>
 #include <vector>
 
@@ -31,6 +35,7 @@ struct isai_ilu_plan {
     int nb = 0, nnzb = 0, bs = 0, nlevels = 0;
     const int *d_row = nullptr, *d_col = nullptr;
     int *d_order = nullptr, *d_diag = nullptr;
+    int *d_upd_ptr = nullptr, *d_upd_q = nullptr, *d_upd_src = nullptr;  // update lists
     int max_row = 0;
 };
 
@@ -38,6 +43,29 @@ namespace isai {
 
 constexpr int kIluMaxRow = 64;       // blocks of one row staged in LDS
 constexpr int kIluSpinLimit = 1 << 20;
+constexpr unsigned long long kIluSentinel = 0x7FF4DEADBEEF0002ull;  // signalling-NaN payload
+
+// Value of another row's factor: relaxed agent-scope loads until it is published.
+__device__ __forceinline__ double ilu_ready(const double *p, bool &timed_out) {
+    const unsigned long long *q = (const unsigned long long *)p;
+    unsigned long long v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int spins = 0;
+    while (v == kIluSentinel) {
+        if (++spins > kIluSpinLimit) {
+            timed_out = true;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return __longlong_as_double((long long)v);
+}
+
+__global__ void ilu_sentinel_kernel(double *p, size_t n) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        p[k] = __longlong_as_double((long long)kIluSentinel);
+}
+constexpr int kIluStage = 32;     // blocks of earlier rows staged in LDS per row
 
 template <int BS>
 struct IluShape {
@@ -56,42 +84,76 @@ __device__ __forceinline__ int ilu_find(const int *__restrict__ rp, const int *_
     return -1;
 }
 
+// Per-wave LDS: the row being factored (kIluMaxRow blocks) and a staging area for
+// the blocks of earlier rows it reads (kIluStage blocks).  Waves per workgroup: four
+// while a wave needs at most 16 KB.
 template <int BS>
-__global__ __launch_bounds__(64) void ilu0_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
-                                                  double *__restrict__ val, const int *__restrict__ order,
-                                                  const int *__restrict__ diagpos, int *__restrict__ flags,
-                                                  unsigned int *__restrict__ ticket, int *__restrict__ zero_pivot,
-                                                  int *__restrict__ err) {
+struct IluWaves {
+    static constexpr int bytes = (kIluMaxRow + kIluStage) * BS * BS * 8;
+    static constexpr int W = bytes <= 16384 ? 4 : (bytes <= 32768 ? 2 : 1);
+};
+
+// Orders the LDS accesses of one wave (its lanes exchange data through LDS).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// One ticket per workgroup covers W consecutive positions of the level order; wave
+// w factors position base + w.  Every row waits only on smaller positions, held by
+// earlier tickets, so the schedule cannot deadlock.
+//
+// The analysis lists, for every stored lower block (i, k) at position p, the
+// updates A'_ij -= L_ik U_kj it causes: upd_q = local index of (i, j) in row i,
+// upd_src = position of (k, j).  Once the rows the row depends on are ready, the
+// wave gathers every block it will read -- the diagonal blocks of its k's, then
+// the U_kj of its updates in list order -- into LDS in one parallel pass, so the
+// sequential part of the factorization runs from LDS.  Rows whose list exceeds
+// the staging area read those blocks from global memory instead.
+template <int BS>
+__global__ __launch_bounds__(64 * IluWaves<BS>::W) void ilu0_kernel(
+    int nb, const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
+    double *fac, const int *__restrict__ order, const int *__restrict__ diagpos, const int *__restrict__ upd_ptr,
+    const int *__restrict__ upd_q, const int *__restrict__ upd_src, unsigned int *__restrict__ ticket,
+    int *__restrict__ zero_pivot, int *__restrict__ err) {
     using S = IluShape<BS>;
     constexpr int B2 = S::B2;
+    constexpr int W = IluWaves<BS>::W;
     constexpr int CB = 64 / BS;  // blocks per pass when one lane owns one row or column
-    __shared__ double rowv[kIluMaxRow * B2];
-    __shared__ int s_row;
-    const int lane = threadIdx.x;
-    if (lane == 0) s_row = order[atomicAdd(ticket, 1u)];
+    __shared__ double rowv_all[W][kIluMaxRow * B2];
+    __shared__ double stage_all[W][kIluStage * B2];
+    __shared__ int s_base;
+    if (threadIdx.x == 0) s_base = (int)atomicAdd(ticket, (unsigned)W);
     __syncthreads();
-    const int i = s_row;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int pos = s_base + w;
+    if (pos >= nb) return;
+    double *rowv = rowv_all[w];
+    double *stage = stage_all[w];
+    const int i = order[pos];
     const int rs = rp[i], re = rp[i + 1], len = re - rs;
     const int dp = diagpos[i];
+    const int nlow = dp - rs;
+    const int u0 = upd_ptr[rs], nupd = upd_ptr[dp] - u0;
+    const int nstage = nlow + nupd;
+    const bool staged = nstage <= kIluStage;
     const int g = lane / S::G, e = lane % S::G;
     const bool ok = e < B2;
     const int r = ok ? e / BS : 0, c = ok ? e % BS : 0;
 
     for (int x = lane; x < len * B2; x += 64) rowv[x] = val[(size_t)rs * B2 + x];
 
-    // Wait for every row this row depends on (stored k < i).
+    // Gather the published blocks this row reads (waiting on each value).
     bool timed_out = false;
-    for (int p = rs + lane; p < dp; p += 64) {
-        const int k = ci[p];
-        int spins = 0;
-        while (__hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-            if (++spins > kIluSpinLimit) { timed_out = true; break; }
-            __builtin_amdgcn_s_sleep(1);
+    if (staged) {
+        for (int x = lane; x < nstage * B2; x += 64) {
+            const int slot = x / B2, el = x - slot * B2;
+            const int src = slot < nlow ? diagpos[ci[rs + slot]] : upd_src[u0 + slot - nlow];
+            stage[x] = ilu_ready(fac + (size_t)src * B2 + el, timed_out);
         }
     }
-    if (timed_out) atomicExch(err, 1);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    __syncthreads();
+    wave_lds_sync();
 
     for (int p = rs; p < dp; ++p) {
         const int k = ci[p];
@@ -99,7 +161,14 @@ __global__ __launch_bounds__(64) void ilu0_kernel(int nb, const int *__restrict_
         // L_ik = A'_ik inv(U_kk) by right substitution with the upper part of the
         // packed diagonal block of row k; lane r owns row r.
         if (lane < BS) {
-            const double *Dk = val + (size_t)diagpos[k] * B2;
+            double Dk[B2];
+            if (staged) {
+#pragma unroll
+                for (int t = 0; t < B2; ++t) Dk[t] = stage[lp * B2 + t];
+            } else {
+#pragma unroll
+                for (int t = 0; t < B2; ++t) Dk[t] = ilu_ready(fac + (size_t)diagpos[k] * B2 + t, timed_out);
+            }
             double *X = rowv + lp * B2 + lane * BS;
 #pragma unroll
             for (int cc = 0; cc < BS; ++cc) {
@@ -109,29 +178,34 @@ __global__ __launch_bounds__(64) void ilu0_kernel(int nb, const int *__restrict_
                 X[cc] = sacc / Dk[cc * BS + cc];
             }
         }
-        __syncthreads();
-        // A'_ij -= L_ik U_kj for the stored j > k with (k, j) stored; one lane per
-        // element (r, c), G lanes per block.
-        for (int q0 = p + 1; q0 < re; q0 += S::C) {
-            const int q = q0 + g;
-            if (q < re && ok) {
-                const int pkj = ilu_find(rp, ci, k, ci[q]);
-                if (pkj >= 0) {
-                    const double *Ukj = val + (size_t)pkj * B2;
-                    const double *Lr = rowv + lp * B2 + r * BS;
-                    double sacc = 0.0;
+        wave_lds_sync();
+        // A'_ij -= L_ik U_kj for the updates of (i, k); one lane per element (r, c),
+        // G lanes per block.
+        const int ue = upd_ptr[p + 1];
+        for (int ub = upd_ptr[p]; ub < ue; ub += S::C) {
+            const int u = ub + g;
+            if (u < ue && ok) {
+                const double *Lr = rowv + lp * B2 + r * BS;
+                double sacc = 0.0;
+                if (staged) {
+                    const double *Ukj = stage + (nlow + u - u0) * B2;
 #pragma unroll
                     for (int m = 0; m < BS; ++m) sacc = sacc + Lr[m] * Ukj[m * BS + c];
-                    rowv[(q - rs) * B2 + e] = rowv[(q - rs) * B2 + e] - sacc;
+                } else {
+                    const double *Ukj = fac + (size_t)upd_src[u] * B2;
+#pragma unroll
+                    for (int m = 0; m < BS; ++m) sacc = sacc + Lr[m] * ilu_ready(Ukj + m * BS + c, timed_out);
                 }
+                const int q = upd_q[u];
+                rowv[q * B2 + e] = rowv[q * B2 + e] - sacc;
             }
         }
-        __syncthreads();
+        wave_lds_sync();
     }
 
     // Diagonal block: scalar LU without pivoting, packed L_ii\U_ii.
     if (lane == 0) {
-        double *D = rowv + (dp - rs) * B2;
+        double *D = rowv + nlow * B2;
         bool zero = false;
         for (int cc = 0; cc < BS; ++cc) {
             if (D[cc * BS + cc] == 0.0) zero = true;
@@ -142,12 +216,12 @@ __global__ __launch_bounds__(64) void ilu0_kernel(int nb, const int *__restrict_
         }
         if (zero) atomicMin(zero_pivot, i);
     }
-    __syncthreads();
+    wave_lds_sync();
 
     // U_ij = inv(L_ii) A'_ij for the stored j > i by forward substitution with the
     // unit-lower part of the diagonal block; lane (block, column c).
     {
-        const double *D = rowv + (dp - rs) * B2;
+        const double *D = rowv + nlow * B2;
         const int gb = lane / BS, col = lane % BS;
         for (int q0 = dp + 1; q0 < re; q0 += CB) {
             const int q = q0 + gb;
@@ -161,17 +235,21 @@ __global__ __launch_bounds__(64) void ilu0_kernel(int nb, const int *__restrict_
             }
         }
     }
-    __syncthreads();
-    for (int x = lane; x < len * B2; x += 64) val[(size_t)rs * B2 + x] = rowv[x];
-    __syncthreads();
-    if (lane == 0) __hip_atomic_store(&flags[i], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    wave_lds_sync();
+    // publish: every value is its own ready flag
+    for (int x = lane; x < len * B2; x += 64)
+        __hip_atomic_store((unsigned long long *)(fac + (size_t)rs * B2 + x), (unsigned long long)__double_as_longlong(rowv[x]),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (timed_out) atomicExch(err, 1);
 }
 
 template <int BS>
-void launch_ilu(const isai_ilu_plan *P, double *val, int *flags, unsigned *ticket, int *zp, int *err,
+void launch_ilu(const isai_ilu_plan *P, const double *val, double *fac, unsigned *ticket, int *zp, int *err,
                 hipStream_t st) {
-    ilu0_kernel<BS><<<P->nb, 64, 0, st>>>(P->nb, P->d_row, P->d_col, val, P->d_order, P->d_diag, flags, ticket,
-                                          zp, err);
+    constexpr int W = IluWaves<BS>::W;
+    ilu0_kernel<BS><<<(P->nb + W - 1) / W, 64 * W, 0, st>>>(P->nb, P->d_row, P->d_col, val, fac, P->d_order,
+                                                           P->d_diag, P->d_upd_ptr, P->d_upd_q, P->d_upd_src, ticket,
+                                                           zp, err);
 }
 
 }  // namespace isai
@@ -214,6 +292,29 @@ int isai_bsrilu0_analysis(int nb, int nnzb, int bs, const int *d_row, const int 
         set_error("bsrilu0: a block row holds %d blocks (limit %d)", max_row, kIluMaxRow);
         return ISAI_ERR_UNSUPPORTED;
     }
+    // update lists: for each stored lower block (i, k) at position p, the (i, j)
+    // with j > k such that (k, j) is stored, in ascending j
+    std::vector<int> upd_ptr(nnzb + 1, 0), upd_q, upd_src;
+    for (int i = 0; i < nb; ++i) {
+        for (int p = rp[i]; p < rp[i + 1]; ++p) {
+            upd_ptr[p] = (int)upd_q.size();
+            const int k = ci[p];
+            if (k >= i) continue;
+            int t = rp[k];
+            for (int q = p + 1; q < rp[i + 1]; ++q) {
+                while (t < rp[k + 1] && ci[t] < ci[q]) ++t;
+                if (t < rp[k + 1] && ci[t] == ci[q]) {
+                    upd_q.push_back(q - rp[i]);
+                    upd_src.push_back(t);
+                }
+            }
+        }
+    }
+    upd_ptr[nnzb] = (int)upd_q.size();
+    if (upd_q.size() > (size_t)INT_MAX / 2) {
+        set_error("bsrilu0 analysis: %zu updates exceed the int32 index range", upd_q.size());
+        return ISAI_ERR_CAPACITY;
+    }
     int nl = 0;
     compute_levels(nb, rp.data(), ci.data(), true, level.data(), &nl);
     level_order(nb, level.data(), nl, order.data(), nullptr);
@@ -225,15 +326,23 @@ int isai_bsrilu0_analysis(int nb, int nnzb, int bs, const int *d_row, const int 
     P->d_row = d_row;
     P->d_col = d_col;
     P->max_row = max_row;
+    const size_t nu = upd_q.size();
     if (hipMalloc(&P->d_order, sizeof(int) * (nb ? nb : 1)) != hipSuccess ||
-        hipMalloc(&P->d_diag, sizeof(int) * (nb ? nb : 1)) != hipSuccess) {
-        (void)hipFree(P->d_order);
-        delete P;
+        hipMalloc(&P->d_diag, sizeof(int) * (nb ? nb : 1)) != hipSuccess ||
+        hipMalloc(&P->d_upd_ptr, sizeof(int) * (nnzb + 1)) != hipSuccess ||
+        hipMalloc(&P->d_upd_q, sizeof(int) * (nu ? nu : 1)) != hipSuccess ||
+        hipMalloc(&P->d_upd_src, sizeof(int) * (nu ? nu : 1)) != hipSuccess) {
+        isai_bsrilu0_destroy(P);
         set_error("bsrilu0 analysis: hipMalloc failed");
         return ISAI_ERR_ALLOC;
     }
     ISAI_HIP(hipMemcpyAsync(P->d_order, order.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
     ISAI_HIP(hipMemcpyAsync(P->d_diag, diag.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
+    ISAI_HIP(hipMemcpyAsync(P->d_upd_ptr, upd_ptr.data(), sizeof(int) * (nnzb + 1), hipMemcpyHostToDevice, st));
+    if (nu) {
+        ISAI_HIP(hipMemcpyAsync(P->d_upd_q, upd_q.data(), sizeof(int) * nu, hipMemcpyHostToDevice, st));
+        ISAI_HIP(hipMemcpyAsync(P->d_upd_src, upd_src.data(), sizeof(int) * nu, hipMemcpyHostToDevice, st));
+    }
     ISAI_HIP(hipStreamSynchronize(st));
     *plan = P;
     return ISAI_OK;
@@ -251,31 +360,35 @@ int isai_bsrilu0_factor(isai_ilu_plan_t P, double *d_val, int *zero_pivot, isai_
     *zero_pivot = -1;
     if (P->nb == 0) return ISAI_OK;
     hipStream_t st = (hipStream_t)stream;
-    // scratch: ready flags, then ticket, zero-pivot word, error word
+    // scratch: the published factor (sentinel-filled), then ticket, zero-pivot word, error word
     char *scratch = nullptr;
-    const size_t off_flags = 0;
-    const size_t off_words = sizeof(int) * ((P->nb + 63) / 64 * 64);
+    const size_t b2 = (size_t)P->bs * P->bs;
+    const size_t nval = b2 * (size_t)P->nnzb;
+    const size_t off_words = sizeof(double) * ((nval + 31) / 32 * 32);
     const size_t bytes = off_words + 4 * sizeof(int);
     ISAI_HIP(hipMallocAsync((void **)&scratch, bytes, st));
-    int *flags = (int *)(scratch + off_flags);
+    double *fac = (double *)scratch;
     int *words = (int *)(scratch + off_words);  // [0] ticket, [1] zero pivot, [2] error
-    ISAI_HIP(hipMemsetAsync(flags, 0, bytes - off_flags, st));
+    const unsigned fill_grid = (unsigned)std::min<size_t>((nval + 255) / 256, 16384);
+    ilu_sentinel_kernel<<<fill_grid ? fill_grid : 1, 256, 0, st>>>(fac, nval);
+    ISAI_HIP(hipMemsetAsync(words, 0, 4 * sizeof(int), st));
     ISAI_HIP(hipMemsetD32Async((hipDeviceptr_t)&words[1], INT_MAX, 1, st));
     unsigned *ticket = (unsigned *)&words[0];
     switch (P->bs) {
-        case 1: launch_ilu<1>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
-        case 2: launch_ilu<2>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
-        case 3: launch_ilu<3>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
-        case 4: launch_ilu<4>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
-        case 5: launch_ilu<5>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
-        case 6: launch_ilu<6>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
-        case 7: launch_ilu<7>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
-        case 8: launch_ilu<8>(P, d_val, flags, ticket, &words[1], &words[2], st); break;
+        case 1: launch_ilu<1>(P, d_val, fac, ticket, &words[1], &words[2], st); break;
+        case 2: launch_ilu<2>(P, d_val, fac, ticket, &words[1], &words[2], st); break;
+        case 3: launch_ilu<3>(P, d_val, fac, ticket, &words[1], &words[2], st); break;
+        case 4: launch_ilu<4>(P, d_val, fac, ticket, &words[1], &words[2], st); break;
+        case 5: launch_ilu<5>(P, d_val, fac, ticket, &words[1], &words[2], st); break;
+        case 6: launch_ilu<6>(P, d_val, fac, ticket, &words[1], &words[2], st); break;
+        case 7: launch_ilu<7>(P, d_val, fac, ticket, &words[1], &words[2], st); break;
+        case 8: launch_ilu<8>(P, d_val, fac, ticket, &words[1], &words[2], st); break;
     }
     ISAI_LAUNCH_CHECK();
     int h[4];
     ISAI_HIP(hipMemcpyAsync(h, words, sizeof(h), hipMemcpyDeviceToHost, st));
     ISAI_HIP(hipStreamSynchronize(st));
+    if (!h[2]) ISAI_HIP(hipMemcpyAsync(d_val, fac, sizeof(double) * nval, hipMemcpyDeviceToDevice, st));
     ISAI_HIP(hipFreeAsync(scratch, st));
     if (h[2]) {
         set_error("bsrilu0: a dependency wait exceeded its spin limit");
@@ -293,6 +406,9 @@ int isai_bsrilu0_destroy(isai_ilu_plan_t P) {
     if (!P) return ISAI_OK;
     (void)hipFree(P->d_order);
     (void)hipFree(P->d_diag);
+    (void)hipFree(P->d_upd_ptr);
+    (void)hipFree(P->d_upd_q);
+    (void)hipFree(P->d_upd_src);
     delete P;
     return ISAI_OK;
 }
