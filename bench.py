@@ -158,6 +158,7 @@ def bench_batching(stream, steps: int) -> dict:
     setup_s = time.perf_counter() - t0
     ms_ilu, ms_isai = bt.setup_times()
     variant, tuned_us, ncand = bt.tuning()
+    table = {"A": [round(u, 1) for u in bt.tuning_table(0)], "isai": [round(u, 1) for u in bt.tuning_table(1)]}
     rng = np.random.default_rng(1)
     B = np.zeros(bt.length)
     for s in range(len(nrhs)):
@@ -181,7 +182,10 @@ def bench_batching(stream, steps: int) -> dict:
            "ms": round(ms, 3), "status": st, "iters_min": int(iters.min()), "iters_mean": round(float(iters.mean()), 2),
            "iters_max": int(iters.max()), "max_relres": float(rr.max()),
            "setup_ms": {"ilu0": round(ms_ilu, 3), "isai_build": round(ms_isai, 3), "host_wall_s": round(setup_s, 2)},
-           "spmm_autotune": {"variant": variant, "us": round(tuned_us, 1), "candidates": ncand},
+           "spmm_autotune": {"variant": variant, "us": round(tuned_us, 1), "candidates": ncand,
+                             "table_us": table,
+                             "candidate_key": "c: lane mapping c//6 (0 row-pair, 1 column-split), "
+                                              "workgroup (64,128,256)[(c//2)%3], XCD remap c%2"},
            "generate_s": round(gen_s, 2)}
     log(f"batching form: {ms:.1f} ms for {bt.npairs} pairs ({int(iters.min())}-{int(iters.max())} it, "
         f"max relres {rr.max():.2e}); setup ILU {ms_ilu:.1f} ms ISAI {ms_isai:.1f} ms; SpMM {tuned_us:.0f} us")

@@ -42,6 +42,11 @@ int isai_batch_vector_offset(isai_batch_t batch, int sys, int rhs, size_t *offse
 int isai_batch_pairs(isai_batch_t batch, int *npairs);
 /* Chosen SpMM variant (index into the candidates) and its time per product in us. */
 int isai_batch_tuning(isai_batch_t batch, int *variant, double *us, int *ncandidates);
+/* The autotuner's table for one operand (0 = A, 1 = the ISAI factors): us[c] = time
+ * per product of candidate c = mapping (c / 6: 0 row-pair lanes, 1 column-split
+ * lanes), workgroup size {64, 128, 256}[(c / 2) % 3], XCD remap c % 2; up to
+ * capacity entries, *count = number of candidates. */
+int isai_batch_tuning_table(isai_batch_t batch, int operand, double *us, int capacity, int *count);
 /* Setup device time: ILU(0) and ISAI build (both over the whole batch). */
 int isai_batch_setup_times(isai_batch_t batch, double *ms_ilu, double *ms_isai);
 

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -197,20 +198,90 @@ __device__ __forceinline__ void lowp_row(const RowCtx &q, const int *__restrict_
     }
 }
 
+// Column-split mapping: lane l of a block row's eight owns block column c = l % 4
+// and the right-hand sides of parity l / 4.  It loads its column of every block
+// (4 values) and one x value per right-hand side it handles, so the eight lanes
+// never load the same x value twice; the four column lanes add their partial row
+// sums at the end and lane c writes row c.  PREC: 0 FP64 values and accumulation,
+// 1 FP32 values, 2 BF16 values (FP32 accumulation, x rounded to FP32 on load).
+template <int PREC>
+__device__ __forceinline__ auto load_val(const void *__restrict__ val, size_t idx) {
+    if constexpr (PREC == 0) {
+        return __builtin_nontemporal_load((const double *)val + idx);
+    } else if constexpr (PREC == 1) {
+        return ((const float *)val)[idx];
+    } else {
+        return __uint_as_float((unsigned)((const unsigned short *)val)[idx] << 16);
+    }
+}
+
+template <int PREC, int R>
+__device__ __forceinline__ void col_row(const RowCtx &q, const int *__restrict__ ci, const void *__restrict__ val,
+                                        const double *__restrict__ X, double *__restrict__ Y) {
+    using Acc = typename std::conditional<PREC == 0, double, float>::type;
+    constexpr int RH = (R + 1) / 2;
+    constexpr int KU = R <= 4 ? 2 : 1;
+    const int c = q.lane & 3, par = q.lane >> 2;
+    // the lane's right-hand sides are j = 2u + par; when R is odd the last one of
+    // the odd lanes does not exist: it reloads j = R - 1 and is never written
+    const bool last_ok = 2 * (RH - 1) + par < R;
+    Acc acc[RH][4];
+#pragma unroll
+    for (int u = 0; u < RH; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[u][r] = Acc(0);
+    const double *xb = X + q.v0 + c;
+#pragma unroll KU
+    for (int k = q.rs; k < q.re; ++k) {
+        const int col = ci[k] - q.base;
+        Acc a[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[r] = load_val<PREC>(val, (size_t)k * 16 + r * 4 + c);
+        Acc xv[RH];
+#pragma unroll
+        for (int u = 0; u < RH; ++u) {
+            const int j = (u == RH - 1 && !last_ok) ? R - 1 : 2 * u + par;
+            xv[u] = (Acc)xb[(size_t)col * 4 + j * q.step];
+        }
+#pragma unroll
+        for (int u = 0; u < RH; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[u][r] = fma(a[r], xv[u], acc[u][r]);
+    }
+#pragma unroll
+    for (int u = 0; u < RH; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            acc[u][r] += __shfl_xor(acc[u][r], 1);
+            acc[u][r] += __shfl_xor(acc[u][r], 2);
+        }
+    double *yr = Y + q.v0 + (size_t)(q.br - q.base) * 4 + c;
+#pragma unroll
+    for (int u = 0; u < RH; ++u) {
+        const int j = 2 * u + par;
+        const Acc out = c == 0 ? acc[u][0] : (c == 1 ? acc[u][1] : (c == 2 ? acc[u][2] : acc[u][3]));
+        if (j < R && (q.mask & (1u << j))) yr[j * q.step] = (double)out;
+    }
+}
+
 #define ISAI_RHS_CASES(CALL) \
     CALL(1) CALL(2) CALL(3) CALL(4) CALL(5) CALL(6) CALL(7) CALL(8) \
     CALL(9) CALL(10) CALL(11) CALL(12) CALL(13) CALL(14) CALL(15) CALL(16)
 
-__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
-                                 const v2d *__restrict__ val, BatchDev B, const double *__restrict__ X,
-                                 double *__restrict__ Y, int remap, int all) {
+// MAP 0: row-pair mapping (f64_row / lowp_row); MAP 1: column-split mapping.
+template <int MAP>
+__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64_kernel(int nb, const int *__restrict__ rp,
+                                                               const int *__restrict__ ci, const v2d *__restrict__ val,
+                                                               BatchDev B, const double *__restrict__ X,
+                                                               double *__restrict__ Y, int remap, int all) {
     RowCtx q;
     int r;
     if (!row_context(B, rp, nb, remap, all, q, r)) return;
     switch (r) {
-#define ISAI_F64_CASE(N) \
-    case N:              \
-        f64_row<N>(q, ci, val, X, Y); \
+#define ISAI_F64_CASE(N)                          \
+    case N:                                       \
+        if (MAP == 0) f64_row<N>(q, ci, val, X, Y); \
+        else col_row<0, N>(q, ci, val, X, Y);     \
         break;
         ISAI_RHS_CASES(ISAI_F64_CASE)
 #undef ISAI_F64_CASE
@@ -219,17 +290,20 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64_kernel(int nb, const int
     }
 }
 
-template <bool BF16>
-__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_lowp_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
-                                  const v4u *__restrict__ val16, BatchDev B, const double *__restrict__ X,
-                                  double *__restrict__ Y, int remap, int all) {
+template <bool BF16, int MAP>
+__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_lowp_kernel(int nb, const int *__restrict__ rp,
+                                                                const int *__restrict__ ci,
+                                                                const v4u *__restrict__ val16, BatchDev B,
+                                                                const double *__restrict__ X, double *__restrict__ Y,
+                                                                int remap, int all) {
     RowCtx q;
     int r;
     if (!row_context(B, rp, nb, remap, all, q, r)) return;
     switch (r) {
-#define ISAI_LOWP_CASE(N) \
-    case N:               \
-        lowp_row<BF16, N>(q, ci, val16, X, Y); \
+#define ISAI_LOWP_CASE(N)                                   \
+    case N:                                                 \
+        if (MAP == 0) lowp_row<BF16, N>(q, ci, val16, X, Y); \
+        else col_row<BF16 ? 2 : 1, N>(q, ci, val16, X, Y);  \
         break;
         ISAI_RHS_CASES(ISAI_LOWP_CASE)
 #undef ISAI_LOWP_CASE
@@ -522,9 +596,10 @@ struct isai_batch {
     double *work = nullptr;  // kWork multi-vectors
     static constexpr int kWork = 9;
     // launch shapes picked by the autotuner: [0] product with A, [1] with the ISAI factors
-    int wg[2] = {256, 256}, remap[2] = {1, 1};
+    int wg[2] = {256, 256}, remap[2] = {1, 1}, map[2] = {1, 1};
     int variant = 0, ncand = 0;
     double tuned_us = 0;
+    double table[2][12] = {};  // autotuner: us per product, [operand][candidate]
     double ms_ilu = 0, ms_isai = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -553,28 +628,37 @@ namespace {
 enum Operand { kOpA = 0, kOpML = 1, kOpMU = 2 };
 
 // Y = op X over the pairs still running (all = 1: over every pair)
-int spmm(isai_batch *b, Operand op, const double *X, double *Y, int all, hipStream_t st) {
+template <int MAP>
+void launch_spmm(isai_batch *b, Operand op, int prec, const int *rp, const int *ci, const void *v, const double *X,
+                 double *Y, int all, hipStream_t st) {
     const BatchDev B = b->dev();
     const int slot = op == kOpA ? 0 : 1;
     const unsigned grid = grid_for(b->nb, b->wg[slot] / 8);
     const unsigned threads = (unsigned)b->wg[slot];
-    if (op == kOpA) {
-        bspmm_f64_kernel<<<grid, threads, 0, st>>>(b->nb, b->d_row, b->d_col, (const v2d *)b->d_val, B, X, Y,
-                                                   b->remap[0], all);
-    } else {
+    const int remap = b->remap[slot];
+    if (prec == ISAI_PREC_FP64)
+        bspmm_f64_kernel<MAP><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v2d *)v, B, X, Y, remap, all);
+    else if (prec == ISAI_PREC_FP32)
+        bspmm_lowp_kernel<false, MAP><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, remap, all);
+    else
+        bspmm_lowp_kernel<true, MAP><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, remap, all);
+}
+
+int spmm(isai_batch *b, Operand op, const double *X, double *Y, int all, hipStream_t st) {
+    const int *rp = b->d_row, *ci = b->d_col;
+    const void *v = b->d_val;
+    int prec = ISAI_PREC_FP64;
+    if (op != kOpA) {
         HandleFactors f;
         const int s = handle_factors_internal(b->pre, &f);
         if (s != ISAI_OK) return s;
-        const int *rp = op == kOpML ? f.Lrow : f.Urow;
-        const int *ci = op == kOpML ? f.Lcol : f.Ucol;
-        const void *v = op == kOpML ? f.Lval : f.Uval;
-        if (f.precision == ISAI_PREC_FP64)
-            bspmm_f64_kernel<<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v2d *)v, B, X, Y, b->remap[1], all);
-        else if (f.precision == ISAI_PREC_FP32)
-            bspmm_lowp_kernel<false><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, b->remap[1], all);
-        else
-            bspmm_lowp_kernel<true><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, b->remap[1], all);
+        rp = op == kOpML ? f.Lrow : f.Urow;
+        ci = op == kOpML ? f.Lcol : f.Ucol;
+        v = op == kOpML ? f.Lval : f.Uval;
+        prec = f.precision;
     }
+    if (b->map[op == kOpA ? 0 : 1] == 0) launch_spmm<0>(b, op, prec, rp, ci, v, X, Y, all, st);
+    else launch_spmm<1>(b, op, prec, rp, ci, v, X, Y, all, st);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
@@ -616,10 +700,11 @@ int batch_iteration(isai_batch *b, double *x, hipStream_t st) {
 }
 
 // Launch-shape autotuner: for the product with A (FP64) and with the ISAI factors
-// (storage precision) time every (workgroup size, XCD remap) candidate on the batch
-// itself, all pairs active, and keep the fastest of each.
+// (storage precision) time every (lane mapping, workgroup size, XCD remap)
+// candidate on the batch itself, all pairs active, and keep the fastest of each.
+// Candidate c: mapping c / 6, workgroup kWgCand[(c / 2) % 3], remap c % 2.
 constexpr int kWgCand[3] = {64, 128, kSpmmMaxWg};
-constexpr int kCandPerOp = 6;
+constexpr int kCandPerOp = 12;
 
 int autotune(isai_batch *b, hipStream_t st) {
     double *X = b->work, *Y = b->work + b->len;
@@ -636,7 +721,8 @@ int autotune(isai_batch *b, hipStream_t st) {
     for (int slot = 0; slot < 2 && status == ISAI_OK; ++slot) {
         const Operand op = slot == 0 ? kOpA : kOpML;
         for (int c = 0; c < kCandPerOp && status == ISAI_OK; ++c) {
-            b->wg[slot] = kWgCand[c / 2];
+            b->map[slot] = c / 6;
+            b->wg[slot] = kWgCand[(c / 2) % 3];
             b->remap[slot] = c % 2;
             status = spmm(b, op, X, Y, 1, st);  // warm-up
             if (status != ISAI_OK) break;
@@ -648,12 +734,14 @@ int autotune(isai_batch *b, hipStream_t st) {
                 status = ISAI_ERR_HIP;
             if (status == ISAI_ERR_HIP) set_error("isai_batch_setup: autotuner timing failed");
             const double us = ms * 1e3 / 5;
+            b->table[slot][c] = us;
             if (status == ISAI_OK && us < best_us[slot]) {
                 best_us[slot] = us;
                 best[slot] = c;
             }
         }
-        b->wg[slot] = kWgCand[best[slot] / 2];
+        b->map[slot] = best[slot] / 6;
+        b->wg[slot] = kWgCand[(best[slot] / 2) % 3];
         b->remap[slot] = best[slot] % 2;
     }
     (void)hipEventDestroy(e0);
@@ -855,6 +943,13 @@ int isai_batch_tuning(isai_batch_t b, int *variant, double *us, int *ncand) {
     if (variant) *variant = b->variant;
     if (us) *us = b->tuned_us;
     if (ncand) *ncand = b->ncand;
+    return ISAI_OK;
+}
+
+int isai_batch_tuning_table(isai_batch_t b, int operand, double *us, int capacity, int *count) {
+    ISAI_REQUIRE(b && (operand == 0 || operand == 1), "isai_batch_tuning_table: bad argument");
+    if (count) *count = kCandPerOp;
+    for (int c = 0; c < kCandPerOp && c < capacity && us; ++c) us[c] = b->table[operand][c];
     return ISAI_OK;
 }
 

@@ -38,10 +38,17 @@ struct Scalars {
     double bnorm2, rnorm2, snorm2, tol2;
     double H[(kMaxRestart + 1) * kMaxRestart];
     double cs[kMaxRestart], sn[kMaxRestart], g[kMaxRestart + 1], y[kMaxRestart];
-    int done;   // 0 running, 1 converged, 2 converged at s (BiCGSTAB), 3 breakdown
+    double hcur[kMaxRestart + 1];  // Gram-Schmidt coefficients of the current pass
+    // 0 running, 1 converged, 2 converged at s (BiCGSTAB), 3 breakdown, 4 iteration limit
+    int done;
     int first;
+    int iters, maxit;  // BiCGSTAB iteration count and limit (device side)
+    int jend;          // FGMRES: Arnoldi steps of the current cycle
+    int pad[3];
     double partial[2][kRedBlocks];
+    double mpart[(kMaxRestart + 1) * kRedBlocks];  // multi-dot partials, [vector][block]
 };
+constexpr int kStateInts = 8;  // done .. pad, read by the host in one copy
 
 __device__ __forceinline__ double block_sum(double v) {
     __shared__ double ws[kRedThreads / 64];
@@ -107,6 +114,12 @@ __global__ __launch_bounds__(kRedThreads) void bicg_init_kernel(size_t n, const 
     }
 }
 
+__global__ void bicg_start_kernel(Scalars *S, int maxit) {
+    S->iters = 0;
+    S->maxit = maxit;
+}
+
+// rhat = r, so the first rho = ||r||^2
 __global__ __launch_bounds__(kRedThreads) void bicg_init_final(Scalars *S, double tol) {
     const double b2 = final_sum(S, 0), r2 = final_sum(S, 1);
     if (threadIdx.x == 0) {
@@ -114,21 +127,44 @@ __global__ __launch_bounds__(kRedThreads) void bicg_init_final(Scalars *S, doubl
         S->rnorm2 = r2;
         S->tol2 = tol * tol * S->bnorm2;
         S->rho_old = S->alpha = S->omega = 1.0;
+        S->rho = r2;
         S->first = 1;
-        S->done = (r2 <= S->tol2) ? 1 : 0;
+        S->done = (r2 <= S->tol2) ? 1 : (S->iters >= S->maxit ? 4 : 0);
     }
 }
 
-__global__ __launch_bounds__(kRedThreads) void bicg_rho_final(Scalars *S) {
+// Sum of G workgroup partials in a fixed order (one block); eight independent
+// accumulators per thread keep eight loads in flight.
+__device__ __forceinline__ double wg_partial_sum(const double *part, int G) {
+    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int k = threadIdx.x;
+    for (; k + 7 * kRedThreads < G; k += 8 * kRedThreads) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] += part[k + u * kRedThreads];
+    }
+    for (; k < G; k += kRedThreads) v[0] += part[k];
+    return block_sum(((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7])));
+}
+
+// alpha = rho / rhat.v, with rhat.v from the product's fused partials
+__global__ __launch_bounds__(kRedThreads) void bicg_alpha_final_wg(Scalars *S, const double *part, int G) {
     if (skipped(S)) return;
-    const double rho = final_sum(S, 0);
+    const double d = wg_partial_sum(part, G);
     if (threadIdx.x == 0) {
-        S->rho = rho;
-        if (rho == 0.0) {
-            S->done = 3;
-        } else {
-            S->beta = S->first ? 0.0 : (rho / S->rho_old) * (S->alpha / S->omega);
-        }
+        S->iters++;
+        if (d == 0.0) S->done = 3;
+        else S->alpha = S->rho / d;
+        S->first = 0;
+    }
+}
+
+// omega = t.s / t.t from the product's fused partials
+__global__ __launch_bounds__(kRedThreads) void bicg_omega_final_wg(Scalars *S, const double *part, int G) {
+    if (skipped(S)) return;
+    const double ts = wg_partial_sum(part, G), tt = wg_partial_sum(part + G, G);
+    if (threadIdx.x == 0) {
+        if (tt == 0.0) S->done = 3;
+        else S->omega = ts / tt;
     }
 }
 
@@ -139,16 +175,6 @@ __global__ void bicg_p_kernel(size_t n, const double *__restrict__ r, double *p,
     const double beta = S->beta, omega = S->omega;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = first ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
-}
-
-__global__ __launch_bounds__(kRedThreads) void bicg_alpha_final(Scalars *S) {
-    if (skipped(S)) return;
-    const double d = final_sum(S, 0);
-    if (threadIdx.x == 0) {
-        if (d == 0.0) S->done = 3;
-        else S->alpha = S->rho / d;
-        S->first = 0;
-    }
 }
 
 // s = r - alpha v; partial ||s||^2
@@ -184,41 +210,55 @@ __global__ void bicg_xs_kernel(size_t n, double *x, const double *__restrict__ p
         x[i] += alpha * phat[i];
 }
 
-__global__ __launch_bounds__(kRedThreads) void bicg_omega_final(Scalars *S) {
-    if (skipped(S)) return;
-    const double ts = final_sum(S, 0), tt = final_sum(S, 1);
-    if (threadIdx.x == 0) {
-        if (tt == 0.0) S->done = 3;
-        else S->omega = ts / tt;
-    }
-}
-
-// x += alpha phat + omega shat; r = s - omega t; partial ||r||^2
+// x += alpha phat + omega shat; r = s - omega t; partials ||r||^2 and rhat.r (the
+// next iteration's rho)
 __global__ __launch_bounds__(kRedThreads) void bicg_xr_kernel(size_t n, double *x, double *r,
                                                               const double *__restrict__ phat,
                                                               const double *__restrict__ shat,
                                                               const double *__restrict__ s,
-                                                              const double *__restrict__ t, Scalars *S) {
+                                                              const double *__restrict__ t,
+                                                              const double *__restrict__ rhat, Scalars *S) {
     if (skipped(S)) return;
     const double alpha = S->alpha, omega = S->omega;
-    double acc = 0.0;
+    double acc = 0.0, accr = 0.0;
     for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads) {
         x[i] += alpha * phat[i] + omega * shat[i];
         const double ri = s[i] - omega * t[i];
         r[i] = ri;
         acc += ri * ri;
+        accr += rhat[i] * ri;
     }
     acc = block_sum(acc);
-    if (threadIdx.x == 0) S->partial[0][blockIdx.x] = acc;
+    accr = block_sum(accr);
+    if (threadIdx.x == 0) {
+        S->partial[0][blockIdx.x] = acc;
+        S->partial[1][blockIdx.x] = accr;
+    }
 }
 
+// End of an iteration: convergence test, next rho and beta, iteration limit.  A
+// solve that converged on s (done = 2, x already updated) becomes done = 1 here, so
+// iterations launched after convergence do nothing.
 __global__ __launch_bounds__(kRedThreads) void bicg_rnorm_final(Scalars *S) {
+    if (S->done == 2) {
+        __syncthreads();
+        if (threadIdx.x == 0) S->done = 1;
+        return;
+    }
     if (skipped(S)) return;
-    const double r2 = final_sum(S, 0);
+    const double r2 = final_sum(S, 0), rho = final_sum(S, 1);
     if (threadIdx.x == 0) {
         S->rnorm2 = r2;
-        S->rho_old = S->rho;
-        if (r2 <= S->tol2) S->done = 1;
+        if (r2 <= S->tol2) {
+            S->done = 1;
+        } else if (rho == 0.0) {
+            S->done = 3;
+        } else {
+            S->beta = (rho / S->rho) * (S->alpha / S->omega);
+            S->rho_old = S->rho;
+            S->rho = rho;
+            if (S->iters >= S->maxit) S->done = 4;
+        }
     }
 }
 
@@ -260,6 +300,7 @@ __global__ __launch_bounds__(kRedThreads) void fg_beta_final(Scalars *S, double 
         S->beta = beta;
         for (int k = 0; k <= kMaxRestart; ++k) S->g[k] = 0.0;
         S->g[0] = beta;
+        S->jend = 0;
         S->done = (w2 <= S->tol2) ? 1 : 0;
     }
 }
@@ -273,17 +314,60 @@ __global__ void fg_scale_kernel(size_t n, const double *__restrict__ w, double *
         v[i] = w[i] / d;
 }
 
-__global__ __launch_bounds__(kRedThreads) void fg_h_final(Scalars *S, int i, int j) {
+// Classical Gram-Schmidt, applied twice (CGS2): each pass computes h = V^T w with
+// one multi-dot (up to kDotChunk vectors per block column of the grid), sums the
+// partials per vector in a fixed order, and subtracts V h from w in one pass.
+constexpr int kDotChunk = 8;
+
+__global__ __launch_bounds__(kRedThreads) void fg_mdot_kernel(size_t n, const double *__restrict__ w,
+                                                              const double *__restrict__ V, int nv, Scalars *S) {
     if (skipped(S)) return;
-    const double h = final_sum(S, 0);
-    if (threadIdx.x == 0) S->H[i * kMaxRestart + j] = h;
+    const int i0 = blockIdx.y * kDotChunk;
+    double acc[kDotChunk];
+#pragma unroll
+    for (int u = 0; u < kDotChunk; ++u) acc[u] = 0.0;
+    for (size_t k = (size_t)blockIdx.x * kRedThreads + threadIdx.x; k < n; k += (size_t)kRedBlocks * kRedThreads) {
+        const double wk = w[k];
+#pragma unroll
+        for (int u = 0; u < kDotChunk; ++u)
+            if (i0 + u < nv) acc[u] += V[(size_t)(i0 + u) * n + k] * wk;
+    }
+#pragma unroll
+    for (int u = 0; u < kDotChunk; ++u) {
+        const double t = block_sum(acc[u]);
+        if (threadIdx.x == 0 && i0 + u < nv) S->mpart[(size_t)(i0 + u) * kRedBlocks + blockIdx.x] = t;
+    }
 }
 
-__global__ void fg_axpy_h_kernel(size_t n, double *w, const double *__restrict__ vi, const Scalars *S, int i, int j) {
+// block i: h_i = sum of its partials; pass 0 sets H[i][j], pass 1 adds to it
+__global__ __launch_bounds__(kRedThreads) void fg_hsum_final(Scalars *S, int j, int pass) {
     if (skipped(S)) return;
-    const double h = S->H[i * kMaxRestart + j];
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
-        w[k] -= h * vi[k];
+    const int i = blockIdx.x;
+    double v = 0.0;
+    for (int k = threadIdx.x; k < kRedBlocks; k += kRedThreads) v += S->mpart[(size_t)i * kRedBlocks + k];
+    v = block_sum(v);
+    if (threadIdx.x == 0) {
+        S->hcur[i] = v;
+        if (pass == 0) S->H[i * kMaxRestart + j] = v;
+        else S->H[i * kMaxRestart + j] += v;
+    }
+}
+
+// w -= sum_i hcur[i] V_i (i ascending); with_norm: partial ||w||^2
+__global__ __launch_bounds__(kRedThreads) void fg_axpy_multi_kernel(size_t n, double *w, const double *__restrict__ V,
+                                                                    int nv, Scalars *S, int with_norm) {
+    if (skipped(S)) return;
+    double acc = 0.0;
+    for (size_t k = (size_t)blockIdx.x * kRedThreads + threadIdx.x; k < n; k += (size_t)kRedBlocks * kRedThreads) {
+        double wk = w[k];
+        for (int i = 0; i < nv; ++i) wk -= S->hcur[i] * V[(size_t)i * n + k];
+        w[k] = wk;
+        acc += wk * wk;
+    }
+    if (with_norm) {
+        acc = block_sum(acc);
+        if (threadIdx.x == 0) S->partial[0][blockIdx.x] = acc;
+    }
 }
 
 // ||w|| -> H[j+1][j]; Givens rotations on column j; residual estimate |g[j+1]|.
@@ -308,6 +392,7 @@ __global__ __launch_bounds__(kRedThreads) void fg_givens_final(Scalars *S, int j
     S->g[j] = S->cs[j] * S->g[j];
     // keep the unrotated norm for the basis scaling
     S->rnorm2 = hn;
+    S->jend = j + 1;
     if (S->g[j + 1] * S->g[j + 1] <= S->tol2 || hn == 0.0) S->done = 1;
 }
 
@@ -320,8 +405,9 @@ __global__ void fg_scale_next_kernel(size_t n, const double *__restrict__ w, dou
 }
 
 // Back substitution H(0:jend,0:jend) y = g.
-__global__ void fg_solve_y_kernel(Scalars *S, int jend) {
+__global__ void fg_solve_y_kernel(Scalars *S) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int jend = S->jend;
     constexpr int m = kMaxRestart;
     for (int i = jend - 1; i >= 0; --i) {
         double s = S->g[i];
@@ -331,7 +417,8 @@ __global__ void fg_solve_y_kernel(Scalars *S, int jend) {
 }
 
 // x += sum_k y_k Z_k
-__global__ void fg_update_x_kernel(size_t n, double *x, const double *__restrict__ Z, const Scalars *S, int jend) {
+__global__ void fg_update_x_kernel(size_t n, double *x, const double *__restrict__ Z, const Scalars *S) {
+    const int jend = S->jend;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         double acc = x[i];
         for (int k = 0; k < jend; ++k) acc += S->y[k] * Z[(size_t)k * n + i];
@@ -367,7 +454,8 @@ struct isai_handle {
     size_t work_vecs = 0;
     Scalars *S = nullptr;
     double *d_res2 = nullptr;  // residual output pair
-    int *h_done = nullptr;      // pinned
+    int *h_done = nullptr;      // pinned, kStateInts words (done, first, iters, maxit, jend, ...)
+    double *d_wgpart = nullptr; // fused-dot partials of the A products, 2 * spmv_dot_groups(nb)
     double *h_res2 = nullptr;   // pinned
     // matrix arrays owned by the handle (adjoint handles own A^T)
     int *own_row = nullptr, *own_col = nullptr;
@@ -377,6 +465,11 @@ struct isai_handle {
 namespace {
 
 int ensure_work(isai_handle *h, size_t vecs) {
+    if (!h->d_wgpart && hipMalloc(&h->d_wgpart, sizeof(double) * 2 * spmv_dot_groups(h->nb)) != hipSuccess) {
+        h->d_wgpart = nullptr;
+        set_error("isai_solve: cannot allocate the reduction partials");
+        return ISAI_ERR_ALLOC;
+    }
     if (h->work_vecs >= vecs) return ISAI_OK;
     if (h->work) (void)hipFree(h->work);
     h->work = nullptr;
@@ -441,12 +534,22 @@ int true_residual(isai_handle *h, const double *b, const double *x, double *r, d
     return ISAI_OK;
 }
 
-int read_done(isai_handle *h, hipStream_t st, int *done) {
-    ISAI_HIP(hipMemcpyAsync(h->h_done, &h->S->done, sizeof(int), hipMemcpyDeviceToHost, st));
+// Copies done, first, iters, maxit, jend of the device record to the host.
+int read_state(isai_handle *h, hipStream_t st) {
+    ISAI_HIP(hipMemcpyAsync(h->h_done, &h->S->done, sizeof(int) * kStateInts, hipMemcpyDeviceToHost, st));
     ISAI_HIP(hipStreamSynchronize(st));
-    *done = *h->h_done;
     return ISAI_OK;
 }
+
+int read_done(isai_handle *h, hipStream_t st, int *done) {
+    const int s = read_state(h, st);
+    *done = h->h_done[0];
+    return s;
+}
+
+// Iterations launched between two host reads of the device state; iterations
+// launched after convergence return at once.
+constexpr int kItersPerRead = 3;
 
 #define TRY(x)                          \
     do {                                \
@@ -461,42 +564,43 @@ int bicgstab(isai_handle *h, double tol, int maxit, const double *b, double *x, 
     double *r = h->work, *rh = r + n, *p = rh + n, *v = p + n, *ph = v + n, *s = ph + n, *sh = s + n, *t = sh + n,
            *tmp = t + n;
     const int *skip = &h->S->done;
-    int it = 0;
+    const int G = spmv_dot_groups(h->nb);
+    double *part = h->d_wgpart;
+    *iters = 0;
+    bicg_start_kernel<<<1, 1, 0, st>>>(h->S, maxit);
     for (int restart = 0; restart < 8; ++restart) {
         TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, x, r, st, nullptr));
         bicg_init_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, b, r, rh, p, v, h->S);
         bicg_init_final<<<1, kRedThreads, 0, st>>>(h->S, tol);
         ISAI_LAUNCH_CHECK();
-        int done = 0;
-        TRY(read_done(h, st, &done));
-        while (!done && it < maxit) {
-            ++it;
-            dot_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, rh, r, nullptr, h->S, 1);
-            bicg_rho_final<<<1, kRedThreads, 0, st>>>(h->S);
-            bicg_p_kernel<<<vec_grid(n), 256, 0, st>>>(n, r, p, v, h->S);
-            TRY(precond_apply(h, p, ph, tmp, st, skip));
-            TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, ph, v, st, skip));
-            dot_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, rh, v, nullptr, h->S, 1);
-            bicg_alpha_final<<<1, kRedThreads, 0, st>>>(h->S);
-            bicg_s_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, r, v, s, h->S);
-            bicg_snorm_final<<<1, kRedThreads, 0, st>>>(h->S);
-            bicg_xs_kernel<<<vec_grid(n), 256, 0, st>>>(n, x, ph, h->S);
-            TRY(precond_apply(h, s, sh, tmp, st, skip));
-            TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, sh, t, st, skip));
-            dot_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, t, s, t, h->S, 1);
-            bicg_omega_final<<<1, kRedThreads, 0, st>>>(h->S);
-            bicg_xr_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, x, r, ph, sh, s, t, h->S);
-            bicg_rnorm_final<<<1, kRedThreads, 0, st>>>(h->S);
-            ISAI_LAUNCH_CHECK();
-            TRY(read_done(h, st, &done));
+        TRY(read_state(h, st));
+        int done = h->h_done[0];
+        for (int launched = h->h_done[2]; !done && launched < maxit;) {
+            for (int k = 0; k < kItersPerRead && launched < maxit; ++k, ++launched) {
+                bicg_p_kernel<<<vec_grid(n), 256, 0, st>>>(n, r, p, v, h->S);
+                TRY(precond_apply(h, p, ph, tmp, st, skip));
+                TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, ph, v, rh, 1, part, st, skip));
+                bicg_alpha_final_wg<<<1, kRedThreads, 0, st>>>(h->S, part, G);
+                bicg_s_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, r, v, s, h->S);
+                bicg_snorm_final<<<1, kRedThreads, 0, st>>>(h->S);
+                bicg_xs_kernel<<<vec_grid(n), 256, 0, st>>>(n, x, ph, h->S);
+                TRY(precond_apply(h, s, sh, tmp, st, skip));
+                TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, sh, t, s, 2, part, st, skip));
+                bicg_omega_final_wg<<<1, kRedThreads, 0, st>>>(h->S, part, G);
+                bicg_xr_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, x, r, ph, sh, s, t, rh, h->S);
+                bicg_rnorm_final<<<1, kRedThreads, 0, st>>>(h->S);
+                ISAI_LAUNCH_CHECK();
+            }
+            TRY(read_state(h, st));
+            done = h->h_done[0];
         }
+        *iters = h->h_done[2];
         TRY(true_residual(h, b, x, r, relres, st));
-        *iters = it;
         if (*relres <= tol) return ISAI_OK;
-        if (it >= maxit) break;
+        if (*iters >= maxit) break;
         // recursive residual drifted from the true one, or a breakdown: restart
     }
-    set_error("bicgstab: not converged after %d iterations (relres %.3e)", it, *relres);
+    set_error("bicgstab: not converged after %d iterations (relres %.3e)", *iters, *relres);
     return ISAI_ERR_NOT_CONVERGED;
 }
 
@@ -522,27 +626,30 @@ int fgmres(isai_handle *h, int m, double tol, int maxit, const double *b, double
         int done = 0;
         TRY(read_done(h, st, &done));
         if (done) break;
-        int jend = 0;
-        for (int j = 0; j < m && it < maxit; ++j) {
-            ++it;
+        const int steps = std::min(m, maxit - it);
+        for (int j = 0; j < steps; ++j) {
             double *vj = V + n * j, *zj = Z + n * j;
             TRY(precond_apply(h, vj, zj, tmp, st, skip));
             TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, zj, w, st, skip));
-            for (int i = 0; i <= j; ++i) {
-                dot_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, V + n * i, nullptr, h->S, 1);
-                fg_h_final<<<1, kRedThreads, 0, st>>>(h->S, i, j);
-                fg_axpy_h_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V + n * i, h->S, i, j);
+            const int nv = j + 1;
+            const dim3 dgrid(kRedBlocks, (nv + kDotChunk - 1) / kDotChunk);
+            for (int pass = 0; pass < 2; ++pass) {
+                fg_mdot_kernel<<<dgrid, kRedThreads, 0, st>>>(n, w, V, nv, h->S);
+                fg_hsum_final<<<nv, kRedThreads, 0, st>>>(h->S, j, pass);
+                fg_axpy_multi_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, V, nv, h->S, pass);
             }
-            dot_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, w, nullptr, h->S, 1);
             fg_givens_final<<<1, kRedThreads, 0, st>>>(h->S, j);
             fg_scale_next_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V + n * (j + 1), h->S);
             ISAI_LAUNCH_CHECK();
-            jend = j + 1;
-            TRY(read_done(h, st, &done));
-            if (done) break;
+            if ((j + 1) % kItersPerRead == 0 || j + 1 == steps) {
+                TRY(read_done(h, st, &done));
+                if (done) break;
+            }
         }
-        fg_solve_y_kernel<<<1, 64, 0, st>>>(h->S, jend);
-        fg_update_x_kernel<<<vec_grid(n), 256, 0, st>>>(n, x, Z, h->S, jend);
+        TRY(read_state(h, st));
+        it += h->h_done[4];  // Arnoldi steps the device took in this cycle
+        fg_solve_y_kernel<<<1, 64, 0, st>>>(h->S);
+        fg_update_x_kernel<<<vec_grid(n), 256, 0, st>>>(n, x, Z, h->S);
         ISAI_LAUNCH_CHECK();
         TRY(true_residual(h, b, x, w, relres, st));
         if (*relres <= tol) break;
@@ -591,6 +698,7 @@ int isai_destroy(isai_handle_t h) {
     (void)hipFree(h->work);
     (void)hipFree(h->S);
     (void)hipFree(h->d_res2);
+    (void)hipFree(h->d_wgpart);
     if (h->h_done) (void)hipHostFree(h->h_done);
     if (h->h_res2) (void)hipHostFree(h->h_res2);
     if (h->sv) isai_bsr_sptrsv_destroy(h->sv);
@@ -632,7 +740,7 @@ int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, con
     const size_t b2 = (size_t)bs * bs;
     if (hipMalloc(&h->d_LU, sizeof(double) * b2 * nnzb) != hipSuccess || hipMalloc(&h->S, sizeof(Scalars)) != hipSuccess ||
         hipMalloc(&h->d_res2, 2 * sizeof(double)) != hipSuccess ||
-        hipHostMalloc(&h->h_done, sizeof(int)) != hipSuccess || hipHostMalloc(&h->h_res2, 2 * sizeof(double)) != hipSuccess) {
+        hipHostMalloc(&h->h_done, sizeof(int) * kStateInts) != hipSuccess || hipHostMalloc(&h->h_res2, 2 * sizeof(double)) != hipSuccess) {
         set_error("isai_setup: allocation failed");
         return fail(ISAI_ERR_ALLOC);
     }
@@ -812,7 +920,7 @@ int isai_adjoint_handle(isai_handle_t h, isai_handle_t *out, isai_stream_t strea
     auto dmalloc = [&](void **p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 1) == hipSuccess; };
     const size_t b16 = 16 * sizeof(double);
     if (!dmalloc((void **)&g->S, sizeof(Scalars)) || !dmalloc((void **)&g->d_res2, 2 * sizeof(double)) ||
-        hipHostMalloc(&g->h_done, sizeof(int)) != hipSuccess || hipHostMalloc(&g->h_res2, 2 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&g->h_done, sizeof(int) * kStateInts) != hipSuccess || hipHostMalloc(&g->h_res2, 2 * sizeof(double)) != hipSuccess ||
         !dmalloc((void **)&g->own_row, sizeof(int) * (nb + 1)) || !dmalloc((void **)&g->own_col, sizeof(int) * nnzb) ||
         !dmalloc((void **)&g->own_val, b16 * nnzb) || !dmalloc((void **)&g->d_LU, b16 * nnzb)) {
         set_error("isai_adjoint_handle: allocation failed");

@@ -67,6 +67,92 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f64_kernel(
     if (br < nb && h == 0) y[(size_t)br * 4 + (s >> 1)] = sum;
 }
 
+constexpr int kDotChunks = 4;  // 32-row chunks per workgroup of the fused-dot product
+
+// FP64 product with fused dot products of the result (the Krylov loop's v = A p^
+// followed by rhat.v, and t = A s^ followed by t.s and t.t): every workgroup writes
+// its partial sums to part[wg] (and part[G + wg]), in a fixed order, so the final
+// sum over the G = gridDim.x partials is deterministic.
+template <int NDOT>
+__global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f64_dot_kernel(
+    int nb, const int *__restrict__ rp, const int *__restrict__ ci, const v2d *__restrict__ val2,
+    const v2d *__restrict__ x2, double *__restrict__ y, const double *__restrict__ a, double *__restrict__ part,
+    const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = threadIdx.x & 7;
+    const int h = s & 1;
+    double d0 = 0.0, d1 = 0.0;
+    for (int chunk = 0; chunk < kDotChunks; ++chunk) {
+        const int br = (wg * kDotChunks + chunk) * (kSpmvBlock / 8) + (threadIdx.x >> 3);
+        double acc0 = 0.0, acc1 = 0.0;
+        if (br < nb) {
+            const int rs = rp[br], re = rp[br + 1];
+            int k = rs;
+            for (; k + 2 <= re; k += 2) {
+                const int c0 = ci[k], c1 = ci[k + 1];
+                const v2d a0 = __builtin_nontemporal_load(&val2[(size_t)k * 8 + s]);
+                const v2d a1 = __builtin_nontemporal_load(&val2[(size_t)(k + 1) * 8 + s]);
+                const v2d x0 = x2[(size_t)c0 * 2 + h], x1 = x2[(size_t)c1 * 2 + h];
+                acc0 = fma(a0.x, x0.x, acc0);
+                acc1 = fma(a0.y, x0.y, acc1);
+                acc0 = fma(a1.x, x1.x, acc0);
+                acc1 = fma(a1.y, x1.y, acc1);
+            }
+            for (; k < re; ++k) {
+                const v2d av = __builtin_nontemporal_load(&val2[(size_t)k * 8 + s]);
+                const v2d xv = x2[(size_t)ci[k] * 2 + h];
+                acc0 = fma(av.x, xv.x, acc0);
+                acc1 = fma(av.y, xv.y, acc1);
+            }
+        }
+        double sum = acc0 + acc1;
+        sum += __shfl_xor(sum, 1);
+        if (br < nb && h == 0) {
+            const size_t row = (size_t)br * 4 + (s >> 1);
+            y[row] = sum;
+            d0 += sum * a[row];
+            if (NDOT == 2) d1 += sum * sum;
+        }
+    }
+    __shared__ double red[2][kSpmvBlock / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        d0 += __shfl_xor(d0, off);
+        if (NDOT == 2) d1 += __shfl_xor(d1, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = d0;
+        red[1][threadIdx.x >> 6] = d1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+        for (int w = 0; w < kSpmvBlock / 64; ++w) {
+            t0 += red[0][w];
+            t1 += red[1][w];
+        }
+        part[wg] = t0;
+        if (NDOT == 2) part[gridDim.x + wg] = t1;
+    }
+}
+
+int spmv_dot_groups(int mb) { return (int)grid_for(mb, kDotChunks * (kSpmvBlock / 8)); }
+
+int spmv_dot_launch(int mb, const int *rp, const int *ci, const double *val, const double *x, double *y,
+                    const double *a, int ndot, double *part, hipStream_t st, const int *skip) {
+    const unsigned g = (unsigned)spmv_dot_groups(mb);
+    if (ndot == 1)
+        bsr4_spmv_f64_dot_kernel<1><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v2d *)val, (const v2d *)x, y, a,
+                                                              part, skip);
+    else
+        bsr4_spmv_f64_dot_kernel<2><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v2d *)val, (const v2d *)x, y, a,
+                                                              part, skip);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
 // ------------------------------------------------- FP32 / BF16 values, bs = 4
 // Eight lanes per block row, each lane 16 B of the row's value array, so one load
 // instruction of the eight lanes covers one whole 128-byte line: BPB = 2 (FP32)

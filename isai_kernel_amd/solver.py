@@ -125,6 +125,12 @@ class Batch:
         call("isai_batch_tuning", self.h, C.byref(v), C.byref(us), C.byref(nc))
         return v.value, us.value, nc.value
 
+    def tuning_table(self, operand: int):
+        us = np.zeros(32)
+        n = C.c_int()
+        call("isai_batch_tuning_table", self.h, operand, us.ctypes.data, len(us), C.byref(n))
+        return us[:n.value]
+
     def setup_times(self):
         a, b = C.c_double(), C.c_double()
         call("isai_batch_setup_times", self.h, C.byref(a), C.byref(b))

@@ -66,7 +66,9 @@ def test_batch_layout_and_spmm(stream):
                 mask[o:o + 4 * int(sys_ptr[s + 1] - sys_ptr[s])] = False
         assert not Y[mask].any()
         v, us, nc = bt.tuning()
-        assert nc == 12 and 0 <= v < 6 and us > 0
+        assert nc == 24 and 0 <= v < 12 and us > 0
+        tab = bt.tuning_table(0)
+        assert len(tab) == 12 and (tab > 0).all() and tab[v] == pytest.approx(us) and tab.min() == pytest.approx(us)
     finally:
         bt.close()
 
