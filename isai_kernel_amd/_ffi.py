@@ -129,6 +129,7 @@ SIGNATURES: dict[str, list] = {
     "isai_batch_pairs": [P, P],
     "isai_batch_tuning": [P, P, P, P],
     "isai_batch_tuning_table": [P, I, P, I, P],
+    "isai_batch_select": [P, I, I],
     "isai_batch_setup_times": [P, P, P],
     "isai_batch_spmm": [P, P, P, P],
     "isai_batch_solve": [P, D, I, P, P, I, P, P, P, P],

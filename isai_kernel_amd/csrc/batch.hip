@@ -264,6 +264,164 @@ __device__ __forceinline__ void col_row(const RowCtx &q, const int *__restrict__
     }
 }
 
+// MFMA mapping.  v_mfma_f64_4x4x4f64 computes four independent 4x4x4 products; its
+// operands (measured on gfx950) sit at lane x + 4b + 16y with A[b][i=x][k=y],
+// B[b][k=y][j=x] and C[b][i=y][j=x].  One wave takes four block rows (b), every
+// step multiplies one block of each row by the 4x4 slice of X for four
+// right-hand sides (one MFMA per group of four), so each lane loads one value of
+// A and one x value per group: no value is loaded twice.  Rows shorter than the
+// longest of the four, and right-hand sides beyond a row's count, feed zeros.
+constexpr int kMfmaF64Rows = 4;    // block rows per wave, FP64 values
+constexpr int kMfmaLowpRows = 16;  // block rows per wave, FP32 / BF16 values
+
+struct MfmaRow {
+    int br, R, rs, len, base;
+    unsigned mask;
+    long long v0, step;
+};
+
+__device__ __forceinline__ void mfma_row(const BatchDev &B, const int *rp, int nb, int br, int all, MfmaRow &m) {
+    m.br = br;
+    m.R = m.rs = m.len = m.base = 0;
+    m.mask = 0;
+    m.v0 = m.step = 0;
+    if (br >= nb) return;
+    const int sys = B.sys_of_row[br];
+    m.R = B.sys_nrhs[sys];
+    m.mask = active_mask(B, B.sys_pair[sys], m.R, all);
+    if (!m.mask) return;
+    m.base = B.sys_ptr[sys];
+    m.v0 = B.sys_vec[sys];
+    m.step = B.sys_step[sys];
+    m.rs = rp[br];
+    m.len = rp[br + 1] - m.rs;
+}
+
+__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_f64_kernel(int nb, const int *__restrict__ rp,
+                                                                    const int *__restrict__ ci,
+                                                                    const double *__restrict__ val, BatchDev B,
+                                                                    const double *__restrict__ X,
+                                                                    double *__restrict__ Y, int remap, int all) {
+    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int x = lane & 3, b = (lane >> 2) & 3, y = lane >> 4;
+    MfmaRow m;
+    mfma_row(B, rp, nb, (wg * (int)(blockDim.x / 64) + (int)(threadIdx.x >> 6)) * kMfmaF64Rows + b, all, m);
+    // wave-uniform bounds: the longest row and the most right-hand-side groups
+    int L = m.len, G = m.len ? (m.R + 3) >> 2 : 0;
+    L = max(L, __shfl_xor(L, 4));
+    L = max(L, __shfl_xor(L, 8));
+    G = max(G, __shfl_xor(G, 4));
+    G = max(G, __shfl_xor(G, 8));
+    if (L == 0) return;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const double *xb = X + m.v0 + y;
+#pragma unroll 2
+    for (int t = 0; t < L; ++t) {
+        const bool live = t < m.len;
+        double a = 0.0;
+        int c = 0;
+        if (live) {
+            const int k = m.rs + t;
+            a = __builtin_nontemporal_load(&val[(size_t)k * 16 + x * 4 + y]);
+            c = ci[k] - m.base;
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            if (g < G) {
+                const int n = 4 * g + x;
+                const double bv = (live && n < m.R) ? xb[(size_t)c * 4 + n * m.step] : 0.0;
+                acc[g] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, bv, acc[g], 0, 0, 0);
+            }
+        }
+    }
+    if (!m.len) return;
+    double *yr = Y + m.v0 + (size_t)(m.br - m.base) * 4 + y;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const int n = 4 * g + x;
+        if (g < G && n < m.R && (m.mask & (1u << n))) yr[n * m.step] = acc[g];
+    }
+}
+
+// v_mfma_f32_4x4x1f32 (sixteen 4x4x1 products): A[b][i] at lane i + 4b, B[b][j] at
+// lane j + 4b, C[b][i][j] in element i of lane j + 4b.  One wave takes sixteen
+// block rows; lane (i, b) loads row i of its block (16 B FP32, 8 B BF16) and, as a
+// B-operand lane (j, b), the four x values of right-hand side j of each group
+// (rounded to FP32); four K = 1 MFMAs per group and block, FP32 accumulation.
+template <bool BF16>
+__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_lowp_kernel(int nb, const int *__restrict__ rp,
+                                                                     const int *__restrict__ ci,
+                                                                     const void *__restrict__ val, BatchDev B,
+                                                                     const double *__restrict__ X,
+                                                                     double *__restrict__ Y, int remap, int all) {
+    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int q = lane & 3, b = lane >> 2;  // q: row i for A, right-hand side j for B and C
+    MfmaRow m;
+    mfma_row(B, rp, nb, (wg * (int)(blockDim.x / 64) + (int)(threadIdx.x >> 6)) * kMfmaLowpRows + b, all, m);
+    int L = m.len, G = m.len ? (m.R + 3) >> 2 : 0;
+#pragma unroll
+    for (int off = 4; off < 64; off <<= 1) {
+        L = max(L, __shfl_xor(L, off));
+        G = max(G, __shfl_xor(G, off));
+    }
+    if (L == 0) return;
+    v4f acc[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) acc[g] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int t = 0; t < L; ++t) {
+        const bool live = t < m.len;
+        float a[4] = {0.f, 0.f, 0.f, 0.f};
+        int c = 0;
+        if (live) {
+            const int k = m.rs + t;
+            c = ci[k] - m.base;
+            if (BF16) {
+                const uint2 w = ((const uint2 *)val)[(size_t)k * 4 + q];
+                a[0] = __uint_as_float(w.x << 16);
+                a[1] = __uint_as_float(w.x & 0xffff0000u);
+                a[2] = __uint_as_float(w.y << 16);
+                a[3] = __uint_as_float(w.y & 0xffff0000u);
+            } else {
+                const v4f w = ((const v4f *)val)[(size_t)k * 4 + q];
+                a[0] = w.x;
+                a[1] = w.y;
+                a[2] = w.z;
+                a[3] = w.w;
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            if (g < G) {
+                const int n = 4 * g + q;
+                float xv[4] = {0.f, 0.f, 0.f, 0.f};
+                if (live && n < m.R) {
+                    const double *xp = X + m.v0 + n * m.step + (size_t)c * 4;
+                    const v2d x01 = *(const v2d *)xp, x23 = *(const v2d *)(xp + 2);
+                    xv[0] = (float)x01.x;
+                    xv[1] = (float)x01.y;
+                    xv[2] = (float)x23.x;
+                    xv[3] = (float)x23.y;
+                }
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[kk], xv[kk], acc[g], 0, 0, 0);
+            }
+        }
+    }
+    if (!m.len) return;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const int n = 4 * g + q;
+        if (g < G && n < m.R && (m.mask & (1u << n))) {
+            double *yr = Y + m.v0 + n * m.step + (size_t)(m.br - m.base) * 4;
+            *(v2d *)yr = v2d{(double)acc[g].x, (double)acc[g].y};
+            *(v2d *)(yr + 2) = v2d{(double)acc[g].z, (double)acc[g].w};
+        }
+    }
+}
+
 #define ISAI_RHS_CASES(CALL) \
     CALL(1) CALL(2) CALL(3) CALL(4) CALL(5) CALL(6) CALL(7) CALL(8) \
     CALL(9) CALL(10) CALL(11) CALL(12) CALL(13) CALL(14) CALL(15) CALL(16)
@@ -596,10 +754,10 @@ struct isai_batch {
     double *work = nullptr;  // kWork multi-vectors
     static constexpr int kWork = 9;
     // launch shapes picked by the autotuner: [0] product with A, [1] with the ISAI factors
-    int wg[2] = {256, 256}, remap[2] = {1, 1}, map[2] = {1, 1};
+    int wg[2] = {256, 256}, remap[2] = {1, 1}, map[2] = {2, 2};
     int variant = 0, ncand = 0;
     double tuned_us = 0;
-    double table[2][12] = {};  // autotuner: us per product, [operand][candidate]
+    double table[2][18] = {};  // autotuner: us per product, [operand][candidate]
     double ms_ilu = 0, ms_isai = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -633,15 +791,25 @@ void launch_spmm(isai_batch *b, Operand op, int prec, const int *rp, const int *
                  double *Y, int all, hipStream_t st) {
     const BatchDev B = b->dev();
     const int slot = op == kOpA ? 0 : 1;
-    const unsigned grid = grid_for(b->nb, b->wg[slot] / 8);
-    const unsigned threads = (unsigned)b->wg[slot];
+    const int threads = b->wg[slot];
+    const int rows = MAP < 2 ? threads / 8
+                             : (threads / 64) * (prec == ISAI_PREC_FP64 ? kMfmaF64Rows : kMfmaLowpRows);
+    const unsigned grid = grid_for(b->nb, rows);
     const int remap = b->remap[slot];
-    if (prec == ISAI_PREC_FP64)
+    if (MAP == 2) {
+        if (prec == ISAI_PREC_FP64)
+            bspmm_mfma_f64_kernel<<<grid, threads, 0, st>>>(b->nb, rp, ci, (const double *)v, B, X, Y, remap, all);
+        else if (prec == ISAI_PREC_FP32)
+            bspmm_mfma_lowp_kernel<false><<<grid, threads, 0, st>>>(b->nb, rp, ci, v, B, X, Y, remap, all);
+        else
+            bspmm_mfma_lowp_kernel<true><<<grid, threads, 0, st>>>(b->nb, rp, ci, v, B, X, Y, remap, all);
+    } else if (prec == ISAI_PREC_FP64) {
         bspmm_f64_kernel<MAP><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v2d *)v, B, X, Y, remap, all);
-    else if (prec == ISAI_PREC_FP32)
+    } else if (prec == ISAI_PREC_FP32) {
         bspmm_lowp_kernel<false, MAP><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, remap, all);
-    else
+    } else {
         bspmm_lowp_kernel<true, MAP><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, remap, all);
+    }
 }
 
 int spmm(isai_batch *b, Operand op, const double *X, double *Y, int all, hipStream_t st) {
@@ -657,8 +825,10 @@ int spmm(isai_batch *b, Operand op, const double *X, double *Y, int all, hipStre
         v = op == kOpML ? f.Lval : f.Uval;
         prec = f.precision;
     }
-    if (b->map[op == kOpA ? 0 : 1] == 0) launch_spmm<0>(b, op, prec, rp, ci, v, X, Y, all, st);
-    else launch_spmm<1>(b, op, prec, rp, ci, v, X, Y, all, st);
+    const int map = b->map[op == kOpA ? 0 : 1];
+    if (map == 0) launch_spmm<0>(b, op, prec, rp, ci, v, X, Y, all, st);
+    else if (map == 1) launch_spmm<1>(b, op, prec, rp, ci, v, X, Y, all, st);
+    else launch_spmm<2>(b, op, prec, rp, ci, v, X, Y, all, st);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
@@ -702,9 +872,10 @@ int batch_iteration(isai_batch *b, double *x, hipStream_t st) {
 // Launch-shape autotuner: for the product with A (FP64) and with the ISAI factors
 // (storage precision) time every (lane mapping, workgroup size, XCD remap)
 // candidate on the batch itself, all pairs active, and keep the fastest of each.
-// Candidate c: mapping c / 6, workgroup kWgCand[(c / 2) % 3], remap c % 2.
+// Candidate c: mapping c / 6 (0 row-pair lanes, 1 column-split lanes, 2 MFMA),
+// workgroup kWgCand[(c / 2) % 3], remap c % 2.
 constexpr int kWgCand[3] = {64, 128, kSpmmMaxWg};
-constexpr int kCandPerOp = 12;
+constexpr int kCandPerOp = 18;
 
 int autotune(isai_batch *b, hipStream_t st) {
     double *X = b->work, *Y = b->work + b->len;
@@ -950,6 +1121,20 @@ int isai_batch_tuning_table(isai_batch_t b, int operand, double *us, int capacit
     ISAI_REQUIRE(b && (operand == 0 || operand == 1), "isai_batch_tuning_table: bad argument");
     if (count) *count = kCandPerOp;
     for (int c = 0; c < kCandPerOp && c < capacity && us; ++c) us[c] = b->table[operand][c];
+    return ISAI_OK;
+}
+
+int isai_batch_select(isai_batch_t b, int operand, int candidate) {
+    ISAI_REQUIRE(b && (operand == 0 || operand == 1) && candidate >= 0 && candidate < kCandPerOp,
+                 "isai_batch_select: bad operand %d / candidate %d", operand, candidate);
+    b->map[operand] = candidate / 6;
+    b->wg[operand] = kWgCand[(candidate / 2) % 3];
+    b->remap[operand] = candidate % 2;
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->g_x = nullptr;
     return ISAI_OK;
 }
 

@@ -131,6 +131,9 @@ class Batch:
         call("isai_batch_tuning_table", self.h, operand, us.ctypes.data, len(us), C.byref(n))
         return us[:n.value]
 
+    def select(self, operand: int, candidate: int) -> None:
+        call("isai_batch_select", self.h, operand, candidate)
+
     def setup_times(self):
         a, b = C.c_double(), C.c_double()
         call("isai_batch_setup_times", self.h, C.byref(a), C.byref(b))

@@ -66,9 +66,9 @@ def test_batch_layout_and_spmm(stream):
                 mask[o:o + 4 * int(sys_ptr[s + 1] - sys_ptr[s])] = False
         assert not Y[mask].any()
         v, us, nc = bt.tuning()
-        assert nc == 24 and 0 <= v < 12 and us > 0
+        assert nc == 36 and 0 <= v < 18 and us > 0
         tab = bt.tuning_table(0)
-        assert len(tab) == 12 and (tab > 0).all() and tab[v] == pytest.approx(us) and tab.min() == pytest.approx(us)
+        assert len(tab) == 18 and (tab > 0).all() and tab[v] == pytest.approx(us) and tab.min() == pytest.approx(us)
     finally:
         bt.close()
 
@@ -154,3 +154,38 @@ def test_batch_iteration_limit_and_errors(stream):
     st = lib.isai_batch_setup(2, bad.ctypes.data, nr.ctypes.data, len(ci2), A2.d_rp.ptr, A2.d_ci.ptr,
                               A2.d_val.ptr, 0, ctypes.byref(h), stream)
     assert st == -4 and not h.value
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
+def test_batch_every_spmm_candidate(stream, precision):
+    """Each lane mapping / workgroup / remap candidate: the A product against the
+    oracle, and a solve whose ISAI products use that candidate."""
+    print(f"batch spmm candidates {precision}", flush=True)
+    (rp, ci, val, sys_ptr, nrhs), bt = _make(5, 3, 12, 21, 16, precision, stream)
+    try:
+        X = _rhs(sys_ptr, nrhs, 4)
+        d_X = DeviceArray.from_numpy(bt.pack(X))
+        d_Y = DeviceArray.zeros((bt.length,), np.float64)
+        B = DeviceArray.from_numpy(bt.pack(_rhs(sys_ptr, nrhs, 5)))
+        refs = [[O.spmv(*_system(rp, ci, val, sys_ptr, s), X[s][j]) for j in range(nrhs[s])] for s in range(len(nrhs))]
+        its = []
+        for c in range(18):
+            bt.select(0, c)
+            bt.select(1, c)
+            bt.spmm(d_X, d_Y)
+            call("isai_stream_synchronize", stream)
+            got = bt.unpack(d_Y.to_numpy())
+            for s in range(len(nrhs)):
+                for j in range(nrhs[s]):
+                    assert rel_err(got[s][j], refs[s][j]) <= 1e-14, (c, s, j)
+            d_x = DeviceArray.zeros((bt.length,), np.float64)
+            st, iters, rr = bt.solve(B, d_x, tol=1e-10, maxit=500)
+            assert st == 0 and (rr <= 1e-10).all(), c
+            its.append(iters)
+        # FP64 factors: every candidate's products agree to rounding, so the
+        # iteration counts agree within one
+        if precision == "fp64":
+            for it in its[1:]:
+                assert np.abs(it.astype(int) - its[0].astype(int)).max() <= 1
+    finally:
+        bt.close()
