@@ -380,7 +380,9 @@ def main() -> int:
         ms = timer.time(one_solve, args.steps)
         wall = (time.perf_counter() - t0) / args.steps * 1e3
         x = d_x.to_numpy()
-        solves[form] = {"ms": round(ms, 3), "wall_ms": round(wall, 3), "iters": state["it"],
+        setup = ms_ilu + (ms_isai if pc == "isai" else 0.0)
+        solves[form] = {"ms": round(ms, 3), "wall_ms": round(wall, 3), "ms_with_setup": round(ms + setup, 3),
+                        "iters": state["it"],
                         "relres": state["rr"], "status": state["st"],
                         "max_err_vs_ones": float(np.max(np.abs(x - 1.0)))}
         log(f"solve {form}: {ms:.2f} ms ({state['it']} it, relres {state['rr']:.2e}, status {state['st']})")
