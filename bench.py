@@ -264,6 +264,8 @@ def main() -> int:
     ap.add_argument("--chunk", type=int, default=0, help="rows per wave of the sync-free solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-forms", action="store_true", help="skip the precision / batching / differentiation legs")
+    ap.add_argument("--leg", choices=["precision", "batching", "differentiation"],
+                    help="run only this form's leg and print its object (for profiler runs)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the kernels only (no solves, no CPU leg): for profiler runs")
     args = ap.parse_args()
@@ -279,6 +281,17 @@ def main() -> int:
     cus = ctypes.c_int()
     call("isai_device_info", name, 256, ctypes.byref(cus), None)
     log(f"device {name.value.decode()} CUs={cus.value}")
+
+    if args.leg:
+        stream = ctypes.c_void_p()
+        call("isai_stream_create", ctypes.byref(stream))
+        timer = Timer(lib, stream)
+        fsteps = max(1, min(args.steps, 3))
+        leg = {"precision": lambda: bench_precision(timer, stream, fsteps),
+               "batching": lambda: bench_batching(stream, fsteps),
+               "differentiation": lambda: bench_differentiation(timer, stream, fsteps)}[args.leg]()
+        print(json.dumps({"leg": args.leg, args.leg: leg}), flush=True)
+        return 0
 
     t0 = time.perf_counter()
     rp, ci, val = problems.generate(args.problem, 0, seed=0)

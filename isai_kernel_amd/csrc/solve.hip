@@ -44,7 +44,9 @@ struct Scalars {
     int first;
     int iters, maxit;  // BiCGSTAB iteration count and limit (device side)
     int jend;          // FGMRES: Arnoldi steps of the current cycle
-    int pad[3];
+    int reorth;        // FGMRES: 1 when the current step needs the second Gram-Schmidt pass
+    int pad[2];
+    double wnorm2;     // FGMRES: ||w||^2 before the first Gram-Schmidt pass
     double partial[2][kRedBlocks];
     double mpart[(kMaxRestart + 1) * kRedBlocks];  // multi-dot partials, [vector][block]
 };
@@ -133,21 +135,33 @@ __global__ __launch_bounds__(kRedThreads) void bicg_init_final(Scalars *S, doubl
     }
 }
 
-// Sum of G workgroup partials in a fixed order (one block); eight independent
-// accumulators per thread keep eight loads in flight.
+// Sum of G workgroup partials in a fixed order by one block of kWgSumThreads;
+// eight independent accumulators per thread keep eight loads in flight.
+constexpr int kWgSumThreads = 1024;
+
 __device__ __forceinline__ double wg_partial_sum(const double *part, int G) {
     double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int k = threadIdx.x;
-    for (; k + 7 * kRedThreads < G; k += 8 * kRedThreads) {
+    for (; k + 7 * kWgSumThreads < G; k += 8 * kWgSumThreads) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] += part[k + u * kRedThreads];
+        for (int u = 0; u < 8; ++u) v[u] += part[k + u * kWgSumThreads];
     }
-    for (; k < G; k += kRedThreads) v[0] += part[k];
-    return block_sum(((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7])));
+    for (; k < G; k += kWgSumThreads) v[0] += part[k];
+    double t = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+    __shared__ double ws[kWgSumThreads / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) t += __shfl_xor(t, off);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = t;
+    __syncthreads();
+    double sum = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWgSumThreads / 64; ++w) sum += ws[w];
+    return sum;
 }
 
 // alpha = rho / rhat.v, with rhat.v from the product's fused partials
-__global__ __launch_bounds__(kRedThreads) void bicg_alpha_final_wg(Scalars *S, const double *part, int G) {
+__global__ __launch_bounds__(kWgSumThreads) void bicg_alpha_final_wg(Scalars *S, const double *part, int G) {
     if (skipped(S)) return;
     const double d = wg_partial_sum(part, G);
     if (threadIdx.x == 0) {
@@ -159,7 +173,7 @@ __global__ __launch_bounds__(kRedThreads) void bicg_alpha_final_wg(Scalars *S, c
 }
 
 // omega = t.s / t.t from the product's fused partials
-__global__ __launch_bounds__(kRedThreads) void bicg_omega_final_wg(Scalars *S, const double *part, int G) {
+__global__ __launch_bounds__(kWgSumThreads) void bicg_omega_final_wg(Scalars *S, const double *part, int G) {
     if (skipped(S)) return;
     const double ts = wg_partial_sum(part, G), tt = wg_partial_sum(part + G, G);
     if (threadIdx.x == 0) {
@@ -314,20 +328,30 @@ __global__ void fg_scale_kernel(size_t n, const double *__restrict__ w, double *
         v[i] = w[i] / d;
 }
 
-// Classical Gram-Schmidt, applied twice (CGS2): each pass computes h = V^T w with
-// one multi-dot (up to kDotChunk vectors per block column of the grid), sums the
-// partials per vector in a fixed order, and subtracts V h from w in one pass.
+// Classical Gram-Schmidt with a conditional second pass (DGKS criterion, Daniel,
+// Gragg, Kaufman and Stewart 1976): each pass computes h = V^T w with one
+// multi-dot (up to kDotChunk vectors per block column of the grid), sums the
+// partials per vector in a fixed order, and subtracts V h from w in one pass.  The
+// second pass runs only when the first shrank ||w|| below ||w_before|| / sqrt(2);
+// its kernels return at once otherwise.  Pass 0 also measures ||w_before||^2.
 constexpr int kDotChunk = 8;
 
+__device__ __forceinline__ bool pass_skipped(const Scalars *S, int pass) {
+    return S->done != 0 || (pass == 1 && !S->reorth);
+}
+
 __global__ __launch_bounds__(kRedThreads) void fg_mdot_kernel(size_t n, const double *__restrict__ w,
-                                                              const double *__restrict__ V, int nv, Scalars *S) {
-    if (skipped(S)) return;
+                                                              const double *__restrict__ V, int nv, Scalars *S,
+                                                              int pass) {
+    if (pass_skipped(S, pass)) return;
     const int i0 = blockIdx.y * kDotChunk;
-    double acc[kDotChunk];
+    const bool self = pass == 0 && blockIdx.y == 0;  // this block column also sums w.w
+    double acc[kDotChunk], ww = 0.0;
 #pragma unroll
     for (int u = 0; u < kDotChunk; ++u) acc[u] = 0.0;
     for (size_t k = (size_t)blockIdx.x * kRedThreads + threadIdx.x; k < n; k += (size_t)kRedBlocks * kRedThreads) {
         const double wk = w[k];
+        if (self) ww += wk * wk;
 #pragma unroll
         for (int u = 0; u < kDotChunk; ++u)
             if (i0 + u < nv) acc[u] += V[(size_t)(i0 + u) * n + k] * wk;
@@ -337,26 +361,34 @@ __global__ __launch_bounds__(kRedThreads) void fg_mdot_kernel(size_t n, const do
         const double t = block_sum(acc[u]);
         if (threadIdx.x == 0 && i0 + u < nv) S->mpart[(size_t)(i0 + u) * kRedBlocks + blockIdx.x] = t;
     }
+    if (self) {
+        ww = block_sum(ww);
+        if (threadIdx.x == 0) S->partial[1][blockIdx.x] = ww;
+    }
 }
 
-// block i: h_i = sum of its partials; pass 0 sets H[i][j], pass 1 adds to it
+// block i: h_i = sum of its partials; pass 0 sets H[i][j] (and block 0 stores
+// ||w_before||^2), pass 1 adds to H[i][j]
 __global__ __launch_bounds__(kRedThreads) void fg_hsum_final(Scalars *S, int j, int pass) {
-    if (skipped(S)) return;
+    if (pass_skipped(S, pass)) return;
     const int i = blockIdx.x;
     double v = 0.0;
     for (int k = threadIdx.x; k < kRedBlocks; k += kRedThreads) v += S->mpart[(size_t)i * kRedBlocks + k];
     v = block_sum(v);
+    double wb = 0.0;
+    if (pass == 0 && i == 0) wb = final_sum(S, 1);
     if (threadIdx.x == 0) {
         S->hcur[i] = v;
         if (pass == 0) S->H[i * kMaxRestart + j] = v;
         else S->H[i * kMaxRestart + j] += v;
+        if (pass == 0 && i == 0) S->wnorm2 = wb;
     }
 }
 
-// w -= sum_i hcur[i] V_i (i ascending); with_norm: partial ||w||^2
+// w -= sum_i hcur[i] V_i (i ascending); partial ||w||^2
 __global__ __launch_bounds__(kRedThreads) void fg_axpy_multi_kernel(size_t n, double *w, const double *__restrict__ V,
-                                                                    int nv, Scalars *S, int with_norm) {
-    if (skipped(S)) return;
+                                                                    int nv, Scalars *S, int pass) {
+    if (pass_skipped(S, pass)) return;
     double acc = 0.0;
     for (size_t k = (size_t)blockIdx.x * kRedThreads + threadIdx.x; k < n; k += (size_t)kRedBlocks * kRedThreads) {
         double wk = w[k];
@@ -364,10 +396,15 @@ __global__ __launch_bounds__(kRedThreads) void fg_axpy_multi_kernel(size_t n, do
         w[k] = wk;
         acc += wk * wk;
     }
-    if (with_norm) {
-        acc = block_sum(acc);
-        if (threadIdx.x == 0) S->partial[0][blockIdx.x] = acc;
-    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) S->partial[0][blockIdx.x] = acc;
+}
+
+// After pass 0: second pass needed when ||w||^2 < ||w_before||^2 / 2.
+__global__ __launch_bounds__(kRedThreads) void fg_reorth_check(Scalars *S) {
+    if (skipped(S)) return;
+    const double w2 = final_sum(S, 0);
+    if (threadIdx.x == 0) S->reorth = (w2 < 0.5 * S->wnorm2) ? 1 : 0;
 }
 
 // ||w|| -> H[j+1][j]; Givens rotations on column j; residual estimate |g[j+1]|.
@@ -580,13 +617,13 @@ int bicgstab(isai_handle *h, double tol, int maxit, const double *b, double *x, 
                 bicg_p_kernel<<<vec_grid(n), 256, 0, st>>>(n, r, p, v, h->S);
                 TRY(precond_apply(h, p, ph, tmp, st, skip));
                 TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, ph, v, rh, 1, part, st, skip));
-                bicg_alpha_final_wg<<<1, kRedThreads, 0, st>>>(h->S, part, G);
+                bicg_alpha_final_wg<<<1, kWgSumThreads, 0, st>>>(h->S, part, G);
                 bicg_s_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, r, v, s, h->S);
                 bicg_snorm_final<<<1, kRedThreads, 0, st>>>(h->S);
                 bicg_xs_kernel<<<vec_grid(n), 256, 0, st>>>(n, x, ph, h->S);
                 TRY(precond_apply(h, s, sh, tmp, st, skip));
                 TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, sh, t, s, 2, part, st, skip));
-                bicg_omega_final_wg<<<1, kRedThreads, 0, st>>>(h->S, part, G);
+                bicg_omega_final_wg<<<1, kWgSumThreads, 0, st>>>(h->S, part, G);
                 bicg_xr_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, x, r, ph, sh, s, t, rh, h->S);
                 bicg_rnorm_final<<<1, kRedThreads, 0, st>>>(h->S);
                 ISAI_LAUNCH_CHECK();
@@ -634,9 +671,10 @@ int fgmres(isai_handle *h, int m, double tol, int maxit, const double *b, double
             const int nv = j + 1;
             const dim3 dgrid(kRedBlocks, (nv + kDotChunk - 1) / kDotChunk);
             for (int pass = 0; pass < 2; ++pass) {
-                fg_mdot_kernel<<<dgrid, kRedThreads, 0, st>>>(n, w, V, nv, h->S);
+                fg_mdot_kernel<<<dgrid, kRedThreads, 0, st>>>(n, w, V, nv, h->S, pass);
                 fg_hsum_final<<<nv, kRedThreads, 0, st>>>(h->S, j, pass);
                 fg_axpy_multi_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, V, nv, h->S, pass);
+                if (pass == 0) fg_reorth_check<<<1, kRedThreads, 0, st>>>(h->S);
             }
             fg_givens_final<<<1, kRedThreads, 0, st>>>(h->S, j);
             fg_scale_next_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V + n * (j + 1), h->S);

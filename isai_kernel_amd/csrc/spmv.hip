@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f64_kernel(
     if (br < nb && h == 0) y[(size_t)br * 4 + (s >> 1)] = sum;
 }
 
-constexpr int kDotChunks = 4;  // 32-row chunks per workgroup of the fused-dot product
+constexpr int kDotChunks = 1;  // 32-row chunks per workgroup of the fused-dot product
 
 // FP64 product with fused dot products of the result (the Krylov loop's v = A p^
 // followed by rhat.v, and t = A s^ followed by t.s and t.t): every workgroup writes
