@@ -60,15 +60,26 @@ int isai_handle_info(isai_handle_t handle, int *nnzbL, int *nnzbU, int *nlevels)
 int isai_handle_factors(isai_handle_t handle, const int **Lrow, const int **Lcol, const double **Lval,
                         const int **Urow, const int **Ucol, const double **Uval, const double **LU);
 
-/* d_z = M d_v with the handle's current preconditioner.  Asynchronous. */
+/* d_z = M d_v with the handle's current preconditioner.  Asynchronous; with the
+ * ILU(0) preconditioner a triangular-solve wait that ran out is reported by
+ * isai_handle_status (and by the next isai_solve). */
 int isai_precond_apply(isai_handle_t handle, const double *d_v, double *d_z, isai_stream_t stream);
+
+/* Synchronizes the stream; ISAI_ERR_TIMEOUT if a sync-free triangular solve of the
+ * handle ran out of its wait budget since the last check (the flag is then cleared). */
+int isai_handle_status(isai_handle_t handle, isai_stream_t stream);
+
+/* Bound on the polls of one dependency of the handle's sync-free triangular solves
+ * (ILU(0) preconditioner); see isai_bsr_sptrsv_set_spin_limit. */
+int isai_handle_set_spin_limit(isai_handle_t handle, int limit);
 
 /* Solve A x = b.  d_x holds the initial guess on entry.  Stops when the
  * recursive residual estimate reaches tol * ||b|| (then confirmed with the true
  * FP64 residual, iterating on if it is not met) or after maxit iterations.
  * restart = FGMRES cycle length (ignored by BiCGSTAB).  *iters = iterations
  * (matrix products with A), *relres = ||b - A x|| / ||b|| computed in FP64.
- * Returns ISAI_ERR_NOT_CONVERGED when maxit is reached. */
+ * Returns ISAI_ERR_NOT_CONVERGED when maxit is reached, ISAI_ERR_TIMEOUT when a
+ * sync-free triangular solve of the ILU(0) preconditioner ran out of its wait budget. */
 int isai_solve(isai_handle_t handle, int solver, int restart, double tol, int maxit,
                const double *d_b, double *d_x, int *iters, double *relres, isai_stream_t stream);
 

@@ -116,6 +116,8 @@ SIGNATURES: dict[str, list] = {
     "isai_handle_info": [P, P, P, P],
     "isai_handle_factors": [P, P, P, P, P, P, P, P],
     "isai_precond_apply": [P, P, P, P],
+    "isai_handle_status": [P, P],
+    "isai_handle_set_spin_limit": [P, I],
     "isai_solve": [P, I, I, D, I, P, P, P, P, P],
     "isai_destroy": [P],
     # isai_adjoint.h

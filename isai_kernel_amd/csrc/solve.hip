@@ -1104,10 +1104,32 @@ int isai_solve(isai_handle_t h, int solver, int restart, double tol, int maxit, 
     *iters = 0;
     *relres = INFINITY;
     hipStream_t st = (hipStream_t)stream;
-    if (solver == ISAI_SOLVER_BICGSTAB) return bicgstab(h, tol, maxit, d_b, d_x, iters, relres, st);
-    if (solver == ISAI_SOLVER_FGMRES) return fgmres(h, restart, tol, maxit, d_b, d_x, iters, relres, st);
-    set_error("isai_solve: unknown solver %d", solver);
-    return ISAI_ERR_INVALID_ARG;
+    int s;
+    if (solver == ISAI_SOLVER_BICGSTAB) {
+        s = bicgstab(h, tol, maxit, d_b, d_x, iters, relres, st);
+    } else if (solver == ISAI_SOLVER_FGMRES) {
+        s = fgmres(h, restart, tol, maxit, d_b, d_x, iters, relres, st);
+    } else {
+        set_error("isai_solve: unknown solver %d", solver);
+        return ISAI_ERR_INVALID_ARG;
+    }
+    // a triangular-solve wait that gave up makes the iterates meaningless
+    const int t = isai_handle_status(h, stream);
+    return t != ISAI_OK ? t : s;
+}
+
+int isai_handle_status(isai_handle_t h, isai_stream_t stream) {
+    ISAI_REQUIRE(h, "isai_handle_status: null handle");
+    if (h->sv) return isai_sptrsv_status(h->sv, stream);
+    ISAI_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return ISAI_OK;
+}
+
+int isai_handle_set_spin_limit(isai_handle_t h, int limit) {
+    ISAI_REQUIRE(h && limit > 0, "isai_handle_set_spin_limit: bad argument");
+    ISAI_REQUIRE(h->sv, "isai_handle_set_spin_limit: the handle has no triangular-solve plan "
+                        "(set the ILU(0) preconditioner first)");
+    return isai_bsr_sptrsv_set_spin_limit(h->sv, limit);
 }
 
 }  // extern "C"

@@ -6,7 +6,7 @@ import pytest
 
 import _oracle as O
 from gpu_pipeline import dev, gpu_ilu, rel_err, split_lu
-from isai_kernel_amd import DeviceArray, call, problems, ptr
+from isai_kernel_amd import DeviceArray, call, lib, problems, ptr
 
 pytestmark = pytest.mark.gpu
 RNG = np.random.default_rng(77)
@@ -98,3 +98,59 @@ def test_bsr_sptrsv_full_size_residual(gpu):
     Ls, Us = split_lu(rp, ci, d_lu.to_numpy())
     r = Ls @ (Us @ x) - b
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-13
+
+
+def test_spin_limit_timeout_is_reported_and_cleared(gpu):
+    """A wait budget of one poll makes dependent chunks give up (bounded waits: every
+    kernel still exits); the plan reports ISAI_ERR_TIMEOUT once, then solves
+    correctly again with the default budget."""
+    print("bsr sptrsv forced timeout (spin limit 1), problem 1 L=48", flush=True)
+    rp, ci, val = problems.generate(1, 48, seed=6)
+    d_rp, d_ci, d_lu = gpu_ilu(rp, ci, val)
+    n = (len(rp) - 1) * 4
+    b = RNG.standard_normal(n)
+    plan = _plan(rp, ci, d_rp, d_ci, d_lu, 0)
+    try:
+        d_b, d_x = dev(b), DeviceArray((n,), np.float64)
+        call("isai_bsr_sptrsv_set_spin_limit", plan, 1)
+        statuses = []
+        for _ in range(3):
+            call("isai_bsr_sptrsv_solve", plan, d_lu.ptr, d_b.ptr, d_x.ptr, None)
+            statuses.append(lib.isai_sptrsv_status(plan, None))
+            if statuses[-1] != 0:
+                break
+        assert statuses[-1] == -7, statuses
+        assert lib.isai_sptrsv_status(plan, None) == 0  # cleared by the read
+        call("isai_bsr_sptrsv_set_spin_limit", plan, 1 << 16)
+        call("isai_bsr_sptrsv_solve", plan, d_lu.ptr, d_b.ptr, d_x.ptr, None)
+        call("isai_sptrsv_status", plan, None)
+        assert rel_err(d_x.to_numpy(), O.sptrsv_lu(rp, ci, d_lu.to_numpy(), b)) <= TOL
+    finally:
+        call("isai_bsr_sptrsv_destroy", plan)
+
+
+def test_solver_reports_triangular_solve_timeout(gpu):
+    """isai_solve with the ILU(0) preconditioner returns ISAI_ERR_TIMEOUT when a wait
+    of its triangular solves ran out (forced from the host with a one-poll budget)."""
+    print("solver ILU-SpTRSV forced timeout, problem 1 L=48", flush=True)
+    from isai_kernel_amd.solver import Handle, Matrix
+    rp, ci, val = problems.generate(1, 48, seed=6)
+    A = Matrix(rp, ci, val)
+    h = Handle(A, precond="ilu")
+    try:
+        call("isai_handle_set_spin_limit", h.h, 1)
+        d_b = dev(RNG.standard_normal(A.n))
+        st = None
+        for _ in range(3):
+            d_x = DeviceArray.zeros((A.n,), np.float64)
+            st, it, rr = h.solve(d_b, d_x, "bicgstab", tol=1e-10, maxit=5, check=False)
+            if st == -7:
+                break
+        assert st == -7
+        assert lib.isai_handle_status(h.h, None) == 0
+        call("isai_handle_set_spin_limit", h.h, 1 << 16)
+        d_x = DeviceArray.zeros((A.n,), np.float64)
+        st, it, rr = h.solve(d_b, d_x, "bicgstab", tol=1e-10, maxit=500)
+        assert st == 0 and rr <= 1e-10
+    finally:
+        h.close()
