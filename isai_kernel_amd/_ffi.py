@@ -72,6 +72,8 @@ SIGNATURES: dict[str, list] = {
     # csr2bsr.h
     "isai_csr_to_bsr_count": [I, I, P, P, P, P],
     "isai_csr_to_bsr_fill": [I, I, P, P, P, P, P, P],
+    "isai_csr_to_bsr_device_count": [I, I, P, P, P, P, P],
+    "isai_csr_to_bsr_device_fill": [I, I, P, P, P, P, P, P, P],
     # read_matrix.h
     "isai_read_matrix_market": [C.c_char_p, P, P, P, P, P, P],
     "isai_rel_errors_csr": [I, P, P, P, P, P, P, P, P],

@@ -316,7 +316,6 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_f64_kernel(int nb, cons
     if (L == 0) return;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const double *xb = X + m.v0 + y;
-#pragma unroll 2
     for (int t = 0; t < L; ++t) {
         const bool live = t < m.len;
         double a = 0.0;
@@ -370,7 +369,6 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_lowp_kernel(int nb, con
     v4f acc[4];
 #pragma unroll
     for (int g = 0; g < 4; ++g) acc[g] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
     for (int t = 0; t < L; ++t) {
         const bool live = t < m.len;
         float a[4] = {0.f, 0.f, 0.f, 0.f};

@@ -1,0 +1,235 @@
+// convert.hip -- device CSR -> BSR conversion (the device form of csr_to_bsr5,
+// reference csr2bsr.h:14-76), bit-identical to the host form.
+//
+// One 64-lane workgroup per block row.  The block columns of the row's entries
+// are sorted in LDS (bitonic) and compacted; each block's first entry in CSR
+// order is found with an LDS atomicMin; then lane lr walks scalar row lr in CSR
+// order and, for each entry, assigns (first entry of its block) or adds (any other
+// entry) into the block held in LDS -- the host form's traversal, so every value
+// sees the same operations in the same order.  LDS is sized on the host from the
+// longest block row.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <vector>
+
+#include "common.hpp"
+#include "csr2bsr.h"
+#include "scan.hpp"
+
+namespace isai {
+
+constexpr int kC2BMaxEntries = 16384;  // CSR entries of one block row
+
+__device__ __forceinline__ int c2b_pow2(int m) {
+    int p = 1;
+    while (p < m) p <<= 1;
+    return p;
+}
+
+// Ascending bitonic sort of keys[0, m2) by one 64-lane workgroup (m2 a power of two).
+__device__ void c2b_sort(int *keys, int m2) {
+    for (int k = 2; k <= m2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < m2; t += 64) {
+                const int u = t ^ j;
+                if (u > t) {
+                    const bool up = (t & k) == 0;
+                    const int a = keys[t], b = keys[u];
+                    if ((a > b) == up) {
+                        keys[t] = b;
+                        keys[u] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ int c2b_lower_bound(const int *a, int n, int key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Loads and sorts the block columns of block row I; returns the number of unique
+// block columns, compacted into ukeys[0, u).  keys and ukeys hold m2 ints.
+__device__ int c2b_unique(int bs, const int *__restrict__ ci, int e0, int m, int m2, int *keys, int *ukeys) {
+    for (int k = threadIdx.x; k < m2; k += 64) keys[k] = k < m ? ci[e0 + k] / bs : INT_MAX;
+    __syncthreads();
+    c2b_sort(keys, m2);
+    // compaction: each lane takes a contiguous segment of the sorted keys
+    const int seg = (m + 63) / 64;
+    const int s0 = min(m, (int)threadIdx.x * seg), s1 = min(m, s0 + seg);
+    int c = 0;
+    for (int k = s0; k < s1; ++k) c += (k == 0 || keys[k] != keys[k - 1]);
+    int incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off);
+        if ((int)threadIdx.x >= off) incl += t;
+    }
+    int w = incl - c;
+    for (int k = s0; k < s1; ++k)
+        if (k == 0 || keys[k] != keys[k - 1]) ukeys[w++] = keys[k];
+    const int u = __shfl(incl, 63);
+    __syncthreads();
+    return u;
+}
+
+__global__ __launch_bounds__(64) void c2b_count_kernel(int nb, int bs, const int *__restrict__ rp,
+                                                       const int *__restrict__ ci, int m2max, int *counts) {
+    extern __shared__ int lds[];
+    const int I = blockIdx.x;
+    const int e0 = rp[I * bs], m = rp[(I + 1) * bs] - e0;
+    const int m2 = c2b_pow2(max(m, 1));
+    const int u = c2b_unique(bs, ci, e0, m, m2, lds, lds + m2max);
+    if (threadIdx.x == 0) counts[I] = u;
+}
+
+__global__ __launch_bounds__(64) void c2b_fill_kernel(int nb, int bs, const int *__restrict__ rp,
+                                                      const int *__restrict__ ci, const double *__restrict__ v,
+                                                      const int *__restrict__ brp, int *__restrict__ bci,
+                                                      double *__restrict__ bv, int m2max, int umax, int *err) {
+    extern __shared__ int lds[];
+    int *keys = lds, *ukeys = lds + m2max, *first = ukeys + m2max;
+    double *vals = (double *)(first + ((umax + 1) & ~1));  // 8-byte aligned
+    const int I = blockIdx.x;
+    const int e0 = rp[I * bs], m = rp[(I + 1) * bs] - e0;
+    const int m2 = c2b_pow2(max(m, 1));
+    const int bs2 = bs * bs;
+    const int u = c2b_unique(bs, ci, e0, m, m2, keys, ukeys);
+    if (u != brp[I + 1] - brp[I]) {  // row pointer not from the count pass: write nothing
+        if (threadIdx.x == 0) atomicOr(err, 1);
+        return;
+    }
+    for (int q = threadIdx.x; q < u; q += 64) first[q] = INT_MAX;
+    for (int t = threadIdx.x; t < u * bs2; t += 64) vals[t] = 0.0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < m; k += 64) {
+        const int q = c2b_lower_bound(ukeys, u, ci[e0 + k] / bs);
+        atomicMin(&first[q], e0 + k);
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < bs) {
+        const int lr = threadIdx.x, row = I * bs + lr;
+        for (int p = rp[row]; p < rp[row + 1]; ++p) {
+            const int c = ci[p];
+            const int q = c2b_lower_bound(ukeys, u, c / bs);
+            double &dst = vals[q * bs2 + lr * bs + c % bs];
+            if (p == first[q]) dst = v[p];
+            else dst += v[p];
+        }
+    }
+    __syncthreads();
+    const int base = brp[I];
+    for (int q = threadIdx.x; q < u; q += 64) bci[base + q] = ukeys[q];
+    for (int t = threadIdx.x; t < u * bs2; t += 64) bv[(size_t)base * bs2 + t] = vals[t];
+}
+
+// Longest block row (CSR entries) and the LDS sizes that follow from it.
+static int c2b_prepare(int n, int bs, const int *d_rp, hipStream_t st, int *m2max) {
+    const int nb = n / bs;
+    std::vector<int> rp(n + 1);
+    ISAI_HIP(hipMemcpyAsync(rp.data(), d_rp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipStreamSynchronize(st));
+    int mmax = 1;
+    for (int I = 0; I < nb; ++I) {
+        const int m = rp[(I + 1) * bs] - rp[I * bs];
+        if (m < 0) {
+            set_error("csr_to_bsr (device): row pointer decreases at block row %d", I);
+            return ISAI_ERR_INVALID_ARG;
+        }
+        mmax = std::max(mmax, m);
+    }
+    if (mmax > kC2BMaxEntries) {
+        set_error("csr_to_bsr (device): a block row holds %d CSR entries (limit %d); use the host form", mmax,
+                  kC2BMaxEntries);
+        return ISAI_ERR_UNSUPPORTED;
+    }
+    int p = 1;
+    while (p < mmax) p <<= 1;
+    *m2max = p;
+    return ISAI_OK;
+}
+
+}  // namespace isai
+
+using namespace isai;
+
+extern "C" {
+
+int isai_csr_to_bsr_device_count(int n, int bs, const int *d_rp, const int *d_ci, int *d_brp, int *nnzb,
+                                 isai_stream_t stream) {
+    ISAI_REQUIRE(d_rp && d_ci && d_brp && nnzb, "isai_csr_to_bsr_device_count: null argument");
+    ISAI_REQUIRE(bs >= 1 && bs <= 8 && n >= 0, "isai_csr_to_bsr_device_count: bad size n=%d bs=%d", n, bs);
+    ISAI_REQUIRE(n % bs == 0, "csr_to_bsr: n = %d is not a multiple of block size %d", n, bs);
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = n / bs;
+    *nnzb = 0;
+    if (nb == 0) {
+        const int zero = 0;
+        ISAI_HIP(hipMemcpyAsync(d_brp, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+        ISAI_HIP(hipStreamSynchronize(st));
+        return ISAI_OK;
+    }
+    int m2max = 1;
+    int s = c2b_prepare(n, bs, d_rp, st, &m2max);
+    if (s != ISAI_OK) return s;
+    // counts in d_brp[0, nb), then an exclusive scan over nb + 1 entries
+    c2b_count_kernel<<<nb, 64, sizeof(int) * 2 * m2max, st>>>(nb, bs, d_rp, d_ci, m2max, d_brp);
+    ISAI_LAUNCH_CHECK();
+    const int zero = 0;
+    ISAI_HIP(hipMemcpyAsync(d_brp + nb, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+    s = exclusive_scan_i32(d_brp, d_brp, (long long)nb + 1, st);
+    if (s != ISAI_OK) return s;
+    ISAI_HIP(hipMemcpyAsync(nnzb, d_brp + nb, sizeof(int), hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipStreamSynchronize(st));
+    return ISAI_OK;
+}
+
+int isai_csr_to_bsr_device_fill(int n, int bs, const int *d_rp, const int *d_ci, const double *d_v,
+                                const int *d_brp, int *d_bci, double *d_bv, isai_stream_t stream) {
+    ISAI_REQUIRE(d_rp && d_ci && d_v && d_brp && d_bci && d_bv, "isai_csr_to_bsr_device_fill: null argument");
+    ISAI_REQUIRE(bs >= 1 && bs <= 8 && n >= 0, "isai_csr_to_bsr_device_fill: bad size n=%d bs=%d", n, bs);
+    ISAI_REQUIRE(n % bs == 0, "csr_to_bsr: n = %d is not a multiple of block size %d", n, bs);
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = n / bs;
+    if (nb == 0) return ISAI_OK;
+    int m2max = 1;
+    int s = c2b_prepare(n, bs, d_rp, st, &m2max);
+    if (s != ISAI_OK) return s;
+    // unique blocks per row <= entries per row
+    std::vector<int> brp(nb + 1);
+    ISAI_HIP(hipMemcpyAsync(brp.data(), d_brp, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipStreamSynchronize(st));
+    int umax = 1;
+    for (int I = 0; I < nb; ++I) umax = std::max(umax, brp[I + 1] - brp[I]);
+    const size_t lds = sizeof(int) * (2 * (size_t)m2max + ((umax + 1) & ~1)) + sizeof(double) * (size_t)umax * bs * bs;
+    if (lds > 64 * 1024) {
+        set_error("csr_to_bsr (device): a block row needs %zu bytes of LDS (limit 65536); use the host form", lds);
+        return ISAI_ERR_UNSUPPORTED;
+    }
+    int *d_err = nullptr;
+    ISAI_HIP(hipMallocAsync((void **)&d_err, sizeof(int), st));
+    ISAI_HIP(hipMemsetAsync(d_err, 0, sizeof(int), st));
+    c2b_fill_kernel<<<nb, 64, lds, st>>>(nb, bs, d_rp, d_ci, d_v, d_brp, d_bci, d_bv, m2max, umax, d_err);
+    ISAI_LAUNCH_CHECK();
+    int err = 0;
+    ISAI_HIP(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipStreamSynchronize(st));
+    ISAI_HIP(hipFreeAsync(d_err, st));
+    if (err) {
+        set_error("csr_to_bsr (device): the block row pointer does not match the matrix (use the count pass)");
+        return ISAI_ERR_INVALID_ARG;
+    }
+    return ISAI_OK;
+}
+
+}  // extern "C"
