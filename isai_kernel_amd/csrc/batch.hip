@@ -316,10 +316,11 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_f64_kernel(int nb, cons
     if (L == 0) return;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const double *xb = X + m.v0 + y;
-    for (int t = 0; t < L; ++t) {
+    // operands of step t (one block of each of the four rows); zero when past the row
+    auto load_step = [&](int t, double &a, double (&bv)[4]) {
         const bool live = t < m.len;
-        double a = 0.0;
         int c = 0;
+        a = 0.0;
         if (live) {
             const int k = m.rs + t;
             a = __builtin_nontemporal_load(&val[(size_t)k * 16 + x * 4 + y]);
@@ -327,12 +328,29 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_f64_kernel(int nb, cons
         }
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
+            const int n = 4 * g + x;
+            bv[g] = (g < G && live && n < m.R) ? xb[(size_t)c * 4 + n * m.step] : 0.0;
+        }
+    };
+    int t = 0;
+    for (; t + 2 <= L; t += 2) {  // two steps' loads in flight before their MFMAs
+        double a0, a1, b0[4], b1[4];
+        load_step(t, a0, b0);
+        load_step(t + 1, a1, b1);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
             if (g < G) {
-                const int n = 4 * g + x;
-                const double bv = (live && n < m.R) ? xb[(size_t)c * 4 + n * m.step] : 0.0;
-                acc[g] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, bv, acc[g], 0, 0, 0);
+                acc[g] = __builtin_amdgcn_mfma_f64_4x4x4f64(a0, b0[g], acc[g], 0, 0, 0);
+                acc[g] = __builtin_amdgcn_mfma_f64_4x4x4f64(a1, b1[g], acc[g], 0, 0, 0);
             }
         }
+    }
+    if (t < L) {
+        double a0, b0[4];
+        load_step(t, a0, b0);
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+            if (g < G) acc[g] = __builtin_amdgcn_mfma_f64_4x4x4f64(a0, b0[g], acc[g], 0, 0, 0);
     }
     if (!m.len) return;
     double *yr = Y + m.v0 + (size_t)(m.br - m.base) * 4 + y;
@@ -369,10 +387,13 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_lowp_kernel(int nb, con
     v4f acc[4];
 #pragma unroll
     for (int g = 0; g < 4; ++g) acc[g] = v4f{0.f, 0.f, 0.f, 0.f};
-    for (int t = 0; t < L; ++t) {
+    // operands of step t: row q of each block (A) and the four x values of right-hand
+    // side q of each group (B), zero when past the row
+    auto load_step = [&](int t, float (&a)[4], float (&xv)[4][4]) {
         const bool live = t < m.len;
-        float a[4] = {0.f, 0.f, 0.f, 0.f};
         int c = 0;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) a[kk] = 0.f;
         if (live) {
             const int k = m.rs + t;
             c = ci[k] - m.base;
@@ -392,21 +413,38 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_lowp_kernel(int nb, con
         }
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            if (g < G) {
-                const int n = 4 * g + q;
-                float xv[4] = {0.f, 0.f, 0.f, 0.f};
-                if (live && n < m.R) {
-                    const double *xp = X + m.v0 + n * m.step + (size_t)c * 4;
-                    const v2d x01 = *(const v2d *)xp, x23 = *(const v2d *)(xp + 2);
-                    xv[0] = (float)x01.x;
-                    xv[1] = (float)x01.y;
-                    xv[2] = (float)x23.x;
-                    xv[3] = (float)x23.y;
-                }
+            const int n = 4 * g + q;
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[kk], xv[kk], acc[g], 0, 0, 0);
+            for (int kk = 0; kk < 4; ++kk) xv[g][kk] = 0.f;
+            if (g < G && live && n < m.R) {
+                const double *xp = X + m.v0 + n * m.step + (size_t)c * 4;
+                const v2d x01 = *(const v2d *)xp, x23 = *(const v2d *)(xp + 2);
+                xv[g][0] = (float)x01.x;
+                xv[g][1] = (float)x01.y;
+                xv[g][2] = (float)x23.x;
+                xv[g][3] = (float)x23.y;
             }
         }
+    };
+    auto mfma_step = [&](const float (&a)[4], const float (&xv)[4][4]) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+            if (g < G)
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[kk], xv[g][kk], acc[g], 0, 0, 0);
+    };
+    int t = 0;
+    for (; t + 2 <= L; t += 2) {  // two steps' loads in flight before their MFMAs
+        float a0[4], a1[4], x0[4][4], x1[4][4];
+        load_step(t, a0, x0);
+        load_step(t + 1, a1, x1);
+        mfma_step(a0, x0);
+        mfma_step(a1, x1);
+    }
+    if (t < L) {
+        float a0[4], x0[4][4];
+        load_step(t, a0, x0);
+        mfma_step(a0, x0);
     }
     if (!m.len) return;
 #pragma unroll
@@ -481,12 +519,22 @@ __device__ __forceinline__ double wg_sum(double v) {
     return (ws[0] + ws[1]) + (ws[2] + ws[3]);
 }
 
+// The per-pair finals run one wave per pair (grid = npairs, 64 threads): the wave
+// sums the pair's tile partials lane-strided, then by a fixed butterfly, so every
+// lane holds the same, deterministic totals; lane 0 updates the pair's state.
 __device__ __forceinline__ void pair_sums(const BatchDev &B, int p, double &s0, double &s1) {
-    s0 = s1 = 0.0;
-    for (int t = B.ptile[p]; t < B.ptile[p + 1]; ++t) {
-        s0 += B.part0[t];
-        s1 += B.part1[t];
+    double a = 0.0, c = 0.0;
+    for (int t = B.ptile[p] + (int)threadIdx.x; t < B.ptile[p + 1]; t += 64) {
+        a += B.part0[t];
+        c += B.part1[t];
     }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        a += __shfl_xor(a, off);
+        c += __shfl_xor(c, off);
+    }
+    s0 = a;
+    s1 = c;
 }
 
 // part0[t] = a.b and part1[t] = a.c (c optional) over tile t, for running pairs.
@@ -541,13 +589,13 @@ __global__ __launch_bounds__(kRed) void binit_kernel(const double *__restrict__ 
     }
 }
 
-__global__ void binit_final(BatchDev B, int npairs, double tol) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+__global__ __launch_bounds__(64) void binit_final(BatchDev B, double tol) {
+    const int p = blockIdx.x;
     PairState &S = B.ps[p];
     if (S.status != 0) return;
     double b2, r2;
     pair_sums(B, p, b2, r2);
+    if (threadIdx.x != 0) return;
     S.bnorm2 = b2 > 0.0 ? b2 : 1.0;
     S.tol2 = tol * tol * S.bnorm2;
     S.rho_old = S.alpha = S.omega = 1.0;
@@ -555,17 +603,17 @@ __global__ void binit_final(BatchDev B, int npairs, double tol) {
     if (r2 <= S.tol2) S.status = 1;
 }
 
-__global__ void brho_final(BatchDev B, int npairs) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+__global__ __launch_bounds__(64) void brho_final(BatchDev B) {
+    const int p = blockIdx.x;
     PairState &S = B.ps[p];
     if (S.status != 0) return;
     if (S.iters >= B.ctl->maxit) {
-        S.status = 5;
+        if (threadIdx.x == 0) S.status = 5;
         return;
     }
     double rho, unused;
     pair_sums(B, p, rho, unused);
+    if (threadIdx.x != 0) return;
     S.iters++;
     S.rho = rho;
     if (rho == 0.0) S.status = 3;
@@ -588,13 +636,13 @@ __global__ __launch_bounds__(kRed) void bp_kernel(const double *__restrict__ r, 
     }
 }
 
-__global__ void balpha_final(BatchDev B, int npairs) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+__global__ __launch_bounds__(64) void balpha_final(BatchDev B) {
+    const int p = blockIdx.x;
     PairState &S = B.ps[p];
     if (S.status != 0) return;
     double d, unused;
     pair_sums(B, p, d, unused);
+    if (threadIdx.x != 0) return;
     if (d == 0.0) S.status = 3;
     else S.alpha = S.rho / d;
     S.first = 0;
@@ -620,14 +668,13 @@ __global__ __launch_bounds__(kRed) void bs_kernel(const double *__restrict__ r, 
     if (threadIdx.x == 0) B.part0[t] = acc;
 }
 
-__global__ void bsnorm_final(BatchDev B, int npairs) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+__global__ __launch_bounds__(64) void bsnorm_final(BatchDev B) {
+    const int p = blockIdx.x;
     PairState &S = B.ps[p];
     if (S.status != 0) return;
     double s2, unused;
     pair_sums(B, p, s2, unused);
-    if (s2 <= S.tol2) S.status = 2;
+    if (threadIdx.x == 0 && s2 <= S.tol2) S.status = 2;
 }
 
 // pairs converged on s: x += alpha ph
@@ -644,13 +691,13 @@ __global__ __launch_bounds__(kRed) void bxs_kernel(double *x, const double *__re
     }
 }
 
-__global__ void bomega_final(BatchDev B, int npairs) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+__global__ __launch_bounds__(64) void bomega_final(BatchDev B) {
+    const int p = blockIdx.x;
     PairState &S = B.ps[p];
     if (S.status != 0) return;
     double ts, tt;
     pair_sums(B, p, ts, tt);  // part0 = t.s, part1 = t.t
+    if (threadIdx.x != 0) return;
     if (tt == 0.0) S.status = 3;
     else S.omega = ts / tt;
 }
@@ -677,14 +724,15 @@ __global__ __launch_bounds__(kRed) void bxr_kernel(double *x, double *r, const d
     if (threadIdx.x == 0) B.part0[t] = acc;
 }
 
-__global__ void brnorm_final(BatchDev B, int npairs) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+__global__ __launch_bounds__(64) void brnorm_final(BatchDev B) {
+    const int p = blockIdx.x;
     PairState &S = B.ps[p];
-    if (S.status == 2) S.status = 1;
-    if (S.status != 0) return;
+    const int st = S.status;
+    if (st == 2 && threadIdx.x == 0) S.status = 1;
+    if (st != 0) return;
     double r2, unused;
     pair_sums(B, p, r2, unused);
+    if (threadIdx.x != 0) return;
     S.rho_old = S.rho;
     if (r2 <= S.tol2) S.status = 1;
     else atomicAdd(&B.ctl->nactive, 1);
@@ -712,11 +760,11 @@ __global__ __launch_bounds__(kRed) void bres_kernel(const double *__restrict__ b
 }
 
 // relres per pair; pairs whose true residual misses tol restart while iterations remain
-__global__ void bres_final(BatchDev B, int npairs, double tol, double *relres) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+__global__ __launch_bounds__(64) void bres_final(BatchDev B, double tol, double *relres) {
+    const int p = blockIdx.x;
     double r2, b2;
     pair_sums(B, p, r2, b2);
+    if (threadIdx.x != 0) return;
     const double rr = sqrt(r2) / sqrt(b2 > 0.0 ? b2 : 1.0);
     relres[p] = rr;
     PairState &S = B.ps[p];
@@ -846,23 +894,23 @@ int batch_iteration(isai_batch *b, double *x, hipStream_t st) {
     const int np = b->npairs, pg = (np + 255) / 256, T = b->ntiles;
     zero_active_kernel<<<1, 1, 0, st>>>(b->d_ctl);
     tile_dot_kernel<<<T, kRed, 0, st>>>(rh, r, nullptr, B);
-    brho_final<<<pg, 256, 0, st>>>(B, np);
+    brho_final<<<np, 64, 0, st>>>(B);
     bp_kernel<<<T, kRed, 0, st>>>(r, p, v, B);
     BTRY(spmm(b, kOpML, p, tmp, 0, st));
     BTRY(spmm(b, kOpMU, tmp, ph, 0, st));
     BTRY(spmm(b, kOpA, ph, v, 0, st));
     tile_dot_kernel<<<T, kRed, 0, st>>>(rh, v, nullptr, B);
-    balpha_final<<<pg, 256, 0, st>>>(B, np);
+    balpha_final<<<np, 64, 0, st>>>(B);
     bs_kernel<<<T, kRed, 0, st>>>(r, v, s, B);
-    bsnorm_final<<<pg, 256, 0, st>>>(B, np);
+    bsnorm_final<<<np, 64, 0, st>>>(B);
     bxs_kernel<<<T, kRed, 0, st>>>(x, ph, B);
     BTRY(spmm(b, kOpML, s, tmp, 0, st));
     BTRY(spmm(b, kOpMU, tmp, sh, 0, st));
     BTRY(spmm(b, kOpA, sh, t, 0, st));
     tile_dot_kernel<<<T, kRed, 0, st>>>(t, s, t, B);
-    bomega_final<<<pg, 256, 0, st>>>(B, np);
+    bomega_final<<<np, 64, 0, st>>>(B);
     bxr_kernel<<<T, kRed, 0, st>>>(x, r, ph, sh, s, t, B);
-    brnorm_final<<<pg, 256, 0, st>>>(B, np);
+    brnorm_final<<<np, 64, 0, st>>>(B);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
@@ -1169,7 +1217,7 @@ int isai_batch_solve(isai_batch_t b, double tol, int maxit, const double *d_B, d
         // (re)start the running pairs from their current x
         BTRY(spmm(b, kOpA, d_X, r, 0, st));
         binit_kernel<<<T, kRed, 0, st>>>(d_B, r, rh, p, v, B);
-        binit_final<<<pg, 256, 0, st>>>(B, np, tol);
+        binit_final<<<np, 64, 0, st>>>(B, tol);
         ISAI_LAUNCH_CHECK();
         for (int rep = 0; rep < max_replays; ++rep) {
             if (use_graph) {
@@ -1184,7 +1232,7 @@ int isai_batch_solve(isai_batch_t b, double tol, int maxit, const double *d_B, d
         // true FP64 residual of every pair; pairs that miss tol restart
         BTRY(spmm(b, kOpA, d_X, r, 1, st));
         bres_kernel<<<T, kRed, 0, st>>>(d_B, r, B);
-        bres_final<<<pg, 256, 0, st>>>(B, np, tol, b->d_relres);
+        bres_final<<<np, 64, 0, st>>>(B, tol, b->d_relres);
         ISAI_LAUNCH_CHECK();
         ISAI_HIP(hipMemcpyAsync(hs.data(), b->d_ps, sizeof(PairState) * np, hipMemcpyDeviceToHost, st));
         ISAI_HIP(hipStreamSynchronize(st));
