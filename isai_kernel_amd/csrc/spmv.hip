@@ -163,62 +163,99 @@ int spmv_dot_launch(int mb, const int *rp, const int *ci, const double *val, con
 __device__ __forceinline__ float bf16_lo(unsigned int w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf16_hi(unsigned int w) { return __uint_as_float(w & 0xffff0000u); }
 
-template <bool BF16>
+// Each 8-lane group serves kLowpRows block rows (32 consecutive rows of the
+// workgroup per step, so x/y accesses stay coalesced) and issues the loads of all
+// of them before accumulating: a reduced-precision row carries too few bytes to keep
+// enough loads in flight on its own.  Per row the lanes accumulate in the same
+// order as with one row per group, so results do not depend on kLowpRows.
+// Measured on the 96^3 7-pt ISAI factors (4 blocks per row): BF16 73.6 us with 2 or
+// 4 rows (91 us with 1, 95 us with 8); FP32 134-136 us with 1 or 2 rows, 151 us with
+// 4 (its 64-B blocks share lines).  On the 64^3 27-pt factors (14 blocks per row) one
+// row already keeps enough loads in flight and several rows per group are slower, so
+// the launcher uses several rows only for rows of at most two lines of values.
+constexpr int kLowpRowsF32 = 2, kLowpRowsBF16 = 4;
+
+template <bool BF16, int kLowpRows>
 __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci, const v4u *__restrict__ val16,
     const v2d *__restrict__ x2, double *__restrict__ y, const int *__restrict__ skip) {
     if (skip && *skip) return;
     constexpr int BPB = BF16 ? 4 : 2;  // blocks per 128-byte line
     constexpr int LPB = 8 / BPB;       // lanes per block
+    constexpr int G = kSpmvBlock / 8;  // lane groups per workgroup
     const int wg = xcd_remap(blockIdx.x, gridDim.x);
-    const int br = wg * (kSpmvBlock / 8) + (threadIdx.x >> 3);
+    const int row0 = wg * G * kLowpRows + (threadIdx.x >> 3);
     const int s = threadIdx.x & 7;
     const int b = s / LPB, sub = s % LPB;
-    float acc0 = 0.0f, acc1 = 0.0f;
-    if (br < nb) {
-        const int rs = rp[br], re = rp[br + 1];
-        for (int k = rs; k < re; k += BPB) {
-            const int kb = k + b;
-            if (kb < re) {
+    int rs[kLowpRows], re[kLowpRows], lmax = 0;
+#pragma unroll
+    for (int r = 0; r < kLowpRows; ++r) {
+        const int br = row0 + r * G;
+        rs[r] = br < nb ? rp[br] : 0;
+        re[r] = br < nb ? rp[br + 1] : 0;
+        lmax = max(lmax, re[r] - rs[r]);
+    }
+    float acc0[kLowpRows], acc1[kLowpRows];
+#pragma unroll
+    for (int r = 0; r < kLowpRows; ++r) acc0[r] = acc1[r] = 0.0f;
+    for (int t = 0; t < lmax; t += BPB) {
+        v4u w[kLowpRows];
+        float f[kLowpRows][4];
+        bool live[kLowpRows];
+#pragma unroll
+        for (int r = 0; r < kLowpRows; ++r) {
+            const int kb = rs[r] + t + b;
+            live[r] = kb < re[r];
+            if (live[r]) {
                 const int c = ci[kb];
-                // 16 B per lane: the row's value array viewed as 16-byte words
-                // BF16 rows fill whole lines; a 64-B FP32 block shares its line with a
-                // neighbour that another wave may read, so FP32 keeps the line in L2.
-                const v4u w = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kb * LPB + sub])
-                                   : val16[(size_t)kb * LPB + sub];
+                // 16 B per lane: BF16 rows fill whole lines (streamed); a 64-B FP32
+                // block shares its line with a neighbour another wave may read, so
+                // FP32 keeps the line in L2
+                w[r] = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kb * LPB + sub]) : val16[(size_t)kb * LPB + sub];
                 const v2d x01 = x2[(size_t)c * 2 + 0];
                 const v2d x23 = x2[(size_t)c * 2 + 1];
-                const float f0 = (float)x01.x, f1 = (float)x01.y, f2 = (float)x23.x, f3 = (float)x23.y;
-                if (BF16) {  // rows 2*sub, 2*sub+1
-                    acc0 = fmaf(bf16_lo(w.x), f0, acc0);
-                    acc0 = fmaf(bf16_hi(w.x), f1, acc0);
-                    acc0 = fmaf(bf16_lo(w.y), f2, acc0);
-                    acc0 = fmaf(bf16_hi(w.y), f3, acc0);
-                    acc1 = fmaf(bf16_lo(w.z), f0, acc1);
-                    acc1 = fmaf(bf16_hi(w.z), f1, acc1);
-                    acc1 = fmaf(bf16_lo(w.w), f2, acc1);
-                    acc1 = fmaf(bf16_hi(w.w), f3, acc1);
-                } else {  // row sub
-                    acc0 = fmaf(__uint_as_float(w.x), f0, acc0);
-                    acc0 = fmaf(__uint_as_float(w.y), f1, acc0);
-                    acc0 = fmaf(__uint_as_float(w.z), f2, acc0);
-                    acc0 = fmaf(__uint_as_float(w.w), f3, acc0);
-                }
+                f[r][0] = (float)x01.x;
+                f[r][1] = (float)x01.y;
+                f[r][2] = (float)x23.x;
+                f[r][3] = (float)x23.y;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kLowpRows; ++r) {
+            if (!live[r]) continue;
+            if (BF16) {  // rows 2*sub, 2*sub+1
+                acc0[r] = fmaf(bf16_lo(w[r].x), f[r][0], acc0[r]);
+                acc0[r] = fmaf(bf16_hi(w[r].x), f[r][1], acc0[r]);
+                acc0[r] = fmaf(bf16_lo(w[r].y), f[r][2], acc0[r]);
+                acc0[r] = fmaf(bf16_hi(w[r].y), f[r][3], acc0[r]);
+                acc1[r] = fmaf(bf16_lo(w[r].z), f[r][0], acc1[r]);
+                acc1[r] = fmaf(bf16_hi(w[r].z), f[r][1], acc1[r]);
+                acc1[r] = fmaf(bf16_lo(w[r].w), f[r][2], acc1[r]);
+                acc1[r] = fmaf(bf16_hi(w[r].w), f[r][3], acc1[r]);
+            } else {  // row sub
+                acc0[r] = fmaf(__uint_as_float(w[r].x), f[r][0], acc0[r]);
+                acc0[r] = fmaf(__uint_as_float(w[r].y), f[r][1], acc0[r]);
+                acc0[r] = fmaf(__uint_as_float(w[r].z), f[r][2], acc0[r]);
+                acc0[r] = fmaf(__uint_as_float(w[r].w), f[r][3], acc0[r]);
             }
         }
     }
-    // sum the lanes holding the same row(s): same sub, different b
 #pragma unroll
-    for (int off = LPB; off < 8; off <<= 1) {
-        acc0 += __shfl_xor(acc0, off);
-        if (BF16) acc1 += __shfl_xor(acc1, off);
-    }
-    if (br < nb && b == 0) {
-        if (BF16) {
-            y[(size_t)br * 4 + 2 * sub] = (double)acc0;
-            y[(size_t)br * 4 + 2 * sub + 1] = (double)acc1;
-        } else {
-            y[(size_t)br * 4 + sub] = (double)acc0;
+    for (int r = 0; r < kLowpRows; ++r) {
+        // sum the lanes holding the same row(s): same sub, different b
+#pragma unroll
+        for (int off = LPB; off < 8; off <<= 1) {
+            acc0[r] += __shfl_xor(acc0[r], off);
+            if (BF16) acc1[r] += __shfl_xor(acc1[r], off);
+        }
+        const int br = row0 + r * G;
+        if (br < nb && b == 0) {
+            if (BF16) {
+                y[(size_t)br * 4 + 2 * sub] = (double)acc0[r];
+                y[(size_t)br * 4 + 2 * sub + 1] = (double)acc1[r];
+            } else {
+                y[(size_t)br * 4 + sub] = (double)acc0[r];
+            }
         }
     }
 }
@@ -305,14 +342,22 @@ int spmv_launch(int mb, int nnzb, int bs, isai_precision_t prec, const int *rp, 
             const unsigned g = grid_for(mb, kSpmvBlock / 8);
             bsr4_spmv_f64_kernel<2><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v2d *)val,
                                                               (const v2d *)x, y, skip);
-        } else if (prec == ISAI_PREC_FP32) {
-            const unsigned g = grid_for(mb, kSpmvBlock / 8);
-            bsr4_spmv_lowp_kernel<false><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v4u *)val,
-                                                                   (const v2d *)x, y, skip);
-        } else if (prec == ISAI_PREC_BF16) {
-            const unsigned g = grid_for(mb, kSpmvBlock / 8);
-            bsr4_spmv_lowp_kernel<true><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, (const v4u *)val,
-                                                                  (const v2d *)x, y, skip);
+        } else if (prec == ISAI_PREC_FP32 || prec == ISAI_PREC_BF16) {
+            const bool bf16 = prec == ISAI_PREC_BF16;
+            const int bpb = bf16 ? 4 : 2;  // blocks per 128-byte line
+            const bool short_rows = nnzb <= (long long)2 * bpb * mb;
+            const int rows = short_rows ? (bf16 ? kLowpRowsBF16 : kLowpRowsF32) : 1;
+            const unsigned g = grid_for(mb, rows * kSpmvBlock / 8);
+            const v4u *v16 = (const v4u *)val;
+            const v2d *x2 = (const v2d *)x;
+            if (bf16 && short_rows)
+                bsr4_spmv_lowp_kernel<true, kLowpRowsBF16><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x2, y, skip);
+            else if (bf16)
+                bsr4_spmv_lowp_kernel<true, 1><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x2, y, skip);
+            else if (short_rows)
+                bsr4_spmv_lowp_kernel<false, kLowpRowsF32><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x2, y, skip);
+            else
+                bsr4_spmv_lowp_kernel<false, 1><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x2, y, skip);
         } else {
             set_error("bsr_spmv: unknown precision %d", (int)prec);
             return ISAI_ERR_INVALID_ARG;
