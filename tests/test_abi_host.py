@@ -170,3 +170,16 @@ def test_matrix_market_reader_and_residual():
     assert ex.value == 0.0 and rr.value == 0.0
     assert lib.isai_read_matrix_market(b"/nonexistent.mtx", ctypes.byref(nr), ctypes.byref(nc), ctypes.byref(nnz),
                                        ctypes.byref(prp), ctypes.byref(pci), ctypes.byref(pv)) == -10
+
+
+def test_driver_usage_without_arguments():
+    """The C driver (built with the library) parses its arguments before touching
+    the GPU: without a matrix it prints its usage and exits."""
+    import subprocess
+    drv = ROOT / "isai_kernel_amd" / "bin" / "isai_driver"
+    if not drv.exists():
+        pytest.skip("driver not built")
+    p = subprocess.run([str(drv)], capture_output=True, text=True, timeout=30)
+    assert p.returncode == 0 and "Usage" in p.stderr
+    p = subprocess.run([str(drv), "--bogus"], capture_output=True, text=True, timeout=30)
+    assert p.returncode == 2 and "unknown option" in p.stderr
