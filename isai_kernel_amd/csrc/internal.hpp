@@ -17,10 +17,10 @@ int spmv_launch(int mb, int nnzb, int bs, isai_precision_t prec, const int *rp, 
 
 // bs = 4 FP64 product y = A x with fused dot products of y: part[wg] = sum y.a over
 // the rows of workgroup wg, and with ndot = 2 also part[G + wg] = sum y.y, where
-// G = spmv_dot_groups(mb) partials per dot.
+// G = spmv_dot_groups(mb) partials per dot; nnzb picks the row batch width.
 int spmv_dot_groups(int mb);
 int spmv_dot_launch(int mb, const int *rp, const int *ci, const double *val, const double *x, double *y,
-                    const double *a, int ndot, double *part, hipStream_t st, const int *skip);
+                    const double *a, int ndot, double *part, hipStream_t st, const int *skip, long long nnzb);
 
 // Sync-free L then U solve of a plan, with the same skip convention.
 int sptrsv_solve_internal(isai_sptrsv_plan_t plan, const double *d_LU, const double *d_b, double *d_x,

@@ -616,13 +616,13 @@ int bicgstab(isai_handle *h, double tol, int maxit, const double *b, double *x, 
             for (int k = 0; k < kItersPerRead && launched < maxit; ++k, ++launched) {
                 bicg_p_kernel<<<vec_grid(n), 256, 0, st>>>(n, r, p, v, h->S);
                 TRY(precond_apply(h, p, ph, tmp, st, skip));
-                TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, ph, v, rh, 1, part, st, skip));
+                TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, ph, v, rh, 1, part, st, skip, h->nnzb));
                 bicg_alpha_final_wg<<<1, kWgSumThreads, 0, st>>>(h->S, part, G);
                 bicg_s_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, r, v, s, h->S);
                 bicg_snorm_final<<<1, kRedThreads, 0, st>>>(h->S);
                 bicg_xs_kernel<<<vec_grid(n), 256, 0, st>>>(n, x, ph, h->S);
                 TRY(precond_apply(h, s, sh, tmp, st, skip));
-                TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, sh, t, s, 2, part, st, skip));
+                TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, sh, t, s, 2, part, st, skip, h->nnzb));
                 bicg_omega_final_wg<<<1, kWgSumThreads, 0, st>>>(h->S, part, G);
                 bicg_xr_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, x, r, ph, sh, s, t, rh, h->S);
                 bicg_rnorm_final<<<1, kRedThreads, 0, st>>>(h->S);

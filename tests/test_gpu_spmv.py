@@ -44,6 +44,28 @@ def test_spmv_fp64_matches_oracle(gpu, problem, L):
     assert np.array_equal(y, _spmv_gpu(rp, ci, val, x))  # deterministic
 
 
+def _ragged(nb, maxlen, seed):
+    """Random block pattern with row lengths 0..maxlen (sorted unique columns)."""
+    rng = np.random.default_rng(seed)
+    rows = [np.sort(rng.choice(nb, size=rng.integers(0, maxlen + 1), replace=False)) for _ in range(nb)]
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    ci = np.concatenate(rows).astype(np.int32)
+    val = rng.standard_normal((len(ci), 4, 4))
+    return rp, ci, val
+
+
+@pytest.mark.parametrize("maxlen", [6, 12, 60])
+def test_spmv_fp64_ragged_rows(gpu, maxlen):
+    """Row lengths across the product's batch widths (4, 8, 16 blocks in flight,
+    picked from the average length) and several batches per row."""
+    print(f"spmv fp64 ragged maxlen={maxlen}", flush=True)
+    rp, ci, val = _ragged(300, maxlen, seed=maxlen)
+    x = RNG.standard_normal(300 * 4)
+    y = _spmv_gpu(rp, ci, val, x)
+    assert np.max(np.abs(y - O.spmv(rp, ci, val, x))) <= TOL64 * _scale(rp, ci, val, x)
+    assert np.all(y[np.repeat(np.diff(rp) == 0, 4)] == 0)
+
+
 @pytest.mark.parametrize("bs", [1, 2, 3, 5, 7, 8])
 def test_spmv_generic_block_sizes(gpu, bs):
     print(f"spmv generic bs={bs}", flush=True)
@@ -69,10 +91,20 @@ def test_spmv_empty_rows_and_zero_rows(gpu):
     assert np.all(y[0:4] == 0) and np.all(y[8:12] == 0)
 
 
+@pytest.mark.parametrize("case", ["27pt", "7pt", "lower7pt", "ragged"])
 @pytest.mark.parametrize("prec,bf16", [(1, False), (2, True)])
-def test_spmv_reduced_precision_bitexact(gpu, prec, bf16):
-    print(f"spmv prec={prec}", flush=True)
-    rp, ci, val = problems.generate(2, 7, seed=3)
+def test_spmv_reduced_precision_bitexact(gpu, prec, bf16, case):
+    """Long rows (27 blocks), short rows (7 and 4 blocks: several rows per lane group)
+    and ragged rows with empty ones, each against the oracle's rounded product."""
+    print(f"spmv prec={prec} case={case}", flush=True)
+    if case == "ragged":
+        rp, ci, val = _ragged(300, 40, seed=5)
+    else:
+        rp, ci, val = problems.generate(2 if case == "27pt" else 1, 7, seed=3)
+    if case == "lower7pt":
+        keep = ci <= np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+        rp = np.concatenate([[0], np.cumsum(np.add.reduceat(keep, rp[:-1]))]).astype(np.int32)
+        ci, val = ci[keep].copy(), val[keep].copy()
     x = RNG.standard_normal((len(rp) - 1) * 4)
     d_val64 = DeviceArray.from_numpy(val)
     d_low = DeviceArray((val.size,), np.float32 if prec == 1 else np.uint16)
@@ -91,7 +123,7 @@ def test_spmv_reduced_precision_bitexact(gpu, prec, bf16):
     # |y - ref| <= 2 * len * 2^-24 * (|A_rounded| |x|) row by row
     absval = np.abs(val.astype(np.float32).astype(np.float64)) if not bf16 else np.abs(
         (low.astype(np.uint32) << 16).view(np.float32).astype(np.float64).reshape(val.shape))
-    bound = O.spmv(rp, ci, absval, np.abs(x)) * (2 * 27 * 4 * 2.0**-24)
+    bound = O.spmv(rp, ci, absval, np.abs(x)) * (2 * int(np.diff(rp).max()) * 4 * 2.0**-24)
     assert np.all(np.abs(y - ref) <= bound + 1e-300)
     assert np.array_equal(y, _spmv_gpu(rp, ci, val, x, prec=prec, dval=d_low))  # deterministic
 
