@@ -51,6 +51,15 @@ def spmv_bytes(nb: int, nnzb: int, value_bytes: int = 8) -> int:
     return (16 * value_bytes + 4) * nnzb + 4 * (nb + 1) + 2 * 4 * 8 * nb
 
 
+def apply_bytes(nb: int, nnzb_l: int, nnzb_u: int, value_bytes: int) -> int:
+    """M_U (M_L v): both factors' values and indices once, v and the result FP64; the
+    intermediate M_L v is written and gathered in FP64 with FP64 factors and in FP32
+    (the precision the second product reads it in) with FP32/BF16 factors."""
+    mid = 8 if value_bytes == 8 else 4
+    return ((16 * value_bytes + 4) * (nnzb_l + nnzb_u) + 2 * 4 * (nb + 1) + 4 * 8 * nb
+            + 2 * 4 * mid * nb + 4 * 8 * nb)
+
+
 def sptrsv_bytes(nb: int, nnzb_l: int, nnzb_u: int) -> int:
     return (16 * 8 + 4) * (nnzb_l + nnzb_u) + 2 * 4 * (nb + 1) + 4 * 4 * 8 * nb + 2 * 4 * nb
 
@@ -379,7 +388,7 @@ def main() -> int:
         for _ in range(5):
             apply()
         ms = timer.time(apply, K)
-        record(f"isai_apply_{prec}", ms, 2 * ((16 * vb + 4) * nnzbL + 4 * (nb + 1) + 2 * 4 * 8 * nb))
+        record(f"isai_apply_{prec}", ms, apply_bytes(nb, nnzbL, nnzbU, vb))
 
     # ---- sync-free SpTRSV (L then U with the packed ILU(0) factor)
     h.set_preconditioner("ilu", "fp64")

@@ -15,6 +15,12 @@ namespace isai {
 int spmv_launch(int mb, int nnzb, int bs, isai_precision_t prec, const int *rp, const int *ci,
                 const void *val, const double *x, double *y, hipStream_t st, const int *skip);
 
+// bs = 4 product with FP32/BF16 values; x and y each either FP64 or FP32 vectors
+// (x is rounded to FP32 on load either way, so an FP32 x gives the same result as
+// the FP64 vector it was rounded from).
+int spmv_lowp_launch(int mb, long long nnzb, isai_precision_t prec, const int *rp, const int *ci, const void *val,
+                     const void *x, bool x_f32, void *y, bool y_f32, hipStream_t st, const int *skip);
+
 // bs = 4 FP64 product y = A x with fused dot products of y: part[wg] = sum y.a over
 // the rows of workgroup wg, and with ndot = 2 also part[G + wg] = sum y.y, where
 // G = spmv_dot_groups(mb) partials per dot; nnzb picks the row batch width.

@@ -546,6 +546,14 @@ int precond_apply(isai_handle *h, const double *v, double *z, double *tmp, hipSt
             const isai_precision_t p = (isai_precision_t)h->precision;
             const void *L = p == ISAI_PREC_FP64 ? (const void *)h->d_Lval : h->d_Llp;
             const void *U = p == ISAI_PREC_FP64 ? (const void *)h->d_Uval : h->d_Ulp;
+            if (p != ISAI_PREC_FP64 && h->bs == 4) {
+                // M_L v is kept in FP32, the precision the M_U product reads it in:
+                // half the intermediate's bytes, same result
+                int s = spmv_lowp_launch(h->nb, h->nnzbL, p, h->d_Lrow, h->d_Lcol, L, v, false, tmp, true, st, skip);
+                if (s == ISAI_OK)
+                    s = spmv_lowp_launch(h->nb, h->nnzbU, p, h->d_Urow, h->d_Ucol, U, tmp, true, z, false, st, skip);
+                return s;
+            }
             int s = spmv_launch(h->nb, h->nnzbL, h->bs, p, h->d_Lrow, h->d_Lcol, L, v, tmp, st, skip);
             if (s == ISAI_OK) s = spmv_launch(h->nb, h->nnzbU, h->bs, p, h->d_Urow, h->d_Ucol, U, tmp, z, st, skip);
             return s;

@@ -170,6 +170,29 @@ int spmv_dot_launch(int mb, const int *rp, const int *ci, const double *val, con
 __device__ __forceinline__ float bf16_lo(unsigned int w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf16_hi(unsigned int w) { return __uint_as_float(w & 0xffff0000u); }
 
+// x of one block column rounded to FP32, from an FP64 vector (two 16-B loads) or
+// from an FP32 vector (one 16-B load); the two give the same floats.
+template <bool XF32>
+__device__ __forceinline__ void load_x4(const void *__restrict__ x, int c, float f[4]) {
+    if (XF32) {
+        const v4f v = ((const v4f *)x)[c];
+        f[0] = v.x, f[1] = v.y, f[2] = v.z, f[3] = v.w;
+    } else {
+        const v2d *x2 = (const v2d *)x;
+        const v2d x01 = x2[(size_t)c * 2 + 0];
+        const v2d x23 = x2[(size_t)c * 2 + 1];
+        f[0] = (float)x01.x, f[1] = (float)x01.y, f[2] = (float)x23.x, f[3] = (float)x23.y;
+    }
+}
+
+template <bool YF32>
+__device__ __forceinline__ void store_y(void *__restrict__ y, size_t i, float v) {
+    if (YF32)
+        ((float *)y)[i] = v;
+    else
+        ((double *)y)[i] = (double)v;
+}
+
 // Each 8-lane group serves kLowpRows block rows (32 consecutive rows of the
 // workgroup per step, so x/y accesses stay coalesced) and issues the loads of all
 // of them before accumulating: a reduced-precision row carries too few bytes to keep
@@ -182,10 +205,10 @@ __device__ __forceinline__ float bf16_hi(unsigned int w) { return __uint_as_floa
 // the launcher uses several rows only for rows of at most two lines of values.
 constexpr int kLowpRowsF32 = 2, kLowpRowsBF16 = 4;
 
-template <bool BF16, int kLowpRows>
+template <bool BF16, int kLowpRows, bool XF32, bool YF32>
 __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci, const v4u *__restrict__ val16,
-    const v2d *__restrict__ x2, double *__restrict__ y, const int *__restrict__ skip) {
+    const void *__restrict__ x, void *__restrict__ y, const int *__restrict__ skip) {
     if (skip && *skip) return;
     constexpr int BPB = BF16 ? 4 : 2;  // blocks per 128-byte line
     constexpr int LPB = 8 / BPB;       // lanes per block
@@ -219,12 +242,7 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
                 // block shares its line with a neighbour another wave may read, so
                 // FP32 keeps the line in L2
                 w[r] = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kb * LPB + sub]) : val16[(size_t)kb * LPB + sub];
-                const v2d x01 = x2[(size_t)c * 2 + 0];
-                const v2d x23 = x2[(size_t)c * 2 + 1];
-                f[r][0] = (float)x01.x;
-                f[r][1] = (float)x01.y;
-                f[r][2] = (float)x23.x;
-                f[r][3] = (float)x23.y;
+                load_x4<XF32>(x, c, f[r]);
             }
         }
 #pragma unroll
@@ -258,10 +276,10 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
         const int br = row0 + r * G;
         if (br < nb && b == 0) {
             if (BF16) {
-                y[(size_t)br * 4 + 2 * sub] = (double)acc0[r];
-                y[(size_t)br * 4 + 2 * sub + 1] = (double)acc1[r];
+                store_y<YF32>(y, (size_t)br * 4 + 2 * sub, acc0[r]);
+                store_y<YF32>(y, (size_t)br * 4 + 2 * sub + 1, acc1[r]);
             } else {
-                y[(size_t)br * 4 + sub] = (double)acc0[r];
+                store_y<YF32>(y, (size_t)br * 4 + sub, acc0[r]);
             }
         }
     }
@@ -274,10 +292,10 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
 // Measured on the 64^3 27-pt ISAI factor (14 blocks per row): 33.3 us with NL = 4
 // against 43.4 us with one line per step; on 4-block rows it is slower than the
 // several-rows form above, which the launcher keeps for them.
-template <int NL>
+template <int NL, bool XF32, bool YF32>
 __global__ __launch_bounds__(kSpmvF64Block) void bsr4_spmv_bf16_batch_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci, const v4u *__restrict__ val16,
-    const v2d *__restrict__ x2, double *__restrict__ y, const int *__restrict__ skip) {
+    const void *__restrict__ x, void *__restrict__ y, const int *__restrict__ skip) {
     if (skip && *skip) return;
     constexpr int KB = NL * 4;
     static_assert(KB <= 16, "KB");
@@ -303,14 +321,7 @@ __global__ __launch_bounds__(kSpmvF64Block) void bsr4_spmv_bf16_batch_kernel(
             const int clo_u = __shfl(clo, u & 7, 8);
             const int chi_u = KB > 8 ? __shfl(chi, u & 7, 8) : 0;
             const int c = u < 8 ? clo_u : chi_u;
-            if (t + u < len) {
-                const v2d x01 = x2[(size_t)c * 2 + 0];
-                const v2d x23 = x2[(size_t)c * 2 + 1];
-                f[l][0] = (float)x01.x;
-                f[l][1] = (float)x01.y;
-                f[l][2] = (float)x23.x;
-                f[l][3] = (float)x23.y;
-            }
+            if (t + u < len) load_x4<XF32>(x, c, f[l]);
         }
 #pragma unroll
         for (int l = 0; l < NL; ++l) {
@@ -331,8 +342,8 @@ __global__ __launch_bounds__(kSpmvF64Block) void bsr4_spmv_bf16_batch_kernel(
         acc1 += __shfl_xor(acc1, off);
     }
     if (br < nb && b == 0) {
-        y[(size_t)br * 4 + 2 * sub] = (double)acc0;
-        y[(size_t)br * 4 + 2 * sub + 1] = (double)acc1;
+        store_y<YF32>(y, (size_t)br * 4 + 2 * sub, acc0);
+        store_y<YF32>(y, (size_t)br * 4 + 2 * sub + 1, acc1);
     }
 }
 
@@ -407,6 +418,43 @@ __global__ void bsr2csr_kernel(int nb, int bs, const int *__restrict__ brp, cons
     }
 }
 
+template <bool XF32, bool YF32>
+static void lowp_launch(int mb, long long nnzb, bool bf16, const int *rp, const int *ci, const v4u *v16,
+                        const void *x, void *y, hipStream_t st, const int *skip) {
+    const int bpb = bf16 ? 4 : 2;  // blocks per 128-byte line
+    const bool short_rows = nnzb <= (long long)2 * bpb * mb;
+    const int rows = short_rows ? (bf16 ? kLowpRowsBF16 : kLowpRowsF32) : 1;
+    const unsigned g = grid_for(mb, rows * kSpmvBlock / 8);
+    if (bf16 && short_rows)
+        bsr4_spmv_lowp_kernel<true, kLowpRowsBF16, XF32, YF32><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x, y, skip);
+    else if (bf16)
+        bsr4_spmv_bf16_batch_kernel<4, XF32, YF32><<<grid_for(mb, kSpmvF64Block / 8), kSpmvF64Block, 0, st>>>(
+            mb, rp, ci, v16, x, y, skip);
+    else if (short_rows)
+        bsr4_spmv_lowp_kernel<false, kLowpRowsF32, XF32, YF32><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x, y, skip);
+    else
+        bsr4_spmv_lowp_kernel<false, 1, XF32, YF32><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x, y, skip);
+}
+
+int spmv_lowp_launch(int mb, long long nnzb, isai_precision_t prec, const int *rp, const int *ci, const void *val,
+                     const void *x, bool x_f32, void *y, bool y_f32, hipStream_t st, const int *skip) {
+    ISAI_REQUIRE(prec == ISAI_PREC_FP32 || prec == ISAI_PREC_BF16, "bsr_spmv: reduced precision expected (got %d)",
+                 (int)prec);
+    if (mb == 0) return ISAI_OK;
+    const bool bf16 = prec == ISAI_PREC_BF16;
+    const v4u *v16 = (const v4u *)val;
+    if (x_f32 && y_f32)
+        lowp_launch<true, true>(mb, nnzb, bf16, rp, ci, v16, x, y, st, skip);
+    else if (x_f32)
+        lowp_launch<true, false>(mb, nnzb, bf16, rp, ci, v16, x, y, st, skip);
+    else if (y_f32)
+        lowp_launch<false, true>(mb, nnzb, bf16, rp, ci, v16, x, y, st, skip);
+    else
+        lowp_launch<false, false>(mb, nnzb, bf16, rp, ci, v16, x, y, st, skip);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
 int spmv_launch(int mb, int nnzb, int bs, isai_precision_t prec, const int *rp, const int *ci,
                 const void *val, const double *x, double *y, hipStream_t st, const int *skip) {
     ISAI_REQUIRE(mb >= 0 && nnzb >= 0, "bsr_spmv: negative size (mb=%d nnzb=%d)", mb, nnzb);
@@ -425,22 +473,7 @@ int spmv_launch(int mb, int nnzb, int bs, isai_precision_t prec, const int *rp, 
             else
                 bsr4_spmv_f64_kernel<16><<<g, kSpmvF64Block, 0, st>>>(mb, rp, ci, v2, x2, y, skip);
         } else if (prec == ISAI_PREC_FP32 || prec == ISAI_PREC_BF16) {
-            const bool bf16 = prec == ISAI_PREC_BF16;
-            const int bpb = bf16 ? 4 : 2;  // blocks per 128-byte line
-            const bool short_rows = nnzb <= (long long)2 * bpb * mb;
-            const int rows = short_rows ? (bf16 ? kLowpRowsBF16 : kLowpRowsF32) : 1;
-            const unsigned g = grid_for(mb, rows * kSpmvBlock / 8);
-            const v4u *v16 = (const v4u *)val;
-            const v2d *x2 = (const v2d *)x;
-            if (bf16 && short_rows)
-                bsr4_spmv_lowp_kernel<true, kLowpRowsBF16><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x2, y, skip);
-            else if (bf16)
-                bsr4_spmv_bf16_batch_kernel<4><<<grid_for(mb, kSpmvF64Block / 8), kSpmvF64Block, 0, st>>>(
-                    mb, rp, ci, v16, x2, y, skip);
-            else if (short_rows)
-                bsr4_spmv_lowp_kernel<false, kLowpRowsF32><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x2, y, skip);
-            else
-                bsr4_spmv_lowp_kernel<false, 1><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x2, y, skip);
+            return spmv_lowp_launch(mb, nnzb, prec, rp, ci, val, x, false, y, false, st, skip);
         } else {
             set_error("bsr_spmv: unknown precision %d", (int)prec);
             return ISAI_ERR_INVALID_ARG;

@@ -45,6 +45,19 @@ def test_setup_factors_bitexact_and_apply(gpu):
         if kind == "isai" and prec != "fp64":
             # same rounded operands as the oracle's reduced path, FP32 accumulation
             assert rel_err(z, ref) <= 1e-5, (prec, rel_err(z, ref))
+            # the handle keeps M_L v in FP32 between the two products; the public
+            # two-product apply keeps it in FP64: the products read it rounded to
+            # FP32 either way, so the results are identical
+            p = 1 if prec == "fp32" else 2
+            dt = np.float32 if p == 1 else np.uint16
+            lo_L, lo_U = DeviceArray((L[2].size,), dt), DeviceArray((U[2].size,), dt)
+            call("isai_convert_values", L[2].size, p, Lval, lo_L.ptr, None)
+            call("isai_convert_values", U[2].size, p, Uval, lo_U.ptr, None)
+            d_y, d_x = DeviceArray((A.n,), np.float64), DeviceArray((A.n,), np.float64)
+            call("isai_apply", A.nb, 4, p, Lrow, Lcol, lo_L.ptr, nnzbL,
+                 Urow, Ucol, lo_U.ptr, nnzbU, d_v.ptr, d_y.ptr, d_x.ptr, None)
+            call("isai_device_synchronize")
+            assert np.array_equal(d_x.to_numpy(), z), prec
         else:
             assert rel_err(z, ref) <= 1e-12, (kind, rel_err(z, ref))
     h.close()
