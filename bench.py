@@ -37,10 +37,11 @@ sys.path.insert(0, str(ROOT))
 METRIC = "solve time to 1e-10 rel-res, mixed vs FP64 ISAI; BSR SpMV/SpTRSV HBM GB/s"
 HBM_PEAK_GBPS = 8000.0
 TOL = 1e-10
-# HBM bytes per launch of the A SpMV at 96^3, from the FETCH_SIZE / WRITE_SIZE counter
-# passes summarised in profiles/hbm_traffic_96cube_7pt_pmc.md (FETCH_SIZE x2 by the
-# copy-kernel calibration): 413,194.25 KiB x 2 + 27,648 KiB.
-SPMV_TRAFFIC_BYTES_96CUBE = 413194.25 * 2 * 1024 + 27648 * 1024
+# Bytes per launch of the A SpMV at 96^3 beyond the L2 (HBM and Infinity Cache), from
+# the FETCH_SIZE / WRITE_SIZE counter passes summarised in
+# profiles/hbm_traffic_96cube_7pt_pmc.md (FETCH_SIZE x2 by the copy-kernel
+# calibration): 451,177.8 KiB x 2 + 27,648 KiB.
+SPMV_TRAFFIC_BYTES_96CUBE = 451177.8 * 2 * 1024 + 27648 * 1024
 
 
 def log(msg: str) -> None:
@@ -280,7 +281,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+                          "(the path is single-GPU: sharding across GPUs is not built)"}))
         return 2
 
     from isai_kernel_amd import DeviceArray, call, lib, problems
