@@ -285,7 +285,7 @@ def main() -> int:
         return 2
 
     from isai_kernel_amd import DeviceArray, call, lib, problems
-    from isai_kernel_amd.solver import Handle, Matrix
+    from isai_kernel_amd.solver import Handle, Matrix, pair_stats
 
     name = ctypes.create_string_buffer(256)
     cus = ctypes.c_int()
@@ -349,6 +349,24 @@ def main() -> int:
     csr2bsr_ms = min(c2b_ms)
     log(f"device csr2bsr (count + fill, {nnzb * 16} CSR entries): {csr2bsr_ms:.1f} ms wall")
     del d_crp, d_cci, d_cv, d_brp, d_bci, d_bv
+
+    # ---- pair statistics of the ISAI(L) pattern (the input of the build plan)
+    mbL, mcL = problems.bcsc_pattern(rp, ci, True)
+    d_mb, d_mc = DeviceArray.from_numpy(mbL), DeviceArray.from_numpy(mcL)
+    d_pp = DeviceArray((len(mcL) + 1,), np.int32)
+    tot = ctypes.c_int()
+    call("isai_pairs_count", nb, len(mcL), A.d_rp.ptr, A.d_ci.ptr, d_mb.ptr, d_mc.ptr, 0, d_pp.ptr,
+         ctypes.byref(tot), stream)
+    t0 = time.perf_counter()
+    ps = pair_stats(d_mb, d_pp, nb, stream)
+    stats_ms = (time.perf_counter() - t0) * 1e3
+    plan = h.build_plan()
+    pair_stats_l = {k: ps[k] for k in ("mnnz", "total_pairs", "n_avg", "n_max", "n_p99", "q_avg", "q_max", "q_p50",
+                                       "q_p99", "qlen_avg", "qlen_max", "qlen_hist")}
+    pair_stats_l["wall_ms"] = round(stats_ms, 2)
+    log(f"ISAI(L) pair statistics in {stats_ms:.1f} ms: N avg {ps['n_avg']:.2f} max {ps['n_max']}, "
+        f"Q avg {ps['q_avg']:.2f} max {ps['q_max']}; build plan {plan}")
+    del d_mb, d_mc, d_pp
 
     kernels = {}
     K = args.kernel_iters
@@ -463,6 +481,8 @@ def main() -> int:
         "kernels": kernels,
         "setup_ms": {"ilu0": round(ms_ilu, 3), "isai_build": round(ms_isai, 3), "levels": nlevels,
                      "csr2bsr_device_wall": round(csr2bsr_ms, 2)},
+        "isai_build_plan": plan,
+        "pair_stats_L": pair_stats_l,
         "copy_kernel_GBps": round(copy_gbps, 1),
     }
     h.close()

@@ -47,6 +47,38 @@ int isai_pairs_fill(int nb, const int *d_Arow, const int *d_Acol, const int *d_M
                     const int *d_Mcol, int upper, const int *d_PairPtr, int *d_PairPLocal,
                     int *d_PairSrc, isai_stream_t stream);
 
+/* Pair-list statistics of one ISAI factor: the figures that analyze_pair_lengths
+ * and analyze_column_q_bins print (reference analyze_pair.h:6-163, 250-301),
+ * computed on the device and returned in a struct.  Per column j: N_j =
+ * Mbrow[j+1] - Mbrow[j], Q_j = the pairs of its entries (PairPtr[Mbrow[j+1]] -
+ * PairPtr[Mbrow[j]]), qlen = the pairs of one entry.  Percentiles are the
+ * reference's: sort ascending and take v[(size_t)((size - 1) * p)].  qlen
+ * histogram bins: 0, 1, 2-3, 4-7, 8-15, 16-31, 32-63, 64-127, 128-255, >= 256.
+ * Column bin of j: j * 10 / nb.  The ISAI build uses these statistics to size
+ * its LDS Y cache and to order its columns (isai_handle_build_plan). */
+typedef struct {
+    int nb, mnnz;
+    long long total_pairs;
+    double n_avg, q_avg, qlen_avg;
+    int n_max, n_p50, n_p90, n_p95, n_p99;
+    int q_max, q_p50, q_p90, q_p95, q_p99;
+    int qlen_max, qlen_p50, qlen_p90, qlen_p95, qlen_p99;
+    int bin_cols[10];
+    int bin_max_q[10];
+    long long qlen_hist[10];
+    long long bin_sum_q[10];
+} isai_pair_stats_t;
+
+/* Device pointers d_Mbrow[nb+1], d_PairPtr[Mnnz+1] (from isai_pairs_count);
+ * synchronizes the stream and allocates temporaries. */
+int isai_pair_stats(int nb, const int *d_Mbrow, const int *d_PairPtr, isai_pair_stats_t *stats,
+                    isai_stream_t stream);
+/* Per-column figures on the device (the input of analyze_column_q_distribution /
+ * analyze_last_columns, analyze_pair.h:165-248, 303-327): d_N[nb], d_Q[nb] and
+ * d_maxqlen[nb] (the largest qlen of the column's entries); asynchronous. */
+int isai_pair_column_stats(int nb, const int *d_Mbrow, const int *d_PairPtr, int *d_N, int *d_Q,
+                           int *d_maxqlen, isai_stream_t stream);
+
 /* Replaces the ISAI(L) kernels (reference isai_bsr.h:288-399 cachedY, 869-974
  * tensor cachedY, 1189-1353 globalY, cacheY_isai.h:159-341 hybrid; launched at
  * main.cu:51-152): column-wise forward substitution over S_j with the pair lists

@@ -73,6 +73,14 @@ int isai_handle_status(isai_handle_t handle, isai_stream_t stream);
  * (ILU(0) preconditioner); see isai_bsr_sptrsv_set_spin_limit. */
 int isai_handle_set_spin_limit(isai_handle_t handle, int limit);
 
+/* The ISAI build plan isai_setup derived from the pair statistics (the choice the
+ * reference makes by hand from analyze_pair.h's output and the hybrid cache
+ * switch of cacheY_isai.h:192): *ncache = Y blocks per column kept in LDS (columns
+ * longer than that read Y back from global memory; 0 = every column), and
+ * *ordered_L / *ordered_U = 1 when that factor's columns were scheduled by
+ * descending pair count Q_j because the Q distribution is skewed. */
+int isai_handle_build_plan(isai_handle_t handle, int *ncache, int *ordered_L, int *ordered_U);
+
 /* Solve A x = b.  d_x holds the initial guess on entry.  Stops when the
  * recursive residual estimate reaches tol * ||b|| (then confirmed with the true
  * FP64 residual, iterating on if it is not met) or after maxit iterations.

@@ -94,6 +94,8 @@ SIGNATURES: dict[str, list] = {
     "isai_build_upper": [I, I, P, P, P, P, P, P, P, P, I, P],
     "isai_build_upper_lookup": [I, I, P, P, P, P, P, I, P, P, I, P],
     "isai_build_lu": [I, I, P] + [P] * 7 + [P] * 7 + [I, P],
+    "isai_pair_stats": [I, P, P, P, P],
+    "isai_pair_column_stats": [I, P, P, P, P, P, P],
     # bsrilu.h
     "isai_bsrilu0_inplace": [I, I, I, P, P, P, P, P],
     "isai_bsrilu0_analysis": [I, I, I, P, P, P, P, P],
@@ -120,6 +122,7 @@ SIGNATURES: dict[str, list] = {
     "isai_precond_apply": [P, P, P, P],
     "isai_handle_status": [P, P],
     "isai_handle_set_spin_limit": [P, I],
+    "isai_handle_build_plan": [P, P, P, P],
     "isai_solve": [P, I, I, D, I, P, P, P, P, P],
     "isai_destroy": [P],
     # isai_adjoint.h
@@ -156,6 +159,22 @@ for _name, _args in SIGNATURES.items():
 class SetupOpts(C.Structure):
     """isai_setup_opts_t"""
     _fields_ = [("precond", I), ("precision", I), ("sptrsv_chunk", I), ("reserved", I * 5)]
+
+
+class PairStats(C.Structure):
+    """isai_pair_stats_t"""
+    _fields_ = ([("nb", I), ("mnnz", I), ("total_pairs", C.c_longlong),
+                 ("n_avg", D), ("q_avg", D), ("qlen_avg", D)]
+                + [(f"{k}_{p}", I) for k in ("n", "q", "qlen") for p in ("max", "p50", "p90", "p95", "p99")]
+                + [("bin_cols", I * 10), ("bin_max_q", I * 10),
+                   ("qlen_hist", C.c_longlong * 10), ("bin_sum_q", C.c_longlong * 10)])
+
+    def as_dict(self) -> dict:
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if isinstance(v, C.Array) else v
+        return d
 
 
 class IsaiError(RuntimeError):

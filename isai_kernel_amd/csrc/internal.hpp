@@ -28,6 +28,27 @@ int spmv_dot_groups(int mb);
 int spmv_dot_launch(int mb, const int *rp, const int *ci, const double *val, const double *x, double *y,
                     const double *a, int ndot, double *part, hipStream_t st, const int *skip, long long nnzb);
 
+// Q_j = PairPtr[Mbrow[j+1]] - PairPtr[Mbrow[j]] for every column and, through
+// the caller's device scratch d_acc (pair_acc_bytes()), the largest Q_j in *max_q
+// (synchronizes the stream).
+size_t pair_acc_bytes();
+int pair_column_q_max(int nb, const int *d_Mbrow, const int *d_PairPtr, int *d_Q, void *d_acc, int *max_q,
+                      hipStream_t st);
+
+// One ISAI factor for the column-wise build.  order (optional) lists the columns
+// in the order the workgroups take them; the values do not depend on it.
+struct FactorArgs {
+    const int *Mbrow, *Mcol, *PairPtr, *PairPLocal, *PairSrc;
+    const double *Dinv;
+    double *Mval;
+    const int *order;
+};
+
+// ISAI(L) and/or ISAI(U) in one launch; Nmax = LDS Y-cache blocks per column
+// (columns longer than the cache read Y back from Mval).
+int build_dispatch(int nb, int bs, const double *Aval, const FactorArgs *L, const FactorArgs *U, int Nmax,
+                   hipStream_t st);
+
 // Sync-free L then U solve of a plan, with the same skip convention.
 int sptrsv_solve_internal(isai_sptrsv_plan_t plan, const double *d_LU, const double *d_b, double *d_x,
                           hipStream_t st, const int *skip);

@@ -16,6 +16,7 @@
 // Mval (the "globalY" form, cacheY_isai.h:159-341 picks between the two the same
 // way).  Every column of both factors is solved in a single launch.
 #include "common.hpp"
+#include "internal.hpp"
 #include "isai_bsr.h"
 #include "scan.hpp"
 
@@ -147,12 +148,6 @@ struct IsaiShape {
     static constexpr int C = 64 / G;                                 // columns per wave
 };
 
-struct FactorArgs {
-    const int *Mbrow, *Mcol, *PairPtr, *PairPLocal, *PairSrc;
-    const double *Dinv;
-    double *Mval;
-};
-
 // blockIdx.x < nblk_lower builds ISAI(L) columns, the rest ISAI(U) columns.
 template <int BS, bool GLOBAL_Y>
 __global__ __launch_bounds__(64) void isai_build_kernel(int nb, const double *__restrict__ Aval,
@@ -168,7 +163,10 @@ __global__ __launch_bounds__(64) void isai_build_kernel(int nb, const double *__
     const int e = threadIdx.x % S::G;
     const bool lane_ok = e < B2;
     const int i = lane_ok ? e / BS : 0, jj = lane_ok ? e % BS : 0;
-    const int j = blk * S::C + g;
+    // column of this lane group: position blk * C + g of the factor's schedule
+    // (natural order, or the build plan's descending-Q order)
+    const int pos = blk * S::C + g;
+    const int j = (F.order && pos < nb) ? F.order[pos] : pos;
     double *rhsS = lds + g * (B2 + (GLOBAL_Y ? 0 : ncache * B2));
     double *Yc = rhsS + B2;
 

@@ -485,6 +485,7 @@ struct isai_handle {
     const double *d_val = nullptr;
     double *d_LU = nullptr;
     int nnzbL = 0, nnzbU = 0, nlevels = 0;
+    int plan_ncache = 0, plan_ordered[2] = {0, 0};  // ISAI build plan (isai_handle_build_plan)
     int *d_Lrow = nullptr, *d_Lcol = nullptr, *d_Urow = nullptr, *d_Ucol = nullptr;
     double *d_Lval = nullptr, *d_Uval = nullptr;
     void *d_Llp = nullptr, *d_Ulp = nullptr;  // reduced-precision copies
@@ -837,6 +838,31 @@ int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, con
     h->nnzbL = (int)F[0].mc.size();
     h->nnzbU = (int)F[1].mc.size();
 
+    // --- ISAI build plan, part 1: the LDS Y cache from the column lengths N_j of
+    // both factors (the statistics analyze_pair.h prints; the per-column switch of
+    // cacheY_isai.h:192).  Every column is cached when the longest fits
+    // kCacheCap blocks; otherwise the cache covers 99 % of the columns and the
+    // rest read Y back from global memory; otherwise no column is cached.  A
+    // cache of kCacheCap blocks keeps a 64-lane workgroup at 16 KB of LDS.
+    {
+        constexpr int kCacheCap = 31;
+        const int nmax = std::max(F[0].nmax, F[1].nmax);
+        std::vector<int> cnt((size_t)nmax + 1, 0);
+        for (int f = 0; f < 2; ++f)
+            for (int j = 0; j < nb; ++j) cnt[F[f].mb[j + 1] - F[f].mb[j]]++;
+        const long long k99 = (long long)(size_t)((double)(2LL * nb - 1) * 0.99);
+        long long cum = 0;
+        int p99 = nmax;
+        for (int v = 0; v <= nmax; ++v) {
+            cum += cnt[v];
+            if (cum > k99) {
+                p99 = v;
+                break;
+            }
+        }
+        h->plan_ncache = nmax <= kCacheCap ? nmax : (p99 <= kCacheCap ? p99 : 0);
+    }
+
     // --- ILU(0) on a copy of A
     isai_ilu_plan_t ilu = nullptr;
     int zp = -1;
@@ -852,7 +878,8 @@ int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, con
     // --- ISAI construction
     int *d_buf[2][5] = {};  // mb, mc, pp, pl, ps
     double *d_dinv[2] = {}, *d_mval[2] = {};
-    int *d_map[2] = {};
+    int *d_map[2] = {}, *d_order[2] = {}, *d_colq = nullptr;
+    void *d_acc = nullptr;
     int totals[2] = {0, 0};
     auto cleanup = [&]() {
         for (int f = 0; f < 2; ++f) {
@@ -860,7 +887,10 @@ int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, con
             (void)hipFree(d_dinv[f]);
             (void)hipFree(d_mval[f]);
             (void)hipFree(d_map[f]);
+            (void)hipFree(d_order[f]);
         }
+        (void)hipFree(d_colq);
+        (void)hipFree(d_acc);
     };
     for (int f = 0; f < 2; ++f) {
         const size_t nnz = F[f].mc.size();
@@ -917,10 +947,45 @@ int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, con
         s = isai_pairs_fill(nb, d_row, d_col, d_buf[f][0], d_buf[f][1], f == 1, d_buf[f][2], d_buf[f][3], d_buf[f][4], stream);
         if (s == ISAI_OK) s = isai_build_dinv(nb, bs, d_row, d_col, h->d_LU, f == 0, d_dinv[f], stream);
     }
-    if (s == ISAI_OK)
-        s = isai_build_lu(nb, bs, h->d_LU, d_buf[0][0], d_buf[0][1], d_buf[0][2], d_buf[0][3], d_buf[0][4], d_dinv[0],
-                          d_mval[0], d_buf[1][0], d_buf[1][1], d_buf[1][2], d_buf[1][3], d_buf[1][4], d_dinv[1],
-                          d_mval[1], std::max(F[0].nmax, F[1].nmax), stream);
+    // --- ISAI build plan, part 2: a column's build time grows with its pair count
+    // Q_j, and the four columns of a wave run in lockstep.  When a factor's Q
+    // distribution is skewed (largest Q_j above kOrderSkew times the mean), its
+    // columns are scheduled by descending Q_j (stable counting sort): the longest
+    // columns start first and share waves with columns of similar length.
+    if (s == ISAI_OK && nb > 0) {
+        constexpr double kOrderSkew = 8.0;
+        if (hipMalloc(&d_colq, sizeof(int) * nb) != hipSuccess || hipMalloc(&d_acc, pair_acc_bytes()) != hipSuccess) {
+            cleanup();
+            set_error("isai_setup: allocation failed");
+            return fail(ISAI_ERR_ALLOC);
+        }
+        for (int f = 0; f < 2 && s == ISAI_OK; ++f) {
+            int qmax = 0;
+            s = pair_column_q_max(nb, d_buf[f][0], d_buf[f][2], d_colq, d_acc, &qmax, st);
+            const double qavg = (double)totals[f] / nb;
+            if (s != ISAI_OK || qmax <= kOrderSkew * std::max(qavg, 1.0)) continue;
+            std::vector<int> q(nb), order(nb);
+            std::vector<long long> start((size_t)qmax + 2, 0);
+            ISAI_HIP(hipMemcpyAsync(q.data(), d_colq, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+            ISAI_HIP(hipStreamSynchronize(st));
+            for (int j = 0; j < nb; ++j) start[qmax - q[j] + 1]++;  // bucket 0 = largest Q
+            for (int v = 0; v <= qmax; ++v) start[v + 1] += start[v];
+            for (int j = 0; j < nb; ++j) order[start[qmax - q[j]]++] = j;
+            if (hipMalloc(&d_order[f], sizeof(int) * nb) != hipSuccess) {
+                cleanup();
+                set_error("isai_setup: allocation failed");
+                return fail(ISAI_ERR_ALLOC);
+            }
+            ISAI_HIP(hipMemcpyAsync(d_order[f], order.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
+            ISAI_HIP(hipStreamSynchronize(st));
+            h->plan_ordered[f] = 1;
+        }
+    }
+    if (s == ISAI_OK) {
+        FactorArgs Lf{d_buf[0][0], d_buf[0][1], d_buf[0][2], d_buf[0][3], d_buf[0][4], d_dinv[0], d_mval[0], d_order[0]};
+        FactorArgs Uf{d_buf[1][0], d_buf[1][1], d_buf[1][2], d_buf[1][3], d_buf[1][4], d_dinv[1], d_mval[1], d_order[1]};
+        s = build_dispatch(nb, bs, h->d_LU, &Lf, &Uf, h->plan_ncache, st);
+    }
     for (int f = 0; f < 2 && s == ISAI_OK; ++f)
         s = isai_reorder_bcsc_to_bcsr((int)F[f].mc.size(), bs, d_map[f], d_mval[f], factor_vals[f], stream);
     if (s == ISAI_OK && h->precision != ISAI_PREC_FP64) s = convert_lp(h, h->precision, st);
@@ -1135,6 +1200,14 @@ int isai_handle_status(isai_handle_t h, isai_stream_t stream) {
     ISAI_REQUIRE(h, "isai_handle_status: null handle");
     if (h->sv) return isai_sptrsv_status(h->sv, stream);
     ISAI_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return ISAI_OK;
+}
+
+int isai_handle_build_plan(isai_handle_t h, int *ncache, int *ordered_L, int *ordered_U) {
+    ISAI_REQUIRE(h, "isai_handle_build_plan: null handle");
+    if (ncache) *ncache = h->plan_ncache;
+    if (ordered_L) *ordered_L = h->plan_ordered[0];
+    if (ordered_U) *ordered_U = h->plan_ordered[1];
     return ISAI_OK;
 }
 

@@ -54,6 +54,12 @@ class Handle:
         call("isai_handle_factors", self.h, *[C.byref(p) for p in ptrs])
         return [p.value for p in ptrs]
 
+    def build_plan(self) -> dict:
+        """The ISAI build plan chosen from the pair statistics (isai_handle_build_plan)."""
+        n, ol, ou = C.c_int(), C.c_int(), C.c_int()
+        call("isai_handle_build_plan", self.h, C.byref(n), C.byref(ol), C.byref(ou))
+        return {"ncache": n.value, "ordered_L": ol.value, "ordered_U": ou.value}
+
     def set_preconditioner(self, precond: str, precision: str = "fp64") -> None:
         call("isai_set_preconditioner", self.h, PRECOND[precond], PRECISION[precision], self.stream)
 
@@ -80,6 +86,22 @@ class Handle:
             self.close()
         except Exception:
             pass
+
+
+def pair_stats(d_mbrow: DeviceArray, d_pair_ptr: DeviceArray, nb: int, stream=None) -> dict:
+    """isai_pair_stats: the analyze_pair.h figures of one factor's pair lists."""
+    from ._ffi import PairStats
+    out = PairStats()
+    call("isai_pair_stats", nb, d_mbrow.ptr, d_pair_ptr.ptr, C.byref(out), stream)
+    return out.as_dict()
+
+
+def pair_column_stats(d_mbrow: DeviceArray, d_pair_ptr: DeviceArray, nb: int, stream=None):
+    """isai_pair_column_stats: per-column N_j, Q_j and largest qlen (numpy arrays)."""
+    outs = [DeviceArray((max(nb, 1),), np.int32) for _ in range(3)]
+    call("isai_pair_column_stats", nb, d_mbrow.ptr, d_pair_ptr.ptr, *[o.ptr for o in outs], stream)
+    call("isai_device_synchronize")
+    return [o.to_numpy()[:nb] for o in outs]
 
 
 class Batch:

@@ -33,6 +33,9 @@
  *                        U' upper; equality with ora_bsrilu0 of A^T on patterns
  *                        where the block ILU(0) is exact (full pattern).
  *   ora_bsr_outer_grad   dense -lam x^T masked by the block pattern.
+ *   ora_pair_stats       numpy brute force (np.sort and explicit index, bincount,
+ *                        np.digitize) on random pair pointers, and a hand-counted
+ *                        small case.
  */
 #include "oracle.h"
 
@@ -54,6 +57,82 @@ int ora_num_threads(void) {
 static int cmp_int(const void *a, const void *b) {
     int x = *(const int *)a, y = *(const int *)b;
     return (x > y) - (x < y);
+}
+
+/* ------------------------------------------------------------ pair statistics */
+static int pct_sorted(const int *v, size_t n, double p) {
+    if (n == 0) return 0;
+    return v[(size_t)((double)(n - 1) * p)];
+}
+
+int ora_pair_stats(int nb, const int *Mbrow, const int *PairPtr, int *ints, long long *i64, double *dbl,
+                   int *colN, int *colQ, int *colMaxQlen) {
+    if (nb < 0 || !Mbrow || !PairPtr || !ints || !i64 || !dbl) return ORA_ERR_INVALID;
+    const int mnnz = Mbrow[nb];
+    int *N = (int *)malloc(sizeof(int) * (nb > 0 ? nb : 1));
+    int *Q = (int *)malloc(sizeof(int) * (nb > 0 ? nb : 1));
+    int *ql = (int *)malloc(sizeof(int) * (mnnz > 0 ? mnnz : 1));
+    if (!N || !Q || !ql) {
+        free(N), free(Q), free(ql);
+        return ORA_ERR_INVALID;
+    }
+    long long totalN = 0, totalQ = 0, hist[10] = {0}, bsum[10] = {0};
+    int maxN = 0, maxQ = 0, maxQlen = 0, bcols[10] = {0}, bmax[10] = {0};
+    for (int j = 0; j < nb; ++j) {
+        const int st = Mbrow[j], en = Mbrow[j + 1];
+        int Qj = 0, mq = 0;
+        for (int g = st; g < en; ++g) {
+            const int q = PairPtr[g + 1] - PairPtr[g];
+            ql[g] = q;
+            Qj += q;
+            if (q > mq) mq = q;
+            hist[q == 0 ? 0 : q == 1 ? 1 : q < 4 ? 2 : q < 8 ? 3 : q < 16 ? 4 : q < 32 ? 5 : q < 64 ? 6
+                 : q < 128 ? 7 : q < 256 ? 8 : 9]++;
+        }
+        N[j] = en - st;
+        Q[j] = Qj;
+        totalN += en - st;
+        totalQ += Qj;
+        if (en - st > maxN) maxN = en - st;
+        if (Qj > maxQ) maxQ = Qj;
+        if (mq > maxQlen) maxQlen = mq;
+        int b = (int)((long long)j * 10 / nb);
+        if (b >= 10) b = 9;
+        bsum[b] += Qj;
+        bcols[b]++;
+        if (Qj > bmax[b]) bmax[b] = Qj;
+        if (colN) colN[j] = en - st;
+        if (colQ) colQ[j] = Qj;
+        if (colMaxQlen) colMaxQlen[j] = mq;
+    }
+    qsort(N, (size_t)nb, sizeof(int), cmp_int);
+    qsort(Q, (size_t)nb, sizeof(int), cmp_int);
+    qsort(ql, (size_t)mnnz, sizeof(int), cmp_int);
+    static const double P[4] = {0.50, 0.90, 0.95, 0.99};
+    ints[0] = nb;
+    ints[1] = mnnz;
+    ints[2] = maxN;
+    ints[3] = maxQ;
+    ints[4] = maxQlen;
+    for (int k = 0; k < 4; ++k) {
+        ints[5 + k] = pct_sorted(N, (size_t)nb, P[k]);
+        ints[9 + k] = pct_sorted(Q, (size_t)nb, P[k]);
+        ints[13 + k] = pct_sorted(ql, (size_t)mnnz, P[k]);
+    }
+    for (int b = 0; b < 10; ++b) {
+        ints[17 + b] = bcols[b];
+        ints[27 + b] = bmax[b];
+    }
+    i64[0] = totalQ;
+    for (int b = 0; b < 10; ++b) {
+        i64[1 + b] = hist[b];
+        i64[11 + b] = bsum[b];
+    }
+    dbl[0] = nb ? (double)totalN / nb : 0.0;
+    dbl[1] = nb ? (double)totalQ / nb : 0.0;
+    dbl[2] = mnnz ? (double)totalQ / mnnz : 0.0;
+    free(N), free(Q), free(ql);
+    return ORA_OK;
 }
 
 /* ------------------------------------------------------------------ csr2bsr */

@@ -79,6 +79,19 @@ int ora_pairs_lower(int nb, const int *Arow, const int *Acol, const int *Mbrow, 
 int ora_pairs_upper(int nb, const int *Arow, const int *Acol, const int *Mbrow, const int *Mcol,
                     int *PairPtr, int *PairPLocal, int *PairSrc, int cap, int *total);
 
+/* analyze_pair.h:6-163 (analyze_pair_lengths) and :250-301 (analyze_column_q_bins,
+ * numBins = 10).  Per column j: N_j = Mbrow[j+1]-Mbrow[j], Q_j = sum of the pair
+ * counts qlen = PairPtr[g+1]-PairPtr[g] over its entries g.  Percentiles follow
+ * the reference: sort ascending, take v[(size_t)((size-1) * p)].  Averages: total
+ * N / nb, total Q / nb, total Q / Mnnz (0 when empty).  qlen histogram bins: 0, 1,
+ * 2-3, 4-7, 8-15, 16-31, 32-63, 64-127, 128-255, >= 256.  Column bin of j:
+ * j * 10 / nb (int64), capped at 9.  ints[]: nb, Mnnz, maxN, maxQ, maxQlen,
+ * p50/p90/p95/p99 of N, Q and qlen (12 values), bin_cols[10], bin_maxQ[10]
+ * (39 ints).  i64[]: totalQ, qlen_hist[10], bin_sumQ[10] (21).  dbl[]: avgN,
+ * avgQ, avgQlen (3).  Per-column arrays colN/colQ/colMaxQlen may be NULL. */
+int ora_pair_stats(int nb, const int *Mbrow, const int *PairPtr, int *ints, long long *i64, double *dbl,
+                   int *colN, int *colQ, int *colMaxQlen);
+
 /* isai_bsr.h:60-88 (invert5_dense): Gauss-Jordan without pivoting, fails when
  * |pivot| < eps.  Returns 1 on success, 0 on failure. */
 int ora_invert_block(int bs, const double *A, double *Ainv, double eps);

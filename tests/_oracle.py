@@ -49,6 +49,7 @@ _SIGS = {
     "ora_bsr_transpose": ([I, I, P, P, P, P, P, P], None),
     "ora_transpose_packed_lu": ([I, I, P, P, P, P, P, P], I),
     "ora_bsr_outer_grad": ([I, I, P, P, P, P, D, P], None),
+    "ora_pair_stats": ([I, P, P, P, P, P, P, P, P], I),
 }
 for _n, (_a, _r) in _SIGS.items():
     _f = getattr(ora, _n)
@@ -60,12 +61,26 @@ def _p(a):
     return None if a is None else a.ctypes.data
 
 
+# Converted copies stay referenced here until well after the call that uses their
+# address: `_p(_i32(x))` would otherwise hand C the address of a freed temporary
+# whenever x is not already contiguous int32 / float64.
+_KEEP: list = []
+
+
+def _keep(a, r):
+    if r is not a:
+        _KEEP.append(r)
+        if len(_KEEP) > 256:
+            del _KEEP[:128]
+    return r
+
+
 def _i32(a):
-    return np.ascontiguousarray(a, np.int32)
+    return _keep(a, np.ascontiguousarray(a, np.int32))
 
 
 def _f64(a):
-    return np.ascontiguousarray(a, np.float64)
+    return _keep(a, np.ascontiguousarray(a, np.float64))
 
 
 class PrecondT(C.Structure):
@@ -310,6 +325,34 @@ def outer_grad(rp, ci, lam, x, alpha, bs=4):
     g = np.empty((len(ci), bs, bs))
     ora.ora_bsr_outer_grad(nb, bs, _p(_i32(rp)), _p(_i32(ci)), _p(_f64(lam)), _p(_f64(x)), alpha, _p(g))
     return g
+
+
+PCTS = (50, 90, 95, 99)
+
+
+def stats_dict(ints, i64, dbl):
+    """Pair statistics as a dict, from the 39 ints / 21 int64 / 3 doubles layout
+    shared by ora_pair_stats and isai_pair_stats."""
+    d = {"nb": int(ints[0]), "mnnz": int(ints[1]), "n_max": int(ints[2]), "q_max": int(ints[3]),
+         "qlen_max": int(ints[4]), "total_pairs": int(i64[0]),
+         "n_avg": float(dbl[0]), "q_avg": float(dbl[1]), "qlen_avg": float(dbl[2]),
+         "qlen_hist": [int(v) for v in i64[1:11]], "bin_sum_q": [int(v) for v in i64[11:21]],
+         "bin_cols": [int(v) for v in ints[17:27]], "bin_max_q": [int(v) for v in ints[27:37]]}
+    for k, pc in enumerate(PCTS):
+        d[f"n_p{pc}"], d[f"q_p{pc}"], d[f"qlen_p{pc}"] = int(ints[5 + k]), int(ints[9 + k]), int(ints[13 + k])
+    return d
+
+
+def pair_stats(mb, pair_ptr):
+    """analyze_pair_lengths / analyze_column_q_bins figures and the per-column
+    (N, Q, max qlen) arrays."""
+    nb = len(mb) - 1
+    ints, i64, dbl = np.zeros(39, np.int32), np.zeros(21, np.int64), np.zeros(3)
+    cn, cq, cm = np.zeros(nb, np.int32), np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+    st = ora.ora_pair_stats(nb, _p(_i32(mb)), _p(_i32(pair_ptr)), _p(ints), _p(i64), _p(dbl), _p(cn), _p(cq), _p(cm))
+    if st:
+        raise ValueError(f"ora_pair_stats failed {st}")
+    return stats_dict(ints, i64, dbl), cn, cq, cm
 
 
 def num_threads():

@@ -171,6 +171,59 @@ def test_pairs_brute_force(upper):
             assert got == exp
 
 
+def _stats_brute(mb, pp):
+    """analyze_pair_lengths / analyze_column_q_bins by numpy: sorted arrays indexed
+    at int((n-1)*p), np.digitize for the qlen bins, explicit column bins."""
+    nb = len(mb) - 1
+    qlen = np.diff(pp)[: mb[-1]]
+    N = np.diff(mb)
+    Q = np.array([qlen[mb[j]:mb[j + 1]].sum() for j in range(nb)], np.int64)
+    pct = lambda v, p: int(np.sort(v)[int((len(v) - 1) * p)]) if len(v) else 0
+    edges = [1, 2, 4, 8, 16, 32, 64, 128, 256]
+    hist = np.bincount(np.digitize(qlen, edges), minlength=10)
+    b = np.minimum(np.arange(nb, dtype=np.int64) * 10 // nb, 9)
+    d = {"nb": nb, "mnnz": int(mb[-1]), "n_max": int(N.max()), "q_max": int(Q.max()),
+         "qlen_max": int(qlen.max()) if len(qlen) else 0, "total_pairs": int(Q.sum()),
+         "n_avg": N.sum() / nb, "q_avg": Q.sum() / nb, "qlen_avg": Q.sum() / len(qlen) if len(qlen) else 0.0,
+         "qlen_hist": [int(v) for v in hist], "bin_sum_q": [int(Q[b == k].sum()) for k in range(10)],
+         "bin_cols": [int((b == k).sum()) for k in range(10)],
+         "bin_max_q": [int(Q[b == k].max()) if (b == k).any() else 0 for k in range(10)]}
+    for pc in O.PCTS:
+        d[f"n_p{pc}"], d[f"q_p{pc}"], d[f"qlen_p{pc}"] = pct(N, pc / 100), pct(Q, pc / 100), pct(qlen, pc / 100)
+    return d, N, Q
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_pair_stats_brute_force(seed):
+    """analyze_pair.h statistics against numpy on random column lengths and pair
+    counts, with qlen values spread over every histogram bin (0 .. 300)."""
+    rng = np.random.default_rng(seed)
+    nb = int(rng.integers(1, 400))
+    N = rng.integers(0, 12, nb)
+    mb = np.concatenate([[0], np.cumsum(N)]).astype(np.int32)
+    qlen = rng.choice([0, 1, 2, 3, 5, 9, 17, 40, 90, 200, 300], size=mb[-1])
+    pp = np.concatenate([[0], np.cumsum(qlen)]).astype(np.int32)
+    got, cn, cq, cm = O.pair_stats(mb, pp)
+    exp, EN, EQ = _stats_brute(mb, pp)
+    assert got == exp
+    assert np.array_equal(cn, EN) and np.array_equal(cq, EQ)
+    assert np.array_equal(cm, [qlen[mb[j]:mb[j + 1]].max() if N[j] else 0 for j in range(nb)])
+
+
+def test_pair_stats_hand_case():
+    """Three columns with N = 1, 2, 0 and pair counts (3), (0, 1): counted by hand."""
+    got, cn, cq, cm = O.pair_stats(np.array([0, 1, 3, 3]), np.array([0, 3, 3, 4]))
+    assert (got["mnnz"], got["total_pairs"], got["n_max"], got["q_max"], got["qlen_max"]) == (3, 4, 2, 3, 3)
+    assert list(cn) == [1, 2, 0] and list(cq) == [3, 1, 0] and list(cm) == [3, 1, 0]
+    # sorted N = 0 1 2, Q = 0 1 3, qlen = 0 1 3: index int(2 p) is 1 for p = .5, .9, .95, .99
+    assert (got["n_p50"], got["n_p99"], got["q_p50"], got["q_p90"], got["qlen_p95"]) == (1, 1, 1, 1, 1)
+    assert got["qlen_hist"][:3] == [1, 1, 1] and sum(got["qlen_hist"]) == 3
+    # column bins j * 10 // 3 = 0, 3, 6
+    assert got["bin_cols"][0] == got["bin_cols"][3] == got["bin_cols"][6] == 1
+    assert got["bin_sum_q"][0] == 3 and got["bin_sum_q"][3] == 1 and got["bin_max_q"][6] == 0
+    assert got["n_avg"] == 1.0 and got["q_avg"] == 4 / 3 and got["qlen_avg"] == 4 / 3
+
+
 # ------------------------------------------------------------ block inverse
 def test_invert_block_and_guard():
     for bs in (1, 3, 4, 5, 7):
