@@ -25,7 +25,9 @@ int spmv_lowp_launch(int mb, long long nnzb, isai_precision_t prec, const int *r
 // the rows of workgroup wg, and with ndot = 2 also part[G + wg] = sum y.y, where
 // G = spmv_dot_groups(mb) partials per dot; nnzb picks the row batch width.
 int spmv_dot_groups(int mb);
-int spmv_dot_launch(int mb, const int *rp, const int *ci, const double *val, const double *x, double *y,
+// x_f32: x holds FP32 values (widened exactly on load), e.g. the output of a
+// reduced-precision ISAI apply.
+int spmv_dot_launch(int mb, const int *rp, const int *ci, const double *val, const void *x, bool x_f32, double *y,
                     const double *a, int ndot, double *part, hipStream_t st, const int *skip, long long nnzb);
 
 // Q_j = PairPtr[Mbrow[j+1]] - PairPtr[Mbrow[j]] for every column and, through

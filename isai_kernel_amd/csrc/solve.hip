@@ -221,27 +221,29 @@ __global__ __launch_bounds__(kRedThreads) void bicg_snorm_final(Scalars *S) {
     }
 }
 
-// x += alpha phat, only when the solve converged on s.
-__global__ void bicg_xs_kernel(size_t n, double *x, const double *__restrict__ phat, const Scalars *S) {
-    if (S->done != 2) return;
-    const double alpha = S->alpha;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        x[i] += alpha * phat[i];
-}
-
 // x += alpha phat + omega shat; r = s - omega t; partials ||r||^2 and rhat.r (the
-// next iteration's rho)
+// next iteration's rho).  When the solve converged on s (done = 2) it only adds
+// alpha phat.  HatT: double, or float when the preconditioner's output is held in
+// FP32 (a reduced-precision ISAI apply, whose values are FP32 numbers either way).
+template <typename HatT>
 __global__ __launch_bounds__(kRedThreads) void bicg_xr_kernel(size_t n, double *x, double *r,
-                                                              const double *__restrict__ phat,
-                                                              const double *__restrict__ shat,
+                                                              const HatT *__restrict__ phat,
+                                                              const HatT *__restrict__ shat,
                                                               const double *__restrict__ s,
                                                               const double *__restrict__ t,
                                                               const double *__restrict__ rhat, Scalars *S) {
-    if (skipped(S)) return;
+    const int done = S->done;
+    if (done == 2) {
+        const double alpha = S->alpha;
+        for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads)
+            x[i] += alpha * (double)phat[i];
+        return;
+    }
+    if (done) return;
     const double alpha = S->alpha, omega = S->omega;
     double acc = 0.0, accr = 0.0;
     for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads) {
-        x[i] += alpha * phat[i] + omega * shat[i];
+        x[i] += alpha * (double)phat[i] + omega * (double)shat[i];
         const double ri = s[i] - omega * t[i];
         r[i] = ri;
         acc += ri * ri;
@@ -542,8 +544,16 @@ int convert_lp(isai_handle *h, int prec, hipStream_t st) {
     return s;
 }
 
-// z = M v; tmp is a work vector (ISAI form)
-int precond_apply(isai_handle *h, const double *v, double *z, double *tmp, hipStream_t st, const int *skip) {
+// True when the preconditioner's results are FP32 numbers (the reduced-precision
+// ISAI forms), so that they can be held in FP32 without changing a bit.
+bool apply_out_f32(const isai_handle *h) {
+    return h->precond == ISAI_PRECOND_ISAI && h->precision != ISAI_PREC_FP64 && h->bs == 4;
+}
+
+// z = M v; tmp is a work vector (ISAI form).  out_f32 (only when apply_out_f32):
+// z is written as FP32 values.
+int precond_apply(isai_handle *h, const double *v, double *z, double *tmp, hipStream_t st, const int *skip,
+                  bool out_f32) {
     switch (h->precond) {
         case ISAI_PRECOND_NONE:
             ISAI_HIP(hipMemcpyAsync(z, v, sizeof(double) * h->n, hipMemcpyDeviceToDevice, st));
@@ -557,7 +567,8 @@ int precond_apply(isai_handle *h, const double *v, double *z, double *tmp, hipSt
                 // half the intermediate's bytes, same result
                 int s = spmv_lowp_launch(h->nb, h->nnzbL, p, h->d_Lrow, h->d_Lcol, L, v, false, tmp, true, st, skip);
                 if (s == ISAI_OK)
-                    s = spmv_lowp_launch(h->nb, h->nnzbU, p, h->d_Urow, h->d_Ucol, U, tmp, true, z, false, st, skip);
+                    s = spmv_lowp_launch(h->nb, h->nnzbU, p, h->d_Urow, h->d_Ucol, U, tmp, true, z, out_f32, st,
+                                         skip);
                 return s;
             }
             int s = spmv_launch(h->nb, h->nnzbL, h->bs, p, h->d_Lrow, h->d_Lcol, L, v, tmp, st, skip);
@@ -617,6 +628,9 @@ int bicgstab(isai_handle *h, double tol, int maxit, const double *b, double *x, 
     const int *skip = &h->S->done;
     const int G = spmv_dot_groups(h->nb);
     double *part = h->d_wgpart;
+    // p^ and s^ held in FP32 when the ISAI apply produces FP32 numbers: the A
+    // products and the x update read the same values with half the bytes
+    const bool hat32 = apply_out_f32(h);
     *iters = 0;
     bicg_start_kernel<<<1, 1, 0, st>>>(h->S, maxit);
     for (int restart = 0; restart < 8; ++restart) {
@@ -629,16 +643,21 @@ int bicgstab(isai_handle *h, double tol, int maxit, const double *b, double *x, 
         for (int launched = h->h_done[2]; !done && launched < maxit;) {
             for (int k = 0; k < kItersPerRead && launched < maxit; ++k, ++launched) {
                 bicg_p_kernel<<<vec_grid(n), 256, 0, st>>>(n, r, p, v, h->S);
-                TRY(precond_apply(h, p, ph, tmp, st, skip));
-                TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, ph, v, rh, 1, part, st, skip, h->nnzb));
+                TRY(precond_apply(h, p, ph, tmp, st, skip, hat32));
+                TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, ph, hat32, v, rh, 1, part, st, skip,
+                                    h->nnzb));
                 bicg_alpha_final_wg<<<1, kWgSumThreads, 0, st>>>(h->S, part, G);
                 bicg_s_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, r, v, s, h->S);
                 bicg_snorm_final<<<1, kRedThreads, 0, st>>>(h->S);
-                bicg_xs_kernel<<<vec_grid(n), 256, 0, st>>>(n, x, ph, h->S);
-                TRY(precond_apply(h, s, sh, tmp, st, skip));
-                TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, sh, t, s, 2, part, st, skip, h->nnzb));
+                TRY(precond_apply(h, s, sh, tmp, st, skip, hat32));
+                TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, sh, hat32, t, s, 2, part, st, skip,
+                                    h->nnzb));
                 bicg_omega_final_wg<<<1, kWgSumThreads, 0, st>>>(h->S, part, G);
-                bicg_xr_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, x, r, ph, sh, s, t, rh, h->S);
+                if (hat32)
+                    bicg_xr_kernel<float><<<kRedBlocks, kRedThreads, 0, st>>>(n, x, r, (const float *)ph,
+                                                                             (const float *)sh, s, t, rh, h->S);
+                else
+                    bicg_xr_kernel<double><<<kRedBlocks, kRedThreads, 0, st>>>(n, x, r, ph, sh, s, t, rh, h->S);
                 bicg_rnorm_final<<<1, kRedThreads, 0, st>>>(h->S);
                 ISAI_LAUNCH_CHECK();
             }
@@ -680,7 +699,7 @@ int fgmres(isai_handle *h, int m, double tol, int maxit, const double *b, double
         const int steps = std::min(m, maxit - it);
         for (int j = 0; j < steps; ++j) {
             double *vj = V + n * j, *zj = Z + n * j;
-            TRY(precond_apply(h, vj, zj, tmp, st, skip));
+            TRY(precond_apply(h, vj, zj, tmp, st, skip, false));
             TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, zj, w, st, skip));
             const int nv = j + 1;
             const dim3 dgrid(kRedBlocks, (nv + kDotChunk - 1) / kDotChunk);
@@ -1172,7 +1191,7 @@ int isai_precond_apply(isai_handle_t h, const double *d_v, double *d_z, isai_str
     int s = ensure_work(h, 9);
     if (s != ISAI_OK) return s;
     double *tmp = h->work + 8 * h->n;
-    return precond_apply(h, d_v, d_z, tmp, (hipStream_t)stream, nullptr);
+    return precond_apply(h, d_v, d_z, tmp, (hipStream_t)stream, nullptr, false);
 }
 
 int isai_solve(isai_handle_t h, int solver, int restart, double tol, int maxit, const double *d_b, double *d_x,

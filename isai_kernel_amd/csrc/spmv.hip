@@ -34,27 +34,37 @@ constexpr int kSpmvF64Block = 128;  // plain FP64 product: 16 block rows per wor
 // a group share the row and take the same path, so the shuffles only read active
 // lanes.  Products accumulate block by block into two partial sums per lane, the
 // same order for every KB.
-template <int KB>
+// x2: the FP64 vector as pairs; with XF32 it holds FP32 values (pairs of floats),
+// each widened exactly to FP64 on load.
+template <int KB, bool XF32 = false>
 __device__ __forceinline__ double bsr4_row_f64(int rs, int len, int s, const int *__restrict__ ci,
-                                               const v2d *__restrict__ val2, const v2d *__restrict__ x2) {
+                                               const v2d *__restrict__ val2, const void *__restrict__ x2v) {
     static_assert(KB == 4 || KB == 8 || KB == 16, "KB");
+    const v2d *x2 = (const v2d *)x2v;
+    const float2 *xf2 = (const float2 *)x2v;
     const int h = s & 1;
     double acc0 = 0.0, acc1 = 0.0;
     for (int t = 0; t < len; t += KB) {
         const int clo = s < KB && t + s < len ? ci[rs + t + s] : 0;
         const int chi = KB > 8 && t + 8 + s < len ? ci[rs + t + 8 + s] : 0;
         v2d a[KB], xv[KB];
+        float2 xf[KB];
 #pragma unroll
         for (int u = 0; u < KB; ++u)
             if (t + u < len) a[u] = __builtin_nontemporal_load(&val2[(size_t)(rs + t + u) * 8 + s]);
+        // every gather of the batch is issued before any of them is used
 #pragma unroll
         for (int u = 0; u < KB; ++u) {
             const int c = u < 8 ? __shfl(clo, u, 8) : __shfl(chi, u - 8, 8);
-            if (t + u < len) xv[u] = x2[(size_t)c * 2 + h];
+            if (t + u < len) {
+                if (XF32) xf[u] = xf2[(size_t)c * 2 + h];
+                else xv[u] = x2[(size_t)c * 2 + h];
+            }
         }
 #pragma unroll
         for (int u = 0; u < KB; ++u)
             if (t + u < len) {
+                if (XF32) xv[u] = v2d{(double)xf[u].x, (double)xf[u].y};
                 acc0 = fma(a[u].x, xv[u].x, acc0);
                 acc1 = fma(a[u].y, xv[u].y, acc1);
             }
@@ -93,10 +103,10 @@ __global__ __launch_bounds__(kSpmvF64Block) void bsr4_spmv_f64_kernel(
 // its partial sums to part[wg] (and part[G + wg]), in a fixed order, so the final
 // sum over the G = gridDim.x partials is deterministic.  32 block rows per workgroup
 // keep the partial count at nb / 32.
-template <int NDOT, int KB>
+template <int NDOT, int KB, bool XF32>
 __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f64_dot_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci, const v2d *__restrict__ val2,
-    const v2d *__restrict__ x2, double *__restrict__ y, const double *__restrict__ a, double *__restrict__ part,
+    const void *__restrict__ x2, double *__restrict__ y, const double *__restrict__ a, double *__restrict__ part,
     const int *__restrict__ skip) {
     if (skip && *skip) return;
     const int wg = blockIdx.x;
@@ -104,7 +114,7 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f64_dot_kernel(
     const int br = wg * (kSpmvBlock / 8) + (threadIdx.x >> 3);
     const int rs = br < nb ? rp[br] : 0;
     const int len = br < nb ? rp[br + 1] - rs : 0;
-    const double sum = bsr4_row_f64<KB>(rs, len, s, ci, val2, x2);
+    const double sum = bsr4_row_f64<KB, XF32>(rs, len, s, ci, val2, x2);
     double d0 = 0.0, d1 = 0.0;
     if (br < nb && (s & 1) == 0) {
         const size_t row = (size_t)br * 4 + (s >> 1);
@@ -137,25 +147,26 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_f64_dot_kernel(
 
 int spmv_dot_groups(int mb) { return (int)grid_for(mb, kSpmvBlock / 8); }
 
-template <int NDOT>
-static void dot_launch(unsigned g, int kb, int mb, const int *rp, const int *ci, const v2d *val, const v2d *x,
+template <int NDOT, bool XF32>
+static void dot_launch(unsigned g, int kb, int mb, const int *rp, const int *ci, const v2d *val, const void *x,
                        double *y, const double *a, double *part, hipStream_t st, const int *skip) {
     if (kb == 4)
-        bsr4_spmv_f64_dot_kernel<NDOT, 4><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, val, x, y, a, part, skip);
+        bsr4_spmv_f64_dot_kernel<NDOT, 4, XF32><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, val, x, y, a, part, skip);
     else if (kb == 8)
-        bsr4_spmv_f64_dot_kernel<NDOT, 8><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, val, x, y, a, part, skip);
+        bsr4_spmv_f64_dot_kernel<NDOT, 8, XF32><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, val, x, y, a, part, skip);
     else
-        bsr4_spmv_f64_dot_kernel<NDOT, 16><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, val, x, y, a, part, skip);
+        bsr4_spmv_f64_dot_kernel<NDOT, 16, XF32><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, val, x, y, a, part, skip);
 }
 
-int spmv_dot_launch(int mb, const int *rp, const int *ci, const double *val, const double *x, double *y,
+int spmv_dot_launch(int mb, const int *rp, const int *ci, const double *val, const void *x, bool x_f32, double *y,
                     const double *a, int ndot, double *part, hipStream_t st, const int *skip, long long nnzb) {
     const unsigned g = (unsigned)spmv_dot_groups(mb);
     const int kb = f64_batch(nnzb, mb);
-    if (ndot == 1)
-        dot_launch<1>(g, kb, mb, rp, ci, (const v2d *)val, (const v2d *)x, y, a, part, st, skip);
-    else
-        dot_launch<2>(g, kb, mb, rp, ci, (const v2d *)val, (const v2d *)x, y, a, part, st, skip);
+    const v2d *v = (const v2d *)val;
+    if (ndot == 1 && x_f32) dot_launch<1, true>(g, kb, mb, rp, ci, v, x, y, a, part, st, skip);
+    else if (ndot == 1) dot_launch<1, false>(g, kb, mb, rp, ci, v, x, y, a, part, st, skip);
+    else if (x_f32) dot_launch<2, true>(g, kb, mb, rp, ci, v, x, y, a, part, st, skip);
+    else dot_launch<2, false>(g, kb, mb, rp, ci, v, x, y, a, part, st, skip);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
