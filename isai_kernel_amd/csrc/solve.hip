@@ -78,6 +78,33 @@ __device__ __forceinline__ double final_sum(const Scalars *S, int slot) {
     return block_sum(v);
 }
 
+// final_sum of both slots in one pass (each sum in final_sum's order).
+__device__ __forceinline__ void final_sum2(const Scalars *S, double &s0, double &s1) {
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < kRedBlocks; k += kRedThreads) {
+        a += S->partial[0][k];
+        b += S->partial[1][k];
+    }
+    __shared__ double wa[kRedThreads / 64], wb[kRedThreads / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        wa[threadIdx.x >> 6] = a;
+        wb[threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    s0 = s1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < kRedThreads / 64; ++k) {
+        s0 += wa[k];
+        s1 += wb[k];
+    }
+}
+
 __device__ __forceinline__ bool skipped(const Scalars *S) { return S->done != 0; }
 
 // partial[0] += a.b ; partial[1] += a.c (when c != null)
@@ -128,7 +155,8 @@ __global__ void bicg_start_kernel(Scalars *S, int maxit) {
 
 // rhat = r, so the first rho = ||r||^2
 __global__ __launch_bounds__(kRedThreads) void bicg_init_final(Scalars *S, double tol) {
-    const double b2 = final_sum(S, 0), r2 = final_sum(S, 1);
+    double b2, r2;
+    final_sum2(S, b2, r2);
     if (threadIdx.x == 0) {
         S->bnorm2 = b2 > 0.0 ? b2 : 1.0;
         S->rnorm2 = r2;
@@ -165,6 +193,44 @@ __device__ __forceinline__ double wg_partial_sum(const double *part, int G) {
     return sum;
 }
 
+// The same sum for two partial arrays in one pass (each in the order of
+// wg_partial_sum, so the results are identical): 16 loads in flight per thread.
+__device__ __forceinline__ void wg_partial_sum2(const double *pa, const double *pb, int G, double &sa, double &sb) {
+    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, w[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int k = threadIdx.x;
+    for (; k + 7 * kWgSumThreads < G; k += 8 * kWgSumThreads) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            v[u] += pa[k + u * kWgSumThreads];
+            w[u] += pb[k + u * kWgSumThreads];
+        }
+    }
+    for (; k < G; k += kWgSumThreads) {
+        v[0] += pa[k];
+        w[0] += pb[k];
+    }
+    double ta = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+    double tb = ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
+    __shared__ double wsa[kWgSumThreads / 64], wsb[kWgSumThreads / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        ta += __shfl_xor(ta, off);
+        tb += __shfl_xor(tb, off);
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        wsa[threadIdx.x >> 6] = ta;
+        wsb[threadIdx.x >> 6] = tb;
+    }
+    __syncthreads();
+    sa = sb = 0.0;
+#pragma unroll
+    for (int q = 0; q < kWgSumThreads / 64; ++q) {
+        sa += wsa[q];
+        sb += wsb[q];
+    }
+}
+
 // alpha = rho / rhat.v, with rhat.v from the product's fused partials
 __global__ __launch_bounds__(kWgSumThreads) void bicg_alpha_final_wg(Scalars *S, const double *part, int G) {
     if (skipped(S)) return;
@@ -180,7 +246,8 @@ __global__ __launch_bounds__(kWgSumThreads) void bicg_alpha_final_wg(Scalars *S,
 // omega = t.s / t.t from the product's fused partials
 __global__ __launch_bounds__(kWgSumThreads) void bicg_omega_final_wg(Scalars *S, const double *part, int G) {
     if (skipped(S)) return;
-    const double ts = wg_partial_sum(part, G), tt = wg_partial_sum(part + G, G);
+    double ts, tt;
+    wg_partial_sum2(part, part + G, G, ts, tt);
     if (threadIdx.x == 0) {
         if (tt == 0.0) S->done = 3;
         else S->omega = ts / tt;
@@ -267,7 +334,8 @@ __global__ __launch_bounds__(kRedThreads) void bicg_rnorm_final(Scalars *S) {
         return;
     }
     if (skipped(S)) return;
-    const double r2 = final_sum(S, 0), rho = final_sum(S, 1);
+    double r2, rho;
+    final_sum2(S, r2, rho);
     if (threadIdx.x == 0) {
         S->rnorm2 = r2;
         if (r2 <= S->tol2) {
@@ -303,7 +371,8 @@ __global__ __launch_bounds__(kRedThreads) void residual_kernel(size_t n, const d
 }
 
 __global__ __launch_bounds__(kRedThreads) void residual_final(Scalars *S, double *out2) {
-    const double r2 = final_sum(S, 0), b2 = final_sum(S, 1);
+    double r2, b2;
+    final_sum2(S, r2, b2);
     if (threadIdx.x == 0) {
         out2[0] = r2;
         out2[1] = b2;
@@ -313,7 +382,8 @@ __global__ __launch_bounds__(kRedThreads) void residual_final(Scalars *S, double
 // ------------------------------------------------------------------- FGMRES
 // cycle start: w holds A x -> w = b - Ax, partial ||w||^2, ||b||^2
 __global__ __launch_bounds__(kRedThreads) void fg_beta_final(Scalars *S, double tol) {
-    const double w2 = final_sum(S, 0), b2 = final_sum(S, 1);
+    double w2, b2;
+    final_sum2(S, w2, b2);
     if (threadIdx.x == 0) {
         S->bnorm2 = b2 > 0.0 ? b2 : 1.0;
         S->tol2 = tol * tol * S->bnorm2;
