@@ -58,6 +58,17 @@ __device__ __forceinline__ double poll(const double *p, int limit, bool &timed_o
     return __longlong_as_double((long long)v);
 }
 
+// Value of lane K of the quad (lanes 0..3), in every lane of the quad: a DPP
+// quad_perm [K,K,K,K] move of both halves, no LDS round trip.
+template <int K>
+__device__ __forceinline__ double quad_bcast(double v) {
+    constexpr int ctrl = K | (K << 2) | (K << 4) | (K << 6);
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(u & 0xffffffffll), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), ctrl, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ __forceinline__ void publish(double *p, double v) {
     __hip_atomic_store((unsigned long long *)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
@@ -91,8 +102,10 @@ constexpr int kPipeRemote = 512; // remote items per chunk
 constexpr int kPipeLocal = 128;  // local items per chunk
 
 static size_t pipe_lds_bytes(int rows, int rem, int loc) {
+    // at least one remote item slot, so that the chain's clamped reads stay inside
+    rem = std::max(rem, 1);
     return sizeof(double) * ((size_t)rows * 8 + (size_t)rem * 4 + (size_t)loc * 16) +
-           sizeof(int) * ((size_t)rem * 3 + loc + 2 * ((size_t)rows + 1));
+           sizeof(int) * ((size_t)rem * 3 + 2 * (size_t)loc + 2 * ((size_t)rows + 1));
 }
 
 template <bool UPPER>
@@ -100,10 +113,12 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
     int nb, const int4 *__restrict__ chunks, const int *__restrict__ ci, const double *__restrict__ S,
     const double *__restrict__ dinv, const int *__restrict__ rem_ptr, const int *__restrict__ rem_items,
     const int *__restrict__ loc_ptr, const int *__restrict__ loc_items, const double *rhs, double *out,
-    double *sentinel_fill, Counters *cnt, const int *__restrict__ skip, int max_rows, int max_rem, int max_loc) {
+    double *sentinel_fill, Counters *cnt, const int *__restrict__ skip, int max_rows, int max_rem, int max_loc,
+    int poll_min) {
     if (skip && *skip) return;
     // LDS carved to the largest chunk of this triangle
     extern __shared__ double dyn[];
+    max_rem = max(max_rem, 1);
     double *zb = dyn;                         // max_rows * 4
     double *xs = zb + max_rows * 4;           // max_rows * 4
     double *rem = xs + max_rows * 4;          // max_rem * 4
@@ -114,6 +129,7 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
     int *lk = ip + max_rem;                   // max_loc
     int *rrp = lk + max_loc;                  // max_rows + 1
     int *lrp = rrp + max_rows + 1;            // max_rows + 1
+    int *lpos = lrp + max_rows + 1;           // max_loc: block position of the local item
     __shared__ int s_c, s_chain;
     const int lane = threadIdx.x;
     unsigned int *ticket = UPPER ? &cnt->ticket_u : &cnt->ticket_l;
@@ -128,24 +144,31 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
     const int nrows = qhi - qlo, nrem = ce.x - r0, nloc = ce.y - l0;
     auto row_of = [&](int q) { return UPPER ? nb - 1 - q : q; };
 
-    // ---- prologue
-    for (int t = lane; t < nrows * 4; t += 64) {
+    // ---- prologue.  The loops are unrolled so that several iterations' loads
+    // are in flight together: a chunk holds its residency slot while it stages,
+    // and one dependent load chain per iteration took a quarter of its life.
+    const int nz = nrows * 4;
+#pragma unroll 4
+    for (int t = lane; t < nz; t += 64) {
         const int rr = t >> 2, m = t & 3;
         const int i = row_of(qlo + rr);
         const double *D = dinv + (size_t)i * 16 + m * 4;
         const double *b = rhs + (size_t)i * 4;
         zb[t] = D[0] * b[0] + D[1] * b[1] + D[2] * b[2] + D[3] * b[3];
     }
+#pragma unroll 2
     for (int t = lane; t <= nrows; t += 64) {
         rrp[t] = rem_ptr[qlo + t] - r0;
         lrp[t] = loc_ptr[qlo + t] - l0;
     }
+#pragma unroll 2
     for (int t = lane; t < nloc; t += 64) {
         const int p = loc_items[l0 + t];
         const int k = ci[p];
+        lpos[t] = p;
         lk[t] = (UPPER ? nb - 1 - k : k) - qlo;
     }
-    for (int t = lane; t < nloc * 16; t += 64) locv[t] = S[(size_t)loc_items[l0 + (t >> 4)] * 16 + (t & 15)];
+#pragma unroll 4
     for (int t = lane; t < nrem; t += 64) {
         const int p = rem_items[r0 + t];
         ip[t] = p;
@@ -153,23 +176,43 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
         done[t] = 0;
     }
     __syncthreads();
+    // the local blocks: positions from LDS, all loads of the loop in flight
+#pragma unroll 4
+    for (int t = lane; t < nloc * 16; t += 64) locv[t] = S[(size_t)lpos[t >> 4] * 16 + (t & 15)];
+    __syncthreads();
 
     int chain = 0, stall = 0;
     bool timed_out = false;
+    // The S block of the item a lane polls is fetched once, when the item enters
+    // the lane's window (usually several rounds before its x is published), and
+    // kept in registers: a published x then costs one poll, not a poll followed
+    // by a dependent load of the block.
+    int held = -1;
+    v2d b0 = {0.0, 0.0}, b1 = b0, b2 = b0, b3 = b0, b4 = b0, b5 = b0, b6 = b0, b7 = b0;
+    // chain lanes: x of the row solved last (row xp_row of the chunk), all four components
+    int xp_row = -1;
+    double xp0 = 0.0, xp1 = 0.0, xp2 = 0.0, xp3 = 0.0;
+    // Polls are write-through loads served beyond L2, and every resident chunk
+    // polls at once, so the window adapts: a round polls about twice as many items
+    // as the previous round found published (at least kPollMin), instead of always
+    // 64 items up to 32 rows ahead of what the producers have reached.
+    int window = 64;
     while (chain < nrows) {
         // ---- (a) remote phase: one poll per pending item of the window
         int got = 0;
         const int w = rrp[chain] + lane;
-        if (w < nrem && !done[w]) {
-            const int p = ip[w];
+        if (lane < window && w < nrem && !done[w]) {
             const int k = ik[w];
             unsigned long long *xq = (unsigned long long *)(out + (size_t)k * 4);
             const unsigned long long u0 = __hip_atomic_load(xq + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const unsigned long long u1 = __hip_atomic_load(xq + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const unsigned long long u2 = __hip_atomic_load(xq + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const unsigned long long u3 = __hip_atomic_load(xq + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const v2d *B = (const v2d *)(S + (size_t)p * 16);
-            const v2d b0 = B[0], b1 = B[1], b2 = B[2], b3 = B[3], b4 = B[4], b5 = B[5], b6 = B[6], b7 = B[7];
+            if (w != held) {
+                const v2d *B = (const v2d *)(S + (size_t)ip[w] * 16);
+                b0 = B[0], b1 = B[1], b2 = B[2], b3 = B[3], b4 = B[4], b5 = B[5], b6 = B[6], b7 = B[7];
+                held = w;
+            }
             if (u0 != kSentinel && u1 != kSentinel && u2 != kSentinel && u3 != kSentinel) {
                 const double x0 = __longlong_as_double((long long)u0), x1 = __longlong_as_double((long long)u1);
                 const double x2 = __longlong_as_double((long long)u2), x3 = __longlong_as_double((long long)u3);
@@ -183,21 +226,59 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
         }
         __syncthreads();
         // ---- (b) chain phase: lanes 0..3 advance while the next row is complete
+        // Rows with at most 2 remote and 1 local item (stencil rows) take a fast
+        // path: all of the row's LDS reads are issued before any is used (indices
+        // clamped into the arrays, results selected afterwards), and the local item
+        // of the row solved just before reads its x from registers (quad broadcast)
+        // instead of LDS.  The arithmetic and its order are those of the loop path.
         if (lane < 4) {
             int adv = 0;
             while (chain < nrows && adv < 64) {
                 const int ra = rrp[chain], rb = rrp[chain + 1];
-                bool ready = true;
-                for (int t = ra; t < rb; ++t) ready = ready && done[t];
-                if (!ready) break;
-                double z = zb[chain * 4 + lane];
-                for (int t = ra; t < rb; ++t) z -= rem[t * 4 + lane];
-                for (int t = lrp[chain]; t < lrp[chain + 1]; ++t) {
-                    const double *Sv = locv + t * 16 + lane * 4;
-                    const double *xk = xs + lk[t] * 4;
-                    z -= Sv[0] * xk[0] + Sv[1] * xk[1] + Sv[2] * xk[2] + Sv[3] * xk[3];
+                const int la = lrp[chain], lb = lrp[chain + 1];
+                double z;
+                if (rb - ra <= 2 && lb - la <= 1) {
+                    const int r1 = min(ra + 1, nrem - 1), r0c = min(ra, nrem - 1);
+                    const int l0c = min(la, max(nloc - 1, 0));
+                    const int d0 = done[r0c], d1 = done[r1];
+                    const double v0 = rem[r0c * 4 + lane], v1 = rem[r1 * 4 + lane];
+                    const double zc = zb[chain * 4 + lane];
+                    const int kk = lk[l0c];
+                    const double s0 = locv[l0c * 16 + lane * 4 + 0], s1 = locv[l0c * 16 + lane * 4 + 1];
+                    const double s2 = locv[l0c * 16 + lane * 4 + 2], s3 = locv[l0c * 16 + lane * 4 + 3];
+                    const bool ready = (ra >= rb || d0) && (ra + 1 >= rb || d1);
+                    if (!ready) break;
+                    z = zc;
+                    if (ra < rb) z -= v0;
+                    if (ra + 1 < rb) z -= v1;
+                    if (la < lb) {
+                        double x0, x1, x2, x3;
+                        if (kk == xp_row) {
+                            x0 = xp0, x1 = xp1, x2 = xp2, x3 = xp3;
+                        } else {
+                            const double *xk = xs + kk * 4;
+                            x0 = xk[0], x1 = xk[1], x2 = xk[2], x3 = xk[3];
+                        }
+                        z -= s0 * x0 + s1 * x1 + s2 * x2 + s3 * x3;
+                    }
+                } else {
+                    bool ready = true;
+                    for (int t = ra; t < rb; ++t) ready = ready && done[t];
+                    if (!ready) break;
+                    z = zb[chain * 4 + lane];
+                    for (int t = ra; t < rb; ++t) z -= rem[t * 4 + lane];
+                    for (int t = la; t < lb; ++t) {
+                        const double *Sv = locv + t * 16 + lane * 4;
+                        const double *xk = xs + lk[t] * 4;
+                        z -= Sv[0] * xk[0] + Sv[1] * xk[1] + Sv[2] * xk[2] + Sv[3] * xk[3];
+                    }
                 }
                 xs[chain * 4 + lane] = z;
+                xp0 = quad_bcast<0>(z);
+                xp1 = quad_bcast<1>(z);
+                xp2 = quad_bcast<2>(z);
+                xp3 = quad_bcast<3>(z);
+                xp_row = chain;
                 const int i = row_of(qlo + chain);
                 publish(out + (size_t)i * 4 + lane, z);
                 if (sentinel_fill) sentinel_fill[(size_t)i * 4 + lane] = __longlong_as_double((long long)kSentinel);
@@ -210,6 +291,10 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
         }
         __syncthreads();
         const int sc = s_chain;
+        if (poll_min < 64) {
+            const int found = __popcll(__ballot(got));
+            window = min(64, max(poll_min, 2 * found + poll_min));
+        }
         const bool progressed = (sc & 0x40000000) || __any(got);
         chain = sc & 0x3fffffff;
         stall = progressed ? 0 : stall + 1;
@@ -337,6 +422,7 @@ using namespace isai;
 
 struct isai_sptrsv_plan {
     int nb = 0, nnzb = 0, bs = 0, chunk = 0;
+    int poll_min = 64;  // smallest poll window of the adaptive poll (64 = fixed 64-item window)
     int nchunks[2] = {0, 0};
     int maxr[2] = {0, 0}, maxrem[2] = {0, 0}, maxloc[2] = {0, 0};
     const int *d_row = nullptr, *d_col = nullptr;
@@ -357,10 +443,10 @@ int sptrsv_solve_internal(isai_sptrsv_plan_t P, const double *d_LU, const double
     fill_sentinel_kernel<<<fill_blocks(n), 256, 0, st>>>(P->d_y, n, skip);
     bsr4_sptrsv_pipe_kernel<false><<<P->nchunks[0], 64, pipe_lds_bytes(P->maxr[0], P->maxrem[0], P->maxloc[0]), st>>>(
         P->nb, P->d_chunks[0], P->d_col, P->d_S, P->d_dinvL, P->d_rem_ptr[0], P->d_rem_items[0], P->d_loc_ptr[0],
-        P->d_loc_items[0], d_b, P->d_y, d_x, P->d_cnt, skip, P->maxr[0], P->maxrem[0], P->maxloc[0]);
+        P->d_loc_items[0], d_b, P->d_y, d_x, P->d_cnt, skip, P->maxr[0], P->maxrem[0], P->maxloc[0], P->poll_min);
     bsr4_sptrsv_pipe_kernel<true><<<P->nchunks[1], 64, pipe_lds_bytes(P->maxr[1], P->maxrem[1], P->maxloc[1]), st>>>(
         P->nb, P->d_chunks[1], P->d_col, P->d_S, P->d_dinvU, P->d_rem_ptr[1], P->d_rem_items[1], P->d_loc_ptr[1],
-        P->d_loc_items[1], P->d_y, d_x, nullptr, P->d_cnt, skip, P->maxr[1], P->maxrem[1], P->maxloc[1]);
+        P->d_loc_items[1], P->d_y, d_x, nullptr, P->d_cnt, skip, P->maxr[1], P->maxrem[1], P->maxloc[1], P->poll_min);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
