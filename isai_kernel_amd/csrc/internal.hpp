@@ -30,12 +30,13 @@ int spmv_dot_groups(int mb);
 int spmv_dot_launch(int mb, const int *rp, const int *ci, const double *val, const void *x, bool x_f32, double *y,
                     const double *a, int ndot, double *part, hipStream_t st, const int *skip, long long nnzb);
 
-// Q_j = PairPtr[Mbrow[j+1]] - PairPtr[Mbrow[j]] for every column and, through
-// the caller's device scratch d_acc (pair_acc_bytes()), the largest Q_j in *max_q
-// (synchronizes the stream).
+// Q_j = PairPtr[Mbrow[j+1]] - PairPtr[Mbrow[j]] for every column into d_Q, and the
+// largest Q_j into the int at (char *)d_acc + pair_acc_max_q_offset(); d_acc is
+// pair_acc_bytes() bytes of device scratch the caller zeroes beforehand.
+// Asynchronous.
 size_t pair_acc_bytes();
-int pair_column_q_max(int nb, const int *d_Mbrow, const int *d_PairPtr, int *d_Q, void *d_acc, int *max_q,
-                      hipStream_t st);
+size_t pair_acc_max_q_offset();
+int pair_column_q_launch(int nb, const int *d_Mbrow, const int *d_PairPtr, int *d_Q, void *d_acc, hipStream_t st);
 
 // One ISAI factor for the column-wise build.  order (optional) lists the columns
 // in the order the workgroups take them; the values do not depend on it.

@@ -9,6 +9,7 @@
 // Q and qlen (LDS-privatised for values below 1024) are walked on the host, which
 // gives the same element as sorting and indexing (size - 1) * p.
 #include <algorithm>
+#include <cstddef>
 #include <vector>
 
 #include "common.hpp"
@@ -125,18 +126,13 @@ static int percentile(const std::vector<int> &cnt, long long n, double p) {
 }
 
 size_t pair_acc_bytes() { return sizeof(StatsAcc); }
+size_t pair_acc_max_q_offset() { return offsetof(StatsAcc, max_q); }
 
-int pair_column_q_max(int nb, const int *d_Mbrow, const int *d_PairPtr, int *d_Q, void *d_acc, int *max_q,
-                      hipStream_t st) {
-    *max_q = 0;
+int pair_column_q_launch(int nb, const int *d_Mbrow, const int *d_PairPtr, int *d_Q, void *d_acc, hipStream_t st) {
     if (nb == 0) return ISAI_OK;
-    StatsAcc *acc = (StatsAcc *)d_acc;
-    ISAI_HIP(hipMemsetAsync(acc, 0, sizeof(StatsAcc), st));
     pair_col_kernel<<<grid_for(nb, kStatsBlock), kStatsBlock, 0, st>>>(nb, d_Mbrow, d_PairPtr, nullptr, d_Q,
-                                                                      nullptr, acc);
+                                                                      nullptr, (StatsAcc *)d_acc);
     ISAI_LAUNCH_CHECK();
-    ISAI_HIP(hipMemcpyAsync(max_q, &acc->max_q, sizeof(int), hipMemcpyDeviceToHost, st));
-    ISAI_HIP(hipStreamSynchronize(st));
     return ISAI_OK;
 }
 

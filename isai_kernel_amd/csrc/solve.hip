@@ -1018,6 +1018,14 @@ int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, con
     h->d_Urow = factor_rows[1];
     h->d_Ucol = factor_cols[1];
     h->d_Uval = factor_vals[1];
+    // build-plan scratch: per-column pair counts and one statistics record per factor
+    const size_t acc_bytes = pair_acc_bytes();
+    if (hipMalloc(&d_colq, sizeof(int) * 2 * (size_t)std::max(nb, 1)) != hipSuccess ||
+        hipMalloc(&d_acc, 2 * acc_bytes) != hipSuccess) {
+        cleanup();
+        set_error("isai_setup: allocation failed");
+        return fail(ISAI_ERR_ALLOC);
+    }
     ISAI_HIP(hipStreamSynchronize(st));
     ISAI_HIP(hipEventRecord(e1b, st));
     // pair lists: the count pass returns the totals (one sync each, as main.cu:690)
@@ -1043,19 +1051,22 @@ int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, con
     // columns start first and share waves with columns of similar length.
     if (s == ISAI_OK && nb > 0) {
         constexpr double kOrderSkew = 8.0;
-        if (hipMalloc(&d_colq, sizeof(int) * nb) != hipSuccess || hipMalloc(&d_acc, pair_acc_bytes()) != hipSuccess) {
-            cleanup();
-            set_error("isai_setup: allocation failed");
-            return fail(ISAI_ERR_ALLOC);
-        }
+        int qmax2[2] = {0, 0};
+        ISAI_HIP(hipMemsetAsync(d_acc, 0, 2 * acc_bytes, st));
+        for (int f = 0; f < 2 && s == ISAI_OK; ++f)
+            s = pair_column_q_launch(nb, d_buf[f][0], d_buf[f][2], d_colq + (size_t)f * nb,
+                                     (char *)d_acc + f * acc_bytes, st);
+        for (int f = 0; f < 2 && s == ISAI_OK; ++f)
+            ISAI_HIP(hipMemcpyAsync(&qmax2[f], (char *)d_acc + f * acc_bytes + pair_acc_max_q_offset(), sizeof(int),
+                                    hipMemcpyDeviceToHost, st));
+        if (s == ISAI_OK) ISAI_HIP(hipStreamSynchronize(st));
         for (int f = 0; f < 2 && s == ISAI_OK; ++f) {
-            int qmax = 0;
-            s = pair_column_q_max(nb, d_buf[f][0], d_buf[f][2], d_colq, d_acc, &qmax, st);
+            const int qmax = qmax2[f];
             const double qavg = (double)totals[f] / nb;
-            if (s != ISAI_OK || qmax <= kOrderSkew * std::max(qavg, 1.0)) continue;
+            if (qmax <= kOrderSkew * std::max(qavg, 1.0)) continue;
             std::vector<int> q(nb), order(nb);
             std::vector<long long> start((size_t)qmax + 2, 0);
-            ISAI_HIP(hipMemcpyAsync(q.data(), d_colq, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+            ISAI_HIP(hipMemcpyAsync(q.data(), d_colq + (size_t)f * nb, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
             ISAI_HIP(hipStreamSynchronize(st));
             for (int j = 0; j < nb; ++j) start[qmax - q[j] + 1]++;  // bucket 0 = largest Q
             for (int v = 0; v <= qmax; ++v) start[v + 1] += start[v];
