@@ -35,14 +35,20 @@ int isai_csr_sptrsv_lu_device(int m, int nnz, const int *d_rowptr, const int *d_
                               const double *d_val, const double *d_b, double *d_x, void *d_work,
                               int chunk, int spin_limit, isai_stream_t stream);
 
-/* BSR-native form (bs = 4): one wave per chunk of consecutive block rows whose
- * boundaries follow the dependency chains; dependencies outside the chunk are
- * fetched ahead of the serial chain.  The analysis builds the chunk tables from
- * the pattern and pre-scales the off-diagonal blocks by the inverted diagonal
- * parts of the packed factor; isai_bsr_sptrsv_update_values redoes the scaling for
- * new values on the same pattern.  chunk = cap on rows per chunk (1..128, <= 0
- * picks 128).  A block row may depend on at most 64 blocks outside its chunk
- * (ISAI_ERR_UNSUPPORTED otherwise; the CSR form has no such limit). */
+/* BSR-native form (bs = 4), two schedules:
+ *   chunk schedule: one wave per chunk of consecutive block rows whose boundaries
+ *     follow the dependency chains; dependencies outside the chunk are fetched
+ *     ahead of the serial chain.  A block row may depend on at most 64 blocks
+ *     outside its chunk (ISAI_ERR_UNSUPPORTED otherwise; the CSR form has no such
+ *     limit);
+ *   row schedule: one quad of lanes per block row, rows in level order, every
+ *     dependency polled from the solution vector; no limit on the row length.
+ * chunk = 1..128: chunk schedule with at most that many rows per chunk; chunk < 0:
+ * row schedule; chunk = 0: the row schedule when the levels of the triangles
+ * average at least 256 block rows, else the chunk schedule with 128.  The analysis
+ * builds the tables from the pattern and pre-scales the off-diagonal blocks by the
+ * inverted diagonal parts of the packed factor; isai_bsr_sptrsv_update_values
+ * redoes the scaling for new values on the same pattern. */
 typedef struct isai_sptrsv_plan *isai_sptrsv_plan_t;
 int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const int *d_col,
                              const double *d_LU, int chunk, isai_sptrsv_plan_t *plan,

@@ -36,7 +36,7 @@ typedef enum { ISAI_SOLVER_BICGSTAB = 0, ISAI_SOLVER_FGMRES = 1 } isai_solver_ki
 typedef struct {
     int precond;      /* isai_precond_kind_t */
     int precision;    /* isai_precision_t of the ISAI factor values */
-    int sptrsv_chunk; /* rows per wave of the sync-free solve, 0 = default */
+    int sptrsv_chunk; /* schedule of the sync-free solve, as isai_bsr_sptrsv_analysis's chunk (0 = automatic) */
     int reserved[5];
 } isai_setup_opts_t;
 

@@ -5,11 +5,13 @@
 // flag of each dependency and relies on independent thread scheduling inside a
 // warp (cape_sptrsv.h:36-51).  A wave64 has no independent thread scheduling, so
 // a lane spinning on a value another lane of its own wave must produce would
-// never finish.  Here:
-//   * one wave owns a CHUNK of consecutive rows (ascending for L, descending for
-//     U) taken from an atomic ticket, and walks them in order: dependencies inside
-//     the chunk are read from LDS, never waited on;
-//   * dependencies on other chunks are waited on with per-row ready flags that
+// never finish.  Here, in two schedules of the BSR form:
+//   * chunk schedule: one wave owns a CHUNK of consecutive rows (ascending for L,
+//     descending for U) taken from an atomic ticket, and walks them in order:
+//     dependencies inside the chunk are read from LDS, never waited on;
+//   * row schedule: one quad of lanes per block row, rows in level order, and the
+//     quads of a wave advance independently of each other;
+//   * dependencies on other chunks (or rows) are waited on with per-row ready flags that
 //     live in the solution vector itself: it is pre-filled with a NaN sentinel
 //     and every value is published with an agent-scope atomic store, so a reader
 //     polling with agent-scope atomic loads sees either the sentinel or the final
@@ -320,6 +322,150 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
     }
 }
 
+// ------------------------------------------------ BSR, bs = 4, row schedule
+// One quad (four lanes, one per row of the 4x4 block) per block row; the rows are
+// taken in level order (a row's level is one more than the deepest level among
+// its dependencies), 64 rows per workgroup from an atomic ticket.  A row only
+// depends on rows of lower levels, hence lower positions, hence a lower or equal
+// ticket: their workgroups are resident.  Within a wave the quads advance
+// independently of each other: a quad whose inputs are all published computes
+// and publishes its row in the same pass in which the other quads keep polling,
+// so rows of a wave may depend on each other.
+//
+// Per pass a quad holds up to kRowBatch dependencies in registers (each lane
+// holds row m of every S block and polls component m of every x_k); with all of
+// them published the quad subtracts their products, in the row's column order,
+// and moves on to the next batch or publishes.
+constexpr int kRowWG = 256;  // threads per workgroup
+constexpr int kRowsPerWG = kRowWG / 4;
+constexpr int kRowBatch = 8;
+// Residency.  Every resident row polls, and the polls of all of them share the
+// fabric beyond L2: at four workgroups per CU (the register limit) the 96^3
+// 7-point factor's publish-to-observe hop took 2.4 us, at one 2.0 us, and the
+// solve ran 16 % faster.  Unused dynamic LDS of 96 KB per workgroup keeps one
+// workgroup (64 rows) per CU; its start-up stays off the critical path.
+constexpr int kRowLds = 96 * 1024;
+// The row schedule is chosen by default when the triangles' levels average at
+// least this many rows (four workgroups); narrower level structures (long chains,
+// small 2-D grids) keep the chunk schedule, whose chains resolve inside a wave.
+constexpr int kRowMinWidth = 256;
+
+// AND of a predicate over the four lanes of each quad (DPP quad_perm swaps).
+__device__ __forceinline__ bool quad_all(bool v) {
+    int a = v ? 1 : 0;
+    a &= __builtin_amdgcn_mov_dpp(a, 0xB1, 0xF, 0xF, false);  // [1,0,3,2]
+    a &= __builtin_amdgcn_mov_dpp(a, 0x4E, 0xF, 0xF, false);  // [2,3,0,1]
+    return a != 0;
+}
+
+template <bool UPPER>
+__global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
+    int nb, const int *__restrict__ order, const int *__restrict__ rp, const int *__restrict__ ci,
+    const int *__restrict__ diag, const double *__restrict__ S, const double *__restrict__ dinv, const double *rhs,
+    double *out, double *sentinel_fill, Counters *cnt, const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ int s_t;
+    unsigned int *ticket = UPPER ? &cnt->ticket_u : &cnt->ticket_l;
+    unsigned int *donec = UPPER ? &cnt->done_u : &cnt->done_l;
+    const int limit = cnt->spin_limit;
+    if (threadIdx.x == 0) s_t = (int)atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int q = s_t * kRowsPerWG + (int)(threadIdx.x >> 2);
+    const int m = threadIdx.x & 3;
+    const bool active = q < nb;
+    int i = 0, pb = 0, pe = 0;
+    double z = 0.0;
+    if (active) {
+        i = order[q];
+        const int d = diag[i];
+        pb = UPPER ? d + 1 : rp[i];
+        pe = UPPER ? rp[i + 1] : d;
+        const double *D = dinv + (size_t)i * 16 + m * 4;
+        const double *b = rhs + (size_t)i * 4;
+        z = D[0] * b[0] + D[1] * b[1] + D[2] * b[2] + D[3] * b[3];
+    }
+    // batch registers: column of each dependency, row m of its block, polled x_k[m]
+    int kk[kRowBatch];
+    double s0[kRowBatch], s1[kRowBatch], s2[kRowBatch], s3[kRowBatch];
+    unsigned long long u[kRowBatch];
+    unsigned miss = 0;
+    int p0 = pb;
+    auto load_batch = [&]() {
+#pragma unroll
+        for (int j = 0; j < kRowBatch; ++j)
+            if (p0 + j < pe) {
+                kk[j] = ci[p0 + j];
+                const double *B = S + (size_t)(p0 + j) * 16 + m * 4;
+                s0[j] = B[0], s1[j] = B[1], s2[j] = B[2], s3[j] = B[3];
+            }
+#pragma unroll
+        for (int j = 0; j < kRowBatch; ++j)
+            if (p0 + j < pe) {
+                u[j] = __hip_atomic_load((unsigned long long *)(out + (size_t)kk[j] * 4 + m), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+                if (u[j] == kSentinel) miss |= 1u << j;
+            }
+    };
+    if (p0 < pe) load_batch();
+    bool done_row = !active;
+    bool timed_out = false;
+    int stall = 0;
+    while (true) {
+        bool progressed = false;
+        const bool ready = quad_all(miss == 0);  // all lanes take part in the DPP moves
+        if (!done_row && ready) {
+            // every input of this batch is published: x_k arrives by quad broadcast
+#pragma unroll
+            for (int j = 0; j < kRowBatch; ++j)
+                if (p0 + j < pe) {
+                    const double xm = __longlong_as_double((long long)u[j]);
+                    const double x0 = quad_bcast<0>(xm), x1 = quad_bcast<1>(xm);
+                    const double x2 = quad_bcast<2>(xm), x3 = quad_bcast<3>(xm);
+                    z -= s0[j] * x0 + s1[j] * x1 + s2[j] * x2 + s3[j] * x3;
+                }
+            p0 += kRowBatch;
+            if (p0 < pe) {
+                load_batch();
+            } else {
+                publish(out + (size_t)i * 4 + m, z);
+                if (sentinel_fill) sentinel_fill[(size_t)i * 4 + m] = __longlong_as_double((long long)kSentinel);
+                done_row = true;
+            }
+            progressed = true;
+        }
+        if (__all(done_row)) break;
+        // one poll of every missing input of the current batch (polling only the
+        // input expected last, and the others once it is in, cost a round trip
+        // more per level than it saved in fabric load)
+        if (!done_row && miss) {
+#pragma unroll
+            for (int j = 0; j < kRowBatch; ++j)
+                if (miss & (1u << j)) {
+                    u[j] = __hip_atomic_load((unsigned long long *)(out + (size_t)kk[j] * 4 + m), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+                    if (u[j] != kSentinel) {
+                        miss &= ~(1u << j);
+                        progressed = true;
+                    }
+                }
+        }
+        stall = __any(progressed) ? 0 : stall + 1;
+        if (stall > limit) {
+            // give up on the missing inputs: they enter the row as NaN
+            if (miss) timed_out = true;
+            miss = 0;
+            stall = 0;
+        }
+    }
+    if (__any(timed_out) && threadIdx.x % 64 == 0) atomicOr(&cnt->err, 1);
+    if (threadIdx.x == 0) {
+        if (atomicAdd(donec, 1u) == (unsigned)gridDim.x - 1) {
+            atomicExch(ticket, 0u);
+            atomicExch(donec, 0u);
+        }
+    }
+}
+
 // S_p = inv(D_i) T_p for every off-diagonal block p of row i (T = L below the
 // diagonal, U above), one thread per block row.
 __global__ void scale_offdiag_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ diagpos,
@@ -422,6 +568,8 @@ using namespace isai;
 
 struct isai_sptrsv_plan {
     int nb = 0, nnzb = 0, bs = 0, chunk = 0;
+    bool rows = false;  // row schedule (level-ordered quads) instead of chunks
+    int *d_order[2] = {nullptr, nullptr};  // row schedule: block rows in level order, per triangle
     int poll_min = 64;  // smallest poll window of the adaptive poll (64 = fixed 64-item window)
     int nchunks[2] = {0, 0};
     int maxr[2] = {0, 0}, maxrem[2] = {0, 0}, maxloc[2] = {0, 0};
@@ -441,6 +589,15 @@ int sptrsv_solve_internal(isai_sptrsv_plan_t P, const double *d_LU, const double
     (void)d_LU;  // the solve reads the pre-scaled copy made from these values
     const size_t n = (size_t)P->nb * 4;
     fill_sentinel_kernel<<<fill_blocks(n), 256, 0, st>>>(P->d_y, n, skip);
+    if (P->rows) {
+        const unsigned g = (unsigned)((P->nb + kRowsPerWG - 1) / kRowsPerWG);
+        bsr4_sptrsv_row_kernel<false><<<g, kRowWG, kRowLds, st>>>(P->nb, P->d_order[0], P->d_row, P->d_col, P->d_diag,
+                                                            P->d_S, P->d_dinvL, d_b, P->d_y, d_x, P->d_cnt, skip);
+        bsr4_sptrsv_row_kernel<true><<<g, kRowWG, kRowLds, st>>>(P->nb, P->d_order[1], P->d_row, P->d_col, P->d_diag,
+                                                           P->d_S, P->d_dinvU, P->d_y, d_x, nullptr, P->d_cnt, skip);
+        ISAI_LAUNCH_CHECK();
+        return ISAI_OK;
+    }
     bsr4_sptrsv_pipe_kernel<false><<<P->nchunks[0], 64, pipe_lds_bytes(P->maxr[0], P->maxrem[0], P->maxloc[0]), st>>>(
         P->nb, P->d_chunks[0], P->d_col, P->d_S, P->d_dinvL, P->d_rem_ptr[0], P->d_rem_items[0], P->d_loc_ptr[0],
         P->d_loc_items[0], d_b, P->d_y, d_x, P->d_cnt, skip, P->maxr[0], P->maxrem[0], P->maxloc[0], P->poll_min);
@@ -583,6 +740,32 @@ static int build_tables(int nb, const std::vector<int> &rp, const std::vector<in
     return ISAI_OK;
 }
 
+// Level order of one triangle for the row schedule: level(i) = 1 + the deepest
+// level among the rows i depends on (0 without dependencies), rows in
+// processing-position order inside a level.  Returns the number of levels.
+static int level_order(int nb, const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &diag,
+                       bool upper, std::vector<int> &order) {
+    std::vector<int> lev(nb, 0);
+    int nlev = 0;
+    for (int q = 0; q < nb; ++q) {
+        const int i = upper ? nb - 1 - q : q;
+        const int pb = upper ? diag[i] + 1 : rp[i], pe = upper ? rp[i + 1] : diag[i];
+        int l = 0;
+        for (int p = pb; p < pe; ++p) l = std::max(l, lev[ci[p]] + 1);
+        lev[i] = l;
+        nlev = std::max(nlev, l + 1);
+    }
+    std::vector<int> cnt(nlev + 1, 0);
+    for (int i = 0; i < nb; ++i) cnt[lev[i] + 1]++;
+    for (int l = 0; l < nlev; ++l) cnt[l + 1] += cnt[l];
+    order.resize(nb);
+    for (int q = 0; q < nb; ++q) {
+        const int i = upper ? nb - 1 - q : q;
+        order[cnt[lev[i]]++] = i;
+    }
+    return nlev;
+}
+
 }  // namespace isai
 
 extern "C" {
@@ -596,7 +779,9 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         set_error("isai_bsr_sptrsv: the BSR-native solve is built for bs = 4 (got %d); use the CSR form", bs);
         return ISAI_ERR_UNSUPPORTED;
     }
-    if (chunk <= 0) chunk = kPipeRows;
+    bool rows = chunk < 0;
+    const bool automatic = chunk == 0;
+    if (rows || automatic) chunk = kPipeRows;
     ISAI_REQUIRE(chunk <= kPipeRows, "isai_bsr_sptrsv_analysis: chunk %d above %d", chunk, kPipeRows);
     hipStream_t st = (hipStream_t)stream;
     std::vector<int> rp(nb + 1), ci(nnzb), diag(nb, -1);
@@ -617,12 +802,24 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         }
     }
     std::vector<int4> chunks[2];
-    std::vector<int> rptr[2], ritems[2], lptr[2], litems[2];
-    for (int u = 0; u < 2; ++u) {
+    std::vector<int> rptr[2], ritems[2], lptr[2], litems[2], order[2];
+    if (rows || automatic) {
+        int nlev = 1;
+        for (int u = 0; u < 2; ++u) nlev = std::max(nlev, level_order(nb, rp, ci, diag, u == 1, order[u]));
+        if (automatic) rows = nb / nlev >= kRowMinWidth;
+    }
+    if (rows) {
+        ISAI_HIP(hipFuncSetAttribute((const void *)bsr4_sptrsv_row_kernel<false>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, kRowLds));
+        ISAI_HIP(hipFuncSetAttribute((const void *)bsr4_sptrsv_row_kernel<true>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, kRowLds));
+    }
+    for (int u = 0; u < 2 && !rows; ++u) {
         int s = build_tables(nb, rp, ci, diag, u == 1, chunk, chunks[u], rptr[u], ritems[u], lptr[u], litems[u]);
         if (s != ISAI_OK) return s;
     }
     auto *P = new isai_sptrsv_plan;
+    P->rows = rows;
     P->nb = nb;
     P->nnzb = nnzb;
     P->bs = bs;
@@ -644,7 +841,8 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
             P->maxloc[u] = std::max(P->maxloc[u], b.y - a.w);
         }
     }
-    for (int u = 0; u < 2 && ok; ++u) {
+    for (int u = 0; u < 2 && ok && rows; ++u) ok = hipMalloc(&P->d_order[u], sizeof(int) * nb) == hipSuccess;
+    for (int u = 0; u < 2 && ok && !rows; ++u) {
         ok = hipMalloc(&P->d_chunks[u], sizeof(int4) * chunks[u].size()) == hipSuccess &&
              hipMalloc(&P->d_rem_ptr[u], sizeof(int) * (nb + 1)) == hipSuccess &&
              hipMalloc(&P->d_loc_ptr[u], sizeof(int) * (nb + 1)) == hipSuccess &&
@@ -657,7 +855,9 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         return ISAI_ERR_ALLOC;
     }
     ISAI_HIP(hipMemcpy(P->d_diag, diag.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < 2 && rows; ++u)
+        ISAI_HIP(hipMemcpy(P->d_order[u], order[u].data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+    for (int u = 0; u < 2 && !rows; ++u) {
         ISAI_HIP(hipMemcpy(P->d_chunks[u], chunks[u].data(), sizeof(int4) * chunks[u].size(), hipMemcpyHostToDevice));
         ISAI_HIP(hipMemcpy(P->d_rem_ptr[u], rptr[u].data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
         ISAI_HIP(hipMemcpy(P->d_loc_ptr[u], lptr[u].data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
@@ -723,6 +923,7 @@ int isai_bsr_sptrsv_destroy(isai_sptrsv_plan_t P) {
     (void)hipFree(P->d_S);
     (void)hipFree(P->d_cnt);
     for (int u = 0; u < 2; ++u) {
+        (void)hipFree(P->d_order[u]);
         (void)hipFree(P->d_chunks[u]);
         (void)hipFree(P->d_rem_ptr[u]);
         (void)hipFree(P->d_rem_items[u]);

@@ -21,7 +21,7 @@ def _plan(rp, ci, d_rp, d_ci, d_lu, chunk):
 
 
 @pytest.mark.parametrize("problem,L", [(0, 16), (1, 8), (2, 5), (3, 12)])
-@pytest.mark.parametrize("chunk", [1, 5, 64, 128])
+@pytest.mark.parametrize("chunk", [1, 5, 64, 128, -1])
 def test_bsr_sptrsv_matches_oracle(gpu, problem, L, chunk):
     print(f"bsr sptrsv problem={problem} L={L} chunk={chunk}", flush=True)
     rp, ci, val = problems.generate(problem, L, seed=2)
@@ -80,14 +80,16 @@ def test_csr_sptrsv_device_form(gpu, chunk):
     assert rel_err(d_x.to_numpy(), ref) <= TOL
 
 
-def test_bsr_sptrsv_full_size_residual(gpu):
-    """96^3 7-point: || L (U x) - b || / || b || with the GPU factor and solve."""
-    print("bsr sptrsv full size 96^3", flush=True)
+@pytest.mark.parametrize("chunk", [0, -1])
+def test_bsr_sptrsv_full_size_residual(gpu, chunk):
+    """96^3 7-point: || L (U x) - b || / || b || with the GPU factor and solve, for
+    the chunk schedule (0) and the row schedule (-1)."""
+    print(f"bsr sptrsv full size 96^3 chunk={chunk}", flush=True)
     rp, ci, val = problems.generate(1, 96, seed=0)
     d_rp, d_ci, d_lu = gpu_ilu(rp, ci, val)
     n = (len(rp) - 1) * 4
     b = RNG.standard_normal(n)
-    plan = _plan(rp, ci, d_rp, d_ci, d_lu, 0)
+    plan = _plan(rp, ci, d_rp, d_ci, d_lu, chunk)
     try:
         d_b, d_x = dev(b), DeviceArray((n,), np.float64)
         call("isai_bsr_sptrsv_solve", plan, d_lu.ptr, d_b.ptr, d_x.ptr, None)
@@ -100,16 +102,17 @@ def test_bsr_sptrsv_full_size_residual(gpu):
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-13
 
 
-def test_spin_limit_timeout_is_reported_and_cleared(gpu):
-    """A wait budget of one poll makes dependent chunks give up (bounded waits: every
-    kernel still exits); the plan reports ISAI_ERR_TIMEOUT once, then solves
-    correctly again with the default budget."""
-    print("bsr sptrsv forced timeout (spin limit 1), problem 1 L=48", flush=True)
+@pytest.mark.parametrize("chunk", [0, -1])
+def test_spin_limit_timeout_is_reported_and_cleared(gpu, chunk):
+    """A wait budget of one poll makes dependent chunks (or rows) give up (bounded
+    waits: every kernel still exits); the plan reports ISAI_ERR_TIMEOUT once, then
+    solves correctly again with the default budget."""
+    print(f"bsr sptrsv forced timeout (spin limit 1), problem 1 L=48 chunk={chunk}", flush=True)
     rp, ci, val = problems.generate(1, 48, seed=6)
     d_rp, d_ci, d_lu = gpu_ilu(rp, ci, val)
     n = (len(rp) - 1) * 4
     b = RNG.standard_normal(n)
-    plan = _plan(rp, ci, d_rp, d_ci, d_lu, 0)
+    plan = _plan(rp, ci, d_rp, d_ci, d_lu, chunk)
     try:
         d_b, d_x = dev(b), DeviceArray((n,), np.float64)
         call("isai_bsr_sptrsv_set_spin_limit", plan, 1)
