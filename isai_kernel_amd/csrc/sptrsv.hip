@@ -346,9 +346,9 @@ constexpr int kRowBatch = 8;
 // workgroup (64 rows) per CU; its start-up stays off the critical path.
 constexpr int kRowLds = 96 * 1024;
 // The row schedule is chosen by default when the triangles' levels average at
-// least this many rows (four workgroups); narrower level structures (long chains,
-// small 2-D grids) keep the chunk schedule, whose chains resolve inside a wave.
-constexpr int kRowMinWidth = 256;
+// least this many rows; narrower level structures (long chains) keep the chunk
+// schedule, whose chains resolve inside a wave.
+constexpr int kRowMinWidth = 16;
 
 // AND of a predicate over the four lanes of each quad (DPP quad_perm swaps).
 __device__ __forceinline__ bool quad_all(bool v) {
@@ -358,11 +358,13 @@ __device__ __forceinline__ bool quad_all(bool v) {
     return a != 0;
 }
 
+// info[q] = (row, first dependency, end of dependencies, 1) for level-order
+// position q.
 template <bool UPPER>
 __global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
-    int nb, const int *__restrict__ order, const int *__restrict__ rp, const int *__restrict__ ci,
-    const int *__restrict__ diag, const double *__restrict__ S, const double *__restrict__ dinv, const double *rhs,
-    double *out, double *sentinel_fill, Counters *cnt, const int *__restrict__ skip) {
+    int nb, const int4 *__restrict__ info, const int *__restrict__ ci, const double *__restrict__ S,
+    const double *__restrict__ dinv, const double *rhs, double *out, double *sentinel_fill, Counters *cnt,
+    const int *__restrict__ skip) {
     if (skip && *skip) return;
     __shared__ int s_t;
     unsigned int *ticket = UPPER ? &cnt->ticket_u : &cnt->ticket_l;
@@ -376,10 +378,8 @@ __global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
     int i = 0, pb = 0, pe = 0;
     double z = 0.0;
     if (active) {
-        i = order[q];
-        const int d = diag[i];
-        pb = UPPER ? d + 1 : rp[i];
-        pe = UPPER ? rp[i + 1] : d;
+        const int4 f = info[q];
+        i = f.x, pb = f.y, pe = f.z;
         const double *D = dinv + (size_t)i * 16 + m * 4;
         const double *b = rhs + (size_t)i * 4;
         z = D[0] * b[0] + D[1] * b[1] + D[2] * b[2] + D[3] * b[3];
@@ -390,21 +390,33 @@ __global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
     unsigned long long u[kRowBatch];
     unsigned miss = 0;
     int p0 = pb;
-    auto load_batch = [&]() {
+    // One poll of every input in `want`: all loads are issued before any result is
+    // compared, so a round costs one round trip however many inputs are missing
+    // (a compare next to each load makes the compiler wait for every load in turn).
+    // Returns whether an input arrived.
+    auto poll_round = [&](unsigned want) {
 #pragma unroll
         for (int j = 0; j < kRowBatch; ++j)
-            if (p0 + j < pe) {
+            if (want & (1u << j))
+                u[j] = __hip_atomic_load((unsigned long long *)(out + (size_t)kk[j] * 4 + m), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned before = miss;
+#pragma unroll
+        for (int j = 0; j < kRowBatch; ++j)
+            if ((want & (1u << j)) && u[j] != kSentinel) miss &= ~(1u << j);
+        return miss != before;
+    };
+    auto load_batch = [&]() {
+        const int n = min(kRowBatch, pe - p0);
+#pragma unroll
+        for (int j = 0; j < kRowBatch; ++j)
+            if (j < n) {
                 kk[j] = ci[p0 + j];
                 const double *B = S + (size_t)(p0 + j) * 16 + m * 4;
                 s0[j] = B[0], s1[j] = B[1], s2[j] = B[2], s3[j] = B[3];
             }
-#pragma unroll
-        for (int j = 0; j < kRowBatch; ++j)
-            if (p0 + j < pe) {
-                u[j] = __hip_atomic_load((unsigned long long *)(out + (size_t)kk[j] * 4 + m), __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-                if (u[j] == kSentinel) miss |= 1u << j;
-            }
+        miss = (1u << n) - 1u;
+        poll_round(miss);
     };
     if (p0 < pe) load_batch();
     bool done_row = !active;
@@ -437,18 +449,7 @@ __global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
         // one poll of every missing input of the current batch (polling only the
         // input expected last, and the others once it is in, cost a round trip
         // more per level than it saved in fabric load)
-        if (!done_row && miss) {
-#pragma unroll
-            for (int j = 0; j < kRowBatch; ++j)
-                if (miss & (1u << j)) {
-                    u[j] = __hip_atomic_load((unsigned long long *)(out + (size_t)kk[j] * 4 + m), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-                    if (u[j] != kSentinel) {
-                        miss &= ~(1u << j);
-                        progressed = true;
-                    }
-                }
-        }
+        if (!done_row && miss && poll_round(miss)) progressed = true;
         stall = __any(progressed) ? 0 : stall + 1;
         if (stall > limit) {
             // give up on the missing inputs: they enter the row as NaN
@@ -569,7 +570,7 @@ using namespace isai;
 struct isai_sptrsv_plan {
     int nb = 0, nnzb = 0, bs = 0, chunk = 0;
     bool rows = false;  // row schedule (level-ordered quads) instead of chunks
-    int *d_order[2] = {nullptr, nullptr};  // row schedule: block rows in level order, per triangle
+    int4 *d_info[2] = {nullptr, nullptr};  // row schedule: (row, first, end, 1) per level-order position
     int poll_min = 64;  // smallest poll window of the adaptive poll (64 = fixed 64-item window)
     int nchunks[2] = {0, 0};
     int maxr[2] = {0, 0}, maxrem[2] = {0, 0}, maxloc[2] = {0, 0};
@@ -591,10 +592,10 @@ int sptrsv_solve_internal(isai_sptrsv_plan_t P, const double *d_LU, const double
     fill_sentinel_kernel<<<fill_blocks(n), 256, 0, st>>>(P->d_y, n, skip);
     if (P->rows) {
         const unsigned g = (unsigned)((P->nb + kRowsPerWG - 1) / kRowsPerWG);
-        bsr4_sptrsv_row_kernel<false><<<g, kRowWG, kRowLds, st>>>(P->nb, P->d_order[0], P->d_row, P->d_col, P->d_diag,
-                                                            P->d_S, P->d_dinvL, d_b, P->d_y, d_x, P->d_cnt, skip);
-        bsr4_sptrsv_row_kernel<true><<<g, kRowWG, kRowLds, st>>>(P->nb, P->d_order[1], P->d_row, P->d_col, P->d_diag,
-                                                           P->d_S, P->d_dinvU, P->d_y, d_x, nullptr, P->d_cnt, skip);
+        bsr4_sptrsv_row_kernel<false><<<g, kRowWG, kRowLds, st>>>(P->nb, P->d_info[0], P->d_col, P->d_S,
+                                                                  P->d_dinvL, d_b, P->d_y, d_x, P->d_cnt, skip);
+        bsr4_sptrsv_row_kernel<true><<<g, kRowWG, kRowLds, st>>>(P->nb, P->d_info[1], P->d_col, P->d_S,
+                                                                 P->d_dinvU, P->d_y, d_x, nullptr, P->d_cnt, skip);
         ISAI_LAUNCH_CHECK();
         return ISAI_OK;
     }
@@ -841,7 +842,8 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
             P->maxloc[u] = std::max(P->maxloc[u], b.y - a.w);
         }
     }
-    for (int u = 0; u < 2 && ok && rows; ++u) ok = hipMalloc(&P->d_order[u], sizeof(int) * nb) == hipSuccess;
+    std::vector<int4> info;
+    for (int u = 0; u < 2 && ok && rows; ++u) ok = hipMalloc(&P->d_info[u], sizeof(int4) * nb) == hipSuccess;
     for (int u = 0; u < 2 && ok && !rows; ++u) {
         ok = hipMalloc(&P->d_chunks[u], sizeof(int4) * chunks[u].size()) == hipSuccess &&
              hipMalloc(&P->d_rem_ptr[u], sizeof(int) * (nb + 1)) == hipSuccess &&
@@ -855,8 +857,14 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         return ISAI_ERR_ALLOC;
     }
     ISAI_HIP(hipMemcpy(P->d_diag, diag.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
-    for (int u = 0; u < 2 && rows; ++u)
-        ISAI_HIP(hipMemcpy(P->d_order[u], order[u].data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+    for (int u = 0; u < 2 && rows; ++u) {
+        info.assign(nb, make_int4(0, 0, 0, 0));
+        for (int q = 0; q < nb; ++q) {
+            const int i = order[u][q];
+            info[q] = u == 0 ? make_int4(i, rp[i], diag[i], 1) : make_int4(i, diag[i] + 1, rp[i + 1], 1);
+        }
+        ISAI_HIP(hipMemcpy(P->d_info[u], info.data(), sizeof(int4) * info.size(), hipMemcpyHostToDevice));
+    }
     for (int u = 0; u < 2 && !rows; ++u) {
         ISAI_HIP(hipMemcpy(P->d_chunks[u], chunks[u].data(), sizeof(int4) * chunks[u].size(), hipMemcpyHostToDevice));
         ISAI_HIP(hipMemcpy(P->d_rem_ptr[u], rptr[u].data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
@@ -923,7 +931,7 @@ int isai_bsr_sptrsv_destroy(isai_sptrsv_plan_t P) {
     (void)hipFree(P->d_S);
     (void)hipFree(P->d_cnt);
     for (int u = 0; u < 2; ++u) {
-        (void)hipFree(P->d_order[u]);
+        (void)hipFree(P->d_info[u]);
         (void)hipFree(P->d_chunks[u]);
         (void)hipFree(P->d_rem_ptr[u]);
         (void)hipFree(P->d_rem_items[u]);
