@@ -45,7 +45,7 @@ int isai_csr_sptrsv_lu_device(int m, int nnz, const int *d_rowptr, const int *d_
  *     dependency polled from the solution vector; no limit on the row length.
  * chunk = 1..128: chunk schedule with at most that many rows per chunk; chunk < 0:
  * row schedule; chunk = 0: the row schedule when the levels of the triangles
- * average at least 256 block rows, else the chunk schedule with 128.  The analysis
+ * average at least 16 block rows, else the chunk schedule with 128.  The analysis
  * builds the tables from the pattern and pre-scales the off-diagonal blocks by the
  * inverted diagonal parts of the packed factor; isai_bsr_sptrsv_update_values
  * redoes the scaling for new values on the same pattern. */
