@@ -209,9 +209,9 @@ __device__ __forceinline__ void store_y(void *__restrict__ y, size_t i, float v)
 // of them before accumulating: a reduced-precision row carries too few bytes to keep
 // enough loads in flight on its own.  Per row the lanes accumulate in the same
 // order as with one row per group, so results do not depend on kLowpRows.
-// Measured on the 96^3 7-pt ISAI factors (4 blocks per row): BF16 73.6 us with 2 or
-// 4 rows (91 us with 1, 95 us with 8); FP32 134-136 us with 1 or 2 rows, 151 us with
-// 4 (its 64-B blocks share lines).  On the 64^3 27-pt factors (14 blocks per row) one
+// Measured on the 96^3 7-pt ISAI factors (4 blocks per row), M_L and M_U together:
+// BF16 60.5 us with 4 rows (63.5 with 2, 66.9 with 8); FP32 120.7 us with 2 rows
+// (119.8 with 4, 131.0 with 1).  On the 64^3 27-pt factors (14 blocks per row) one
 // row already keeps enough loads in flight and several rows per group are slower, so
 // the launcher uses several rows only for rows of at most two lines of values.
 constexpr int kLowpRowsF32 = 2, kLowpRowsBF16 = 4;
@@ -243,19 +243,23 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
         v4u w[kLowpRows];
         float f[kLowpRows][4];
         bool live[kLowpRows];
+        int c[kLowpRows];
+        // The loads are unconditional (a finished row reads block 0 and discards
+        // it): loads under a per-row branch made the compiler wait for each row's
+        // column index and x in turn, one row after another.
 #pragma unroll
         for (int r = 0; r < kLowpRows; ++r) {
             const int kb = rs[r] + t + b;
             live[r] = kb < re[r];
-            if (live[r]) {
-                const int c = ci[kb];
-                // 16 B per lane: BF16 rows fill whole lines (streamed); a 64-B FP32
-                // block shares its line with a neighbour another wave may read, so
-                // FP32 keeps the line in L2
-                w[r] = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kb * LPB + sub]) : val16[(size_t)kb * LPB + sub];
-                load_x4<XF32>(x, c, f[r]);
-            }
+            const int kc = live[r] ? kb : 0;
+            c[r] = ci[kc];
+            // 16 B per lane: BF16 rows fill whole lines (streamed); a 64-B FP32
+            // block shares its line with a neighbour another wave may read, so
+            // FP32 keeps the line in L2
+            w[r] = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kc * LPB + sub]) : val16[(size_t)kc * LPB + sub];
         }
+#pragma unroll
+        for (int r = 0; r < kLowpRows; ++r) load_x4<XF32>(x, c[r], f[r]);
 #pragma unroll
         for (int r = 0; r < kLowpRows; ++r) {
             if (!live[r]) continue;
@@ -319,20 +323,21 @@ __global__ __launch_bounds__(kSpmvF64Block) void bsr4_spmv_bf16_batch_kernel(
     for (int t = 0; t < len; t += KB) {
         const int clo = s < KB && t + s < len ? ci[rs + t + s] : 0;
         const int chi = KB > 8 && t + 8 + s < len ? ci[rs + t + 8 + s] : 0;
+        // loads past the row's end read its first block / column 0 and are
+        // discarded: unconditional loads stay in flight together
         v4u w[NL];
         float f[NL][4];
 #pragma unroll
         for (int l = 0; l < NL; ++l) {
             const int kk = t + l * 4 + b;
-            if (kk < len) w[l] = __builtin_nontemporal_load(&val16[(size_t)(rs + kk) * 2 + sub]);
+            w[l] = __builtin_nontemporal_load(&val16[(size_t)(rs + (kk < len ? kk : 0)) * 2 + sub]);
         }
 #pragma unroll
         for (int l = 0; l < NL; ++l) {
             const int u = l * 4 + b;
             const int clo_u = __shfl(clo, u & 7, 8);
             const int chi_u = KB > 8 ? __shfl(chi, u & 7, 8) : 0;
-            const int c = u < 8 ? clo_u : chi_u;
-            if (t + u < len) load_x4<XF32>(x, c, f[l]);
+            load_x4<XF32>(x, u < 8 ? clo_u : chi_u, f[l]);
         }
 #pragma unroll
         for (int l = 0; l < NL; ++l) {
