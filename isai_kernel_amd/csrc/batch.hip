@@ -621,19 +621,27 @@ __global__ __launch_bounds__(64) void brho_final(BatchDev B) {
 }
 
 // p = r + beta (p - omega v)  (p = r on a pair's first iteration)
-__global__ __launch_bounds__(kRed) void bp_kernel(const double *__restrict__ r, double *p, const double *__restrict__ v,
-                                                  BatchDev B) {
+__global__ __launch_bounds__(kRed) void bp_kernel(const double *__restrict__ r, double *__restrict__ p,
+                                                  const double *__restrict__ v, BatchDev B) {
     const int t = blockIdx.x;
     const PairState &S = B.ps[B.tile_pair[t]];
     if (S.status != 0) return;
     const size_t base = (size_t)t * kTile;
     const double beta = S.beta, omega = S.omega;
     const bool first = S.first;
+    // every load of the tile before any store (the stores to p otherwise kept the
+    // next element's loads waiting)
+    double rr[kTile / kRed], pp[kTile / kRed], vv[kTile / kRed];
 #pragma unroll
     for (int k = 0; k < kTile / kRed; ++k) {
         const size_t i = base + k * kRed + threadIdx.x;
-        p[i] = first ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
+        rr[k] = r[i];
+        pp[k] = first ? 0.0 : p[i];
+        vv[k] = first ? 0.0 : v[i];
     }
+#pragma unroll
+    for (int k = 0; k < kTile / kRed; ++k)
+        p[base + k * kRed + threadIdx.x] = first ? rr[k] : rr[k] + beta * (pp[k] - omega * vv[k]);
 }
 
 __global__ __launch_bounds__(64) void balpha_final(BatchDev B) {

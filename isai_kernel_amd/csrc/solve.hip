@@ -426,12 +426,17 @@ __global__ __launch_bounds__(kMdotThreads) void fg_mdot_kernel(size_t n, const d
     double acc[kDotChunk], ww = 0.0;
 #pragma unroll
     for (int u = 0; u < kDotChunk; ++u) acc[u] = 0.0;
+    // The loads of a column are unconditional (a block column past the last vector
+    // re-reads the last one and discards it): loads under a per-vector branch were
+    // waited for one at a time.  Each vector's partial sum keeps its order.
     for (size_t k = (size_t)blockIdx.x * kMdotThreads + threadIdx.x; k < n; k += (size_t)kRedBlocks * kMdotThreads) {
         const double wk = w[k];
+        double vv[kDotChunk];
+#pragma unroll
+        for (int u = 0; u < kDotChunk; ++u) vv[u] = V[(size_t)min(i0 + u, nv - 1) * n + k];
         if (self) ww += wk * wk;
 #pragma unroll
-        for (int u = 0; u < kDotChunk; ++u)
-            if (i0 + u < nv) acc[u] += V[(size_t)(i0 + u) * n + k] * wk;
+        for (int u = 0; u < kDotChunk; ++u) acc[u] = i0 + u < nv ? acc[u] + vv[u] * wk : acc[u];
     }
 #pragma unroll
     for (int u = 0; u < kDotChunk; ++u) {
