@@ -196,6 +196,35 @@ __device__ __forceinline__ void load_x4(const void *__restrict__ x, int c, float
     }
 }
 
+// The same four floats for the LPB lanes that share a block (lanes of one quad, or
+// one pair of it): each lane loads only its own part of the block column and the
+// lanes exchange the rest with DPP moves, instead of every lane loading all of it.
+template <int K>
+__device__ __forceinline__ float quad_bcast_f(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K | (K << 2) | (K << 4) | (K << 6), 0xF, 0xF, false));
+}
+__device__ __forceinline__ float pair_swap_f(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+}
+template <int LPB, bool XF32>
+__device__ __forceinline__ void load_x4_split(const void *__restrict__ x, int c, int sub, float f[4]) {
+    if (LPB == 4) {  // lane sub loads component sub
+        const float v = XF32 ? ((const float *)x)[(size_t)c * 4 + sub] : (float)((const double *)x)[(size_t)c * 4 + sub];
+        f[0] = quad_bcast_f<0>(v), f[1] = quad_bcast_f<1>(v), f[2] = quad_bcast_f<2>(v), f[3] = quad_bcast_f<3>(v);
+    } else {  // LPB == 2: lane sub loads components 2 sub, 2 sub + 1
+        float a, b;
+        if (XF32) {
+            const float2 t = ((const float2 *)x)[(size_t)c * 2 + sub];
+            a = t.x, b = t.y;
+        } else {
+            const v2d t = ((const v2d *)x)[(size_t)c * 2 + sub];
+            a = (float)t.x, b = (float)t.y;
+        }
+        const float pa = pair_swap_f(a), pb = pair_swap_f(b);
+        f[0] = sub ? pa : a, f[1] = sub ? pb : b, f[2] = sub ? a : pa, f[3] = sub ? b : pb;
+    }
+}
+
 template <bool YF32>
 __device__ __forceinline__ void store_y(void *__restrict__ y, size_t i, float v) {
     if (YF32)
@@ -259,7 +288,7 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
             w[r] = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kc * LPB + sub]) : val16[(size_t)kc * LPB + sub];
         }
 #pragma unroll
-        for (int r = 0; r < kLowpRows; ++r) load_x4<XF32>(x, c[r], f[r]);
+        for (int r = 0; r < kLowpRows; ++r) load_x4_split<LPB, XF32>(x, c[r], sub, f[r]);
 #pragma unroll
         for (int r = 0; r < kLowpRows; ++r) {
             if (!live[r]) continue;
