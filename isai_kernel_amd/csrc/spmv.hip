@@ -273,22 +273,30 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
         float f[kLowpRows][4];
         bool live[kLowpRows];
         int c[kLowpRows];
-        // The loads are unconditional (a finished row reads block 0 and discards
-        // it): loads under a per-row branch made the compiler wait for each row's
-        // column index and x in turn, one row after another.
+        // With several rows per group the loads are unconditional (a finished row
+        // reads block 0 and discards it): loads under a per-row branch made the
+        // compiler wait for each row's column index and x in turn, one row after
+        // another.  With one row there is nothing to overlap, and the branch saves
+        // the loads past the row's end.  The lanes of one block share `live`, so
+        // the DPP exchange of x stays inside active lanes.
 #pragma unroll
         for (int r = 0; r < kLowpRows; ++r) {
             const int kb = rs[r] + t + b;
             live[r] = kb < re[r];
-            const int kc = live[r] ? kb : 0;
-            c[r] = ci[kc];
-            // 16 B per lane: BF16 rows fill whole lines (streamed); a 64-B FP32
-            // block shares its line with a neighbour another wave may read, so
-            // FP32 keeps the line in L2
-            w[r] = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kc * LPB + sub]) : val16[(size_t)kc * LPB + sub];
         }
+        if (kLowpRows > 1 || live[0]) {
 #pragma unroll
-        for (int r = 0; r < kLowpRows; ++r) load_x4_split<LPB, XF32>(x, c[r], sub, f[r]);
+            for (int r = 0; r < kLowpRows; ++r) {
+                const int kc = live[r] ? rs[r] + t + b : 0;
+                c[r] = ci[kc];
+                // 16 B per lane: BF16 rows fill whole lines (streamed); a 64-B FP32
+                // block shares its line with a neighbour another wave may read, so
+                // FP32 keeps the line in L2
+                w[r] = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kc * LPB + sub]) : val16[(size_t)kc * LPB + sub];
+            }
+#pragma unroll
+            for (int r = 0; r < kLowpRows; ++r) load_x4_split<LPB, XF32>(x, c[r], sub, f[r]);
+        }
 #pragma unroll
         for (int r = 0; r < kLowpRows; ++r) {
             if (!live[r]) continue;
@@ -352,21 +360,20 @@ __global__ __launch_bounds__(kSpmvF64Block) void bsr4_spmv_bf16_batch_kernel(
     for (int t = 0; t < len; t += KB) {
         const int clo = s < KB && t + s < len ? ci[rs + t + s] : 0;
         const int chi = KB > 8 && t + 8 + s < len ? ci[rs + t + 8 + s] : 0;
-        // loads past the row's end read its first block / column 0 and are
-        // discarded: unconditional loads stay in flight together
         v4u w[NL];
         float f[NL][4];
 #pragma unroll
         for (int l = 0; l < NL; ++l) {
             const int kk = t + l * 4 + b;
-            w[l] = __builtin_nontemporal_load(&val16[(size_t)(rs + (kk < len ? kk : 0)) * 2 + sub]);
+            if (kk < len) w[l] = __builtin_nontemporal_load(&val16[(size_t)(rs + kk) * 2 + sub]);
         }
 #pragma unroll
         for (int l = 0; l < NL; ++l) {
             const int u = l * 4 + b;
             const int clo_u = __shfl(clo, u & 7, 8);
             const int chi_u = KB > 8 ? __shfl(chi, u & 7, 8) : 0;
-            load_x4<XF32>(x, u < 8 ? clo_u : chi_u, f[l]);
+            const int c = u < 8 ? clo_u : chi_u;
+            if (t + u < len) load_x4<XF32>(x, c, f[l]);
         }
 #pragma unroll
         for (int l = 0; l < NL; ++l) {
