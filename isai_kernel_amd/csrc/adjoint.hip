@@ -95,6 +95,7 @@ int transpose_packed_lu(int nnzb, const int *d_trow_of, const int *d_tcol, const
 // g_k = alpha * lam_I x_J^T: eight lanes per block row, lane l writes row l/2,
 // columns 2(l%2)..+1 of every block of the row (a full 128-byte line per block).
 constexpr int kGradBlock = 256;
+constexpr int kGradBatch = 8;
 
 __global__ __launch_bounds__(kGradBlock) void outer_grad_kernel(int nb, const int *__restrict__ rp,
                                                                 const int *__restrict__ ci,
@@ -107,12 +108,25 @@ __global__ __launch_bounds__(kGradBlock) void outer_grad_kernel(int nb, const in
     const int l = threadIdx.x & 7, r = l >> 1, h = l & 1;
     const double a = alpha * lam[(size_t)I * 4 + r];
     const v2d *x2 = (const v2d *)x;
-    for (int k = rp[I]; k < rp[I + 1]; ++k) {
-        const v2d xv = x2[(size_t)ci[k] * 2 + h];
-        v2d o;
-        o.x = a * xv.x;
-        o.y = a * xv.y;
-        __builtin_nontemporal_store(o, &g[(size_t)k * 8 + l]);
+    const int ke = rp[I + 1];
+    // eight blocks per pass: their column indices, then their x, all in flight
+    // together (one block at a time the gathers waited on each index in turn);
+    // positions past the row's end re-read its last block and store nothing
+    for (int k0 = rp[I]; k0 < ke; k0 += kGradBatch) {
+        int c[kGradBatch];
+        v2d xv[kGradBatch];
+#pragma unroll
+        for (int j = 0; j < kGradBatch; ++j) c[j] = ci[min(k0 + j, ke - 1)];
+#pragma unroll
+        for (int j = 0; j < kGradBatch; ++j) xv[j] = x2[(size_t)c[j] * 2 + h];
+#pragma unroll
+        for (int j = 0; j < kGradBatch; ++j)
+            if (k0 + j < ke) {
+                v2d o;
+                o.x = a * xv[j].x;
+                o.y = a * xv[j].y;
+                __builtin_nontemporal_store(o, &g[(size_t)(k0 + j) * 8 + l]);
+            }
     }
 }
 

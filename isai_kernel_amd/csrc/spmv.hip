@@ -253,7 +253,10 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
     constexpr int BPB = BF16 ? 4 : 2;  // blocks per 128-byte line
     constexpr int LPB = 8 / BPB;       // lanes per block
     constexpr int G = kSpmvBlock / 8;  // lane groups per workgroup
-    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    // BF16 workgroups are spread over the XCDs in contiguous runs of rows (58.9 us
+    // against 60.3 us in launch order); FP32 runs in launch order (108.9 us against
+    // 116.5 us remapped), as the FP64 product does.
+    const int wg = BF16 ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int row0 = wg * G * kLowpRows + (threadIdx.x >> 3);
     const int s = threadIdx.x & 7;
     const int b = s / LPB, sub = s % LPB;
