@@ -468,22 +468,26 @@ __global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
 }
 
 // S_p = inv(D_i) T_p for every off-diagonal block p of row i (T = L below the
-// diagonal, U above), one thread per block row.
-__global__ void scale_offdiag_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ diagpos,
-                                     const double *__restrict__ LU, const double *__restrict__ dinvL,
-                                     const double *__restrict__ dinvU, double *__restrict__ S) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// diagonal, U above).  One wave per block row, one lane per element of the row's
+// blocks, so the block reads and writes are whole 128-byte lines (one thread per
+// row, walking its blocks, took 873 us on the 96^3 7-pt factor).
+constexpr int kScaleBlock = 256;
+__global__ __launch_bounds__(kScaleBlock) void scale_offdiag_kernel(int nb, const int *__restrict__ rp,
+                                                                    const int *__restrict__ diagpos,
+                                                                    const double *__restrict__ LU,
+                                                                    const double *__restrict__ dinvL,
+                                                                    const double *__restrict__ dinvU,
+                                                                    double *__restrict__ S) {
+    const int i = blockIdx.x * (kScaleBlock / 64) + (int)(threadIdx.x >> 6);
     if (i >= nb) return;
-    const int dp = diagpos[i];
-    for (int p = rp[i]; p < rp[i + 1]; ++p) {
+    const int lane = threadIdx.x & 63;
+    const int dp = diagpos[i], p0 = rp[i], len = rp[i + 1] - p0;
+    for (int t = lane; t < len * 16; t += 64) {
+        const int p = p0 + (t >> 4), e = t & 15, r = e >> 2, c = e & 3;
         if (p == dp) continue;
-        const double *D = (p < dp ? dinvL : dinvU) + (size_t)i * 16;
-        const double *T = LU + (size_t)p * 16;
-        double *o = S + (size_t)p * 16;
-        for (int r = 0; r < 4; ++r)
-            for (int c = 0; c < 4; ++c)
-                o[r * 4 + c] = D[r * 4 + 0] * T[0 * 4 + c] + D[r * 4 + 1] * T[1 * 4 + c] + D[r * 4 + 2] * T[2 * 4 + c] +
-                               D[r * 4 + 3] * T[3 * 4 + c];
+        const double *D = (p < dp ? dinvL : dinvU) + (size_t)i * 16 + r * 4;
+        const double *T = LU + (size_t)p * 16 + c;
+        S[(size_t)p * 16 + e] = D[0] * T[0 * 4] + D[1] * T[1 * 4] + D[2] * T[2 * 4] + D[3] * T[3 * 4];
     }
 }
 
@@ -889,8 +893,8 @@ int isai_bsr_sptrsv_update_values(isai_sptrsv_plan_t P, const double *d_LU, isai
     int s = isai_build_dinv(P->nb, 4, P->d_row, P->d_col, d_LU, 1, P->d_dinvL, stream);
     if (s == ISAI_OK) s = isai_build_dinv(P->nb, 4, P->d_row, P->d_col, d_LU, 0, P->d_dinvU, stream);
     if (s != ISAI_OK) return s;
-    scale_offdiag_kernel<<<grid_for(P->nb, 128), 128, 0, (hipStream_t)stream>>>(P->nb, P->d_row, P->d_diag, d_LU,
-                                                                               P->d_dinvL, P->d_dinvU, P->d_S);
+    scale_offdiag_kernel<<<grid_for(P->nb, kScaleBlock / 64), kScaleBlock, 0, (hipStream_t)stream>>>(
+        P->nb, P->d_row, P->d_diag, d_LU, P->d_dinvL, P->d_dinvU, P->d_S);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
