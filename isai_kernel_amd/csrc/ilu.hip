@@ -29,6 +29,7 @@
 
 #include "bsrilu.h"
 #include "common.hpp"
+#include "internal.hpp"
 #include "levels.hpp"
 
 struct isai_ilu_plan {
@@ -388,8 +389,9 @@ int isai_bsrilu0_factor(isai_ilu_plan_t P, double *d_val, int *zero_pivot, isai_
     int h[4];
     ISAI_HIP(hipMemcpyAsync(h, words, sizeof(h), hipMemcpyDeviceToHost, st));
     ISAI_HIP(hipStreamSynchronize(st));
-    if (!h[2]) ISAI_HIP(hipMemcpyAsync(d_val, fac, sizeof(double) * nval, hipMemcpyDeviceToDevice, st));
+    const int cs = h[2] ? ISAI_OK : copy_f64_d2d(d_val, fac, nval, st);
     ISAI_HIP(hipFreeAsync(scratch, st));
+    if (cs != ISAI_OK) return cs;
     if (h[2]) {
         set_error("bsrilu0: a dependency wait exceeded its spin limit");
         return ISAI_ERR_TIMEOUT;

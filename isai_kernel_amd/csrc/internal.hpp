@@ -36,6 +36,11 @@ int spmv_dot_launch(int mb, const int *rp, const int *ci, const double *val, con
 // Asynchronous.
 size_t pair_acc_bytes();
 size_t pair_acc_max_q_offset();
+// Device-to-device copy of `count` doubles on stream `st` (streaming copy kernel).
+int copy_f64_d2d(double *dst, const double *src, size_t count, hipStream_t st);
+// isai_pairs_count without the read-back of the total (left at d_PairPtr[Mnnz]).
+int pairs_count_async(int nb, int Mnnz, const int *d_Arow, const int *d_Acol, const int *d_Mbrow,
+                      const int *d_Mcol, int upper, int *d_PairPtr, hipStream_t st);
 int pair_column_q_launch(int nb, const int *d_Mbrow, const int *d_PairPtr, int *d_Q, void *d_acc, hipStream_t st);
 
 // One ISAI factor for the column-wise build.  order (optional) lists the columns

@@ -287,6 +287,21 @@ int build_dispatch(int nb, int bs, const double *Aval, const FactorArgs *L, cons
 
 using namespace isai;
 
+namespace isai {
+// The count pass of isai_pairs_count without its read-back: PairPtr holds the
+// exclusive scan of the per-entry pair counts, the total at PairPtr[Mnnz].
+int pairs_count_async(int nb, int Mnnz, const int *d_Arow, const int *d_Acol, const int *d_Mbrow,
+                      const int *d_Mcol, int upper, int *d_PairPtr, hipStream_t st) {
+    ISAI_HIP(hipMemsetAsync(d_PairPtr, 0, sizeof(int) * ((size_t)Mnnz + 1), st));
+    if (nb > 0) {
+        pairs_kernel<false><<<grid_for(nb, 128), 128, 0, st>>>(nb, d_Arow, d_Acol, d_Mbrow, d_Mcol, upper,
+                                                              d_PairPtr, nullptr, nullptr);
+        ISAI_LAUNCH_CHECK();
+    }
+    return exclusive_scan_i32(d_PairPtr, d_PairPtr, (long long)Mnnz + 1, st);
+}
+}  // namespace isai
+
 extern "C" {
 
 int isai_build_dinv(int nb, int bs, const int *d_Arow, const int *d_Acol, const double *d_Aval, int lower,
@@ -318,13 +333,7 @@ int isai_pairs_count(int nb, int Mnnz, const int *d_Arow, const int *d_Acol, con
     ISAI_REQUIRE(nb >= 0 && Mnnz >= 0 && total_pairs, "isai_pairs_count: bad argument");
     ISAI_REQUIRE(d_PairPtr, "isai_pairs_count: null PairPtr");
     hipStream_t st = (hipStream_t)stream;
-    ISAI_HIP(hipMemsetAsync(d_PairPtr, 0, sizeof(int) * ((size_t)Mnnz + 1), st));
-    if (nb > 0) {
-        pairs_kernel<false><<<grid_for(nb, 128), 128, 0, st>>>(nb, d_Arow, d_Acol, d_Mbrow, d_Mcol, upper,
-                                                              d_PairPtr, nullptr, nullptr);
-        ISAI_LAUNCH_CHECK();
-    }
-    int s = exclusive_scan_i32(d_PairPtr, d_PairPtr, (long long)Mnnz + 1, st);
+    int s = isai::pairs_count_async(nb, Mnnz, d_Arow, d_Acol, d_Mbrow, d_Mcol, upper, d_PairPtr, st);
     if (s != ISAI_OK) return s;
     ISAI_HIP(hipMemcpyAsync(total_pairs, d_PairPtr + Mnnz, sizeof(int), hipMemcpyDeviceToHost, st));
     ISAI_HIP(hipStreamSynchronize(st));

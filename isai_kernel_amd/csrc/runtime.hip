@@ -3,6 +3,7 @@
 #include <cstring>
 
 #include "common.hpp"
+#include "internal.hpp"
 #include "isai_runtime.h"
 
 namespace isai {
@@ -36,6 +37,23 @@ __global__ void copy_f64_kernel(v2d *__restrict__ dst, const v2d *__restrict__ s
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n2; i += stride) dst[i] = src[i];
+}
+
+// Device-to-device copy of doubles with the streaming copy kernel (the runtime's
+// blit ran at 0.7 TB/s on the 98 MB ILU factor); odd counts or pointers not
+// 16-byte aligned go through hipMemcpyAsync.
+int copy_f64_d2d(double *dst, const double *src, size_t count, hipStream_t st) {
+    if (!count) return ISAI_OK;
+    if (count % 2 || ((uintptr_t)dst | (uintptr_t)src) % 16) {
+        ISAI_HIP(hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyDeviceToDevice, st));
+        return ISAI_OK;
+    }
+    const size_t n2 = count / 2;
+    size_t g = (n2 + 255) / 256;
+    if (g > 16384) g = 16384;
+    copy_f64_kernel<<<(unsigned)g, 256, 0, st>>>((v2d *)dst, (const v2d *)src, n2);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
 }
 
 static unsigned fill_grid(size_t n) {

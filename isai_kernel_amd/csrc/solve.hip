@@ -960,7 +960,7 @@ int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, con
     // --- ILU(0) on a copy of A
     isai_ilu_plan_t ilu = nullptr;
     int zp = -1;
-    ISAI_HIP(hipMemcpyAsync(h->d_LU, d_val, sizeof(double) * b2 * nnzb, hipMemcpyDeviceToDevice, st));
+    if ((s = copy_f64_d2d(h->d_LU, d_val, b2 * nnzb, st)) != ISAI_OK) return fail(s);
     if ((s = isai_bsrilu0_analysis(nb, nnzb, bs, d_row, d_col, &ilu, &zp, stream)) != ISAI_OK) return fail(s);
     isai_bsrilu0_plan_info(ilu, &h->nlevels, nullptr);
     ISAI_HIP(hipEventRecord(e0, st));
@@ -972,12 +972,13 @@ int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, con
     // --- ISAI construction
     int *d_buf[2][5] = {};  // mb, mc, pp, pl, ps
     double *d_dinv[2] = {}, *d_mval[2] = {};
-    int *d_map[2] = {}, *d_order[2] = {}, *d_colq = nullptr;
+    int *d_map[2] = {}, *d_order[2] = {}, *d_colq = nullptr, *d_pairs = nullptr;
     void *d_acc = nullptr;
     int totals[2] = {0, 0};
     auto cleanup = [&]() {
+        (void)hipFree(d_pairs);  // d_buf[f][3], d_buf[f][4] point into it
         for (int f = 0; f < 2; ++f) {
-            for (int k = 0; k < 5; ++k) (void)hipFree(d_buf[f][k]);
+            for (int k = 0; k < 3; ++k) (void)hipFree(d_buf[f][k]);
             (void)hipFree(d_dinv[f]);
             (void)hipFree(d_mval[f]);
             (void)hipFree(d_map[f]);
@@ -1033,17 +1034,35 @@ int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, con
     }
     ISAI_HIP(hipStreamSynchronize(st));
     ISAI_HIP(hipEventRecord(e1b, st));
-    // pair lists: the count pass returns the totals (one sync each, as main.cu:690)
+    // Pair counts of both factors and the build plan's per-column statistics, then
+    // one read-back of the two totals and the two largest Q_j (the reference reads
+    // each total with its own synchronisation, main.cu:690).
+    int qmax2[2] = {0, 0};
+    ISAI_HIP(hipMemsetAsync(d_acc, 0, 2 * acc_bytes, st));
+    for (int f = 0; f < 2 && s == ISAI_OK; ++f)
+        s = pairs_count_async(nb, (int)F[f].mc.size(), d_row, d_col, d_buf[f][0], d_buf[f][1], f == 1, d_buf[f][2], st);
+    for (int f = 0; f < 2 && s == ISAI_OK && nb > 0; ++f)
+        s = pair_column_q_launch(nb, d_buf[f][0], d_buf[f][2], d_colq + (size_t)f * nb,
+                                 (char *)d_acc + f * acc_bytes, st);
+    if (s != ISAI_OK) { cleanup(); return fail(s); }
     for (int f = 0; f < 2; ++f) {
-        const int nnz = (int)F[f].mc.size();
-        s = isai_pairs_count(nb, nnz, d_row, d_col, d_buf[f][0], d_buf[f][1], f == 1, d_buf[f][2], &totals[f], stream);
-        if (s != ISAI_OK) { cleanup(); return fail(s); }
-        if (hipMalloc(&d_buf[f][3], sizeof(int) * std::max(totals[f], 1)) != hipSuccess ||
-            hipMalloc(&d_buf[f][4], sizeof(int) * std::max(totals[f], 1)) != hipSuccess) {
+        ISAI_HIP(hipMemcpyAsync(&totals[f], d_buf[f][2] + F[f].mc.size(), sizeof(int), hipMemcpyDeviceToHost, st));
+        ISAI_HIP(hipMemcpyAsync(&qmax2[f], (char *)d_acc + f * acc_bytes + pair_acc_max_q_offset(), sizeof(int),
+                                hipMemcpyDeviceToHost, st));
+    }
+    ISAI_HIP(hipStreamSynchronize(st));
+    // one allocation holds both factors' pair lists (PairPLocal, PairSrc)
+    {
+        const size_t t0 = (size_t)std::max(totals[0], 1), t1 = (size_t)std::max(totals[1], 1);
+        if (hipMalloc(&d_pairs, sizeof(int) * 2 * (t0 + t1)) != hipSuccess) {
             cleanup();
             set_error("isai_setup: allocation failed");
             return fail(ISAI_ERR_ALLOC);
         }
+        d_buf[0][3] = d_pairs;
+        d_buf[0][4] = d_pairs + t0;
+        d_buf[1][3] = d_pairs + 2 * t0;
+        d_buf[1][4] = d_pairs + 2 * t0 + t1;
     }
     for (int f = 0; f < 2 && s == ISAI_OK; ++f) {
         s = isai_pairs_fill(nb, d_row, d_col, d_buf[f][0], d_buf[f][1], f == 1, d_buf[f][2], d_buf[f][3], d_buf[f][4], stream);
@@ -1056,15 +1075,6 @@ int isai_setup(int nb, int nnzb, int bs, const int *d_row, const int *d_col, con
     // columns start first and share waves with columns of similar length.
     if (s == ISAI_OK && nb > 0) {
         constexpr double kOrderSkew = 8.0;
-        int qmax2[2] = {0, 0};
-        ISAI_HIP(hipMemsetAsync(d_acc, 0, 2 * acc_bytes, st));
-        for (int f = 0; f < 2 && s == ISAI_OK; ++f)
-            s = pair_column_q_launch(nb, d_buf[f][0], d_buf[f][2], d_colq + (size_t)f * nb,
-                                     (char *)d_acc + f * acc_bytes, st);
-        for (int f = 0; f < 2 && s == ISAI_OK; ++f)
-            ISAI_HIP(hipMemcpyAsync(&qmax2[f], (char *)d_acc + f * acc_bytes + pair_acc_max_q_offset(), sizeof(int),
-                                    hipMemcpyDeviceToHost, st));
-        if (s == ISAI_OK) ISAI_HIP(hipStreamSynchronize(st));
         for (int f = 0; f < 2 && s == ISAI_OK; ++f) {
             const int qmax = qmax2[f];
             const double qavg = (double)totals[f] / nb;
