@@ -1,13 +1,20 @@
 // convert.hip -- device CSR -> BSR conversion (the device form of csr_to_bsr5,
 // reference csr2bsr.h:14-76), bit-identical to the host form.
 //
-// One 64-lane workgroup per block row.  The block columns of the row's entries
-// are sorted in LDS (bitonic) and compacted; each block's first entry in CSR
-// order is found with an LDS atomicMin; then lane lr walks scalar row lr in CSR
-// order and, for each entry, assigns (first entry of its block) or adds (any other
-// entry) into the block held in LDS -- the host form's traversal, so every value
-// sees the same operations in the same order.  LDS is sized on the host from the
-// longest block row.
+// One 64-lane workgroup per block row.
+//   * The block columns of the row's entries are sorted and compacted: in
+//     registers by a wave-level bitonic network (up to 512 entries, 8 per lane,
+//     no LDS and no barrier), or for longer block rows in LDS.
+//   * Each entry's value slot (block, row, column) is found in parallel, with the
+//     block's first entry in CSR order (LDS atomicMin) and the number of entries
+//     per slot (LDS atomicAdd).
+//   * A slot with one entry takes it directly: the entry itself if it is its
+//     block's first entry, else 0.0 + value -- what the host form's traversal gives
+//     (assign the block's first entry, add every other into the zeroed block).
+//   * Slots with several entries (duplicates) are filled by the host form's
+//     traversal itself: lane lr walks scalar row lr in CSR order, applying only
+//     those entries, so every value sees the same operations in the same order.
+// LDS is sized on the host from the longest block row.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -83,13 +90,91 @@ __device__ int c2b_unique(int bs, const int *__restrict__ ci, int e0, int m, int
     return u;
 }
 
+// Ascending bitonic sort of 64 * KPL keys held KPL per lane (element e = lane * KPL
+// + i) by one wave: partners in other lanes are exchanged with shuffles, partners
+// in the same lane swapped in registers.
+template <int KPL>
+__device__ __forceinline__ void c2b_sort_regs(int (&key)[KPL]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 2; k <= 64 * KPL; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= KPL) {
+                const int lj = j / KPL;
+                const bool lower = (lane & lj) == 0;
+#pragma unroll
+                for (int i = 0; i < KPL; ++i) {
+                    const bool up = ((lane * KPL + i) & k) == 0;
+                    const int other = __shfl_xor(key[i], lj);
+                    key[i] = (lower == up) ? min(key[i], other) : max(key[i], other);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < KPL; ++i) {
+                    if (i & j) continue;
+                    const bool up = ((lane * KPL + i) & k) == 0;
+                    const int a = key[i], b = key[i | j];
+                    key[i] = up ? min(a, b) : max(a, b);
+                    key[i | j] = up ? max(a, b) : min(a, b);
+                }
+            }
+        }
+    }
+}
+
+// c2b_unique for m <= 64 * KPL entries, sorted in registers; ukeys in LDS.
+template <int KPL>
+__device__ int c2b_unique_regs(int bs, const int *__restrict__ ci, int e0, int m, int *ukeys) {
+    const int lane = threadIdx.x & 63;
+    int key[KPL];
+#pragma unroll
+    for (int i = 0; i < KPL; ++i) {
+        const int e = lane * KPL + i;
+        key[i] = e < m ? ci[e0 + e] / bs : INT_MAX;
+    }
+    c2b_sort_regs<KPL>(key);
+    const int prev_last = __shfl_up(key[KPL - 1], 1);
+    bool flag[KPL];
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < KPL; ++i) {
+        const int prev = i ? key[i - 1] : prev_last;
+        flag[i] = key[i] != INT_MAX && ((lane == 0 && i == 0) || key[i] != prev);
+        c += flag[i];
+    }
+    int incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off);
+        if (lane >= off) incl += t;
+    }
+    int w = incl - c;
+#pragma unroll
+    for (int i = 0; i < KPL; ++i)
+        if (flag[i]) ukeys[w++] = key[i];
+    const int u = __shfl(incl, 63);
+    __syncthreads();
+    return u;
+}
+
+// Unique sorted block columns of block row I (m entries from e0) in ukeys; keys is
+// LDS scratch for rows longer than the register networks take.
+__device__ int c2b_unique_any(int bs, const int *__restrict__ ci, int e0, int m, int *keys, int *ukeys) {
+    const int m2 = c2b_pow2(max(m, 1));
+    if (m2 <= 64) return c2b_unique_regs<1>(bs, ci, e0, m, ukeys);
+    if (m2 <= 128) return c2b_unique_regs<2>(bs, ci, e0, m, ukeys);
+    if (m2 <= 256) return c2b_unique_regs<4>(bs, ci, e0, m, ukeys);
+    if (m2 <= 512) return c2b_unique_regs<8>(bs, ci, e0, m, ukeys);
+    return c2b_unique(bs, ci, e0, m, m2, keys, ukeys);
+}
+
 __global__ __launch_bounds__(64) void c2b_count_kernel(int nb, int bs, const int *__restrict__ rp,
                                                        const int *__restrict__ ci, int m2max, int *counts) {
     extern __shared__ int lds[];
     const int I = blockIdx.x;
     const int e0 = rp[I * bs], m = rp[(I + 1) * bs] - e0;
-    const int m2 = c2b_pow2(max(m, 1));
-    const int u = c2b_unique(bs, ci, e0, m, m2, lds, lds + m2max);
+    const int u = c2b_unique_any(bs, ci, e0, m, lds, lds + m2max);
     if (threadIdx.x == 0) counts[I] = u;
 }
 
@@ -98,31 +183,64 @@ __global__ __launch_bounds__(64) void c2b_fill_kernel(int nb, int bs, const int 
                                                       const int *__restrict__ brp, int *__restrict__ bci,
                                                       double *__restrict__ bv, int m2max, int umax, int *err) {
     extern __shared__ int lds[];
+    const int bs2 = bs * bs;
     int *keys = lds, *ukeys = lds + m2max, *first = ukeys + m2max;
-    double *vals = (double *)(first + ((umax + 1) & ~1));  // 8-byte aligned
+    int *cnt = first + umax;                                      // umax * bs2 entries per slot
+    double *vals = (double *)(cnt + ((umax + umax * bs2 + 1) & ~1) - umax);  // 8-byte aligned
+    __shared__ int s_dup;
     const int I = blockIdx.x;
     const int e0 = rp[I * bs], m = rp[(I + 1) * bs] - e0;
-    const int m2 = c2b_pow2(max(m, 1));
-    const int bs2 = bs * bs;
-    const int u = c2b_unique(bs, ci, e0, m, m2, keys, ukeys);
+    int rstart[8];  // first entry of each scalar row of the block row
+#pragma unroll
+    for (int r = 0; r < 8; ++r) rstart[r] = r < bs ? rp[I * bs + r] : INT_MAX;
+    const int u = c2b_unique_any(bs, ci, e0, m, keys, ukeys);
     if (u != brp[I + 1] - brp[I]) {  // row pointer not from the count pass: write nothing
         if (threadIdx.x == 0) atomicOr(err, 1);
         return;
     }
+    const int nslot = u * bs2;
     for (int q = threadIdx.x; q < u; q += 64) first[q] = INT_MAX;
-    for (int t = threadIdx.x; t < u * bs2; t += 64) vals[t] = 0.0;
+    for (int t = threadIdx.x; t < nslot; t += 64) {
+        vals[t] = 0.0;
+        cnt[t] = 0;
+    }
+    if (threadIdx.x == 0) s_dup = 0;
     __syncthreads();
+    // slot of entry k: block q, scalar row lr (from the row starts), column c % bs
+    auto slot_of = [&](int k, int &q) {
+        const int c = ci[e0 + k];
+        q = c2b_lower_bound(ukeys, u, c / bs);
+        int lr = 0;
+#pragma unroll
+        for (int r = 1; r < 8; ++r) lr += rstart[r] <= e0 + k;
+        return q * bs2 + lr * bs + c % bs;
+    };
     for (int k = threadIdx.x; k < m; k += 64) {
-        const int q = c2b_lower_bound(ukeys, u, ci[e0 + k] / bs);
+        int q;
+        const int sl = slot_of(k, q);
         atomicMin(&first[q], e0 + k);
+        atomicAdd(&cnt[sl], 1);
     }
     __syncthreads();
-    if ((int)threadIdx.x < bs) {
+    for (int k = threadIdx.x; k < m; k += 64) {
+        int q;
+        const int sl = slot_of(k, q);
+        if (cnt[sl] == 1) {
+            const double x = v[e0 + k];
+            vals[sl] = e0 + k == first[q] ? x : 0.0 + x;
+        } else {
+            s_dup = 1;
+        }
+    }
+    __syncthreads();
+    if (s_dup && (int)threadIdx.x < bs) {
         const int lr = threadIdx.x, row = I * bs + lr;
         for (int p = rp[row]; p < rp[row + 1]; ++p) {
             const int c = ci[p];
             const int q = c2b_lower_bound(ukeys, u, c / bs);
-            double &dst = vals[q * bs2 + lr * bs + c % bs];
+            const int sl = q * bs2 + lr * bs + c % bs;
+            if (cnt[sl] == 1) continue;
+            double &dst = vals[sl];
             if (p == first[q]) dst = v[p];
             else dst += v[p];
         }
@@ -130,7 +248,7 @@ __global__ __launch_bounds__(64) void c2b_fill_kernel(int nb, int bs, const int 
     __syncthreads();
     const int base = brp[I];
     for (int q = threadIdx.x; q < u; q += 64) bci[base + q] = ukeys[q];
-    for (int t = threadIdx.x; t < u * bs2; t += 64) bv[(size_t)base * bs2 + t] = vals[t];
+    for (int t = threadIdx.x; t < nslot; t += 64) bv[(size_t)base * bs2 + t] = vals[t];
 }
 
 // Longest block row (CSR entries) and the LDS sizes that follow from it.
@@ -211,7 +329,9 @@ int isai_csr_to_bsr_device_fill(int n, int bs, const int *d_rp, const int *d_ci,
     ISAI_HIP(hipStreamSynchronize(st));
     int umax = 1;
     for (int I = 0; I < nb; ++I) umax = std::max(umax, brp[I + 1] - brp[I]);
-    const size_t lds = sizeof(int) * (2 * (size_t)m2max + ((umax + 1) & ~1)) + sizeof(double) * (size_t)umax * bs * bs;
+    const size_t bs2 = (size_t)bs * bs;
+    const size_t lds = sizeof(int) * (2 * (size_t)m2max + (((size_t)umax + umax * bs2 + 1) & ~(size_t)1)) +
+                       sizeof(double) * (size_t)umax * bs2;
     if (lds > 64 * 1024) {
         set_error("csr_to_bsr (device): a block row needs %zu bytes of LDS (limit 65536); use the host form", lds);
         return ISAI_ERR_UNSUPPORTED;

@@ -451,25 +451,28 @@ __global__ void reorder_kernel(long long total, int bs2, const int *__restrict__
 }
 
 // ------------------------------------------------ BSR -> CSR (all block entries)
-// One thread per scalar row: row r = br*bs + lr holds its blocks in block-column
-// order and lc = 0..bs-1 inside each block; nnz = nnzb*bs*bs.
-__global__ void bsr2csr_kernel(int nb, int bs, const int *__restrict__ brp, const int *__restrict__ bci,
-                               const double *__restrict__ bv, int *__restrict__ rp,
-                               int *__restrict__ ci, double *__restrict__ v) {
-    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= (long long)nb * bs) return;
-    const int br = (int)(row / bs), lr = (int)(row % bs), bs2 = bs * bs;
-    const int len = brp[br + 1] - brp[br];
-    long long out = (long long)brp[br] * bs2 + (long long)lr * len * bs;
-    rp[row + 1] = (int)(out + (long long)len * bs);
-    if (row == 0) rp[0] = 0;
-    for (int q = brp[br]; q < brp[br + 1]; ++q) {
-        const int c0 = bci[q] * bs;
-        const double *blk = bv + (size_t)q * bs2 + lr * bs;
-        for (int lc = 0; lc < bs; ++lc, ++out) {
-            ci[out] = c0 + lc;
-            v[out] = blk[lc];
-        }
+// Scalar row r = br*bs + lr holds its blocks in block-column order and lc =
+// 0..bs-1 inside each block; nnz = nnzb*bs*bs.  The CSR entries of block row br
+// form one contiguous range [brp[br]*bs2, brp[br+1]*bs2), so one wave per block row
+// writes it in order, one entry per lane: the stores are coalesced (one thread per
+// scalar row, each writing its own run, took 2.4 ms on the 96^3 7-pt matrix).
+constexpr int kB2cBlock = 256;
+__global__ __launch_bounds__(kB2cBlock) void bsr2csr_kernel(int nb, int bs, const int *__restrict__ brp,
+                                                            const int *__restrict__ bci, const double *__restrict__ bv,
+                                                            int *__restrict__ rp, int *__restrict__ ci,
+                                                            double *__restrict__ v) {
+    const int br = blockIdx.x * (kB2cBlock / 64) + (int)(threadIdx.x >> 6);
+    if (br >= nb) return;
+    const int lane = threadIdx.x & 63, bs2 = bs * bs;
+    const int b0 = brp[br], len = brp[br + 1] - b0;
+    const int rowlen = len * bs;  // entries per scalar row of this block row
+    const size_t out0 = (size_t)b0 * bs2;
+    if (lane < bs) rp[(size_t)br * bs + lane + 1] = (int)(out0 + (size_t)(lane + 1) * rowlen);
+    if (br == 0 && lane == 0) rp[0] = 0;
+    for (int o = lane; o < len * bs2; o += 64) {
+        const int lr = o / rowlen, rem = o - lr * rowlen, q = rem / bs, lc = rem - q * bs;
+        ci[out0 + o] = bci[b0 + q] * bs + lc;
+        v[out0 + o] = bv[(size_t)(b0 + q) * bs2 + lr * bs + lc];
     }
 }
 
@@ -611,9 +614,8 @@ int isai_bsr_to_csr(int mb, int nnzb, int bs, const int *d_brp, const int *d_bci
     ISAI_REQUIRE(mb >= 0 && nnzb >= 0 && bs >= 1 && bs <= 8, "isai_bsr_to_csr: bad size");
     ISAI_REQUIRE((long long)nnzb * bs * bs < (1LL << 31), "isai_bsr_to_csr: nnz overflows int32");
     if (mb == 0) return ISAI_OK;
-    const long long rows = (long long)mb * bs;
-    bsr2csr_kernel<<<grid_for(rows, 256), 256, 0, (hipStream_t)stream>>>(mb, bs, d_brp, d_bci, d_bv,
-                                                                          d_rp, d_ci, d_v);
+    bsr2csr_kernel<<<grid_for(mb, kB2cBlock / 64), kB2cBlock, 0, (hipStream_t)stream>>>(mb, bs, d_brp, d_bci, d_bv,
+                                                                                        d_rp, d_ci, d_v);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
