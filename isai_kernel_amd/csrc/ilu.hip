@@ -37,7 +37,8 @@ struct isai_ilu_plan {
     const int *d_row = nullptr, *d_col = nullptr;
     int *d_order = nullptr, *d_diag = nullptr;
     int *d_upd_ptr = nullptr, *d_upd_q = nullptr, *d_upd_src = nullptr;  // update lists
-    int max_row = 0;
+    int max_row = 0;    // blocks of the longest row
+    int max_stage = 0;  // largest staged set of one row (diagonal blocks read + update sources), <= kIluStage
 };
 
 namespace isai {
@@ -85,14 +86,28 @@ __device__ __forceinline__ int ilu_find(const int *__restrict__ rp, const int *_
     return -1;
 }
 
-// Per-wave LDS: the row being factored (kIluMaxRow blocks) and a staging area for
-// the blocks of earlier rows it reads (kIluStage blocks).  Waves per workgroup: four
-// while a wave needs at most 16 KB.
-template <int BS>
-struct IluWaves {
-    static constexpr int bytes = (kIluMaxRow + kIluStage) * BS * BS * 8;
-    static constexpr int W = bytes <= 16384 ? 4 : (bytes <= 32768 ? 2 : 1);
+// Per-wave LDS, sized by the analysis: the row being factored (the longest row of
+// the matrix) and a staging area for the blocks of earlier rows it reads (the
+// largest such set, at most kIluStage blocks).  A workgroup takes one ticket for
+// all its waves (rows), and tickets are same-address atomics, which serialise:
+// when the levels are wide (kIluWideLevel rows on average) 16 rows share a ticket
+// (96^3 7-pt: 2.88 ms against 3.45 ms with 4), else 4 (2-D 256^2 9-pt, 85 rows per
+// level: 5.05 ms with 4 against 5.86 ms with 16), as long as their LDS fits in 64 KB.
+// Narrow level structures also keep the fixed per-wave layout (kIluMaxRow +
+// kIluStage blocks): its LDS holds a CU to fewer resident rows, and with narrow
+// levels more resident rows only add waiting ones (5.56 ms with the plan-sized
+// layout on the 2-D case).
+constexpr int kIluMaxWaves = 16, kIluWideLevel = 512;
+struct IluLaunch {
+    int waves, row_blocks, stage_blocks;
 };
+static IluLaunch ilu_launch_shape(const isai_ilu_plan *P) {
+    const bool wide = (long long)P->nb >= (long long)kIluWideLevel * std::max(P->nlevels, 1);
+    IluLaunch L{wide ? kIluMaxWaves : 4, wide ? P->max_row : kIluMaxRow, wide ? P->max_stage : kIluStage};
+    const size_t wave = (size_t)(L.row_blocks + L.stage_blocks) * P->bs * P->bs * sizeof(double);
+    while (L.waves > 1 && wave * L.waves > 65536) L.waves >>= 1;
+    return L;
+}
 
 // Orders the LDS accesses of one wave (its lanes exchange data through LDS).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -113,32 +128,31 @@ __device__ __forceinline__ void wave_lds_sync() {
 // sequential part of the factorization runs from LDS.  Rows whose list exceeds
 // the staging area read those blocks from global memory instead.
 template <int BS>
-__global__ __launch_bounds__(64 * IluWaves<BS>::W) void ilu0_kernel(
+__global__ __launch_bounds__(64 * kIluMaxWaves) void ilu0_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
     double *fac, const int *__restrict__ order, const int *__restrict__ diagpos, const int *__restrict__ upd_ptr,
     const int *__restrict__ upd_q, const int *__restrict__ upd_src, unsigned int *__restrict__ ticket,
-    int *__restrict__ zero_pivot, int *__restrict__ err) {
+    int *__restrict__ zero_pivot, int *__restrict__ err, int max_row, int max_stage) {
     using S = IluShape<BS>;
     constexpr int B2 = S::B2;
-    constexpr int W = IluWaves<BS>::W;
     constexpr int CB = 64 / BS;  // blocks per pass when one lane owns one row or column
-    __shared__ double rowv_all[W][kIluMaxRow * B2];
-    __shared__ double stage_all[W][kIluStage * B2];
+    extern __shared__ double ilu_lds[];
     __shared__ int s_base;
+    const int W = blockDim.x >> 6;
     if (threadIdx.x == 0) s_base = (int)atomicAdd(ticket, (unsigned)W);
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int pos = s_base + w;
     if (pos >= nb) return;
-    double *rowv = rowv_all[w];
-    double *stage = stage_all[w];
+    double *rowv = ilu_lds + (size_t)w * (max_row + max_stage) * B2;
+    double *stage = rowv + (size_t)max_row * B2;
     const int i = order[pos];
     const int rs = rp[i], re = rp[i + 1], len = re - rs;
     const int dp = diagpos[i];
     const int nlow = dp - rs;
     const int u0 = upd_ptr[rs], nupd = upd_ptr[dp] - u0;
     const int nstage = nlow + nupd;
-    const bool staged = nstage <= kIluStage;
+    const bool staged = nstage <= max_stage;
     const int g = lane / S::G, e = lane % S::G;
     const bool ok = e < B2;
     const int r = ok ? e / BS : 0, c = ok ? e % BS : 0;
@@ -247,10 +261,12 @@ __global__ __launch_bounds__(64 * IluWaves<BS>::W) void ilu0_kernel(
 template <int BS>
 void launch_ilu(const isai_ilu_plan *P, const double *val, double *fac, unsigned *ticket, int *zp, int *err,
                 hipStream_t st) {
-    constexpr int W = IluWaves<BS>::W;
-    ilu0_kernel<BS><<<(P->nb + W - 1) / W, 64 * W, 0, st>>>(P->nb, P->d_row, P->d_col, val, fac, P->d_order,
-                                                           P->d_diag, P->d_upd_ptr, P->d_upd_q, P->d_upd_src, ticket,
-                                                           zp, err);
+    const IluLaunch L = ilu_launch_shape(P);
+    const int W = L.waves;
+    const size_t lds = (size_t)W * (L.row_blocks + L.stage_blocks) * BS * BS * sizeof(double);
+    ilu0_kernel<BS><<<(P->nb + W - 1) / W, 64 * W, lds, st>>>(P->nb, P->d_row, P->d_col, val, fac, P->d_order,
+                                                             P->d_diag, P->d_upd_ptr, P->d_upd_q, P->d_upd_src,
+                                                             ticket, zp, err, L.row_blocks, L.stage_blocks);
 }
 
 }  // namespace isai
@@ -312,6 +328,13 @@ int isai_bsrilu0_analysis(int nb, int nnzb, int bs, const int *d_row, const int 
         }
     }
     upd_ptr[nnzb] = (int)upd_q.size();
+    // the largest staged set of a row that fits the staging cap (longer ones read global memory)
+    int max_stage = 0;
+    for (int i = 0; i < nb; ++i) {
+        const int d = diag[i];
+        const int n = (d - rp[i]) + (upd_ptr[d] - upd_ptr[rp[i]]);
+        if (n <= kIluStage) max_stage = std::max(max_stage, n);
+    }
     if (upd_q.size() > (size_t)INT_MAX / 2) {
         set_error("bsrilu0 analysis: %zu updates exceed the int32 index range", upd_q.size());
         return ISAI_ERR_CAPACITY;
@@ -327,6 +350,7 @@ int isai_bsrilu0_analysis(int nb, int nnzb, int bs, const int *d_row, const int 
     P->d_row = d_row;
     P->d_col = d_col;
     P->max_row = max_row;
+    P->max_stage = max_stage;
     const size_t nu = upd_q.size();
     if (hipMalloc(&P->d_order, sizeof(int) * (nb ? nb : 1)) != hipSuccess ||
         hipMalloc(&P->d_diag, sizeof(int) * (nb ? nb : 1)) != hipSuccess ||
