@@ -41,14 +41,19 @@ int isai_csr_sptrsv_lu_device(int m, int nnz, const int *d_rowptr, const int *d_
  *     ahead of the serial chain.  A block row may depend on at most 64 blocks
  *     outside its chunk (ISAI_ERR_UNSUPPORTED otherwise; the CSR form has no such
  *     limit);
- *   row schedule: one quad of lanes per block row, rows in level order, every
- *     dependency polled from the solution vector; no limit on the row length.
- * chunk = 1..128: chunk schedule with at most that many rows per chunk; chunk < 0:
- * row schedule; chunk = 0: the row schedule when the levels of the triangles
- * average at least 16 block rows, else the chunk schedule with 128.  The analysis
- * builds the tables from the pattern and pre-scales the off-diagonal blocks by the
- * inverted diagonal parts of the packed factor; isai_bsr_sptrsv_update_values
- * redoes the scaling for new values on the same pattern. */
+ *   row schedule: one quad of lanes per block row, 64 rows per workgroup; no
+ *     limit on the row length.  In unit order (the default) the rows are grouped
+ *     into units of 64 so that most dependencies fall inside a unit: those travel
+ *     through LDS, and one wave per workgroup imports the others from the solution
+ *     vector.  In level order every dependency is polled from the solution vector.
+ * chunk = 1..128: chunk schedule with at most that many rows per chunk; chunk = -1:
+ * row schedule in unit order; -2: in level order; -3: unit order without the
+ * import wave (what -1 falls back to when a unit imports more than 512 rows);
+ * chunk = 0: -1 when the levels of the triangles average at least 16 block rows,
+ * else the chunk schedule with 128.  The analysis builds the tables from the
+ * pattern and pre-scales the off-diagonal blocks by the inverted diagonal parts of
+ * the packed factor; isai_bsr_sptrsv_update_values redoes the scaling for new
+ * values on the same pattern. */
 typedef struct isai_sptrsv_plan *isai_sptrsv_plan_t;
 int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const int *d_col,
                              const double *d_LU, int chunk, isai_sptrsv_plan_t *plan,

@@ -9,8 +9,10 @@
 //   * chunk schedule: one wave owns a CHUNK of consecutive rows (ascending for L,
 //     descending for U) taken from an atomic ticket, and walks them in order:
 //     dependencies inside the chunk are read from LDS, never waited on;
-//   * row schedule: one quad of lanes per block row, rows in level order, and the
-//     quads of a wave advance independently of each other;
+//   * row schedule: one quad of lanes per block row, and the quads of a wave
+//     advance independently of each other; rows in level order, or in unit order
+//     (units of 64 rows grown so that most dependencies stay inside, exchanged
+//     through LDS, with one wave per workgroup importing the rest);
 //   * dependencies on other chunks (or rows) are waited on with per-row ready flags that
 //     live in the solution vector itself: it is pre-filled with a NaN sentinel
 //     and every value is published with an agent-scope atomic store, so a reader
@@ -25,6 +27,7 @@
 // (or graph replays) need no memset.
 #include <algorithm>
 #include <climits>
+#include <queue>
 #include <vector>
 
 #include "cape_sptrsv.h"
@@ -324,10 +327,11 @@ __global__ __launch_bounds__(64) void bsr4_sptrsv_pipe_kernel(
 
 // ------------------------------------------------ BSR, bs = 4, row schedule
 // One quad (four lanes, one per row of the 4x4 block) per block row; the rows are
-// taken in level order (a row's level is one more than the deepest level among
-// its dependencies), 64 rows per workgroup from an atomic ticket.  A row only
-// depends on rows of lower levels, hence lower positions, hence a lower or equal
-// ticket: their workgroups are resident.  Within a wave the quads advance
+// taken in the plan's order -- level order (a row's level is one more than the
+// deepest level among its dependencies) or unit order -- 64 rows per workgroup
+// from an atomic ticket.  Both orders place a row after all of its dependencies,
+// so they have lower positions, hence a lower or equal ticket: their workgroups
+// are resident.  Within a wave the quads advance
 // independently of each other: a quad whose inputs are all published computes
 // and publishes its row in the same pass in which the other quads keep polling,
 // so rows of a wave may depend on each other.
@@ -345,6 +349,9 @@ constexpr int kRowBatch = 8;
 // solve ran 16 % faster.  Unused dynamic LDS of 96 KB per workgroup keeps one
 // workgroup (64 rows) per CU; its start-up stays off the critical path.
 constexpr int kRowLds = 96 * 1024;
+// The same kernel in unit order (below) holds rows longer -- a unit spans about
+// nine levels -- and needs more rows resident: three workgroups per CU.
+constexpr int kUnitRowLds = 48 * 1024;
 // The row schedule is chosen by default when the triangles' levels average at
 // least this many rows; narrower level structures (long chains) keep the chunk
 // schedule, whose chains resolve inside a wave.
@@ -358,19 +365,24 @@ __device__ __forceinline__ bool quad_all(bool v) {
     return a != 0;
 }
 
-// info[q] = (row, first dependency, end of dependencies, 1) for level-order
-// position q.
+// info[q] = (row, first dependency, end of dependencies, 1) for position q of
+// the processing order.  cx[p] is the column of dependency block p, or ~s when
+// that column is the row at slot s (position % 64) of the same workgroup: such a
+// value is exchanged through LDS (a row_x slot, pre-filled with the sentinel)
+// instead of the round trip through L2 that an agent-scope store and poll cost.
 template <bool UPPER>
 __global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
-    int nb, const int4 *__restrict__ info, const int *__restrict__ ci, const double *__restrict__ S,
+    int nb, const int4 *__restrict__ info, const int *__restrict__ cx, const double *__restrict__ S,
     const double *__restrict__ dinv, const double *rhs, double *out, double *sentinel_fill, Counters *cnt,
     const int *__restrict__ skip) {
     if (skip && *skip) return;
     __shared__ int s_t;
+    extern __shared__ unsigned long long row_x[];  // kRowsPerWG x 4 published values
     unsigned int *ticket = UPPER ? &cnt->ticket_u : &cnt->ticket_l;
     unsigned int *donec = UPPER ? &cnt->done_u : &cnt->done_l;
     const int limit = cnt->spin_limit;
     if (threadIdx.x == 0) s_t = (int)atomicAdd(ticket, 1u);
+    row_x[threadIdx.x] = kSentinel;
     __syncthreads();
     const int q = s_t * kRowsPerWG + (int)(threadIdx.x >> 2);
     const int m = threadIdx.x & 3;
@@ -384,8 +396,9 @@ __global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
         const double *b = rhs + (size_t)i * 4;
         z = D[0] * b[0] + D[1] * b[1] + D[2] * b[2] + D[3] * b[3];
     }
-    // batch registers: column of each dependency, row m of its block, polled x_k[m]
-    int kk[kRowBatch];
+    // batch registers: where each dependency's value is polled (its entry of the
+    // solution vector, or its LDS slot), row m of its block, polled x_k[m]
+    unsigned long long *src[kRowBatch];
     double s0[kRowBatch], s1[kRowBatch], s2[kRowBatch], s3[kRowBatch];
     unsigned long long u[kRowBatch];
     unsigned miss = 0;
@@ -393,13 +406,14 @@ __global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
     // One poll of every input in `want`: all loads are issued before any result is
     // compared, so a round costs one round trip however many inputs are missing
     // (a compare next to each load makes the compiler wait for every load in turn).
-    // Returns whether an input arrived.
+    // The loads are flat loads of addresses in LDS or in the solution vector, so
+    // the round is straight-line code (separate LDS and global loads under
+    // divergent branches made the compiler wait between them).  Returns whether
+    // an input arrived.
     auto poll_round = [&](unsigned want) {
 #pragma unroll
         for (int j = 0; j < kRowBatch; ++j)
-            if (want & (1u << j))
-                u[j] = __hip_atomic_load((unsigned long long *)(out + (size_t)kk[j] * 4 + m), __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
+            if (want & (1u << j)) u[j] = __hip_atomic_load(src[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned before = miss;
 #pragma unroll
         for (int j = 0; j < kRowBatch; ++j)
@@ -408,13 +422,22 @@ __global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
     };
     auto load_batch = [&]() {
         const int n = min(kRowBatch, pe - p0);
+        int kk[kRowBatch];
 #pragma unroll
-        for (int j = 0; j < kRowBatch; ++j)
+        for (int j = 0; j < kRowBatch; ++j) {
+            kk[j] = cx[p0 + min(j, n - 1)];  // unconditional: no register copy waits for it
             if (j < n) {
-                kk[j] = ci[p0 + j];
                 const double *B = S + (size_t)(p0 + j) * 16 + m * 4;
                 s0[j] = B[0], s1[j] = B[1], s2[j] = B[2], s3[j] = B[3];
             }
+        }
+#pragma unroll
+        for (int j = 0; j < kRowBatch; ++j)
+            if (j < n)
+                src[j] = kk[j] >= 0 ? (unsigned long long *)(out + (size_t)kk[j] * 4 + m) : &row_x[(~kk[j]) * 4 + m];
+        // every address before the first poll: a flat load in flight makes the
+        // compiler wait on both counters before the next address computation
+        __builtin_amdgcn_sched_barrier(0);
         miss = (1u << n) - 1u;
         poll_round(miss);
     };
@@ -439,6 +462,8 @@ __global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
             if (p0 < pe) {
                 load_batch();
             } else {
+                __hip_atomic_store(&row_x[threadIdx.x], (unsigned long long)__double_as_longlong(z), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
                 publish(out + (size_t)i * 4 + m, z);
                 if (sentinel_fill) sentinel_fill[(size_t)i * 4 + m] = __longlong_as_double((long long)kSentinel);
                 done_row = true;
@@ -456,6 +481,187 @@ __global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
             if (miss) timed_out = true;
             miss = 0;
             stall = 0;
+        }
+    }
+    if (__any(timed_out) && threadIdx.x % 64 == 0) atomicOr(&cnt->err, 1);
+    if (threadIdx.x == 0) {
+        if (atomicAdd(donec, 1u) == (unsigned)gridDim.x - 1) {
+            atomicExch(ticket, 0u);
+            atomicExch(donec, 0u);
+        }
+    }
+}
+
+// ---------------------------------------------- BSR, bs = 4, unit schedule
+// The row schedule's order cut into units of 64 rows (one workgroup each) grown
+// so that most dependencies fall inside a unit.  A workgroup has four compute
+// waves (one quad per row, as above) and one import wave:
+//   * the import wave polls the solution vector for the unit's imports (the
+//     distinct rows of other units its rows depend on) and copies each value into
+//     an LDS slot as soon as it is published;
+//   * the compute waves only ever poll LDS -- the unit's own rows' slots and the
+//     import slots -- so a pass of a compute wave costs an LDS round trip, not
+//     the L2 round trip of an agent-scope poll, and a row whose inputs are inside
+//     the unit follows its producer within a few hundred nanoseconds.
+// cxu[p] = ~slot for every dependency: slot < 64 is a row of the unit, 64 + k its
+// k-th import.  Imports of unit w are imp_col[imp_ptr[w] .. imp_ptr[w+1]).
+// kUnitBatch dependencies are held per pass, all polled every pass without a
+// branch; the plan takes the smallest of 2, 4, 8 that covers the longest row
+// (96^3 7-point, 3 dependencies: 840 us for L+U with 4 against 883 with 8;
+// 64^3 27-point, 13: 2.78 ms with 8 against 3.51 with 4).
+constexpr int kUnitWG = kRowWG + 64;
+constexpr int kUnitImpMax = 512;                 // imports per unit (2048 LDS words)
+constexpr int kUnitLds = 48 * 1024;              // residency cap: three workgroups per CU
+constexpr int kUnitItemsPerLane = kUnitImpMax * 4 / 64;
+
+template <bool UPPER, int kUnitBatch>
+__global__ __launch_bounds__(kUnitWG) void bsr4_sptrsv_unit_kernel(
+    int nb, const int4 *__restrict__ info, const int *__restrict__ cxu, const int *__restrict__ imp_ptr,
+    const int *__restrict__ imp_col, const double *__restrict__ S, const double *__restrict__ dinv,
+    const double *rhs, double *out, double *sentinel_fill, Counters *cnt, const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ int s_t;
+    extern __shared__ unsigned long long unit_x[];  // (64 + imports) x 4 values
+    unsigned int *ticket = UPPER ? &cnt->ticket_u : &cnt->ticket_l;
+    unsigned int *donec = UPPER ? &cnt->done_u : &cnt->done_l;
+    const int limit = cnt->spin_limit;
+    if (threadIdx.x == 0) s_t = (int)atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int w = s_t;
+    const int ib = imp_ptr[w], nit = (imp_ptr[w + 1] - ib) * 4;
+    for (int t = threadIdx.x; t < kRowWG + nit; t += kUnitWG) unit_x[t] = kSentinel;
+    __syncthreads();
+    bool timed_out = false;
+    if (threadIdx.x >= kRowWG) {
+        // import wave: lane l owns items e = l + 64 r (import e / 4, component e % 4)
+        const int l = threadIdx.x - kRowWG;
+        // word of the solution vector behind each item; the loads are issued
+        // together (clamped, no default value to merge, so no wait between them)
+        int at[kUnitItemsPerLane];
+        unsigned pend = 0;
+        if (nit > 0) {
+#pragma unroll
+            for (int r = 0; r < kUnitItemsPerLane; ++r) {
+                const int e = min(l + 64 * r, nit - 1);
+                at[r] = imp_col[ib + (e >> 2)] * 4 + (e & 3);
+                if (l + 64 * r < nit) pend |= 1u << r;
+            }
+        }
+        int stall = 0;
+        while (__any(pend != 0)) {
+            bool got = false;
+#pragma unroll
+            for (int g = 0; g < kUnitItemsPerLane / 8; ++g) {
+                const unsigned pm = (pend >> (8 * g)) & 0xffu;
+                if (!__any(pm != 0)) continue;
+                unsigned long long v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (pm & (1u << j))
+                        v[j] = __hip_atomic_load((unsigned long long *)(out + at[8 * g + j]), __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if ((pm & (1u << j)) && v[j] != kSentinel) {
+                        __hip_atomic_store(&unit_x[kRowWG + l + 64 * (8 * g + j)], v[j], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                        pend &= ~(1u << (8 * g + j));
+                        got = true;
+                    }
+            }
+            stall = __any(got) ? 0 : stall + 1;
+            if (stall > limit) {
+                // give up: the compute waves time out on the missing slots
+                if (pend) timed_out = true;
+                pend = 0;
+            }
+        }
+    } else {
+        const int q = w * kRowsPerWG + (int)(threadIdx.x >> 2);
+        const int m = threadIdx.x & 3;
+        const bool active = q < nb;
+        int i = 0, pb = 0, pe = 0;
+        double z = 0.0;
+        if (active) {
+            const int4 f = info[q];
+            i = f.x, pb = f.y, pe = f.z;
+            const double *D = dinv + (size_t)i * 16 + m * 4;
+            const double *b = rhs + (size_t)i * 4;
+            z = D[0] * b[0] + D[1] * b[1] + D[2] * b[2] + D[3] * b[3];
+        }
+        int sl[kUnitBatch];  // LDS word of each dependency's component m
+        double s0[kUnitBatch], s1[kUnitBatch], s2[kUnitBatch], s3[kUnitBatch];
+        unsigned long long u[kUnitBatch];
+        unsigned miss = 0;
+        int p0 = pb;
+        // every slot of the batch is read on every pass (clamped slots for the
+        // unused ones): no branch per input, and an arrived value reads the same
+        auto poll_round = [&]() {
+#pragma unroll
+            for (int j = 0; j < kUnitBatch; ++j)
+                u[j] = __hip_atomic_load(&unit_x[sl[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            unsigned got = 0;
+#pragma unroll
+            for (int j = 0; j < kUnitBatch; ++j) got |= (u[j] != kSentinel ? 1u : 0u) << j;
+            const unsigned before = miss;
+            miss &= ~got;
+            return miss != before;
+        };
+        auto load_batch = [&]() {
+            const int n = min(kUnitBatch, pe - p0);
+#pragma unroll
+            for (int j = 0; j < kUnitBatch; ++j) {
+                sl[j] = (~cxu[p0 + min(j, n - 1)]) * 4 + m;
+                if (j < n) {
+                    const double *B = S + (size_t)(p0 + j) * 16 + m * 4;
+                    s0[j] = B[0], s1[j] = B[1], s2[j] = B[2], s3[j] = B[3];
+                }
+            }
+            miss = (1u << n) - 1u;
+            poll_round();
+        };
+        if (p0 < pe) load_batch();
+        bool done_row = !active;
+        int stall = 0;
+        long long t_stall = 0;
+        while (true) {
+            bool progressed = false;
+            const bool ready = quad_all(miss == 0);
+            if (!done_row && ready) {
+#pragma unroll
+                for (int j = 0; j < kUnitBatch; ++j)
+                    if (p0 + j < pe) {
+                        const double xm = __longlong_as_double((long long)u[j]);
+                        const double x0 = quad_bcast<0>(xm), x1 = quad_bcast<1>(xm);
+                        const double x2 = quad_bcast<2>(xm), x3 = quad_bcast<3>(xm);
+                        z -= s0[j] * x0 + s1[j] * x1 + s2[j] * x2 + s3[j] * x3;
+                    }
+                p0 += kUnitBatch;
+                if (p0 < pe) {
+                    load_batch();
+                } else {
+                    __hip_atomic_store(&unit_x[threadIdx.x], (unsigned long long)__double_as_longlong(z),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    publish(out + (size_t)i * 4 + m, z);
+                    if (sentinel_fill) sentinel_fill[(size_t)i * 4 + m] = __longlong_as_double((long long)kSentinel);
+                    done_row = true;
+                }
+                progressed = true;
+            }
+            if (__all(done_row)) break;
+            if (!done_row && miss && poll_round()) progressed = true;
+            // LDS passes take tens of nanoseconds: the wait is bounded by the clock
+            // (100 MHz), spin_limit microseconds without progress
+            if (__any(progressed)) {
+                stall = 0;
+            } else {
+                if (stall == 0) t_stall = __builtin_amdgcn_s_memrealtime();
+                if ((++stall & 255) == 0 && __builtin_amdgcn_s_memrealtime() - t_stall > (long long)limit * 100) {
+                    if (miss) timed_out = true;
+                    miss = 0;
+                    stall = 0;
+                }
+            }
         }
     }
     if (__any(timed_out) && threadIdx.x % 64 == 0) atomicOr(&cnt->err, 1);
@@ -573,8 +779,15 @@ using namespace isai;
 
 struct isai_sptrsv_plan {
     int nb = 0, nnzb = 0, bs = 0, chunk = 0;
-    bool rows = false;  // row schedule (level-ordered quads) instead of chunks
-    int4 *d_info[2] = {nullptr, nullptr};  // row schedule: (row, first, end, 1) per level-order position
+    bool rows = false;   // row schedule (one quad per block row) instead of chunks
+    bool units = false;        // row schedule in unit order (else level order)
+    bool unit_kernel = false;  // units solved by the import-wave kernel (else the row kernel)
+    int4 *d_info[2] = {nullptr, nullptr};  // row schedule: (row, first, end, 1) per position
+    int *d_cx = nullptr;  // row kernel: column of each dependency block, ~slot inside its workgroup;
+                          // unit kernel: ~slot of each dependency (unit row or import)
+    int *d_imp_ptr[2] = {nullptr, nullptr}, *d_imp_col[2] = {nullptr, nullptr};  // unit kernel imports
+    int row_lds = 0;     // dynamic LDS per workgroup of the row or unit kernel
+    int unit_batch = 4;  // unit kernel: dependencies held per pass (2, 4 or 8)
     int poll_min = 64;  // smallest poll window of the adaptive poll (64 = fixed 64-item window)
     int nchunks[2] = {0, 0};
     int maxr[2] = {0, 0}, maxrem[2] = {0, 0}, maxloc[2] = {0, 0};
@@ -589,16 +802,47 @@ struct isai_sptrsv_plan {
 
 namespace isai {
 
+template <int B>
+static void unit_launch(isai_sptrsv_plan_t P, const double *d_b, double *d_x, hipStream_t st, const int *skip) {
+    const unsigned g = (unsigned)((P->nb + kRowsPerWG - 1) / kRowsPerWG);
+    bsr4_sptrsv_unit_kernel<false, B><<<g, kUnitWG, P->row_lds, st>>>(P->nb, P->d_info[0], P->d_cx, P->d_imp_ptr[0],
+                                                                     P->d_imp_col[0], P->d_S, P->d_dinvL, d_b,
+                                                                     P->d_y, d_x, P->d_cnt, skip);
+    bsr4_sptrsv_unit_kernel<true, B><<<g, kUnitWG, P->row_lds, st>>>(P->nb, P->d_info[1], P->d_cx, P->d_imp_ptr[1],
+                                                                    P->d_imp_col[1], P->d_S, P->d_dinvU, P->d_y,
+                                                                    d_x, nullptr, P->d_cnt, skip);
+}
+
+template <int B>
+static hipError_t unit_attr(int lds) {
+    hipError_t e = hipFuncSetAttribute((const void *)bsr4_sptrsv_unit_kernel<false, B>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)bsr4_sptrsv_unit_kernel<true, B>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    return e;
+}
+
 int sptrsv_solve_internal(isai_sptrsv_plan_t P, const double *d_LU, const double *d_b, double *d_x,
                           hipStream_t st, const int *skip) {
     (void)d_LU;  // the solve reads the pre-scaled copy made from these values
     const size_t n = (size_t)P->nb * 4;
     fill_sentinel_kernel<<<fill_blocks(n), 256, 0, st>>>(P->d_y, n, skip);
+    if (P->unit_kernel) {
+        if (P->unit_batch == 2)
+            unit_launch<2>(P, d_b, d_x, st, skip);
+        else if (P->unit_batch == 4)
+            unit_launch<4>(P, d_b, d_x, st, skip);
+        else
+            unit_launch<8>(P, d_b, d_x, st, skip);
+        ISAI_LAUNCH_CHECK();
+        return ISAI_OK;
+    }
     if (P->rows) {
         const unsigned g = (unsigned)((P->nb + kRowsPerWG - 1) / kRowsPerWG);
-        bsr4_sptrsv_row_kernel<false><<<g, kRowWG, kRowLds, st>>>(P->nb, P->d_info[0], P->d_col, P->d_S,
+        bsr4_sptrsv_row_kernel<false><<<g, kRowWG, P->row_lds, st>>>(P->nb, P->d_info[0], P->d_cx, P->d_S,
                                                                   P->d_dinvL, d_b, P->d_y, d_x, P->d_cnt, skip);
-        bsr4_sptrsv_row_kernel<true><<<g, kRowWG, kRowLds, st>>>(P->nb, P->d_info[1], P->d_col, P->d_S,
+        bsr4_sptrsv_row_kernel<true><<<g, kRowWG, P->row_lds, st>>>(P->nb, P->d_info[1], P->d_cx, P->d_S,
                                                                  P->d_dinvU, P->d_y, d_x, nullptr, P->d_cnt, skip);
         ISAI_LAUNCH_CHECK();
         return ISAI_OK;
@@ -745,12 +989,11 @@ static int build_tables(int nb, const std::vector<int> &rp, const std::vector<in
     return ISAI_OK;
 }
 
-// Level order of one triangle for the row schedule: level(i) = 1 + the deepest
-// level among the rows i depends on (0 without dependencies), rows in
-// processing-position order inside a level.  Returns the number of levels.
-static int level_order(int nb, const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &diag,
-                       bool upper, std::vector<int> &order) {
-    std::vector<int> lev(nb, 0);
+// Levels of one triangle: level(i) = 1 + the deepest level among the rows i
+// depends on (0 without dependencies).  Returns the number of levels.
+static int row_levels(int nb, const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &diag,
+                      bool upper, std::vector<int> &lev) {
+    lev.assign(nb, 0);
     int nlev = 0;
     for (int q = 0; q < nb; ++q) {
         const int i = upper ? nb - 1 - q : q;
@@ -760,6 +1003,12 @@ static int level_order(int nb, const std::vector<int> &rp, const std::vector<int
         lev[i] = l;
         nlev = std::max(nlev, l + 1);
     }
+    return nlev;
+}
+
+// Level order of one triangle for the row schedule: rows by level, in
+// processing-position order inside a level.
+static void level_order(int nb, const std::vector<int> &lev, int nlev, bool upper, std::vector<int> &order) {
     std::vector<int> cnt(nlev + 1, 0);
     for (int i = 0; i < nb; ++i) cnt[lev[i] + 1]++;
     for (int l = 0; l < nlev; ++l) cnt[l + 1] += cnt[l];
@@ -768,7 +1017,71 @@ static int level_order(int nb, const std::vector<int> &rp, const std::vector<int
         const int i = upper ? nb - 1 - q : q;
         order[cnt[lev[i]]++] = i;
     }
-    return nlev;
+}
+
+// Unit order of one triangle for the row schedule: the order is cut into units
+// of kRowsPerWG consecutive rows (one workgroup each), and it is grown so that
+// as many dependencies as possible fall inside a unit, where they travel through
+// LDS.  Greedy: a unit starts at the shallowest row whose dependencies are all
+// placed; it then repeatedly takes, among the rows that became ready while it
+// grew, the one with the most dependencies inside the unit (the shallower level,
+// then the processing position, on ties), and the shallowest ready row when
+// none is left.  A row is placed only after all of its dependencies, so every
+// dependency has a lower position (a lower or equal ticket).
+static void unit_order(int nb, const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &diag,
+                       bool upper, const std::vector<int> &lev, std::vector<int> &order) {
+    auto range = [&](int i, int &pb, int &pe) {
+        pb = upper ? diag[i] + 1 : rp[i];
+        pe = upper ? rp[i + 1] : diag[i];
+    };
+    auto rank = [&](int i) { return upper ? nb - 1 - i : i; };
+    // consumers of every row: the transpose of the triangle's pattern
+    std::vector<int> cp(nb + 1, 0), need(nb), unit(nb, -1);
+    for (int i = 0; i < nb; ++i) {
+        int pb, pe;
+        range(i, pb, pe);
+        need[i] = pe - pb;
+        for (int p = pb; p < pe; ++p) cp[ci[p] + 1]++;
+    }
+    for (int i = 0; i < nb; ++i) cp[i + 1] += cp[i];
+    std::vector<int> cons(cp[nb]), cur(cp.begin(), cp.end() - 1);
+    for (int i = 0; i < nb; ++i) {
+        int pb, pe;
+        range(i, pb, pe);
+        for (int p = pb; p < pe; ++p) cons[cur[ci[p]]++] = i;
+    }
+    using Ready = std::pair<long long, int>;  // (level << 32 | position rank, row)
+    using Local = std::pair<std::pair<int, long long>, int>;  // ((-inside, level key), row)
+    std::priority_queue<Ready, std::vector<Ready>, std::greater<Ready>> pool;
+    std::priority_queue<Local, std::vector<Local>, std::greater<Local>> local;
+    auto key = [&](int i) { return ((long long)lev[i] << 32) | (long long)rank(i); };
+    for (int i = 0; i < nb; ++i)
+        if (need[i] == 0) pool.push({key(i), i});
+    order.clear();
+    order.reserve(nb);
+    for (int u = 0; (int)order.size() < nb; ++u) {
+        for (int size = 0; size < kRowsPerWG && (int)order.size() < nb; ++size) {
+            int i;
+            if (!local.empty()) {
+                i = local.top().second;
+                local.pop();
+            } else {
+                i = pool.top().second;
+                pool.pop();
+            }
+            unit[i] = u;
+            order.push_back(i);
+            for (int t = cp[i]; t < cp[i + 1]; ++t) {
+                const int c = cons[t];
+                if (--need[c] != 0) continue;
+                int pb, pe, inside = 0;
+                range(c, pb, pe);
+                for (int p = pb; p < pe; ++p) inside += unit[ci[p]] == u;
+                local.push({{-inside, key(c)}, c});
+            }
+        }
+        for (; !local.empty(); local.pop()) pool.push({key(local.top().second), local.top().second});
+    }
 }
 
 }  // namespace isai
@@ -784,10 +1097,16 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         set_error("isai_bsr_sptrsv: the BSR-native solve is built for bs = 4 (got %d); use the CSR form", bs);
         return ISAI_ERR_UNSUPPORTED;
     }
+    // chunk < 0: the row schedule; -1 in unit order, -2 in level order, -3 in
+    // unit order with the row kernel (the fallback of -1 when a unit has more than
+    // kUnitImpMax imports)
     bool rows = chunk < 0;
+    const bool units = chunk != -2;
+    bool unit_kernel = chunk == -1 || chunk == 0;
     const bool automatic = chunk == 0;
     if (rows || automatic) chunk = kPipeRows;
-    ISAI_REQUIRE(chunk <= kPipeRows, "isai_bsr_sptrsv_analysis: chunk %d above %d", chunk, kPipeRows);
+    ISAI_REQUIRE(chunk <= kPipeRows && chunk >= -3, "isai_bsr_sptrsv_analysis: chunk %d outside [-3, %d]", chunk,
+                 kPipeRows);
     hipStream_t st = (hipStream_t)stream;
     std::vector<int> rp(nb + 1), ci(nnzb), diag(nb, -1);
     ISAI_HIP(hipMemcpyAsync(rp.data(), d_row, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, st));
@@ -810,14 +1129,62 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
     std::vector<int> rptr[2], ritems[2], lptr[2], litems[2], order[2];
     if (rows || automatic) {
         int nlev = 1;
-        for (int u = 0; u < 2; ++u) nlev = std::max(nlev, level_order(nb, rp, ci, diag, u == 1, order[u]));
+        std::vector<int> lev[2];
+        int nl[2];
+        for (int u = 0; u < 2; ++u) {
+            nl[u] = row_levels(nb, rp, ci, diag, u == 1, lev[u]);
+            nlev = std::max(nlev, nl[u]);
+        }
         if (automatic) rows = nb / nlev >= kRowMinWidth;
+        for (int u = 0; u < 2 && rows; ++u) {
+            if (units)
+                unit_order(nb, rp, ci, diag, u == 1, lev[u], order[u]);
+            else
+                level_order(nb, lev[u], nl[u], u == 1, order[u]);
+        }
     }
-    if (rows) {
+    // imports of every unit (unit kernel): the distinct rows of other units that
+    // its rows depend on, in order of first use
+    std::vector<int> imp_ptr[2], imp_col[2], pos[2];
+    int max_imp = 0;
+    if (rows && units && unit_kernel) {
+        std::vector<int> stamp(nb, -1);
+        const int nu = (nb + kRowsPerWG - 1) / kRowsPerWG;
+        for (int u = 0; u < 2; ++u) {
+            pos[u].resize(nb);
+            for (int q = 0; q < nb; ++q) pos[u][order[u][q]] = q;
+            std::fill(stamp.begin(), stamp.end(), -1);
+            imp_ptr[u].assign(nu + 1, 0);
+            for (int w = 0; w < nu; ++w) {
+                for (int q = w * kRowsPerWG; q < std::min(nb, (w + 1) * kRowsPerWG); ++q) {
+                    const int i = order[u][q];
+                    const int pb = u == 0 ? rp[i] : diag[i] + 1, pe = u == 0 ? diag[i] : rp[i + 1];
+                    for (int p = pb; p < pe; ++p) {
+                        const int c = ci[p];
+                        if (pos[u][c] / kRowsPerWG == w || stamp[c] == w) continue;
+                        stamp[c] = w;
+                        imp_col[u].push_back(c);
+                    }
+                }
+                imp_ptr[u][w + 1] = (int)imp_col[u].size();
+                max_imp = std::max(max_imp, imp_ptr[u][w + 1] - imp_ptr[u][w]);
+            }
+        }
+        if (max_imp > kUnitImpMax) unit_kernel = false;
+    }
+    unit_kernel = unit_kernel && rows && units;
+    const int row_lds = unit_kernel ? std::max(kUnitLds, (kRowsPerWG + max_imp) * 32) : units ? kUnitRowLds : kRowLds;
+    // the smallest batch that holds the longest row of either triangle
+    int max_deps = 0;
+    for (int i = 0; i < nb; ++i) max_deps = std::max({max_deps, diag[i] - rp[i], rp[i + 1] - diag[i] - 1});
+    const int unit_batch = max_deps <= 2 ? 2 : max_deps <= 4 ? 4 : 8;
+    if (unit_kernel) {
+        ISAI_HIP(unit_batch == 2 ? unit_attr<2>(row_lds) : unit_batch == 4 ? unit_attr<4>(row_lds) : unit_attr<8>(row_lds));
+    } else if (rows) {
         ISAI_HIP(hipFuncSetAttribute((const void *)bsr4_sptrsv_row_kernel<false>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, kRowLds));
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, row_lds));
         ISAI_HIP(hipFuncSetAttribute((const void *)bsr4_sptrsv_row_kernel<true>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, kRowLds));
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, row_lds));
     }
     for (int u = 0; u < 2 && !rows; ++u) {
         int s = build_tables(nb, rp, ci, diag, u == 1, chunk, chunks[u], rptr[u], ritems[u], lptr[u], litems[u]);
@@ -825,6 +1192,10 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
     }
     auto *P = new isai_sptrsv_plan;
     P->rows = rows;
+    P->units = rows && units;
+    P->unit_kernel = unit_kernel;
+    P->unit_batch = unit_batch;
+    P->row_lds = row_lds;
     P->nb = nb;
     P->nnzb = nnzb;
     P->bs = bs;
@@ -848,6 +1219,10 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
     }
     std::vector<int4> info;
     for (int u = 0; u < 2 && ok && rows; ++u) ok = hipMalloc(&P->d_info[u], sizeof(int4) * nb) == hipSuccess;
+    if (ok && rows) ok = hipMalloc(&P->d_cx, sizeof(int) * (size_t)nnzb) == hipSuccess;
+    for (int u = 0; u < 2 && ok && unit_kernel; ++u)
+        ok = hipMalloc(&P->d_imp_ptr[u], sizeof(int) * imp_ptr[u].size()) == hipSuccess &&
+             hipMalloc(&P->d_imp_col[u], sizeof(int) * std::max<size_t>(imp_col[u].size(), 1)) == hipSuccess;
     for (int u = 0; u < 2 && ok && !rows; ++u) {
         ok = hipMalloc(&P->d_chunks[u], sizeof(int4) * chunks[u].size()) == hipSuccess &&
              hipMalloc(&P->d_rem_ptr[u], sizeof(int) * (nb + 1)) == hipSuccess &&
@@ -868,6 +1243,39 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
             info[q] = u == 0 ? make_int4(i, rp[i], diag[i], 1) : make_int4(i, diag[i] + 1, rp[i + 1], 1);
         }
         ISAI_HIP(hipMemcpy(P->d_info[u], info.data(), sizeof(int4) * info.size(), hipMemcpyHostToDevice));
+    }
+    if (rows) {
+        // row kernel: a dependency on a row of the same workgroup (64 consecutive
+        // positions) becomes ~slot, read from LDS; unit kernel: every dependency is
+        // ~slot, an import k being slot 64 + k
+        std::vector<int> cx(ci), at(nb), imp_slot(nb);
+        for (int u = 0; u < 2; ++u) {
+            for (int q = 0; q < nb; ++q) at[order[u][q]] = q;
+            const int nu = (nb + kRowsPerWG - 1) / kRowsPerWG;
+            for (int w = 0; w < nu; ++w) {
+                if (unit_kernel)
+                    for (int k = imp_ptr[u][w]; k < imp_ptr[u][w + 1]; ++k) imp_slot[imp_col[u][k]] = k - imp_ptr[u][w];
+                for (int q = w * kRowsPerWG; q < std::min(nb, (w + 1) * kRowsPerWG); ++q) {
+                    const int i = order[u][q];
+                    const int pb = u == 0 ? rp[i] : diag[i] + 1, pe = u == 0 ? diag[i] : rp[i + 1];
+                    for (int p = pb; p < pe; ++p) {
+                        const int c = ci[p];
+                        if (at[c] / kRowsPerWG == w)
+                            cx[p] = ~(at[c] % kRowsPerWG);
+                        else if (unit_kernel)
+                            cx[p] = ~(kRowsPerWG + imp_slot[c]);
+                    }
+                }
+            }
+            if (unit_kernel) {
+                ISAI_HIP(hipMemcpy(P->d_imp_ptr[u], imp_ptr[u].data(), sizeof(int) * imp_ptr[u].size(),
+                                   hipMemcpyHostToDevice));
+                if (!imp_col[u].empty())
+                    ISAI_HIP(hipMemcpy(P->d_imp_col[u], imp_col[u].data(), sizeof(int) * imp_col[u].size(),
+                                       hipMemcpyHostToDevice));
+            }
+        }
+        ISAI_HIP(hipMemcpy(P->d_cx, cx.data(), sizeof(int) * (size_t)nnzb, hipMemcpyHostToDevice));
     }
     for (int u = 0; u < 2 && !rows; ++u) {
         ISAI_HIP(hipMemcpy(P->d_chunks[u], chunks[u].data(), sizeof(int4) * chunks[u].size(), hipMemcpyHostToDevice));
@@ -934,6 +1342,11 @@ int isai_bsr_sptrsv_destroy(isai_sptrsv_plan_t P) {
     (void)hipFree(P->d_y);
     (void)hipFree(P->d_S);
     (void)hipFree(P->d_cnt);
+    (void)hipFree(P->d_cx);
+    for (int u = 0; u < 2; ++u) {
+        (void)hipFree(P->d_imp_ptr[u]);
+        (void)hipFree(P->d_imp_col[u]);
+    }
     for (int u = 0; u < 2; ++u) {
         (void)hipFree(P->d_info[u]);
         (void)hipFree(P->d_chunks[u]);

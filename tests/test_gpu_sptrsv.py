@@ -21,7 +21,7 @@ def _plan(rp, ci, d_rp, d_ci, d_lu, chunk):
 
 
 @pytest.mark.parametrize("problem,L", [(0, 16), (1, 8), (2, 5), (3, 12)])
-@pytest.mark.parametrize("chunk", [1, 5, 64, 128, -1])
+@pytest.mark.parametrize("chunk", [1, 5, 64, 128, -1, -2, -3])
 def test_bsr_sptrsv_matches_oracle(gpu, problem, L, chunk):
     print(f"bsr sptrsv problem={problem} L={L} chunk={chunk}", flush=True)
     rp, ci, val = problems.generate(problem, L, seed=2)
@@ -80,10 +80,11 @@ def test_csr_sptrsv_device_form(gpu, chunk):
     assert rel_err(d_x.to_numpy(), ref) <= TOL
 
 
-@pytest.mark.parametrize("chunk", [0, -1])
+@pytest.mark.parametrize("chunk", [128, -1, -2, -3])
 def test_bsr_sptrsv_full_size_residual(gpu, chunk):
     """96^3 7-point: || L (U x) - b || / || b || with the GPU factor and solve, for
-    the chunk schedule (0) and the row schedule (-1)."""
+    the chunk schedule (128), the unit kernel (-1) and the row kernel in level (-2)
+    and unit (-3) order."""
     print(f"bsr sptrsv full size 96^3 chunk={chunk}", flush=True)
     rp, ci, val = problems.generate(1, 96, seed=0)
     d_rp, d_ci, d_lu = gpu_ilu(rp, ci, val)
@@ -102,7 +103,7 @@ def test_bsr_sptrsv_full_size_residual(gpu, chunk):
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-13
 
 
-@pytest.mark.parametrize("chunk", [0, -1])
+@pytest.mark.parametrize("chunk", [128, -1, -2])
 def test_spin_limit_timeout_is_reported_and_cleared(gpu, chunk):
     """A wait budget of one poll makes dependent chunks (or rows) give up (bounded
     waits: every kernel still exits); the plan reports ISAI_ERR_TIMEOUT once, then
