@@ -158,3 +158,46 @@ def test_solver_reports_triangular_solve_timeout(gpu):
         assert st == 0 and rr <= 1e-10
     finally:
         h.close()
+
+
+def _random_wide_pattern(nb, per_row, seed):
+    """Structurally symmetric block pattern with about `per_row` random partners per block row on each
+    side of the diagonal, diagonally dominant 4x4 blocks.  Rows depend on rows all over the matrix, so
+    a 64-row unit of the unit order imports more than 512 rows (up to about 1,000 here)."""
+    rng = np.random.default_rng(seed)
+    cols = [{i} for i in range(nb)]
+    for i in range(nb):
+        for j in rng.choice(nb, per_row, replace=False):
+            cols[i].add(int(j))
+            cols[int(j)].add(i)
+    rp = np.zeros(nb + 1, np.int32)
+    ci = []
+    for i in range(nb):
+        c = sorted(cols[i])
+        ci += c
+        rp[i + 1] = rp[i] + len(c)
+    ci = np.array(ci, np.int32)
+    val = rng.standard_normal((len(ci), 4, 4)) * 0.01
+    for i in range(nb):
+        val[rp[i] + np.searchsorted(ci[rp[i]:rp[i + 1]], i)] += 4.0 * np.eye(4)
+    return rp, ci, val
+
+
+@pytest.mark.parametrize("chunk", [0, -1, -2])
+def test_bsr_sptrsv_unit_import_cap_fallback(gpu, chunk):
+    """Unit order on a pattern whose units would import more than 512 rows: the plan falls back to the
+    row kernel in unit order, with the same results as the level order and the oracle."""
+    print(f"bsr sptrsv random wide pattern nb=2048 chunk={chunk}", flush=True)
+    rp, ci, val = _random_wide_pattern(2048, 12, seed=9)
+    d_rp, d_ci, d_lu = gpu_ilu(rp, ci, val)
+    n = (len(rp) - 1) * 4
+    b = RNG.standard_normal(n)
+    plan = _plan(rp, ci, d_rp, d_ci, d_lu, chunk)
+    try:
+        d_b, d_x = dev(b), DeviceArray((n,), np.float64)
+        for _ in range(2):
+            call("isai_bsr_sptrsv_solve", plan, d_lu.ptr, d_b.ptr, d_x.ptr, None)
+            call("isai_sptrsv_status", plan, None)
+            assert rel_err(d_x.to_numpy(), O.sptrsv_lu(rp, ci, d_lu.to_numpy(), b)) <= TOL
+    finally:
+        call("isai_bsr_sptrsv_destroy", plan)
