@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <climits>
 #include <queue>
+#include <thread>
 #include <vector>
 
 #include "cape_sptrsv.h"
@@ -1136,11 +1137,13 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
             nlev = std::max(nlev, nl[u]);
         }
         if (automatic) rows = nb / nlev >= kRowMinWidth;
-        for (int u = 0; u < 2 && rows; ++u) {
-            if (units)
-                unit_order(nb, rp, ci, diag, u == 1, lev[u], order[u]);
-            else
-                level_order(nb, lev[u], nl[u], u == 1, order[u]);
+        if (rows && units) {
+            // the two triangles' orders are independent: U's on a second host thread
+            std::thread tu([&] { unit_order(nb, rp, ci, diag, true, lev[1], order[1]); });
+            unit_order(nb, rp, ci, diag, false, lev[0], order[0]);
+            tu.join();
+        } else if (rows) {
+            for (int u = 0; u < 2; ++u) level_order(nb, lev[u], nl[u], u == 1, order[u]);
         }
     }
     // imports of every unit (unit kernel): the distinct rows of other units that
