@@ -512,7 +512,8 @@ __global__ __launch_bounds__(kRowWG) void bsr4_sptrsv_row_kernel(
 // 64^3 27-point, 13: 2.78 ms with 8 against 3.51 with 4).
 constexpr int kUnitWG = kRowWG + 64;
 constexpr int kUnitImpMax = 512;                 // imports per unit (2048 LDS words)
-constexpr int kUnitLds = 48 * 1024;              // residency cap: three workgroups per CU
+// residency cap: two workgroups per CU (96^3 7-point L+U: 827 us; three 838, four 838)
+constexpr int kUnitLds = 64 * 1024;
 constexpr int kUnitItemsPerLane = kUnitImpMax * 4 / 64;
 
 template <bool UPPER, int kUnitBatch>
