@@ -201,3 +201,24 @@ def test_bsr_sptrsv_unit_import_cap_fallback(gpu, chunk):
             assert rel_err(d_x.to_numpy(), O.sptrsv_lu(rp, ci, d_lu.to_numpy(), b)) <= TOL
     finally:
         call("isai_bsr_sptrsv_destroy", plan)
+
+
+@pytest.mark.parametrize("chunk", [0, -1])
+def test_bsr_sptrsv_unit_many_imports(gpu, chunk):
+    """Unit schedule on a random pattern whose units import up to ~440 rows (below the 512 cap): every
+    item group of the import wave is used, with the 8-dependency batch, against the oracle."""
+    print(f"bsr sptrsv random pattern nb=4096 (many imports) chunk={chunk}", flush=True)
+    rp, ci, val = _random_wide_pattern(4096, 4, seed=11)
+    d_rp, d_ci, d_lu = gpu_ilu(rp, ci, val)
+    n = (len(rp) - 1) * 4
+    b = RNG.standard_normal(n)
+    plan = _plan(rp, ci, d_rp, d_ci, d_lu, chunk)
+    try:
+        d_b, d_x = dev(b), DeviceArray((n,), np.float64)
+        ref = O.sptrsv_lu(rp, ci, d_lu.to_numpy(), b)
+        for _ in range(3):
+            call("isai_bsr_sptrsv_solve", plan, d_lu.ptr, d_b.ptr, d_x.ptr, None)
+            call("isai_sptrsv_status", plan, None)
+            assert rel_err(d_x.to_numpy(), ref) <= TOL
+    finally:
+        call("isai_bsr_sptrsv_destroy", plan)
