@@ -1139,10 +1139,25 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         }
         if (automatic) rows = nb / nlev >= kRowMinWidth;
         if (rows && units) {
-            // the two triangles' orders are independent: U's on a second host thread
-            std::thread tu([&] { unit_order(nb, rp, ci, diag, true, lev[1], order[1]); });
-            unit_order(nb, rp, ci, diag, false, lev[0], order[0]);
-            tu.join();
+            // A structurally symmetric pattern makes U's dependency graph L's reversed,
+            // so the reversed L order is a U order with the same dependencies inside
+            // units; otherwise U's order is grown on a second host thread.
+            long long nl = 0, nu = 0;
+            for (int i = 0; i < nb; ++i) nl += diag[i] - rp[i], nu += rp[i + 1] - diag[i] - 1;
+            bool sym = nl == nu;  // with L inside U's transpose below: L = U transposed
+            for (int i = 0; i < nb && sym; ++i)
+                for (int p = rp[i]; p < diag[i] && sym; ++p) {
+                    const int k = ci[p];
+                    sym = std::binary_search(ci.begin() + diag[k] + 1, ci.begin() + rp[k + 1], i);
+                }
+            if (sym) {
+                unit_order(nb, rp, ci, diag, false, lev[0], order[0]);
+                order[1].assign(order[0].rbegin(), order[0].rend());
+            } else {
+                std::thread tu([&] { unit_order(nb, rp, ci, diag, true, lev[1], order[1]); });
+                unit_order(nb, rp, ci, diag, false, lev[0], order[0]);
+                tu.join();
+            }
         } else if (rows) {
             for (int u = 0; u < 2; ++u) level_order(nb, lev[u], nl[u], u == 1, order[u]);
         }

@@ -160,7 +160,7 @@ def test_solver_reports_triangular_solve_timeout(gpu):
         h.close()
 
 
-def _random_wide_pattern(nb, per_row, seed):
+def _random_wide_pattern(nb, per_row, seed, symmetric=True):
     """Structurally symmetric block pattern with about `per_row` random partners per block row on each
     side of the diagonal, diagonally dominant 4x4 blocks.  Rows depend on rows all over the matrix, so
     a 64-row unit of the unit order imports more than 512 rows (up to about 1,000 here)."""
@@ -169,7 +169,8 @@ def _random_wide_pattern(nb, per_row, seed):
     for i in range(nb):
         for j in rng.choice(nb, per_row, replace=False):
             cols[i].add(int(j))
-            cols[int(j)].add(i)
+            if symmetric:
+                cols[int(j)].add(i)
     rp = np.zeros(nb + 1, np.int32)
     ci = []
     for i in range(nb):
@@ -217,6 +218,27 @@ def test_bsr_sptrsv_unit_many_imports(gpu, chunk):
         d_b, d_x = dev(b), DeviceArray((n,), np.float64)
         ref = O.sptrsv_lu(rp, ci, d_lu.to_numpy(), b)
         for _ in range(3):
+            call("isai_bsr_sptrsv_solve", plan, d_lu.ptr, d_b.ptr, d_x.ptr, None)
+            call("isai_sptrsv_status", plan, None)
+            assert rel_err(d_x.to_numpy(), ref) <= TOL
+    finally:
+        call("isai_bsr_sptrsv_destroy", plan)
+
+
+@pytest.mark.parametrize("chunk", [-1, -3])
+def test_bsr_sptrsv_unit_nonsymmetric_pattern(gpu, chunk):
+    """Unit order on a structurally non-symmetric pattern: U's order is grown by its own greedy walk
+    (a symmetric pattern reuses L's order reversed), against the oracle."""
+    print(f"bsr sptrsv random non-symmetric pattern nb=4096 chunk={chunk}", flush=True)
+    rp, ci, val = _random_wide_pattern(4096, 3, seed=13, symmetric=False)
+    d_rp, d_ci, d_lu = gpu_ilu(rp, ci, val)
+    n = (len(rp) - 1) * 4
+    b = RNG.standard_normal(n)
+    plan = _plan(rp, ci, d_rp, d_ci, d_lu, chunk)
+    try:
+        d_b, d_x = dev(b), DeviceArray((n,), np.float64)
+        ref = O.sptrsv_lu(rp, ci, d_lu.to_numpy(), b)
+        for _ in range(2):
             call("isai_bsr_sptrsv_solve", plan, d_lu.ptr, d_b.ptr, d_x.ptr, None)
             call("isai_sptrsv_status", plan, None)
             assert rel_err(d_x.to_numpy(), ref) <= TOL
