@@ -40,6 +40,16 @@ int isai_apply(int mb, int bs, isai_precision_t prec,
                const int *d_Urow, const int *d_Ucol, const void *d_Uval, int nnzbU,
                const double *d_b, double *d_y, double *d_x, isai_stream_t stream);
 
+/* The FP32 product of the mixed-precision inner solve (bs = 4, no reference
+ * counterpart): d_y = A d_x with d_val the FP32 block values (isai_convert_values),
+ * FP32 x and y and FP32 accumulation, plus fused FP64 dot products of the result:
+ * over the rows of workgroup g, d_part[g] = sum y.a and, with ndot = 2,
+ * d_part[G + g] = sum y.y.  *groups receives G = min(ceil(mb / 32), 2048); d_part
+ * holds 2 G doubles. */
+int isai_bsr_spmv_f32_dot(int mb, int nnzb, const int *d_rowptr, const int *d_colind, const float *d_val,
+                          const float *d_x, float *d_y, const float *d_a, int ndot, double *d_part, int *groups,
+                          isai_stream_t stream);
+
 /* Host-pointer benchmark mirroring bsr_SpMV_tensorcore: copies A and B in, runs
  * warmup + iters launches timed with hipEvents, copies C out (C = res_Y as in the
  * reference's two outputs) and returns the average milliseconds per launch. */

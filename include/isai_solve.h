@@ -31,7 +31,19 @@ typedef enum {
     ISAI_PRECOND_ILU_SPTRSV = 2   /* z = U^{-1} L^{-1} v, sync-free solves */
 } isai_precond_kind_t;
 
-typedef enum { ISAI_SOLVER_BICGSTAB = 0, ISAI_SOLVER_FGMRES = 1 } isai_solver_kind_t;
+/* ISAI_SOLVER_FGMRES_MIXED: the mixed-precision form.  An FP64 FGMRES(restart)
+ * outer loop (residual r = b - A x, Arnoldi products with A, Gram-Schmidt and
+ * Givens rotations in FP64) whose flexible preconditioner is an inner BiCGSTAB in
+ * FP32 storage: A rounded to FP32 (FP32 accumulation), the ISAI factors in the
+ * handle's storage precision (FP32 when that is FP64), FP32 vectors, dot products
+ * accumulated in FP64.  Each inner solve is asked for the reduction still missing
+ * (see isai_set_inner_solve); *iters counts inner iterations and maxit bounds them.
+ * Needs the ISAI preconditioner. */
+typedef enum {
+    ISAI_SOLVER_BICGSTAB = 0,
+    ISAI_SOLVER_FGMRES = 1,
+    ISAI_SOLVER_FGMRES_MIXED = 2
+} isai_solver_kind_t;
 
 typedef struct {
     int precond;      /* isai_precond_kind_t */
@@ -90,6 +102,14 @@ int isai_handle_build_plan(isai_handle_t handle, int *ncache, int *ordered_L, in
  * sync-free triangular solve of the ILU(0) preconditioner ran out of its wait budget. */
 int isai_solve(isai_handle_t handle, int solver, int restart, double tol, int maxit,
                const double *d_b, double *d_x, int *iters, double *relres, isai_stream_t stream);
+
+/* Inner solve of ISAI_SOLVER_FGMRES_MIXED: an inner solve stops when its residual
+ * has fallen by max(inner_tol, 0.5 tol ||b|| / (current outer residual)) (at most
+ * 0.5) or after inner_maxit iterations.  Defaults 1e-5 and 40. */
+int isai_set_inner_solve(isai_handle_t handle, double inner_tol, int inner_maxit);
+/* Outer FGMRES steps and inner iterations of the last ISAI_SOLVER_FGMRES_MIXED
+ * solve (0 and the iteration count for the other solvers). */
+int isai_last_solve_stats(isai_handle_t handle, int *outer_steps, int *inner_iters);
 
 int isai_destroy(isai_handle_t handle);
 

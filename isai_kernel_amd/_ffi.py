@@ -125,6 +125,8 @@ SIGNATURES: dict[str, list] = {
     "isai_handle_build_plan": [P, P, P, P],
     "isai_solve": [P, I, I, D, I, P, P, P, P, P],
     "isai_destroy": [P],
+    "isai_set_inner_solve": [P, D, I],
+    "isai_last_solve_stats": [P, P, P],
     # isai_adjoint.h
     "isai_bsr_transpose": [I, I, I, P, P, P, P, P, P, P],
     "isai_adjoint_handle": [P, P, P],
@@ -146,6 +148,7 @@ SIGNATURES: dict[str, list] = {
     "isai_bsr_spmv_prec": [I, I, I, I, P, P, P, P, P, P],
     "isai_convert_values": [SZ, I, P, P, P],
     "isai_apply": [I, I, I, P, P, P, I, P, P, P, I, P, P, P, P],
+    "isai_bsr_spmv_f32_dot": [I, I, P, P, P, P, P, P, I, P, P, P],
     "isai_bsr_spmv_bench": [P, P, P, I, I, I, P, P, I, I, P],
 }
 _RESTYPE = {"isai_last_error": C.c_char_p, "isai_status_string": C.c_char_p}

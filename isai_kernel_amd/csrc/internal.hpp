@@ -79,4 +79,40 @@ int transpose_blocks(long long nnzb, int bs, const int *d_perm, const double *sr
 int transpose_packed_lu(int nnzb, const int *d_trow_of, const int *d_tcol, const int *d_perm, const int *d_diag,
                         const double *d_lu, double *d_lut, hipStream_t st);
 
+// ---- mixed-precision inner solve (mixed.hip)
+// Workgroups of the FP32 A product (grid-stride over 32-row chunks): its dot
+// partials, mixed_dot_groups(nb) per dot, are summed by every workgroup of the
+// consuming vector kernel.
+constexpr int kMixedGroups = 2048;
+struct InnerScalars {
+    double rho_cur, rho_prev, alpha, omega, tol2, tol_in, rnorm2, pad;
+    int done, iters;  // done: 0 running, 1 converged, 3 breakdown
+    int pad2[2];
+};
+struct InnerVecs {  // FP32 vectors of n entries, and the FP64 partial sums
+    float *r, *rh, *p, *v, *s, *t, *ph, *sh, *x, *tmp;
+    double *part;   // 2 * mixed_dot_groups(nb)
+    double *rpart;  // 2 * 512
+};
+struct InnerPrecond {  // ISAI factors in BCSR with FP32 / BF16 values
+    isai_precision_t prec;
+    int nnzbL, nnzbU;
+    const int *Lrow, *Lcol, *Urow, *Ucol;
+    const void *Lval, *Uval;
+};
+int mixed_dot_groups(int nb);
+size_t inner_part_doubles(int nb);
+// y = A x with FP32 values, FP32 x and y; part[wg] = sum y.a (and part[G + wg] =
+// sum y.y for ndot = 2) over the rows of workgroup wg, in FP64.
+int mixed_spmv_f32_dot(int nb, long long nnzb, const int *rp, const int *ci, const float *val, const float *x,
+                       float *y, const float *a, int ndot, double *part, hipStream_t st, const int *skip);
+// r = rhs (rounded to FP32), rhat = r, x = p = v = 0; tolerance tol_in relative to
+// ||r||; starts done when *outer_done != 0.
+int inner_init(size_t n, const double *rhs, InnerVecs *w, InnerScalars *IS, const int *outer_done, double tol_in,
+               hipStream_t st);
+int inner_iteration(size_t n, int nb, long long nnzb, const int *rp, const int *ci, const float *A32,
+                    const InnerPrecond *M, InnerVecs *w, InnerScalars *IS, bool first, hipStream_t st);
+// z = x in FP64.
+int inner_finish(size_t n, const InnerVecs *w, double *z, hipStream_t st);
+
 }  // namespace isai

@@ -1,0 +1,343 @@
+// mixed.hip -- the inner solve of the mixed-precision form on gfx950: BiCGSTAB
+// with FP32 storage (A rounded to FP32, the handle's FP32 / BF16 ISAI factors,
+// FP32 vectors) and FP64 dot products, used as the flexible preconditioner of an
+// FP64 FGMRES outer loop.  No reference counterpart: the reference applies its
+// ISAI factors once (main.cu:817-849) and has no Krylov method.
+//
+// Scalars without single-block kernels: the A products run on a fixed grid of
+// kMixedGroups workgroups (grid-stride over 32-row chunks), so each product leaves
+// kMixedGroups partial sums per dot, and the consumer kernel of a scalar (alpha in
+// the s update, omega in the x/r update, rho and beta in the p update) has every
+// workgroup sum those partials itself, in the same fixed order, before it streams
+// its vectors.  Every workgroup therefore computes bit-identical scalars; the
+// first workgroup also records them for the later kernels.  A scalar record field
+// is only written by a kernel that does not read it.
+#include "bsr_tensor_SpMV.h"
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace isai {
+
+constexpr int kMixedBlock = 256;       // A products: 32 block rows per chunk
+constexpr int kMixedVecThreads = 1024; // vector kernels
+constexpr int kMixedVecBlocks = 512;
+
+// -------------------------------------------------- FP32 A product, bs = 4
+// Eight lanes per block row; lane s loads 16 B = row s%4 of block 2j + s/4 of the
+// batch, so one load instruction of the eight lanes reads two consecutive blocks
+// (one 128-byte line when the pair is line-aligned).  The lanes of one block (a
+// quad) each gather one FP32 component of the block's x column and exchange the
+// rest with DPP moves.  The batch's column indices are loaded once per lane group
+// and handed out by shuffle; all value loads and x gathers of a batch are issued
+// before the first product.  FP32 accumulation: lane (b, sub) sums blocks b, b+2,
+// b+4, ... of the row in order and the two lanes of a row are added at the end.
+template <int NDOT, int KB>
+__global__ __launch_bounds__(kMixedBlock) void bsr4_spmv_f32_dot_kernel(
+    int nb, const int *__restrict__ rp, const int *__restrict__ ci, const v4f *__restrict__ val4,
+    const float *__restrict__ x, float *__restrict__ y, const float *__restrict__ a, double *__restrict__ part,
+    const int *__restrict__ skip) {
+    static_assert(KB == 4 || KB == 8 || KB == 16, "KB");
+    if (skip && *skip) return;
+    constexpr int kJ = KB / 2;
+    const int s = threadIdx.x & 7;
+    const int b = s >> 2, sub = s & 3;
+    double d0 = 0.0, d1 = 0.0;
+    for (int chunk = blockIdx.x; chunk * (kMixedBlock / 8) < nb; chunk += gridDim.x) {
+        const int br = chunk * (kMixedBlock / 8) + (threadIdx.x >> 3);
+        const int rs = br < nb ? rp[br] : 0;
+        const int len = br < nb ? rp[br + 1] - rs : 0;
+        float acc = 0.0f;
+        for (int t = 0; t < len; t += KB) {
+            const int clo = s < KB && t + s < len ? ci[rs + t + s] : 0;
+            const int chi = KB > 8 && t + 8 + s < len ? ci[rs + t + 8 + s] : 0;
+            v4f w[kJ];
+            float f[kJ][4];
+#pragma unroll
+            for (int j = 0; j < kJ; ++j) {
+                const int k = t + 2 * j + b;
+                if (k < len) w[j] = val4[(size_t)(rs + k) * 4 + sub];
+            }
+#pragma unroll
+            for (int j = 0; j < kJ; ++j) {
+                const int u = 2 * j + b;
+                const int clo_u = __shfl(clo, u & 7, 8);
+                const int chi_u = KB > 8 ? __shfl(chi, u & 7, 8) : 0;
+                const int c = u < 8 ? clo_u : chi_u;
+                if (t + u < len) {
+                    const float v = x[(size_t)c * 4 + sub];
+                    f[j][0] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x00, 0xF, 0xF, false));
+                    f[j][1] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x55, 0xF, 0xF, false));
+                    f[j][2] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xAA, 0xF, 0xF, false));
+                    f[j][3] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xFF, 0xF, 0xF, false));
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kJ; ++j) {
+                if (t + 2 * j + b >= len) continue;
+                acc = fmaf(w[j].x, f[j][0], acc);
+                acc = fmaf(w[j].y, f[j][1], acc);
+                acc = fmaf(w[j].z, f[j][2], acc);
+                acc = fmaf(w[j].w, f[j][3], acc);
+            }
+        }
+        acc += __shfl_xor(acc, 4);
+        if (br < nb && b == 0) {
+            const size_t row = (size_t)br * 4 + sub;
+            y[row] = acc;
+            d0 += (double)acc * (double)a[row];
+            if (NDOT == 2) d1 += (double)acc * (double)acc;
+        }
+    }
+    __shared__ double red[2][kMixedBlock / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        d0 += __shfl_xor(d0, off);
+        if (NDOT == 2) d1 += __shfl_xor(d1, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = d0;
+        red[1][threadIdx.x >> 6] = d1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+        for (int w = 0; w < kMixedBlock / 64; ++w) {
+            t0 += red[0][w];
+            t1 += red[1][w];
+        }
+        part[blockIdx.x] = t0;
+        if (NDOT == 2) part[gridDim.x + blockIdx.x] = t1;
+    }
+}
+
+int mixed_dot_groups(int nb) {
+    const int chunks = (int)grid_for(nb, kMixedBlock / 8);
+    return chunks < kMixedGroups ? (chunks > 0 ? chunks : 1) : kMixedGroups;
+}
+
+int mixed_spmv_f32_dot(int nb, long long nnzb, const int *rp, const int *ci, const float *val, const float *x,
+                       float *y, const float *a, int ndot, double *part, hipStream_t st, const int *skip) {
+    if (nb <= 0) return ISAI_OK;
+    const unsigned g = (unsigned)mixed_dot_groups(nb);
+    const long long avg = (nnzb + nb - 1) / nb;
+    const int kb = avg <= 4 ? 4 : avg <= 8 ? 8 : 16;
+    const v4f *v = (const v4f *)val;
+#define ISAI_F32_DOT(ND, K) \
+    bsr4_spmv_f32_dot_kernel<ND, K><<<g, kMixedBlock, 0, st>>>(nb, rp, ci, v, x, y, a, part, skip)
+    if (ndot == 1) {
+        if (kb == 4) ISAI_F32_DOT(1, 4);
+        else if (kb == 8) ISAI_F32_DOT(1, 8);
+        else ISAI_F32_DOT(1, 16);
+    } else {
+        if (kb == 4) ISAI_F32_DOT(2, 4);
+        else if (kb == 8) ISAI_F32_DOT(2, 8);
+        else ISAI_F32_DOT(2, 16);
+    }
+#undef ISAI_F32_DOT
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+// ---------------------------------------------------- inner BiCGSTAB kernels
+template <int T>
+__device__ __forceinline__ double mixed_block_sum(double v) {
+    __shared__ double ws[T / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < T / 64; ++k) t += ws[k];
+    return t;
+}
+
+// Sum of part[0..G) in a fixed order (thread-strided, then the block tree): every
+// workgroup that calls it gets the same bits.
+__device__ __forceinline__ double partial_sum(const double *__restrict__ part, int G) {
+    double v = 0.0;
+    for (int k = threadIdx.x; k < G; k += kMixedVecThreads) v += part[k];
+    return mixed_block_sum<kMixedVecThreads>(v);
+}
+
+// r = rhs (FP64 input rounded to FP32), rhat = r, x = p = v = 0; rpart[0] = ||r||^2,
+// rpart[1] = rhat.r (= ||r||^2).  The solve starts done when the outer loop is done.
+__global__ __launch_bounds__(kMixedVecThreads) void inner_init_kernel(
+    size_t n, const double *__restrict__ rhs, float *__restrict__ r, float *__restrict__ rh, float *__restrict__ x,
+    float *__restrict__ p, float *__restrict__ v, double *__restrict__ rpart, InnerScalars *IS,
+    const int *__restrict__ outer_done, double tol_in) {
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kMixedVecThreads + threadIdx.x; i < n;
+         i += (size_t)kMixedVecBlocks * kMixedVecThreads) {
+        const float ri = (float)rhs[i];
+        r[i] = ri;
+        rh[i] = ri;
+        x[i] = 0.0f;
+        p[i] = 0.0f;
+        v[i] = 0.0f;
+        acc += (double)ri * (double)ri;
+    }
+    acc = mixed_block_sum<kMixedVecThreads>(acc);
+    if (threadIdx.x == 0) {
+        rpart[blockIdx.x] = acc;
+        rpart[kMixedVecBlocks + blockIdx.x] = acc;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        IS->done = (outer_done && *outer_done) ? 1 : 0;
+        IS->iters = 0;
+        IS->tol_in = tol_in;
+    }
+}
+
+// Start of an iteration: rho = rhat.r and ||r||^2 from the previous kernel's
+// partials; convergence test ||r|| <= tol_in ||r_0||; then p = r (first
+// iteration) or p = r + beta (p - omega v), beta = (rho / rho_prev)(alpha / omega).
+// `first` comes from the host (it knows which iteration it launches): a flag in the
+// record would be written by this kernel's first workgroup while others read it.
+__global__ __launch_bounds__(kMixedVecThreads) void inner_p_kernel(size_t n, const float *__restrict__ r,
+                                                                   float *__restrict__ p, const float *__restrict__ v,
+                                                                   const double *__restrict__ rpart, InnerScalars *IS,
+                                                                   int first) {
+    if (IS->done) return;
+    const double r2 = partial_sum(rpart, kMixedVecBlocks);
+    const double rho = partial_sum(rpart + kMixedVecBlocks, kMixedVecBlocks);
+    const double tol2 = first ? IS->tol_in * IS->tol_in * r2 : IS->tol2;
+    int stop = 0;
+    if (!first && r2 <= tol2) stop = 1;
+    else if (rho == 0.0) stop = 3;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (first) IS->tol2 = tol2;
+        IS->rnorm2 = r2;
+        if (stop) IS->done = stop;
+        else {
+            IS->rho_cur = rho;
+            IS->iters++;  // only this thread reads or writes the count
+        }
+    }
+    if (stop) return;
+    const float beta = first ? 0.0f : (float)((rho / IS->rho_prev) * (IS->alpha / IS->omega));
+    const float omega = first ? 0.0f : (float)IS->omega;
+    for (size_t i = (size_t)blockIdx.x * kMixedVecThreads + threadIdx.x; i < n;
+         i += (size_t)kMixedVecBlocks * kMixedVecThreads)
+        p[i] = first ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
+}
+
+// alpha = rho / rhat.v (partials of the v = A p^ product); s = r - alpha v.
+__global__ __launch_bounds__(kMixedVecThreads) void inner_s_kernel(size_t n, const float *__restrict__ r,
+                                                                   const float *__restrict__ v, float *__restrict__ s,
+                                                                   const double *__restrict__ part, int G,
+                                                                   InnerScalars *IS) {
+    if (IS->done) return;
+    const double d = partial_sum(part, G);
+    const double rho = IS->rho_cur;
+    if (d == 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) IS->done = 3;
+        return;
+    }
+    const double alpha = rho / d;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        IS->alpha = alpha;
+        IS->rho_prev = rho;
+    }
+    const float af = (float)alpha;
+    for (size_t i = (size_t)blockIdx.x * kMixedVecThreads + threadIdx.x; i < n;
+         i += (size_t)kMixedVecBlocks * kMixedVecThreads)
+        s[i] = r[i] - af * v[i];
+}
+
+// omega = t.s / t.t (partials of the t = A s^ product); x += alpha p^ + omega s^;
+// r = s - omega t; rpart[0] = ||r||^2, rpart[1] = rhat.r.
+__global__ __launch_bounds__(kMixedVecThreads) void inner_xr_kernel(
+    size_t n, float *__restrict__ x, float *__restrict__ r, const float *__restrict__ ph, const float *__restrict__ sh,
+    const float *__restrict__ s, const float *__restrict__ t, const float *__restrict__ rh,
+    const double *__restrict__ part, int G, double *__restrict__ rpart, InnerScalars *IS) {
+    if (IS->done) return;
+    const double ts = partial_sum(part, G);
+    const double tt = partial_sum(part + G, G);
+    if (tt == 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) IS->done = 3;
+        return;
+    }
+    const double omega = ts / tt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) IS->omega = omega;
+    const float af = (float)IS->alpha, of = (float)omega;
+    double acc = 0.0, accr = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kMixedVecThreads + threadIdx.x; i < n;
+         i += (size_t)kMixedVecBlocks * kMixedVecThreads) {
+        x[i] = x[i] + (af * ph[i] + of * sh[i]);
+        const float ri = s[i] - of * t[i];
+        r[i] = ri;
+        acc += (double)ri * (double)ri;
+        accr += (double)rh[i] * (double)ri;
+    }
+    acc = mixed_block_sum<kMixedVecThreads>(acc);
+    accr = mixed_block_sum<kMixedVecThreads>(accr);
+    if (threadIdx.x == 0) {
+        rpart[blockIdx.x] = acc;
+        rpart[kMixedVecBlocks + blockIdx.x] = accr;
+    }
+}
+
+// z = x widened to FP64 (the flexible preconditioner's output).
+__global__ void inner_finish_kernel(size_t n, const float *__restrict__ x, double *__restrict__ z) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        z[i] = (double)x[i];
+}
+
+int inner_init(size_t n, const double *rhs, InnerVecs *w, InnerScalars *IS, const int *outer_done, double tol_in,
+               hipStream_t st) {
+    inner_init_kernel<<<kMixedVecBlocks, kMixedVecThreads, 0, st>>>(n, rhs, w->r, w->rh, w->x, w->p, w->v, w->rpart,
+                                                                    IS, outer_done, tol_in);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+// One iteration: p update, p^ = M p, v = A p^, s update, s^ = M s, t = A s^, x/r update.
+int inner_iteration(size_t n, int nb, long long nnzb, const int *rp, const int *ci, const float *A32,
+                    const InnerPrecond *M, InnerVecs *w, InnerScalars *IS, bool first, hipStream_t st) {
+    const int *skip = &IS->done;
+    const int G = mixed_dot_groups(nb);
+    inner_p_kernel<<<kMixedVecBlocks, kMixedVecThreads, 0, st>>>(n, w->r, w->p, w->v, w->rpart, IS, first ? 1 : 0);
+    ISAI_LAUNCH_CHECK();
+    int s = spmv_lowp_launch(nb, M->nnzbL, M->prec, M->Lrow, M->Lcol, M->Lval, w->p, true, w->tmp, true, st, skip);
+    if (s == ISAI_OK)
+        s = spmv_lowp_launch(nb, M->nnzbU, M->prec, M->Urow, M->Ucol, M->Uval, w->tmp, true, w->ph, true, st, skip);
+    if (s == ISAI_OK) s = mixed_spmv_f32_dot(nb, nnzb, rp, ci, A32, w->ph, w->v, w->rh, 1, w->part, st, skip);
+    if (s != ISAI_OK) return s;
+    inner_s_kernel<<<kMixedVecBlocks, kMixedVecThreads, 0, st>>>(n, w->r, w->v, w->s, w->part, G, IS);
+    ISAI_LAUNCH_CHECK();
+    s = spmv_lowp_launch(nb, M->nnzbL, M->prec, M->Lrow, M->Lcol, M->Lval, w->s, true, w->tmp, true, st, skip);
+    if (s == ISAI_OK)
+        s = spmv_lowp_launch(nb, M->nnzbU, M->prec, M->Urow, M->Ucol, M->Uval, w->tmp, true, w->sh, true, st, skip);
+    if (s == ISAI_OK) s = mixed_spmv_f32_dot(nb, nnzb, rp, ci, A32, w->sh, w->t, w->s, 2, w->part, st, skip);
+    if (s != ISAI_OK) return s;
+    inner_xr_kernel<<<kMixedVecBlocks, kMixedVecThreads, 0, st>>>(n, w->x, w->r, w->ph, w->sh, w->s, w->t, w->rh,
+                                                                  w->part, G, w->rpart, IS);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+int inner_finish(size_t n, const InnerVecs *w, double *z, hipStream_t st) {
+    size_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    inner_finish_kernel<<<(unsigned)g, 256, 0, st>>>(n, w->x, z);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
+size_t inner_part_doubles(int nb) { return 2 * (size_t)mixed_dot_groups(nb) + 2 * (size_t)kMixedVecBlocks; }
+
+}  // namespace isai
+
+extern "C" int isai_bsr_spmv_f32_dot(int mb, int nnzb, const int *d_rowptr, const int *d_colind, const float *d_val,
+                                     const float *d_x, float *d_y, const float *d_a, int ndot, double *d_part,
+                                     int *groups, isai_stream_t stream) {
+    ISAI_REQUIRE(mb >= 0 && nnzb >= 0 && (ndot == 1 || ndot == 2), "isai_bsr_spmv_f32_dot: bad size or ndot");
+    ISAI_REQUIRE(mb == 0 || (d_rowptr && d_colind && d_val && d_x && d_y && d_a && d_part),
+                 "isai_bsr_spmv_f32_dot: null pointer");
+    if (groups) *groups = isai::mixed_dot_groups(mb);
+    return isai::mixed_spmv_f32_dot(mb, nnzb, d_rowptr, d_colind, d_val, d_x, d_y, d_a, ndot, d_part,
+                                    (hipStream_t)stream, nullptr);
+}

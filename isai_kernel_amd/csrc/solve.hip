@@ -580,6 +580,18 @@ struct isai_handle {
     // matrix arrays owned by the handle (adjoint handles own A^T)
     int *own_row = nullptr, *own_col = nullptr;
     double *own_val = nullptr;
+    // mixed-precision form: A rounded to FP32, the inner solve's FP32 vectors and
+    // FP64 partials, its scalar record, and the inner-solve parameters
+    float *d_A32 = nullptr;
+    float *inner_vecs = nullptr;
+    double *inner_part = nullptr;
+    InnerScalars *IS = nullptr;
+    double *h_scal = nullptr;      // pinned: outer residual estimate / norms read back
+    int *h_inner = nullptr;        // pinned: inner done, iters
+    double inner_tol = 1e-5;       // smallest relative reduction asked of one inner solve
+    int inner_maxit = 40;          // inner iterations per outer step
+    double inner_rate = 1.6;       // inner iterations per decade of reduction seen so far
+    int last_outer = 0, last_inner = 0;
 };
 
 namespace {
@@ -806,6 +818,156 @@ int fgmres(isai_handle *h, int m, double tol, int maxit, const double *b, double
     return ISAI_ERR_NOT_CONVERGED;
 }
 
+// ------------------------------------------------- mixed-precision form
+// Buffers of the inner solve: A in FP32 (converted once per handle), ten FP32
+// vectors, the partial sums and the scalar record.
+int ensure_inner(isai_handle *h, hipStream_t st) {
+    if (h->IS) return ISAI_OK;
+    const size_t n = h->n;
+    const size_t b2 = 16;
+    if (hipMalloc(&h->d_A32, sizeof(float) * b2 * h->nnzb) != hipSuccess ||
+        hipMalloc(&h->inner_vecs, sizeof(float) * 10 * n) != hipSuccess ||
+        hipMalloc(&h->inner_part, sizeof(double) * inner_part_doubles(h->nb)) != hipSuccess ||
+        hipMalloc(&h->IS, sizeof(InnerScalars)) != hipSuccess ||
+        hipHostMalloc(&h->h_scal, sizeof(double) * 4) != hipSuccess ||
+        hipHostMalloc(&h->h_inner, sizeof(int) * 2) != hipSuccess) {
+        (void)hipFree(h->d_A32);
+        (void)hipFree(h->inner_vecs);
+        (void)hipFree(h->inner_part);
+        (void)hipFree(h->IS);
+        if (h->h_scal) (void)hipHostFree(h->h_scal);
+        if (h->h_inner) (void)hipHostFree(h->h_inner);
+        h->d_A32 = nullptr, h->inner_vecs = nullptr, h->inner_part = nullptr, h->IS = nullptr;
+        h->h_scal = nullptr, h->h_inner = nullptr;
+        set_error("isai_solve: cannot allocate the mixed-precision inner solve");
+        return ISAI_ERR_ALLOC;
+    }
+    return isai_convert_values(b2 * h->nnzb, ISAI_PREC_FP32, h->d_val, h->d_A32, (isai_stream_t)st);
+}
+
+InnerVecs inner_vecs(isai_handle *h) {
+    const size_t n = h->n;
+    float *b = h->inner_vecs;
+    InnerVecs w;
+    w.r = b, w.rh = b + n, w.p = b + 2 * n, w.v = b + 3 * n, w.s = b + 4 * n, w.t = b + 5 * n;
+    w.ph = b + 6 * n, w.sh = b + 7 * n, w.x = b + 8 * n, w.tmp = b + 9 * n;
+    w.part = h->inner_part;
+    w.rpart = h->inner_part + 2 * (size_t)mixed_dot_groups(h->nb);
+    return w;
+}
+
+// z = (inner BiCGSTAB on A z = v, to tol_in relative to ||v||, at most budget
+// iterations).  The host launches a predicted number of iterations (from the rate
+// of the previous inner solves), then tops up two at a time, reading the inner
+// state after each batch; iterations launched after convergence return at once.
+int inner_solve(isai_handle *h, const InnerPrecond *M, const double *v, double *z, double tol_in, int budget,
+                int *count, hipStream_t st) {
+    InnerVecs w = inner_vecs(h);
+    TRY(inner_init(h->n, v, &w, h->IS, &h->S->done, tol_in, st));
+    const double decades = -std::log10(tol_in);
+    int batch = std::max(1, std::min(budget, (int)(h->inner_rate * decades)));
+    int launched = 0;
+    while (true) {
+        for (int k = 0; k < batch; ++k, ++launched)
+            TRY(inner_iteration(h->n, h->nb, h->nnzb, h->d_row, h->d_col, h->d_A32, M, &w, h->IS, launched == 0,
+                                st));
+        ISAI_HIP(hipMemcpyAsync(h->h_inner, &h->IS->done, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+        ISAI_HIP(hipStreamSynchronize(st));
+        if (h->h_inner[0] || launched >= budget) break;
+        batch = std::min(2, budget - launched);
+    }
+    TRY(inner_finish(h->n, &w, z, st));
+    *count = h->h_inner[1];
+    if (h->h_inner[0] == 1 && *count > 0) h->inner_rate = 0.5 * h->inner_rate + 0.5 * (*count / decades);
+    return ISAI_OK;
+}
+
+// FP64 FGMRES(m) whose flexible preconditioner is the inner mixed-precision solve:
+// z_j = inner(v_j).  The residual r = b - A x, the Arnoldi products w = A z_j, the
+// Gram-Schmidt orthogonalisation and the Givens rotations are FP64.  The inner
+// tolerance for step j asks for the reduction still missing, 0.5 tol ||b|| /
+// |g_j| (|g_j| = the outer residual estimate), between inner_tol and 0.5.
+// *iters = inner BiCGSTAB iterations, bounded by maxit.
+int fgmres_mixed(isai_handle *h, int m, double tol, int maxit, const double *b, double *x, int *iters,
+                 double *relres, hipStream_t st) {
+    if (m <= 0) m = 10;
+    if (m > kMaxRestart) {
+        set_error("fgmres: restart %d above %d", m, kMaxRestart);
+        return ISAI_ERR_INVALID_ARG;
+    }
+    if (h->precond != ISAI_PRECOND_ISAI) {
+        set_error("mixed-precision solve: the inner solve needs the ISAI preconditioner");
+        return ISAI_ERR_UNSUPPORTED;
+    }
+    // the inner ISAI apply runs in the handle's storage precision (FP32 for FP64)
+    const int lp = h->precision == ISAI_PREC_FP64 ? ISAI_PREC_FP32 : h->precision;
+    TRY(convert_lp(h, lp, st));
+    TRY(ensure_work(h, (size_t)2 * m + 3));
+    TRY(ensure_inner(h, st));
+    const InnerPrecond M{(isai_precision_t)lp, h->nnzbL, h->nnzbU, h->d_Lrow, h->d_Lcol, h->d_Urow, h->d_Ucol,
+                         h->d_Llp, h->d_Ulp};
+    const size_t n = h->n;
+    double *V = h->work, *Z = V + n * (m + 1), *w = Z + n * m;
+    const int *skip = &h->S->done;
+    int inner_total = 0, outer = 0;
+    *relres = INFINITY;
+    while (inner_total < maxit) {
+        TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, x, w, st, nullptr));
+        residual_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, b, w, h->S);
+        fg_beta_final<<<1, kRedThreads, 0, st>>>(h->S, tol);
+        fg_scale_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V, h->S, 0, 0, 1);
+        ISAI_LAUNCH_CHECK();
+        ISAI_HIP(hipMemcpyAsync(h->h_scal, &h->S->beta, sizeof(double), hipMemcpyDeviceToHost, st));
+        ISAI_HIP(hipMemcpyAsync(h->h_scal + 1, &h->S->tol2, sizeof(double), hipMemcpyDeviceToHost, st));
+        int done = 0;
+        TRY(read_done(h, st, &done));
+        if (done) break;
+        const double target = std::sqrt(h->h_scal[1]);  // tol * ||b||
+        double gj = h->h_scal[0];
+        int jend = 0;
+        for (int j = 0; j < m && inner_total < maxit; ++j) {
+            double *vj = V + n * j, *zj = Z + n * j;
+            double tol_in = gj > 0.0 ? 0.5 * target / gj : 0.5;
+            tol_in = std::min(0.5, std::max(h->inner_tol, tol_in));
+            int k = 0;
+            TRY(inner_solve(h, &M, vj, zj, tol_in, std::min(h->inner_maxit, maxit - inner_total), &k, st));
+            inner_total += k;
+            ++outer;
+            TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, zj, w, st, skip));
+            const int nv = j + 1;
+            const dim3 dgrid(kRedBlocks, (nv + kDotChunk - 1) / kDotChunk);
+            for (int pass = 0; pass < 2; ++pass) {
+                fg_mdot_kernel<<<dgrid, kMdotThreads, 0, st>>>(n, w, V, nv, h->S, pass);
+                fg_hsum_final<<<nv, kRedThreads, 0, st>>>(h->S, j, pass);
+                fg_axpy_multi_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, V, nv, h->S, pass);
+                if (pass == 0) fg_reorth_check<<<1, kRedThreads, 0, st>>>(h->S);
+            }
+            fg_givens_final<<<1, kRedThreads, 0, st>>>(h->S, j);
+            fg_scale_next_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V + n * (j + 1), h->S);
+            ISAI_LAUNCH_CHECK();
+            ISAI_HIP(hipMemcpyAsync(h->h_scal, &h->S->g[j + 1], sizeof(double), hipMemcpyDeviceToHost, st));
+            TRY(read_done(h, st, &done));
+            gj = std::fabs(h->h_scal[0]);
+            jend = j + 1;
+            if (done) break;
+        }
+        if (jend == 0) break;
+        fg_solve_y_kernel<<<1, 64, 0, st>>>(h->S);
+        fg_update_x_kernel<<<vec_grid(n), 256, 0, st>>>(n, x, Z, h->S);
+        ISAI_LAUNCH_CHECK();
+        TRY(true_residual(h, b, x, w, relres, st));
+        if (*relres <= tol) break;
+    }
+    *iters = inner_total;
+    h->last_outer = outer;
+    h->last_inner = inner_total;
+    if (*relres <= tol) return ISAI_OK;
+    if (!std::isfinite(*relres)) TRY(true_residual(h, b, x, w, relres, st));
+    if (*relres <= tol) return ISAI_OK;
+    set_error("mixed-precision fgmres: not converged after %d inner iterations (relres %.3e)", inner_total, *relres);
+    return ISAI_ERR_NOT_CONVERGED;
+}
+
 }  // namespace
 
 namespace isai {
@@ -851,6 +1013,12 @@ int isai_destroy(isai_handle_t h) {
     (void)hipFree(h->own_row);
     (void)hipFree(h->own_col);
     (void)hipFree(h->own_val);
+    (void)hipFree(h->d_A32);
+    (void)hipFree(h->inner_vecs);
+    (void)hipFree(h->inner_part);
+    (void)hipFree(h->IS);
+    if (h->h_scal) (void)hipHostFree(h->h_scal);
+    if (h->h_inner) (void)hipHostFree(h->h_inner);
     delete h;
     return ISAI_OK;
 }
@@ -1302,13 +1470,35 @@ int isai_solve(isai_handle_t h, int solver, int restart, double tol, int maxit, 
         s = bicgstab(h, tol, maxit, d_b, d_x, iters, relres, st);
     } else if (solver == ISAI_SOLVER_FGMRES) {
         s = fgmres(h, restart, tol, maxit, d_b, d_x, iters, relres, st);
+    } else if (solver == ISAI_SOLVER_FGMRES_MIXED) {
+        s = fgmres_mixed(h, restart, tol, maxit, d_b, d_x, iters, relres, st);
     } else {
         set_error("isai_solve: unknown solver %d", solver);
         return ISAI_ERR_INVALID_ARG;
     }
+    if (solver != ISAI_SOLVER_FGMRES_MIXED) {
+        h->last_outer = 0;
+        h->last_inner = *iters;
+    }
     // a triangular-solve wait that gave up makes the iterates meaningless
     const int t = isai_handle_status(h, stream);
     return t != ISAI_OK ? t : s;
+}
+
+int isai_set_inner_solve(isai_handle_t h, double inner_tol, int inner_maxit) {
+    ISAI_REQUIRE(h, "isai_set_inner_solve: null handle");
+    ISAI_REQUIRE(inner_tol > 0.0 && inner_tol < 1.0 && inner_maxit > 0,
+                 "isai_set_inner_solve: bad inner_tol %g / inner_maxit %d", inner_tol, inner_maxit);
+    h->inner_tol = inner_tol;
+    h->inner_maxit = inner_maxit;
+    return ISAI_OK;
+}
+
+int isai_last_solve_stats(isai_handle_t h, int *outer_steps, int *inner_iters) {
+    ISAI_REQUIRE(h, "isai_last_solve_stats: null handle");
+    if (outer_steps) *outer_steps = h->last_outer;
+    if (inner_iters) *inner_iters = h->last_inner;
+    return ISAI_OK;
 }
 
 int isai_handle_status(isai_handle_t h, isai_stream_t stream) {

@@ -9,7 +9,7 @@ from ._ffi import DeviceArray, SetupOpts, call, lib
 
 PRECOND = {"none": 0, "isai": 1, "ilu": 2}
 PRECISION = {"fp64": 0, "fp32": 1, "bf16": 2}
-SOLVER = {"bicgstab": 0, "fgmres": 1}
+SOLVER = {"bicgstab": 0, "fgmres": 1, "fgmres_mixed": 2}
 
 
 class Matrix:
@@ -75,6 +75,16 @@ class Handle:
             from ._ffi import check as _check
             _check(st, "isai_solve")
         return st, it.value, rr.value
+
+    def set_inner(self, inner_tol: float = 1e-5, inner_maxit: int = 40) -> None:
+        """Inner-solve parameters of the mixed-precision solver (isai_set_inner_solve)."""
+        call("isai_set_inner_solve", self.h, inner_tol, inner_maxit)
+
+    def last_stats(self):
+        """(outer FGMRES steps, inner iterations) of the last solve."""
+        o, i = C.c_int(), C.c_int()
+        call("isai_last_solve_stats", self.h, C.byref(o), C.byref(i))
+        return o.value, i.value
 
     def close(self) -> None:
         if getattr(self, "h", None):

@@ -28,6 +28,10 @@
  *   ora_bsr_sptrsv_lu    L (U x) = b residual with dense factors.
  *   ora_csr_sptrsv_lu    equals ora_bsr_sptrsv_lu through ora_bsr2csr.
  *   ora_bicgstab/fgmres  true residual <= tol; equality with a direct dense solve.
+ *   ora_fgmres_mixed     equality with a direct dense solve; the FP32 floor of one
+ *                        inner solve (a single inner solve asked for 1e-12 stops
+ *                        between 1e-9 and 1e-4 true residual: an FP64 inner would
+ *                        pass 1e-9); at most 3 outer steps with a 1e-5 inner tolerance.
  *   ora_bsr_transpose    dense transpose equality (scipy-free, brute force).
  *   ora_transpose_packed_lu  dense identity L'U' = (L U)^T with L' unit lower and
  *                        U' upper; equality with ora_bsrilu0 of A^T on patterns
@@ -867,6 +871,145 @@ int ora_fgmres(int nb, int bs, const int *rp, const int *ci, const double *val,
     *iters = it;
     *relres = ora_rel_residual_bsr(nb, bs, rp, ci, val, b, x);
     free(V); free(Z); free(w); free(H); free(cs); free(sn); free(g); free(yv);
+    return conv ? ORA_OK : ORA_ERR_NOT_CONVERGED;
+}
+
+/* ---------------------------------------------------- mixed-precision form */
+static double dot32(size_t n, const float *a, const float *b) {
+    double s = 0.0;
+    for (size_t i = 0; i < n; ++i) s += (double)a[i] * (double)b[i];
+    return s;
+}
+
+/* z = inner BiCGSTAB(A z = rhs) in FP32 storage; returns the iterations taken. */
+static int inner_bicgstab32(int nb, int bs, const int *rp, const int *ci, const double *val,
+                            const ora_precond_t *P, const double *rhs, double *z, double tol_in,
+                            int maxit_in, float *f, double *d) {
+    size_t n = (size_t)nb * bs;
+    float *r = f, *rh = f + n, *p = f + 2 * n, *v = f + 3 * n, *s = f + 4 * n, *t = f + 5 * n;
+    float *ph = f + 6 * n, *sh = f + 7 * n, *xx = f + 8 * n;
+    double *d0 = d, *d1 = d + n;
+    for (size_t i = 0; i < n; ++i) {
+        r[i] = (float)rhs[i];
+        rh[i] = r[i];
+        p[i] = v[i] = xx[i] = 0.0f;
+    }
+    double tol2 = tol_in * tol_in * dot32(n, r, r);
+    double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+    int it = 0;
+    while (it < maxit_in) {
+        double rho = dot32(n, rh, r), r2 = dot32(n, r, r);
+        if (it > 0 && r2 <= tol2) break;
+        if (rho == 0.0) break;
+        if (it == 0) memcpy(p, r, sizeof(float) * n);
+        else {
+            float beta = (float)((rho / rho_prev) * (alpha / omega)), om = (float)omega;
+            for (size_t i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - om * v[i]);
+        }
+        ++it;
+        for (size_t i = 0; i < n; ++i) d0[i] = p[i];
+        ora_precond_apply(P, d0, d1);
+        for (size_t i = 0; i < n; ++i) ph[i] = (float)d1[i];
+        for (size_t i = 0; i < n; ++i) d0[i] = ph[i];
+        ora_bsr_spmv_f32acc(nb, bs, rp, ci, val, 0, d0, d1);
+        for (size_t i = 0; i < n; ++i) v[i] = (float)d1[i];
+        double den = dot32(n, rh, v);
+        if (den == 0.0) break;
+        alpha = rho / den;
+        float af = (float)alpha;
+        for (size_t i = 0; i < n; ++i) s[i] = r[i] - af * v[i];
+        for (size_t i = 0; i < n; ++i) d0[i] = s[i];
+        ora_precond_apply(P, d0, d1);
+        for (size_t i = 0; i < n; ++i) sh[i] = (float)d1[i];
+        for (size_t i = 0; i < n; ++i) d0[i] = sh[i];
+        ora_bsr_spmv_f32acc(nb, bs, rp, ci, val, 0, d0, d1);
+        for (size_t i = 0; i < n; ++i) t[i] = (float)d1[i];
+        double tt = dot32(n, t, t);
+        if (tt == 0.0) break;
+        omega = dot32(n, t, s) / tt;
+        float of = (float)omega;
+        for (size_t i = 0; i < n; ++i) {
+            xx[i] = xx[i] + (af * ph[i] + of * sh[i]);
+            r[i] = s[i] - of * t[i];
+        }
+        rho_prev = rho;
+    }
+    for (size_t i = 0; i < n; ++i) z[i] = xx[i];
+    return it;
+}
+
+int ora_fgmres_mixed(int nb, int bs, const int *rp, const int *ci, const double *val,
+                     const ora_precond_t *P, const double *b, double *x, int m, double tol,
+                     int maxit, double inner_tol, int inner_maxit, int *iters, int *outer,
+                     double *relres) {
+    size_t n = (size_t)nb * bs;
+    double *V = malloc(sizeof(double) * n * (size_t)(m + 1));
+    double *Z = malloc(sizeof(double) * n * (size_t)m);
+    double *w = malloc(sizeof(double) * n);
+    double *H = calloc((size_t)(m + 1) * m, sizeof(double));
+    double *cs = malloc(sizeof(double) * m), *sn = malloc(sizeof(double) * m);
+    double *g = malloc(sizeof(double) * (m + 1)), *yv = malloc(sizeof(double) * m);
+    float *fw = malloc(sizeof(float) * 9 * n);
+    double *dw = malloc(sizeof(double) * 2 * n);
+    double bnorm = sqrt(dot(n, b, b));
+    if (bnorm == 0.0) bnorm = 1.0;
+    int it = 0, steps = 0, conv = 0;
+    while (it < maxit && !conv) {
+        ora_bsr_spmv(nb, bs, rp, ci, val, x, w);
+        for (size_t i = 0; i < n; ++i) w[i] = b[i] - w[i];
+        double beta = sqrt(dot(n, w, w));
+        if (beta / bnorm <= tol) { conv = 1; break; }
+        for (size_t i = 0; i < n; ++i) V[i] = w[i] / beta;
+        for (int k = 0; k <= m; ++k) g[k] = 0.0;
+        g[0] = beta;
+        int jend = 0;
+        for (int j = 0; j < m && it < maxit; ++j) {
+            double gj = fabs(g[j]);
+            double tol_in = gj > 0.0 ? 0.5 * tol * bnorm / gj : 0.5;
+            if (tol_in < inner_tol) tol_in = inner_tol;
+            if (tol_in > 0.5) tol_in = 0.5;
+            int budget = inner_maxit < maxit - it ? inner_maxit : maxit - it;
+            it += inner_bicgstab32(nb, bs, rp, ci, val, P, V + (size_t)j * n, Z + (size_t)j * n, tol_in,
+                                   budget, fw, dw);
+            ++steps;
+            ora_bsr_spmv(nb, bs, rp, ci, val, Z + (size_t)j * n, w);
+            for (int i = 0; i <= j; ++i) {
+                double h = dot(n, w, V + (size_t)i * n);
+                H[i * m + j] = h;
+                for (size_t q = 0; q < n; ++q) w[q] -= h * V[(size_t)i * n + q];
+            }
+            double hn = sqrt(dot(n, w, w));
+            H[(j + 1) * m + j] = hn;
+            if (hn != 0.0) for (size_t q = 0; q < n; ++q) V[(size_t)(j + 1) * n + q] = w[q] / hn;
+            for (int i = 0; i < j; ++i) {
+                double a = H[i * m + j], c = H[(i + 1) * m + j];
+                H[i * m + j] = cs[i] * a + sn[i] * c;
+                H[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+            }
+            double a = H[j * m + j], c = H[(j + 1) * m + j];
+            double den = sqrt(a * a + c * c);
+            cs[j] = den == 0.0 ? 1.0 : a / den;
+            sn[j] = den == 0.0 ? 0.0 : c / den;
+            H[j * m + j] = cs[j] * a + sn[j] * c;
+            H[(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            jend = j + 1;
+            if (fabs(g[j + 1]) / bnorm <= tol || hn == 0.0) break;
+        }
+        for (int i = jend - 1; i >= 0; --i) {
+            double sacc = g[i];
+            for (int k = i + 1; k < jend; ++k) sacc -= H[i * m + k] * yv[k];
+            yv[i] = sacc / H[i * m + i];
+        }
+        for (int k = 0; k < jend; ++k)
+            for (size_t q = 0; q < n; ++q) x[q] += yv[k] * Z[(size_t)k * n + q];
+        if (ora_rel_residual_bsr(nb, bs, rp, ci, val, b, x) <= tol) conv = 1;
+    }
+    *iters = it;
+    *outer = steps;
+    *relres = ora_rel_residual_bsr(nb, bs, rp, ci, val, b, x);
+    free(V); free(Z); free(w); free(H); free(cs); free(sn); free(g); free(yv); free(fw); free(dw);
     return conv ? ORA_OK : ORA_ERR_NOT_CONVERGED;
 }
 

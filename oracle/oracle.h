@@ -177,6 +177,27 @@ int ora_fgmres(int nb, int bs, const int *rp, const int *ci, const double *val,
                const ora_precond_t *P, const double *b, double *x, int m,
                double tol, int maxit, int *iters, double *relres);
 
+/* Mixed-precision form (no reference counterpart): FP64 FGMRES(m) (modified
+ * Gram-Schmidt, Saad Alg. 9.6) whose flexible preconditioner is an inner
+ * right-preconditioned BiCGSTAB in FP32 storage:
+ *   - A's values rounded to FP32 (RNE), products accumulated in FP32
+ *     (ora_bsr_spmv_f32acc);
+ *   - the ISAI apply of P (give P reduced-precision flags for FP32 / BF16 factors);
+ *   - every inner vector rounded to FP32 after each update; the scalars alpha,
+ *     beta, omega rounded to FP32 where they scale a vector;
+ *   - dot products of the FP32 vectors accumulated in FP64.
+ * Inner solve j starts from x = 0 on rhs = v_j and stops when ||r_k|| <= tol_in
+ * ||r_0|| (tested from the second iteration on), rho = 0, or inner_maxit
+ * iterations, with tol_in = min(0.5, max(inner_tol, 0.5 tol ||b|| / |g_j|)) and
+ * |g_j| the outer residual estimate before step j.  Outer steps stop when |g_{j+1}|
+ * <= tol ||b||; a cycle ends with the FP64 true residual.  *iters = inner
+ * iterations (bounded by maxit), *outer = outer steps, *relres = the true FP64
+ * relative residual. */
+int ora_fgmres_mixed(int nb, int bs, const int *rp, const int *ci, const double *val,
+                     const ora_precond_t *P, const double *b, double *x, int m, double tol,
+                     int maxit, double inner_tol, int inner_maxit, int *iters, int *outer,
+                     double *relres);
+
 /* Apply the preconditioner once: z = P v. */
 void ora_precond_apply(const ora_precond_t *P, const double *v, double *z);
 

@@ -1,0 +1,101 @@
+"""GPU parity of the mixed-precision form (ISAI_SOLVER_FGMRES_MIXED): an FP64 FGMRES
+outer loop whose flexible preconditioner is an inner BiCGSTAB in FP32 storage, against
+the oracle's restatement of the same inner-outer scheme (ora_fgmres_mixed) and an
+independent FP64 residual."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from isai_kernel_amd import DeviceArray, call, problems
+from isai_kernel_amd.solver import Handle, Matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_precond(rp, ci, val, prec):
+    _, L, U = O.isai_factors(rp, ci, val)
+    return O.make_precond(1, len(rp) - 1, 4, L=L, U=U, f32=prec == "fp32", bf16=prec == "bf16")
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp32", "fp64"])
+@pytest.mark.parametrize("problem,L", [(1, 24), (2, 12), (0, 64)])
+def test_mixed_solve_matches_oracle_scheme(gpu, prec, problem, L):
+    print(f"mixed fgmres problem={problem} L={L} isai={prec}", flush=True)
+    rp, ci, val = problems.generate(problem, L, seed=2)
+    A = Matrix(rp, ci, val)
+    b = problems.normal(7, A.n)
+    h = Handle(A, precond="isai", precision=prec)
+    d_b, d_x = DeviceArray.from_numpy(b), DeviceArray.zeros((A.n,), np.float64)
+    st, it, rr = h.solve(d_b, d_x, "fgmres_mixed", tol=1e-10, maxit=400, restart=10)
+    outer, inner = h.last_stats()
+    x = d_x.to_numpy()
+    assert st == 0 and rr <= 1e-10 and inner == it
+    rr_ind = O.rel_residual(rp, ci, val, b, x)  # independent FP64 residual
+    assert rr_ind <= 1e-10, rr_ind
+    # the oracle's restatement: FP64 ISAI handles run the inner apply with FP32 factors
+    P = _oracle_precond(rp, ci, val, "fp32" if prec == "fp64" else prec)
+    xo, sto, ito, outo, rro = O.fgmres_mixed(rp, ci, val, P, b, m=10, tol=1e-10, maxit=400, inner_tol=1e-5)
+    assert sto == 0 and rro <= 1e-10
+    # the same outer steps; inner counts may differ by the FP32 rounding order of the
+    # products (one iteration per inner solve)
+    assert outer == outo, (outer, outo)
+    assert abs(it - ito) <= outer, (it, ito)
+    # both solutions are within the 1e-10 residual of the exact one
+    xd = O.bicgstab(rp, ci, val, _oracle_precond(rp, ci, val, "fp64"), b, tol=1e-13, maxit=500)[0]
+    scale = np.max(np.abs(xd))
+    assert np.max(np.abs(x - xd)) <= 1e-8 * scale
+    assert np.max(np.abs(xo - xd)) <= 1e-8 * scale
+    h.close()
+
+
+def test_mixed_inner_product_matches_oracle_rounding(gpu):
+    """The inner solve's FP32 A product (A rounded to FP32, FP32 x, FP32 accumulation)
+    against the oracle's rounded product, bounded by the FP32 accumulation error, and
+    its fused FP64 dot partials against the FP64 dot of the FP32 results."""
+    rp, ci, val = problems.generate(2, 10, seed=3)
+    A = Matrix(rp, ci, val)
+    n = A.n
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n).astype(np.float32)
+    a = rng.standard_normal(n).astype(np.float32)
+    A32 = DeviceArray((len(ci) * 16,), np.float32)
+    call("isai_convert_values", len(ci) * 16, 1, A.d_val.ptr, A32.ptr, None)
+    d_x, d_a = DeviceArray.from_numpy(x), DeviceArray.from_numpy(a)
+    d_y = DeviceArray((n,), np.float32)
+    d_part = DeviceArray((2 * 4096,), np.float64)
+    groups = ctypes.c_int()
+    call("isai_bsr_spmv_f32_dot", A.nb, A.nnzb, A.d_rp.ptr, A.d_ci.ptr, A32.ptr, d_x.ptr, d_y.ptr, d_a.ptr, 2,
+         d_part.ptr, ctypes.byref(groups), None)
+    call("isai_device_synchronize")
+    groups = groups.value
+    assert groups == int(np.ceil(A.nb / 32))
+    y = d_y.to_numpy().astype(np.float64)
+    ref = O.spmv_f32acc(rp, ci, val, x.astype(np.float64))
+    absA = np.abs(O.bsr_to_dense(rp, ci, val))
+    bound = 2 * 27 * 4 * 2.0**-24 * (absA @ np.abs(x.astype(np.float64)))
+    assert np.all(np.abs(y - ref) <= bound + 1e-30)
+    part = d_part.to_numpy()
+    assert abs(part[:groups].sum() - float(np.dot(y, a.astype(np.float64)))) <= 1e-12 * np.sum(np.abs(y * a))
+    assert abs(part[groups:2 * groups].sum() - float(np.dot(y, y))) <= 1e-12 * np.dot(y, y)
+
+
+def test_mixed_solve_full_size_96cube(gpu):
+    """The north-star workload (3-D 7-pt 96^3, b = A*1) with BF16 ISAI: FP64 relative
+    residual <= 1e-10 checked on the host, and the solution of all ones."""
+    print("mixed fgmres 96^3 bf16", flush=True)
+    rp, ci, val = problems.generate(1, 0, seed=0)
+    A = Matrix(rp, ci, val)
+    d_one = DeviceArray.from_numpy(np.ones(A.n))
+    d_b, d_x = DeviceArray((A.n,), np.float64), DeviceArray.zeros((A.n,), np.float64)
+    A.spmv(d_one, d_b)
+    h = Handle(A, precond="isai", precision="bf16")
+    st, it, rr = h.solve(d_b, d_x, "fgmres_mixed", tol=1e-10, maxit=400, restart=10)
+    outer, inner = h.last_stats()
+    print(f"  outer {outer} inner {inner} relres {rr:.3e}", flush=True)
+    x = d_x.to_numpy()
+    b = d_b.to_numpy()
+    assert st == 0 and O.rel_residual(rp, ci, val, b, x) <= 1e-10
+    assert np.max(np.abs(x - 1.0)) <= 1e-8
+    h.close()

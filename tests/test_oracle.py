@@ -426,6 +426,52 @@ def test_krylov_solvers_converge_to_direct_solution(solver, kind):
     assert np.allclose(xd, 1.0, atol=1e-10)
 
 
+def _isai_precond(rp, ci, val, f32=False, bf16=False):
+    nb = len(rp) - 1
+    lu, _, _ = O.bsrilu0(rp, ci, val)
+    (mbL, mcL, ML), (mbU, mcU, MU) = _isai_factors(rp, ci, lu)
+    trpL, tciL = O.tri_bcsr_pattern(rp, ci, True)
+    trpU, tciU = O.tri_bcsr_pattern(rp, ci, False)
+    _, mapL = O.map_bcsc2bcsr(mbL, mcL, trpL, tciL)
+    _, mapU = O.map_bcsc2bcsr(mbU, mcU, trpU, tciU)
+    L = (trpL, tciL, O.reorder(mapL, ML))
+    U = (trpU, tciU, O.reorder(mapU, MU))
+    return O.make_precond(1, nb, 4, L=L, U=U, f32=f32, bf16=bf16)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("problem,L", [(0, 8), (1, 4), (2, 4)])
+def test_fgmres_mixed_converges_to_direct_solution(prec, problem, L):
+    """The inner-outer mixed scheme reaches the FP64 answer: the outer FGMRES wins
+    back what the FP32 inner solves (FP32 A, FP32 / BF16 ISAI, FP32 vectors) lose."""
+    rp, ci, val = problems.generate(problem, L, seed=3)
+    n = (len(rp) - 1) * 4
+    b = RNG.standard_normal(n)
+    P = _isai_precond(rp, ci, val, f32=prec == "fp32", bf16=prec == "bf16")
+    x, st, it, outer, rr = O.fgmres_mixed(rp, ci, val, P, b, m=10, tol=1e-10, maxit=400, inner_tol=1e-5)
+    assert st == 0 and rr <= 1e-10
+    assert 2 <= outer <= 3, outer  # a 1e-5 inner tolerance: two or three outer steps for 1e-10
+    xd = np.linalg.solve(O.bsr_to_dense(rp, ci, val), b)
+    assert np.max(np.abs(x - xd)) <= 1e-8 * np.max(np.abs(xd))
+
+
+def test_fgmres_mixed_inner_solve_has_the_fp32_floor():
+    """Twelve inner iterations in FP32 storage leave a true FP64 residual at the FP32
+    floor (between 1e-8 and 1e-5); twelve FP64 BiCGSTAB iterations with the same
+    preconditioner pass 1e-10.  An inner solve that forgot to round A or its vectors
+    to FP32 would pass 1e-8."""
+    rp, ci, val = problems.generate(1, 6, seed=4)
+    n = (len(rp) - 1) * 4
+    b = RNG.standard_normal(n)
+    P = _isai_precond(rp, ci, val, f32=True)
+    x, st, it, outer, rr = O.fgmres_mixed(rp, ci, val, P, b, m=1, tol=1e-13, maxit=12, inner_tol=1e-14,
+                                          inner_maxit=12)
+    assert st != 0 and outer == 1 and it == 12
+    assert 1e-8 < rr < 1e-5, rr
+    _, _, it32, rr32 = O.bicgstab(rp, ci, val, P, b, tol=1e-16, maxit=12)
+    assert it32 == 12 and rr32 <= 1e-10, rr32
+
+
 # ---------------------------------------------------------- differentiation form
 def test_bsr_transpose_dense():
     nb, bs = 14, 4
