@@ -105,7 +105,7 @@ int isai_solve(isai_handle_t handle, int solver, int restart, double tol, int ma
 
 /* Inner solve of ISAI_SOLVER_FGMRES_MIXED: an inner solve stops when its residual
  * has fallen by max(inner_tol, 0.5 tol ||b|| / (current outer residual)) (at most
- * 0.5) or after inner_maxit iterations.  Defaults 1e-5 and 40. */
+ * 0.5) or after inner_maxit iterations.  Defaults 3e-5 and 40. */
 int isai_set_inner_solve(isai_handle_t handle, double inner_tol, int inner_maxit);
 /* Outer FGMRES steps and inner iterations of the last ISAI_SOLVER_FGMRES_MIXED
  * solve (0 and the iteration count for the other solvers). */

@@ -106,13 +106,18 @@ size_t inner_part_doubles(int nb);
 // sum y.y for ndot = 2) over the rows of workgroup wg, in FP64.
 int mixed_spmv_f32_dot(int nb, long long nnzb, const int *rp, const int *ci, const float *val, const float *x,
                        float *y, const float *a, int ndot, double *part, hipStream_t st, const int *skip);
-// r = rhs (rounded to FP32), rhat = r, x = p = v = 0; tolerance tol_in relative to
-// ||r||; starts done when *outer_done != 0.
-int inner_init(size_t n, const double *rhs, InnerVecs *w, InnerScalars *IS, const int *outer_done, double tol_in,
-               hipStream_t st);
+struct InnerLog {  // outcome of one inner solve, recorded on the device
+    int done, iters;
+    double tol_in;
+};
+// r = rhs (rounded to FP32), rhat = r, x = 0; tolerance relative to ||r||:
+// min(0.5, max(floor_tol, 0.5 sqrt(*tol2) / |*gj|)) with *gj the outer residual
+// estimate and *tol2 = tol^2 ||b||^2; starts done when *outer_done != 0.
+int inner_init(size_t n, const double *rhs, InnerVecs *w, InnerScalars *IS, const int *outer_done,
+               const double *gj, const double *tol2, double floor_tol, hipStream_t st);
 int inner_iteration(size_t n, int nb, long long nnzb, const int *rp, const int *ci, const float *A32,
                     const InnerPrecond *M, InnerVecs *w, InnerScalars *IS, bool first, hipStream_t st);
-// z = x in FP64.
-int inner_finish(size_t n, const InnerVecs *w, double *z, hipStream_t st);
+// z = x in FP64; *log = the inner solve's done flag, iterations and tolerance.
+int inner_finish(size_t n, const InnerVecs *w, double *z, const InnerScalars *IS, InnerLog *log, hipStream_t st);
 
 }  // namespace isai

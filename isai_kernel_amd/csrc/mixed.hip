@@ -28,9 +28,14 @@ constexpr int kMixedVecBlocks = 512;
 // (one 128-byte line when the pair is line-aligned).  The lanes of one block (a
 // quad) each gather one FP32 component of the block's x column and exchange the
 // rest with DPP moves.  The batch's column indices are loaded once per lane group
-// and handed out by shuffle; all value loads and x gathers of a batch are issued
-// before the first product.  FP32 accumulation: lane (b, sub) sums blocks b, b+2,
-// b+4, ... of the row in order and the two lanes of a row are added at the end.
+// and handed out by shuffle.  All value loads and x gathers of a batch are issued
+// before the first product and are unconditional (a lane past the row's end reads
+// block 0 and discards it): loads under a per-block branch were waited for one at
+// a time.  The grid strides over 32-row chunks.  FP32 accumulation: lane (b, sub)
+// sums blocks b, b+2, b+4, ... of the row in order and the two lanes of a row are
+// added at the end.  Measured on the 96^3 7-pt A (FP32 values, two dots): 83.4 us
+// against 88.6 us with per-block branches; column-major blocks (lane = block
+// column, no DPP) 102 us; two rows per lane group 82-88 us.
 template <int NDOT, int KB>
 __global__ __launch_bounds__(kMixedBlock) void bsr4_spmv_f32_dot_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci, const v4f *__restrict__ val4,
@@ -51,11 +56,11 @@ __global__ __launch_bounds__(kMixedBlock) void bsr4_spmv_f32_dot_kernel(
             const int clo = s < KB && t + s < len ? ci[rs + t + s] : 0;
             const int chi = KB > 8 && t + 8 + s < len ? ci[rs + t + 8 + s] : 0;
             v4f w[kJ];
-            float f[kJ][4];
+            float xv[kJ];
 #pragma unroll
             for (int j = 0; j < kJ; ++j) {
                 const int k = t + 2 * j + b;
-                if (k < len) w[j] = val4[(size_t)(rs + k) * 4 + sub];
+                w[j] = val4[(size_t)(k < len ? rs + k : 0) * 4 + sub];
             }
 #pragma unroll
             for (int j = 0; j < kJ; ++j) {
@@ -63,21 +68,20 @@ __global__ __launch_bounds__(kMixedBlock) void bsr4_spmv_f32_dot_kernel(
                 const int clo_u = __shfl(clo, u & 7, 8);
                 const int chi_u = KB > 8 ? __shfl(chi, u & 7, 8) : 0;
                 const int c = u < 8 ? clo_u : chi_u;
-                if (t + u < len) {
-                    const float v = x[(size_t)c * 4 + sub];
-                    f[j][0] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x00, 0xF, 0xF, false));
-                    f[j][1] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x55, 0xF, 0xF, false));
-                    f[j][2] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xAA, 0xF, 0xF, false));
-                    f[j][3] = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xFF, 0xF, 0xF, false));
-                }
+                xv[j] = x[(size_t)c * 4 + sub];
             }
 #pragma unroll
             for (int j = 0; j < kJ; ++j) {
+                const int v = __float_as_int(xv[j]);
+                const float f0 = __int_as_float(__builtin_amdgcn_mov_dpp(v, 0x00, 0xF, 0xF, false));
+                const float f1 = __int_as_float(__builtin_amdgcn_mov_dpp(v, 0x55, 0xF, 0xF, false));
+                const float f2 = __int_as_float(__builtin_amdgcn_mov_dpp(v, 0xAA, 0xF, 0xF, false));
+                const float f3 = __int_as_float(__builtin_amdgcn_mov_dpp(v, 0xFF, 0xF, 0xF, false));
                 if (t + 2 * j + b >= len) continue;
-                acc = fmaf(w[j].x, f[j][0], acc);
-                acc = fmaf(w[j].y, f[j][1], acc);
-                acc = fmaf(w[j].z, f[j][2], acc);
-                acc = fmaf(w[j].w, f[j][3], acc);
+                acc = fmaf(w[j].x, f0, acc);
+                acc = fmaf(w[j].y, f1, acc);
+                acc = fmaf(w[j].z, f2, acc);
+                acc = fmaf(w[j].w, f3, acc);
             }
         }
         acc += __shfl_xor(acc, 4);
@@ -162,12 +166,16 @@ __device__ __forceinline__ double partial_sum(const double *__restrict__ part, i
     return mixed_block_sum<kMixedVecThreads>(v);
 }
 
-// r = rhs (FP64 input rounded to FP32), rhat = r, x = p = v = 0; rpart[0] = ||r||^2,
-// rpart[1] = rhat.r (= ||r||^2).  The solve starts done when the outer loop is done.
+// r = rhs (FP64 input rounded to FP32), rhat = r, x = 0 (p and v are first read
+// after the first iteration wrote them); rpart[0] = ||r||^2, rpart[1] = rhat.r
+// (= ||r||^2).  The tolerance asks for the reduction still missing in the outer
+// loop, tol_in = min(0.5, max(floor, 0.5 tol ||b|| / |g_j|)), from the outer
+// residual estimate |g_j| and tol2 = tol^2 ||b||^2 on the device.  The solve starts
+// done when the outer loop is done.
 __global__ __launch_bounds__(kMixedVecThreads) void inner_init_kernel(
     size_t n, const double *__restrict__ rhs, float *__restrict__ r, float *__restrict__ rh, float *__restrict__ x,
-    float *__restrict__ p, float *__restrict__ v, double *__restrict__ rpart, InnerScalars *IS,
-    const int *__restrict__ outer_done, double tol_in) {
+    double *__restrict__ rpart, InnerScalars *IS, const int *__restrict__ outer_done,
+    const double *__restrict__ gj, const double *__restrict__ tol2, double floor_tol) {
     double acc = 0.0;
     for (size_t i = (size_t)blockIdx.x * kMixedVecThreads + threadIdx.x; i < n;
          i += (size_t)kMixedVecBlocks * kMixedVecThreads) {
@@ -175,8 +183,6 @@ __global__ __launch_bounds__(kMixedVecThreads) void inner_init_kernel(
         r[i] = ri;
         rh[i] = ri;
         x[i] = 0.0f;
-        p[i] = 0.0f;
-        v[i] = 0.0f;
         acc += (double)ri * (double)ri;
     }
     acc = mixed_block_sum<kMixedVecThreads>(acc);
@@ -185,9 +191,12 @@ __global__ __launch_bounds__(kMixedVecThreads) void inner_init_kernel(
         rpart[kMixedVecBlocks + blockIdx.x] = acc;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double g = fabs(*gj);
+        double t = g > 0.0 ? 0.5 * sqrt(*tol2) / g : 0.5;
+        t = fmin(0.5, fmax(floor_tol, t));
         IS->done = (outer_done && *outer_done) ? 1 : 0;
         IS->iters = 0;
-        IS->tol_in = tol_in;
+        IS->tol_in = t;
     }
 }
 
@@ -280,16 +289,23 @@ __global__ __launch_bounds__(kMixedVecThreads) void inner_xr_kernel(
     }
 }
 
-// z = x widened to FP64 (the flexible preconditioner's output).
-__global__ void inner_finish_kernel(size_t n, const float *__restrict__ x, double *__restrict__ z) {
+// z = x widened to FP64 (the flexible preconditioner's output); the first thread
+// records the inner solve's outcome in *log for the host's next read.
+__global__ void inner_finish_kernel(size_t n, const float *__restrict__ x, double *__restrict__ z,
+                                    const InnerScalars *IS, InnerLog *log) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        log->done = IS->done;
+        log->iters = IS->iters;
+        log->tol_in = IS->tol_in;
+    }
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         z[i] = (double)x[i];
 }
 
-int inner_init(size_t n, const double *rhs, InnerVecs *w, InnerScalars *IS, const int *outer_done, double tol_in,
-               hipStream_t st) {
-    inner_init_kernel<<<kMixedVecBlocks, kMixedVecThreads, 0, st>>>(n, rhs, w->r, w->rh, w->x, w->p, w->v, w->rpart,
-                                                                    IS, outer_done, tol_in);
+int inner_init(size_t n, const double *rhs, InnerVecs *w, InnerScalars *IS, const int *outer_done,
+               const double *gj, const double *tol2, double floor_tol, hipStream_t st) {
+    inner_init_kernel<<<kMixedVecBlocks, kMixedVecThreads, 0, st>>>(n, rhs, w->r, w->rh, w->x, w->rpart, IS,
+                                                                    outer_done, gj, tol2, floor_tol);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
@@ -319,10 +335,10 @@ int inner_iteration(size_t n, int nb, long long nnzb, const int *rp, const int *
     return ISAI_OK;
 }
 
-int inner_finish(size_t n, const InnerVecs *w, double *z, hipStream_t st) {
+int inner_finish(size_t n, const InnerVecs *w, double *z, const InnerScalars *IS, InnerLog *log, hipStream_t st) {
     size_t g = (n + 255) / 256;
     if (g > 4096) g = 4096;
-    inner_finish_kernel<<<(unsigned)g, 256, 0, st>>>(n, w->x, z);
+    inner_finish_kernel<<<(unsigned)g, 256, 0, st>>>(n, w->x, z, IS, log);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }

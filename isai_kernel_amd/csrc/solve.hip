@@ -586,11 +586,14 @@ struct isai_handle {
     float *inner_vecs = nullptr;
     double *inner_part = nullptr;
     InnerScalars *IS = nullptr;
-    double *h_scal = nullptr;      // pinned: outer residual estimate / norms read back
-    int *h_inner = nullptr;        // pinned: inner done, iters
-    double inner_tol = 1e-5;       // smallest relative reduction asked of one inner solve
+    int *h_inner = nullptr;        // pinned: inner done, iters (the inner solve's own read loop)
+    InnerLog *d_log = nullptr;     // outcome of each inner solve since the last host read
+    InnerLog *h_log = nullptr;     // pinned copy
+    double inner_tol = 3e-5;       // smallest relative reduction asked of one inner solve
     int inner_maxit = 40;          // inner iterations per outer step
     double inner_rate = 1.6;       // inner iterations per decade of reduction seen so far
+    int inner_hist[64] = {};       // iterations the inner solve of outer step j took last time
+    int outer_pred = 0;            // outer steps the last converged solve took (0 = unknown)
     int last_outer = 0, last_inner = 0;
 };
 
@@ -819,8 +822,10 @@ int fgmres(isai_handle *h, int m, double tol, int maxit, const double *b, double
 }
 
 // ------------------------------------------------- mixed-precision form
+constexpr int kLogSteps = 16;  // inner outcomes held on the device between two host reads
+
 // Buffers of the inner solve: A in FP32 (converted once per handle), ten FP32
-// vectors, the partial sums and the scalar record.
+// vectors, the partial sums, the scalar record and the outcome log.
 int ensure_inner(isai_handle *h, hipStream_t st) {
     if (h->IS) return ISAI_OK;
     const size_t n = h->n;
@@ -829,16 +834,18 @@ int ensure_inner(isai_handle *h, hipStream_t st) {
         hipMalloc(&h->inner_vecs, sizeof(float) * 10 * n) != hipSuccess ||
         hipMalloc(&h->inner_part, sizeof(double) * inner_part_doubles(h->nb)) != hipSuccess ||
         hipMalloc(&h->IS, sizeof(InnerScalars)) != hipSuccess ||
-        hipHostMalloc(&h->h_scal, sizeof(double) * 4) != hipSuccess ||
-        hipHostMalloc(&h->h_inner, sizeof(int) * 2) != hipSuccess) {
+        hipMalloc(&h->d_log, sizeof(InnerLog) * kLogSteps) != hipSuccess ||
+        hipHostMalloc(&h->h_inner, sizeof(int) * 2) != hipSuccess ||
+        hipHostMalloc(&h->h_log, sizeof(InnerLog) * kLogSteps) != hipSuccess) {
         (void)hipFree(h->d_A32);
         (void)hipFree(h->inner_vecs);
         (void)hipFree(h->inner_part);
         (void)hipFree(h->IS);
-        if (h->h_scal) (void)hipHostFree(h->h_scal);
+        (void)hipFree(h->d_log);
         if (h->h_inner) (void)hipHostFree(h->h_inner);
+        if (h->h_log) (void)hipHostFree(h->h_log);
         h->d_A32 = nullptr, h->inner_vecs = nullptr, h->inner_part = nullptr, h->IS = nullptr;
-        h->h_scal = nullptr, h->h_inner = nullptr;
+        h->d_log = nullptr, h->h_inner = nullptr, h->h_log = nullptr;
         set_error("isai_solve: cannot allocate the mixed-precision inner solve");
         return ISAI_ERR_ALLOC;
     }
@@ -856,38 +863,60 @@ InnerVecs inner_vecs(isai_handle *h) {
     return w;
 }
 
-// z = (inner BiCGSTAB on A z = v, to tol_in relative to ||v||, at most budget
-// iterations).  The host launches a predicted number of iterations (from the rate
-// of the previous inner solves), then tops up two at a time, reading the inner
-// state after each batch; iterations launched after convergence return at once.
-int inner_solve(isai_handle *h, const InnerPrecond *M, const double *v, double *z, double tol_in, int budget,
-                int *count, hipStream_t st) {
+// z = (inner BiCGSTAB on A z = v_j, tolerance from the outer residual estimate g_j,
+// at most budget iterations); its outcome goes to log.  When the inner solve of
+// the same outer step has run before, the host launches the iteration count it
+// needed then and does not wait: the outer loop is flexible, so an inner solve
+// that stops short only costs accuracy, and the outcome is read back later with
+// the outer state (inner_settle adjusts the next prediction).  Otherwise the host
+// launches an estimate from the iterations per decade seen so far and tops up one
+// at a time, reading the inner state after each batch.  Iterations launched after
+// convergence return at once.  A host read idles the GPU for about 30 us; a
+// skipped iteration costs about 40 us of empty launches.
+int inner_solve(isai_handle *h, const InnerPrecond *M, const double *v, double *z, const double *gj, int budget,
+                int step, InnerLog *log, int *launched_out, hipStream_t st) {
     InnerVecs w = inner_vecs(h);
-    TRY(inner_init(h->n, v, &w, h->IS, &h->S->done, tol_in, st));
-    const double decades = -std::log10(tol_in);
-    int batch = std::max(1, std::min(budget, (int)(h->inner_rate * decades)));
+    TRY(inner_init(h->n, v, &w, h->IS, &h->S->done, gj, &h->S->tol2, h->inner_tol, st));
+    const int hist = step < 64 ? h->inner_hist[step] : 0;
+    int batch = hist > 0 ? hist : (int)std::ceil(h->inner_rate * -std::log10(h->inner_tol));
+    batch = std::max(1, std::min(budget, batch));
     int launched = 0;
     while (true) {
         for (int k = 0; k < batch; ++k, ++launched)
             TRY(inner_iteration(h->n, h->nb, h->nnzb, h->d_row, h->d_col, h->d_A32, M, &w, h->IS, launched == 0,
                                 st));
+        if (hist > 0) break;
         ISAI_HIP(hipMemcpyAsync(h->h_inner, &h->IS->done, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
         ISAI_HIP(hipStreamSynchronize(st));
         if (h->h_inner[0] || launched >= budget) break;
-        batch = std::min(2, budget - launched);
+        batch = 1;
     }
-    TRY(inner_finish(h->n, &w, z, st));
-    *count = h->h_inner[1];
-    if (h->h_inner[0] == 1 && *count > 0) h->inner_rate = 0.5 * h->inner_rate + 0.5 * (*count / decades);
+    TRY(inner_finish(h->n, &w, z, h->IS, log, st));
+    *launched_out = launched;
     return ISAI_OK;
+}
+
+// An inner outcome read back: a solve that converged sets the next prediction for
+// its outer step to its count, one that ran out of launched iterations without
+// converging to one more.
+void inner_settle(isai_handle *h, int step, int launched, const InnerLog &lg) {
+    if (lg.done == 1 && lg.iters > 0) {
+        h->inner_rate = 0.5 * h->inner_rate + 0.5 * (lg.iters / -std::log10(lg.tol_in));
+        if (step < 64) h->inner_hist[step] = lg.iters;
+    } else if (lg.done == 0 && step < 64 && h->inner_hist[step] > 0) {
+        h->inner_hist[step] = std::min(h->inner_maxit, launched + 1);
+    }
 }
 
 // FP64 FGMRES(m) whose flexible preconditioner is the inner mixed-precision solve:
 // z_j = inner(v_j).  The residual r = b - A x, the Arnoldi products w = A z_j, the
 // Gram-Schmidt orthogonalisation and the Givens rotations are FP64.  The inner
 // tolerance for step j asks for the reduction still missing, 0.5 tol ||b|| /
-// |g_j| (|g_j| = the outer residual estimate), between inner_tol and 0.5.
-// *iters = inner BiCGSTAB iterations, bounded by maxit.
+// |g_j| (|g_j| = the outer residual estimate), between inner_tol and 0.5,
+// computed on the device.  When the last converged solve took P outer steps, the
+// host launches the first P steps without reading the state in between (steps
+// launched after convergence return at once), then reads the convergence word and
+// the inner outcomes once.  *iters = inner BiCGSTAB iterations, bounded by maxit.
 int fgmres_mixed(isai_handle *h, int m, double tol, int maxit, const double *b, double *x, int *iters,
                  double *relres, hipStream_t st) {
     if (m <= 0) m = 10;
@@ -909,30 +938,48 @@ int fgmres_mixed(isai_handle *h, int m, double tol, int maxit, const double *b, 
     const size_t n = h->n;
     double *V = h->work, *Z = V + n * (m + 1), *w = Z + n * m;
     const int *skip = &h->S->done;
-    int inner_total = 0, outer = 0;
+    int inner_total = 0, steps = 0, pending = 0, launched_est = 0, real_steps = 0;
+    int launched[kLogSteps] = {};
+    const int pred = h->outer_pred;
+    bool converged = false;
     *relres = INFINITY;
-    while (inner_total < maxit) {
+    // reads the convergence word and the outcomes of the pending inner solves
+    auto read_back = [&](int *done) -> int {
+        if (pending) ISAI_HIP(hipMemcpyAsync(h->h_log, h->d_log, sizeof(InnerLog) * pending, hipMemcpyDeviceToHost, st));
+        TRY(read_done(h, st, done));
+        for (int k = 0; k < pending; ++k) {
+            inner_total += h->h_log[k].iters;
+            real_steps += h->h_log[k].iters > 0;  // steps launched after convergence did nothing
+            inner_settle(h, steps - pending + k, launched[k], h->h_log[k]);
+        }
+        pending = 0;
+        launched_est = inner_total;
+        return ISAI_OK;
+    };
+    while (launched_est < maxit) {
         TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, x, w, st, nullptr));
         residual_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, b, w, h->S);
         fg_beta_final<<<1, kRedThreads, 0, st>>>(h->S, tol);
         fg_scale_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V, h->S, 0, 0, 1);
         ISAI_LAUNCH_CHECK();
-        ISAI_HIP(hipMemcpyAsync(h->h_scal, &h->S->beta, sizeof(double), hipMemcpyDeviceToHost, st));
-        ISAI_HIP(hipMemcpyAsync(h->h_scal + 1, &h->S->tol2, sizeof(double), hipMemcpyDeviceToHost, st));
         int done = 0;
-        TRY(read_done(h, st, &done));
-        if (done) break;
-        const double target = std::sqrt(h->h_scal[1]);  // tol * ||b||
-        double gj = h->h_scal[0];
+        if (pred <= 0 || steps >= pred) {
+            TRY(read_done(h, st, &done));
+            if (done) {
+                converged = true;
+                break;
+            }
+        }
         int jend = 0;
-        for (int j = 0; j < m && inner_total < maxit; ++j) {
+        for (int j = 0; j < m && launched_est < maxit; ++j) {
             double *vj = V + n * j, *zj = Z + n * j;
-            double tol_in = gj > 0.0 ? 0.5 * target / gj : 0.5;
-            tol_in = std::min(0.5, std::max(h->inner_tol, tol_in));
-            int k = 0;
-            TRY(inner_solve(h, &M, vj, zj, tol_in, std::min(h->inner_maxit, maxit - inner_total), &k, st));
-            inner_total += k;
-            ++outer;
+            const int step = steps++;
+            // g_j is the residual estimate before step j (beta for j = 0)
+            const double *gj = j == 0 ? &h->S->beta : &h->S->g[j];
+            TRY(inner_solve(h, &M, vj, zj, gj, std::min(h->inner_maxit, maxit - launched_est), step,
+                            &h->d_log[pending], &launched[pending], st));
+            launched_est += launched[pending];
+            ++pending;
             TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, zj, w, st, skip));
             const int nv = j + 1;
             const dim3 dgrid(kRedBlocks, (nv + kDotChunk - 1) / kDotChunk);
@@ -945,23 +992,35 @@ int fgmres_mixed(isai_handle *h, int m, double tol, int maxit, const double *b, 
             fg_givens_final<<<1, kRedThreads, 0, st>>>(h->S, j);
             fg_scale_next_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V + n * (j + 1), h->S);
             ISAI_LAUNCH_CHECK();
-            ISAI_HIP(hipMemcpyAsync(h->h_scal, &h->S->g[j + 1], sizeof(double), hipMemcpyDeviceToHost, st));
-            TRY(read_done(h, st, &done));
-            gj = std::fabs(h->h_scal[0]);
             jend = j + 1;
-            if (done) break;
+            if (pred <= 0 || steps >= pred || pending == kLogSteps || j + 1 == m) {
+                TRY(read_back(&done));
+                if (done) break;
+            }
+        }
+        if (pending) {
+            int d = 0;
+            TRY(read_back(&d));
         }
         if (jend == 0) break;
         fg_solve_y_kernel<<<1, 64, 0, st>>>(h->S);
         fg_update_x_kernel<<<vec_grid(n), 256, 0, st>>>(n, x, Z, h->S);
         ISAI_LAUNCH_CHECK();
         TRY(true_residual(h, b, x, w, relres, st));
-        if (*relres <= tol) break;
+        if (*relres <= tol) {
+            converged = true;
+            break;
+        }
     }
     *iters = inner_total;
-    h->last_outer = outer;
+    h->last_outer = real_steps;
     h->last_inner = inner_total;
-    if (*relres <= tol) return ISAI_OK;
+    if (converged && !std::isfinite(*relres)) TRY(true_residual(h, b, x, w, relres, st));
+    if (*relres <= tol) {
+        h->outer_pred = real_steps;
+        return ISAI_OK;
+    }
+    h->outer_pred = 0;
     if (!std::isfinite(*relres)) TRY(true_residual(h, b, x, w, relres, st));
     if (*relres <= tol) return ISAI_OK;
     set_error("mixed-precision fgmres: not converged after %d inner iterations (relres %.3e)", inner_total, *relres);
@@ -1017,8 +1076,9 @@ int isai_destroy(isai_handle_t h) {
     (void)hipFree(h->inner_vecs);
     (void)hipFree(h->inner_part);
     (void)hipFree(h->IS);
-    if (h->h_scal) (void)hipHostFree(h->h_scal);
+    (void)hipFree(h->d_log);
     if (h->h_inner) (void)hipHostFree(h->h_inner);
+    if (h->h_log) (void)hipHostFree(h->h_log);
     delete h;
     return ISAI_OK;
 }

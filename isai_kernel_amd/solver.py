@@ -76,7 +76,7 @@ class Handle:
             _check(st, "isai_solve")
         return st, it.value, rr.value
 
-    def set_inner(self, inner_tol: float = 1e-5, inner_maxit: int = 40) -> None:
+    def set_inner(self, inner_tol: float = 3e-5, inner_maxit: int = 40) -> None:
         """Inner-solve parameters of the mixed-precision solver (isai_set_inner_solve)."""
         call("isai_set_inner_solve", self.h, inner_tol, inner_maxit)
 

@@ -27,6 +27,7 @@ def test_mixed_solve_matches_oracle_scheme(gpu, prec, problem, L):
     A = Matrix(rp, ci, val)
     b = problems.normal(7, A.n)
     h = Handle(A, precond="isai", precision=prec)
+    h.set_inner(1e-5, 40)
     d_b, d_x = DeviceArray.from_numpy(b), DeviceArray.zeros((A.n,), np.float64)
     st, it, rr = h.solve(d_b, d_x, "fgmres_mixed", tol=1e-10, maxit=400, restart=10)
     outer, inner = h.last_stats()
@@ -36,7 +37,8 @@ def test_mixed_solve_matches_oracle_scheme(gpu, prec, problem, L):
     assert rr_ind <= 1e-10, rr_ind
     # the oracle's restatement: FP64 ISAI handles run the inner apply with FP32 factors
     P = _oracle_precond(rp, ci, val, "fp32" if prec == "fp64" else prec)
-    xo, sto, ito, outo, rro = O.fgmres_mixed(rp, ci, val, P, b, m=10, tol=1e-10, maxit=400, inner_tol=1e-5)
+    xo, sto, ito, outo, rro = O.fgmres_mixed(rp, ci, val, P, b, m=10, tol=1e-10, maxit=400, inner_tol=1e-5,
+                                             inner_maxit=40)
     assert sto == 0 and rro <= 1e-10
     # the same outer steps; inner counts may differ by the FP32 rounding order of the
     # products (one iteration per inner solve)
