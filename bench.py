@@ -4,14 +4,17 @@
 Workload (N=1): 3-D 7-point stencil on a 96^3 grid, 4 dof per node (bs = 4,
 884,736 block rows, 6,137,856 blocks), FP64, synthetic (generated in-process,
 seed 0), b = A*1.  A "step" is one preconditioned solve to 1e-10 relative
-residual: ISAI-preconditioned BiCGSTAB with the ISAI factors stored in reduced
-precision (the mixed form), FP64 everywhere else.  The plain FP64 forms (FP64
-ISAI, and exact ILU(0) applied with the sync-free triangular solves) are timed
-the same way for the speed-up.
+residual (FP64, checked with the true residual): the mixed-precision form, an
+FP64 FGMRES outer loop whose flexible preconditioner is an inner BiCGSTAB in FP32
+storage (A rounded to FP32, BF16 ISAI factors, FP32 vectors, FP64 dot products).
+The plain FP64 form (ISAI-preconditioned BiCGSTAB with FP64 factors), BiCGSTAB with
+reduced-precision ISAI factors and exact ILU(0) applied with the sync-free
+triangular solves are timed the same way for the speed-ups.
 
 Kernel figures are hipEvent timings on the stream the kernels run on; the
-roofline object is for the dominant kernel of the solve (the FP64 BSR SpMV with
-A), with algorithmic bytes per launch (16*8+4)*nnzb + 4*(nb+1) + 2*4*8*nb.
+roofline object is for the dominant kernel of the timed solve, the FP32 BSR
+product with A and two fused dot products, with algorithmic bytes per launch
+(16*4+4)*nnzb + 4*(nb+1) + 3*4*4*nb (FP32 values and x, y and the dot vector).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-forms]
 
@@ -37,11 +40,11 @@ sys.path.insert(0, str(ROOT))
 METRIC = "solve time to 1e-10 rel-res, mixed vs FP64 ISAI; BSR SpMV/SpTRSV HBM GB/s"
 HBM_PEAK_GBPS = 8000.0
 TOL = 1e-10
-# Bytes per launch of the A SpMV at 96^3 beyond the L2 (HBM and Infinity Cache), from
-# the FETCH_SIZE / WRITE_SIZE counter passes summarised in
+# Bytes per launch of the FP32 A product (two dots) at 96^3 beyond the L2 (HBM and
+# Infinity Cache), from the FETCH_SIZE / WRITE_SIZE counter passes summarised in
 # profiles/hbm_traffic_96cube_7pt_pmc.md (FETCH_SIZE x2 by the copy-kernel
-# calibration): 451,177.8 KiB x 2 + 27,648 KiB.
-SPMV_TRAFFIC_BYTES_96CUBE = 451177.8 * 2 * 1024 + 27648 * 1024
+# calibration); None until measured.
+F32_SPMV_TRAFFIC_BYTES_96CUBE = None
 
 
 def log(msg: str) -> None:
@@ -59,6 +62,18 @@ def apply_bytes(nb: int, nnzb_l: int, nnzb_u: int, value_bytes: int) -> int:
     mid = 8 if value_bytes == 8 else 4
     return ((16 * value_bytes + 4) * (nnzb_l + nnzb_u) + 2 * 4 * (nb + 1) + 4 * 8 * nb
             + 2 * 4 * mid * nb + 4 * 8 * nb)
+
+
+def f32_spmv_dot_bytes(nb: int, nnzb: int) -> int:
+    """The mixed solve's A product: FP32 values + int32 indices once, the FP32 x
+    gathered once, the FP32 result written and the FP32 dot vector read."""
+    return (16 * 4 + 4) * nnzb + 4 * (nb + 1) + 3 * 4 * 4 * nb
+
+
+def apply_bytes_baseline_formula(nb: int, nnzb_l: int, nnzb_u: int, value_bytes: int) -> int:
+    """The ISAI-apply byte count as the benchmark targets state it: two products with
+    FP64 input and output vectors, 2 x [(16 s_v + 4) nnzb_T + 4 (nb + 1) + 2*4*8*nb]."""
+    return ((16 * value_bytes + 4) * (nnzb_l + nnzb_u) + 2 * 4 * (nb + 1) + 2 * 2 * 4 * 8 * nb)
 
 
 def sptrsv_bytes(nb: int, nnzb_l: int, nnzb_u: int) -> int:
@@ -116,9 +131,30 @@ def cpu_baseline(rp, ci, val, b, budget_s: float = 40.0):
     return dict(setup_s=t_setup, solve_s=t_solve, iters=it, relres=rr, cores=O.num_threads(), status=st)
 
 
+def precision_reference():
+    """The oracle's FP64 FGMRES(30) solution of the precision form's system, as the
+    sampled entries and projections that tests/golden/make_precision_reference.py
+    recorded (the oracle itself takes about 90 s on 8 host cores)."""
+    path = ROOT / "tests" / "golden" / "precision_64cube_oracle.json"
+    return json.loads(path.read_text()) if path.exists() else None
+
+
+def error_vs_reference(x, ref) -> dict:
+    idx = np.asarray(ref["sample_index"])
+    xs = np.asarray(ref["x_samples"])
+    rng = np.random.default_rng(ref["sample_seed"] + 1)
+    proj = [float(np.dot(rng.standard_normal(len(x)), x)) for _ in range(len(ref["projections"]))]
+    pe = max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(proj, ref["projections"]))
+    return {"max_rel_err_samples_vs_oracle": float(np.max(np.abs(x[idx] - xs)) / ref["x_norm_inf"]),
+            "max_rel_err_projections_vs_oracle": float(pe)}
+
+
 def bench_precision(timer, stream, steps: int) -> dict:
     """Precision form: 3-D 27-pt convection-diffusion 64^3 (bs = 4, 262,144 block rows),
-    FGMRES(30) to 1e-10 with the ISAI factors stored in FP64 / FP32 / BF16."""
+    to 1e-10 with the ISAI factors stored in FP64 / FP32 / BF16: FP64 FGMRES(30) with
+    the ISAI apply as its preconditioner, and the mixed form (FP64 FGMRES(30) outer
+    loop around the inner FP32-storage BiCGSTAB).  Errors are against the oracle's
+    FP64 FGMRES(30) solution (sampled, precision_reference)."""
     from isai_kernel_amd import DeviceArray, call, problems
     from isai_kernel_amd.solver import Handle, Matrix
 
@@ -129,25 +165,35 @@ def bench_precision(timer, stream, steps: int) -> dict:
     d_b = DeviceArray.from_numpy(problems.normal(1, n))
     d_x = DeviceArray((n,), np.float64)
     h = Handle(A, precond="isai", precision="fp64", stream=stream)
+    ref = precision_reference()
     out = {"workload": "3-D 27-pt convection-diffusion 64^3, bs=4, FGMRES(30) to 1e-10, b ~ N(0,1)",
-           "nb": nb, "nnzb": nnzb}
+           "nb": nb, "nnzb": nnzb,
+           "oracle": ({"iters": ref["iters"], "relres": ref["relres"], "seconds": ref["oracle_seconds"],
+                       "threads": ref["threads"]} if ref else None)}
     xs = {}
-    for prec in ("fp64", "fp32", "bf16"):
+    legs = [("fp64", "fp64", "fgmres"), ("fp32", "fp32", "fgmres"), ("bf16", "bf16", "fgmres"),
+            ("mixed_fp32", "fp32", "fgmres_mixed"), ("mixed_bf16", "bf16", "fgmres_mixed")]
+    for key, prec, solver in legs:
         h.set_preconditioner("isai", prec)
         state = {}
 
         def one():
             call("isai_fill_f64_async", d_x.ptr, 0.0, n, stream)
-            st, it, rr = h.solve(d_b, d_x, "fgmres", tol=TOL, maxit=3000, restart=30, check=False)
+            st, it, rr = h.solve(d_b, d_x, solver, tol=TOL, maxit=3000, restart=30, check=False)
             state.update(st=st, it=it, rr=rr)
         one()
         ms = timer.time(one, steps)
-        xs[prec] = d_x.to_numpy()
-        out[prec] = {"ms": round(ms, 3), "iters": state["it"], "relres": state["rr"], "status": state["st"]}
-        log(f"precision form {prec}: {ms:.2f} ms ({state['it']} it, relres {state['rr']:.2e})")
-    for prec in ("fp32", "bf16"):
-        out[prec]["max_rel_diff_vs_fp64"] = float(np.max(np.abs(xs[prec] - xs["fp64"])) / np.max(np.abs(xs["fp64"])))
-        out[prec]["speedup_vs_fp64"] = round(out["fp64"]["ms"] / out[prec]["ms"], 3)
+        xs[key] = d_x.to_numpy()
+        out[key] = {"ms": round(ms, 3), "iters": state["it"], "relres": state["rr"], "status": state["st"],
+                    "solver": solver}
+        if solver == "fgmres_mixed":
+            out[key]["outer_steps"] = h.last_stats()[0]
+        if ref:
+            out[key].update(error_vs_reference(xs[key], ref))
+        log(f"precision form {key}: {ms:.2f} ms ({state['it']} it, relres {state['rr']:.2e})")
+    for key, _, _ in legs[1:]:
+        out[key]["max_rel_diff_vs_fp64"] = float(np.max(np.abs(xs[key] - xs["fp64"])) / np.max(np.abs(xs["fp64"])))
+        out[key]["speedup_vs_fp64"] = round(out["fp64"]["ms"] / out[key]["ms"], 3)
     h.close()
     return out
 
@@ -391,10 +437,10 @@ def main() -> int:
     log(f"copy kernel: {copy_ms*1e3:.1f} us for {2*8*ncopy/2**20:.0f} MiB moved, {copy_gbps:.0f} GB/s")
     del d_c0, d_c1
 
-    def record(name, ms, nbytes):
+    def record(name, ms, nbytes, **extra):
         gbps = nbytes / (ms * 1e-3) / 1e9
         kernels[name] = {"us": round(ms * 1e3, 2), "GBps": round(gbps, 1),
-                         "frac_hbm": round(gbps / HBM_PEAK_GBPS, 4), "algorithmic_bytes": nbytes}
+                         "frac_hbm": round(gbps / HBM_PEAK_GBPS, 4), "algorithmic_bytes": nbytes, **extra}
         log(f"{name}: {ms*1e3:.1f} us, {gbps:.0f} GB/s ({100*gbps/HBM_PEAK_GBPS:.1f}% of 8 TB/s)")
 
     # ---- BSR SpMV with A (FP64)
@@ -405,6 +451,23 @@ def main() -> int:
     spmv_ms = timer.time(spmv, K)
     record("bsr_spmv_fp64", spmv_ms, spmv_bytes(nb, nnzb))
 
+    # ---- the mixed solve's A product: FP32 values, FP32 x and y, two fused dots
+    d_a32 = DeviceArray((nnzb * 16,), np.float32)
+    call("isai_convert_values", nnzb * 16, 1, A.d_val.ptr, d_a32.ptr, stream)
+    d_xf = DeviceArray.from_numpy(np.ones(n, np.float32))
+    d_yf = DeviceArray((n,), np.float32)
+    d_part = DeviceArray((2 * 4096,), np.float64)
+    groups = ctypes.c_int()
+
+    def spmv32():
+        lib.isai_bsr_spmv_f32_dot(nb, nnzb, A.d_rp.ptr, A.d_ci.ptr, d_a32.ptr, d_xf.ptr, d_yf.ptr, d_xf.ptr, 2,
+                                  d_part.ptr, ctypes.byref(groups), stream)
+    for _ in range(10):
+        spmv32()
+    spmv32_ms = timer.time(spmv32, K)
+    record("bsr_spmv_fp32_dot", spmv32_ms, f32_spmv_dot_bytes(nb, nnzb))
+    del d_a32, d_xf, d_yf, d_part
+
     # ---- ISAI apply (M_U (M_L v)) in each storage precision
     for prec, vb in (("fp64", 8), ("fp32", 4), ("bf16", 2)):
         h.set_preconditioner("isai", prec)
@@ -414,7 +477,9 @@ def main() -> int:
         for _ in range(5):
             apply()
         ms = timer.time(apply, K)
-        record(f"isai_apply_{prec}", ms, apply_bytes(nb, nnzbL, nnzbU, vb))
+        formula = apply_bytes_baseline_formula(nb, nnzbL, nnzbU, vb)
+        record(f"isai_apply_{prec}", ms, apply_bytes(nb, nnzbL, nnzbU, vb), baseline_formula_bytes=formula,
+               frac_hbm_baseline_formula=round(formula / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4))
 
     # ---- sync-free SpTRSV (L then U with the packed ILU(0) factor)
     # the first selection of ILU(0) runs the triangular-solve analysis (host order
@@ -437,15 +502,20 @@ def main() -> int:
         return 0
 
     # ---- solves (the steps)
+    # mixed_*: FP64 FGMRES(10) outer loop, inner FP32-storage BiCGSTAB (the mixed form);
+    # isai_*: FP64 BiCGSTAB with the ISAI factors stored in that precision;
+    # ilu_sptrsv_fp64: FP64 BiCGSTAB with exact ILU(0) solves
     solves = {}
-    for form, (pc, prec) in {"isai_bf16": ("isai", "bf16"), "isai_fp32": ("isai", "fp32"),
-                             "isai_fp64": ("isai", "fp64"), "ilu_sptrsv_fp64": ("ilu", "fp64")}.items():
+    forms = {"mixed_bf16": ("isai", "bf16", "fgmres_mixed"), "mixed_fp32": ("isai", "fp32", "fgmres_mixed"),
+             "isai_bf16": ("isai", "bf16", "bicgstab"), "isai_fp32": ("isai", "fp32", "bicgstab"),
+             "isai_fp64": ("isai", "fp64", "bicgstab"), "ilu_sptrsv_fp64": ("ilu", "fp64", "bicgstab")}
+    for form, (pc, prec, solver) in forms.items():
         h.set_preconditioner(pc, prec)
         state = {}
 
         def one_solve():
             call("isai_fill_f64_async", d_x.ptr, 0.0, n, stream)
-            st, it, rr = h.solve(d_b, d_x, "bicgstab", tol=TOL, maxit=1000, check=False)
+            st, it, rr = h.solve(d_b, d_x, solver, tol=TOL, maxit=1000, restart=10, check=False)
             state.update(st=st, it=it, rr=rr)
         for _ in range(args.warmup):
             one_solve()
@@ -454,20 +524,27 @@ def main() -> int:
         ms = timer.time(one_solve, args.steps)
         wall = (time.perf_counter() - t0) / args.steps * 1e3
         x = d_x.to_numpy()
+        # the mixed form also converts A to FP32 once per handle (outside the timed solves)
         setup = ms_ilu + (ms_isai if pc == "isai" else ms_sv)
+        outer, inner = h.last_stats()
         solves[form] = {"ms": round(ms, 3), "wall_ms": round(wall, 3), "ms_with_setup": round(ms + setup, 3),
-                        "iters": state["it"],
+                        "iters": state["it"], "solver": solver,
                         "relres": state["rr"], "status": state["st"],
                         "max_err_vs_ones": float(np.max(np.abs(x - 1.0)))}
+        if solver == "fgmres_mixed":
+            solves[form]["outer_steps"] = outer
         log(f"solve {form}: {ms:.2f} ms ({state['it']} it, relres {state['rr']:.2e}, status {state['st']})")
 
     ok = {k: v for k, v in solves.items() if v["status"] == 0 and v["relres"] <= TOL}
-    mixed = min((k for k in ("isai_bf16", "isai_fp32") if k in ok), key=lambda k: ok[k]["ms"], default=None)
-    plain = [k for k in ("isai_fp64", "ilu_sptrsv_fp64") if k in ok]
+    mixed = "mixed_bf16" if "mixed_bf16" in ok else None
     value = ok[mixed]["ms"] if mixed else None
-    speedups = {f"{mixed}_vs_{p}": round(ok[p]["ms"] / ok[mixed]["ms"], 3) for p in plain} if mixed else {}
+    speedups = {}
+    for fast in ("mixed_bf16", "mixed_fp32", "isai_bf16", "isai_fp32"):
+        for base in ("isai_fp64", "ilu_sptrsv_fp64"):
+            if fast in ok and base in ok:
+                speedups[f"{fast}_vs_{base}"] = round(ok[base]["ms"] / ok[fast]["ms"], 3)
 
-    gbps = kernels["bsr_spmv_fp64"]["GBps"]
+    gbps = kernels["bsr_spmv_fp32_dot"]["GBps"]
     result = {
         "metric": METRIC,
         "value": value,
@@ -479,15 +556,18 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+bf16 inner, f64 outer",
         "data": "synthetic",
-        "config": {"workload": "ISAI-preconditioned BiCGSTAB to 1e-10 (mixed-precision ISAI, FP64 Krylov), "
+        "config": {"workload": "mixed-precision ISAI-preconditioned solve to 1e-10 (FP64 FGMRES(10) outer loop, "
+                               "inner FP32-storage BiCGSTAB with BF16 ISAI factors, FP64 dot products), "
                                "3-D 7-pt 96^3, bs=4 (884736 block rows, 6137856 blocks), b = A*1",
                    "problem": args.problem, "nb": nb, "nnzb": nnzb, "mixed_form": mixed},
         "roofline": {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(gbps / HBM_PEAK_GBPS, 4),
-                     "traffic": int(SPMV_TRAFFIC_BYTES_96CUBE) if args.problem == 1 else None,
-                     "kernel": "bsr4_spmv_f64_kernel", "algorithmic_bytes": spmv_bytes(nb, nnzb)},
+                     "traffic": (int(F32_SPMV_TRAFFIC_BYTES_96CUBE)
+                                 if args.problem == 1 and F32_SPMV_TRAFFIC_BYTES_96CUBE else None),
+                     "kernel": "bsr4_spmv_f32_dot_kernel<2, 8>",
+                     "algorithmic_bytes": f32_spmv_dot_bytes(nb, nnzb)},
         "speedup": speedups,
         "solves": solves,
         "kernels": kernels,

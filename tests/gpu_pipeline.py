@@ -97,3 +97,31 @@ def split_lu(rp, ci, lu):
 
 def rel_err(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def round_values(val, bf16):
+    """FP64 block values rounded as the library stores them: FP32 (RNE), or BF16
+    (RNE on the top 16 bits of the FP32 value)."""
+    f = np.asarray(val, np.float64).astype(np.float32)
+    if not bf16:
+        return f.astype(np.float64)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def reduced_apply_bound(L, U, v, y_ref, bf16):
+    """Row-wise bound on |z_gpu - z_oracle| for z = M_U (M_L v) with FP32 / BF16
+    factor values and FP32 accumulation, when both sides use the same rounded
+    operands and differ only in the order of the FP32 sums (k terms per row:
+    |sum error| <= k 2^-24 sum |terms| for either order).  The first product's
+    difference dy passes through |M_U|; the second adds its own order term."""
+    from _oracle import spmv
+    u = 2.0**-24
+    vf = np.abs(np.asarray(v, np.float32).astype(np.float64))
+    kL = 4 * int(np.diff(L[0]).max())
+    kU = 4 * int(np.diff(U[0]).max())
+    aL = np.abs(round_values(L[2], bf16))
+    aU = np.abs(round_values(U[2], bf16))
+    dy = 2 * kL * u * spmv(L[0], L[1], aL, vf)
+    return (spmv(U[0], U[1], aU, dy) + 2 * kU * u * spmv(U[0], U[1], aU, np.abs(y_ref))) * 1.01

@@ -189,3 +189,52 @@ def test_batch_every_spmm_candidate(stream, precision):
                 assert np.abs(it.astype(int) - its[0].astype(int)).max() <= 1
     finally:
         bt.close()
+
+
+def test_batch_full_size_config(stream):
+    """The batching form at its own size: 256 systems, 2-D 9-pt L x L with L in
+    [24, 200] (4.1M block rows, 36.9M blocks), 1..16 right-hand sides N(0,1) each,
+    FP32 ISAI, graph-replayed BiCGSTAB to 1e-10.  Checked with the oracle: the FP64
+    residual of 16 pairs spanning L and the right-hand-side count, and the oracle's
+    single-system BiCGSTAB on three systems (smallest, median and largest L)."""
+    print("batch full size 256 systems L 24..200", flush=True)
+    rp, ci, val, sys_ptr, Ls, nrhs = problems.batch(256, 24, 200, seed=0, rhs_max=16)
+    A = Matrix(rp, ci, val)
+    bt = Batch(A, sys_ptr, nrhs, precision="fp32", stream=stream)
+    try:
+        B = _rhs(sys_ptr, nrhs, 1)
+        d_B = DeviceArray.from_numpy(bt.pack(B))
+        d_X = DeviceArray.zeros((bt.length,), np.float64)
+        st, iters, rr = bt.solve(d_B, d_X, tol=1e-10, maxit=1000, use_graph=True)
+        assert st == 0 and (rr <= 1e-10).all() and len(rr) == int(nrhs.sum())
+        X = bt.unpack(d_X.to_numpy())
+        first = np.concatenate([[0], np.cumsum(nrhs)[:-1]])  # pair index of (s, 0)
+        order = np.argsort(Ls, kind="stable")
+        picks = [int(order[k]) for k in np.linspace(0, len(order) - 1, 8).round().astype(int)]
+        checked = 0
+        for s in picks:
+            srp, sci, sval = _system(rp, ci, val, sys_ptr, s)
+            for j in sorted({0, int(nrhs[s]) - 1}):
+                assert O.rel_residual(srp, sci, sval, B[s][j], X[s][j]) <= 1e-10, (s, j)
+                checked += 1
+        assert checked >= 9
+        # sixteen pairs at least: the first right-hand side of eight more systems
+        for s in [int(order[k]) for k in np.linspace(1, len(order) - 2, 8).round().astype(int)]:
+            srp, sci, sval = _system(rp, ci, val, sys_ptr, s)
+            assert O.rel_residual(srp, sci, sval, B[s][0], X[s][0]) <= 1e-10, s
+            checked += 1
+        assert checked >= 16
+        for s in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
+            srp, sci, sval = _system(rp, ci, val, sys_ptr, s)
+            lu, L, U = O.isai_factors(srp, sci, sval)
+            P = O.make_precond(1, len(srp) - 1, 4, L=L, U=U, f32=True)
+            xo, sto, ito, _ = O.bicgstab(srp, sci, sval, P, B[s][0], tol=1e-10, maxit=1000)
+            print(f"  system {s} L={Ls[s]}: batch {iters[first[s]]} it, oracle {ito} it", flush=True)
+            assert sto == 0 and rel_err(X[s][0], xo) <= 1e-8
+            # BiCGSTAB's count moves with the rounding of the preconditioner: on these
+            # three systems the oracle itself takes 20-27 iterations over FP64 / FP32 /
+            # BF16 factors and three right-hand sides (e.g. 22 / 23 / 20 for the L = 24
+            # system), so the batch (another FP32 summation order) is held to 20 %
+            assert abs(int(iters[first[s]]) - ito) <= max(2, 0.2 * ito)
+    finally:
+        bt.close()

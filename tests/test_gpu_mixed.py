@@ -101,3 +101,34 @@ def test_mixed_solve_full_size_96cube(gpu):
     assert st == 0 and O.rel_residual(rp, ci, val, b, x) <= 1e-10
     assert np.max(np.abs(x - 1.0)) <= 1e-8
     h.close()
+
+
+@pytest.mark.parametrize("form", [("fp32", "fgmres"), ("bf16", "fgmres"), ("bf16", "fgmres_mixed")])
+def test_precision_form_64cube_against_oracle_reference(gpu, form):
+    """The precision form at its own size (3-D 27-pt convection-diffusion 64^3,
+    FGMRES(30) to 1e-10, b = isai_gen_normal(1)) against the oracle's FP64
+    FGMRES(30) solution (sampled entries and projections recorded by
+    tests/golden/make_precision_reference.py).  Both residuals are <= 1e-10, so the
+    solutions differ by at most about cond * 2e-10; the oracle's own FGMRES and
+    BiCGSTAB solutions differ by 3.1e-8 relative here, and 1e-6 leaves a factor 30."""
+    import json
+    from pathlib import Path
+    prec, solver = form
+    print(f"precision form 64^3 {prec} {solver}", flush=True)
+    ref = json.loads((Path(__file__).resolve().parent / "golden" / "precision_64cube_oracle.json").read_text())
+    rp, ci, val = problems.generate(2, 64, seed=0)
+    A = Matrix(rp, ci, val)
+    assert A.n == ref["n"]
+    d_b, d_x = DeviceArray.from_numpy(problems.normal(1, A.n)), DeviceArray.zeros((A.n,), np.float64)
+    h = Handle(A, precond="isai", precision=prec)
+    st, it, rr = h.solve(d_b, d_x, solver, tol=1e-10, maxit=3000, restart=30)
+    x = d_x.to_numpy()
+    assert st == 0 and rr <= 1e-10
+    idx = np.asarray(ref["sample_index"])
+    err = np.max(np.abs(x[idx] - np.asarray(ref["x_samples"]))) / ref["x_norm_inf"]
+    rng = np.random.default_rng(ref["sample_seed"] + 1)
+    proj = [float(np.dot(rng.standard_normal(A.n), x)) for _ in range(len(ref["projections"]))]
+    perr = max(abs(a - b) / abs(b) for a, b in zip(proj, ref["projections"]))
+    print(f"  {it} it, sample err {err:.2e}, projection err {perr:.2e}", flush=True)
+    assert err <= 1e-6 and perr <= 1e-6
+    h.close()

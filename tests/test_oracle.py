@@ -472,6 +472,34 @@ def test_fgmres_mixed_inner_solve_has_the_fp32_floor():
     assert it32 == 12 and rr32 <= 1e-10, rr32
 
 
+def test_differentiation_config_matrix_as_specified_is_indefinite():
+    """The differentiation form's matrix as its text specifies it (2-D 9-point
+    stencil with the coefficients of the 5-point plain case: diagonal 4.5 I + S
+    against eight neighbours -I U(0.9, 1.1)) is indefinite: at L = 24 the real
+    parts of its eigenvalues span about -3.41 .. 8.47.  The oracle's
+    ISAI-preconditioned BiCGSTAB and FGMRES(30) do not converge on it, so x = A^-1 b
+    and its gradient are not computable by the specified method; the form runs on
+    the same stencil with the batching form's coefficients (diagonal 8.5 I), whose
+    eigenvalues all have positive real parts and where the oracle converges."""
+    rp, ci, val = problems.generate(4, 24, seed=0)
+    ev = np.linalg.eigvals(O.bsr_to_dense(rp, ci, val)).real
+    assert -3.5 < ev.min() < -3.3 and 8.4 < ev.max() < 8.55 and (ev < 0).sum() > 100
+    _, L, U = O.isai_factors(rp, ci, val)
+    P = O.make_precond(1, len(rp) - 1, 4, L=L, U=U)
+    b = problems.normal(1, len(ev))
+    _, st, it, rr = O.bicgstab(rp, ci, val, P, b, tol=1e-12, maxit=500)
+    assert st != 0 and rr > 1.0
+    _, st, it, rr = O.fgmres(rp, ci, val, P, b, m=30, tol=1e-12, maxit=500)
+    assert st != 0 and rr > 0.5
+    rp3, ci3, val3 = problems.generate(3, 24, seed=0)
+    ev3 = np.linalg.eigvals(O.bsr_to_dense(rp3, ci3, val3)).real
+    assert ev3.min() > 0.4
+    _, L3, U3 = O.isai_factors(rp3, ci3, val3)
+    P3 = O.make_precond(1, len(rp3) - 1, 4, L=L3, U=U3)
+    _, st, it, rr = O.bicgstab(rp3, ci3, val3, P3, b, tol=1e-12, maxit=500)
+    assert st == 0 and rr <= 1e-12
+
+
 # ---------------------------------------------------------- differentiation form
 def test_bsr_transpose_dense():
     nb, bs = 14, 4
