@@ -58,6 +58,24 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
     return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
+// BF16 halves of a 32-bit word as floats.
+__device__ __forceinline__ float bf16_lo(unsigned int w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(unsigned int w) { return __uint_as_float(w & 0xffff0000u); }
+
+// Lane K of each quad to all four lanes of the quad, and the swap of the two lanes
+// of each pair (DPP quad_perm [1,0,3,2]).
+template <int K>
+__device__ __forceinline__ float quad_bcast_f(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K | (K << 2) | (K << 4) | (K << 6), 0xF, 0xF, false));
+}
+__device__ __forceinline__ float pair_swap_f(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
+// Block rows per 8-lane group of the reduced-precision products for rows of at most
+// two 128-byte lines of values (several rows keep enough loads in flight).
+constexpr int kLowpRowsF32 = 2, kLowpRowsBF16 = 4;
+
 inline unsigned grid_for(long long work, int per_block) {
     return (unsigned)((work + per_block - 1) / per_block);
 }

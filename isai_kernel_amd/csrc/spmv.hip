@@ -178,8 +178,6 @@ int spmv_dot_launch(int mb, const int *rp, const int *ci, const double *val, con
 // within it, element run (s % LPB) * VPL .. + VPL - 1 (row-major).  Products
 // accumulate in FP32; lanes that share a row are summed with cross-lane adds at
 // the end (x is FP64 in memory and rounded to FP32 on load).
-__device__ __forceinline__ float bf16_lo(unsigned int w) { return __uint_as_float(w << 16); }
-__device__ __forceinline__ float bf16_hi(unsigned int w) { return __uint_as_float(w & 0xffff0000u); }
 
 // x of one block column rounded to FP32, from an FP64 vector (two 16-B loads) or
 // from an FP32 vector (one 16-B load); the two give the same floats.
@@ -199,13 +197,6 @@ __device__ __forceinline__ void load_x4(const void *__restrict__ x, int c, float
 // The same four floats for the LPB lanes that share a block (lanes of one quad, or
 // one pair of it): each lane loads only its own part of the block column and the
 // lanes exchange the rest with DPP moves, instead of every lane loading all of it.
-template <int K>
-__device__ __forceinline__ float quad_bcast_f(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K | (K << 2) | (K << 4) | (K << 6), 0xF, 0xF, false));
-}
-__device__ __forceinline__ float pair_swap_f(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
-}
 template <int LPB, bool XF32>
 __device__ __forceinline__ void load_x4_split(const void *__restrict__ x, int c, int sub, float f[4]) {
     if (LPB == 4) {  // lane sub loads component sub
@@ -242,9 +233,8 @@ __device__ __forceinline__ void store_y(void *__restrict__ y, size_t i, float v)
 // BF16 60.5 us with 4 rows (63.5 with 2, 66.9 with 8); FP32 120.7 us with 2 rows
 // (119.8 with 4, 131.0 with 1).  On the 64^3 27-pt factors (14 blocks per row) one
 // row already keeps enough loads in flight and several rows per group are slower, so
-// the launcher uses several rows only for rows of at most two lines of values.
-constexpr int kLowpRowsF32 = 2, kLowpRowsBF16 = 4;
-
+// the launcher uses several rows only for rows of at most two lines of values
+// (kLowpRowsBF16 / kLowpRowsF32).
 template <bool BF16, int kLowpRows, bool XF32, bool YF32>
 __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci, const v4u *__restrict__ val16,
