@@ -43,8 +43,8 @@ TOL = 1e-10
 # Bytes per launch of the FP32 A product (two dots) at 96^3 beyond the L2 (HBM and
 # Infinity Cache), from the FETCH_SIZE / WRITE_SIZE counter passes summarised in
 # profiles/hbm_traffic_96cube_7pt_pmc.md (FETCH_SIZE x2 by the copy-kernel
-# calibration); None until measured.
-F32_SPMV_TRAFFIC_BYTES_96CUBE = None
+# calibration): 233,667.1 KiB x 2 + 13,955.7 KiB.
+F32_SPMV_TRAFFIC_BYTES_96CUBE = 233667.1 * 2 * 1024 + 13955.7 * 1024
 
 
 def log(msg: str) -> None:
@@ -324,6 +324,8 @@ def main() -> int:
                     help="run only this form's leg and print its object (for profiler runs)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the kernels only (no solves, no CPU leg): for profiler runs")
+    ap.add_argument("--solve-forms", default="",
+                    help="comma-separated subset of the timed solve forms (default: all): for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1 "
@@ -509,6 +511,9 @@ def main() -> int:
     forms = {"mixed_bf16": ("isai", "bf16", "fgmres_mixed"), "mixed_fp32": ("isai", "fp32", "fgmres_mixed"),
              "isai_bf16": ("isai", "bf16", "bicgstab"), "isai_fp32": ("isai", "fp32", "bicgstab"),
              "isai_fp64": ("isai", "fp64", "bicgstab"), "ilu_sptrsv_fp64": ("ilu", "fp64", "bicgstab")}
+    if args.solve_forms:
+        keep = args.solve_forms.split(",")
+        forms = {k: v for k, v in forms.items() if k in keep}
     for form, (pc, prec, solver) in forms.items():
         h.set_preconditioner(pc, prec)
         state = {}
