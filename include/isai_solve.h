@@ -111,6 +111,16 @@ int isai_set_inner_solve(isai_handle_t handle, double inner_tol, int inner_maxit
  * solve (0 and the iteration count for the other solvers). */
 int isai_last_solve_stats(isai_handle_t handle, int *outer_steps, int *inner_iters);
 
+/* Inspection of the last FGMRES cycle (no reference counterpart; for tests of the
+ * Arnoldi relation A Z_k = V_{k+1} Hbar_k): *m = the cycle length of the last
+ * FGMRES solve, *steps = the Arnoldi steps k of its last cycle, *d_V = the m + 1
+ * basis vectors and *d_Z = the m preconditioned vectors (device, n doubles each,
+ * consecutive), Hbar = the (m + 1) x m Hessenberg matrix before the Givens
+ * rotations, row-major (capacity >= (m + 1) m doubles; may be null).  For the
+ * mixed-precision solver Z holds the inner solutions.  Synchronizes the device. */
+int isai_fgmres_arnoldi(isai_handle_t handle, int *m, int *steps, const double **d_V, const double **d_Z,
+                        double *Hbar, int capacity);
+
 int isai_destroy(isai_handle_t handle);
 
 #ifdef __cplusplus

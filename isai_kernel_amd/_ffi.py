@@ -127,6 +127,7 @@ SIGNATURES: dict[str, list] = {
     "isai_destroy": [P],
     "isai_set_inner_solve": [P, D, I],
     "isai_last_solve_stats": [P, P, P],
+    "isai_fgmres_arnoldi": [P, P, P, P, P, P, I],
     # isai_adjoint.h
     "isai_bsr_transpose": [I, I, I, P, P, P, P, P, P, P],
     "isai_adjoint_handle": [P, P, P],

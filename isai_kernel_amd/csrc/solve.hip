@@ -41,6 +41,7 @@ struct Scalars {
     double rho, rho_old, alpha, omega, beta;
     double bnorm2, rnorm2, snorm2, tol2;
     double H[(kMaxRestart + 1) * kMaxRestart];
+    double Hbar[(kMaxRestart + 1) * kMaxRestart];  // H before the Givens rotations (Arnoldi relation)
     double cs[kMaxRestart], sn[kMaxRestart], g[kMaxRestart + 1], y[kMaxRestart];
     double hcur[kMaxRestart + 1];  // Gram-Schmidt coefficients of the current pass
     // 0 running, 1 converged, 2 converged at s (BiCGSTAB), 3 breakdown, 4 iteration limit
@@ -497,6 +498,7 @@ __global__ __launch_bounds__(kRedThreads) void fg_givens_final(Scalars *S, int j
     constexpr int m = kMaxRestart;
     const double hn = sqrt(w2);
     S->H[(j + 1) * m + j] = hn;
+    for (int i = 0; i <= j + 1; ++i) S->Hbar[i * m + j] = S->H[i * m + j];
     for (int i = 0; i < j; ++i) {
         const double a = S->H[i * m + j], c = S->H[(i + 1) * m + j];
         S->H[i * m + j] = S->cs[i] * a + S->sn[i] * c;
@@ -595,6 +597,7 @@ struct isai_handle {
     int inner_hist[64] = {};       // iterations the inner solve of outer step j took last time
     int outer_pred = 0;            // outer steps the last converged solve took (0 = unknown)
     int last_outer = 0, last_inner = 0;
+    int last_restart = 0;          // FGMRES cycle length of the last FGMRES solve
 };
 
 namespace {
@@ -772,6 +775,7 @@ int fgmres(isai_handle *h, int m, double tol, int maxit, const double *b, double
         return ISAI_ERR_INVALID_ARG;
     }
     TRY(ensure_work(h, (size_t)2 * m + 3));
+    h->last_restart = m;
     const size_t n = h->n;
     double *V = h->work, *Z = V + n * (m + 1), *w = Z + n * m, *tmp = w + n;
     const int *skip = &h->S->done;
@@ -933,6 +937,7 @@ int fgmres_mixed(isai_handle *h, int m, double tol, int maxit, const double *b, 
     TRY(convert_lp(h, lp, st));
     TRY(ensure_work(h, (size_t)2 * m + 3));
     TRY(ensure_inner(h, st));
+    h->last_restart = m;
     const InnerPrecond M{(isai_precision_t)lp, h->nnzbL, h->nnzbU, h->d_Lrow, h->d_Lcol, h->d_Urow, h->d_Ucol,
                          h->d_Llp, h->d_Ulp};
     const size_t n = h->n;
@@ -1558,6 +1563,26 @@ int isai_last_solve_stats(isai_handle_t h, int *outer_steps, int *inner_iters) {
     ISAI_REQUIRE(h, "isai_last_solve_stats: null handle");
     if (outer_steps) *outer_steps = h->last_outer;
     if (inner_iters) *inner_iters = h->last_inner;
+    return ISAI_OK;
+}
+
+int isai_fgmres_arnoldi(isai_handle_t h, int *m, int *steps, const double **d_V, const double **d_Z, double *Hbar,
+                        int capacity) {
+    ISAI_REQUIRE(h && m && steps, "isai_fgmres_arnoldi: null argument");
+    ISAI_REQUIRE(h->last_restart > 0 && h->work, "isai_fgmres_arnoldi: the handle has run no FGMRES solve");
+    const int mm = h->last_restart;
+    *m = mm;
+    ISAI_HIP(hipDeviceSynchronize());
+    ISAI_HIP(hipMemcpy(steps, &h->S->jend, sizeof(int), hipMemcpyDeviceToHost));
+    if (d_V) *d_V = h->work;
+    if (d_Z) *d_Z = h->work + h->n * (mm + 1);
+    if (Hbar) {
+        ISAI_REQUIRE(capacity >= (mm + 1) * mm, "isai_fgmres_arnoldi: Hbar needs (m + 1) m = %d doubles", (mm + 1) * mm);
+        std::vector<double> full((size_t)(kMaxRestart + 1) * kMaxRestart);
+        ISAI_HIP(hipMemcpy(full.data(), h->S->Hbar, sizeof(double) * full.size(), hipMemcpyDeviceToHost));
+        for (int i = 0; i <= mm; ++i)
+            for (int j = 0; j < mm; ++j) Hbar[i * mm + j] = i <= j + 1 ? full[(size_t)i * kMaxRestart + j] : 0.0;
+    }
     return ISAI_OK;
 }
 

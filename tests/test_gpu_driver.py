@@ -42,3 +42,36 @@ def test_driver_flow_and_solve(gpu, args):
     assert "Converted to BSR with block size 4" in out
     assert _value(out, "[M_U M_L apply] rel_residual") < 1.0  # one application reduces the residual
     assert _value(out, "[solve] rel_residual") <= 1e-10
+
+
+def test_driver_matrix_market_input_equals_generated(gpu, tmp_path):
+    """The reference's drop-in usage, `isai_driver matrix.mtx` (main.cu:450-455,483):
+    the plain small case (problem 0) written to a MatrixMarket file through the
+    device BSR -> CSR expansion and read back by the driver gives the same setup,
+    apply and solve figures as the generated matrix."""
+    import numpy as np
+
+    from isai_kernel_amd import DeviceArray, call, problems
+    rp, ci, val = problems.generate(0, 0, seed=0)
+    nb, nnzb = len(rp) - 1, len(ci)
+    n, nnz = 4 * nb, 16 * nnzb
+    d = [DeviceArray.from_numpy(a) for a in (rp, ci, val)]
+    d_crp, d_cci, d_cv = DeviceArray((n + 1,), np.int32), DeviceArray((nnz,), np.int32), DeviceArray((nnz,), np.float64)
+    call("isai_bsr_to_csr", nb, nnzb, 4, d[0].ptr, d[1].ptr, d[2].ptr, d_crp.ptr, d_cci.ptr, d_cv.ptr, None)
+    call("isai_device_synchronize")
+    crp, cci, cv = d_crp.to_numpy(), d_cci.to_numpy(), d_cv.to_numpy()
+    rows = np.repeat(np.arange(n), np.diff(crp))
+    path = tmp_path / "problem0.mtx"
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real general\n% problem 0, 2-D 5-pt 64^2, bs = 4\n")
+        f.write(f"{n} {n} {nnz}\n")
+        f.writelines(f"{r + 1} {c + 1} {v!r}\n" for r, c, v in zip(rows.tolist(), cci.tolist(), cv.tolist()))
+    out_file = _run(str(path), "--iters", "3")
+    out_gen = _run("--problem", "0", "--iters", "3")
+    assert f"Read matrix {path} of size {n} x {n}, nnz = {nnz}" in out_file
+    for tag in ("[M_U M_L apply] rel_residual", "[M_U M_L apply] rel_error_x", "[solve] rel_residual",
+                "[solve] rel_error_x"):
+        assert _value(out_file, tag) == _value(out_gen, tag), tag
+    it = [re.search(r"iterations = (\d+)", o).group(1) for o in (out_file, out_gen)]
+    assert it[0] == it[1]
+    assert _value(out_file, "[solve] rel_residual") <= 1e-10

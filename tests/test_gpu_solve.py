@@ -131,3 +131,43 @@ def test_bicgstab_full_size_fp64_and_mixed(gpu):
         res[prec] = it
     print(f"  iterations {res}", flush=True)
     h.close()
+
+
+@pytest.mark.parametrize("solver,prec", [("fgmres", "fp64"), ("fgmres", "bf16"), ("fgmres_mixed", "bf16")])
+def test_fgmres_arnoldi_relation(gpu, solver, prec):
+    """One full FGMRES cycle (m = 10, no early stop) satisfies the flexible Arnoldi
+    relation A Z_m = V_{m+1} Hbar_m (Saad, Iterative Methods, 2nd ed., Prop. 9.2)
+    to 1e-12, with the basis V orthonormal to 1e-12 (CGS2 with the DGKS criterion);
+    Z holds the preconditioned vectors (the mixed solver: the inner solutions)."""
+    print(f"arnoldi relation {solver} {prec}", flush=True)
+    rp, ci, val = problems.generate(2, 8, seed=4)
+    A = Matrix(rp, ci, val)
+    h = Handle(A, precond="isai", precision=prec)
+    b = problems.normal(3, A.n)
+    d_b, d_x = DeviceArray.from_numpy(b), DeviceArray.zeros((A.n,), np.float64)
+    m = 10
+    if solver == "fgmres_mixed":
+        # two inner iterations per step keep the outer estimate |g_m| above tol ||b||
+        # through the cycle (with three it falls below 1e-30 ||b|| at the last step, the
+        # cycle ends as converged and V_m is not formed)
+        h.set_inner(1e-5, 2)
+        maxit = 2 * m
+    else:
+        maxit = m
+    h.solve(d_b, d_x, solver, tol=1e-30, maxit=maxit, restart=m, check=False)
+    mm, steps = ctypes.c_int(), ctypes.c_int()
+    pv, pz = ctypes.c_void_p(), ctypes.c_void_p()
+    hbar = np.zeros((m + 1) * m)
+    call("isai_fgmres_arnoldi", h.h, ctypes.byref(mm), ctypes.byref(steps), ctypes.byref(pv), ctypes.byref(pz),
+         hbar.ctypes.data, hbar.size)
+    assert mm.value == m and steps.value == m
+    V = _download(pv.value, (m + 1, A.n), np.float64)
+    Z = _download(pz.value, (m, A.n), np.float64)
+    Hb = hbar.reshape(m + 1, m)
+    AZ = np.stack([O.spmv(rp, ci, val, Z[j]) for j in range(m)])
+    VH = Hb.T @ V  # row j = sum_i Hbar[i, j] V_i
+    for j in range(m):
+        assert np.linalg.norm(AZ[j] - VH[j]) <= 1e-12 * np.linalg.norm(AZ[j]), j
+    assert np.max(np.abs(V @ V.T - np.eye(m + 1))) <= 1e-12
+    assert np.all(np.tril(Hb, -2) == 0) and np.all(np.diag(Hb, -1) > 0)  # upper Hessenberg, h_{j+1,j} = ||w||
+    h.close()
