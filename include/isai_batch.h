@@ -16,11 +16,18 @@
  *
  * Solve: right-preconditioned BiCGSTAB with one set of Krylov scalars per
  * (system, right-hand side) pair, each pair stopping on its own; FP64 Krylov,
- * ISAI factors in the chosen storage precision.  One iteration of the whole
- * batch is captured once as a hipGraph and replayed; the host reads one word
- * (pairs still running) per replay.  The multi-RHS BSR product loads every block
- * once for all right-hand sides of its system; its launch shape is chosen at
- * setup by timing the candidates on the batch itself (isai_batch_tuning).
+ * ISAI factors in the chosen storage precision.  Two schedules compute the same
+ * iterates bit for bit:
+ *   - per-step launches: the steps of four iterations of the whole batch are
+ *     captured once as a hipGraph and replayed, the host reading one word (pairs
+ *     still running) per replay;
+ *   - one persistent kernel: a workgroup per CU runs the whole solve (restarts
+ *     included), the steps separated by grid barriers whose waits are bounded
+ *     (isai_batch_set_spin_limit; an abandoned wait returns ISAI_ERR_TIMEOUT);
+ *     the launch is captured once as a hipGraph and replayed per solve.
+ * The multi-RHS BSR product loads every block once for all right-hand sides of
+ * its system; its launch shape is chosen at setup by timing the candidates on the
+ * batch itself (isai_batch_tuning).
  */
 #ifndef ISAI_BATCH_H
 #define ISAI_BATCH_H
@@ -59,9 +66,14 @@ int isai_batch_spmm(isai_batch_t batch, const double *d_X, double *d_Y, isai_str
 /* Solve A_s x_{s,j} = b_{s,j} for every pair; d_X holds initial guesses.
  * iters[npairs] and relres[npairs] (host arrays, may be null) receive each pair's
  * iteration count and true FP64 relative residual; *max_iters the largest count.
- * use_graph != 0 replays a captured hipGraph of one batch iteration. */
+ * use_graph: 0 per-step launches, 1 per-step launches replayed as a captured
+ * hipGraph, 2 the persistent kernel replayed as a captured hipGraph, 3 the
+ * persistent kernel launched directly.  Modes 1 and 2 need a created stream. */
 int isai_batch_solve(isai_batch_t batch, double tol, int maxit, const double *d_B, double *d_X, int use_graph,
                      int *iters, double *relres, int *max_iters, isai_stream_t stream);
+/* Bound on one grid-barrier wait of the persistent kernel, microseconds (default
+ * 5 s); a wait beyond it makes the solve return ISAI_ERR_TIMEOUT. */
+int isai_batch_set_spin_limit(isai_batch_t batch, long long us);
 
 int isai_batch_destroy(isai_batch_t batch);
 

@@ -36,8 +36,14 @@ struct PairState {
     int status, iters, first, pad2;
 };
 
+// nactive: pairs still running after an iteration (zeroed per iteration by the
+// graph path; a running total in the persistent kernel); nrestart: running total
+// of pairs sent back by a true-residual check; arrive / abort / spin_us: the
+// persistent kernel's grid barrier (arrivals, abandoned wait, wait bound in us).
 struct Ctl {
-    int nactive, maxit, pad[2];
+    int nactive, maxit, nrestart, abort;
+    unsigned long long arrive;
+    long long spin_us;
 };
 
 struct BatchDev {
@@ -52,6 +58,7 @@ struct BatchDev {
     PairState *ps;              // [npairs]
     double *part0, *part1;      // [ntiles]
     Ctl *ctl;
+    int ntiles, npairs;
 };
 
 __device__ __forceinline__ unsigned active_mask(const BatchDev &B, int p0, int r, int all) {
@@ -74,10 +81,10 @@ struct RowCtx {
     long long v0, step;
 };
 
-__device__ __forceinline__ bool row_context(const BatchDev &B, const int *rp, int nb, int remap, int all, RowCtx &q,
-                                            int &r) {
-    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    q.br = wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3);
+// Block row br handled by an 8-lane group; lane = position in the group.
+__device__ __forceinline__ bool row_context(const BatchDev &B, const int *rp, int nb, int br, int lane, int all,
+                                            RowCtx &q, int &r) {
+    q.br = br;
     if (q.br >= nb) return false;  // whole 8-lane groups leave together
     const int sys = B.sys_of_row[q.br];
     r = B.sys_nrhs[sys];
@@ -88,7 +95,7 @@ __device__ __forceinline__ bool row_context(const BatchDev &B, const int *rp, in
     q.step = B.sys_step[sys];
     q.rs = rp[q.br];
     q.re = rp[q.br + 1];
-    q.lane = threadIdx.x & 7;
+    q.lane = lane;
     return true;
 }
 
@@ -297,16 +304,14 @@ __device__ __forceinline__ void mfma_row(const BatchDev &B, const int *rp, int n
     m.len = rp[br + 1] - m.rs;
 }
 
-__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_f64_kernel(int nb, const int *__restrict__ rp,
-                                                                    const int *__restrict__ ci,
-                                                                    const double *__restrict__ val, BatchDev B,
-                                                                    const double *__restrict__ X,
-                                                                    double *__restrict__ Y, int remap, int all) {
-    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    const int lane = threadIdx.x & 63;
+// Wave unit u: block rows u * kMfmaF64Rows .. + 3 (lane = lane in the wave).
+__device__ __forceinline__ void mfma_f64_unit(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+                                              const double *__restrict__ val, const BatchDev &B,
+                                              const double *__restrict__ X, double *__restrict__ Y, int u, int lane,
+                                              int all) {
     const int x = lane & 3, b = (lane >> 2) & 3, y = lane >> 4;
     MfmaRow m;
-    mfma_row(B, rp, nb, (wg * (int)(blockDim.x / 64) + (int)(threadIdx.x >> 6)) * kMfmaF64Rows + b, all, m);
+    mfma_row(B, rp, nb, u * kMfmaF64Rows + b, all, m);
     // wave-uniform bounds: the longest row and the most right-hand-side groups
     int L = m.len, G = m.len ? (m.R + 3) >> 2 : 0;
     L = max(L, __shfl_xor(L, 4));
@@ -361,22 +366,30 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_f64_kernel(int nb, cons
     }
 }
 
+__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_f64_kernel(int nb, const int *__restrict__ rp,
+                                                                    const int *__restrict__ ci,
+                                                                    const double *__restrict__ val, BatchDev B,
+                                                                    const double *__restrict__ X,
+                                                                    double *__restrict__ Y, int remap, int all) {
+    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    mfma_f64_unit(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 64) + (int)(threadIdx.x >> 6),
+                  (int)(threadIdx.x & 63), all);
+}
+
 // v_mfma_f32_4x4x1f32 (sixteen 4x4x1 products): A[b][i] at lane i + 4b, B[b][j] at
 // lane j + 4b, C[b][i][j] in element i of lane j + 4b.  One wave takes sixteen
 // block rows; lane (i, b) loads row i of its block (16 B FP32, 8 B BF16) and, as a
 // B-operand lane (j, b), the four x values of right-hand side j of each group
 // (rounded to FP32); four K = 1 MFMAs per group and block, FP32 accumulation.
+// Wave unit u: block rows u * kMfmaLowpRows .. + 15.
 template <bool BF16>
-__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_lowp_kernel(int nb, const int *__restrict__ rp,
-                                                                     const int *__restrict__ ci,
-                                                                     const void *__restrict__ val, BatchDev B,
-                                                                     const double *__restrict__ X,
-                                                                     double *__restrict__ Y, int remap, int all) {
-    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    const int lane = threadIdx.x & 63;
+__device__ __forceinline__ void mfma_lowp_unit(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+                                               const void *__restrict__ val, const BatchDev &B,
+                                               const double *__restrict__ X, double *__restrict__ Y, int u, int lane,
+                                               int all) {
     const int q = lane & 3, b = lane >> 2;  // q: row i for A, right-hand side j for B and C
     MfmaRow m;
-    mfma_row(B, rp, nb, (wg * (int)(blockDim.x / 64) + (int)(threadIdx.x >> 6)) * kMfmaLowpRows + b, all, m);
+    mfma_row(B, rp, nb, u * kMfmaLowpRows + b, all, m);
     int L = m.len, G = m.len ? (m.R + 3) >> 2 : 0;
 #pragma unroll
     for (int off = 4; off < 64; off <<= 1) {
@@ -458,19 +471,29 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_lowp_kernel(int nb, con
     }
 }
 
+template <bool BF16>
+__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_lowp_kernel(int nb, const int *__restrict__ rp,
+                                                                     const int *__restrict__ ci,
+                                                                     const void *__restrict__ val, BatchDev B,
+                                                                     const double *__restrict__ X,
+                                                                     double *__restrict__ Y, int remap, int all) {
+    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    mfma_lowp_unit<BF16>(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 64) + (int)(threadIdx.x >> 6),
+                         (int)(threadIdx.x & 63), all);
+}
+
 #define ISAI_RHS_CASES(CALL) \
     CALL(1) CALL(2) CALL(3) CALL(4) CALL(5) CALL(6) CALL(7) CALL(8) \
     CALL(9) CALL(10) CALL(11) CALL(12) CALL(13) CALL(14) CALL(15) CALL(16)
 
 // MAP 0: row-pair mapping (f64_row / lowp_row); MAP 1: column-split mapping.
 template <int MAP>
-__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64_kernel(int nb, const int *__restrict__ rp,
-                                                               const int *__restrict__ ci, const v2d *__restrict__ val,
-                                                               BatchDev B, const double *__restrict__ X,
-                                                               double *__restrict__ Y, int remap, int all) {
+__device__ __forceinline__ void f64_unit(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+                                         const v2d *__restrict__ val, const BatchDev &B, const double *__restrict__ X,
+                                         double *__restrict__ Y, int br, int lane, int all) {
     RowCtx q;
     int r;
-    if (!row_context(B, rp, nb, remap, all, q, r)) return;
+    if (!row_context(B, rp, nb, br, lane, all, q, r)) return;
     switch (r) {
 #define ISAI_F64_CASE(N)                          \
     case N:                                       \
@@ -484,15 +507,24 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64_kernel(int nb, const int
     }
 }
 
+template <int MAP>
+__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64_kernel(int nb, const int *__restrict__ rp,
+                                                               const int *__restrict__ ci, const v2d *__restrict__ val,
+                                                               BatchDev B, const double *__restrict__ X,
+                                                               double *__restrict__ Y, int remap, int all) {
+    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    f64_unit<MAP>(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3),
+                  (int)(threadIdx.x & 7), all);
+}
+
 template <bool BF16, int MAP>
-__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_lowp_kernel(int nb, const int *__restrict__ rp,
-                                                                const int *__restrict__ ci,
-                                                                const v4u *__restrict__ val16, BatchDev B,
-                                                                const double *__restrict__ X, double *__restrict__ Y,
-                                                                int remap, int all) {
+__device__ __forceinline__ void lowp_unit(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+                                          const v4u *__restrict__ val16, const BatchDev &B,
+                                          const double *__restrict__ X, double *__restrict__ Y, int br, int lane,
+                                          int all) {
     RowCtx q;
     int r;
-    if (!row_context(B, rp, nb, remap, all, q, r)) return;
+    if (!row_context(B, rp, nb, br, lane, all, q, r)) return;
     switch (r) {
 #define ISAI_LOWP_CASE(N)                                   \
     case N:                                                 \
@@ -506,25 +538,41 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_lowp_kernel(int nb, const in
     }
 }
 
-// --------------------------------------------------------------- reductions
-constexpr int kRed = 256;  // threads of the per-tile kernels
+template <bool BF16, int MAP>
+__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_lowp_kernel(int nb, const int *__restrict__ rp,
+                                                                const int *__restrict__ ci,
+                                                                const v4u *__restrict__ val16, BatchDev B,
+                                                                const double *__restrict__ X, double *__restrict__ Y,
+                                                                int remap, int all) {
+    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    lowp_unit<BF16, MAP>(nb, rp, ci, val16, B, X, Y, wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3),
+                         (int)(threadIdx.x & 7), all);
+}
 
-__device__ __forceinline__ double wg_sum(double v) {
-    __shared__ double ws[kRed / 64];
+// --------------------------------------------------------------- reductions
+constexpr int kRed = 256;               // threads per tile
+constexpr int kPersistThreads = 1024;   // persistent kernel: four tile groups of kRed threads
+
+// Sum of v over the kRed threads of tile group `sub` (threads sub*kRed .. +kRed-1),
+// in a fixed order.  Every thread of the workgroup calls it the same number of
+// times (it holds two workgroup barriers).
+__device__ __forceinline__ double wg_sum(double v, int sub) {
+    __shared__ double ws[kPersistThreads / 64];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
     __syncthreads();
-    return (ws[0] + ws[1]) + (ws[2] + ws[3]);
+    const double *w = ws + (kRed / 64) * sub;
+    return (w[0] + w[1]) + (w[2] + w[3]);
 }
 
-// The per-pair finals run one wave per pair (grid = npairs, 64 threads): the wave
-// sums the pair's tile partials lane-strided, then by a fixed butterfly, so every
-// lane holds the same, deterministic totals; lane 0 updates the pair's state.
-__device__ __forceinline__ void pair_sums(const BatchDev &B, int p, double &s0, double &s1) {
+// The per-pair finals run one wave per pair: the wave sums the pair's tile partials
+// lane-strided, then by a fixed butterfly, so every lane holds the same,
+// deterministic totals; lane 0 updates the pair's state.
+__device__ __forceinline__ void pair_sums(const BatchDev &B, int p, int lane, double &s0, double &s1) {
     double a = 0.0, c = 0.0;
-    for (int t = B.ptile[p] + (int)threadIdx.x; t < B.ptile[p + 1]; t += 64) {
+    for (int t = B.ptile[p] + lane; t < B.ptile[p + 1]; t += 64) {
         a += B.part0[t];
         c += B.part1[t];
     }
@@ -537,22 +585,31 @@ __device__ __forceinline__ void pair_sums(const BatchDev &B, int p, double &s0, 
     s1 = c;
 }
 
+// Tile t of a running pair (t may lie past the last tile in the persistent kernel,
+// whose tile groups step through the tiles together).
+__device__ __forceinline__ bool tile_live(const BatchDev &B, int t) {
+    return t < B.ntiles && B.ps[B.tile_pair[t]].status == 0;
+}
+
 // part0[t] = a.b and part1[t] = a.c (c optional) over tile t, for running pairs.
-__global__ __launch_bounds__(kRed) void tile_dot_kernel(const double *__restrict__ a, const double *__restrict__ b,
-                                                        const double *__restrict__ c, BatchDev B) {
-    const int t = blockIdx.x;
-    if (B.ps[B.tile_pair[t]].status != 0) return;  // the pair's finals skip it too
+__device__ __forceinline__ void tile_dot_body(int t, int sub, const double *__restrict__ a,
+                                              const double *__restrict__ b, const double *__restrict__ c,
+                                              const BatchDev &B) {
+    const bool live = tile_live(B, t);  // the pair's finals skip it too
+    const int tid = threadIdx.x & (kRed - 1);
     const size_t base = (size_t)t * kTile;
     double s0 = 0.0, s1 = 0.0;
+    if (live) {
 #pragma unroll
-    for (int k = 0; k < kTile / kRed; ++k) {
-        const size_t i = base + k * kRed + threadIdx.x;
-        s0 = fma(a[i], b[i], s0);
-        if (c) s1 = fma(a[i], c[i], s1);
+        for (int k = 0; k < kTile / kRed; ++k) {
+            const size_t i = base + k * kRed + tid;
+            s0 = fma(a[i], b[i], s0);
+            if (c) s1 = fma(a[i], c[i], s1);
+        }
     }
-    s0 = wg_sum(s0);
-    if (c) s1 = wg_sum(s1);
-    if (threadIdx.x == 0) {
+    s0 = wg_sum(s0, sub);
+    if (c) s1 = wg_sum(s1, sub);
+    if (live && tid == 0) {
         B.part0[t] = s0;
         B.part1[t] = s1;
     }
@@ -561,41 +618,41 @@ __global__ __launch_bounds__(kRed) void tile_dot_kernel(const double *__restrict
 // ------------------------------------------------------------ BiCGSTAB kernels
 // Right-preconditioned BiCGSTAB (Saad, Iterative Methods, Alg. 7.7 with M = M_U M_L),
 // the same recurrences as the single-system solver, with one set of scalars per pair.
-__global__ void zero_active_kernel(Ctl *ctl) { ctl->nactive = 0; }
 
 // r = b - r (r holds A x), rhat = r, p = v = 0 for running pairs; partials ||b||^2, ||r||^2
-__global__ __launch_bounds__(kRed) void binit_kernel(const double *__restrict__ b, double *r, double *rh, double *p,
-                                                     double *v, BatchDev B) {
-    const int t = blockIdx.x;
-    if (B.ps[B.tile_pair[t]].status != 0) return;
+__device__ __forceinline__ void binit_body(int t, int sub, const double *__restrict__ b, double *r, double *rh,
+                                           double *p, double *v, const BatchDev &B) {
+    const bool live = tile_live(B, t);
+    const int tid = threadIdx.x & (kRed - 1);
     const size_t base = (size_t)t * kTile;
     double sb = 0.0, sr = 0.0;
+    if (live) {
 #pragma unroll
-    for (int k = 0; k < kTile / kRed; ++k) {
-        const size_t i = base + k * kRed + threadIdx.x;
-        const double bi = b[i], ri = bi - r[i];
-        r[i] = ri;
-        rh[i] = ri;
-        p[i] = 0.0;
-        v[i] = 0.0;
-        sb = fma(bi, bi, sb);
-        sr = fma(ri, ri, sr);
+        for (int k = 0; k < kTile / kRed; ++k) {
+            const size_t i = base + k * kRed + tid;
+            const double bi = b[i], ri = bi - r[i];
+            r[i] = ri;
+            rh[i] = ri;
+            p[i] = 0.0;
+            v[i] = 0.0;
+            sb = fma(bi, bi, sb);
+            sr = fma(ri, ri, sr);
+        }
     }
-    sb = wg_sum(sb);
-    sr = wg_sum(sr);
-    if (threadIdx.x == 0) {
+    sb = wg_sum(sb, sub);
+    sr = wg_sum(sr, sub);
+    if (live && tid == 0) {
         B.part0[t] = sb;
         B.part1[t] = sr;
     }
 }
 
-__global__ __launch_bounds__(64) void binit_final(BatchDev B, double tol) {
-    const int p = blockIdx.x;
+__device__ __forceinline__ void binit_final_body(const BatchDev &B, int p, int lane, double tol) {
     PairState &S = B.ps[p];
     if (S.status != 0) return;
     double b2, r2;
-    pair_sums(B, p, b2, r2);
-    if (threadIdx.x != 0) return;
+    pair_sums(B, p, lane, b2, r2);
+    if (lane != 0) return;
     S.bnorm2 = b2 > 0.0 ? b2 : 1.0;
     S.tol2 = tol * tol * S.bnorm2;
     S.rho_old = S.alpha = S.omega = 1.0;
@@ -603,17 +660,16 @@ __global__ __launch_bounds__(64) void binit_final(BatchDev B, double tol) {
     if (r2 <= S.tol2) S.status = 1;
 }
 
-__global__ __launch_bounds__(64) void brho_final(BatchDev B) {
-    const int p = blockIdx.x;
+__device__ __forceinline__ void brho_final_body(const BatchDev &B, int p, int lane) {
     PairState &S = B.ps[p];
     if (S.status != 0) return;
     if (S.iters >= B.ctl->maxit) {
-        if (threadIdx.x == 0) S.status = 5;
+        if (lane == 0) S.status = 5;
         return;
     }
-    double rho, unused;
-    pair_sums(B, p, rho, unused);
-    if (threadIdx.x != 0) return;
+    double unused, rho;
+    pair_sums(B, p, lane, unused, rho);  // part1 = rhat.r (from the r update, or ||r||^2 when rhat = r)
+    if (lane != 0) return;
     S.iters++;
     S.rho = rho;
     if (rho == 0.0) S.status = 3;
@@ -621,11 +677,11 @@ __global__ __launch_bounds__(64) void brho_final(BatchDev B) {
 }
 
 // p = r + beta (p - omega v)  (p = r on a pair's first iteration)
-__global__ __launch_bounds__(kRed) void bp_kernel(const double *__restrict__ r, double *__restrict__ p,
-                                                  const double *__restrict__ v, BatchDev B) {
-    const int t = blockIdx.x;
+__device__ __forceinline__ void bp_body(int t, const double *__restrict__ r, double *__restrict__ p,
+                                        const double *__restrict__ v, const BatchDev &B) {
+    if (!tile_live(B, t)) return;
     const PairState &S = B.ps[B.tile_pair[t]];
-    if (S.status != 0) return;
+    const int tid = threadIdx.x & (kRed - 1);
     const size_t base = (size_t)t * kTile;
     const double beta = S.beta, omega = S.omega;
     const bool first = S.first;
@@ -634,157 +690,414 @@ __global__ __launch_bounds__(kRed) void bp_kernel(const double *__restrict__ r, 
     double rr[kTile / kRed], pp[kTile / kRed], vv[kTile / kRed];
 #pragma unroll
     for (int k = 0; k < kTile / kRed; ++k) {
-        const size_t i = base + k * kRed + threadIdx.x;
+        const size_t i = base + k * kRed + tid;
         rr[k] = r[i];
         pp[k] = first ? 0.0 : p[i];
         vv[k] = first ? 0.0 : v[i];
     }
 #pragma unroll
     for (int k = 0; k < kTile / kRed; ++k)
-        p[base + k * kRed + threadIdx.x] = first ? rr[k] : rr[k] + beta * (pp[k] - omega * vv[k]);
+        p[base + k * kRed + tid] = first ? rr[k] : rr[k] + beta * (pp[k] - omega * vv[k]);
 }
 
-__global__ __launch_bounds__(64) void balpha_final(BatchDev B) {
-    const int p = blockIdx.x;
+__device__ __forceinline__ void balpha_final_body(const BatchDev &B, int p, int lane) {
     PairState &S = B.ps[p];
     if (S.status != 0) return;
     double d, unused;
-    pair_sums(B, p, d, unused);
-    if (threadIdx.x != 0) return;
+    pair_sums(B, p, lane, d, unused);
+    if (lane != 0) return;
     if (d == 0.0) S.status = 3;
     else S.alpha = S.rho / d;
     S.first = 0;
 }
 
 // s = r - alpha v; partial ||s||^2
-__global__ __launch_bounds__(kRed) void bs_kernel(const double *__restrict__ r, const double *__restrict__ v, double *s,
-                                                  BatchDev B) {
-    const int t = blockIdx.x;
-    const PairState &S = B.ps[B.tile_pair[t]];
-    if (S.status != 0) return;
-    const double alpha = S.alpha;
+__device__ __forceinline__ void bs_body(int t, int sub, const double *__restrict__ r, const double *__restrict__ v,
+                                        double *s, const BatchDev &B) {
+    const bool live = tile_live(B, t);
+    const int tid = threadIdx.x & (kRed - 1);
     const size_t base = (size_t)t * kTile;
     double acc = 0.0;
+    if (live) {
+        const double alpha = B.ps[B.tile_pair[t]].alpha;
 #pragma unroll
-    for (int k = 0; k < kTile / kRed; ++k) {
-        const size_t i = base + k * kRed + threadIdx.x;
-        const double si = r[i] - alpha * v[i];
-        s[i] = si;
-        acc = fma(si, si, acc);
+        for (int k = 0; k < kTile / kRed; ++k) {
+            const size_t i = base + k * kRed + tid;
+            const double si = r[i] - alpha * v[i];
+            s[i] = si;
+            acc = fma(si, si, acc);
+        }
     }
-    acc = wg_sum(acc);
-    if (threadIdx.x == 0) B.part0[t] = acc;
+    acc = wg_sum(acc, sub);
+    if (live && tid == 0) B.part0[t] = acc;
 }
 
-__global__ __launch_bounds__(64) void bsnorm_final(BatchDev B) {
-    const int p = blockIdx.x;
+__device__ __forceinline__ void bsnorm_final_body(const BatchDev &B, int p, int lane) {
     PairState &S = B.ps[p];
     if (S.status != 0) return;
     double s2, unused;
-    pair_sums(B, p, s2, unused);
-    if (threadIdx.x == 0 && s2 <= S.tol2) S.status = 2;
+    pair_sums(B, p, lane, s2, unused);
+    if (lane == 0 && s2 <= S.tol2) S.status = 2;
 }
 
 // pairs converged on s: x += alpha ph
-__global__ __launch_bounds__(kRed) void bxs_kernel(double *x, const double *__restrict__ ph, BatchDev B) {
-    const int t = blockIdx.x;
+__device__ __forceinline__ void bxs_body(int t, double *x, const double *__restrict__ ph, const BatchDev &B) {
+    if (t >= B.ntiles) return;
     const PairState &S = B.ps[B.tile_pair[t]];
     if (S.status != 2) return;
     const double alpha = S.alpha;
+    const int tid = threadIdx.x & (kRed - 1);
     const size_t base = (size_t)t * kTile;
 #pragma unroll
     for (int k = 0; k < kTile / kRed; ++k) {
-        const size_t i = base + k * kRed + threadIdx.x;
+        const size_t i = base + k * kRed + tid;
         x[i] += alpha * ph[i];
     }
 }
 
-__global__ __launch_bounds__(64) void bomega_final(BatchDev B) {
-    const int p = blockIdx.x;
+__device__ __forceinline__ void bomega_final_body(const BatchDev &B, int p, int lane) {
     PairState &S = B.ps[p];
     if (S.status != 0) return;
     double ts, tt;
-    pair_sums(B, p, ts, tt);  // part0 = t.s, part1 = t.t
-    if (threadIdx.x != 0) return;
+    pair_sums(B, p, lane, ts, tt);  // part0 = t.s, part1 = t.t
+    if (lane != 0) return;
     if (tt == 0.0) S.status = 3;
     else S.omega = ts / tt;
 }
 
-// x += alpha ph + omega sh; r = s - omega t; partial ||r||^2
-__global__ __launch_bounds__(kRed) void bxr_kernel(double *x, double *r, const double *__restrict__ ph,
-                                                   const double *__restrict__ sh, const double *__restrict__ s,
-                                                   const double *__restrict__ tv, BatchDev B) {
-    const int t = blockIdx.x;
-    const PairState &S = B.ps[B.tile_pair[t]];
-    if (S.status != 0) return;
-    const double alpha = S.alpha, omega = S.omega;
+// x += alpha ph + omega sh; r = s - omega t; partials ||r||^2 and rhat.r (the next
+// iteration's rho)
+__device__ __forceinline__ void bxr_body(int t, int sub, double *x, double *r, const double *__restrict__ rh,
+                                         const double *__restrict__ ph, const double *__restrict__ sh,
+                                         const double *__restrict__ s, const double *__restrict__ tv,
+                                         const BatchDev &B) {
+    const bool live = tile_live(B, t);
+    const int tid = threadIdx.x & (kRed - 1);
     const size_t base = (size_t)t * kTile;
-    double acc = 0.0;
+    double acc = 0.0, rho = 0.0;
+    if (live) {
+        const PairState &S = B.ps[B.tile_pair[t]];
+        const double alpha = S.alpha, omega = S.omega;
 #pragma unroll
-    for (int k = 0; k < kTile / kRed; ++k) {
-        const size_t i = base + k * kRed + threadIdx.x;
-        x[i] += alpha * ph[i] + omega * sh[i];
-        const double ri = s[i] - omega * tv[i];
-        r[i] = ri;
-        acc = fma(ri, ri, acc);
+        for (int k = 0; k < kTile / kRed; ++k) {
+            const size_t i = base + k * kRed + tid;
+            x[i] += alpha * ph[i] + omega * sh[i];
+            const double ri = s[i] - omega * tv[i];
+            r[i] = ri;
+            acc = fma(ri, ri, acc);
+            rho = fma(rh[i], ri, rho);
+        }
     }
-    acc = wg_sum(acc);
-    if (threadIdx.x == 0) B.part0[t] = acc;
+    acc = wg_sum(acc, sub);
+    rho = wg_sum(rho, sub);
+    if (live && tid == 0) {
+        B.part0[t] = acc;
+        B.part1[t] = rho;
+    }
 }
 
-__global__ __launch_bounds__(64) void brnorm_final(BatchDev B) {
-    const int p = blockIdx.x;
+// Pairs still running after the update add one to nactive.
+__device__ __forceinline__ void brnorm_final_body(const BatchDev &B, int p, int lane) {
     PairState &S = B.ps[p];
     const int st = S.status;
-    if (st == 2 && threadIdx.x == 0) S.status = 1;
+    if (st == 2 && lane == 0) S.status = 1;
     if (st != 0) return;
     double r2, unused;
-    pair_sums(B, p, r2, unused);
-    if (threadIdx.x != 0) return;
+    pair_sums(B, p, lane, r2, unused);
+    if (lane != 0) return;
     S.rho_old = S.rho;
     if (r2 <= S.tol2) S.status = 1;
     else atomicAdd(&B.ctl->nactive, 1);
 }
 
 // true residual of every pair: r = b - r (r holds A x); part0 = ||r||^2, part1 = ||b||^2
-__global__ __launch_bounds__(kRed) void bres_kernel(const double *__restrict__ b, double *r, BatchDev B) {
-    const int t = blockIdx.x;
+__device__ __forceinline__ void bres_body(int t, int sub, const double *__restrict__ b, double *r,
+                                          const BatchDev &B) {
+    const bool live = t < B.ntiles;
+    const int tid = threadIdx.x & (kRed - 1);
     const size_t base = (size_t)t * kTile;
     double sr = 0.0, sb = 0.0;
+    if (live) {
 #pragma unroll
-    for (int k = 0; k < kTile / kRed; ++k) {
-        const size_t i = base + k * kRed + threadIdx.x;
-        const double bi = b[i], ri = bi - r[i];
-        r[i] = ri;
-        sr = fma(ri, ri, sr);
-        sb = fma(bi, bi, sb);
+        for (int k = 0; k < kTile / kRed; ++k) {
+            const size_t i = base + k * kRed + tid;
+            const double bi = b[i], ri = bi - r[i];
+            r[i] = ri;
+            sr = fma(ri, ri, sr);
+            sb = fma(bi, bi, sb);
+        }
     }
-    sr = wg_sum(sr);
-    sb = wg_sum(sb);
-    if (threadIdx.x == 0) {
+    sr = wg_sum(sr, sub);
+    sb = wg_sum(sb, sub);
+    if (live && tid == 0) {
         B.part0[t] = sr;
         B.part1[t] = sb;
     }
 }
 
-// relres per pair; pairs whose true residual misses tol restart while iterations remain
-__global__ __launch_bounds__(64) void bres_final(BatchDev B, double tol, double *relres) {
-    const int p = blockIdx.x;
+// relres per pair; pairs whose true residual misses tol restart while iterations
+// remain (each such pair adds one to nrestart)
+__device__ __forceinline__ void bres_final_body(const BatchDev &B, int p, int lane, double tol, double *relres) {
     double r2, b2;
-    pair_sums(B, p, r2, b2);
-    if (threadIdx.x != 0) return;
+    pair_sums(B, p, lane, r2, b2);
+    if (lane != 0) return;
     const double rr = sqrt(r2) / sqrt(b2 > 0.0 ? b2 : 1.0);
     relres[p] = rr;
     PairState &S = B.ps[p];
-    if (rr <= tol) S.status = 4;
-    else S.status = S.iters < B.ctl->maxit ? 0 : 5;
+    if (rr <= tol) {
+        S.status = 4;
+    } else if (S.iters < B.ctl->maxit) {
+        S.status = 0;
+        atomicAdd(&B.ctl->nrestart, 1);
+    } else {
+        S.status = 5;
+    }
 }
 
-__global__ void breset_kernel(BatchDev B, int npairs) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+__device__ __forceinline__ void breset_body(const BatchDev &B, int p) {
     B.ps[p].status = 0;
     B.ps[p].iters = 0;
+}
+
+// ---- one launch per step (the eager and graph-replayed paths)
+__global__ void zero_active_kernel(Ctl *ctl) { ctl->nactive = 0; }
+
+__global__ __launch_bounds__(kRed) void tile_dot_kernel(const double *__restrict__ a, const double *__restrict__ b,
+                                                        const double *__restrict__ c, BatchDev B) {
+    tile_dot_body(blockIdx.x, 0, a, b, c, B);
+}
+__global__ __launch_bounds__(kRed) void binit_kernel(const double *__restrict__ b, double *r, double *rh, double *p,
+                                                     double *v, BatchDev B) {
+    binit_body(blockIdx.x, 0, b, r, rh, p, v, B);
+}
+__global__ __launch_bounds__(kRed) void bp_kernel(const double *__restrict__ r, double *__restrict__ p,
+                                                  const double *__restrict__ v, BatchDev B) {
+    bp_body(blockIdx.x, r, p, v, B);
+}
+__global__ __launch_bounds__(kRed) void bs_kernel(const double *__restrict__ r, const double *__restrict__ v, double *s,
+                                                  BatchDev B) {
+    bs_body(blockIdx.x, 0, r, v, s, B);
+}
+__global__ __launch_bounds__(kRed) void bxs_kernel(double *x, const double *__restrict__ ph, BatchDev B) {
+    bxs_body(blockIdx.x, x, ph, B);
+}
+__global__ __launch_bounds__(kRed) void bxr_kernel(double *x, double *r, const double *__restrict__ rh,
+                                                   const double *__restrict__ ph, const double *__restrict__ sh,
+                                                   const double *__restrict__ s, const double *__restrict__ tv,
+                                                   BatchDev B) {
+    bxr_body(blockIdx.x, 0, x, r, rh, ph, sh, s, tv, B);
+}
+__global__ __launch_bounds__(kRed) void bres_kernel(const double *__restrict__ b, double *r, BatchDev B) {
+    bres_body(blockIdx.x, 0, b, r, B);
+}
+__global__ __launch_bounds__(64) void binit_final(BatchDev B, double tol) {
+    binit_final_body(B, blockIdx.x, threadIdx.x, tol);
+}
+__global__ __launch_bounds__(64) void brho_final(BatchDev B) { brho_final_body(B, blockIdx.x, threadIdx.x); }
+__global__ __launch_bounds__(64) void balpha_final(BatchDev B) { balpha_final_body(B, blockIdx.x, threadIdx.x); }
+__global__ __launch_bounds__(64) void bsnorm_final(BatchDev B) { bsnorm_final_body(B, blockIdx.x, threadIdx.x); }
+__global__ __launch_bounds__(64) void bomega_final(BatchDev B) { bomega_final_body(B, blockIdx.x, threadIdx.x); }
+__global__ __launch_bounds__(64) void brnorm_final(BatchDev B) { brnorm_final_body(B, blockIdx.x, threadIdx.x); }
+__global__ __launch_bounds__(64) void bres_final(BatchDev B, double tol, double *relres) {
+    bres_final_body(B, blockIdx.x, threadIdx.x, tol, relres);
+}
+__global__ void breset_kernel(BatchDev B, int npairs) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < npairs) breset_body(B, p);
+}
+
+// ------------------------------------------------------- persistent kernel
+// The whole batched solve in one launch: every step above becomes a phase of a
+// grid-stride loop, with a grid barrier between phases.  The grid is one
+// workgroup of kPersistThreads per CU, all resident (checked at launch), so the
+// barrier waits only on workgroups that are running; each wait is still bounded
+// (Ctl::spin_us) and an abandoned wait makes every workgroup leave, reported as
+// ISAI_ERR_TIMEOUT.  Phases call the same step bodies as the per-step launches,
+// so both paths compute bit-identical iterates.
+//
+// Grid barrier: arrivals on one counter (agent-scope release before, acquire
+// after, so the phase's stores are visible to every XCD's next phase), the
+// target moving up by gridDim.x per barrier.
+__device__ __forceinline__ bool grid_barrier(Ctl *c, unsigned long long &target) {
+    __shared__ int s_ok;
+    __syncthreads();
+    target += gridDim.x;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(&c->arrive, 1ULL);
+        int ok = 1;
+        const long long limit = c->spin_us * 100;  // s_memrealtime ticks at 100 MHz
+        const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
+        for (;;) {
+            if (__hip_atomic_load(&c->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+            if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
+                break;
+            }
+            if ((long long)__builtin_amdgcn_s_memrealtime() - t0 > limit) {
+                __hip_atomic_store(&c->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __threadfence();
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+struct PersistArgs {
+    BatchDev B;
+    int nb;
+    const int *rp, *ci;
+    const double *val;                 // A
+    const int *Lrow, *Lcol, *Urow, *Ucol;
+    const void *Lval, *Uval;           // ISAI factors in storage precision
+    const double *b;                   // right-hand sides
+    double *x;                         // solutions (initial guesses in)
+    double *r, *rh, *p, *v, *ph, *s, *sh, *t, *tmp;
+    double tol;
+    double *relres;
+};
+
+enum Operand { kOpA = 0, kOpML = 1, kOpMU = 2 };
+
+// Y = op X over the pairs still running (all = 1: every pair); MAP as the
+// autotuner's mapping (0 row-pair, 1 column-split, 2 MFMA), PREC the operand's
+// value precision.  Units: 8-lane groups (one block row) for MAP 0/1, waves of
+// kMfmaF64Rows / kMfmaLowpRows block rows for MAP 2.
+template <int MAP, int PREC>
+__device__ __forceinline__ void spmm_phase(int nb, const int *rp, const int *ci, const void *val,
+                                           const BatchDev &B, const double *X, double *Y, int all) {
+    const int lane = threadIdx.x & 63;
+    if (MAP == 2) {
+        const int rows = PREC == ISAI_PREC_FP64 ? kMfmaF64Rows : kMfmaLowpRows;
+        const int units = (nb + rows - 1) / rows;
+        const int wpg = kPersistThreads / 64;
+        for (int u = blockIdx.x * wpg + (int)(threadIdx.x >> 6); u < units; u += gridDim.x * wpg) {
+            if (PREC == ISAI_PREC_FP64) mfma_f64_unit(nb, rp, ci, (const double *)val, B, X, Y, u, lane, all);
+            else mfma_lowp_unit<PREC == ISAI_PREC_BF16>(nb, rp, ci, val, B, X, Y, u, lane, all);
+        }
+    } else {
+        const int gpg = kPersistThreads / 8;
+        for (int br = blockIdx.x * gpg + (int)(threadIdx.x >> 3); br < nb; br += gridDim.x * gpg) {
+            if (PREC == ISAI_PREC_FP64) f64_unit<MAP>(nb, rp, ci, (const v2d *)val, B, X, Y, br, lane & 7, all);
+            else lowp_unit<PREC == ISAI_PREC_BF16, MAP>(nb, rp, ci, (const v4u *)val, B, X, Y, br, lane & 7, all);
+        }
+    }
+}
+
+template <int MAPA, int MAPM, int PRECM>
+__global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(PersistArgs a) {
+    const BatchDev &B = a.B;
+    const int sub = threadIdx.x / kRed, lane = threadIdx.x & 63;
+    const int wpg = kPersistThreads / 64, tpg = kPersistThreads / kRed;
+    Ctl *ctl = B.ctl;
+    unsigned long long target = 0;
+    // tile phases: the workgroup's tile groups take tiles t0 + sub together, so
+    // every group meets the same number of reduction barriers
+    auto tiles_end = [&](int t0) { return t0 < B.ntiles; };
+    auto A = [&](const double *X, double *Y, int all) {
+        spmm_phase<MAPA, ISAI_PREC_FP64>(a.nb, a.rp, a.ci, a.val, B, X, Y, all);
+    };
+    auto ML = [&](const double *X, double *Y) {
+        spmm_phase<MAPM, PRECM>(a.nb, a.Lrow, a.Lcol, a.Lval, B, X, Y, 0);
+    };
+    auto MU = [&](const double *X, double *Y) {
+        spmm_phase<MAPM, PRECM>(a.nb, a.Urow, a.Ucol, a.Uval, B, X, Y, 0);
+    };
+#define ISAI_SYNC() \
+    if (!grid_barrier(ctl, target)) return
+#define ISAI_TILES(BODY) \
+    for (int t0 = blockIdx.x * tpg; tiles_end(t0); t0 += gridDim.x * tpg) BODY
+#define ISAI_PAIRS(BODY) \
+    for (int p = blockIdx.x * wpg + (int)(threadIdx.x >> 6); p < B.npairs; p += gridDim.x * wpg) BODY
+
+    ISAI_PAIRS(breset_body(B, p));
+    ISAI_SYNC();
+    int nrestart = 0;
+    for (int round = 0; round <= kMaxRestarts; ++round) {
+        // (re)start the running pairs from their current x
+        A(a.x, a.r, 0);
+        ISAI_SYNC();
+        ISAI_TILES(binit_body(t0 + sub, sub, a.b, a.r, a.rh, a.p, a.v, B));
+        ISAI_SYNC();
+        ISAI_PAIRS({
+            binit_final_body(B, p, lane, a.tol);
+            __threadfence_block();  // lane 0's stores before the wave reads them back
+            brho_final_body(B, p, lane);
+        });
+        ISAI_SYNC();
+        int nactive = __hip_atomic_load(&ctl->nactive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // every pair stops within maxit iterations; the bound only guards the loop
+        for (int it = 0; it <= ctl->maxit; ++it) {
+            ISAI_TILES(bp_body(t0 + sub, a.r, a.p, a.v, B));
+            ISAI_SYNC();
+            ML(a.p, a.tmp);
+            ISAI_SYNC();
+            MU(a.tmp, a.ph);
+            ISAI_SYNC();
+            A(a.ph, a.v, 0);
+            ISAI_SYNC();
+            ISAI_TILES(tile_dot_body(t0 + sub, sub, a.rh, a.v, nullptr, B));
+            ISAI_SYNC();
+            ISAI_PAIRS(balpha_final_body(B, p, lane));
+            ISAI_SYNC();
+            ISAI_TILES(bs_body(t0 + sub, sub, a.r, a.v, a.s, B));
+            ISAI_SYNC();
+            ISAI_PAIRS(bsnorm_final_body(B, p, lane));
+            ISAI_SYNC();
+            // x of the pairs converged on s, and M_L s of the running ones (the two
+            // touch different pairs and vectors)
+            ISAI_TILES(bxs_body(t0 + sub, a.x, a.ph, B));
+            ML(a.s, a.tmp);
+            ISAI_SYNC();
+            MU(a.tmp, a.sh);
+            ISAI_SYNC();
+            A(a.sh, a.t, 0);
+            ISAI_SYNC();
+            ISAI_TILES(tile_dot_body(t0 + sub, sub, a.t, a.s, a.t, B));
+            ISAI_SYNC();
+            ISAI_PAIRS(bomega_final_body(B, p, lane));
+            ISAI_SYNC();
+            ISAI_TILES(bxr_body(t0 + sub, sub, a.x, a.r, a.rh, a.ph, a.sh, a.s, a.t, B));
+            ISAI_SYNC();
+            // end of this iteration and rho / beta of the next
+            ISAI_PAIRS({
+                brnorm_final_body(B, p, lane);
+                __threadfence_block();
+                brho_final_body(B, p, lane);
+            });
+            ISAI_SYNC();
+            const int now = __hip_atomic_load(&ctl->nactive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (now == nactive) break;  // no pair ran on
+            nactive = now;
+        }
+        // true FP64 residual of every pair; pairs that miss tol restart
+        A(a.x, a.r, 1);
+        ISAI_SYNC();
+        ISAI_TILES(bres_body(t0 + sub, sub, a.b, a.r, B));
+        ISAI_SYNC();
+        ISAI_PAIRS(bres_final_body(B, p, lane, a.tol, a.relres));
+        ISAI_SYNC();
+        const int now = __hip_atomic_load(&ctl->nrestart, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (now == nrestart) break;
+        nrestart = now;
+    }
+#undef ISAI_SYNC
+#undef ISAI_TILES
+#undef ISAI_PAIRS
+}
+
+// Control block before a persistent launch (a kernel, so the pair can be captured).
+__global__ void persist_reset_kernel(Ctl *c, int maxit) {
+    c->nactive = 0;
+    c->nrestart = 0;
+    c->abort = 0;
+    c->arrive = 0;
+    c->maxit = maxit;
 }
 
 }  // namespace isai
@@ -816,6 +1129,30 @@ struct isai_batch {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     double *g_x = nullptr;
+    // the persistent kernel's captured launch and the arguments it was captured with
+    hipGraphExec_t pexec = nullptr;
+    hipGraph_t pgraph = nullptr;
+    const double *p_b = nullptr;
+    double *p_x = nullptr;
+    double p_tol = 0.0;
+    int p_maxit = -1;
+    int ncu = 0;                     // persistent grid: one workgroup per CU
+    long long spin_us = 5000000;     // bound on one grid-barrier wait
+
+    void drop_graphs() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (pexec) (void)hipGraphExecDestroy(pexec);
+        if (pgraph) (void)hipGraphDestroy(pgraph);
+        gexec = nullptr;
+        graph = nullptr;
+        g_x = nullptr;
+        pexec = nullptr;
+        pgraph = nullptr;
+        p_b = nullptr;
+        p_x = nullptr;
+        p_maxit = -1;
+    }
 
     BatchDev dev() const {
         BatchDev B;
@@ -831,13 +1168,13 @@ struct isai_batch {
         B.part0 = d_part;
         B.part1 = d_part + ntiles;
         B.ctl = d_ctl;
+        B.ntiles = ntiles;
+        B.npairs = npairs;
         return B;
     }
 };
 
 namespace {
-
-enum Operand { kOpA = 0, kOpML = 1, kOpMU = 2 };
 
 // Y = op X over the pairs still running (all = 1: over every pair)
 template <int MAP>
@@ -901,7 +1238,6 @@ int batch_iteration(isai_batch *b, double *x, hipStream_t st) {
     const BatchDev B = b->dev();
     const int np = b->npairs, pg = (np + 255) / 256, T = b->ntiles;
     zero_active_kernel<<<1, 1, 0, st>>>(b->d_ctl);
-    tile_dot_kernel<<<T, kRed, 0, st>>>(rh, r, nullptr, B);
     brho_final<<<np, 64, 0, st>>>(B);
     bp_kernel<<<T, kRed, 0, st>>>(r, p, v, B);
     BTRY(spmm(b, kOpML, p, tmp, 0, st));
@@ -917,7 +1253,7 @@ int batch_iteration(isai_batch *b, double *x, hipStream_t st) {
     BTRY(spmm(b, kOpA, sh, t, 0, st));
     tile_dot_kernel<<<T, kRed, 0, st>>>(t, s, t, B);
     bomega_final<<<np, 64, 0, st>>>(B);
-    bxr_kernel<<<T, kRed, 0, st>>>(x, r, ph, sh, s, t, B);
+    bxr_kernel<<<T, kRed, 0, st>>>(x, r, rh, ph, sh, s, t, B);
     brnorm_final<<<np, 64, 0, st>>>(B);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
@@ -999,14 +1335,108 @@ int capture_graph(isai_batch *b, double *d_X, hipStream_t st) {
     return ISAI_OK;
 }
 
+typedef void (*PersistFn)(PersistArgs);
+
+template <int MA, int MM>
+PersistFn persist_fn_prec(int prec) {
+    if (prec == ISAI_PREC_FP64) return batch_persistent_kernel<MA, MM, ISAI_PREC_FP64>;
+    if (prec == ISAI_PREC_FP32) return batch_persistent_kernel<MA, MM, ISAI_PREC_FP32>;
+    return batch_persistent_kernel<MA, MM, ISAI_PREC_BF16>;
+}
+
+// The persistent kernel holds the row-pair (0) and MFMA (2) mappings; a
+// column-split choice (1) runs as row-pair there.
+PersistFn persist_fn(int map_a, int map_m, int prec) {
+    const bool ma = map_a == 2, mm = map_m == 2;
+    if (ma && mm) return persist_fn_prec<2, 2>(prec);
+    if (ma) return persist_fn_prec<2, 0>(prec);
+    if (mm) return persist_fn_prec<0, 2>(prec);
+    return persist_fn_prec<0, 0>(prec);
+}
+
+// Queues the whole solve as one persistent launch (after its control reset), or
+// replays the captured pair when graph is set (recaptured when an argument changes).
+int persistent_solve(isai_batch *b, double tol, int maxit, const double *d_B, double *d_X, bool graph,
+                     hipStream_t st) {
+    HandleFactors f;
+    BTRY(handle_factors_internal(b->pre, &f));
+    PersistArgs a;
+    a.B = b->dev();
+    a.nb = b->nb;
+    a.rp = b->d_row;
+    a.ci = b->d_col;
+    a.val = b->d_val;
+    a.Lrow = f.Lrow;
+    a.Lcol = f.Lcol;
+    a.Urow = f.Urow;
+    a.Ucol = f.Ucol;
+    a.Lval = f.Lval;
+    a.Uval = f.Uval;
+    a.b = d_B;
+    a.x = d_X;
+    const size_t n = b->len;
+    a.r = b->work;
+    a.rh = a.r + n;
+    a.p = a.rh + n;
+    a.v = a.p + n;
+    a.ph = a.v + n;
+    a.s = a.ph + n;
+    a.sh = a.s + n;
+    a.t = a.sh + n;
+    a.tmp = a.t + n;
+    a.tol = tol;
+    a.relres = b->d_relres;
+    const PersistFn fn = persist_fn(b->map[0], b->map[1], f.precision);
+    int occ = 0;
+    ISAI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn, kPersistThreads, 0));
+    if (occ < 1 || b->ncu < 1) {
+        set_error("isai_batch_solve: the persistent kernel does not fit one workgroup per CU (occupancy %d)", occ);
+        return ISAI_ERR_UNSUPPORTED;
+    }
+    auto launch = [&]() -> int {
+        persist_reset_kernel<<<1, 1, 0, st>>>(b->d_ctl, maxit);
+        ISAI_LAUNCH_CHECK();
+        void *args[] = {&a};
+        ISAI_HIP(hipLaunchKernel((const void *)fn, dim3(b->ncu), dim3(kPersistThreads), args, 0, st));
+        return ISAI_OK;
+    };
+    if (!graph) return launch();
+    if (!b->pexec || b->p_b != d_B || b->p_x != d_X || b->p_tol != tol || b->p_maxit != maxit) {
+        if (b->pexec) (void)hipGraphExecDestroy(b->pexec);
+        if (b->pgraph) (void)hipGraphDestroy(b->pgraph);
+        b->pexec = nullptr;
+        b->pgraph = nullptr;
+        b->p_maxit = -1;
+        ISAI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int s = launch();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (s != ISAI_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return s;
+        }
+        ISAI_HIP(e);
+        b->pgraph = g;
+        ISAI_HIP(hipGraphInstantiate(&b->pexec, g, nullptr, nullptr, 0));
+        b->p_b = d_B;
+        b->p_x = d_X;
+        b->p_tol = tol;
+        b->p_maxit = maxit;
+    }
+    ISAI_HIP(hipGraphLaunch(b->pexec, st));
+    return ISAI_OK;
+}
+
+int batch_results(isai_batch *b, double tol, int maxit, const std::vector<PairState> &hs, int *iters,
+                  double *relres, int *max_iters);
+
 }  // namespace
 
 extern "C" {
 
 int isai_batch_destroy(isai_batch_t b) {
     if (!b) return ISAI_OK;
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->drop_graphs();
     if (b->pre) isai_destroy(b->pre);
     (void)hipFree(b->d_sys_of_row);
     (void)hipFree(b->d_sys_ptr);
@@ -1139,6 +1569,14 @@ int isai_batch_setup(int nsys, const int *sys_ptr, const int *nrhs, int nnzb, co
     if (s != ISAI_OK) return fail(s);
     s = autotune(b, st);
     if (s != ISAI_OK) return fail(s);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipMemcpyAsync(&b->d_ctl->spin_us, &b->spin_us, sizeof(long long), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("isai_batch_setup: reading the CU count failed");
+        return fail(ISAI_ERR_HIP);
+    }
     *out = b;
     return ISAI_OK;
 }
@@ -1184,11 +1622,14 @@ int isai_batch_select(isai_batch_t b, int operand, int candidate) {
     b->map[operand] = candidate / 6;
     b->wg[operand] = kWgCand[(candidate / 2) % 3];
     b->remap[operand] = candidate % 2;
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->gexec = nullptr;
-    b->graph = nullptr;
-    b->g_x = nullptr;
+    b->drop_graphs();
+    return ISAI_OK;
+}
+
+int isai_batch_set_spin_limit(isai_batch_t b, long long us) {
+    ISAI_REQUIRE(b && us > 0, "isai_batch_set_spin_limit: bad argument");
+    b->spin_us = us;
+    ISAI_HIP(hipMemcpy(&b->d_ctl->spin_us, &us, sizeof(long long), hipMemcpyHostToDevice));
     return ISAI_OK;
 }
 
@@ -1207,20 +1648,33 @@ int isai_batch_spmm(isai_batch_t b, const double *d_X, double *d_Y, isai_stream_
 int isai_batch_solve(isai_batch_t b, double tol, int maxit, const double *d_B, double *d_X, int use_graph, int *iters,
                      double *relres, int *max_iters, isai_stream_t stream) {
     ISAI_REQUIRE(b && d_B && d_X && tol > 0.0 && maxit >= 0, "isai_batch_solve: bad argument");
+    ISAI_REQUIRE(use_graph >= 0 && use_graph <= 3, "isai_batch_solve: unknown mode %d", use_graph);
     hipStream_t st = (hipStream_t)stream;
-    ISAI_REQUIRE(!use_graph || st != nullptr, "isai_batch_solve: graph capture needs a created (non-null) stream");
+    ISAI_REQUIRE(use_graph == 0 || use_graph == 3 || st != nullptr,
+                 "isai_batch_solve: graph capture needs a created (non-null) stream");
     const size_t n = b->len;
     double *r = b->work, *rh = r + n, *p = rh + n, *v = p + n;
     const BatchDev B = b->dev();
     const int np = b->npairs, pg = (np + 255) / 256, T = b->ntiles;
+    std::vector<PairState> hs(np);
+    if (use_graph >= 2) {
+        BTRY(persistent_solve(b, tol, maxit, d_B, d_X, use_graph == 2, st));
+        Ctl hc;
+        ISAI_HIP(hipMemcpyAsync(hs.data(), b->d_ps, sizeof(PairState) * np, hipMemcpyDeviceToHost, st));
+        ISAI_HIP(hipMemcpyAsync(&hc, b->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+        ISAI_HIP(hipStreamSynchronize(st));
+        if (hc.abort) {
+            set_error("isai_batch_solve: a grid-barrier wait of the persistent kernel exceeded %lld us", b->spin_us);
+            return ISAI_ERR_TIMEOUT;
+        }
+        return batch_results(b, tol, maxit, hs, iters, relres, max_iters);
+    }
     if (use_graph && (b->g_x != d_X || !b->gexec)) BTRY(capture_graph(b, d_X, st));
-    b->h_ctl->maxit = maxit;
-    b->h_ctl->nactive = 0;
+    *b->h_ctl = Ctl{0, maxit, 0, 0, 0ULL, b->spin_us};
     ISAI_HIP(hipMemcpyAsync(b->d_ctl, b->h_ctl, sizeof(Ctl), hipMemcpyHostToDevice, st));
     breset_kernel<<<pg, 256, 0, st>>>(B, np);
     ISAI_LAUNCH_CHECK();
     const int max_replays = maxit / kItersPerReplay + 2;
-    std::vector<PairState> hs(np);
     for (int round = 0; round <= kMaxRestarts; ++round) {
         // (re)start the running pairs from their current x
         BTRY(spmm(b, kOpA, d_X, r, 0, st));
@@ -1248,6 +1702,17 @@ int isai_batch_solve(isai_batch_t b, double tol, int maxit, const double *d_B, d
         for (const PairState &q : hs) any = any || q.status == 0;
         if (!any) break;
     }
+    return batch_results(b, tol, maxit, hs, iters, relres, max_iters);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Per-pair iteration counts and true relative residuals of a finished solve.
+int batch_results(isai_batch *b, double tol, int maxit, const std::vector<PairState> &hs, int *iters,
+                  double *relres, int *max_iters) {
+    const int np = b->npairs;
     std::vector<double> rr(np);
     ISAI_HIP(hipMemcpy(rr.data(), b->d_relres, sizeof(double) * np, hipMemcpyDeviceToHost));
     int mx = 0;
@@ -1266,4 +1731,4 @@ int isai_batch_solve(isai_batch_t b, double tol, int maxit, const double *d_B, d
     return ISAI_OK;
 }
 
-}  // extern "C"
+}  // namespace

@@ -175,7 +175,9 @@ class Batch:
         call("isai_batch_spmm", self.h, d_X.ptr, d_Y.ptr, self.stream)
 
     def solve(self, d_B: DeviceArray, d_X: DeviceArray, tol: float = 1e-10, maxit: int = 1000,
-              use_graph: bool = True, check: bool = True):
+              use_graph: int = True, check: bool = True):
+        """use_graph: 0 / False per-step launches, 1 / True their captured graph,
+        2 the persistent kernel's captured graph, 3 the persistent kernel."""
         it = np.zeros(self.npairs, np.int32)
         rr = np.zeros(self.npairs)
         mx = C.c_int()

@@ -102,26 +102,51 @@ def test_batch_solve_matches_oracle(stream, precision):
         bt.close()
 
 
-def test_batch_graph_replay_equals_eager(stream):
-    """The captured graph (replayed with host reads and synchronisations between
-    replays) gives bit-identical iterates to the same kernels launched eagerly."""
-    print("batch graph vs eager 9 systems L 6..24", flush=True)
-    (rp, ci, val, sys_ptr, nrhs), bt = _make(9, 6, 24, 3, 16, "fp32", stream)
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "bf16"])
+def test_batch_graph_replay_equals_eager(stream, precision):
+    """Every schedule gives bit-identical iterates: the per-step launches eagerly and
+    replayed as a captured graph (host reads and synchronisations between replays),
+    and the persistent kernel launched directly and replayed as a captured graph
+    (twice, so the second replay reuses the captured launch)."""
+    print(f"batch schedules 9 systems L 6..24 {precision}", flush=True)
+    (rp, ci, val, sys_ptr, nrhs), bt = _make(9, 6, 24, 3, 16, precision, stream)
     try:
         B = DeviceArray.from_numpy(bt.pack(_rhs(sys_ptr, nrhs, 9)))
         xs, its = [], []
-        for use_graph in (True, False, True):
+        for mode in (1, 0, 1, 3, 2, 2):
             d_X = DeviceArray.zeros((bt.length,), np.float64)
-            st, iters, rr = bt.solve(B, d_X, tol=1e-10, maxit=500, use_graph=use_graph)
-            assert st == 0 and (rr <= 1e-10).all()
+            st, iters, rr = bt.solve(B, d_X, tol=1e-10, maxit=500, use_graph=mode)
+            assert st == 0 and (rr <= 1e-10).all(), mode
             xs.append(d_X.to_numpy())
             its.append(iters)
-        assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
-        assert np.array_equal(its[0], its[1]) and np.array_equal(its[0], its[2])
+        for k in range(1, len(xs)):
+            assert np.array_equal(xs[0], xs[k]) and np.array_equal(its[0], its[k]), k
         # restarting from the solution: every pair stops before its first iteration
         d_X = DeviceArray.from_numpy(xs[0])
         st, iters, rr = bt.solve(B, d_X, tol=1e-10, maxit=500)
         assert st == 0 and not iters.any() and np.array_equal(d_X.to_numpy(), xs[0])
+    finally:
+        bt.close()
+
+
+def test_batch_persistent_barrier_timeout_is_reported(stream):
+    """A grid-barrier wait beyond the bound (1 us here, set from the host) makes every
+    workgroup of the persistent kernel leave and the solve return ISAI_ERR_TIMEOUT;
+    with the bound restored the same batch object solves again."""
+    print("batch persistent timeout 6 systems L 20..40", flush=True)
+    (rp, ci, val, sys_ptr, nrhs), bt = _make(6, 20, 40, 8, 16, "fp32", stream)
+    try:
+        B = DeviceArray.from_numpy(bt.pack(_rhs(sys_ptr, nrhs, 2)))
+        d_X = DeviceArray.zeros((bt.length,), np.float64)
+        call("isai_batch_set_spin_limit", bt.h, 1)
+        st, iters, rr = bt.solve(B, d_X, tol=1e-10, maxit=500, use_graph=3, check=False)
+        assert st == -7, st  # ISAI_ERR_TIMEOUT
+        call("isai_batch_set_spin_limit", bt.h, 5_000_000)
+        d_X = DeviceArray.zeros((bt.length,), np.float64)
+        st, iters, rr = bt.solve(B, d_X, tol=1e-10, maxit=500, use_graph=2)
+        assert st == 0 and (rr <= 1e-10).all()
+        with pytest.raises(IsaiError):
+            bt.solve(B, d_X, tol=1e-10, maxit=500, use_graph=4)
     finally:
         bt.close()
 
@@ -134,6 +159,9 @@ def test_batch_iteration_limit_and_errors(stream):
         d_X = DeviceArray.zeros((bt.length,), np.float64)
         st, iters, rr = bt.solve(d_B, d_X, tol=1e-10, maxit=3, check=False)
         assert st == -6 and iters.max() <= 3 and (rr > 1e-10).any()
+        d_X2 = DeviceArray.zeros((bt.length,), np.float64)  # the persistent kernel stops the same way
+        st2, iters2, rr2 = bt.solve(d_B, d_X2, tol=1e-10, maxit=3, use_graph=2, check=False)
+        assert st2 == -6 and np.array_equal(iters2, iters) and np.array_equal(d_X2.to_numpy(), d_X.to_numpy())
         with pytest.raises(IsaiError):
             bt.offset(0, int(nrhs[0]))
     finally:
@@ -182,6 +210,10 @@ def test_batch_every_spmm_candidate(stream, precision):
             st, iters, rr = bt.solve(B, d_x, tol=1e-10, maxit=500)
             assert st == 0 and (rr <= 1e-10).all(), c
             its.append(iters)
+            if c // 6 != 1:  # the persistent kernel holds the row-pair and MFMA mappings
+                d_p = DeviceArray.zeros((bt.length,), np.float64)
+                st, it_p, rr = bt.solve(B, d_p, tol=1e-10, maxit=500, use_graph=3)
+                assert st == 0 and np.array_equal(it_p, iters) and np.array_equal(d_p.to_numpy(), d_x.to_numpy()), c
         # FP64 factors: every candidate's products agree to rounding, so the
         # iteration counts agree within one
         if precision == "fp64":
