@@ -74,7 +74,7 @@ __device__ __forceinline__ float pair_swap_f(float v) {
 
 // Block rows per 8-lane group of the reduced-precision products for rows of at most
 // two 128-byte lines of values (several rows keep enough loads in flight).
-constexpr int kLowpRowsF32 = 2, kLowpRowsBF16 = 4;
+constexpr int kLowpRowsF32 = 3, kLowpRowsBF16 = 4;
 
 inline unsigned grid_for(long long work, int per_block) {
     return (unsigned)((work + per_block - 1) / per_block);

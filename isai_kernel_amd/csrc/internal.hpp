@@ -18,8 +18,18 @@ int spmv_launch(int mb, int nnzb, int bs, isai_precision_t prec, const int *rp, 
 // bs = 4 product with FP32/BF16 values; x and y each either FP64 or FP32 vectors
 // (x is rounded to FP32 on load either way, so an FP32 x gives the same result as
 // the FP64 vector it was rounded from).
+// aligned: every row starts on a 128-byte line of values (the handle's factors,
+// pad_convert_lowp), which picks the streamed shape for FP32.
 int spmv_lowp_launch(int mb, long long nnzb, isai_precision_t prec, const int *rp, const int *ci, const void *val,
-                     const void *x, bool x_f32, void *y, bool y_f32, hipStream_t st, const int *skip);
+                     const void *x, bool x_f32, void *y, bool y_f32, hipStream_t st, const int *skip,
+                     bool aligned = false);
+
+// Line-aligned reduced-precision copy of a bs = 4 factor (FP32 or BF16 values):
+// prp = padded row pointer (each row's block count rounded up to the blocks per
+// 128-byte line, 2 FP32 / 4 BF16), pci / pval of prp[nb] blocks; the padding
+// blocks are zeros in the row's last column.  Asynchronous.
+int pad_convert_lowp(int nb, const int *rp, const int *ci, const double *val, const int *prp, int *pci, void *pval,
+                     int prec, hipStream_t st);
 
 // bs = 4 FP64 product y = A x with fused dot products of y: part[wg] = sum y.a over
 // the rows of workgroup wg, and with ndot = 2 also part[G + wg] = sum y.y, where
@@ -67,7 +77,9 @@ struct HandleFactors {
     const void *Lval, *Uval;  // in the handle's current precision
     int precision;
 };
-// Factors of a setup handle (isai_handle_t) in its current storage precision.
+// Factors of a setup handle (isai_handle_t) in its current storage precision; for
+// FP32 / BF16 these are the line-aligned copies with their own padded patterns
+// (nnzbL / nnzbU count the padding blocks, which are zeros).
 int handle_factors_internal(isai_handle_t h, HandleFactors *out);
 
 // Transposes (adjoint.hip).  transpose_pattern_host: pattern of A^T and perm[d] =

@@ -347,16 +347,20 @@ int inner_iteration(size_t n, int nb, long long nnzb, const int *rp, const int *
     const int G = mixed_dot_groups(nb);
     inner_p_kernel<<<kMixedVecBlocks, kMixedVecThreads, 0, st>>>(n, w->r, w->p, w->v, w->rpart, IS, first ? 1 : 0);
     ISAI_LAUNCH_CHECK();
-    int s = spmv_lowp_launch(nb, M->nnzbL, M->prec, M->Lrow, M->Lcol, M->Lval, w->p, true, w->tmp, true, st, skip);
+    int s = spmv_lowp_launch(nb, M->nnzbL, M->prec, M->Lrow, M->Lcol, M->Lval, w->p, true, w->tmp, true, st, skip,
+                             true);
     if (s == ISAI_OK)
-        s = spmv_lowp_launch(nb, M->nnzbU, M->prec, M->Urow, M->Ucol, M->Uval, w->tmp, true, w->ph, true, st, skip);
+        s = spmv_lowp_launch(nb, M->nnzbU, M->prec, M->Urow, M->Ucol, M->Uval, w->tmp, true, w->ph, true, st, skip,
+                             true);
     if (s == ISAI_OK) s = mixed_spmv_f32_dot(nb, nnzb, rp, ci, A32, w->ph, w->v, w->rh, 1, w->part, st, skip);
     if (s != ISAI_OK) return s;
     inner_s_kernel<<<kMixedVecBlocks, kMixedVecThreads, 0, st>>>(n, w->r, w->v, w->s, w->part, G, IS);
     ISAI_LAUNCH_CHECK();
-    s = spmv_lowp_launch(nb, M->nnzbL, M->prec, M->Lrow, M->Lcol, M->Lval, w->s, true, w->tmp, true, st, skip);
+    s = spmv_lowp_launch(nb, M->nnzbL, M->prec, M->Lrow, M->Lcol, M->Lval, w->s, true, w->tmp, true, st, skip,
+                         true);
     if (s == ISAI_OK)
-        s = spmv_lowp_launch(nb, M->nnzbU, M->prec, M->Urow, M->Ucol, M->Uval, w->tmp, true, w->sh, true, st, skip);
+        s = spmv_lowp_launch(nb, M->nnzbU, M->prec, M->Urow, M->Ucol, M->Uval, w->tmp, true, w->sh, true, st, skip,
+                             true);
     if (s == ISAI_OK) s = mixed_spmv_f32_dot(nb, nnzb, rp, ci, A32, w->sh, w->t, w->s, 2, w->part, st, skip);
     if (s != ISAI_OK) return s;
     inner_xr_kernel<<<kMixedVecBlocks, kMixedVecThreads, 0, st>>>(n, w->x, w->r, w->ph, w->sh, w->s, w->t, w->rh,

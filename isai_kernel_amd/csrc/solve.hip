@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -567,7 +568,11 @@ struct isai_handle {
     int plan_ncache = 0, plan_ordered[2] = {0, 0};  // ISAI build plan (isai_handle_build_plan)
     int *d_Lrow = nullptr, *d_Lcol = nullptr, *d_Urow = nullptr, *d_Ucol = nullptr;
     double *d_Lval = nullptr, *d_Uval = nullptr;
-    void *d_Llp = nullptr, *d_Ulp = nullptr;  // reduced-precision copies
+    // reduced-precision copies, each row padded to start on a 128-byte line
+    // (pad_convert_lowp), with their own patterns for the current line alignment
+    void *d_Llp = nullptr, *d_Ulp = nullptr;
+    int *d_Lrow_lp = nullptr, *d_Lcol_lp = nullptr, *d_Urow_lp = nullptr, *d_Ucol_lp = nullptr;
+    int nnzbL_lp = 0, nnzbU_lp = 0, lp_align = 0;
     int lp_prec = -1;
     int precond = ISAI_PRECOND_ISAI, precision = ISAI_PREC_FP64, chunk = 0;
     isai_sptrsv_plan_t sv = nullptr;
@@ -620,19 +625,72 @@ int ensure_work(isai_handle *h, size_t vecs) {
     return ISAI_OK;
 }
 
+void free_lp(isai_handle *h) {
+    (void)hipFree(h->d_Llp);
+    (void)hipFree(h->d_Ulp);
+    (void)hipFree(h->d_Lrow_lp);
+    (void)hipFree(h->d_Lcol_lp);
+    (void)hipFree(h->d_Urow_lp);
+    (void)hipFree(h->d_Ucol_lp);
+    h->d_Llp = h->d_Ulp = nullptr;
+    h->d_Lrow_lp = h->d_Lcol_lp = h->d_Urow_lp = h->d_Ucol_lp = nullptr;
+    h->nnzbL_lp = h->nnzbU_lp = h->lp_align = 0;
+    h->lp_prec = -1;
+}
+
+// Padded row pointer of one factor (row lengths rounded up to `align` blocks).
+int padded_rows(int nb, const int *d_row, int align, std::vector<int> &prp) {
+    std::vector<int> rp(nb + 1);
+    ISAI_HIP(hipMemcpy(rp.data(), d_row, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost));
+    prp.assign(nb + 1, 0);
+    long long tot = 0;
+    for (int i = 0; i < nb; ++i) {
+        const int len = rp[i + 1] - rp[i];
+        tot += (len + align - 1) / align * align;
+        if (tot > INT_MAX) {
+            set_error("isai: the padded reduced-precision factor exceeds %d blocks", INT_MAX);
+            return ISAI_ERR_CAPACITY;
+        }
+        prp[i + 1] = (int)tot;
+    }
+    return ISAI_OK;
+}
+
+// Reduced-precision copies of the ISAI factors in their line-aligned layout.
 int convert_lp(isai_handle *h, int prec, hipStream_t st) {
     if (prec == ISAI_PREC_FP64 || h->lp_prec == prec) return ISAI_OK;
-    const size_t esz = prec == ISAI_PREC_FP32 ? 4 : 2;
-    if (!h->d_Llp) {
-        if (hipMalloc(&h->d_Llp, 4 * 16 * (size_t)h->nnzbL) != hipSuccess ||
-            hipMalloc(&h->d_Ulp, 4 * 16 * (size_t)h->nnzbU) != hipSuccess) {
+    const int align = prec == ISAI_PREC_FP32 ? 2 : 4;  // blocks per 128-byte line
+    if (h->lp_align != align) {
+        free_lp(h);
+        std::vector<int> pl, pu;
+        int s = padded_rows(h->nb, h->d_Lrow, align, pl);
+        if (s == ISAI_OK) s = padded_rows(h->nb, h->d_Urow, align, pu);
+        if (s != ISAI_OK) return s;
+        const size_t nl = (size_t)pl[h->nb], nu = (size_t)pu[h->nb];
+        if (hipMalloc(&h->d_Lrow_lp, sizeof(int) * (h->nb + 1)) != hipSuccess ||
+            hipMalloc(&h->d_Urow_lp, sizeof(int) * (h->nb + 1)) != hipSuccess ||
+            hipMalloc(&h->d_Lcol_lp, sizeof(int) * (nl ? nl : 1)) != hipSuccess ||
+            hipMalloc(&h->d_Ucol_lp, sizeof(int) * (nu ? nu : 1)) != hipSuccess ||
+            hipMalloc(&h->d_Llp, 4 * 16 * (nl ? nl : 1)) != hipSuccess ||
+            hipMalloc(&h->d_Ulp, 4 * 16 * (nu ? nu : 1)) != hipSuccess) {
+            (void)hipGetLastError();
+            free_lp(h);
             set_error("isai: cannot allocate reduced-precision factors");
             return ISAI_ERR_ALLOC;
         }
+        if (hipMemcpy(h->d_Lrow_lp, pl.data(), sizeof(int) * (h->nb + 1), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->d_Urow_lp, pu.data(), sizeof(int) * (h->nb + 1), hipMemcpyHostToDevice) != hipSuccess) {
+            free_lp(h);
+            set_error("isai: uploading the reduced-precision patterns failed");
+            return ISAI_ERR_HIP;
+        }
+        h->nnzbL_lp = (int)nl;
+        h->nnzbU_lp = (int)nu;
+        h->lp_align = align;
     }
-    (void)esz;
-    int s = isai_convert_values((size_t)h->nnzbL * 16, (isai_precision_t)prec, h->d_Lval, h->d_Llp, (isai_stream_t)st);
-    if (s == ISAI_OK) s = isai_convert_values((size_t)h->nnzbU * 16, (isai_precision_t)prec, h->d_Uval, h->d_Ulp, (isai_stream_t)st);
+    int s = pad_convert_lowp(h->nb, h->d_Lrow, h->d_Lcol, h->d_Lval, h->d_Lrow_lp, h->d_Lcol_lp, h->d_Llp, prec, st);
+    if (s == ISAI_OK)
+        s = pad_convert_lowp(h->nb, h->d_Urow, h->d_Ucol, h->d_Uval, h->d_Urow_lp, h->d_Ucol_lp, h->d_Ulp, prec, st);
     if (s == ISAI_OK) h->lp_prec = prec;
     return s;
 }
@@ -653,19 +711,20 @@ int precond_apply(isai_handle *h, const double *v, double *z, double *tmp, hipSt
             return ISAI_OK;
         case ISAI_PRECOND_ISAI: {
             const isai_precision_t p = (isai_precision_t)h->precision;
-            const void *L = p == ISAI_PREC_FP64 ? (const void *)h->d_Lval : h->d_Llp;
-            const void *U = p == ISAI_PREC_FP64 ? (const void *)h->d_Uval : h->d_Ulp;
-            if (p != ISAI_PREC_FP64 && h->bs == 4) {
-                // M_L v is kept in FP32, the precision the M_U product reads it in:
-                // half the intermediate's bytes, same result
-                int s = spmv_lowp_launch(h->nb, h->nnzbL, p, h->d_Lrow, h->d_Lcol, L, v, false, tmp, true, st, skip);
+            if (p != ISAI_PREC_FP64) {
+                // line-aligned reduced-precision copies; M_L v is kept in FP32, the
+                // precision the M_U product reads it in: half the intermediate's
+                // bytes, same result
+                int s = spmv_lowp_launch(h->nb, h->nnzbL_lp, p, h->d_Lrow_lp, h->d_Lcol_lp, h->d_Llp, v, false, tmp,
+                                         true, st, skip, true);
                 if (s == ISAI_OK)
-                    s = spmv_lowp_launch(h->nb, h->nnzbU, p, h->d_Urow, h->d_Ucol, U, tmp, true, z, out_f32, st,
-                                         skip);
+                    s = spmv_lowp_launch(h->nb, h->nnzbU_lp, p, h->d_Urow_lp, h->d_Ucol_lp, h->d_Ulp, tmp, true, z,
+                                         out_f32, st, skip, true);
                 return s;
             }
-            int s = spmv_launch(h->nb, h->nnzbL, h->bs, p, h->d_Lrow, h->d_Lcol, L, v, tmp, st, skip);
-            if (s == ISAI_OK) s = spmv_launch(h->nb, h->nnzbU, h->bs, p, h->d_Urow, h->d_Ucol, U, tmp, z, st, skip);
+            int s = spmv_launch(h->nb, h->nnzbL, h->bs, p, h->d_Lrow, h->d_Lcol, h->d_Lval, v, tmp, st, skip);
+            if (s == ISAI_OK)
+                s = spmv_launch(h->nb, h->nnzbU, h->bs, p, h->d_Urow, h->d_Ucol, h->d_Uval, tmp, z, st, skip);
             return s;
         }
         case ISAI_PRECOND_ILU_SPTRSV:
@@ -938,8 +997,8 @@ int fgmres_mixed(isai_handle *h, int m, double tol, int maxit, const double *b, 
     TRY(ensure_work(h, (size_t)2 * m + 3));
     TRY(ensure_inner(h, st));
     h->last_restart = m;
-    const InnerPrecond M{(isai_precision_t)lp, h->nnzbL, h->nnzbU, h->d_Lrow, h->d_Lcol, h->d_Urow, h->d_Ucol,
-                         h->d_Llp, h->d_Ulp};
+    const InnerPrecond M{(isai_precision_t)lp, h->nnzbL_lp, h->nnzbU_lp, h->d_Lrow_lp, h->d_Lcol_lp, h->d_Urow_lp,
+                         h->d_Ucol_lp, h->d_Llp, h->d_Ulp};
     const size_t n = h->n;
     double *V = h->work, *Z = V + n * (m + 1), *w = Z + n * m;
     const int *skip = &h->S->done;
@@ -1047,8 +1106,18 @@ int handle_factors_internal(isai_handle_t h, HandleFactors *o) {
     o->Urow = h->d_Urow;
     o->Ucol = h->d_Ucol;
     o->precision = h->precision;
-    o->Lval = h->precision == ISAI_PREC_FP64 ? (const void *)h->d_Lval : h->d_Llp;
-    o->Uval = h->precision == ISAI_PREC_FP64 ? (const void *)h->d_Uval : h->d_Ulp;
+    o->Lval = h->d_Lval;
+    o->Uval = h->d_Uval;
+    if (h->precision != ISAI_PREC_FP64) {  // the line-aligned copies and their patterns
+        o->nnzbL = h->nnzbL_lp;
+        o->nnzbU = h->nnzbU_lp;
+        o->Lrow = h->d_Lrow_lp;
+        o->Lcol = h->d_Lcol_lp;
+        o->Urow = h->d_Urow_lp;
+        o->Ucol = h->d_Ucol_lp;
+        o->Lval = h->d_Llp;
+        o->Uval = h->d_Ulp;
+    }
     return ISAI_OK;
 }
 
@@ -1065,8 +1134,7 @@ int isai_destroy(isai_handle_t h) {
     (void)hipFree(h->d_Ucol);
     (void)hipFree(h->d_Lval);
     (void)hipFree(h->d_Uval);
-    (void)hipFree(h->d_Llp);
-    (void)hipFree(h->d_Ulp);
+    free_lp(h);
     (void)hipFree(h->work);
     (void)hipFree(h->S);
     (void)hipFree(h->d_res2);

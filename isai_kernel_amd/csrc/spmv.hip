@@ -230,12 +230,14 @@ __device__ __forceinline__ void store_y(void *__restrict__ y, size_t i, float v)
 // enough loads in flight on its own.  Per row the lanes accumulate in the same
 // order as with one row per group, so results do not depend on kLowpRows.
 // Measured on the 96^3 7-pt ISAI factors (4 blocks per row), M_L and M_U together:
-// BF16 60.5 us with 4 rows (63.5 with 2, 66.9 with 8); FP32 120.7 us with 2 rows
-// (119.8 with 4, 131.0 with 1).  On the 64^3 27-pt factors (14 blocks per row) one
+// BF16 60.5 us with 4 rows (63.5 with 2, 66.9 with 8); FP32 on line-aligned rows
+// 95.5 us with 3 rows and streamed loads (105.3 with 2 rows through L2, 117.3 with
+// 1), on unaligned rows 110.5 us with 2 rows through L2 (126.9 streamed, 121.1 with
+// 1 row).  On the 64^3 27-pt factors (14 blocks per row) one
 // row already keeps enough loads in flight and several rows per group are slower, so
 // the launcher uses several rows only for rows of at most two lines of values
 // (kLowpRowsBF16 / kLowpRowsF32).
-template <bool BF16, int kLowpRows, bool XF32, bool YF32>
+template <bool BF16, int kLowpRows, bool XF32, bool YF32, bool NT>
 __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci, const v4u *__restrict__ val16,
     const void *__restrict__ x, void *__restrict__ y, const int *__restrict__ skip) {
@@ -282,10 +284,11 @@ __global__ __launch_bounds__(kSpmvBlock) void bsr4_spmv_lowp_kernel(
             for (int r = 0; r < kLowpRows; ++r) {
                 const int kc = live[r] ? rs[r] + t + b : 0;
                 c[r] = ci[kc];
-                // 16 B per lane: BF16 rows fill whole lines (streamed); a 64-B FP32
-                // block shares its line with a neighbour another wave may read, so
-                // FP32 keeps the line in L2
-                w[r] = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kc * LPB + sub]) : val16[(size_t)kc * LPB + sub];
+                // 16 B per lane.  NT (streamed): rows start on a line (BF16 rows
+                // fill whole lines; the handle's factors are line-aligned by
+                // pad_convert_lowp).  Otherwise a 64-B FP32 block may share its
+                // line with a neighbour another wave reads, and the line stays in L2.
+                w[r] = NT ? __builtin_nontemporal_load(&val16[(size_t)kc * LPB + sub]) : val16[(size_t)kc * LPB + sub];
             }
 #pragma unroll
             for (int r = 0; r < kLowpRows; ++r) load_x4_split<LPB, XF32>(x, c[r], sub, f[r]);
@@ -428,6 +431,43 @@ __global__ void convert_bf16_kernel(size_t n, const double *__restrict__ src, un
         dst[i] = f32_to_bf16_rne((float)src[i]);
 }
 
+// ------------------------------------------------ line-aligned reduced-precision copy
+// Every row of the copy starts on a 128-byte line: its block count is padded up to
+// a multiple of the blocks per line (align = 2 FP32, 4 BF16) with blocks of zeros
+// whose column repeats the row's last column.  A zero block adds exact zeros, so a
+// product with the copy equals the product with the unpadded factor; without the
+// padding a row starting mid-line shares that line with its neighbour and the line
+// is fetched twice.  One 16-lane group per row, lane e converting element e of each
+// block (64-B / 32-B coalesced stores).
+template <bool BF16>
+__global__ void pad_convert_kernel(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+                                   const double *__restrict__ val, const int *__restrict__ prp, int *__restrict__ pci,
+                                   void *__restrict__ pval) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = (int)(gid >> 4), e = (int)(gid & 15);
+    if (row >= nb) return;
+    const int s = rp[row], len = rp[row + 1] - s, ps = prp[row], plen = prp[row + 1] - ps;
+    for (int j = 0; j < plen; ++j) {
+        const bool real = j < len;
+        const int src = real ? s + j : s + len - 1;
+        if (e == 0) pci[ps + j] = ci[src];
+        const double v = real ? val[(size_t)src * 16 + e] : 0.0;
+        if (BF16) ((unsigned short *)pval)[(size_t)(ps + j) * 16 + e] = f32_to_bf16_rne((float)v);
+        else ((float *)pval)[(size_t)(ps + j) * 16 + e] = (float)v;
+    }
+}
+
+int pad_convert_lowp(int nb, const int *rp, const int *ci, const double *val, const int *prp, int *pci, void *pval,
+                     int prec, hipStream_t st) {
+    ISAI_REQUIRE(prec == ISAI_PREC_FP32 || prec == ISAI_PREC_BF16, "pad_convert_lowp: precision %d", prec);
+    if (nb == 0) return ISAI_OK;
+    const unsigned g = grid_for((long long)nb * 16, 256);
+    if (prec == ISAI_PREC_BF16) pad_convert_kernel<true><<<g, 256, 0, st>>>(nb, rp, ci, val, prp, pci, pval);
+    else pad_convert_kernel<false><<<g, 256, 0, st>>>(nb, rp, ci, val, prp, pci, pval);
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
+}
+
 // ------------------------------------------------ BCSC -> BCSR value permutation
 // One thread per (entry, element): reads are fully coalesced, each block's
 // writes form one contiguous bs*bs run.
@@ -467,38 +507,43 @@ __global__ __launch_bounds__(kB2cBlock) void bsr2csr_kernel(int nb, int bs, cons
 }
 
 template <bool XF32, bool YF32>
-static void lowp_launch(int mb, long long nnzb, bool bf16, const int *rp, const int *ci, const v4u *v16,
-                        const void *x, void *y, hipStream_t st, const int *skip) {
+static void lowp_launch(int mb, long long nnzb, bool bf16, bool aligned, const int *rp, const int *ci,
+                        const v4u *v16, const void *x, void *y, hipStream_t st, const int *skip) {
     const int bpb = bf16 ? 4 : 2;  // blocks per 128-byte line
     const bool short_rows = nnzb <= (long long)2 * bpb * mb;
-    const int rows = short_rows ? (bf16 ? kLowpRowsBF16 : kLowpRowsF32) : 1;
+    const int rows = short_rows ? (bf16 ? kLowpRowsBF16 : (aligned ? kLowpRowsF32 : 2)) : 1;
     const unsigned g = grid_for(mb, rows * kSpmvBlock / 8);
     if (bf16 && short_rows)
-        bsr4_spmv_lowp_kernel<true, kLowpRowsBF16, XF32, YF32><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x, y, skip);
+        bsr4_spmv_lowp_kernel<true, kLowpRowsBF16, XF32, YF32, true>
+            <<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x, y, skip);
     else if (bf16)
         bsr4_spmv_bf16_batch_kernel<4, XF32, YF32><<<grid_for(mb, kSpmvF64Block / 8), kSpmvF64Block, 0, st>>>(
             mb, rp, ci, v16, x, y, skip);
+    else if (short_rows && aligned)
+        bsr4_spmv_lowp_kernel<false, kLowpRowsF32, XF32, YF32, true>
+            <<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x, y, skip);
     else if (short_rows)
-        bsr4_spmv_lowp_kernel<false, kLowpRowsF32, XF32, YF32><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x, y, skip);
+        bsr4_spmv_lowp_kernel<false, 2, XF32, YF32, false><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x, y, skip);
     else
-        bsr4_spmv_lowp_kernel<false, 1, XF32, YF32><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x, y, skip);
+        bsr4_spmv_lowp_kernel<false, 1, XF32, YF32, false><<<g, kSpmvBlock, 0, st>>>(mb, rp, ci, v16, x, y, skip);
 }
 
 int spmv_lowp_launch(int mb, long long nnzb, isai_precision_t prec, const int *rp, const int *ci, const void *val,
-                     const void *x, bool x_f32, void *y, bool y_f32, hipStream_t st, const int *skip) {
+                     const void *x, bool x_f32, void *y, bool y_f32, hipStream_t st, const int *skip,
+                     bool aligned) {
     ISAI_REQUIRE(prec == ISAI_PREC_FP32 || prec == ISAI_PREC_BF16, "bsr_spmv: reduced precision expected (got %d)",
                  (int)prec);
     if (mb == 0) return ISAI_OK;
     const bool bf16 = prec == ISAI_PREC_BF16;
     const v4u *v16 = (const v4u *)val;
     if (x_f32 && y_f32)
-        lowp_launch<true, true>(mb, nnzb, bf16, rp, ci, v16, x, y, st, skip);
+        lowp_launch<true, true>(mb, nnzb, bf16, aligned, rp, ci, v16, x, y, st, skip);
     else if (x_f32)
-        lowp_launch<true, false>(mb, nnzb, bf16, rp, ci, v16, x, y, st, skip);
+        lowp_launch<true, false>(mb, nnzb, bf16, aligned, rp, ci, v16, x, y, st, skip);
     else if (y_f32)
-        lowp_launch<false, true>(mb, nnzb, bf16, rp, ci, v16, x, y, st, skip);
+        lowp_launch<false, true>(mb, nnzb, bf16, aligned, rp, ci, v16, x, y, st, skip);
     else
-        lowp_launch<false, false>(mb, nnzb, bf16, rp, ci, v16, x, y, st, skip);
+        lowp_launch<false, false>(mb, nnzb, bf16, aligned, rp, ci, v16, x, y, st, skip);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
