@@ -361,14 +361,23 @@ int isai_bsrilu0_analysis(int nb, int nnzb, int bs, const int *d_row, const int 
         set_error("bsrilu0 analysis: hipMalloc failed");
         return ISAI_ERR_ALLOC;
     }
-    ISAI_HIP(hipMemcpyAsync(P->d_order, order.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
-    ISAI_HIP(hipMemcpyAsync(P->d_diag, diag.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
-    ISAI_HIP(hipMemcpyAsync(P->d_upd_ptr, upd_ptr.data(), sizeof(int) * (nnzb + 1), hipMemcpyHostToDevice, st));
-    if (nu) {
-        ISAI_HIP(hipMemcpyAsync(P->d_upd_q, upd_q.data(), sizeof(int) * nu, hipMemcpyHostToDevice, st));
-        ISAI_HIP(hipMemcpyAsync(P->d_upd_src, upd_src.data(), sizeof(int) * nu, hipMemcpyHostToDevice, st));
+    // uploads; the plan is released if one fails
+    auto upload = [&]() -> int {
+        ISAI_HIP(hipMemcpyAsync(P->d_order, order.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
+        ISAI_HIP(hipMemcpyAsync(P->d_diag, diag.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
+        ISAI_HIP(hipMemcpyAsync(P->d_upd_ptr, upd_ptr.data(), sizeof(int) * (nnzb + 1), hipMemcpyHostToDevice, st));
+        if (nu) {
+            ISAI_HIP(hipMemcpyAsync(P->d_upd_q, upd_q.data(), sizeof(int) * nu, hipMemcpyHostToDevice, st));
+            ISAI_HIP(hipMemcpyAsync(P->d_upd_src, upd_src.data(), sizeof(int) * nu, hipMemcpyHostToDevice, st));
+        }
+        ISAI_HIP(hipStreamSynchronize(st));
+        return ISAI_OK;
+    };
+    if (const int us = upload(); us != ISAI_OK) {
+        (void)hipStreamSynchronize(st);  // no copy may still read the host arrays
+        isai_bsrilu0_destroy(P);
+        return us;
     }
-    ISAI_HIP(hipStreamSynchronize(st));
     *plan = P;
     return ISAI_OK;
 }

@@ -26,6 +26,7 @@
 // The last wave of a solve resets the ticket counters, so consecutive solves
 // (or graph replays) need no memset.
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <queue>
 #include <thread>
@@ -1030,60 +1031,92 @@ static void level_order(int nb, const std::vector<int> &lev, int nlev, bool uppe
 // then the processing position, on ties), and the shallowest ready row when
 // none is left.  A row is placed only after all of its dependencies, so every
 // dependency has a lower position (a lower or equal ticket).
+//
+// The walk moves along the level wavefront, which in row order spans the whole
+// matrix (a 3-D stencil's level planes cut through every row range), so it first
+// copies the triangle's dependency lists into level-major numbering, where the
+// rows it touches at any time sit close together; the order it produces is the
+// same (the keys keep the original level and position).
 static void unit_order(int nb, const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &diag,
-                       bool upper, const std::vector<int> &lev, std::vector<int> &order) {
-    auto range = [&](int i, int &pb, int &pe) {
-        pb = upper ? diag[i] + 1 : rp[i];
-        pe = upper ? rp[i + 1] : diag[i];
-    };
+                       bool upper, const std::vector<int> &lev, int nlev, std::vector<int> &order) {
     auto rank = [&](int i) { return upper ? nb - 1 - i : i; };
-    // consumers of every row: the transpose of the triangle's pattern
-    std::vector<int> cp(nb + 1, 0), need(nb), unit(nb, -1);
-    for (int i = 0; i < nb; ++i) {
-        int pb, pe;
-        range(i, pb, pe);
-        need[i] = pe - pb;
-        for (int p = pb; p < pe; ++p) cp[ci[p] + 1]++;
+    // level-major numbering: perm[n] = row, inv[row] = n
+    std::vector<int> cnt(nlev + 1, 0), perm(nb), inv(nb);
+    for (int i = 0; i < nb; ++i) cnt[lev[i] + 1]++;
+    for (int l = 0; l < nlev; ++l) cnt[l + 1] += cnt[l];
+    for (int q = 0; q < nb; ++q) {
+        const int i = upper ? nb - 1 - q : q;
+        const int n = cnt[lev[i]]++;
+        perm[n] = i;
+        inv[i] = n;
     }
-    for (int i = 0; i < nb; ++i) cp[i + 1] += cp[i];
-    std::vector<int> cons(cp[nb]), cur(cp.begin(), cp.end() - 1);
+    // the triangle's dependencies in that numbering (rq / cq), and their consumers
+    std::vector<int> rq(nb + 1, 0);
+    for (int i = 0; i < nb; ++i) rq[inv[i] + 1] = upper ? rp[i + 1] - diag[i] - 1 : diag[i] - rp[i];
+    for (int n = 0; n < nb; ++n) rq[n + 1] += rq[n];
+    std::vector<int> cq(rq[nb]), cp(nb + 1, 0);
     for (int i = 0; i < nb; ++i) {
-        int pb, pe;
-        range(i, pb, pe);
-        for (int p = pb; p < pe; ++p) cons[cur[ci[p]]++] = i;
+        const int pb = upper ? diag[i] + 1 : rp[i], pe = upper ? rp[i + 1] : diag[i];
+        int o = rq[inv[i]];
+        for (int p = pb; p < pe; ++p) {
+            const int d = inv[ci[p]];
+            cq[o++] = d;
+            cp[d + 1]++;
+        }
     }
-    using Ready = std::pair<long long, int>;  // (level << 32 | position rank, row)
+    for (int n = 0; n < nb; ++n) cp[n + 1] += cp[n];
+    std::vector<int> cons(cp[nb]), cur(cp.begin(), cp.end() - 1), need(nb), unit(nb, -1);
+    for (int n = 0; n < nb; ++n) {
+        need[n] = rq[n + 1] - rq[n];
+        for (int p = rq[n]; p < rq[n + 1]; ++p) cons[cur[cq[p]]++] = n;
+    }
+    using Ready = std::pair<long long, int>;                  // (level << 32 | position rank, row)
     using Local = std::pair<std::pair<int, long long>, int>;  // ((-inside, level key), row)
     std::priority_queue<Ready, std::vector<Ready>, std::greater<Ready>> pool;
     std::priority_queue<Local, std::vector<Local>, std::greater<Local>> local;
-    auto key = [&](int i) { return ((long long)lev[i] << 32) | (long long)rank(i); };
-    for (int i = 0; i < nb; ++i)
-        if (need[i] == 0) pool.push({key(i), i});
+    auto key = [&](int n) { return ((long long)lev[perm[n]] << 32) | (long long)rank(perm[n]); };
+    for (int n = 0; n < nb; ++n)
+        if (need[n] == 0) pool.push({key(n), n});
     order.clear();
     order.reserve(nb);
     for (int u = 0; (int)order.size() < nb; ++u) {
         for (int size = 0; size < kRowsPerWG && (int)order.size() < nb; ++size) {
-            int i;
+            int n;
             if (!local.empty()) {
-                i = local.top().second;
+                n = local.top().second;
                 local.pop();
             } else {
-                i = pool.top().second;
+                n = pool.top().second;
                 pool.pop();
             }
-            unit[i] = u;
-            order.push_back(i);
-            for (int t = cp[i]; t < cp[i + 1]; ++t) {
+            unit[n] = u;
+            order.push_back(n);
+            for (int t = cp[n]; t < cp[n + 1]; ++t) {
                 const int c = cons[t];
                 if (--need[c] != 0) continue;
-                int pb, pe, inside = 0;
-                range(c, pb, pe);
-                for (int p = pb; p < pe; ++p) inside += unit[ci[p]] == u;
+                int inside = 0;
+                for (int p = rq[c]; p < rq[c + 1]; ++p) inside += unit[cq[p]] == u;
                 local.push({{-inside, key(c)}, c});
             }
         }
         for (; !local.empty(); local.pop()) pool.push({key(local.top().second), local.top().second});
     }
+    for (int &n : order) n = perm[n];
+}
+
+// f(lo, hi) over [0, n) split into contiguous ranges, one host thread each.
+template <class F>
+static void parallel_ranges(int n, F f) {
+    const int hw = (int)std::thread::hardware_concurrency();
+    const int T = std::max(1, std::min({16, hw > 0 ? hw : 1, (n + 4095) / 4096}));
+    if (T == 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; ++t) th.emplace_back(f, (int)((long long)n * t / T), (int)((long long)n * (t + 1) / T));
+    f(0, (int)((long long)n / T));
+    for (auto &x : th) x.join();
 }
 
 }  // namespace isai
@@ -1114,16 +1147,30 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
     ISAI_HIP(hipMemcpyAsync(rp.data(), d_row, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, st));
     ISAI_HIP(hipMemcpyAsync(ci.data(), d_col, sizeof(int) * nnzb, hipMemcpyDeviceToHost, st));
     ISAI_HIP(hipStreamSynchronize(st));
-    for (int i = 0; i < nb; ++i) {
-        for (int p = rp[i]; p < rp[i + 1]; ++p) {
-            if (p > rp[i] && ci[p] <= ci[p - 1]) {
-                set_error("isai_bsr_sptrsv_analysis: block columns of row %d are not strictly ascending", i);
-                return ISAI_ERR_INVALID_ARG;
+    // diagonal positions and pattern checks, over row ranges on host threads; the
+    // first offending row is reported
+    {
+        std::atomic<int> bad_order{INT_MAX}, bad_diag{INT_MAX};
+        parallel_ranges(nb, [&](int lo, int hi) {
+            for (int i = lo; i < hi; ++i) {
+                for (int p = rp[i]; p < rp[i + 1]; ++p) {
+                    if (p > rp[i] && ci[p] <= ci[p - 1]) {
+                        for (int b = bad_order.load(); i < b && !bad_order.compare_exchange_weak(b, i);) {
+                        }
+                    }
+                    if (ci[p] == i) diag[i] = p;
+                }
+                if (diag[i] < 0)
+                    for (int b = bad_diag.load(); i < b && !bad_diag.compare_exchange_weak(b, i);) {
+                    }
             }
-            if (ci[p] == i) diag[i] = p;
+        });
+        if (bad_order.load() != INT_MAX && bad_order.load() <= bad_diag.load()) {
+            set_error("isai_bsr_sptrsv_analysis: block columns of row %d are not strictly ascending", bad_order.load());
+            return ISAI_ERR_INVALID_ARG;
         }
-        if (diag[i] < 0) {
-            set_error("isai_bsr_sptrsv_analysis: block row %d has no diagonal block", i);
+        if (bad_diag.load() != INT_MAX) {
+            set_error("isai_bsr_sptrsv_analysis: block row %d has no diagonal block", bad_diag.load());
             return ISAI_ERR_ZERO_PIVOT;
         }
     }
@@ -1133,63 +1180,122 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         int nlev = 1;
         std::vector<int> lev[2];
         int nl[2];
-        for (int u = 0; u < 2; ++u) {
-            nl[u] = row_levels(nb, rp, ci, diag, u == 1, lev[u]);
-            nlev = std::max(nlev, nl[u]);
+        {
+            std::thread tu([&] { nl[1] = row_levels(nb, rp, ci, diag, true, lev[1]); });
+            nl[0] = row_levels(nb, rp, ci, diag, false, lev[0]);
+            tu.join();
         }
+        nlev = std::max({nlev, nl[0], nl[1]});
         if (automatic) rows = nb / nlev >= kRowMinWidth;
         if (rows && units) {
-            // A structurally symmetric pattern makes U's dependency graph L's reversed,
-            // so the reversed L order is a U order with the same dependencies inside
-            // units; otherwise U's order is grown on a second host thread.
-            long long nl = 0, nu = 0;
-            for (int i = 0; i < nb; ++i) nl += diag[i] - rp[i], nu += rp[i + 1] - diag[i] - 1;
-            bool sym = nl == nu;  // with L inside U's transpose below: L = U transposed
-            for (int i = 0; i < nb && sym; ++i)
-                for (int p = rp[i]; p < diag[i] && sym; ++p) {
-                    const int k = ci[p];
-                    sym = std::binary_search(ci.begin() + diag[k] + 1, ci.begin() + rp[k + 1], i);
-                }
-            if (sym) {
-                unit_order(nb, rp, ci, diag, false, lev[0], order[0]);
+            // A structurally symmetric pattern makes U's dependency graph L's
+            // reversed, so the reversed L order is a U order; when nb is a multiple
+            // of kRowsPerWG its units are L's units reversed, with the same
+            // dependencies inside them.  Otherwise U's order is grown on a second
+            // host thread.
+            long long low = 0, up = 0;
+            for (int i = 0; i < nb; ++i) low += diag[i] - rp[i], up += rp[i + 1] - diag[i] - 1;
+            std::atomic<bool> sym{low == up && nb % kRowsPerWG == 0};  // then L = U transposed iff L in U^T
+            parallel_ranges(nb, [&](int lo, int hi) {
+                for (int i = lo; i < hi && sym.load(std::memory_order_relaxed); ++i)
+                    for (int p = rp[i]; p < diag[i]; ++p)
+                        if (!std::binary_search(ci.begin() + diag[ci[p]] + 1, ci.begin() + rp[ci[p] + 1], i)) {
+                            sym.store(false, std::memory_order_relaxed);
+                            break;
+                        }
+            });
+            if (sym.load()) {
+                unit_order(nb, rp, ci, diag, false, lev[0], nl[0], order[0]);
                 order[1].assign(order[0].rbegin(), order[0].rend());
             } else {
-                std::thread tu([&] { unit_order(nb, rp, ci, diag, true, lev[1], order[1]); });
-                unit_order(nb, rp, ci, diag, false, lev[0], order[0]);
+                std::thread tu([&] { unit_order(nb, rp, ci, diag, true, lev[1], nl[1], order[1]); });
+                unit_order(nb, rp, ci, diag, false, lev[0], nl[0], order[0]);
                 tu.join();
             }
         } else if (rows) {
             for (int u = 0; u < 2; ++u) level_order(nb, lev[u], nl[u], u == 1, order[u]);
         }
     }
-    // imports of every unit (unit kernel): the distinct rows of other units that
-    // its rows depend on, in order of first use
-    std::vector<int> imp_ptr[2], imp_col[2], pos[2];
+    // Row schedules: the position of every row, and the per-plan column array cx.
+    // A dependency on a row of the same workgroup (64 consecutive positions) is
+    // ~slot, read from LDS.  With the unit kernel a dependency on another unit is
+    // ~(64 + k), k its import slot: the unit's imports are the distinct rows of
+    // other units its rows depend on, in order of first use.  One pass per
+    // triangle over the units, split over host threads (units are independent).
+    std::vector<int> imp_ptr[2], imp_col[2], cx;
     int max_imp = 0;
-    if (rows && units && unit_kernel) {
-        std::vector<int> stamp(nb, -1);
+    if (rows) {
+        cx = ci;
         const int nu = (nb + kRowsPerWG - 1) / kRowsPerWG;
+        const bool want_imports = units && unit_kernel;
         for (int u = 0; u < 2; ++u) {
-            pos[u].resize(nb);
-            for (int q = 0; q < nb; ++q) pos[u][order[u][q]] = q;
-            std::fill(stamp.begin(), stamp.end(), -1);
-            imp_ptr[u].assign(nu + 1, 0);
-            for (int w = 0; w < nu; ++w) {
-                for (int q = w * kRowsPerWG; q < std::min(nb, (w + 1) * kRowsPerWG); ++q) {
-                    const int i = order[u][q];
-                    const int pb = u == 0 ? rp[i] : diag[i] + 1, pe = u == 0 ? diag[i] : rp[i + 1];
-                    for (int p = pb; p < pe; ++p) {
-                        const int c = ci[p];
-                        if (pos[u][c] / kRowsPerWG == w || stamp[c] == w) continue;
-                        stamp[c] = w;
-                        imp_col[u].push_back(c);
+            std::vector<int> at(nb);
+            parallel_ranges(nb, [&](int lo, int hi) {
+                for (int q = lo; q < hi; ++q) at[order[u][q]] = q;
+            });
+            // a unit's imports fit in the dependency count of its rows
+            std::vector<int> off(nu + 1, 0), cnt(nu, 0), buf;
+            if (want_imports) {
+                for (int w = 0; w < nu; ++w) {
+                    int d = 0;
+                    for (int q = w * kRowsPerWG; q < std::min(nb, (w + 1) * kRowsPerWG); ++q) {
+                        const int i = order[u][q];
+                        d += u == 0 ? diag[i] - rp[i] : rp[i + 1] - diag[i] - 1;
                     }
+                    off[w + 1] = off[w] + d;
                 }
-                imp_ptr[u][w + 1] = (int)imp_col[u].size();
-                max_imp = std::max(max_imp, imp_ptr[u][w + 1] - imp_ptr[u][w]);
+                buf.resize(std::max(off[nu], 1));
+            }
+            parallel_ranges(nu, [&](int w0, int w1) {
+                std::vector<int> stamp, slot;
+                if (want_imports) {
+                    stamp.assign(nb, -1);
+                    slot.resize(nb);
+                }
+                for (int w = w0; w < w1; ++w) {
+                    int k = 0;
+                    for (int q = w * kRowsPerWG; q < std::min(nb, (w + 1) * kRowsPerWG); ++q) {
+                        const int i = order[u][q];
+                        const int pb = u == 0 ? rp[i] : diag[i] + 1, pe = u == 0 ? diag[i] : rp[i + 1];
+                        for (int p = pb; p < pe; ++p) {
+                            const int c = ci[p];
+                            if (at[c] / kRowsPerWG == w) {
+                                cx[p] = ~(at[c] % kRowsPerWG);
+                            } else if (want_imports) {
+                                if (stamp[c] != w) {
+                                    stamp[c] = w;
+                                    slot[c] = k;
+                                    buf[off[w] + k++] = c;
+                                }
+                                cx[p] = ~(kRowsPerWG + slot[c]);
+                            }
+                        }
+                    }
+                    cnt[w] = k;
+                }
+            });
+            if (want_imports) {
+                imp_ptr[u].assign(nu + 1, 0);
+                for (int w = 0; w < nu; ++w) {
+                    imp_ptr[u][w + 1] = imp_ptr[u][w] + cnt[w];
+                    max_imp = std::max(max_imp, cnt[w]);
+                }
+                imp_col[u].resize(imp_ptr[u][nu]);
+                parallel_ranges(nu, [&](int w0, int w1) {
+                    for (int w = w0; w < w1; ++w)
+                        std::copy(buf.begin() + off[w], buf.begin() + off[w] + cnt[w], imp_col[u].begin() + imp_ptr[u][w]);
+                });
             }
         }
-        if (max_imp > kUnitImpMax) unit_kernel = false;
+        if (want_imports && max_imp > kUnitImpMax) {
+            // too many imports for the unit kernel's LDS: the row kernel reads
+            // dependencies on other units from the solution vector (global columns)
+            unit_kernel = false;
+            parallel_ranges(nnzb, [&](int lo, int hi) {
+                for (int p = lo; p < hi; ++p)
+                    if (cx[p] < 0 && ~cx[p] >= kRowsPerWG) cx[p] = ci[p];
+            });
+        }
     }
     unit_kernel = unit_kernel && rows && units;
     const int row_lds = unit_kernel ? std::max(kUnitLds, (kRowsPerWG + max_imp) * 32) : units ? kUnitRowLds : kRowLds;
@@ -1236,7 +1342,6 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
             P->maxloc[u] = std::max(P->maxloc[u], b.y - a.w);
         }
     }
-    std::vector<int4> info;
     for (int u = 0; u < 2 && ok && rows; ++u) ok = hipMalloc(&P->d_info[u], sizeof(int4) * nb) == hipSuccess;
     if (ok && rows) ok = hipMalloc(&P->d_cx, sizeof(int) * (size_t)nnzb) == hipSuccess;
     for (int u = 0; u < 2 && ok && unit_kernel; ++u)
@@ -1250,69 +1355,68 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
              hipMalloc(&P->d_loc_items[u], sizeof(int) * std::max<size_t>(litems[u].size(), 1)) == hipSuccess;
     }
     if (!ok) {
+        (void)hipGetLastError();
         isai_bsr_sptrsv_destroy(P);
         set_error("isai_bsr_sptrsv_analysis: hipMalloc failed");
         return ISAI_ERR_ALLOC;
     }
-    ISAI_HIP(hipMemcpy(P->d_diag, diag.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
-    for (int u = 0; u < 2 && rows; ++u) {
-        info.assign(nb, make_int4(0, 0, 0, 0));
-        for (int q = 0; q < nb; ++q) {
-            const int i = order[u][q];
-            info[q] = u == 0 ? make_int4(i, rp[i], diag[i], 1) : make_int4(i, diag[i] + 1, rp[i + 1], 1);
-        }
-        ISAI_HIP(hipMemcpy(P->d_info[u], info.data(), sizeof(int4) * info.size(), hipMemcpyHostToDevice));
-    }
-    if (rows) {
-        // row kernel: a dependency on a row of the same workgroup (64 consecutive
-        // positions) becomes ~slot, read from LDS; unit kernel: every dependency is
-        // ~slot, an import k being slot 64 + k
-        std::vector<int> cx(ci), at(nb), imp_slot(nb);
-        for (int u = 0; u < 2; ++u) {
-            for (int q = 0; q < nb; ++q) at[order[u][q]] = q;
-            const int nu = (nb + kRowsPerWG - 1) / kRowsPerWG;
-            for (int w = 0; w < nu; ++w) {
-                if (unit_kernel)
-                    for (int k = imp_ptr[u][w]; k < imp_ptr[u][w + 1]; ++k) imp_slot[imp_col[u][k]] = k - imp_ptr[u][w];
-                for (int q = w * kRowsPerWG; q < std::min(nb, (w + 1) * kRowsPerWG); ++q) {
-                    const int i = order[u][q];
-                    const int pb = u == 0 ? rp[i] : diag[i] + 1, pe = u == 0 ? diag[i] : rp[i + 1];
-                    for (int p = pb; p < pe; ++p) {
-                        const int c = ci[p];
-                        if (at[c] / kRowsPerWG == w)
-                            cx[p] = ~(at[c] % kRowsPerWG);
-                        else if (unit_kernel)
-                            cx[p] = ~(kRowsPerWG + imp_slot[c]);
+    // uploads; the plan is released if one fails
+    auto upload = [&]() -> int {
+        ISAI_HIP(hipMemcpy(P->d_diag, diag.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+        if (rows) {
+            std::vector<int4> info(nb);
+            for (int u = 0; u < 2; ++u) {
+                parallel_ranges(nb, [&](int lo, int hi) {
+                    for (int q = lo; q < hi; ++q) {
+                        const int i = order[u][q];
+                        info[q] = u == 0 ? make_int4(i, rp[i], diag[i], 1) : make_int4(i, diag[i] + 1, rp[i + 1], 1);
                     }
+                });
+                ISAI_HIP(hipMemcpy(P->d_info[u], info.data(), sizeof(int4) * nb, hipMemcpyHostToDevice));
+                if (unit_kernel) {
+                    ISAI_HIP(hipMemcpy(P->d_imp_ptr[u], imp_ptr[u].data(), sizeof(int) * imp_ptr[u].size(),
+                                       hipMemcpyHostToDevice));
+                    if (!imp_col[u].empty())
+                        ISAI_HIP(hipMemcpy(P->d_imp_col[u], imp_col[u].data(), sizeof(int) * imp_col[u].size(),
+                                           hipMemcpyHostToDevice));
                 }
             }
-            if (unit_kernel) {
-                ISAI_HIP(hipMemcpy(P->d_imp_ptr[u], imp_ptr[u].data(), sizeof(int) * imp_ptr[u].size(),
-                                   hipMemcpyHostToDevice));
-                if (!imp_col[u].empty())
-                    ISAI_HIP(hipMemcpy(P->d_imp_col[u], imp_col[u].data(), sizeof(int) * imp_col[u].size(),
-                                       hipMemcpyHostToDevice));
-            }
+            ISAI_HIP(hipMemcpy(P->d_cx, cx.data(), sizeof(int) * (size_t)nnzb, hipMemcpyHostToDevice));
         }
-        ISAI_HIP(hipMemcpy(P->d_cx, cx.data(), sizeof(int) * (size_t)nnzb, hipMemcpyHostToDevice));
-    }
-    for (int u = 0; u < 2 && !rows; ++u) {
-        ISAI_HIP(hipMemcpy(P->d_chunks[u], chunks[u].data(), sizeof(int4) * chunks[u].size(), hipMemcpyHostToDevice));
-        ISAI_HIP(hipMemcpy(P->d_rem_ptr[u], rptr[u].data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
-        ISAI_HIP(hipMemcpy(P->d_loc_ptr[u], lptr[u].data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
-        if (!ritems[u].empty())
-            ISAI_HIP(hipMemcpy(P->d_rem_items[u], ritems[u].data(), sizeof(int) * ritems[u].size(), hipMemcpyHostToDevice));
-        if (!litems[u].empty())
-            ISAI_HIP(hipMemcpy(P->d_loc_items[u], litems[u].data(), sizeof(int) * litems[u].size(), hipMemcpyHostToDevice));
+        for (int u = 0; u < 2 && !rows; ++u) {
+            ISAI_HIP(hipMemcpy(P->d_chunks[u], chunks[u].data(), sizeof(int4) * chunks[u].size(), hipMemcpyHostToDevice));
+            ISAI_HIP(hipMemcpy(P->d_rem_ptr[u], rptr[u].data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+            ISAI_HIP(hipMemcpy(P->d_loc_ptr[u], lptr[u].data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+            if (!ritems[u].empty())
+                ISAI_HIP(hipMemcpy(P->d_rem_items[u], ritems[u].data(), sizeof(int) * ritems[u].size(),
+                                   hipMemcpyHostToDevice));
+            if (!litems[u].empty())
+                ISAI_HIP(hipMemcpy(P->d_loc_items[u], litems[u].data(), sizeof(int) * litems[u].size(),
+                                   hipMemcpyHostToDevice));
+        }
+        return ISAI_OK;
+    };
+    if (const int us = upload(); us != ISAI_OK) {
+        isai_bsr_sptrsv_destroy(P);
+        return us;
     }
     Counters h{};
     // steps without progress before a chunk gives up on its missing inputs: a
     // step is one agent-scope poll round (about a microsecond), so this bound is
     // tens of milliseconds, far above any legitimate wait inside one solve.
     h.spin_limit = 1 << 16;
-    ISAI_HIP(hipMemcpy(P->d_cnt, &h, sizeof(h), hipMemcpyHostToDevice));
+    const hipError_t e = hipMemcpy(P->d_cnt, &h, sizeof(h), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        isai_bsr_sptrsv_destroy(P);
+        return hip_fail(e, "hipMemcpy", __FILE__, __LINE__);
+    }
+    const int us = isai_bsr_sptrsv_update_values(P, d_LU, stream);
+    if (us != ISAI_OK) {
+        isai_bsr_sptrsv_destroy(P);
+        return us;
+    }
     *plan = P;
-    return isai_bsr_sptrsv_update_values(P, d_LU, stream);
+    return ISAI_OK;
 }
 
 int isai_bsr_sptrsv_update_values(isai_sptrsv_plan_t P, const double *d_LU, isai_stream_t stream) {
