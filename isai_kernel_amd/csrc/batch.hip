@@ -99,24 +99,56 @@ __device__ __forceinline__ bool row_context(const BatchDev &B, const int *rp, in
     return true;
 }
 
+// Multi-vectors are FP64, or FP32 where they hold the outputs of a reduced-precision
+// ISAI product (FP32 accumulations, so the FP32 copy is exact): loads widen, stores
+// of FP32 results narrow without rounding.
+template <class T>
+__device__ __forceinline__ v2d load2(const T *p) {
+    if constexpr (sizeof(T) == 8) {
+        return *(const v2d *)p;
+    } else {
+        const float2 f = *(const float2 *)p;
+        return v2d{(double)f.x, (double)f.y};
+    }
+}
+// four consecutive entries (16-byte aligned) rounded to FP32
+template <class T>
+__device__ __forceinline__ v4f load4f(const T *p) {
+    if constexpr (sizeof(T) == 8) {
+        const v2d a = *(const v2d *)p, b = *(const v2d *)(p + 2);
+        return v4f{(float)a.x, (float)a.y, (float)b.x, (float)b.y};
+    } else {
+        return *(const v4f *)p;
+    }
+}
+template <class T>
+__device__ __forceinline__ void store4f(T *p, v4f v) {
+    if constexpr (sizeof(T) == 8) {
+        *(v2d *)p = v2d{(double)v.x, (double)v.y};
+        *(v2d *)(p + 2) = v2d{(double)v.z, (double)v.w};
+    } else {
+        *(v4f *)p = v;
+    }
+}
+
 // FP64 values: lane l holds row l/2, columns 2(l%2)..+1; FMA chain per lane, then
 // the two lanes of a row are added.
-template <int R>
+template <int R, class XT>
 __device__ __forceinline__ void f64_row(const RowCtx &q, const int *__restrict__ ci, const v2d *__restrict__ val,
-                                        const double *__restrict__ X, double *__restrict__ Y) {
+                                        const XT *__restrict__ X, double *__restrict__ Y) {
     constexpr int KU = R <= 2 ? 4 : (R <= 6 ? 2 : 1);
     const int h = q.lane & 1;
     double acc0[R], acc1[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) acc0[j] = acc1[j] = 0.0;
-    const double *xb = X + q.v0 + 2 * h;
+    const XT *xb = X + q.v0 + 2 * h;
 #pragma unroll KU
     for (int k = q.rs; k < q.re; ++k) {
         const int c = ci[k] - q.base;
         const v2d a = __builtin_nontemporal_load(&val[(size_t)k * 8 + q.lane]);
         v2d x[R];
 #pragma unroll
-        for (int j = 0; j < R; ++j) x[j] = *(const v2d *)(xb + (size_t)c * 4 + j * q.step);
+        for (int j = 0; j < R; ++j) x[j] = load2(xb + (size_t)c * 4 + j * q.step);
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             acc0[j] = fma(a.x, x[j].x, acc0[j]);
@@ -138,9 +170,9 @@ __device__ __forceinline__ void f64_row(const RowCtx &q, const int *__restrict__
 __device__ __forceinline__ float b16_lo(unsigned int w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float b16_hi(unsigned int w) { return __uint_as_float(w & 0xffff0000u); }
 
-template <bool BF16, int R>
+template <bool BF16, int R, class XT, class YT>
 __device__ __forceinline__ void lowp_row(const RowCtx &q, const int *__restrict__ ci, const v4u *__restrict__ val16,
-                                         const double *__restrict__ X, double *__restrict__ Y) {
+                                         const XT *__restrict__ X, YT *__restrict__ Y) {
     constexpr int BPB = BF16 ? 4 : 2;
     constexpr int LPB = 8 / BPB;
     constexpr int KU = R <= 4 ? 2 : 1;
@@ -155,16 +187,13 @@ __device__ __forceinline__ void lowp_row(const RowCtx &q, const int *__restrict_
             const int c = ci[kb] - q.base;
             const v4u w = BF16 ? __builtin_nontemporal_load(&val16[(size_t)kb * LPB + sub])
                                : val16[(size_t)kb * LPB + sub];
-            const double *xc = X + q.v0 + (size_t)c * 4;
-            v2d x01[R], x23[R];
+            const XT *xc = X + q.v0 + (size_t)c * 4;
+            v4f xf[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) xf[j] = load4f(xc + j * q.step);
 #pragma unroll
             for (int j = 0; j < R; ++j) {
-                x01[j] = *(const v2d *)(xc + j * q.step);
-                x23[j] = *(const v2d *)(xc + j * q.step + 2);
-            }
-#pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const float f0 = (float)x01[j].x, f1 = (float)x01[j].y, f2 = (float)x23[j].x, f3 = (float)x23[j].y;
+                const float f0 = xf[j].x, f1 = xf[j].y, f2 = xf[j].z, f3 = xf[j].w;
                 if (BF16) {
                     acc0[j] = fmaf(b16_lo(w.x), f0, acc0[j]);
                     acc0[j] = fmaf(b16_hi(w.x), f1, acc0[j]);
@@ -192,15 +221,15 @@ __device__ __forceinline__ void lowp_row(const RowCtx &q, const int *__restrict_
         }
     }
     if (b != 0) return;
-    double *yr = Y + q.v0 + (size_t)(q.br - q.base) * 4;
+    YT *yr = Y + q.v0 + (size_t)(q.br - q.base) * 4;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
         if (!(q.mask & (1u << j))) continue;
         if (BF16) {
-            yr[j * q.step + 2 * sub] = (double)acc0[j];
-            yr[j * q.step + 2 * sub + 1] = (double)acc1[j];
+            yr[j * q.step + 2 * sub] = (YT)acc0[j];
+            yr[j * q.step + 2 * sub + 1] = (YT)acc1[j];
         } else {
-            yr[j * q.step + sub] = (double)acc0[j];
+            yr[j * q.step + sub] = (YT)acc0[j];
         }
     }
 }
@@ -222,9 +251,9 @@ __device__ __forceinline__ auto load_val(const void *__restrict__ val, size_t id
     }
 }
 
-template <int PREC, int R>
+template <int PREC, int R, class XT, class YT>
 __device__ __forceinline__ void col_row(const RowCtx &q, const int *__restrict__ ci, const void *__restrict__ val,
-                                        const double *__restrict__ X, double *__restrict__ Y) {
+                                        const XT *__restrict__ X, YT *__restrict__ Y) {
     using Acc = typename std::conditional<PREC == 0, double, float>::type;
     constexpr int RH = (R + 1) / 2;
     constexpr int KU = R <= 4 ? 2 : 1;
@@ -237,7 +266,7 @@ __device__ __forceinline__ void col_row(const RowCtx &q, const int *__restrict__
     for (int u = 0; u < RH; ++u)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[u][r] = Acc(0);
-    const double *xb = X + q.v0 + c;
+    const XT *xb = X + q.v0 + c;
 #pragma unroll KU
     for (int k = q.rs; k < q.re; ++k) {
         const int col = ci[k] - q.base;
@@ -262,12 +291,12 @@ __device__ __forceinline__ void col_row(const RowCtx &q, const int *__restrict__
             acc[u][r] += __shfl_xor(acc[u][r], 1);
             acc[u][r] += __shfl_xor(acc[u][r], 2);
         }
-    double *yr = Y + q.v0 + (size_t)(q.br - q.base) * 4 + c;
+    YT *yr = Y + q.v0 + (size_t)(q.br - q.base) * 4 + c;
 #pragma unroll
     for (int u = 0; u < RH; ++u) {
         const int j = 2 * u + par;
         const Acc out = c == 0 ? acc[u][0] : (c == 1 ? acc[u][1] : (c == 2 ? acc[u][2] : acc[u][3]));
-        if (j < R && (q.mask & (1u << j))) yr[j * q.step] = (double)out;
+        if (j < R && (q.mask & (1u << j))) yr[j * q.step] = (YT)out;
     }
 }
 
@@ -305,9 +334,10 @@ __device__ __forceinline__ void mfma_row(const BatchDev &B, const int *rp, int n
 }
 
 // Wave unit u: block rows u * kMfmaF64Rows .. + 3 (lane = lane in the wave).
+template <class XT>
 __device__ __forceinline__ void mfma_f64_unit(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
                                               const double *__restrict__ val, const BatchDev &B,
-                                              const double *__restrict__ X, double *__restrict__ Y, int u, int lane,
+                                              const XT *__restrict__ X, double *__restrict__ Y, int u, int lane,
                                               int all) {
     const int x = lane & 3, b = (lane >> 2) & 3, y = lane >> 4;
     MfmaRow m;
@@ -320,7 +350,7 @@ __device__ __forceinline__ void mfma_f64_unit(int nb, const int *__restrict__ rp
     G = max(G, __shfl_xor(G, 8));
     if (L == 0) return;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const double *xb = X + m.v0 + y;
+    const XT *xb = X + m.v0 + y;
     // operands of step t (one block of each of the four rows); zero when past the row
     auto load_step = [&](int t, double &a, double (&bv)[4]) {
         const bool live = t < m.len;
@@ -334,7 +364,7 @@ __device__ __forceinline__ void mfma_f64_unit(int nb, const int *__restrict__ rp
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int n = 4 * g + x;
-            bv[g] = (g < G && live && n < m.R) ? xb[(size_t)c * 4 + n * m.step] : 0.0;
+            bv[g] = (g < G && live && n < m.R) ? (double)xb[(size_t)c * 4 + n * m.step] : 0.0;
         }
     };
     int t = 0;
@@ -366,14 +396,15 @@ __device__ __forceinline__ void mfma_f64_unit(int nb, const int *__restrict__ rp
     }
 }
 
+template <class XT>
 __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_f64_kernel(int nb, const int *__restrict__ rp,
                                                                     const int *__restrict__ ci,
                                                                     const double *__restrict__ val, BatchDev B,
-                                                                    const double *__restrict__ X,
+                                                                    const XT *__restrict__ X,
                                                                     double *__restrict__ Y, int remap, int all) {
     const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    mfma_f64_unit(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 64) + (int)(threadIdx.x >> 6),
-                  (int)(threadIdx.x & 63), all);
+    mfma_f64_unit<XT>(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 64) + (int)(threadIdx.x >> 6),
+                      (int)(threadIdx.x & 63), all);
 }
 
 // v_mfma_f32_4x4x1f32 (sixteen 4x4x1 products): A[b][i] at lane i + 4b, B[b][j] at
@@ -382,10 +413,10 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_f64_kernel(int nb, cons
 // B-operand lane (j, b), the four x values of right-hand side j of each group
 // (rounded to FP32); four K = 1 MFMAs per group and block, FP32 accumulation.
 // Wave unit u: block rows u * kMfmaLowpRows .. + 15.
-template <bool BF16>
+template <bool BF16, class XT, class YT>
 __device__ __forceinline__ void mfma_lowp_unit(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
                                                const void *__restrict__ val, const BatchDev &B,
-                                               const double *__restrict__ X, double *__restrict__ Y, int u, int lane,
+                                               const XT *__restrict__ X, YT *__restrict__ Y, int u, int lane,
                                                int all) {
     const int q = lane & 3, b = lane >> 2;  // q: row i for A, right-hand side j for B and C
     MfmaRow m;
@@ -430,12 +461,11 @@ __device__ __forceinline__ void mfma_lowp_unit(int nb, const int *__restrict__ r
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) xv[g][kk] = 0.f;
             if (g < G && live && n < m.R) {
-                const double *xp = X + m.v0 + n * m.step + (size_t)c * 4;
-                const v2d x01 = *(const v2d *)xp, x23 = *(const v2d *)(xp + 2);
-                xv[g][0] = (float)x01.x;
-                xv[g][1] = (float)x01.y;
-                xv[g][2] = (float)x23.x;
-                xv[g][3] = (float)x23.y;
+                const v4f xq = load4f(X + m.v0 + n * m.step + (size_t)c * 4);
+                xv[g][0] = xq.x;
+                xv[g][1] = xq.y;
+                xv[g][2] = xq.z;
+                xv[g][3] = xq.w;
             }
         }
     };
@@ -463,23 +493,19 @@ __device__ __forceinline__ void mfma_lowp_unit(int nb, const int *__restrict__ r
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
         const int n = 4 * g + q;
-        if (g < G && n < m.R && (m.mask & (1u << n))) {
-            double *yr = Y + m.v0 + n * m.step + (size_t)(m.br - m.base) * 4;
-            *(v2d *)yr = v2d{(double)acc[g].x, (double)acc[g].y};
-            *(v2d *)(yr + 2) = v2d{(double)acc[g].z, (double)acc[g].w};
-        }
+        if (g < G && n < m.R && (m.mask & (1u << n))) store4f(Y + m.v0 + n * m.step + (size_t)(m.br - m.base) * 4, acc[g]);
     }
 }
 
-template <bool BF16>
+template <bool BF16, class XT, class YT>
 __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_lowp_kernel(int nb, const int *__restrict__ rp,
                                                                      const int *__restrict__ ci,
                                                                      const void *__restrict__ val, BatchDev B,
-                                                                     const double *__restrict__ X,
-                                                                     double *__restrict__ Y, int remap, int all) {
+                                                                     const XT *__restrict__ X,
+                                                                     YT *__restrict__ Y, int remap, int all) {
     const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    mfma_lowp_unit<BF16>(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 64) + (int)(threadIdx.x >> 6),
-                         (int)(threadIdx.x & 63), all);
+    mfma_lowp_unit<BF16, XT, YT>(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 64) + (int)(threadIdx.x >> 6),
+                                 (int)(threadIdx.x & 63), all);
 }
 
 #define ISAI_RHS_CASES(CALL) \
@@ -487,18 +513,18 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_lowp_kernel(int nb, con
     CALL(9) CALL(10) CALL(11) CALL(12) CALL(13) CALL(14) CALL(15) CALL(16)
 
 // MAP 0: row-pair mapping (f64_row / lowp_row); MAP 1: column-split mapping.
-template <int MAP>
+template <int MAP, class XT>
 __device__ __forceinline__ void f64_unit(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
-                                         const v2d *__restrict__ val, const BatchDev &B, const double *__restrict__ X,
+                                         const v2d *__restrict__ val, const BatchDev &B, const XT *__restrict__ X,
                                          double *__restrict__ Y, int br, int lane, int all) {
     RowCtx q;
     int r;
     if (!row_context(B, rp, nb, br, lane, all, q, r)) return;
     switch (r) {
-#define ISAI_F64_CASE(N)                          \
-    case N:                                       \
-        if (MAP == 0) f64_row<N>(q, ci, val, X, Y); \
-        else col_row<0, N>(q, ci, val, X, Y);     \
+#define ISAI_F64_CASE(N)                              \
+    case N:                                           \
+        if (MAP == 0) f64_row<N, XT>(q, ci, val, X, Y); \
+        else col_row<0, N>(q, ci, val, X, Y);         \
         break;
         ISAI_RHS_CASES(ISAI_F64_CASE)
 #undef ISAI_F64_CASE
@@ -507,21 +533,20 @@ __device__ __forceinline__ void f64_unit(int nb, const int *__restrict__ rp, con
     }
 }
 
-template <int MAP>
+template <int MAP, class XT>
 __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64_kernel(int nb, const int *__restrict__ rp,
                                                                const int *__restrict__ ci, const v2d *__restrict__ val,
-                                                               BatchDev B, const double *__restrict__ X,
+                                                               BatchDev B, const XT *__restrict__ X,
                                                                double *__restrict__ Y, int remap, int all) {
     const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    f64_unit<MAP>(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3),
-                  (int)(threadIdx.x & 7), all);
+    f64_unit<MAP, XT>(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3),
+                      (int)(threadIdx.x & 7), all);
 }
 
-template <bool BF16, int MAP>
+template <bool BF16, int MAP, class XT, class YT>
 __device__ __forceinline__ void lowp_unit(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
-                                          const v4u *__restrict__ val16, const BatchDev &B,
-                                          const double *__restrict__ X, double *__restrict__ Y, int br, int lane,
-                                          int all) {
+                                          const v4u *__restrict__ val16, const BatchDev &B, const XT *__restrict__ X,
+                                          YT *__restrict__ Y, int br, int lane, int all) {
     RowCtx q;
     int r;
     if (!row_context(B, rp, nb, br, lane, all, q, r)) return;
@@ -538,15 +563,15 @@ __device__ __forceinline__ void lowp_unit(int nb, const int *__restrict__ rp, co
     }
 }
 
-template <bool BF16, int MAP>
+template <bool BF16, int MAP, class XT, class YT>
 __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_lowp_kernel(int nb, const int *__restrict__ rp,
                                                                 const int *__restrict__ ci,
                                                                 const v4u *__restrict__ val16, BatchDev B,
-                                                                const double *__restrict__ X, double *__restrict__ Y,
+                                                                const XT *__restrict__ X, YT *__restrict__ Y,
                                                                 int remap, int all) {
     const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    lowp_unit<BF16, MAP>(nb, rp, ci, val16, B, X, Y, wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3),
-                         (int)(threadIdx.x & 7), all);
+    lowp_unit<BF16, MAP, XT, YT>(nb, rp, ci, val16, B, X, Y, wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3),
+                                 (int)(threadIdx.x & 7), all);
 }
 
 // --------------------------------------------------------------- reductions
@@ -741,7 +766,8 @@ __device__ __forceinline__ void bsnorm_final_body(const BatchDev &B, int p, int 
 }
 
 // pairs converged on s: x += alpha ph
-__device__ __forceinline__ void bxs_body(int t, double *x, const double *__restrict__ ph, const BatchDev &B) {
+template <class PT>
+__device__ __forceinline__ void bxs_body(int t, double *x, const PT *__restrict__ ph, const BatchDev &B) {
     if (t >= B.ntiles) return;
     const PairState &S = B.ps[B.tile_pair[t]];
     if (S.status != 2) return;
@@ -751,7 +777,7 @@ __device__ __forceinline__ void bxs_body(int t, double *x, const double *__restr
 #pragma unroll
     for (int k = 0; k < kTile / kRed; ++k) {
         const size_t i = base + k * kRed + tid;
-        x[i] += alpha * ph[i];
+        x[i] += alpha * (double)ph[i];
     }
 }
 
@@ -767,8 +793,9 @@ __device__ __forceinline__ void bomega_final_body(const BatchDev &B, int p, int 
 
 // x += alpha ph + omega sh; r = s - omega t; partials ||r||^2 and rhat.r (the next
 // iteration's rho)
+template <class PT>
 __device__ __forceinline__ void bxr_body(int t, int sub, double *x, double *r, const double *__restrict__ rh,
-                                         const double *__restrict__ ph, const double *__restrict__ sh,
+                                         const PT *__restrict__ ph, const PT *__restrict__ sh,
                                          const double *__restrict__ s, const double *__restrict__ tv,
                                          const BatchDev &B) {
     const bool live = tile_live(B, t);
@@ -781,7 +808,7 @@ __device__ __forceinline__ void bxr_body(int t, int sub, double *x, double *r, c
 #pragma unroll
         for (int k = 0; k < kTile / kRed; ++k) {
             const size_t i = base + k * kRed + tid;
-            x[i] += alpha * ph[i] + omega * sh[i];
+            x[i] += alpha * (double)ph[i] + omega * (double)sh[i];
             const double ri = s[i] - omega * tv[i];
             r[i] = ri;
             acc = fma(ri, ri, acc);
@@ -878,11 +905,13 @@ __global__ __launch_bounds__(kRed) void bs_kernel(const double *__restrict__ r, 
                                                   BatchDev B) {
     bs_body(blockIdx.x, 0, r, v, s, B);
 }
-__global__ __launch_bounds__(kRed) void bxs_kernel(double *x, const double *__restrict__ ph, BatchDev B) {
+template <class PT>
+__global__ __launch_bounds__(kRed) void bxs_kernel(double *x, const PT *__restrict__ ph, BatchDev B) {
     bxs_body(blockIdx.x, x, ph, B);
 }
+template <class PT>
 __global__ __launch_bounds__(kRed) void bxr_kernel(double *x, double *r, const double *__restrict__ rh,
-                                                   const double *__restrict__ ph, const double *__restrict__ sh,
+                                                   const PT *__restrict__ ph, const PT *__restrict__ sh,
                                                    const double *__restrict__ s, const double *__restrict__ tv,
                                                    BatchDev B) {
     bxr_body(blockIdx.x, 0, x, r, rh, ph, sh, s, tv, B);
@@ -957,7 +986,8 @@ struct PersistArgs {
     const void *Lval, *Uval;           // ISAI factors in storage precision
     const double *b;                   // right-hand sides
     double *x;                         // solutions (initial guesses in)
-    double *r, *rh, *p, *v, *ph, *s, *sh, *t, *tmp;
+    double *r, *rh, *p, *v, *s, *t;
+    void *ph, *sh, *tmp;               // FP32 with FP32 / BF16 factors (exact copies of their results)
     double tol;
     double *relres;
 };
@@ -968,23 +998,28 @@ enum Operand { kOpA = 0, kOpML = 1, kOpMU = 2 };
 // autotuner's mapping (0 row-pair, 1 column-split, 2 MFMA), PREC the operand's
 // value precision.  Units: 8-lane groups (one block row) for MAP 0/1, waves of
 // kMfmaF64Rows / kMfmaLowpRows block rows for MAP 2.
-template <int MAP, int PREC>
+template <int MAP, int PREC, class XT, class YT>
 __device__ __forceinline__ void spmm_phase(int nb, const int *rp, const int *ci, const void *val,
-                                           const BatchDev &B, const double *X, double *Y, int all) {
+                                           const BatchDev &B, const XT *X, YT *Y, int all) {
     const int lane = threadIdx.x & 63;
-    if (MAP == 2) {
+    if constexpr (MAP == 2) {
         const int rows = PREC == ISAI_PREC_FP64 ? kMfmaF64Rows : kMfmaLowpRows;
         const int units = (nb + rows - 1) / rows;
         const int wpg = kPersistThreads / 64;
         for (int u = blockIdx.x * wpg + (int)(threadIdx.x >> 6); u < units; u += gridDim.x * wpg) {
-            if (PREC == ISAI_PREC_FP64) mfma_f64_unit(nb, rp, ci, (const double *)val, B, X, Y, u, lane, all);
-            else mfma_lowp_unit<PREC == ISAI_PREC_BF16>(nb, rp, ci, val, B, X, Y, u, lane, all);
+            if constexpr (PREC == ISAI_PREC_FP64)
+                mfma_f64_unit<XT>(nb, rp, ci, (const double *)val, B, X, Y, u, lane, all);
+            else
+                mfma_lowp_unit<PREC == ISAI_PREC_BF16, XT, YT>(nb, rp, ci, val, B, X, Y, u, lane, all);
         }
     } else {
         const int gpg = kPersistThreads / 8;
         for (int br = blockIdx.x * gpg + (int)(threadIdx.x >> 3); br < nb; br += gridDim.x * gpg) {
-            if (PREC == ISAI_PREC_FP64) f64_unit<MAP>(nb, rp, ci, (const v2d *)val, B, X, Y, br, lane & 7, all);
-            else lowp_unit<PREC == ISAI_PREC_BF16, MAP>(nb, rp, ci, (const v4u *)val, B, X, Y, br, lane & 7, all);
+            if constexpr (PREC == ISAI_PREC_FP64)
+                f64_unit<MAP, XT>(nb, rp, ci, (const v2d *)val, B, X, Y, br, lane & 7, all);
+            else
+                lowp_unit<PREC == ISAI_PREC_BF16, MAP, XT, YT>(nb, rp, ci, (const v4u *)val, B, X, Y, br, lane & 7,
+                                                               all);
         }
     }
 }
@@ -996,18 +1031,16 @@ __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(Persi
     const int wpg = kPersistThreads / 64, tpg = kPersistThreads / kRed;
     Ctl *ctl = B.ctl;
     unsigned long long target = 0;
+    using IT = typename std::conditional<PRECM == ISAI_PREC_FP64, double, float>::type;  // ISAI outputs
+    IT *const ph = (IT *)a.ph, *const sh = (IT *)a.sh, *const tmp = (IT *)a.tmp;
     // tile phases: the workgroup's tile groups take tiles t0 + sub together, so
     // every group meets the same number of reduction barriers
     auto tiles_end = [&](int t0) { return t0 < B.ntiles; };
-    auto A = [&](const double *X, double *Y, int all) {
+    auto A = [&](const auto *X, double *Y, int all) {
         spmm_phase<MAPA, ISAI_PREC_FP64>(a.nb, a.rp, a.ci, a.val, B, X, Y, all);
     };
-    auto ML = [&](const double *X, double *Y) {
-        spmm_phase<MAPM, PRECM>(a.nb, a.Lrow, a.Lcol, a.Lval, B, X, Y, 0);
-    };
-    auto MU = [&](const double *X, double *Y) {
-        spmm_phase<MAPM, PRECM>(a.nb, a.Urow, a.Ucol, a.Uval, B, X, Y, 0);
-    };
+    auto ML = [&](const double *X, IT *Y) { spmm_phase<MAPM, PRECM>(a.nb, a.Lrow, a.Lcol, a.Lval, B, X, Y, 0); };
+    auto MU = [&](const IT *X, IT *Y) { spmm_phase<MAPM, PRECM>(a.nb, a.Urow, a.Ucol, a.Uval, B, X, Y, 0); };
 #define ISAI_SYNC() \
     if (!grid_barrier(ctl, target)) return
 #define ISAI_TILES(BODY) \
@@ -1035,11 +1068,11 @@ __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(Persi
         for (int it = 0; it <= ctl->maxit; ++it) {
             ISAI_TILES(bp_body(t0 + sub, a.r, a.p, a.v, B));
             ISAI_SYNC();
-            ML(a.p, a.tmp);
+            ML(a.p, tmp);
             ISAI_SYNC();
-            MU(a.tmp, a.ph);
+            MU(tmp, ph);
             ISAI_SYNC();
-            A(a.ph, a.v, 0);
+            A(ph, a.v, 0);
             ISAI_SYNC();
             ISAI_TILES(tile_dot_body(t0 + sub, sub, a.rh, a.v, nullptr, B));
             ISAI_SYNC();
@@ -1051,18 +1084,18 @@ __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(Persi
             ISAI_SYNC();
             // x of the pairs converged on s, and M_L s of the running ones (the two
             // touch different pairs and vectors)
-            ISAI_TILES(bxs_body(t0 + sub, a.x, a.ph, B));
-            ML(a.s, a.tmp);
+            ISAI_TILES(bxs_body(t0 + sub, a.x, ph, B));
+            ML(a.s, tmp);
             ISAI_SYNC();
-            MU(a.tmp, a.sh);
+            MU(tmp, sh);
             ISAI_SYNC();
-            A(a.sh, a.t, 0);
+            A(sh, a.t, 0);
             ISAI_SYNC();
             ISAI_TILES(tile_dot_body(t0 + sub, sub, a.t, a.s, a.t, B));
             ISAI_SYNC();
             ISAI_PAIRS(bomega_final_body(B, p, lane));
             ISAI_SYNC();
-            ISAI_TILES(bxr_body(t0 + sub, sub, a.x, a.r, a.rh, a.ph, a.sh, a.s, a.t, B));
+            ISAI_TILES(bxr_body(t0 + sub, sub, a.x, a.r, a.rh, ph, sh, a.s, a.t, B));
             ISAI_SYNC();
             // end of this iteration and rho / beta of the next
             ISAI_PAIRS({
@@ -1176,10 +1209,12 @@ struct isai_batch {
 
 namespace {
 
-// Y = op X over the pairs still running (all = 1: over every pair)
-template <int MAP>
-void launch_spmm(isai_batch *b, Operand op, int prec, const int *rp, const int *ci, const void *v, const double *X,
-                 double *Y, int all, hipStream_t st) {
+// Y = op X over the pairs still running (all = 1: over every pair).  X and Y are
+// FP64 or FP32 multi-vectors (FP32 only for the outputs of reduced-precision ISAI
+// products and the products that read them back).
+template <int MAP, class XT, class YT>
+int launch_spmm(isai_batch *b, Operand op, int prec, const int *rp, const int *ci, const void *v, const XT *X, YT *Y,
+                int all, hipStream_t st) {
     const BatchDev B = b->dev();
     const int slot = op == kOpA ? 0 : 1;
     const int threads = b->wg[slot];
@@ -1187,23 +1222,36 @@ void launch_spmm(isai_batch *b, Operand op, int prec, const int *rp, const int *
                              : (threads / 64) * (prec == ISAI_PREC_FP64 ? kMfmaF64Rows : kMfmaLowpRows);
     const unsigned grid = grid_for(b->nb, rows);
     const int remap = b->remap[slot];
-    if (MAP == 2) {
-        if (prec == ISAI_PREC_FP64)
-            bspmm_mfma_f64_kernel<<<grid, threads, 0, st>>>(b->nb, rp, ci, (const double *)v, B, X, Y, remap, all);
-        else if (prec == ISAI_PREC_FP32)
-            bspmm_mfma_lowp_kernel<false><<<grid, threads, 0, st>>>(b->nb, rp, ci, v, B, X, Y, remap, all);
+    if (prec == ISAI_PREC_FP64) {
+        if constexpr (std::is_same<YT, double>::value) {
+            if (MAP == 2)
+                bspmm_mfma_f64_kernel<XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const double *)v, B, X, Y, remap,
+                                                                    all);
+            else
+                bspmm_f64_kernel<MAP == 2 ? 0 : MAP, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v2d *)v, B,
+                                                                                  X, Y, remap, all);
+        } else {
+            set_error("batch spmm: an FP64 operand writes FP64 vectors");
+            return ISAI_ERR_INVALID_ARG;
+        }
+    } else if (MAP == 2) {
+        if (prec == ISAI_PREC_FP32)
+            bspmm_mfma_lowp_kernel<false, XT, YT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v, B, X, Y, remap, all);
         else
-            bspmm_mfma_lowp_kernel<true><<<grid, threads, 0, st>>>(b->nb, rp, ci, v, B, X, Y, remap, all);
-    } else if (prec == ISAI_PREC_FP64) {
-        bspmm_f64_kernel<MAP><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v2d *)v, B, X, Y, remap, all);
+            bspmm_mfma_lowp_kernel<true, XT, YT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v, B, X, Y, remap, all);
     } else if (prec == ISAI_PREC_FP32) {
-        bspmm_lowp_kernel<false, MAP><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, remap, all);
+        bspmm_lowp_kernel<false, MAP == 2 ? 0 : MAP, XT, YT>
+            <<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, remap, all);
     } else {
-        bspmm_lowp_kernel<true, MAP><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, remap, all);
+        bspmm_lowp_kernel<true, MAP == 2 ? 0 : MAP, XT, YT>
+            <<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, remap, all);
     }
+    ISAI_LAUNCH_CHECK();
+    return ISAI_OK;
 }
 
-int spmm(isai_batch *b, Operand op, const double *X, double *Y, int all, hipStream_t st) {
+template <class XT, class YT>
+int spmm_typed(isai_batch *b, Operand op, const XT *X, YT *Y, int all, hipStream_t st) {
     const int *rp = b->d_row, *ci = b->d_col;
     const void *v = b->d_val;
     int prec = ISAI_PREC_FP64;
@@ -1217,11 +1265,21 @@ int spmm(isai_batch *b, Operand op, const double *X, double *Y, int all, hipStre
         prec = f.precision;
     }
     const int map = b->map[op == kOpA ? 0 : 1];
-    if (map == 0) launch_spmm<0>(b, op, prec, rp, ci, v, X, Y, all, st);
-    else if (map == 1) launch_spmm<1>(b, op, prec, rp, ci, v, X, Y, all, st);
-    else launch_spmm<2>(b, op, prec, rp, ci, v, X, Y, all, st);
-    ISAI_LAUNCH_CHECK();
-    return ISAI_OK;
+    if (map == 0) return launch_spmm<0>(b, op, prec, rp, ci, v, X, Y, all, st);
+    if (map == 1) return launch_spmm<1>(b, op, prec, rp, ci, v, X, Y, all, st);
+    return launch_spmm<2>(b, op, prec, rp, ci, v, X, Y, all, st);
+}
+
+// x_f32 / y_f32: X / Y hold FP32 values.
+int spmm(isai_batch *b, Operand op, const void *X, bool x_f32, void *Y, bool y_f32, int all, hipStream_t st) {
+    if (x_f32 && y_f32) return spmm_typed(b, op, (const float *)X, (float *)Y, all, st);
+    if (x_f32) return spmm_typed(b, op, (const float *)X, (double *)Y, all, st);
+    if (y_f32) return spmm_typed(b, op, (const double *)X, (float *)Y, all, st);
+    return spmm_typed(b, op, (const double *)X, (double *)Y, all, st);
+}
+
+int spmm(isai_batch *b, Operand op, const double *X, double *Y, int all, hipStream_t st) {
+    return spmm(b, op, X, false, Y, false, all, st);
 }
 
 #define BTRY(x)                       \
@@ -1231,29 +1289,37 @@ int spmm(isai_batch *b, Operand op, const double *X, double *Y, int all, hipStre
     } while (0)
 
 // One BiCGSTAB iteration of every running pair: kernels only, so it can be captured.
+// With FP32 / BF16 factors the ISAI outputs (tmp, ph, sh) are held in FP32: they are
+// FP32 accumulations, so the copy is exact and the iterates are those of FP64 storage.
 int batch_iteration(isai_batch *b, double *x, hipStream_t st) {
     const size_t n = b->len;
-    double *r = b->work, *rh = r + n, *p = rh + n, *v = p + n, *ph = v + n, *s = ph + n, *sh = s + n, *t = sh + n,
-           *tmp = t + n;
+    double *w = b->work;  // r, rh, p, v, ph, s, sh, t, tmp: n doubles each
+    double *r = w, *rh = w + n, *p = w + 2 * n, *v = w + 3 * n, *s = w + 5 * n, *t = w + 7 * n;
+    void *ph = w + 4 * n, *sh = w + 6 * n, *tmp = w + 8 * n;
+    const bool f32 = b->precision != ISAI_PREC_FP64;
     const BatchDev B = b->dev();
-    const int np = b->npairs, pg = (np + 255) / 256, T = b->ntiles;
+    const int np = b->npairs, T = b->ntiles;
     zero_active_kernel<<<1, 1, 0, st>>>(b->d_ctl);
     brho_final<<<np, 64, 0, st>>>(B);
     bp_kernel<<<T, kRed, 0, st>>>(r, p, v, B);
-    BTRY(spmm(b, kOpML, p, tmp, 0, st));
-    BTRY(spmm(b, kOpMU, tmp, ph, 0, st));
-    BTRY(spmm(b, kOpA, ph, v, 0, st));
+    BTRY(spmm(b, kOpML, p, false, tmp, f32, 0, st));
+    BTRY(spmm(b, kOpMU, tmp, f32, ph, f32, 0, st));
+    BTRY(spmm(b, kOpA, ph, f32, v, false, 0, st));
     tile_dot_kernel<<<T, kRed, 0, st>>>(rh, v, nullptr, B);
     balpha_final<<<np, 64, 0, st>>>(B);
     bs_kernel<<<T, kRed, 0, st>>>(r, v, s, B);
     bsnorm_final<<<np, 64, 0, st>>>(B);
-    bxs_kernel<<<T, kRed, 0, st>>>(x, ph, B);
-    BTRY(spmm(b, kOpML, s, tmp, 0, st));
-    BTRY(spmm(b, kOpMU, tmp, sh, 0, st));
-    BTRY(spmm(b, kOpA, sh, t, 0, st));
+    if (f32) bxs_kernel<float><<<T, kRed, 0, st>>>(x, (const float *)ph, B);
+    else bxs_kernel<double><<<T, kRed, 0, st>>>(x, (const double *)ph, B);
+    BTRY(spmm(b, kOpML, s, false, tmp, f32, 0, st));
+    BTRY(spmm(b, kOpMU, tmp, f32, sh, f32, 0, st));
+    BTRY(spmm(b, kOpA, sh, f32, t, false, 0, st));
     tile_dot_kernel<<<T, kRed, 0, st>>>(t, s, t, B);
     bomega_final<<<np, 64, 0, st>>>(B);
-    bxr_kernel<<<T, kRed, 0, st>>>(x, r, rh, ph, sh, s, t, B);
+    if (f32)
+        bxr_kernel<float><<<T, kRed, 0, st>>>(x, r, rh, (const float *)ph, (const float *)sh, s, t, B);
+    else
+        bxr_kernel<double><<<T, kRed, 0, st>>>(x, r, rh, (const double *)ph, (const double *)sh, s, t, B);
     brnorm_final<<<np, 64, 0, st>>>(B);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
@@ -1285,10 +1351,12 @@ int autotune(isai_batch *b, hipStream_t st) {
             b->map[slot] = c / 6;
             b->wg[slot] = kWgCand[(c / 2) % 3];
             b->remap[slot] = c % 2;
-            status = spmm(b, op, X, Y, 1, st);  // warm-up
+            // the factor product as M_L runs it: FP64 x, FP32 output with FP32 / BF16 factors
+            const bool yf = slot == 1 && b->precision != ISAI_PREC_FP64;
+            status = spmm(b, op, X, false, Y, yf, 1, st);  // warm-up
             if (status != ISAI_OK) break;
             if (hipEventRecord(e0, st) != hipSuccess) status = ISAI_ERR_HIP;
-            for (int k = 0; k < 5 && status == ISAI_OK; ++k) status = spmm(b, op, X, Y, 1, st);
+            for (int k = 0; k < 5 && status == ISAI_OK; ++k) status = spmm(b, op, X, false, Y, yf, 1, st);
             float ms = 0.f;
             if (status == ISAI_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                                       hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
@@ -1375,15 +1443,16 @@ int persistent_solve(isai_batch *b, double tol, int maxit, const double *d_B, do
     a.b = d_B;
     a.x = d_X;
     const size_t n = b->len;
-    a.r = b->work;
-    a.rh = a.r + n;
-    a.p = a.rh + n;
-    a.v = a.p + n;
-    a.ph = a.v + n;
-    a.s = a.ph + n;
-    a.sh = a.s + n;
-    a.t = a.sh + n;
-    a.tmp = a.t + n;
+    double *w = b->work;  // r, rh, p, v, ph, s, sh, t, tmp: n doubles each
+    a.r = w;
+    a.rh = w + n;
+    a.p = w + 2 * n;
+    a.v = w + 3 * n;
+    a.ph = w + 4 * n;
+    a.s = w + 5 * n;
+    a.sh = w + 6 * n;
+    a.t = w + 7 * n;
+    a.tmp = w + 8 * n;
     a.tol = tol;
     a.relres = b->d_relres;
     const PersistFn fn = persist_fn(b->map[0], b->map[1], f.precision);
