@@ -1351,12 +1351,14 @@ int autotune(isai_batch *b, hipStream_t st) {
             b->map[slot] = c / 6;
             b->wg[slot] = kWgCand[(c / 2) % 3];
             b->remap[slot] = c % 2;
-            // the factor product as M_L runs it: FP64 x, FP32 output with FP32 / BF16 factors
-            const bool yf = slot == 1 && b->precision != ISAI_PREC_FP64;
-            status = spmm(b, op, X, false, Y, yf, 1, st);  // warm-up
+            // each product as the iteration runs it: with FP32 / BF16 factors M_L reads
+            // FP64 x and writes FP32, and A reads the FP32 p-hat / s-hat
+            const bool lp = b->precision != ISAI_PREC_FP64;
+            const bool xf = slot == 0 && lp, yf = slot == 1 && lp;
+            status = spmm(b, op, X, xf, Y, yf, 1, st);  // warm-up
             if (status != ISAI_OK) break;
             if (hipEventRecord(e0, st) != hipSuccess) status = ISAI_ERR_HIP;
-            for (int k = 0; k < 5 && status == ISAI_OK; ++k) status = spmm(b, op, X, false, Y, yf, 1, st);
+            for (int k = 0; k < 5 && status == ISAI_OK; ++k) status = spmm(b, op, X, xf, Y, yf, 1, st);
             float ms = 0.f;
             if (status == ISAI_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                                       hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
