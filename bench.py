@@ -549,6 +549,17 @@ def main() -> int:
                         "max_err_vs_ones": float(np.max(np.abs(x - 1.0)))}
         if solver == "fgmres_mixed":
             solves[form]["outer_steps"] = outer
+        if solver == "bicgstab":
+            # the same solve with its iterations replayed from a captured hipGraph
+            h.set_graph(True)
+            eager = (state["it"], state["rr"])
+            one_solve()  # captures
+            ms_g = timer.time(one_solve, args.steps)
+            h.set_graph(False)
+            solves[form]["graph"] = {"ms": round(ms_g, 3), "iters": state["it"],
+                                     "bit_identical": bool((state["it"], state["rr"]) == eager
+                                                           and np.array_equal(d_x.to_numpy(), x))}
+            log(f"solve {form} (graph): {ms_g:.2f} ms")
         log(f"solve {form}: {ms:.2f} ms ({state['it']} it, relres {state['rr']:.2e}, status {state['st']})")
 
     ok = {k: v for k, v in solves.items() if v["status"] == 0 and v["relres"] <= TOL}

@@ -54,6 +54,15 @@ typedef struct {
 
 typedef struct isai_handle *isai_handle_t;
 
+/* BiCGSTAB solves of this handle replay their iterations from a captured hipGraph
+ * (three iterations per replay, captured on the solve's stream on first use and
+ * again when the solution vector, the preconditioner or its precision changes)
+ * instead of launching them; the iterates are the same bit for bit.  The solve's
+ * stream must be a created (non-null) stream; with the null stream the iterations
+ * are launched as before.  No reference counterpart (the reference has no Krylov
+ * method). */
+int isai_set_solve_graph(isai_handle_t handle, int use_graph);
+
 /* Runs ILU(0) on a copy of A, the ISAI patterns / pair lists / Dinv / batched
  * ISAI build / BCSR permutation, the precision conversion and (for
  * ISAI_PRECOND_ILU_SPTRSV) the triangular-solve analysis.  Synchronous.

@@ -144,6 +144,7 @@ SIGNATURES: dict[str, list] = {
     "isai_batch_spmm": [P, P, P, P],
     "isai_batch_solve": [P, D, I, P, P, I, P, P, P, P],
     "isai_batch_set_spin_limit": [P, C.c_longlong],
+    "isai_set_solve_graph": [P, I],
     "isai_batch_destroy": [P],
     # bsr_tensor_SpMV.h
     "isai_bsr_spmv": [I, I, I, P, P, P, P, P, P],

@@ -603,6 +603,13 @@ struct isai_handle {
     int outer_pred = 0;            // outer steps the last converged solve took (0 = unknown)
     int last_outer = 0, last_inner = 0;
     int last_restart = 0;          // FGMRES cycle length of the last FGMRES solve
+    // BiCGSTAB iterations captured as a hipGraph (isai_set_solve_graph) and the
+    // configuration they were captured with
+    bool use_graph = false;
+    hipGraph_t it_graph = nullptr;
+    hipGraphExec_t it_exec = nullptr;
+    const void *it_key[4] = {nullptr, nullptr, nullptr, nullptr};
+    int it_keyi[2] = {-1, -1};
 };
 
 namespace {
@@ -771,6 +778,14 @@ constexpr int kItersPerRead = 3;
         if (_s != ISAI_OK) return _s;   \
     } while (0)
 
+void drop_iter_graph(isai_handle *h) {
+    if (h->it_exec) (void)hipGraphExecDestroy(h->it_exec);
+    if (h->it_graph) (void)hipGraphDestroy(h->it_graph);
+    h->it_exec = nullptr;
+    h->it_graph = nullptr;
+    h->it_keyi[0] = h->it_keyi[1] = -1;
+}
+
 int bicgstab(isai_handle *h, double tol, int maxit, const double *b, double *x, int *iters, double *relres,
              hipStream_t st) {
     TRY(ensure_work(h, 9));
@@ -783,6 +798,53 @@ int bicgstab(isai_handle *h, double tol, int maxit, const double *b, double *x, 
     // p^ and s^ held in FP32 when the ISAI apply produces FP32 numbers: the A
     // products and the x update read the same values with half the bytes
     const bool hat32 = apply_out_f32(h);
+    // one iteration: kernels only (each returns at once once the device record says
+    // done, so iterations past convergence or the limit change nothing)
+    auto iteration = [&]() -> int {
+        bicg_p_kernel<<<vec_grid(n), 256, 0, st>>>(n, r, p, v, h->S);
+        TRY(precond_apply(h, p, ph, tmp, st, skip, hat32));
+        TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, ph, hat32, v, rh, 1, part, st, skip, h->nnzb));
+        bicg_alpha_final_wg<<<1, kWgSumThreads, 0, st>>>(h->S, part, G);
+        bicg_s_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, r, v, s, h->S);
+        bicg_snorm_final<<<1, kRedThreads, 0, st>>>(h->S);
+        TRY(precond_apply(h, s, sh, tmp, st, skip, hat32));
+        TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, sh, hat32, t, s, 2, part, st, skip, h->nnzb));
+        bicg_omega_final_wg<<<1, kWgSumThreads, 0, st>>>(h->S, part, G);
+        if (hat32)
+            bicg_xr_kernel<float><<<kRedBlocks, kRedThreads, 0, st>>>(n, x, r, (const float *)ph, (const float *)sh, s,
+                                                                     t, rh, h->S);
+        else
+            bicg_xr_kernel<double><<<kRedBlocks, kRedThreads, 0, st>>>(n, x, r, ph, sh, s, t, rh, h->S);
+        bicg_rnorm_final<<<1, kRedThreads, 0, st>>>(h->S);
+        ISAI_LAUNCH_CHECK();
+        return ISAI_OK;
+    };
+    // graph mode: kItersPerRead iterations captured once per configuration and
+    // replayed; the device record stops at maxit, so a replay past it is a no-op
+    const bool graph = h->use_graph && st != nullptr;
+    if (graph) {
+        const void *key[4] = {x, h->work, h->d_Llp, h->sv};
+        const bool same = h->it_exec && std::equal(key, key + 4, h->it_key) && h->it_keyi[0] == h->precond &&
+                          h->it_keyi[1] == h->precision;
+        if (!same) {
+            drop_iter_graph(h);
+            ISAI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int cs = ISAI_OK;
+            for (int k = 0; k < kItersPerRead && cs == ISAI_OK; ++k) cs = iteration();
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(st, &g);
+            if (cs != ISAI_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return cs;
+            }
+            ISAI_HIP(e);
+            h->it_graph = g;
+            ISAI_HIP(hipGraphInstantiate(&h->it_exec, g, nullptr, nullptr, 0));
+            std::copy(key, key + 4, h->it_key);
+            h->it_keyi[0] = h->precond;
+            h->it_keyi[1] = h->precision;
+        }
+    }
     *iters = 0;
     bicg_start_kernel<<<1, 1, 0, st>>>(h->S, maxit);
     for (int restart = 0; restart < 8; ++restart) {
@@ -793,25 +855,11 @@ int bicgstab(isai_handle *h, double tol, int maxit, const double *b, double *x, 
         TRY(read_state(h, st));
         int done = h->h_done[0];
         for (int launched = h->h_done[2]; !done && launched < maxit;) {
-            for (int k = 0; k < kItersPerRead && launched < maxit; ++k, ++launched) {
-                bicg_p_kernel<<<vec_grid(n), 256, 0, st>>>(n, r, p, v, h->S);
-                TRY(precond_apply(h, p, ph, tmp, st, skip, hat32));
-                TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, ph, hat32, v, rh, 1, part, st, skip,
-                                    h->nnzb));
-                bicg_alpha_final_wg<<<1, kWgSumThreads, 0, st>>>(h->S, part, G);
-                bicg_s_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, r, v, s, h->S);
-                bicg_snorm_final<<<1, kRedThreads, 0, st>>>(h->S);
-                TRY(precond_apply(h, s, sh, tmp, st, skip, hat32));
-                TRY(spmv_dot_launch(h->nb, h->d_row, h->d_col, h->d_val, sh, hat32, t, s, 2, part, st, skip,
-                                    h->nnzb));
-                bicg_omega_final_wg<<<1, kWgSumThreads, 0, st>>>(h->S, part, G);
-                if (hat32)
-                    bicg_xr_kernel<float><<<kRedBlocks, kRedThreads, 0, st>>>(n, x, r, (const float *)ph,
-                                                                             (const float *)sh, s, t, rh, h->S);
-                else
-                    bicg_xr_kernel<double><<<kRedBlocks, kRedThreads, 0, st>>>(n, x, r, ph, sh, s, t, rh, h->S);
-                bicg_rnorm_final<<<1, kRedThreads, 0, st>>>(h->S);
-                ISAI_LAUNCH_CHECK();
+            if (graph) {
+                ISAI_HIP(hipGraphLaunch(h->it_exec, st));
+                launched += kItersPerRead;
+            } else {
+                for (int k = 0; k < kItersPerRead && launched < maxit; ++k, ++launched) TRY(iteration());
             }
             TRY(read_state(h, st));
             done = h->h_done[0];
@@ -1127,6 +1175,7 @@ extern "C" {
 
 int isai_destroy(isai_handle_t h) {
     if (!h) return ISAI_OK;
+    drop_iter_graph(h);
     (void)hipFree(h->d_LU);
     (void)hipFree(h->d_Lrow);
     (void)hipFree(h->d_Lcol);
@@ -1565,11 +1614,19 @@ int isai_handle_factors(isai_handle_t h, const int **Lrow, const int **Lcol, con
     return ISAI_OK;
 }
 
+int isai_set_solve_graph(isai_handle_t h, int use_graph) {
+    ISAI_REQUIRE(h, "isai_set_solve_graph: null handle");
+    h->use_graph = use_graph != 0;
+    if (!h->use_graph) drop_iter_graph(h);
+    return ISAI_OK;
+}
+
 int isai_set_preconditioner(isai_handle_t h, int precond, int precision, isai_stream_t stream) {
     ISAI_REQUIRE(h, "isai_set_preconditioner: null handle");
     ISAI_REQUIRE(precond >= 0 && precond <= 2 && precision >= 0 && precision <= 2,
                  "isai_set_preconditioner: bad kind %d / precision %d", precond, precision);
     hipStream_t st = (hipStream_t)stream;
+    drop_iter_graph(h);  // the captured iterations name the old factors
     if (precond == ISAI_PRECOND_ILU_SPTRSV && !h->sv) {
         int s = isai_bsr_sptrsv_analysis(h->nb, h->nnzb, h->bs, h->d_row, h->d_col, h->d_LU, h->chunk, &h->sv, stream);
         if (s != ISAI_OK) return s;

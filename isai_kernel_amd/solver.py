@@ -60,6 +60,10 @@ class Handle:
         call("isai_handle_build_plan", self.h, C.byref(n), C.byref(ol), C.byref(ou))
         return {"ncache": n.value, "ordered_L": ol.value, "ordered_U": ou.value}
 
+    def set_graph(self, use_graph: bool = True) -> None:
+        """BiCGSTAB iterations replayed from a captured hipGraph (isai_set_solve_graph)."""
+        call("isai_set_solve_graph", self.h, int(use_graph))
+
     def set_preconditioner(self, precond: str, precision: str = "fp64") -> None:
         call("isai_set_preconditioner", self.h, PRECOND[precond], PRECISION[precision], self.stream)
 

@@ -171,3 +171,51 @@ def test_fgmres_arnoldi_relation(gpu, solver, prec):
     assert np.max(np.abs(V @ V.T - np.eye(m + 1))) <= 1e-12
     assert np.all(np.tril(Hb, -2) == 0) and np.all(np.diag(Hb, -1) > 0)  # upper Hessenberg, h_{j+1,j} = ||w||
     h.close()
+
+
+@pytest.mark.parametrize("precond,precision", [("isai", "fp64"), ("isai", "bf16"), ("ilu", "fp64")])
+def test_bicgstab_graph_replay_bit_identical(gpu, precond, precision):
+    """isai_set_solve_graph: the iterations captured once as a hipGraph and replayed
+    give the launched iterations' x, count and residual bit for bit; the second
+    graph solve reuses the capture, a preconditioner switch recaptures."""
+    print(f"bicgstab graph replay precond={precond} precision={precision}", flush=True)
+    rp, ci, val = problems.generate(0, 0, seed=0)
+    A = Matrix(rp, ci, val)
+    b = O.spmv(rp, ci, val, np.random.default_rng(3).standard_normal(A.n))
+    s = ctypes.c_void_p()
+    call("isai_stream_create", ctypes.byref(s))
+    try:
+        h = Handle(A, precond=precond, precision=precision, stream=s)
+        d_b = DeviceArray.from_numpy(b)
+
+        def run():
+            d_x = DeviceArray.zeros((A.n,), np.float64)
+            st, it, rr = h.solve(d_b, d_x, "bicgstab", tol=1e-10, maxit=2000)
+            assert st == 0 and rr <= 1e-10
+            return d_x.to_numpy(), it, rr
+
+        x0, it0, rr0 = run()
+        h.set_graph(True)
+        for _ in range(2):
+            x1, it1, rr1 = run()
+            assert it1 == it0 and rr1 == rr0 and np.array_equal(x1, x0), (it0, it1, rr0, rr1)
+        # a different preconditioner between graph solves: recaptured, still equal
+        h.set_preconditioner("none")
+        h.set_graph(False)
+        xn, itn, rrn = run()
+        h.set_graph(True)
+        xg, itg, rrg = run()
+        assert itg == itn and rrg == rrn and np.array_equal(xg, xn)
+        # an iteration limit that is not a multiple of the replayed group stops exactly there
+        h.set_preconditioner(precond, precision)
+        d_x = DeviceArray.zeros((A.n,), np.float64)
+        st, it, _ = h.solve(d_b, d_x, "bicgstab", tol=1e-30, maxit=7, check=False)
+        h.set_graph(False)
+        d_y = DeviceArray.zeros((A.n,), np.float64)
+        st2, it2, _ = h.solve(d_b, d_y, "bicgstab", tol=1e-30, maxit=7, check=False)
+        assert (st, it) == (st2, it2) and it == 7
+        assert np.array_equal(d_x.to_numpy(), d_y.to_numpy())
+        h.close()
+    finally:
+        call("isai_stream_synchronize", s)
+        call("isai_stream_destroy", s)
