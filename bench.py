@@ -153,7 +153,8 @@ def bench_precision(timer, stream, steps: int) -> dict:
     """Precision form: 3-D 27-pt convection-diffusion 64^3 (bs = 4, 262,144 block rows),
     to 1e-10 with the ISAI factors stored in FP64 / FP32 / BF16: FP64 FGMRES(30) with
     the ISAI apply as its preconditioner, and the mixed form (FP64 FGMRES(30) outer
-    loop around the inner FP32-storage BiCGSTAB).  Errors are against the oracle's
+    loop around an inner FP32-storage BiCGSTAB, or around inner FP32-storage GMRES
+    cycles of up to 40 steps).  Errors are against the oracle's
     FP64 FGMRES(30) solution (sampled, precision_reference)."""
     from isai_kernel_amd import DeviceArray, call, problems
     from isai_kernel_amd.solver import Handle, Matrix
@@ -172,7 +173,8 @@ def bench_precision(timer, stream, steps: int) -> dict:
                        "threads": ref["threads"]} if ref else None)}
     xs = {}
     legs = [("fp64", "fp64", "fgmres"), ("fp32", "fp32", "fgmres"), ("bf16", "bf16", "fgmres"),
-            ("mixed_fp32", "fp32", "fgmres_mixed"), ("mixed_bf16", "bf16", "fgmres_mixed")]
+            ("mixed_fp32", "fp32", "fgmres_mixed"), ("mixed_bf16", "bf16", "fgmres_mixed"),
+            ("mixed_gmres_fp32", "fp32", "fgmres_mixed_gmres"), ("mixed_gmres_bf16", "bf16", "fgmres_mixed_gmres")]
     for key, prec, solver in legs:
         h.set_preconditioner("isai", prec)
         state = {}
@@ -186,7 +188,7 @@ def bench_precision(timer, stream, steps: int) -> dict:
         xs[key] = d_x.to_numpy()
         out[key] = {"ms": round(ms, 3), "iters": state["it"], "relres": state["rr"], "status": state["st"],
                     "solver": solver}
-        if solver == "fgmres_mixed":
+        if solver.startswith("fgmres_mixed"):
             out[key]["outer_steps"] = h.last_stats()[0]
         if ref:
             out[key].update(error_vs_reference(xs[key], ref))

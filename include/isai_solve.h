@@ -38,11 +38,18 @@ typedef enum {
  * handle's storage precision (FP32 when that is FP64), FP32 vectors, dot products
  * accumulated in FP64.  Each inner solve is asked for the reduction still missing
  * (see isai_set_inner_solve); *iters counts inner iterations and maxit bounds them.
- * Needs the ISAI preconditioner. */
+ * Needs the ISAI preconditioner.
+ * ISAI_SOLVER_FGMRES_MIXED_GMRES: the same FP64 FGMRES(restart) outer loop whose
+ * flexible preconditioner is one GMRES cycle of at most inner_maxit (<= 64) steps
+ * in FP32 storage: FP32 basis vectors, A rounded to FP32, the ISAI factors as
+ * above, Gram-Schmidt dots and updates accumulated in FP64, FP64 Hessenberg and
+ * Givens rotations; the cycle stops early on the same inner tolerance rule.  It
+ * needs no host read per inner step; *iters counts inner Arnoldi steps. */
 typedef enum {
     ISAI_SOLVER_BICGSTAB = 0,
     ISAI_SOLVER_FGMRES = 1,
-    ISAI_SOLVER_FGMRES_MIXED = 2
+    ISAI_SOLVER_FGMRES_MIXED = 2,
+    ISAI_SOLVER_FGMRES_MIXED_GMRES = 3
 } isai_solver_kind_t;
 
 typedef struct {
@@ -112,12 +119,13 @@ int isai_handle_build_plan(isai_handle_t handle, int *ncache, int *ordered_L, in
 int isai_solve(isai_handle_t handle, int solver, int restart, double tol, int maxit,
                const double *d_b, double *d_x, int *iters, double *relres, isai_stream_t stream);
 
-/* Inner solve of ISAI_SOLVER_FGMRES_MIXED: an inner solve stops when its residual
+/* Inner solve of the mixed-precision solvers: an inner solve stops when its residual
  * has fallen by max(inner_tol, 0.5 tol ||b|| / (current outer residual)) (at most
- * 0.5) or after inner_maxit iterations.  Defaults 3e-5 and 40. */
+ * 0.5) or after inner_maxit iterations (GMRES cycle: steps, at most 64).  Defaults
+ * 3e-5 and 40. */
 int isai_set_inner_solve(isai_handle_t handle, double inner_tol, int inner_maxit);
-/* Outer FGMRES steps and inner iterations of the last ISAI_SOLVER_FGMRES_MIXED
- * solve (0 and the iteration count for the other solvers). */
+/* Outer FGMRES steps and inner iterations of the last mixed-precision solve (0 and
+ * the iteration count for the other solvers). */
 int isai_last_solve_stats(isai_handle_t handle, int *outer_steps, int *inner_iters);
 
 /* Inspection of the last FGMRES cycle (no reference counterpart; for tests of the

@@ -115,7 +115,8 @@ struct InnerPrecond {  // ISAI factors in BCSR with FP32 / BF16 values
 int mixed_dot_groups(int nb);
 size_t inner_part_doubles(int nb);
 // y = A x with FP32 values, FP32 x and y; part[wg] = sum y.a (and part[G + wg] =
-// sum y.y for ndot = 2) over the rows of workgroup wg, in FP64.
+// sum y.y for ndot = 2) over the rows of workgroup wg, in FP64; ndot = 0: no sums
+// (a and part unused).
 int mixed_spmv_f32_dot(int nb, long long nnzb, const int *rp, const int *ci, const float *val, const float *x,
                        float *y, const float *a, int ndot, double *part, hipStream_t st, const int *skip);
 struct InnerLog {  // outcome of one inner solve, recorded on the device

@@ -88,10 +88,11 @@ __global__ __launch_bounds__(kMixedBlock) void bsr4_spmv_f32_dot_kernel(
         if (br < nb && b == 0) {
             const size_t row = (size_t)br * 4 + sub;
             y[row] = acc;
-            d0 += (double)acc * (double)a[row];
+            if (NDOT >= 1) d0 += (double)acc * (double)a[row];
             if (NDOT == 2) d1 += (double)acc * (double)acc;
         }
     }
+    if (NDOT == 0) return;
     __shared__ double red[2][kMixedBlock / 64];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -129,7 +130,11 @@ int mixed_spmv_f32_dot(int nb, long long nnzb, const int *rp, const int *ci, con
     const v4f *v = (const v4f *)val;
 #define ISAI_F32_DOT(ND, K) \
     bsr4_spmv_f32_dot_kernel<ND, K><<<g, kMixedBlock, 0, st>>>(nb, rp, ci, v, x, y, a, part, skip)
-    if (ndot == 1) {
+    if (ndot == 0) {
+        if (kb == 4) ISAI_F32_DOT(0, 4);
+        else if (kb == 8) ISAI_F32_DOT(0, 8);
+        else ISAI_F32_DOT(0, 16);
+    } else if (ndot == 1) {
         if (kb == 4) ISAI_F32_DOT(1, 4);
         else if (kb == 8) ISAI_F32_DOT(1, 8);
         else ISAI_F32_DOT(1, 16);

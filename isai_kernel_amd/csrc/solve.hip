@@ -419,9 +419,11 @@ __device__ __forceinline__ bool pass_skipped(const Scalars *S, int pass) {
     return S->done != 0 || (pass == 1 && !S->reorth);
 }
 
-__global__ __launch_bounds__(kMdotThreads) void fg_mdot_kernel(size_t n, const double *__restrict__ w,
-                                                              const double *__restrict__ V, int nv, Scalars *S,
-                                                              int pass) {
+// T = double (FP64 FGMRES) or float (the inner GMRES cycle of the mixed form: FP32
+// storage, FP64 accumulation).
+template <class T>
+__global__ __launch_bounds__(kMdotThreads) void fg_mdot_kernel(size_t n, const T *__restrict__ w,
+                                                              const T *__restrict__ V, int nv, Scalars *S, int pass) {
     if (pass_skipped(S, pass)) return;
     const int i0 = blockIdx.y * kDotChunk;
     const bool self = pass == 0 && blockIdx.y == 0;  // this block column also sums w.w
@@ -469,16 +471,19 @@ __global__ __launch_bounds__(kRedThreads) void fg_hsum_final(Scalars *S, int j, 
     }
 }
 
-// w -= sum_i hcur[i] V_i (i ascending); partial ||w||^2
-__global__ __launch_bounds__(kRedThreads) void fg_axpy_multi_kernel(size_t n, double *w, const double *__restrict__ V,
-                                                                    int nv, Scalars *S, int pass) {
+// w -= sum_i hcur[i] V_i (i ascending, FP64), stored as T; partial ||w||^2 of the
+// stored values
+template <class T>
+__global__ __launch_bounds__(kRedThreads) void fg_axpy_multi_kernel(size_t n, T *w, const T *__restrict__ V, int nv,
+                                                                    Scalars *S, int pass) {
     if (pass_skipped(S, pass)) return;
     double acc = 0.0;
     for (size_t k = (size_t)blockIdx.x * kRedThreads + threadIdx.x; k < n; k += (size_t)kRedBlocks * kRedThreads) {
         double wk = w[k];
         for (int i = 0; i < nv; ++i) wk -= S->hcur[i] * V[(size_t)i * n + k];
-        w[k] = wk;
-        acc += wk * wk;
+        const T wr = (T)wk;
+        w[k] = wr;
+        acc += (double)wr * (double)wr;
     }
     acc = block_sum(acc);
     if (threadIdx.x == 0) S->partial[0][blockIdx.x] = acc;
@@ -518,12 +523,54 @@ __global__ __launch_bounds__(kRedThreads) void fg_givens_final(Scalars *S, int j
     if (S->g[j + 1] * S->g[j + 1] <= S->tol2 || hn == 0.0) S->done = 1;
 }
 
-__global__ void fg_scale_next_kernel(size_t n, const double *__restrict__ w, double *__restrict__ v,
-                                     const Scalars *S) {
+template <class T>
+__global__ void fg_scale_next_kernel(size_t n, const T *__restrict__ w, T *__restrict__ v, const Scalars *S) {
     if (S->done) return;
     const double d = S->rnorm2;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        v[i] = w[i] / d;
+        v[i] = (T)((double)w[i] / d);
+}
+
+// ------------------------------------------- inner GMRES cycle (mixed form)
+// Start of one cycle on A z = v: q_0 = v rounded to FP32 with beta = 1 (v is a unit
+// vector of the outer basis), g = (1, 0, ...), and the cycle stops when |g_{i+1}|
+// <= tol_in, tol_in = min(0.5, max(floor_tol, 0.5 sqrt(*tol2) / |*gj|)) from the
+// outer residual estimate |g_j| and tol2 = tol^2 ||b||^2 (the inner BiCGSTAB's
+// rule).  The cycle starts done when the outer loop is done.
+__global__ void ig_start_kernel(size_t n, const double *__restrict__ v, float *__restrict__ q0, Scalars *Si,
+                                const int *__restrict__ outer_done, const double *__restrict__ gj,
+                                const double *__restrict__ tol2, double floor_tol) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double g = fabs(*gj);
+        double t = g > 0.0 ? 0.5 * sqrt(*tol2) / g : 0.5;
+        t = fmin(0.5, fmax(floor_tol, t));
+        Si->tol2 = t * t;
+        Si->beta = 1.0;
+        for (int k = 0; k <= kMaxRestart; ++k) Si->g[k] = 0.0;
+        Si->g[0] = 1.0;
+        Si->jend = 0;
+        Si->reorth = 0;
+        Si->done = *outer_done ? 1 : 0;
+    }
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        q0[i] = (float)v[i];
+}
+
+// u = sum_{k < jend} y_k q_k (FP64 accumulation, stored in FP32); the first thread
+// records the cycle's outcome (done, steps, tolerance) for the host's next read.
+__global__ void ig_combine_kernel(size_t n, const float *__restrict__ Q, const Scalars *Si, float *__restrict__ u,
+                                  InnerLog *log) {
+    const int jend = Si->jend;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        log->done = Si->done;
+        log->iters = jend;
+        log->tol_in = sqrt(Si->tol2);
+    }
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int k = 0; k < jend; ++k) acc += Si->y[k] * (double)Q[(size_t)k * n + i];
+        u[i] = (float)acc;
+    }
 }
 
 // Back substitution H(0:jend,0:jend) y = g.
@@ -596,6 +643,11 @@ struct isai_handle {
     int *h_inner = nullptr;        // pinned: inner done, iters (the inner solve's own read loop)
     InnerLog *d_log = nullptr;     // outcome of each inner solve since the last host read
     InnerLog *h_log = nullptr;     // pinned copy
+    // inner GMRES cycle (ISAI_SOLVER_FGMRES_MIXED_GMRES): basis, work vectors and
+    // scalar record; capacity in steps
+    float *ig_vecs = nullptr;
+    Scalars *Sin = nullptr;
+    int ig_cap = 0;
     double inner_tol = 3e-5;       // smallest relative reduction asked of one inner solve
     int inner_maxit = 40;          // inner iterations per outer step
     double inner_rate = 1.6;       // inner iterations per decade of reduction seen so far
@@ -1007,6 +1059,71 @@ int inner_solve(isai_handle *h, const InnerPrecond *M, const double *v, double *
     return ISAI_OK;
 }
 
+// Buffers of the inner GMRES cycle: k + 1 basis vectors and three work vectors in
+// FP32, and its own scalar record.
+int ensure_inner_gmres(isai_handle *h, int k) {
+    if (h->Sin && h->ig_cap >= k) return ISAI_OK;
+    (void)hipFree(h->ig_vecs);
+    h->ig_vecs = nullptr;
+    h->ig_cap = 0;
+    if (!h->Sin && hipMalloc(&h->Sin, sizeof(Scalars)) != hipSuccess) {
+        h->Sin = nullptr;
+        set_error("isai_solve: cannot allocate the inner GMRES record");
+        return ISAI_ERR_ALLOC;
+    }
+    if (hipMalloc(&h->ig_vecs, sizeof(float) * (size_t)(k + 4) * h->n) != hipSuccess) {
+        h->ig_vecs = nullptr;
+        set_error("isai_solve: cannot allocate the inner GMRES basis (%d vectors)", k + 4);
+        return ISAI_ERR_ALLOC;
+    }
+    h->ig_cap = k;
+    return ISAI_OK;
+}
+
+// z = P u with u from one inner GMRES cycle of at most k steps on A z = v, in FP32
+// storage: q_i in FP32, zh = M q_i (the handle's reduced-precision ISAI factors,
+// FP32 accumulation), w = A32 zh (FP32 A), classical Gram-Schmidt with the DGKS
+// second pass (FP64 dot products and updates, w stored in FP32), FP64 Hessenberg
+// and Givens rotations in the cycle's own scalar record.  The cycle needs no host
+// read: its kernels return at once after it converged, and its final steps (y, u =
+// Q y, z = M u with z in FP64) run unless the outer loop is done.  *log receives the
+// steps taken.
+int inner_gmres(isai_handle *h, const InnerPrecond *M, const double *v, double *z, const double *gj, int k,
+                InnerLog *log, hipStream_t st) {
+    const size_t n = h->n;
+    float *Q = h->ig_vecs, *w = Q + n * (size_t)(k + 1), *tmp = w + n, *zh = tmp + n;
+    Scalars *Si = h->Sin;
+    const int *skip = &Si->done;
+    ig_start_kernel<<<vec_grid(n), 256, 0, st>>>(n, v, Q, Si, &h->S->done, gj, &h->S->tol2, h->inner_tol);
+    ISAI_LAUNCH_CHECK();
+    for (int i = 0; i < k; ++i) {
+        const float *qi = Q + n * (size_t)i;
+        TRY(spmv_lowp_launch(h->nb, M->nnzbL, M->prec, M->Lrow, M->Lcol, M->Lval, qi, true, tmp, true, st, skip,
+                             true));
+        TRY(spmv_lowp_launch(h->nb, M->nnzbU, M->prec, M->Urow, M->Ucol, M->Uval, tmp, true, zh, true, st, skip,
+                             true));
+        TRY(mixed_spmv_f32_dot(h->nb, h->nnzb, h->d_row, h->d_col, h->d_A32, zh, w, nullptr, 0, nullptr, st, skip));
+        const int nv = i + 1;
+        const dim3 dgrid(kRedBlocks, (nv + kDotChunk - 1) / kDotChunk);
+        for (int pass = 0; pass < 2; ++pass) {
+            fg_mdot_kernel<<<dgrid, kMdotThreads, 0, st>>>(n, (const float *)w, (const float *)Q, nv, Si, pass);
+            fg_hsum_final<<<nv, kRedThreads, 0, st>>>(Si, i, pass);
+            fg_axpy_multi_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, (const float *)Q, nv, Si, pass);
+            if (pass == 0) fg_reorth_check<<<1, kRedThreads, 0, st>>>(Si);
+        }
+        fg_givens_final<<<1, kRedThreads, 0, st>>>(Si, i);
+        fg_scale_next_kernel<<<vec_grid(n), 256, 0, st>>>(n, (const float *)w, Q + n * (size_t)(i + 1), Si);
+        ISAI_LAUNCH_CHECK();
+    }
+    fg_solve_y_kernel<<<1, 64, 0, st>>>(Si);
+    ig_combine_kernel<<<vec_grid(n), 256, 0, st>>>(n, Q, Si, zh, log);
+    ISAI_LAUNCH_CHECK();
+    const int *oskip = &h->S->done;
+    TRY(spmv_lowp_launch(h->nb, M->nnzbL, M->prec, M->Lrow, M->Lcol, M->Lval, zh, true, tmp, true, st, oskip, true));
+    TRY(spmv_lowp_launch(h->nb, M->nnzbU, M->prec, M->Urow, M->Ucol, M->Uval, tmp, true, z, false, st, oskip, true));
+    return ISAI_OK;
+}
+
 // An inner outcome read back: a solve that converged sets the next prediction for
 // its outer step to its count, one that ran out of launched iterations without
 // converging to one more.
@@ -1028,8 +1145,10 @@ void inner_settle(isai_handle *h, int step, int launched, const InnerLog &lg) {
 // host launches the first P steps without reading the state in between (steps
 // launched after convergence return at once), then reads the convergence word and
 // the inner outcomes once.  *iters = inner BiCGSTAB iterations, bounded by maxit.
+// gmres_inner: the inner solve is one GMRES cycle of at most inner_maxit steps
+// (capped at kMaxRestart) instead, launched without host reads; *iters = its steps.
 int fgmres_mixed(isai_handle *h, int m, double tol, int maxit, const double *b, double *x, int *iters,
-                 double *relres, hipStream_t st) {
+                 double *relres, hipStream_t st, bool gmres_inner) {
     if (m <= 0) m = 10;
     if (m > kMaxRestart) {
         set_error("fgmres: restart %d above %d", m, kMaxRestart);
@@ -1044,6 +1163,8 @@ int fgmres_mixed(isai_handle *h, int m, double tol, int maxit, const double *b, 
     TRY(convert_lp(h, lp, st));
     TRY(ensure_work(h, (size_t)2 * m + 3));
     TRY(ensure_inner(h, st));
+    const int kcap = std::min(h->inner_maxit, kMaxRestart);
+    if (gmres_inner) TRY(ensure_inner_gmres(h, kcap));
     h->last_restart = m;
     const InnerPrecond M{(isai_precision_t)lp, h->nnzbL_lp, h->nnzbU_lp, h->d_Lrow_lp, h->d_Lcol_lp, h->d_Urow_lp,
                          h->d_Ucol_lp, h->d_Llp, h->d_Ulp};
@@ -1062,7 +1183,7 @@ int fgmres_mixed(isai_handle *h, int m, double tol, int maxit, const double *b, 
         for (int k = 0; k < pending; ++k) {
             inner_total += h->h_log[k].iters;
             real_steps += h->h_log[k].iters > 0;  // steps launched after convergence did nothing
-            inner_settle(h, steps - pending + k, launched[k], h->h_log[k]);
+            if (!gmres_inner) inner_settle(h, steps - pending + k, launched[k], h->h_log[k]);
         }
         pending = 0;
         launched_est = inner_total;
@@ -1088,8 +1209,13 @@ int fgmres_mixed(isai_handle *h, int m, double tol, int maxit, const double *b, 
             const int step = steps++;
             // g_j is the residual estimate before step j (beta for j = 0)
             const double *gj = j == 0 ? &h->S->beta : &h->S->g[j];
-            TRY(inner_solve(h, &M, vj, zj, gj, std::min(h->inner_maxit, maxit - launched_est), step,
-                            &h->d_log[pending], &launched[pending], st));
+            if (gmres_inner) {
+                launched[pending] = std::min(kcap, maxit - launched_est);
+                TRY(inner_gmres(h, &M, vj, zj, gj, launched[pending], &h->d_log[pending], st));
+            } else {
+                TRY(inner_solve(h, &M, vj, zj, gj, std::min(h->inner_maxit, maxit - launched_est), step,
+                                &h->d_log[pending], &launched[pending], st));
+            }
             launched_est += launched[pending];
             ++pending;
             TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, zj, w, st, skip));
@@ -1195,6 +1321,8 @@ int isai_destroy(isai_handle_t h) {
     (void)hipFree(h->own_col);
     (void)hipFree(h->own_val);
     (void)hipFree(h->d_A32);
+    (void)hipFree(h->ig_vecs);
+    (void)hipFree(h->Sin);
     (void)hipFree(h->inner_vecs);
     (void)hipFree(h->inner_part);
     (void)hipFree(h->IS);
@@ -1660,13 +1788,13 @@ int isai_solve(isai_handle_t h, int solver, int restart, double tol, int maxit, 
         s = bicgstab(h, tol, maxit, d_b, d_x, iters, relres, st);
     } else if (solver == ISAI_SOLVER_FGMRES) {
         s = fgmres(h, restart, tol, maxit, d_b, d_x, iters, relres, st);
-    } else if (solver == ISAI_SOLVER_FGMRES_MIXED) {
-        s = fgmres_mixed(h, restart, tol, maxit, d_b, d_x, iters, relres, st);
+    } else if (solver == ISAI_SOLVER_FGMRES_MIXED || solver == ISAI_SOLVER_FGMRES_MIXED_GMRES) {
+        s = fgmres_mixed(h, restart, tol, maxit, d_b, d_x, iters, relres, st, solver == ISAI_SOLVER_FGMRES_MIXED_GMRES);
     } else {
         set_error("isai_solve: unknown solver %d", solver);
         return ISAI_ERR_INVALID_ARG;
     }
-    if (solver != ISAI_SOLVER_FGMRES_MIXED) {
+    if (solver != ISAI_SOLVER_FGMRES_MIXED && solver != ISAI_SOLVER_FGMRES_MIXED_GMRES) {
         h->last_outer = 0;
         h->last_inner = *iters;
     }

@@ -9,7 +9,7 @@ from ._ffi import DeviceArray, SetupOpts, call, lib
 
 PRECOND = {"none": 0, "isai": 1, "ilu": 2}
 PRECISION = {"fp64": 0, "fp32": 1, "bf16": 2}
-SOLVER = {"bicgstab": 0, "fgmres": 1, "fgmres_mixed": 2}
+SOLVER = {"bicgstab": 0, "fgmres": 1, "fgmres_mixed": 2, "fgmres_mixed_gmres": 3}
 
 
 class Matrix:

@@ -32,6 +32,11 @@
  *                        inner solve (a single inner solve asked for 1e-12 stops
  *                        between 1e-9 and 1e-4 true residual: an FP64 inner would
  *                        pass 1e-9); at most 3 outer steps with a 1e-5 inner tolerance.
+ *   ora_fgmres_mixed_gmres  equality with a direct dense solve; one inner cycle of
+ *                        k steps with an FP64 ISAI reproduces FP64 GMRES(k)'s residual
+ *                        reduction (Saad Alg. 6.9 restated in numpy in the test) to the
+ *                        FP32 floor; with a full-length inner cycle on a small system
+ *                        a single outer step reaches the inner FP32 floor.
  *   ora_bsr_transpose    dense transpose equality (scipy-free, brute force).
  *   ora_transpose_packed_lu  dense identity L'U' = (L U)^T with L' unit lower and
  *                        U' upper; equality with ora_bsrilu0 of A^T on patterns
@@ -938,10 +943,90 @@ static int inner_bicgstab32(int nb, int bs, const int *rp, const int *ci, const 
     return it;
 }
 
-int ora_fgmres_mixed(int nb, int bs, const int *rp, const int *ci, const double *val,
-                     const ora_precond_t *P, const double *b, double *x, int m, double tol,
-                     int maxit, double inner_tol, int inner_maxit, int *iters, int *outer,
-                     double *relres) {
+/* z = one inner GMRES cycle on A z = rhs in FP32 storage, right-preconditioned by P.
+ * q_0 = fp32(rhs) is taken as the start vector with beta = 1 (the outer loop passes
+ * unit vectors); step i: zh = fp32(P q_i), w = fp32(A zh) (A rounded to FP32, FP32
+ * accumulation), classical Gram-Schmidt against q_0..q_i with a second pass when the
+ * first shrank ||w|| below ||w_before|| / sqrt(2) (Daniel, Gragg, Kaufman and
+ * Stewart 1976), dots accumulated in FP64, w updated in FP64 and stored in FP32,
+ * q_{i+1} = fp32(w / ||w||); H and the Givens rotations in FP64 (Saad, Iterative
+ * Methods for Sparse Linear Systems, Alg. 6.9).  Stops when |g_{i+1}| <= tol_in or
+ * after maxit steps; u = fp32(sum_k y_k q_k) (FP64 accumulation), z = P u.  Returns
+ * the steps taken. */
+static int inner_gmres32(int nb, int bs, const int *rp, const int *ci, const double *val,
+                         const ora_precond_t *P, const double *rhs, double *z, double tol_in,
+                         int maxit_in, float *f, double *d) {
+    size_t n = (size_t)nb * bs;
+    int k = maxit_in;
+    float *Q = f, *w = f + n * (size_t)(k + 1);
+    double *d0 = d, *d1 = d + n;
+    double *H = calloc((size_t)(k + 1) * k, sizeof(double));
+    double *cs = malloc(sizeof(double) * k), *sn = malloc(sizeof(double) * k);
+    double *g = calloc((size_t)k + 1, sizeof(double)), *y = malloc(sizeof(double) * k);
+    double *h = malloc(sizeof(double) * (k + 1));
+    for (size_t q = 0; q < n; ++q) Q[q] = (float)rhs[q];
+    g[0] = 1.0;
+    int jend = 0;
+    for (int j = 0; j < k; ++j) {
+        float *qj = Q + (size_t)j * n;
+        for (size_t q = 0; q < n; ++q) d0[q] = qj[q];
+        ora_precond_apply(P, d0, d1);
+        for (size_t q = 0; q < n; ++q) d0[q] = (float)d1[q];
+        ora_bsr_spmv_f32acc(nb, bs, rp, ci, val, 0, d0, d1);
+        for (size_t q = 0; q < n; ++q) w[q] = (float)d1[q];
+        double wb = dot32(n, w, w), w2 = 0.0;
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int i = 0; i <= j; ++i) {
+                h[i] = dot32(n, w, Q + (size_t)i * n);
+                H[i * k + j] = pass == 0 ? h[i] : H[i * k + j] + h[i];
+            }
+            w2 = 0.0;
+            for (size_t q = 0; q < n; ++q) {
+                double wk = w[q];
+                for (int i = 0; i <= j; ++i) wk -= h[i] * Q[(size_t)i * n + q];
+                w[q] = (float)wk;
+                w2 += (double)w[q] * (double)w[q];
+            }
+            if (pass == 0 && !(w2 < 0.5 * wb)) break;
+        }
+        double hn = sqrt(w2);
+        H[(j + 1) * k + j] = hn;
+        if (hn != 0.0)
+            for (size_t q = 0; q < n; ++q) Q[(size_t)(j + 1) * n + q] = (float)((double)w[q] / hn);
+        for (int i = 0; i < j; ++i) {
+            double a = H[i * k + j], c = H[(i + 1) * k + j];
+            H[i * k + j] = cs[i] * a + sn[i] * c;
+            H[(i + 1) * k + j] = -sn[i] * a + cs[i] * c;
+        }
+        double a = H[j * k + j], c = hn;
+        double den = sqrt(a * a + c * c);
+        cs[j] = den == 0.0 ? 1.0 : a / den;
+        sn[j] = den == 0.0 ? 0.0 : c / den;
+        H[j * k + j] = cs[j] * a + sn[j] * c;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        jend = j + 1;
+        if (g[j + 1] * g[j + 1] <= tol_in * tol_in || hn == 0.0) break;
+    }
+    for (int i = jend - 1; i >= 0; --i) {
+        double s = g[i];
+        for (int m = i + 1; m < jend; ++m) s -= H[i * k + m] * y[m];
+        y[i] = s / H[i * k + i];
+    }
+    for (size_t q = 0; q < n; ++q) {
+        double u = 0.0;
+        for (int m = 0; m < jend; ++m) u += y[m] * Q[(size_t)m * n + q];
+        d0[q] = (float)u;
+    }
+    ora_precond_apply(P, d0, z);
+    free(H); free(cs); free(sn); free(g); free(y); free(h);
+    return jend;
+}
+
+static int fgmres_mixed_impl(int nb, int bs, const int *rp, const int *ci, const double *val,
+                             const ora_precond_t *P, const double *b, double *x, int m, double tol,
+                             int maxit, double inner_tol, int inner_maxit, int inner_gmres, int *iters,
+                             int *outer, double *relres) {
     size_t n = (size_t)nb * bs;
     double *V = malloc(sizeof(double) * n * (size_t)(m + 1));
     double *Z = malloc(sizeof(double) * n * (size_t)m);
@@ -949,7 +1034,8 @@ int ora_fgmres_mixed(int nb, int bs, const int *rp, const int *ci, const double 
     double *H = calloc((size_t)(m + 1) * m, sizeof(double));
     double *cs = malloc(sizeof(double) * m), *sn = malloc(sizeof(double) * m);
     double *g = malloc(sizeof(double) * (m + 1)), *yv = malloc(sizeof(double) * m);
-    float *fw = malloc(sizeof(float) * 9 * n);
+    size_t nf = inner_gmres ? (size_t)(inner_maxit + 3) : 9;
+    float *fw = malloc(sizeof(float) * nf * n);
     double *dw = malloc(sizeof(double) * 2 * n);
     double bnorm = sqrt(dot(n, b, b));
     if (bnorm == 0.0) bnorm = 1.0;
@@ -969,8 +1055,12 @@ int ora_fgmres_mixed(int nb, int bs, const int *rp, const int *ci, const double 
             if (tol_in < inner_tol) tol_in = inner_tol;
             if (tol_in > 0.5) tol_in = 0.5;
             int budget = inner_maxit < maxit - it ? inner_maxit : maxit - it;
-            it += inner_bicgstab32(nb, bs, rp, ci, val, P, V + (size_t)j * n, Z + (size_t)j * n, tol_in,
-                                   budget, fw, dw);
+            if (inner_gmres)
+                it += inner_gmres32(nb, bs, rp, ci, val, P, V + (size_t)j * n, Z + (size_t)j * n, tol_in,
+                                    budget, fw, dw);
+            else
+                it += inner_bicgstab32(nb, bs, rp, ci, val, P, V + (size_t)j * n, Z + (size_t)j * n, tol_in,
+                                       budget, fw, dw);
             ++steps;
             ora_bsr_spmv(nb, bs, rp, ci, val, Z + (size_t)j * n, w);
             for (int i = 0; i <= j; ++i) {
@@ -1011,6 +1101,22 @@ int ora_fgmres_mixed(int nb, int bs, const int *rp, const int *ci, const double 
     *relres = ora_rel_residual_bsr(nb, bs, rp, ci, val, b, x);
     free(V); free(Z); free(w); free(H); free(cs); free(sn); free(g); free(yv); free(fw); free(dw);
     return conv ? ORA_OK : ORA_ERR_NOT_CONVERGED;
+}
+
+int ora_fgmres_mixed(int nb, int bs, const int *rp, const int *ci, const double *val,
+                     const ora_precond_t *P, const double *b, double *x, int m, double tol,
+                     int maxit, double inner_tol, int inner_maxit, int *iters, int *outer,
+                     double *relres) {
+    return fgmres_mixed_impl(nb, bs, rp, ci, val, P, b, x, m, tol, maxit, inner_tol, inner_maxit, 0, iters,
+                             outer, relres);
+}
+
+int ora_fgmres_mixed_gmres(int nb, int bs, const int *rp, const int *ci, const double *val,
+                           const ora_precond_t *P, const double *b, double *x, int m, double tol,
+                           int maxit, double inner_tol, int inner_maxit, int *iters, int *outer,
+                           double *relres) {
+    return fgmres_mixed_impl(nb, bs, rp, ci, val, P, b, x, m, tol, maxit, inner_tol, inner_maxit, 1, iters,
+                             outer, relres);
 }
 
 /* ------------------------------------------------------- differentiation form */

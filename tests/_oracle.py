@@ -45,6 +45,7 @@ _SIGS = {
     "ora_bicgstab": ([I, I, P, P, P, P, P, P, D, I, P, P], I),
     "ora_fgmres": ([I, I, P, P, P, P, P, P, I, D, I, P, P], I),
     "ora_fgmres_mixed": ([I, I, P, P, P, P, P, P, I, D, I, D, I, P, P, P], I),
+    "ora_fgmres_mixed_gmres": ([I, I, P, P, P, P, P, P, I, D, I, D, I, P, P, P], I),
     "ora_precond_apply": ([P, P, P], None),
     "ora_num_threads": ([], I),
     "ora_bsr_transpose": ([I, I, P, P, P, P, P, P], None),
@@ -278,14 +279,17 @@ def fgmres(rp, ci, val, P_, b, m=30, x0=None, tol=1e-10, maxit=1000):
     return x, st, it.value, rr.value
 
 
-def fgmres_mixed(rp, ci, val, P_, b, m=10, x0=None, tol=1e-10, maxit=400, inner_tol=1e-5, inner_maxit=40):
-    """FP64 FGMRES(m) with the inner FP32-storage BiCGSTAB as flexible preconditioner.
-    Returns (x, status, inner iterations, outer steps, true relres)."""
+def fgmres_mixed(rp, ci, val, P_, b, m=10, x0=None, tol=1e-10, maxit=400, inner_tol=1e-5, inner_maxit=40,
+                 inner="bicgstab"):
+    """FP64 FGMRES(m) with an inner FP32-storage solve as flexible preconditioner:
+    BiCGSTAB (inner="bicgstab") or one GMRES cycle of at most inner_maxit steps
+    (inner="gmres").  Returns (x, status, inner iterations, outer steps, true relres)."""
     nb, bs = len(rp) - 1, val.shape[1]
     x = np.zeros(nb * bs) if x0 is None else _f64(x0).copy()
     it, out, rr = C.c_int(), C.c_int(), C.c_double()
-    st = ora.ora_fgmres_mixed(nb, bs, _p(rp), _p(ci), _p(_f64(val)), C.byref(P_), _p(_f64(b)), _p(x), m, tol,
-                              maxit, inner_tol, inner_maxit, C.byref(it), C.byref(out), C.byref(rr))
+    fn = ora.ora_fgmres_mixed_gmres if inner == "gmres" else ora.ora_fgmres_mixed
+    st = fn(nb, bs, _p(rp), _p(ci), _p(_f64(val)), C.byref(P_), _p(_f64(b)), _p(x), m, tol,
+            maxit, inner_tol, inner_maxit, C.byref(it), C.byref(out), C.byref(rr))
     return x, st, it.value, out.value, rr.value
 
 

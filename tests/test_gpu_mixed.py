@@ -52,6 +52,40 @@ def test_mixed_solve_matches_oracle_scheme(gpu, prec, problem, L):
     h.close()
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+@pytest.mark.parametrize("problem,L", [(1, 24), (2, 12), (0, 64)])
+def test_mixed_gmres_solve_matches_oracle_scheme(gpu, prec, problem, L):
+    """ISAI_SOLVER_FGMRES_MIXED_GMRES (inner FP32-storage GMRES cycles of at most 30
+    steps) against the oracle's restatement (ora_fgmres_mixed_gmres): the same outer
+    steps, inner steps within one per outer step (a cycle that stops on its tolerance
+    may stop one step apart under a different FP32 rounding order), FP64 residual
+    <= 1e-10 checked independently, and the solution of a tight FP64 solve."""
+    print(f"mixed fgmres/gmres problem={problem} L={L} isai={prec}", flush=True)
+    rp, ci, val = problems.generate(problem, L, seed=2)
+    A = Matrix(rp, ci, val)
+    b = problems.normal(7, A.n)
+    h = Handle(A, precond="isai", precision=prec)
+    h.set_inner(1e-5, 30)
+    d_b, d_x = DeviceArray.from_numpy(b), DeviceArray.zeros((A.n,), np.float64)
+    st, it, rr = h.solve(d_b, d_x, "fgmres_mixed_gmres", tol=1e-10, maxit=600, restart=10)
+    outer, inner = h.last_stats()
+    x = d_x.to_numpy()
+    assert st == 0 and rr <= 1e-10 and inner == it
+    assert O.rel_residual(rp, ci, val, b, x) <= 1e-10
+    P = _oracle_precond(rp, ci, val, prec)
+    xo, sto, ito, outo, rro = O.fgmres_mixed(rp, ci, val, P, b, m=10, tol=1e-10, maxit=600, inner_tol=1e-5,
+                                             inner_maxit=30, inner="gmres")
+    assert sto == 0 and rro <= 1e-10
+    print(f"  outer {outer} / {outo}, inner steps {it} / {ito}", flush=True)
+    assert outer == outo, (outer, outo)
+    assert abs(it - ito) <= outer, (it, ito)
+    xd = O.bicgstab(rp, ci, val, _oracle_precond(rp, ci, val, "fp64"), b, tol=1e-13, maxit=500)[0]
+    scale = np.max(np.abs(xd))
+    assert np.max(np.abs(x - xd)) <= 1e-8 * scale
+    assert np.max(np.abs(xo - xd)) <= 1e-8 * scale
+    h.close()
+
+
 def test_mixed_inner_product_matches_oracle_rounding(gpu):
     """The inner solve's FP32 A product (A rounded to FP32, FP32 x, FP32 accumulation)
     against the oracle's rounded product, bounded by the FP32 accumulation error, and
@@ -103,7 +137,8 @@ def test_mixed_solve_full_size_96cube(gpu):
     h.close()
 
 
-@pytest.mark.parametrize("form", [("fp32", "fgmres"), ("bf16", "fgmres"), ("bf16", "fgmres_mixed")])
+@pytest.mark.parametrize("form", [("fp32", "fgmres"), ("bf16", "fgmres"), ("bf16", "fgmres_mixed"),
+                                  ("bf16", "fgmres_mixed_gmres")])
 def test_precision_form_64cube_against_oracle_reference(gpu, form):
     """The precision form at its own size (3-D 27-pt convection-diffusion 64^3,
     FGMRES(30) to 1e-10, b = isai_gen_normal(1)) against the oracle's FP64

@@ -133,7 +133,8 @@ def test_bicgstab_full_size_fp64_and_mixed(gpu):
     h.close()
 
 
-@pytest.mark.parametrize("solver,prec", [("fgmres", "fp64"), ("fgmres", "bf16"), ("fgmres_mixed", "bf16")])
+@pytest.mark.parametrize("solver,prec", [("fgmres", "fp64"), ("fgmres", "bf16"), ("fgmres_mixed", "bf16"),
+                                         ("fgmres_mixed_gmres", "bf16")])
 def test_fgmres_arnoldi_relation(gpu, solver, prec):
     """One full FGMRES cycle (m = 10, no early stop) satisfies the flexible Arnoldi
     relation A Z_m = V_{m+1} Hbar_m (Saad, Iterative Methods, 2nd ed., Prop. 9.2)
@@ -146,7 +147,7 @@ def test_fgmres_arnoldi_relation(gpu, solver, prec):
     b = problems.normal(3, A.n)
     d_b, d_x = DeviceArray.from_numpy(b), DeviceArray.zeros((A.n,), np.float64)
     m = 10
-    if solver == "fgmres_mixed":
+    if solver.startswith("fgmres_mixed"):
         # two inner iterations per step keep the outer estimate |g_m| above tol ||b||
         # through the cycle (with three it falls below 1e-30 ||b|| at the last step, the
         # cycle ends as converged and V_m is not formed)

@@ -472,6 +472,73 @@ def test_fgmres_mixed_inner_solve_has_the_fp32_floor():
     assert it32 == 12 and rr32 <= 1e-10, rr32
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("problem,L", [(0, 8), (1, 4), (2, 4)])
+def test_fgmres_mixed_gmres_converges_to_direct_solution(prec, problem, L):
+    """The inner-GMRES mixed scheme (FP32-storage GMRES cycles as the flexible
+    preconditioner of FP64 FGMRES) reaches the FP64 answer of a direct solve."""
+    rp, ci, val = problems.generate(problem, L, seed=3)
+    n = (len(rp) - 1) * 4
+    b = RNG.standard_normal(n)
+    P = _isai_precond(rp, ci, val, f32=prec == "fp32", bf16=prec == "bf16")
+    x, st, it, outer, rr = O.fgmres_mixed(rp, ci, val, P, b, m=10, tol=1e-10, maxit=400, inner_tol=1e-5,
+                                          inner_maxit=30, inner="gmres")
+    assert st == 0 and rr <= 1e-10
+    assert 2 <= outer <= 4, outer
+    xd = np.linalg.solve(O.bsr_to_dense(rp, ci, val), b)
+    assert np.max(np.abs(x - xd)) <= 1e-8 * np.max(np.abs(xd))
+
+
+def _np_gmres_cycle(Ad, apply, v, k):
+    """Right-preconditioned GMRES(k) from x = 0 on A z = v in FP64 numpy (Arnoldi
+    with modified Gram-Schmidt and a dense least-squares solve): z = M Q_k y."""
+    Q = np.zeros((k + 1, len(v)))
+    H = np.zeros((k + 1, k))
+    beta = np.linalg.norm(v)
+    Q[0] = v / beta
+    for j in range(k):
+        w = Ad @ apply(Q[j])
+        for i in range(j + 1):
+            H[i, j] = w @ Q[i]
+            w = w - H[i, j] * Q[i]
+        H[j + 1, j] = np.linalg.norm(w)
+        Q[j + 1] = w / H[j + 1, j]
+    e1 = np.zeros(k + 1)
+    e1[0] = beta
+    y = np.linalg.lstsq(H, e1, rcond=None)[0]
+    return apply(Q[:k].T @ y)
+
+
+@pytest.mark.parametrize("k", [3, 5])
+def test_fgmres_mixed_gmres_inner_cycle_matches_numpy_gmres(k):
+    """One outer step with one inner cycle of k steps: the outer residual is
+    min_a ||b - a A z|| for the inner solution z, so it equals that of FP64 GMRES(k)
+    written independently in numpy with the same preconditioner, up to the FP32
+    storage of the inner cycle (1e-3 of the residual; a dropped Givens term, a
+    wrong basis index or a missing Gram-Schmidt pass changes it by far more)."""
+    rp, ci, val = problems.generate(2, 8, seed=5)
+    n = (len(rp) - 1) * 4
+    b = RNG.standard_normal(n)
+    P = _isai_precond(rp, ci, val, f32=True)
+    _, st, it, outer, rr = O.fgmres_mixed(rp, ci, val, P, b, m=1, tol=1e-14, maxit=k, inner_tol=1e-14,
+                                          inner_maxit=k, inner="gmres")
+    assert outer == 1 and it == k
+    Ad = O.bsr_to_dense(rp, ci, val)
+    z = _np_gmres_cycle(Ad, lambda v: O.precond_apply(P, v), b / np.linalg.norm(b), k)
+    Az = Ad @ z
+    a = (Az @ b) / (Az @ Az)
+    rr_np = np.linalg.norm(b - a * Az) / np.linalg.norm(b)
+    # the cycle made progress and stays far above the FP32 floor (about 1e-7),
+    # so the comparison has content
+    assert 1e-4 < rr_np < 0.5
+    assert abs(rr - rr_np) <= 1e-3 * rr_np, (rr, rr_np)
+    # one step fewer is distinguishably worse, so k is pinned too
+    z1 = _np_gmres_cycle(Ad, lambda v: O.precond_apply(P, v), b / np.linalg.norm(b), k - 1)
+    Az1 = Ad @ z1
+    rr1 = np.linalg.norm(b - ((Az1 @ b) / (Az1 @ Az1)) * Az1) / np.linalg.norm(b)
+    assert rr1 > rr * 1.01
+
+
 def test_differentiation_config_matrix_as_specified_is_indefinite():
     """The differentiation form's matrix as its text specifies it (2-D 9-point
     stencil with the coefficients of the 5-point plain case: diagonal 4.5 I + S
