@@ -421,10 +421,20 @@ __device__ __forceinline__ bool pass_skipped(const Scalars *S, int pass) {
 
 // T = double (FP64 FGMRES) or float (the inner GMRES cycle of the mixed form: FP32
 // storage, FP64 accumulation).
+// Pass 1 decides itself whether it is needed: every block sums pass 0's ||w||^2
+// partials in the same order and compares with ||w_before||^2 / 2 (block (0, 0)
+// records the decision for the pass's other kernels), which saves a launch.
 template <class T>
 __global__ __launch_bounds__(kMdotThreads) void fg_mdot_kernel(size_t n, const T *__restrict__ w,
                                                               const T *__restrict__ V, int nv, Scalars *S, int pass) {
-    if (pass_skipped(S, pass)) return;
+    if (S->done) return;
+    if (pass == 1) {
+        double p = 0.0;
+        for (int k = threadIdx.x; k < kRedBlocks; k += kMdotThreads) p += S->partial[0][k];
+        const bool again = block_sum<kMdotThreads>(p) < 0.5 * S->wnorm2;
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) S->reorth = again ? 1 : 0;
+        if (!again) return;
+    }
     const int i0 = blockIdx.y * kDotChunk;
     const bool self = pass == 0 && blockIdx.y == 0;  // this block column also sums w.w
     double acc[kDotChunk], ww = 0.0;
@@ -432,15 +442,37 @@ __global__ __launch_bounds__(kMdotThreads) void fg_mdot_kernel(size_t n, const T
     for (int u = 0; u < kDotChunk; ++u) acc[u] = 0.0;
     // The loads of a column are unconditional (a block column past the last vector
     // re-reads the last one and discards it): loads under a per-vector branch were
-    // waited for one at a time.  Each vector's partial sum keeps its order.
-    for (size_t k = (size_t)blockIdx.x * kMdotThreads + threadIdx.x; k < n; k += (size_t)kRedBlocks * kMdotThreads) {
-        const double wk = w[k];
-        double vv[kDotChunk];
+    // waited for one at a time.  Each vector's partial sum keeps its order.  FP32
+    // vectors go 16 B per lane (four entries; n is a multiple of four).
+    if constexpr (sizeof(T) == 4) {
+        const size_t n4 = n / 4;
+        const v4f *w4 = (const v4f *)w, *V4 = (const v4f *)V;
+        for (size_t k = (size_t)blockIdx.x * kMdotThreads + threadIdx.x; k < n4;
+             k += (size_t)kRedBlocks * kMdotThreads) {
+            const v4f wk = w4[k];
+            v4f vv[kDotChunk];
 #pragma unroll
-        for (int u = 0; u < kDotChunk; ++u) vv[u] = V[(size_t)min(i0 + u, nv - 1) * n + k];
-        if (self) ww += wk * wk;
+            for (int u = 0; u < kDotChunk; ++u) vv[u] = V4[(size_t)min(i0 + u, nv - 1) * n4 + k];
+            if (self)
+                ww += (double)wk.x * wk.x + (double)wk.y * wk.y + (double)wk.z * wk.z + (double)wk.w * wk.w;
 #pragma unroll
-        for (int u = 0; u < kDotChunk; ++u) acc[u] = i0 + u < nv ? acc[u] + vv[u] * wk : acc[u];
+            for (int u = 0; u < kDotChunk; ++u) {
+                const double d = (double)vv[u].x * wk.x + (double)vv[u].y * wk.y + (double)vv[u].z * wk.z +
+                                 (double)vv[u].w * wk.w;
+                acc[u] = i0 + u < nv ? acc[u] + d : acc[u];
+            }
+        }
+    } else {
+        for (size_t k = (size_t)blockIdx.x * kMdotThreads + threadIdx.x; k < n;
+             k += (size_t)kRedBlocks * kMdotThreads) {
+            const double wk = w[k];
+            double vv[kDotChunk];
+#pragma unroll
+            for (int u = 0; u < kDotChunk; ++u) vv[u] = V[(size_t)min(i0 + u, nv - 1) * n + k];
+            if (self) ww += wk * wk;
+#pragma unroll
+            for (int u = 0; u < kDotChunk; ++u) acc[u] = i0 + u < nv ? acc[u] + vv[u] * wk : acc[u];
+        }
     }
 #pragma unroll
     for (int u = 0; u < kDotChunk; ++u) {
@@ -478,49 +510,88 @@ __global__ __launch_bounds__(kRedThreads) void fg_axpy_multi_kernel(size_t n, T 
                                                                     Scalars *S, int pass) {
     if (pass_skipped(S, pass)) return;
     double acc = 0.0;
-    for (size_t k = (size_t)blockIdx.x * kRedThreads + threadIdx.x; k < n; k += (size_t)kRedBlocks * kRedThreads) {
-        double wk = w[k];
-        for (int i = 0; i < nv; ++i) wk -= S->hcur[i] * V[(size_t)i * n + k];
-        const T wr = (T)wk;
-        w[k] = wr;
-        acc += (double)wr * (double)wr;
+    if constexpr (sizeof(T) == 4) {  // 16 B per lane
+        const size_t n4 = n / 4;
+        v4f *w4 = (v4f *)w;
+        const v4f *V4 = (const v4f *)V;
+        for (size_t k = (size_t)blockIdx.x * kRedThreads + threadIdx.x; k < n4;
+             k += (size_t)kRedBlocks * kRedThreads) {
+            const v4f w0 = w4[k];
+            double a = w0.x, b = w0.y, c = w0.z, d = w0.w;
+            for (int i = 0; i < nv; ++i) {
+                const double hi = S->hcur[i];
+                const v4f q = V4[(size_t)i * n4 + k];
+                a -= hi * q.x;
+                b -= hi * q.y;
+                c -= hi * q.z;
+                d -= hi * q.w;
+            }
+            const v4f r = v4f{(float)a, (float)b, (float)c, (float)d};
+            w4[k] = r;
+            acc += (double)r.x * r.x + (double)r.y * r.y + (double)r.z * r.z + (double)r.w * r.w;
+        }
+    } else {
+        for (size_t k = (size_t)blockIdx.x * kRedThreads + threadIdx.x; k < n;
+             k += (size_t)kRedBlocks * kRedThreads) {
+            double wk = w[k];
+            for (int i = 0; i < nv; ++i) wk -= S->hcur[i] * V[(size_t)i * n + k];
+            w[k] = wk;
+            acc += wk * wk;
+        }
     }
     acc = block_sum(acc);
     if (threadIdx.x == 0) S->partial[0][blockIdx.x] = acc;
 }
 
-// After pass 0: second pass needed when ||w||^2 < ||w_before||^2 / 2.
-__global__ __launch_bounds__(kRedThreads) void fg_reorth_check(Scalars *S) {
-    if (skipped(S)) return;
-    const double w2 = final_sum(S, 0);
-    if (threadIdx.x == 0) S->reorth = (w2 < 0.5 * S->wnorm2) ? 1 : 0;
-}
-
 // ||w|| -> H[j+1][j]; Givens rotations on column j; residual estimate |g[j+1]|.
+// The column, its copy into Hbar and the rotations' cosines and sines are staged in
+// LDS by the whole block; one thread then runs the chain of rotations with the
+// running entry in a register (the same operations in the same order as a direct
+// walk over H, without a global-memory round trip per rotation).
 __global__ __launch_bounds__(kRedThreads) void fg_givens_final(Scalars *S, int j) {
     if (skipped(S)) return;
     const double w2 = final_sum(S, 0);
-    if (threadIdx.x != 0) return;
     constexpr int m = kMaxRestart;
+    __shared__ double col[kMaxRestart + 1], c_[kMaxRestart], s_[kMaxRestart];
     const double hn = sqrt(w2);
-    S->H[(j + 1) * m + j] = hn;
-    for (int i = 0; i <= j + 1; ++i) S->Hbar[i * m + j] = S->H[i * m + j];
-    for (int i = 0; i < j; ++i) {
-        const double a = S->H[i * m + j], c = S->H[(i + 1) * m + j];
-        S->H[i * m + j] = S->cs[i] * a + S->sn[i] * c;
-        S->H[(i + 1) * m + j] = -S->sn[i] * a + S->cs[i] * c;
+    for (int i = threadIdx.x; i <= j; i += blockDim.x) {
+        const double v = S->H[i * m + j];
+        col[i] = v;
+        S->Hbar[i * m + j] = v;
+        if (i < j) {
+            c_[i] = S->cs[i];
+            s_[i] = S->sn[i];
+        }
     }
-    const double a = S->H[j * m + j], c = hn;
-    const double den = sqrt(a * a + c * c);
-    S->cs[j] = den == 0.0 ? 1.0 : a / den;
-    S->sn[j] = den == 0.0 ? 0.0 : c / den;
-    S->H[j * m + j] = S->cs[j] * a + S->sn[j] * c;
-    S->g[j + 1] = -S->sn[j] * S->g[j];
-    S->g[j] = S->cs[j] * S->g[j];
-    // keep the unrotated norm for the basis scaling
-    S->rnorm2 = hn;
-    S->jend = j + 1;
-    if (S->g[j + 1] * S->g[j + 1] <= S->tol2 || hn == 0.0) S->done = 1;
+    if (threadIdx.x == 0) {
+        S->H[(j + 1) * m + j] = hn;
+        S->Hbar[(j + 1) * m + j] = hn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double cur = col[0];
+        for (int i = 0; i < j; ++i) {
+            const double c = col[i + 1];
+            col[i] = c_[i] * cur + s_[i] * c;
+            cur = -s_[i] * cur + c_[i] * c;
+        }
+        const double a = cur;
+        const double den = sqrt(a * a + hn * hn);
+        const double cj = den == 0.0 ? 1.0 : a / den;
+        const double sj = den == 0.0 ? 0.0 : hn / den;
+        S->cs[j] = cj;
+        S->sn[j] = sj;
+        col[j] = cj * a + sj * hn;
+        const double gj = S->g[j];
+        S->g[j + 1] = -sj * gj;
+        S->g[j] = cj * gj;
+        // keep the unrotated norm for the basis scaling
+        S->rnorm2 = hn;
+        S->jend = j + 1;
+        if (S->g[j + 1] * S->g[j + 1] <= S->tol2 || hn == 0.0) S->done = 1;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= j; i += blockDim.x) S->H[i * m + j] = col[i];
 }
 
 template <class T>
@@ -960,7 +1031,6 @@ int fgmres(isai_handle *h, int m, double tol, int maxit, const double *b, double
                 fg_mdot_kernel<<<dgrid, kMdotThreads, 0, st>>>(n, w, V, nv, h->S, pass);
                 fg_hsum_final<<<nv, kRedThreads, 0, st>>>(h->S, j, pass);
                 fg_axpy_multi_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, V, nv, h->S, pass);
-                if (pass == 0) fg_reorth_check<<<1, kRedThreads, 0, st>>>(h->S);
             }
             fg_givens_final<<<1, kRedThreads, 0, st>>>(h->S, j);
             fg_scale_next_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V + n * (j + 1), h->S);
@@ -1109,7 +1179,6 @@ int inner_gmres(isai_handle *h, const InnerPrecond *M, const double *v, double *
             fg_mdot_kernel<<<dgrid, kMdotThreads, 0, st>>>(n, (const float *)w, (const float *)Q, nv, Si, pass);
             fg_hsum_final<<<nv, kRedThreads, 0, st>>>(Si, i, pass);
             fg_axpy_multi_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, (const float *)Q, nv, Si, pass);
-            if (pass == 0) fg_reorth_check<<<1, kRedThreads, 0, st>>>(Si);
         }
         fg_givens_final<<<1, kRedThreads, 0, st>>>(Si, i);
         fg_scale_next_kernel<<<vec_grid(n), 256, 0, st>>>(n, (const float *)w, Q + n * (size_t)(i + 1), Si);
@@ -1225,7 +1294,6 @@ int fgmres_mixed(isai_handle *h, int m, double tol, int maxit, const double *b, 
                 fg_mdot_kernel<<<dgrid, kMdotThreads, 0, st>>>(n, w, V, nv, h->S, pass);
                 fg_hsum_final<<<nv, kRedThreads, 0, st>>>(h->S, j, pass);
                 fg_axpy_multi_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, V, nv, h->S, pass);
-                if (pass == 0) fg_reorth_check<<<1, kRedThreads, 0, st>>>(h->S);
             }
             fg_givens_final<<<1, kRedThreads, 0, st>>>(h->S, j);
             fg_scale_next_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V + n * (j + 1), h->S);
