@@ -93,7 +93,7 @@ def test_spmv_empty_rows_and_zero_rows(gpu):
 
 @pytest.mark.parametrize("case", ["27pt", "7pt", "lower7pt", "ragged"])
 @pytest.mark.parametrize("prec,bf16", [(1, False), (2, True)])
-def test_spmv_reduced_precision_bitexact(gpu, prec, bf16, case):
+def test_spmv_reduced_precision_within_fp32_accumulation_bound(gpu, prec, bf16, case):
     """Long rows (27 blocks), short rows (7 and 4 blocks: several rows per lane group)
     and ragged rows with empty ones, each against the oracle's rounded product."""
     print(f"spmv prec={prec} case={case}", flush=True)
