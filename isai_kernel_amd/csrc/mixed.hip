@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kMixedBlock) void bsr4_spmv_f32_dot_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci, const v4f *__restrict__ val4,
     const float *__restrict__ x, float *__restrict__ y, const float *__restrict__ a, double *__restrict__ part,
     const int *__restrict__ skip) {
-    static_assert(KB == 4 || KB == 8 || KB == 16, "KB");
+    static_assert(KB == 4 || KB == 8 || KB == 16 || KB == 32, "KB");
     if (skip && *skip) return;
     constexpr int kJ = KB / 2;
     const int s = threadIdx.x & 7;
@@ -53,8 +53,11 @@ __global__ __launch_bounds__(kMixedBlock) void bsr4_spmv_f32_dot_kernel(
         const int len = br < nb ? rp[br + 1] - rs : 0;
         float acc = 0.0f;
         for (int t = 0; t < len; t += KB) {
-            const int clo = s < KB && t + s < len ? ci[rs + t + s] : 0;
-            const int chi = KB > 8 && t + 8 + s < len ? ci[rs + t + 8 + s] : 0;
+            // lane s holds the column of blocks t + 8q + s
+            constexpr int kC = KB < 8 ? 1 : KB / 8;
+            int cv[kC];
+#pragma unroll
+            for (int q = 0; q < kC; ++q) cv[q] = s < KB && t + 8 * q + s < len ? ci[rs + t + 8 * q + s] : 0;
             v4f w[kJ];
             float xv[kJ];
 #pragma unroll
@@ -64,10 +67,7 @@ __global__ __launch_bounds__(kMixedBlock) void bsr4_spmv_f32_dot_kernel(
             }
 #pragma unroll
             for (int j = 0; j < kJ; ++j) {
-                const int u = 2 * j + b;
-                const int clo_u = __shfl(clo, u & 7, 8);
-                const int chi_u = KB > 8 ? __shfl(chi, u & 7, 8) : 0;
-                const int c = u < 8 ? clo_u : chi_u;
+                const int c = __shfl(cv[j >> 2], (2 * j + b) & 7, 8);  // block 2j + b
                 xv[j] = x[(size_t)c * 4 + sub];
             }
 #pragma unroll
@@ -126,22 +126,25 @@ int mixed_spmv_f32_dot(int nb, long long nnzb, const int *rp, const int *ci, con
     if (nb <= 0) return ISAI_OK;
     const unsigned g = (unsigned)mixed_dot_groups(nb);
     const long long avg = (nnzb + nb - 1) / nb;
-    const int kb = avg <= 4 ? 4 : avg <= 8 ? 8 : 16;
+    const int kb = avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : 32;
     const v4f *v = (const v4f *)val;
 #define ISAI_F32_DOT(ND, K) \
     bsr4_spmv_f32_dot_kernel<ND, K><<<g, kMixedBlock, 0, st>>>(nb, rp, ci, v, x, y, a, part, skip)
     if (ndot == 0) {
         if (kb == 4) ISAI_F32_DOT(0, 4);
         else if (kb == 8) ISAI_F32_DOT(0, 8);
-        else ISAI_F32_DOT(0, 16);
+        else if (kb == 16) ISAI_F32_DOT(0, 16);
+        else ISAI_F32_DOT(0, 32);
     } else if (ndot == 1) {
         if (kb == 4) ISAI_F32_DOT(1, 4);
         else if (kb == 8) ISAI_F32_DOT(1, 8);
-        else ISAI_F32_DOT(1, 16);
+        else if (kb == 16) ISAI_F32_DOT(1, 16);
+        else ISAI_F32_DOT(1, 32);
     } else {
         if (kb == 4) ISAI_F32_DOT(2, 4);
         else if (kb == 8) ISAI_F32_DOT(2, 8);
-        else ISAI_F32_DOT(2, 16);
+        else if (kb == 16) ISAI_F32_DOT(2, 16);
+        else ISAI_F32_DOT(2, 32);
     }
 #undef ISAI_F32_DOT
     ISAI_LAUNCH_CHECK();
