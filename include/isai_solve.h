@@ -12,9 +12,10 @@
  *
  *   BiCGSTAB: van der Vorst (1992), right preconditioning (Saad, Iterative
  *             Methods for Sparse Linear Systems, 2nd ed., Alg. 7.7).
- *   FGMRES(m): flexible GMRES with modified Gram-Schmidt (Saad Alg. 9.6).
+ *   FGMRES(m): flexible GMRES (Saad Alg. 9.6) with classical Gram-Schmidt and a
+ *              second pass when the first one cancelled (DGKS criterion).
  * Scalars (alpha, beta, omega, Hessenberg entries, Givens rotations) live on the
- * device; the host only reads a convergence word once per iteration.
+ * device; the host reads a convergence word once per few iterations.
  */
 #ifndef ISAI_SOLVE_H
 #define ISAI_SOLVE_H
@@ -42,7 +43,8 @@ typedef enum {
  * ISAI_SOLVER_FGMRES_MIXED_GMRES: the same FP64 FGMRES(restart) outer loop whose
  * flexible preconditioner is one GMRES cycle of at most inner_maxit (<= 64) steps
  * in FP32 storage: FP32 basis vectors, A rounded to FP32, the ISAI factors as
- * above, Gram-Schmidt dots and updates accumulated in FP64, FP64 Hessenberg and
+ * above, one classical Gram-Schmidt pass with dots and updates accumulated in
+ * FP64, FP64 Hessenberg and
  * Givens rotations; the cycle stops early on the same inner tolerance rule.  It
  * needs no host read per inner step; *iters counts inner Arnoldi steps. */
 typedef enum {

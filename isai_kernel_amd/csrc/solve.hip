@@ -73,6 +73,28 @@ __device__ __forceinline__ double block_sum(double v) {
     return t;
 }
 
+// N block sums with one pair of barriers: on return thread u < N holds the block
+// total of v[u] (each in block_sum's order: wave butterfly, then the waves in order).
+template <int N, int T>
+__device__ __forceinline__ double block_sum_many(double (&v)[N]) {
+    __shared__ double ws[T / 64][N];
+#pragma unroll
+    for (int u = 0; u < N; ++u)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v[u] += __shfl_xor(v[u], off);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int u = 0; u < N; ++u) ws[w][u] = v[u];
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x < N)
+#pragma unroll
+        for (int k = 0; k < T / 64; ++k) t += ws[k][threadIdx.x];
+    return t;
+}
+
 // Sum of partial[slot][0..kRedBlocks) in a fixed order (one block).
 __device__ __forceinline__ double final_sum(const Scalars *S, int slot) {
     double v = 0.0;
@@ -447,6 +469,7 @@ __global__ __launch_bounds__(kMdotThreads) void fg_mdot_kernel(size_t n, const T
     if constexpr (sizeof(T) == 4) {
         const size_t n4 = n / 4;
         const v4f *w4 = (const v4f *)w, *V4 = (const v4f *)V;
+#pragma unroll 2
         for (size_t k = (size_t)blockIdx.x * kMdotThreads + threadIdx.x; k < n4;
              k += (size_t)kRedBlocks * kMdotThreads) {
             const v4f wk = w4[k];
@@ -474,15 +497,14 @@ __global__ __launch_bounds__(kMdotThreads) void fg_mdot_kernel(size_t n, const T
             for (int u = 0; u < kDotChunk; ++u) acc[u] = i0 + u < nv ? acc[u] + vv[u] * wk : acc[u];
         }
     }
+    double sums[kDotChunk + 1];
 #pragma unroll
-    for (int u = 0; u < kDotChunk; ++u) {
-        const double t = block_sum<kMdotThreads>(acc[u]);
-        if (threadIdx.x == 0 && i0 + u < nv) S->mpart[(size_t)(i0 + u) * kRedBlocks + blockIdx.x] = t;
-    }
-    if (self) {
-        ww = block_sum<kMdotThreads>(ww);
-        if (threadIdx.x == 0) S->partial[1][blockIdx.x] = ww;
-    }
+    for (int u = 0; u < kDotChunk; ++u) sums[u] = acc[u];
+    sums[kDotChunk] = ww;
+    const double t = block_sum_many<kDotChunk + 1, kMdotThreads>(sums);
+    const int u = threadIdx.x;
+    if (u < kDotChunk && i0 + u < nv) S->mpart[(size_t)(i0 + u) * kRedBlocks + blockIdx.x] = t;
+    if (self && u == kDotChunk) S->partial[1][blockIdx.x] = t;
 }
 
 // block i: h_i = sum of its partials; pass 0 sets H[i][j] (and block 0 stores
@@ -509,6 +531,12 @@ template <class T>
 __global__ __launch_bounds__(kRedThreads) void fg_axpy_multi_kernel(size_t n, T *w, const T *__restrict__ V, int nv,
                                                                     Scalars *S, int pass) {
     if (pass_skipped(S, pass)) return;
+    // the coefficients in LDS; each thread issues the loads of eight basis vectors
+    // before it uses them (a clamped index re-reads the last vector and is not
+    // used), so a step waits once per eight vectors instead of once per vector
+    __shared__ double hs[kMaxRestart + 1];
+    for (int i = threadIdx.x; i < nv; i += kRedThreads) hs[i] = S->hcur[i];
+    __syncthreads();
     double acc = 0.0;
     if constexpr (sizeof(T) == 4) {  // 16 B per lane
         const size_t n4 = n / 4;
@@ -518,13 +546,20 @@ __global__ __launch_bounds__(kRedThreads) void fg_axpy_multi_kernel(size_t n, T 
              k += (size_t)kRedBlocks * kRedThreads) {
             const v4f w0 = w4[k];
             double a = w0.x, b = w0.y, c = w0.z, d = w0.w;
-            for (int i = 0; i < nv; ++i) {
-                const double hi = S->hcur[i];
-                const v4f q = V4[(size_t)i * n4 + k];
-                a -= hi * q.x;
-                b -= hi * q.y;
-                c -= hi * q.z;
-                d -= hi * q.w;
+            for (int i0 = 0; i0 < nv; i0 += kDotChunk) {
+                v4f q[kDotChunk];
+#pragma unroll
+                for (int u = 0; u < kDotChunk; ++u) q[u] = V4[(size_t)min(i0 + u, nv - 1) * n4 + k];
+#pragma unroll
+                for (int u = 0; u < kDotChunk; ++u) {
+                    if (i0 + u < nv) {
+                        const double hi = hs[i0 + u];
+                        a -= hi * q[u].x;
+                        b -= hi * q[u].y;
+                        c -= hi * q[u].z;
+                        d -= hi * q[u].w;
+                    }
+                }
             }
             const v4f r = v4f{(float)a, (float)b, (float)c, (float)d};
             w4[k] = r;
@@ -534,7 +569,14 @@ __global__ __launch_bounds__(kRedThreads) void fg_axpy_multi_kernel(size_t n, T 
         for (size_t k = (size_t)blockIdx.x * kRedThreads + threadIdx.x; k < n;
              k += (size_t)kRedBlocks * kRedThreads) {
             double wk = w[k];
-            for (int i = 0; i < nv; ++i) wk -= S->hcur[i] * V[(size_t)i * n + k];
+            for (int i0 = 0; i0 < nv; i0 += kDotChunk) {
+                double q[kDotChunk];
+#pragma unroll
+                for (int u = 0; u < kDotChunk; ++u) q[u] = V[(size_t)min(i0 + u, nv - 1) * n + k];
+#pragma unroll
+                for (int u = 0; u < kDotChunk; ++u)
+                    if (i0 + u < nv) wk -= hs[i0 + u] * q[u];
+            }
             w[k] = wk;
             acc += wk * wk;
         }
@@ -639,7 +681,14 @@ __global__ void ig_combine_kernel(size_t n, const float *__restrict__ Q, const S
     }
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         double acc = 0.0;
-        for (int k = 0; k < jend; ++k) acc += Si->y[k] * (double)Q[(size_t)k * n + i];
+        for (int k0 = 0; k0 < jend; k0 += 8) {  // eight loads in flight, then their terms in order
+            float q[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) q[t] = Q[(size_t)min(k0 + t, jend - 1) * n + i];
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (k0 + t < jend) acc += Si->y[k0 + t] * (double)q[t];
+        }
         u[i] = (float)acc;
     }
 }
@@ -661,7 +710,14 @@ __global__ void fg_update_x_kernel(size_t n, double *x, const double *__restrict
     const int jend = S->jend;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         double acc = x[i];
-        for (int k = 0; k < jend; ++k) acc += S->y[k] * Z[(size_t)k * n + i];
+        for (int k0 = 0; k0 < jend; k0 += 8) {  // eight loads in flight, then their terms in order
+            double z[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) z[t] = Z[(size_t)min(k0 + t, jend - 1) * n + i];
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (k0 + t < jend) acc += S->y[k0 + t] * z[t];
+        }
         x[i] = acc;
     }
 }
@@ -1152,9 +1208,9 @@ int ensure_inner_gmres(isai_handle *h, int k) {
 
 // z = P u with u from one inner GMRES cycle of at most k steps on A z = v, in FP32
 // storage: q_i in FP32, zh = M q_i (the handle's reduced-precision ISAI factors,
-// FP32 accumulation), w = A32 zh (FP32 A), classical Gram-Schmidt with the DGKS
-// second pass (FP64 dot products and updates, w stored in FP32), FP64 Hessenberg
-// and Givens rotations in the cycle's own scalar record.  The cycle needs no host
+// FP32 accumulation), w = A32 zh (FP32 A), one classical Gram-Schmidt pass (FP64
+// dot products and updates, w stored in FP32), FP64 Hessenberg and Givens rotations
+// in the cycle's own scalar record.  The cycle needs no host
 // read: its kernels return at once after it converged, and its final steps (y, u =
 // Q y, z = M u with z in FP64) run unless the outer loop is done.  *log receives the
 // steps taken.
@@ -1173,13 +1229,13 @@ int inner_gmres(isai_handle *h, const InnerPrecond *M, const double *v, double *
         TRY(spmv_lowp_launch(h->nb, M->nnzbU, M->prec, M->Urow, M->Ucol, M->Uval, tmp, true, zh, true, st, skip,
                              true));
         TRY(mixed_spmv_f32_dot(h->nb, h->nnzb, h->d_row, h->d_col, h->d_A32, zh, w, nullptr, 0, nullptr, st, skip));
+        // one classical Gram-Schmidt pass: the cycle only has to reach the inner
+        // tolerance, and a second pass changed neither its steps nor the outer steps
         const int nv = i + 1;
         const dim3 dgrid(kRedBlocks, (nv + kDotChunk - 1) / kDotChunk);
-        for (int pass = 0; pass < 2; ++pass) {
-            fg_mdot_kernel<<<dgrid, kMdotThreads, 0, st>>>(n, (const float *)w, (const float *)Q, nv, Si, pass);
-            fg_hsum_final<<<nv, kRedThreads, 0, st>>>(Si, i, pass);
-            fg_axpy_multi_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, (const float *)Q, nv, Si, pass);
-        }
+        fg_mdot_kernel<<<dgrid, kMdotThreads, 0, st>>>(n, (const float *)w, (const float *)Q, nv, Si, 0);
+        fg_hsum_final<<<nv, kRedThreads, 0, st>>>(Si, i, 0);
+        fg_axpy_multi_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, w, (const float *)Q, nv, Si, 0);
         fg_givens_final<<<1, kRedThreads, 0, st>>>(Si, i);
         fg_scale_next_kernel<<<vec_grid(n), 256, 0, st>>>(n, (const float *)w, Q + n * (size_t)(i + 1), Si);
         ISAI_LAUNCH_CHECK();

@@ -946,9 +946,9 @@ static int inner_bicgstab32(int nb, int bs, const int *rp, const int *ci, const 
 /* z = one inner GMRES cycle on A z = rhs in FP32 storage, right-preconditioned by P.
  * q_0 = fp32(rhs) is taken as the start vector with beta = 1 (the outer loop passes
  * unit vectors); step i: zh = fp32(P q_i), w = fp32(A zh) (A rounded to FP32, FP32
- * accumulation), classical Gram-Schmidt against q_0..q_i with a second pass when the
- * first shrank ||w|| below ||w_before|| / sqrt(2) (Daniel, Gragg, Kaufman and
- * Stewart 1976), dots accumulated in FP64, w updated in FP64 and stored in FP32,
+ * accumulation), one classical Gram-Schmidt pass against q_0..q_i (the cycle only
+ * has to reach the inner tolerance), dots accumulated in FP64, w updated in FP64
+ * and stored in FP32,
  * q_{i+1} = fp32(w / ||w||); H and the Givens rotations in FP64 (Saad, Iterative
  * Methods for Sparse Linear Systems, Alg. 6.9).  Stops when |g_{i+1}| <= tol_in or
  * after maxit steps; u = fp32(sum_k y_k q_k) (FP64 accumulation), z = P u.  Returns
@@ -974,20 +974,16 @@ static int inner_gmres32(int nb, int bs, const int *rp, const int *ci, const dou
         for (size_t q = 0; q < n; ++q) d0[q] = (float)d1[q];
         ora_bsr_spmv_f32acc(nb, bs, rp, ci, val, 0, d0, d1);
         for (size_t q = 0; q < n; ++q) w[q] = (float)d1[q];
-        double wb = dot32(n, w, w), w2 = 0.0;
-        for (int pass = 0; pass < 2; ++pass) {
-            for (int i = 0; i <= j; ++i) {
-                h[i] = dot32(n, w, Q + (size_t)i * n);
-                H[i * k + j] = pass == 0 ? h[i] : H[i * k + j] + h[i];
-            }
-            w2 = 0.0;
-            for (size_t q = 0; q < n; ++q) {
-                double wk = w[q];
-                for (int i = 0; i <= j; ++i) wk -= h[i] * Q[(size_t)i * n + q];
-                w[q] = (float)wk;
-                w2 += (double)w[q] * (double)w[q];
-            }
-            if (pass == 0 && !(w2 < 0.5 * wb)) break;
+        for (int i = 0; i <= j; ++i) {
+            h[i] = dot32(n, w, Q + (size_t)i * n);
+            H[i * k + j] = h[i];
+        }
+        double w2 = 0.0;
+        for (size_t q = 0; q < n; ++q) {
+            double wk = w[q];
+            for (int i = 0; i <= j; ++i) wk -= h[i] * Q[(size_t)i * n + q];
+            w[q] = (float)wk;
+            w2 += (double)w[q] * (double)w[q];
         }
         double hn = sqrt(w2);
         H[(j + 1) * k + j] = hn;

@@ -201,7 +201,7 @@ int ora_fgmres_mixed(int nb, int bs, const int *rp, const int *ci, const double 
 /* The same outer loop whose flexible preconditioner is one inner GMRES cycle of at
  * most inner_maxit steps in FP32 storage (right-preconditioned by P): q_0 =
  * fp32(v_j) with beta = 1, zh = fp32(P q_i), w = fp32(A zh) with A rounded to FP32
- * and FP32 accumulation, classical Gram-Schmidt with the DGKS second pass (dots in
+ * and FP32 accumulation, one classical Gram-Schmidt pass (dots in
  * FP64, w stored in FP32), FP64 Hessenberg and Givens rotations, stop when |g_{i+1}|
  * <= tol_in (same tol_in rule), z_j = P fp32(Q y).  *iters = inner Arnoldi steps. */
 int ora_fgmres_mixed_gmres(int nb, int bs, const int *rp, const int *ci, const double *val,
