@@ -466,21 +466,32 @@ def main() -> int:
     spmv_ms = timer.time(spmv, K)
     record("bsr_spmv_fp64", spmv_ms, spmv_bytes(nb, nnzb))
 
-    # ---- the mixed solve's A product: FP32 values, FP32 x and y, two fused dots
+    # ---- the mixed solve's A product: FP32 values, FP32 x and y, two fused dots, on
+    # the handle's line-aligned FP32 copy of A (what the solve runs), and the public
+    # product on the unpadded FP32 values for comparison
     d_a32 = DeviceArray((nnzb * 16,), np.float32)
     call("isai_convert_values", nnzb * 16, 1, A.d_val.ptr, d_a32.ptr, stream)
     d_xf = DeviceArray.from_numpy(np.ones(n, np.float32))
     d_yf = DeviceArray((n,), np.float32)
     d_part = DeviceArray((2 * 4096,), np.float64)
-    groups = ctypes.c_int()
+    groups, rows_per_group = ctypes.c_int(), ctypes.c_int()
 
     def spmv32():
+        lib.isai_handle_spmv_f32_dot(h.h, d_xf.ptr, d_yf.ptr, d_xf.ptr, 2, d_part.ptr, ctypes.byref(groups),
+                                     ctypes.byref(rows_per_group), stream)
+
+    def spmv32_unaligned():
         lib.isai_bsr_spmv_f32_dot(nb, nnzb, A.d_rp.ptr, A.d_ci.ptr, d_a32.ptr, d_xf.ptr, d_yf.ptr, d_xf.ptr, 2,
                                   d_part.ptr, ctypes.byref(groups), stream)
     for _ in range(10):
         spmv32()
     spmv32_ms = timer.time(spmv32, K)
-    record("bsr_spmv_fp32_dot", spmv32_ms, f32_spmv_dot_bytes(nb, nnzb))
+    record("bsr_spmv_fp32_dot", spmv32_ms, f32_spmv_dot_bytes(nb, nnzb), rows_per_group=rows_per_group.value,
+           layout="line-aligned FP32 copy of A (handle), non-temporal value loads")
+    for _ in range(10):
+        spmv32_unaligned()
+    record("bsr_spmv_fp32_dot_unaligned", timer.time(spmv32_unaligned, K), f32_spmv_dot_bytes(nb, nnzb),
+           layout="caller's FP32 values (isai_bsr_spmv_f32_dot), through L2")
     del d_a32, d_xf, d_yf, d_part
 
     # ---- ISAI apply (M_U (M_L v)) in each storage precision
@@ -595,7 +606,8 @@ def main() -> int:
                      "frac": round(gbps / HBM_PEAK_GBPS, 4),
                      "traffic": (int(F32_SPMV_TRAFFIC_BYTES_96CUBE)
                                  if args.problem == 1 and F32_SPMV_TRAFFIC_BYTES_96CUBE else None),
-                     "kernel": "bsr4_spmv_f32_dot_kernel<2, 8>",
+                     "kernel": (f"bsr4_spmv_f32_dot_kernel<2, 8, {kernels['bsr_spmv_fp32_dot']['rows_per_group']}, "
+                                "true>"),
                      "algorithmic_bytes": f32_spmv_dot_bytes(nb, nnzb)},
         "speedup": speedups,
         "solves": solves,

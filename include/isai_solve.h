@@ -130,6 +130,19 @@ int isai_set_inner_solve(isai_handle_t handle, double inner_tol, int inner_maxit
  * the iteration count for the other solvers). */
 int isai_last_solve_stats(isai_handle_t handle, int *outer_steps, int *inner_iters);
 
+/* The FP32 product with A that the handle's mixed-precision solves run (bs = 4; no
+ * reference counterpart): d_y = A d_x with FP32 x and y, FP32 accumulation, and the
+ * fused FP64 dots of isai_bsr_spmv_f32_dot (*groups = G, d_part holds 2 G doubles;
+ * ndot 0 = no dots, d_a and d_part unused).  It multiplies with the handle's FP32
+ * copy of A (built on first use), whose rows are laid out on 128-byte lines and
+ * streamed with non-temporal loads: one row per lane group, every row padded with
+ * zero blocks to an even count, or row pairs stored from a line with the pair's
+ * second row padded (*rows_per_group = 1 or 2, may be NULL).  y equals
+ * isai_bsr_spmv_f32_dot's on the unpadded FP32 values; with row pairs a
+ * workgroup's partials cover other rows, so the dots differ in rounding. */
+int isai_handle_spmv_f32_dot(isai_handle_t handle, const float *d_x, float *d_y, const float *d_a, int ndot,
+                             double *d_part, int *groups, int *rows_per_group, isai_stream_t stream);
+
 /* Inspection of the last FGMRES cycle (no reference counterpart; for tests of the
  * Arnoldi relation A Z_k = V_{k+1} Hbar_k): *m = the cycle length of the last
  * FGMRES solve, *steps = the Arnoldi steps k of its last cycle, *d_V = the m + 1

@@ -152,6 +152,7 @@ SIGNATURES: dict[str, list] = {
     "isai_convert_values": [SZ, I, P, P, P],
     "isai_apply": [I, I, I, P, P, P, I, P, P, P, I, P, P, P, P],
     "isai_bsr_spmv_f32_dot": [I, I, P, P, P, P, P, P, I, P, P, P],
+    "isai_handle_spmv_f32_dot": [P, P, P, P, I, P, P, P, P],
     "isai_bsr_spmv_bench": [P, P, P, I, I, I, P, P, I, I, P],
 }
 _RESTYPE = {"isai_last_error": C.c_char_p, "isai_status_string": C.c_char_p}

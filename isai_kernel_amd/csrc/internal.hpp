@@ -114,11 +114,21 @@ struct InnerPrecond {  // ISAI factors in BCSR with FP32 / BF16 values
 };
 int mixed_dot_groups(int nb);
 size_t inner_part_doubles(int nb);
+// Value layouts of the FP32 A product: the caller's rows (read through L2), every
+// row starting on a 128-byte line, or row pairs starting on a line (both padded
+// with zero blocks and streamed with non-temporal loads).
+enum { kA32Unaligned = 0, kA32RowsAligned = 1, kA32PairsAligned = 2 };
+struct A32Operand {  // A with FP32 values in one of those layouts
+    long long nnzb;  // blocks stored, padding included
+    const int *rp, *ci;
+    const float *val;
+    int layout;
+};
 // y = A x with FP32 values, FP32 x and y; part[wg] = sum y.a (and part[G + wg] =
 // sum y.y for ndot = 2) over the rows of workgroup wg, in FP64; ndot = 0: no sums
-// (a and part unused).
+// (a and part unused).  G = mixed_dot_groups(nb) for every layout.
 int mixed_spmv_f32_dot(int nb, long long nnzb, const int *rp, const int *ci, const float *val, const float *x,
-                       float *y, const float *a, int ndot, double *part, hipStream_t st, const int *skip);
+                       float *y, const float *a, int ndot, double *part, hipStream_t st, const int *skip, int layout);
 struct InnerLog {  // outcome of one inner solve, recorded on the device
     int done, iters;
     double tol_in;
@@ -128,8 +138,8 @@ struct InnerLog {  // outcome of one inner solve, recorded on the device
 // estimate and *tol2 = tol^2 ||b||^2; starts done when *outer_done != 0.
 int inner_init(size_t n, const double *rhs, InnerVecs *w, InnerScalars *IS, const int *outer_done,
                const double *gj, const double *tol2, double floor_tol, hipStream_t st);
-int inner_iteration(size_t n, int nb, long long nnzb, const int *rp, const int *ci, const float *A32,
-                    const InnerPrecond *M, InnerVecs *w, InnerScalars *IS, bool first, hipStream_t st);
+int inner_iteration(size_t n, int nb, const A32Operand &A, const InnerPrecond *M, InnerVecs *w, InnerScalars *IS,
+                    bool first, hipStream_t st);
 // z = x in FP64; *log = the inner solve's done flag, iterations and tolerance.
 int inner_finish(size_t n, const InnerVecs *w, double *z, const InnerScalars *IS, InnerLog *log, hipStream_t st);
 

@@ -23,38 +23,56 @@ constexpr int kMixedVecThreads = 1024; // vector kernels
 constexpr int kMixedVecBlocks = 512;
 
 // -------------------------------------------------- FP32 A product, bs = 4
-// Eight lanes per block row; lane s loads 16 B = row s%4 of block 2j + s/4 of the
+// Eight lanes per lane group; lane s loads 16 B = row s%4 of block 2j + s/4 of the
 // batch, so one load instruction of the eight lanes reads two consecutive blocks
 // (one 128-byte line when the pair is line-aligned).  The lanes of one block (a
 // quad) each gather one FP32 component of the block's x column and exchange the
 // rest with DPP moves.  The batch's column indices are loaded once per lane group
 // and handed out by shuffle.  All value loads and x gathers of a batch are issued
-// before the first product and are unconditional (a lane past the row's end reads
-// block 0 and discards it): loads under a per-block branch were waited for one at
-// a time.  The grid strides over 32-row chunks.  FP32 accumulation: lane (b, sub)
-// sums blocks b, b+2, b+4, ... of the row in order and the two lanes of a row are
-// added at the end.  Measured on the 96^3 7-pt A (FP32 values, two dots): 83.4 us
-// against 88.6 us with per-block branches; column-major blocks (lane = block
-// column, no DPP) 102 us; two rows per lane group 82-88 us.
-template <int NDOT, int KB>
+// before the first product and are unconditional (a lane past the range's end
+// reads block 0 and discards it): loads under a per-block branch were waited for
+// one at a time.  The grid strides over chunks of 32 R block rows.  FP32
+// accumulation: lane (b, sub) sums the blocks of a row at even (b = 0) or odd
+// (b = 1) positions of its range, in order, and the two lanes are added at the end.
+//
+// R = rows per lane group, and the value layout:
+//  * R = 1, NT = false: any row layout (the caller's arrays); values through L2.
+//  * R = 1, NT = true: every row starts on a 128-byte line (padded with zero
+//    blocks to an even count); values streamed with non-temporal loads.
+//  * R = 2, NT = true: a lane group takes a row pair stored contiguously, the pair
+//    starting on a line and its second row padded so the pair fills whole lines.
+//    A block counts to the first row before position lenA of the pair's range and
+//    to the second after it; the second row's blocks land on the lanes of their
+//    parity relative to the pair, so each lane still sums every other block of the
+//    row in order and the row's result equals the R = 1 result.
+// A line that two lane groups share is read twice when streamed non-temporally
+// (+30 % reads on 7-block rows), and padding every 7-block row to 8 blocks adds
+// 13.5 % of the blocks, which row pairs avoid (0.3 %).  Measured (FP32 values, two
+// dots): 96^3 7-pt 86.0 us (R = 1 through L2), 83.1 us (R = 1 aligned), 78.5 us
+// (R = 2); 27-pt 64^3 83.4, 76.9 (3.5 % padding) and 80.2 us.  Pipelining the
+// next chunk's row pointers or whole first batch changed nothing (85.7-86.4 us),
+// and staging a chunk's contiguous blocks through LDS was slower (129 us).
+template <int NDOT, int KB, int R, bool NT>
 __global__ __launch_bounds__(kMixedBlock) void bsr4_spmv_f32_dot_kernel(
     int nb, const int *__restrict__ rp, const int *__restrict__ ci, const v4f *__restrict__ val4,
     const float *__restrict__ x, float *__restrict__ y, const float *__restrict__ a, double *__restrict__ part,
     const int *__restrict__ skip) {
     static_assert(KB == 4 || KB == 8 || KB == 16 || KB == 32, "KB");
+    static_assert(R == 1 || R == 2, "R");
     if (skip && *skip) return;
-    constexpr int kJ = KB / 2;
+    constexpr int kJ = KB / 2, kC = KB < 8 ? 1 : KB / 8, kRows = R * (kMixedBlock / 8);
     const int s = threadIdx.x & 7;
     const int b = s >> 2, sub = s & 3;
     double d0 = 0.0, d1 = 0.0;
-    for (int chunk = blockIdx.x; chunk * (kMixedBlock / 8) < nb; chunk += gridDim.x) {
-        const int br = chunk * (kMixedBlock / 8) + (threadIdx.x >> 3);
-        const int rs = br < nb ? rp[br] : 0;
-        const int len = br < nb ? rp[br + 1] - rs : 0;
-        float acc = 0.0f;
+    for (int chunk = blockIdx.x; chunk * kRows < nb; chunk += gridDim.x) {
+        const int r0 = chunk * kRows + R * (threadIdx.x >> 3);  // the group's first row
+        const int rs = r0 < nb ? rp[r0] : 0;
+        const int mid = r0 < nb ? rp[r0 + 1] : 0;
+        const int re = R == 2 && r0 + 1 < nb ? rp[r0 + 2] : mid;
+        const int len = re - rs, lenA = mid - rs;
+        float accA = 0.0f, accB = 0.0f;
         for (int t = 0; t < len; t += KB) {
             // lane s holds the column of blocks t + 8q + s
-            constexpr int kC = KB < 8 ? 1 : KB / 8;
             int cv[kC];
 #pragma unroll
             for (int q = 0; q < kC; ++q) cv[q] = s < KB && t + 8 * q + s < len ? ci[rs + t + 8 * q + s] : 0;
@@ -63,7 +81,8 @@ __global__ __launch_bounds__(kMixedBlock) void bsr4_spmv_f32_dot_kernel(
 #pragma unroll
             for (int j = 0; j < kJ; ++j) {
                 const int k = t + 2 * j + b;
-                w[j] = val4[(size_t)(k < len ? rs + k : 0) * 4 + sub];
+                const size_t e = (size_t)(k < len ? rs + k : 0) * 4 + sub;
+                w[j] = NT ? __builtin_nontemporal_load(&val4[e]) : val4[e];
             }
 #pragma unroll
             for (int j = 0; j < kJ; ++j) {
@@ -77,19 +96,28 @@ __global__ __launch_bounds__(kMixedBlock) void bsr4_spmv_f32_dot_kernel(
                 const float f1 = __int_as_float(__builtin_amdgcn_mov_dpp(v, 0x55, 0xF, 0xF, false));
                 const float f2 = __int_as_float(__builtin_amdgcn_mov_dpp(v, 0xAA, 0xF, 0xF, false));
                 const float f3 = __int_as_float(__builtin_amdgcn_mov_dpp(v, 0xFF, 0xF, 0xF, false));
-                if (t + 2 * j + b >= len) continue;
+                const int k = t + 2 * j + b;
+                if (k >= len) continue;
+                const bool first = R == 1 || k < lenA;  // a select, not a branch: both rows share the lanes
+                float acc = first ? accA : accB;
                 acc = fmaf(w[j].x, f0, acc);
                 acc = fmaf(w[j].y, f1, acc);
                 acc = fmaf(w[j].z, f2, acc);
                 acc = fmaf(w[j].w, f3, acc);
+                accA = first ? acc : accA;
+                if (R == 2) accB = first ? accB : acc;
             }
         }
-        acc += __shfl_xor(acc, 4);
-        if (br < nb && b == 0) {
-            const size_t row = (size_t)br * 4 + sub;
-            y[row] = acc;
-            if (NDOT >= 1) d0 += (double)acc * (double)a[row];
-            if (NDOT == 2) d1 += (double)acc * (double)acc;
+        const float tA = accA + __shfl_xor(accA, 4);
+        const float tB = R == 2 ? accB + __shfl_xor(accB, 4) : 0.0f;
+        // R = 1: lanes b = 0 write the row; R = 2: lanes b write row r0 + b
+        const int row = r0 + (R == 2 ? b : 0);
+        if (row < nb && (R == 2 || b == 0)) {
+            const float res = R == 2 && b ? tB : tA;
+            const size_t i = (size_t)row * 4 + sub;
+            y[i] = res;
+            if (NDOT >= 1) d0 += (double)res * (double)a[i];
+            if (NDOT == 2) d1 += (double)res * (double)res;
         }
     }
     if (NDOT == 0) return;
@@ -121,32 +149,36 @@ int mixed_dot_groups(int nb) {
     return chunks < kMixedGroups ? (chunks > 0 ? chunks : 1) : kMixedGroups;
 }
 
+namespace {
+template <int ND, int R, bool NT>
+void launch_f32_dot(int kb, unsigned g, int nb, const int *rp, const int *ci, const v4f *v, const float *x, float *y,
+                    const float *a, double *part, hipStream_t st, const int *skip) {
+    if (kb == 4) bsr4_spmv_f32_dot_kernel<ND, 4, R, NT><<<g, kMixedBlock, 0, st>>>(nb, rp, ci, v, x, y, a, part, skip);
+    else if (kb == 8) bsr4_spmv_f32_dot_kernel<ND, 8, R, NT><<<g, kMixedBlock, 0, st>>>(nb, rp, ci, v, x, y, a, part, skip);
+    else if (kb == 16) bsr4_spmv_f32_dot_kernel<ND, 16, R, NT><<<g, kMixedBlock, 0, st>>>(nb, rp, ci, v, x, y, a, part, skip);
+    else bsr4_spmv_f32_dot_kernel<ND, 32, R, NT><<<g, kMixedBlock, 0, st>>>(nb, rp, ci, v, x, y, a, part, skip);
+}
+template <int ND>
+void launch_f32_dot_layout(int layout, int kb, unsigned g, int nb, const int *rp, const int *ci, const v4f *v,
+                           const float *x, float *y, const float *a, double *part, hipStream_t st, const int *skip) {
+    if (layout == kA32RowsAligned) launch_f32_dot<ND, 1, true>(kb, g, nb, rp, ci, v, x, y, a, part, st, skip);
+    else if (layout == kA32PairsAligned) launch_f32_dot<ND, 2, true>(kb, g, nb, rp, ci, v, x, y, a, part, st, skip);
+    else launch_f32_dot<ND, 1, false>(kb, g, nb, rp, ci, v, x, y, a, part, st, skip);
+}
+}  // namespace
+
 int mixed_spmv_f32_dot(int nb, long long nnzb, const int *rp, const int *ci, const float *val, const float *x,
-                       float *y, const float *a, int ndot, double *part, hipStream_t st, const int *skip) {
+                       float *y, const float *a, int ndot, double *part, hipStream_t st, const int *skip, int layout) {
     if (nb <= 0) return ISAI_OK;
+    // Every workgroup writes its partials, also one left without a chunk (row pairs
+    // halve the chunk count): G = mixed_dot_groups(nb) for every layout.
     const unsigned g = (unsigned)mixed_dot_groups(nb);
     const long long avg = (nnzb + nb - 1) / nb;
     const int kb = avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : 32;
     const v4f *v = (const v4f *)val;
-#define ISAI_F32_DOT(ND, K) \
-    bsr4_spmv_f32_dot_kernel<ND, K><<<g, kMixedBlock, 0, st>>>(nb, rp, ci, v, x, y, a, part, skip)
-    if (ndot == 0) {
-        if (kb == 4) ISAI_F32_DOT(0, 4);
-        else if (kb == 8) ISAI_F32_DOT(0, 8);
-        else if (kb == 16) ISAI_F32_DOT(0, 16);
-        else ISAI_F32_DOT(0, 32);
-    } else if (ndot == 1) {
-        if (kb == 4) ISAI_F32_DOT(1, 4);
-        else if (kb == 8) ISAI_F32_DOT(1, 8);
-        else if (kb == 16) ISAI_F32_DOT(1, 16);
-        else ISAI_F32_DOT(1, 32);
-    } else {
-        if (kb == 4) ISAI_F32_DOT(2, 4);
-        else if (kb == 8) ISAI_F32_DOT(2, 8);
-        else if (kb == 16) ISAI_F32_DOT(2, 16);
-        else ISAI_F32_DOT(2, 32);
-    }
-#undef ISAI_F32_DOT
+    if (ndot == 0) launch_f32_dot_layout<0>(layout, kb, g, nb, rp, ci, v, x, y, a, part, st, skip);
+    else if (ndot == 1) launch_f32_dot_layout<1>(layout, kb, g, nb, rp, ci, v, x, y, a, part, st, skip);
+    else launch_f32_dot_layout<2>(layout, kb, g, nb, rp, ci, v, x, y, a, part, st, skip);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
 }
@@ -349,8 +381,8 @@ int inner_init(size_t n, const double *rhs, InnerVecs *w, InnerScalars *IS, cons
 }
 
 // One iteration: p update, p^ = M p, v = A p^, s update, s^ = M s, t = A s^, x/r update.
-int inner_iteration(size_t n, int nb, long long nnzb, const int *rp, const int *ci, const float *A32,
-                    const InnerPrecond *M, InnerVecs *w, InnerScalars *IS, bool first, hipStream_t st) {
+int inner_iteration(size_t n, int nb, const A32Operand &A, const InnerPrecond *M, InnerVecs *w, InnerScalars *IS,
+                    bool first, hipStream_t st) {
     const int *skip = &IS->done;
     const int G = mixed_dot_groups(nb);
     inner_p_kernel<<<kMixedVecBlocks, kMixedVecThreads, 0, st>>>(n, w->r, w->p, w->v, w->rpart, IS, first ? 1 : 0);
@@ -360,7 +392,7 @@ int inner_iteration(size_t n, int nb, long long nnzb, const int *rp, const int *
     if (s == ISAI_OK)
         s = spmv_lowp_launch(nb, M->nnzbU, M->prec, M->Urow, M->Ucol, M->Uval, w->tmp, true, w->ph, true, st, skip,
                              true);
-    if (s == ISAI_OK) s = mixed_spmv_f32_dot(nb, nnzb, rp, ci, A32, w->ph, w->v, w->rh, 1, w->part, st, skip);
+    if (s == ISAI_OK) s = mixed_spmv_f32_dot(nb, A.nnzb, A.rp, A.ci, A.val, w->ph, w->v, w->rh, 1, w->part, st, skip, A.layout);
     if (s != ISAI_OK) return s;
     inner_s_kernel<<<kMixedVecBlocks, kMixedVecThreads, 0, st>>>(n, w->r, w->v, w->s, w->part, G, IS);
     ISAI_LAUNCH_CHECK();
@@ -369,7 +401,7 @@ int inner_iteration(size_t n, int nb, long long nnzb, const int *rp, const int *
     if (s == ISAI_OK)
         s = spmv_lowp_launch(nb, M->nnzbU, M->prec, M->Urow, M->Ucol, M->Uval, w->tmp, true, w->sh, true, st, skip,
                              true);
-    if (s == ISAI_OK) s = mixed_spmv_f32_dot(nb, nnzb, rp, ci, A32, w->sh, w->t, w->s, 2, w->part, st, skip);
+    if (s == ISAI_OK) s = mixed_spmv_f32_dot(nb, A.nnzb, A.rp, A.ci, A.val, w->sh, w->t, w->s, 2, w->part, st, skip, A.layout);
     if (s != ISAI_OK) return s;
     inner_xr_kernel<<<kMixedVecBlocks, kMixedVecThreads, 0, st>>>(n, w->x, w->r, w->ph, w->sh, w->s, w->t, w->rh,
                                                                   w->part, G, w->rpart, IS);
@@ -397,5 +429,5 @@ extern "C" int isai_bsr_spmv_f32_dot(int mb, int nnzb, const int *d_rowptr, cons
                  "isai_bsr_spmv_f32_dot: null pointer");
     if (groups) *groups = isai::mixed_dot_groups(mb);
     return isai::mixed_spmv_f32_dot(mb, nnzb, d_rowptr, d_colind, d_val, d_x, d_y, d_a, ndot, d_part,
-                                    (hipStream_t)stream, nullptr);
+                                    (hipStream_t)stream, nullptr, isai::kA32Unaligned);
 }

@@ -763,7 +763,11 @@ struct isai_handle {
     double *own_val = nullptr;
     // mixed-precision form: A rounded to FP32, the inner solve's FP32 vectors and
     // FP64 partials, its scalar record, and the inner-solve parameters
+    // (A32 is laid out on 128-byte lines with its own pattern: build_a32)
     float *d_A32 = nullptr;
+    int *d_A32row = nullptr, *d_A32col = nullptr;
+    long long nnzb32 = 0;
+    int a32_layout = kA32Unaligned;
     float *inner_vecs = nullptr;
     double *inner_part = nullptr;
     InnerScalars *IS = nullptr;
@@ -1113,32 +1117,93 @@ int fgmres(isai_handle *h, int m, double tol, int maxit, const double *b, double
 // ------------------------------------------------- mixed-precision form
 constexpr int kLogSteps = 16;  // inner outcomes held on the device between two host reads
 
-// Buffers of the inner solve: A in FP32 (converted once per handle), ten FP32
-// vectors, the partial sums, the scalar record and the outcome log.
+// The FP32 copy of A that the inner solves multiply with (converted once per
+// handle).  The FP32 product streams its values with non-temporal loads, which
+// read a 128-byte line twice when two lane groups share it, so the copy is laid
+// out on lines: every row padded with zero blocks to an even block count when that
+// adds at most 5 % of the blocks (the 27-pt stencil: 3.5 %), otherwise row pairs
+// stored contiguously from a line and the pair's second row padded (7-block rows:
+// 13.5 % as single rows, 0.3 % as pairs).  The products equal those with the
+// unpadded values: a zero block adds exact zeros.
+int build_a32(isai_handle *h, hipStream_t st) {
+    const int nb = h->nb;
+    std::vector<int> rp(nb + 1), prp(nb + 1, 0);
+    ISAI_HIP(hipMemcpy(rp.data(), h->d_row, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost));
+    long long odd = 0;
+    for (int i = 0; i < nb; ++i) odd += (rp[i + 1] - rp[i]) & 1;
+    const int layout = odd * 20 <= h->nnzb ? kA32RowsAligned : kA32PairsAligned;
+    long long tot = 0;
+    for (int i = 0; i < nb && tot <= INT_MAX;) {
+        const long long la = rp[i + 1] - rp[i];
+        if (layout == kA32RowsAligned || i + 1 == nb) {
+            tot += (la + 1) / 2 * 2;
+            prp[++i] = (int)std::min(tot, (long long)INT_MAX);
+        } else {
+            const long long lb = rp[i + 2] - rp[i + 1];
+            prp[i + 1] = (int)std::min(tot + la, (long long)INT_MAX);
+            tot += (la + lb + 1) / 2 * 2;
+            prp[i + 2] = (int)std::min(tot, (long long)INT_MAX);
+            i += 2;
+        }
+    }
+    if (tot > INT_MAX) {
+        set_error("isai: the line-aligned FP32 copy of A exceeds %d blocks", INT_MAX);
+        return ISAI_ERR_CAPACITY;
+    }
+    if (hipMalloc(&h->d_A32row, sizeof(int) * (nb + 1)) != hipSuccess ||
+        hipMalloc(&h->d_A32col, sizeof(int) * (size_t)std::max(tot, 1LL)) != hipSuccess ||
+        hipMalloc(&h->d_A32, sizeof(float) * 16 * (size_t)std::max(tot, 1LL)) != hipSuccess) {
+        (void)hipFree(h->d_A32row);
+        (void)hipFree(h->d_A32col);
+        (void)hipFree(h->d_A32);
+        h->d_A32row = h->d_A32col = nullptr;
+        h->d_A32 = nullptr;
+        set_error("isai_solve: cannot allocate the FP32 copy of A (%lld blocks)", tot);
+        return ISAI_ERR_ALLOC;
+    }
+    h->nnzb32 = tot;
+    h->a32_layout = layout;
+    ISAI_HIP(hipMemcpyAsync(h->d_A32row, prp.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st));
+    TRY(pad_convert_lowp(nb, h->d_row, h->d_col, h->d_val, h->d_A32row, h->d_A32col, h->d_A32, ISAI_PREC_FP32, st));
+    // prp is a host temporary: wait for its upload before it goes out of scope
+    ISAI_HIP(hipStreamSynchronize(st));
+    return ISAI_OK;
+}
+
+A32Operand a32_operand(const isai_handle *h) {
+    A32Operand A;
+    A.nnzb = h->nnzb32;
+    A.rp = h->d_A32row;
+    A.ci = h->d_A32col;
+    A.val = h->d_A32;
+    A.layout = h->a32_layout;
+    return A;
+}
+
+// Buffers of the inner solve: the FP32 copy of A (build_a32), ten FP32 vectors, the
+// partial sums, the scalar record and the outcome log.
 int ensure_inner(isai_handle *h, hipStream_t st) {
     if (h->IS) return ISAI_OK;
     const size_t n = h->n;
-    const size_t b2 = 16;
-    if (hipMalloc(&h->d_A32, sizeof(float) * b2 * h->nnzb) != hipSuccess ||
-        hipMalloc(&h->inner_vecs, sizeof(float) * 10 * n) != hipSuccess ||
+    if (!h->d_A32) TRY(build_a32(h, st));
+    if (hipMalloc(&h->inner_vecs, sizeof(float) * 10 * n) != hipSuccess ||
         hipMalloc(&h->inner_part, sizeof(double) * inner_part_doubles(h->nb)) != hipSuccess ||
         hipMalloc(&h->IS, sizeof(InnerScalars)) != hipSuccess ||
         hipMalloc(&h->d_log, sizeof(InnerLog) * kLogSteps) != hipSuccess ||
         hipHostMalloc(&h->h_inner, sizeof(int) * 2) != hipSuccess ||
         hipHostMalloc(&h->h_log, sizeof(InnerLog) * kLogSteps) != hipSuccess) {
-        (void)hipFree(h->d_A32);
         (void)hipFree(h->inner_vecs);
         (void)hipFree(h->inner_part);
         (void)hipFree(h->IS);
         (void)hipFree(h->d_log);
         if (h->h_inner) (void)hipHostFree(h->h_inner);
         if (h->h_log) (void)hipHostFree(h->h_log);
-        h->d_A32 = nullptr, h->inner_vecs = nullptr, h->inner_part = nullptr, h->IS = nullptr;
+        h->inner_vecs = nullptr, h->inner_part = nullptr, h->IS = nullptr;
         h->d_log = nullptr, h->h_inner = nullptr, h->h_log = nullptr;
         set_error("isai_solve: cannot allocate the mixed-precision inner solve");
         return ISAI_ERR_ALLOC;
     }
-    return isai_convert_values(b2 * h->nnzb, ISAI_PREC_FP32, h->d_val, h->d_A32, (isai_stream_t)st);
+    return ISAI_OK;
 }
 
 InnerVecs inner_vecs(isai_handle *h) {
@@ -1172,8 +1237,7 @@ int inner_solve(isai_handle *h, const InnerPrecond *M, const double *v, double *
     int launched = 0;
     while (true) {
         for (int k = 0; k < batch; ++k, ++launched)
-            TRY(inner_iteration(h->n, h->nb, h->nnzb, h->d_row, h->d_col, h->d_A32, M, &w, h->IS, launched == 0,
-                                st));
+            TRY(inner_iteration(h->n, h->nb, a32_operand(h), M, &w, h->IS, launched == 0, st));
         if (hist > 0) break;
         ISAI_HIP(hipMemcpyAsync(h->h_inner, &h->IS->done, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
         ISAI_HIP(hipStreamSynchronize(st));
@@ -1228,7 +1292,8 @@ int inner_gmres(isai_handle *h, const InnerPrecond *M, const double *v, double *
                              true));
         TRY(spmv_lowp_launch(h->nb, M->nnzbU, M->prec, M->Urow, M->Ucol, M->Uval, tmp, true, zh, true, st, skip,
                              true));
-        TRY(mixed_spmv_f32_dot(h->nb, h->nnzb, h->d_row, h->d_col, h->d_A32, zh, w, nullptr, 0, nullptr, st, skip));
+        TRY(mixed_spmv_f32_dot(h->nb, h->nnzb32, h->d_A32row, h->d_A32col, h->d_A32, zh, w, nullptr, 0, nullptr, st,
+                               skip, h->a32_layout));
         // one classical Gram-Schmidt pass: the cycle only has to reach the inner
         // tolerance, and a second pass changed neither its steps nor the outer steps
         const int nv = i + 1;
@@ -1445,6 +1510,8 @@ int isai_destroy(isai_handle_t h) {
     (void)hipFree(h->own_col);
     (void)hipFree(h->own_val);
     (void)hipFree(h->d_A32);
+    (void)hipFree(h->d_A32row);
+    (void)hipFree(h->d_A32col);
     (void)hipFree(h->ig_vecs);
     (void)hipFree(h->Sin);
     (void)hipFree(h->inner_vecs);
@@ -1925,6 +1992,19 @@ int isai_solve(isai_handle_t h, int solver, int restart, double tol, int maxit, 
     // a triangular-solve wait that gave up makes the iterates meaningless
     const int t = isai_handle_status(h, stream);
     return t != ISAI_OK ? t : s;
+}
+
+int isai_handle_spmv_f32_dot(isai_handle_t h, const float *d_x, float *d_y, const float *d_a, int ndot,
+                             double *d_part, int *groups, int *rows_per_group, isai_stream_t stream) {
+    ISAI_REQUIRE(h && d_x && d_y, "isai_handle_spmv_f32_dot: null argument");
+    ISAI_REQUIRE(ndot >= 0 && ndot <= 2, "isai_handle_spmv_f32_dot: ndot %d", ndot);
+    ISAI_REQUIRE(ndot == 0 || (d_a && d_part), "isai_handle_spmv_f32_dot: null dot operand or partials");
+    hipStream_t st = (hipStream_t)stream;
+    if (!h->d_A32) TRY(build_a32(h, st));
+    if (groups) *groups = mixed_dot_groups(h->nb);
+    if (rows_per_group) *rows_per_group = h->a32_layout == kA32PairsAligned ? 2 : 1;
+    return mixed_spmv_f32_dot(h->nb, h->nnzb32, h->d_A32row, h->d_A32col, h->d_A32, d_x, d_y, d_a, ndot, d_part, st,
+                              nullptr, h->a32_layout);
 }
 
 int isai_set_inner_solve(isai_handle_t h, double inner_tol, int inner_maxit) {

@@ -434,7 +434,8 @@ __global__ void convert_bf16_kernel(size_t n, const double *__restrict__ src, un
 // ------------------------------------------------ line-aligned reduced-precision copy
 // Every row of the copy starts on a 128-byte line: its block count is padded up to
 // a multiple of the blocks per line (align = 2 FP32, 4 BF16) with blocks of zeros
-// whose column repeats the row's last column.  A zero block adds exact zeros, so a
+// whose column repeats the row's last column (any padded row pointer works: the
+// FP32 copy of A pads row pairs instead).  A zero block adds exact zeros, so a
 // product with the copy equals the product with the unpadded factor; without the
 // padding a row starting mid-line shares that line with its neighbour and the line
 // is fetched twice.  One 16-lane group per row, lane e converting element e of each
@@ -449,9 +450,9 @@ __global__ void pad_convert_kernel(int nb, const int *__restrict__ rp, const int
     const int s = rp[row], len = rp[row + 1] - s, ps = prp[row], plen = prp[row + 1] - ps;
     for (int j = 0; j < plen; ++j) {
         const bool real = j < len;
-        const int src = real ? s + j : s + len - 1;
-        if (e == 0) pci[ps + j] = ci[src];
-        const double v = real ? val[(size_t)src * 16 + e] : 0.0;
+        // padding repeats the row's last column (the row index for an empty row)
+        if (e == 0) pci[ps + j] = real ? ci[s + j] : len > 0 ? ci[s + len - 1] : row;
+        const double v = real ? val[(size_t)(s + j) * 16 + e] : 0.0;
         if (BF16) ((unsigned short *)pval)[(size_t)(ps + j) * 16 + e] = f32_to_bf16_rne((float)v);
         else ((float *)pval)[(size_t)(ps + j) * 16 + e] = (float)v;
     }

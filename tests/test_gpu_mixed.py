@@ -117,6 +117,50 @@ def test_mixed_inner_product_matches_oracle_rounding(gpu):
     assert abs(part[groups:2 * groups].sum() - float(np.dot(y, y))) <= 1e-12 * np.dot(y, y)
 
 
+@pytest.mark.parametrize("problem,L,rows", [(1, 24, 2), (2, 10, 1), (2, 9, 1), (0, 33, 2), (3, 21, 2)])
+def test_handle_fp32_product_line_aligned_layouts(gpu, problem, L, rows):
+    """The product the mixed solves run, on the handle's line-aligned FP32 copy of A
+    (one row per lane group with every row padded to an even block count, or row
+    pairs): y equal to the public product on the unpadded FP32 values, and the fused
+    FP64 dots equal to the host dots of y up to FP64 rounding, for ndot = 0, 1, 2.
+    Odd block-row counts (9^3, 33^2, 21^2) leave the last pair with one row."""
+    print(f"handle fp32 product problem={problem} L={L}", flush=True)
+    rp, ci, val = problems.generate(problem, L, seed=4)
+    A = Matrix(rp, ci, val)
+    n = A.n
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(n).astype(np.float32)
+    a = rng.standard_normal(n).astype(np.float32)
+    A32 = DeviceArray((len(ci) * 16,), np.float32)
+    call("isai_convert_values", len(ci) * 16, 1, A.d_val.ptr, A32.ptr, None)
+    d_x, d_a = DeviceArray.from_numpy(x), DeviceArray.from_numpy(a)
+    d_y0 = DeviceArray((n,), np.float32)
+    d_p0, d_part = DeviceArray((2 * 4096,), np.float64), DeviceArray((2 * 4096,), np.float64)
+    g0, g, rpg = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    call("isai_bsr_spmv_f32_dot", A.nb, A.nnzb, A.d_rp.ptr, A.d_ci.ptr, A32.ptr, d_x.ptr, d_y0.ptr, d_a.ptr, 2,
+         d_p0.ptr, ctypes.byref(g0), None)
+    h = Handle(A, precond="isai", precision="bf16")
+    y0 = None
+    for ndot in (0, 1, 2):
+        d_y = DeviceArray.from_numpy(np.full(n, np.nan, np.float32))  # no stale result can pass
+        call("isai_handle_spmv_f32_dot", h.h, d_x.ptr, d_y.ptr, d_a.ptr, ndot, d_part.ptr, ctypes.byref(g),
+             ctypes.byref(rpg), None)
+        call("isai_device_synchronize")
+        assert rpg.value == rows and g.value == g0.value
+        y0 = d_y0.to_numpy() if y0 is None else y0
+        y = d_y.to_numpy()
+        assert np.array_equal(y, y0), ndot
+        G = g.value
+        yd = y.astype(np.float64)
+        part = d_part.to_numpy()
+        if ndot >= 1:
+            ya = float(np.dot(yd, a.astype(np.float64)))
+            assert abs(part[:G].sum() - ya) <= 1e-12 * np.sum(np.abs(yd * a))
+        if ndot == 2:
+            assert abs(part[G:2 * G].sum() - float(np.dot(yd, yd))) <= 1e-12 * np.dot(yd, yd)
+    h.close()
+
+
 def test_mixed_solve_full_size_96cube(gpu):
     """The north-star workload (3-D 7-pt 96^3, b = A*1) with BF16 ISAI: FP64 relative
     residual <= 1e-10 checked on the host, and the solution of all ones."""
