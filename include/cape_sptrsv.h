@@ -46,9 +46,15 @@ int isai_csr_sptrsv_lu_device(int m, int nnz, const int *d_rowptr, const int *d_
  *     into units of 64 so that most dependencies fall inside a unit: those travel
  *     through LDS, and one wave per workgroup imports the others from the solution
  *     vector.  In level order every dependency is polled from the solution vector.
+ *     Level-synchronous units: units of 512 rows (8x8x8 blocks of a 3-D 7-point
+ *     grid), each row sorted by its level inside its unit, and the unit's waves
+ *     step through those levels together with one barrier per level; rows with
+ *     at most 3 dependencies.
  * chunk = 1..128: chunk schedule with at most that many rows per chunk; chunk = -1:
  * row schedule in unit order; -2: in level order; -3: unit order without the
  * import wave (what -1 falls back to when a unit imports more than 512 rows);
+ * -4: level-synchronous units (falls back to -1 when a row has more than 3
+ * dependencies, a unit imports more than 512 rows or holds levels too wide for it);
  * chunk = 0: -1 when the levels of the triangles average at least 16 block rows,
  * else the chunk schedule with 128.  The analysis builds the tables from the
  * pattern and pre-scales the off-diagonal blocks by the inverted diagonal parts of
@@ -67,6 +73,10 @@ int isai_bsr_sptrsv_set_spin_limit(isai_sptrsv_plan_t plan, int limit);
 /* Synchronizes the stream; ISAI_ERR_TIMEOUT if a wait of the previous solves
  * ran out (the error word is then cleared). */
 int isai_sptrsv_status(isai_sptrsv_plan_t plan, isai_stream_t stream);
+/* The schedule the plan runs, in the chunk argument's terms: -4 level-synchronous
+ * units, -1 unit order with the import wave, -3 unit order through the row kernel,
+ * -2 level order, or the chunk schedule's rows per chunk (> 0). */
+int isai_bsr_sptrsv_schedule(isai_sptrsv_plan_t plan, int *schedule);
 int isai_bsr_sptrsv_destroy(isai_sptrsv_plan_t plan);
 
 #ifdef __cplusplus

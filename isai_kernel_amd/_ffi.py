@@ -113,6 +113,7 @@ SIGNATURES: dict[str, list] = {
     "isai_bsr_sptrsv_set_spin_limit": [P, I],
     "isai_sptrsv_status": [P, P],
     "isai_bsr_sptrsv_destroy": [P],
+    "isai_bsr_sptrsv_schedule": [P, P],
     # isai_solve.h
     "isai_setup": [I, I, I, P, P, P, P, P, P],
     "isai_setup_times": [P, P, P, P],

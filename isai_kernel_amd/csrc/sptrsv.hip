@@ -517,6 +517,58 @@ constexpr int kUnitImpMax = 512;                 // imports per unit (2048 LDS w
 constexpr int kUnitLds = 64 * 1024;
 constexpr int kUnitItemsPerLane = kUnitImpMax * 4 / 64;
 
+// The import wave of a unit: lane l owns items e = l + 64 r (import e / 4,
+// component e % 4), polls the solution vector for each item until it is published
+// and copies it into slot[e] (LDS).  All polls of a group of eight items are
+// issued before any is compared.  Gives up after `limit` passes without progress
+// (the compute waves then time out on the missing slots); returns whether it did.
+__device__ __forceinline__ bool import_wave(int l, int nit, const int *__restrict__ imp, const double *out,
+                                            unsigned long long *slot, int limit) {
+    // word of the solution vector behind each item; the loads are issued
+    // together (clamped, no default value to merge, so no wait between them)
+    int at[kUnitItemsPerLane];
+    unsigned pend = 0;
+    if (nit > 0) {
+#pragma unroll
+        for (int r = 0; r < kUnitItemsPerLane; ++r) {
+            const int e = min(l + 64 * r, nit - 1);
+            at[r] = imp[e >> 2] * 4 + (e & 3);
+            if (l + 64 * r < nit) pend |= 1u << r;
+        }
+    }
+    bool timed_out = false;
+    int stall = 0;
+    while (__any(pend != 0)) {
+        bool got = false;
+#pragma unroll
+        for (int g = 0; g < kUnitItemsPerLane / 8; ++g) {
+            const unsigned pm = (pend >> (8 * g)) & 0xffu;
+            if (!__any(pm != 0)) continue;
+            unsigned long long v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (pm & (1u << j))
+                    v[j] = __hip_atomic_load((unsigned long long *)(out + at[8 * g + j]), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if ((pm & (1u << j)) && v[j] != kSentinel) {
+                    __hip_atomic_store(&slot[l + 64 * (8 * g + j)], v[j], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                    pend &= ~(1u << (8 * g + j));
+                    got = true;
+                }
+        }
+        stall = __any(got) ? 0 : stall + 1;
+        if (stall > limit) {
+            // give up: the compute waves time out on the missing slots
+            if (pend) timed_out = true;
+            pend = 0;
+        }
+    }
+    return timed_out;
+}
+
 template <bool UPPER, int kUnitBatch>
 __global__ __launch_bounds__(kUnitWG) void bsr4_sptrsv_unit_kernel(
     int nb, const int4 *__restrict__ info, const int *__restrict__ cxu, const int *__restrict__ imp_ptr,
@@ -536,49 +588,7 @@ __global__ __launch_bounds__(kUnitWG) void bsr4_sptrsv_unit_kernel(
     __syncthreads();
     bool timed_out = false;
     if (threadIdx.x >= kRowWG) {
-        // import wave: lane l owns items e = l + 64 r (import e / 4, component e % 4)
-        const int l = threadIdx.x - kRowWG;
-        // word of the solution vector behind each item; the loads are issued
-        // together (clamped, no default value to merge, so no wait between them)
-        int at[kUnitItemsPerLane];
-        unsigned pend = 0;
-        if (nit > 0) {
-#pragma unroll
-            for (int r = 0; r < kUnitItemsPerLane; ++r) {
-                const int e = min(l + 64 * r, nit - 1);
-                at[r] = imp_col[ib + (e >> 2)] * 4 + (e & 3);
-                if (l + 64 * r < nit) pend |= 1u << r;
-            }
-        }
-        int stall = 0;
-        while (__any(pend != 0)) {
-            bool got = false;
-#pragma unroll
-            for (int g = 0; g < kUnitItemsPerLane / 8; ++g) {
-                const unsigned pm = (pend >> (8 * g)) & 0xffu;
-                if (!__any(pm != 0)) continue;
-                unsigned long long v[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (pm & (1u << j))
-                        v[j] = __hip_atomic_load((unsigned long long *)(out + at[8 * g + j]), __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if ((pm & (1u << j)) && v[j] != kSentinel) {
-                        __hip_atomic_store(&unit_x[kRowWG + l + 64 * (8 * g + j)], v[j], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                        pend &= ~(1u << (8 * g + j));
-                        got = true;
-                    }
-            }
-            stall = __any(got) ? 0 : stall + 1;
-            if (stall > limit) {
-                // give up: the compute waves time out on the missing slots
-                if (pend) timed_out = true;
-                pend = 0;
-            }
-        }
+        timed_out = import_wave(threadIdx.x - kRowWG, nit, imp_col + ib, out, unit_x + kRowWG, limit);
     } else {
         const int q = w * kRowsPerWG + (int)(threadIdx.x >> 2);
         const int m = threadIdx.x & 3;
@@ -666,6 +676,179 @@ __global__ __launch_bounds__(kUnitWG) void bsr4_sptrsv_unit_kernel(
                 }
             }
         }
+    }
+    if (__any(timed_out) && threadIdx.x % 64 == 0) atomicOr(&cnt->err, 1);
+    if (threadIdx.x == 0) {
+        if (atomicAdd(donec, 1u) == (unsigned)gridDim.x - 1) {
+            atomicExch(ticket, 0u);
+            atomicExch(donec, 0u);
+        }
+    }
+}
+
+// ------------------------------------------------ level-synchronous units
+// Units of kLvUnit = 512 rows, grown by the unit order below with that size (on a
+// 3-D 7-point grid in natural order the greedy growth cuts the grid into 8x8x8
+// blocks: 88 % of the dependencies inside a unit, 22 local levels, at most 48 rows
+// per level and 192 imports per unit).  Inside a unit the rows are sorted by their
+// local level (1 + the deepest local level of their dependencies in the unit), and
+// the unit's eight compute waves step through the levels together: a row's inputs
+// from its own unit were written to LDS at a lower level, so after the level
+// barrier they are read without polling; inputs from other units arrive through
+// the import wave as in the unit kernel.  A level costs one barrier and one row
+// computation, where the unit kernel's per-row LDS polling cost about 0.8 us per
+// hop, and a dependency path of the 96^3 factor crosses 33 units instead of 69.
+//
+// Compute quad q (of 128) owns the rows at slots q + 128 k, k < 4, and loads all of
+// their inputs (block row m of each S_ik, inv(D_i) rhs_i) into registers at the
+// unit's start, so no global load lies on a level's path.  A wave's quads hold 16
+// consecutive slots of each k; the analysis checks that every wave meets its rows
+// in increasing level order (the highest level of its rows k - 1 below the lowest
+// of its rows k) and that no row has more than B dependencies, else the plan uses
+// the unit kernel.  The level barrier of the compute waves is an LDS counter (the
+// import wave keeps polling; s_barrier would tie it to the levels), preceded by a
+// wait for the wave's LDS stores only: the global stores that publish values are
+// not waited for.
+constexpr int kLvUnit = 512;
+constexpr int kLvQuads = 128;
+constexpr int kLvRows = kLvUnit / kLvQuads;  // rows per quad
+constexpr int kLvCompute = kLvQuads * 4;
+constexpr int kLvWG = kLvCompute + 64;
+constexpr int kLvWaves = kLvCompute / 64;
+
+template <bool UPPER, int B>
+__global__ __launch_bounds__(kLvWG) void bsr4_sptrsv_level_kernel(
+    int nb, const int4 *__restrict__ info, const int *__restrict__ cxu, const int *__restrict__ imp_ptr,
+    const int *__restrict__ imp_col, const double *__restrict__ S, const double *__restrict__ dinv,
+    const double *rhs, double *out, double *sentinel_fill, Counters *cnt, const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ int s_t;
+    __shared__ unsigned int s_bar;
+    extern __shared__ unsigned long long lv_x[];  // (kLvUnit + imports) x 4 values
+    unsigned int *ticket = UPPER ? &cnt->ticket_u : &cnt->ticket_l;
+    unsigned int *donec = UPPER ? &cnt->done_u : &cnt->done_l;
+    const int limit = cnt->spin_limit;
+    if (threadIdx.x == 0) {
+        s_t = (int)atomicAdd(ticket, 1u);
+        s_bar = 0u;
+    }
+    __syncthreads();
+    const int w = s_t;
+    const int ib = imp_ptr[w], nit = (imp_ptr[w + 1] - ib) * 4;
+    for (int t = threadIdx.x; t < nit; t += kLvWG) lv_x[kLvUnit * 4 + t] = kSentinel;
+    __syncthreads();
+    bool timed_out = false;
+    if (threadIdx.x >= kLvCompute) {
+        timed_out = import_wave(threadIdx.x - kLvCompute, nit, imp_col + ib, out, lv_x + kLvUnit * 4, limit);
+    } else {
+        const int quad = threadIdx.x >> 2, m = threadIdx.x & 3, wv = threadIdx.x >> 6;
+        const int base = w * kLvUnit;
+        const int nrows = min(kLvUnit, nb - base);
+        // this quad's rows: level (INT_MAX: none), row, dependency count, LDS word of
+        // each dependency's component m, row m of each S_ik, and c = (inv(D_i) rhs_i)[m]
+        int lvl[kLvRows], irow[kLvRows], nd[kLvRows], hi[kLvRows];
+        int sl[kLvRows][B];
+        double c[kLvRows], s[kLvRows][B][4];
+        const int nl = info[base + nrows - 1].w + 1;
+#pragma unroll
+        for (int k = 0; k < kLvRows; ++k) {
+            const int slot = quad + kLvQuads * k;
+            // highest level among the wave's rows k (its slots are consecutive and
+            // sorted by level), -1 when the wave has none
+            const int first = kLvQuads * k + 16 * wv;
+            hi[k] = first < nrows ? info[base + min(first + 15, nrows - 1)].w : -1;
+            lvl[k] = INT_MAX;
+            irow[k] = 0;
+            nd[k] = 0;
+            c[k] = 0.0;
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                sl[k][j] = m;
+                s[k][j][0] = s[k][j][1] = s[k][j][2] = s[k][j][3] = 0.0;
+            }
+            if (slot < nrows) {
+                const int4 f = info[base + slot];
+                irow[k] = f.x;
+                nd[k] = f.z - f.y;
+                lvl[k] = f.w;
+                const double *D = dinv + (size_t)f.x * 16 + m * 4;
+                const double *b = rhs + (size_t)f.x * 4;
+                c[k] = D[0] * b[0] + D[1] * b[1] + D[2] * b[2] + D[3] * b[3];
+#pragma unroll
+                for (int j = 0; j < B; ++j)
+                    if (j < nd[k]) {
+                        sl[k][j] = (~cxu[f.y + j]) * 4 + m;
+                        const double *Bk = S + (size_t)(f.y + j) * 16 + m * 4;
+                        s[k][j][0] = Bk[0], s[k][j][1] = Bk[1], s[k][j][2] = Bk[2], s[k][j][3] = Bk[3];
+                    }
+            }
+        }
+        // one row of this quad: read its inputs (own-unit inputs are final after the
+        // previous level's barrier; imports are polled in LDS), compute, publish
+        auto row = [&](int k) {
+            unsigned long long u[B];
+            unsigned miss = 0;
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                u[j] = __hip_atomic_load(&lv_x[sl[k][j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (j < nd[k] && u[j] == kSentinel) miss |= 1u << j;
+            }
+            int stall = 0;
+            long long t_stall = 0;
+            while (!quad_all(miss == 0)) {
+#pragma unroll
+                for (int j = 0; j < B; ++j)
+                    if (miss & (1u << j)) {
+                        u[j] = __hip_atomic_load(&lv_x[sl[k][j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (u[j] != kSentinel) miss &= ~(1u << j);
+                    }
+                // bounded by the clock (100 MHz): spin_limit microseconds
+                if (stall == 0) t_stall = __builtin_amdgcn_s_memrealtime();
+                if ((++stall & 255) == 0 && __builtin_amdgcn_s_memrealtime() - t_stall > (long long)limit * 100) {
+                    if (miss) timed_out = true;
+                    miss = 0;
+                }
+            }
+            double z = c[k];
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const double xm = __longlong_as_double((long long)u[j]);
+                const double x0 = quad_bcast<0>(xm), x1 = quad_bcast<1>(xm);
+                const double x2 = quad_bcast<2>(xm), x3 = quad_bcast<3>(xm);
+                if (j < nd[k]) z -= s[k][j][0] * x0 + s[k][j][1] * x1 + s[k][j][2] * x2 + s[k][j][3] * x3;
+            }
+            const int slot = quad + kLvQuads * k;
+            __hip_atomic_store(&lv_x[slot * 4 + m], (unsigned long long)__double_as_longlong(z), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+            publish(out + (size_t)irow[k] * 4 + m, z);
+            if (sentinel_fill) sentinel_fill[(size_t)irow[k] * 4 + m] = __longlong_as_double((long long)kSentinel);
+        };
+        // the compute waves' barrier after level l: this wave's LDS stores first
+        // (not its global stores), then one arrival per wave and a bounded wait
+        auto level_barrier = [&](int l) {
+            __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(&s_bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const unsigned target = (unsigned)kLvWaves * (unsigned)(l + 1);
+            int stall = 0;
+            long long t0 = 0;
+            while (__hip_atomic_load(&s_bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target) {
+                if (stall == 0) t0 = __builtin_amdgcn_s_memrealtime();
+                if ((++stall & 255) == 0 && __builtin_amdgcn_s_memrealtime() - t0 > (long long)limit * 100) {
+                    timed_out = true;
+                    break;
+                }
+            }
+            __asm__ volatile("" ::: "memory");
+        };
+        int l = 0;
+#pragma unroll
+        for (int k = 0; k < kLvRows; ++k) {
+            for (; l <= hi[k]; ++l) {
+                if (lvl[k] == l) row(k);
+                level_barrier(l);
+            }
+        }
+        for (; l < nl; ++l) level_barrier(l);
     }
     if (__any(timed_out) && threadIdx.x % 64 == 0) atomicOr(&cnt->err, 1);
     if (threadIdx.x == 0) {
@@ -785,6 +968,8 @@ struct isai_sptrsv_plan {
     bool rows = false;   // row schedule (one quad per block row) instead of chunks
     bool units = false;        // row schedule in unit order (else level order)
     bool unit_kernel = false;  // units solved by the import-wave kernel (else the row kernel)
+    bool level_kernel = false; // level-synchronous units of kLvUnit rows (bsr4_sptrsv_level_kernel)
+    int lv_batch = 3;          // level kernel: dependencies held per row (2 or 3)
     int4 *d_info[2] = {nullptr, nullptr};  // row schedule: (row, first, end, 1) per position
     int *d_cx = nullptr;  // row kernel: column of each dependency block, ~slot inside its workgroup;
                           // unit kernel: ~slot of each dependency (unit row or import)
@@ -817,6 +1002,27 @@ static void unit_launch(isai_sptrsv_plan_t P, const double *d_b, double *d_x, hi
 }
 
 template <int B>
+static void level_launch(isai_sptrsv_plan_t P, const double *d_b, double *d_x, hipStream_t st, const int *skip) {
+    const unsigned g = (unsigned)((P->nb + kLvUnit - 1) / kLvUnit);
+    bsr4_sptrsv_level_kernel<false, B><<<g, kLvWG, P->row_lds, st>>>(P->nb, P->d_info[0], P->d_cx,
+                                                                     P->d_imp_ptr[0], P->d_imp_col[0], P->d_S,
+                                                                     P->d_dinvL, d_b, P->d_y, d_x, P->d_cnt, skip);
+    bsr4_sptrsv_level_kernel<true, B><<<g, kLvWG, P->row_lds, st>>>(P->nb, P->d_info[1], P->d_cx, P->d_imp_ptr[1],
+                                                                    P->d_imp_col[1], P->d_S, P->d_dinvU, P->d_y,
+                                                                    d_x, nullptr, P->d_cnt, skip);
+}
+
+template <int B>
+static hipError_t level_attr(int lds) {
+    hipError_t e = hipFuncSetAttribute((const void *)bsr4_sptrsv_level_kernel<false, B>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)bsr4_sptrsv_level_kernel<true, B>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    return e;
+}
+
+template <int B>
 static hipError_t unit_attr(int lds) {
     hipError_t e = hipFuncSetAttribute((const void *)bsr4_sptrsv_unit_kernel<false, B>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -831,6 +1037,14 @@ int sptrsv_solve_internal(isai_sptrsv_plan_t P, const double *d_LU, const double
     (void)d_LU;  // the solve reads the pre-scaled copy made from these values
     const size_t n = (size_t)P->nb * 4;
     fill_sentinel_kernel<<<fill_blocks(n), 256, 0, st>>>(P->d_y, n, skip);
+    if (P->level_kernel) {
+        if (P->lv_batch == 2)
+            level_launch<2>(P, d_b, d_x, st, skip);
+        else
+            level_launch<3>(P, d_b, d_x, st, skip);
+        ISAI_LAUNCH_CHECK();
+        return ISAI_OK;
+    }
     if (P->unit_kernel) {
         if (P->unit_batch == 2)
             unit_launch<2>(P, d_b, d_x, st, skip);
@@ -1023,7 +1237,7 @@ static void level_order(int nb, const std::vector<int> &lev, int nlev, bool uppe
 }
 
 // Unit order of one triangle for the row schedule: the order is cut into units
-// of kRowsPerWG consecutive rows (one workgroup each), and it is grown so that
+// of usz consecutive rows (one workgroup each), and it is grown so that
 // as many dependencies as possible fall inside a unit, where they travel through
 // LDS.  Greedy: a unit starts at the shallowest row whose dependencies are all
 // placed; it then repeatedly takes, among the rows that became ready while it
@@ -1038,7 +1252,7 @@ static void level_order(int nb, const std::vector<int> &lev, int nlev, bool uppe
 // rows it touches at any time sit close together; the order it produces is the
 // same (the keys keep the original level and position).
 static void unit_order(int nb, const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &diag,
-                       bool upper, const std::vector<int> &lev, int nlev, std::vector<int> &order) {
+                       bool upper, const std::vector<int> &lev, int nlev, int usz, std::vector<int> &order) {
     auto rank = [&](int i) { return upper ? nb - 1 - i : i; };
     // level-major numbering: perm[n] = row, inv[row] = n
     std::vector<int> cnt(nlev + 1, 0), perm(nb), inv(nb);
@@ -1080,7 +1294,7 @@ static void unit_order(int nb, const std::vector<int> &rp, const std::vector<int
     order.clear();
     order.reserve(nb);
     for (int u = 0; (int)order.size() < nb; ++u) {
-        for (int size = 0; size < kRowsPerWG && (int)order.size() < nb; ++size) {
+        for (int size = 0; size < usz && (int)order.size() < nb; ++size) {
             int n;
             if (!local.empty()) {
                 n = local.top().second;
@@ -1102,6 +1316,68 @@ static void unit_order(int nb, const std::vector<int> &rp, const std::vector<int
         for (; !local.empty(); local.pop()) pool.push({key(local.top().second), local.top().second});
     }
     for (int &n : order) n = perm[n];
+}
+
+// f(lo, hi) over [0, n) split into contiguous ranges, one host thread each.
+template <class F>
+static void parallel_ranges(int n, F f);
+
+// Level-synchronous units (bsr4_sptrsv_level_kernel): within each unit of kLvUnit
+// consecutive positions of a unit order, the local level of every row (1 + the
+// deepest local level of its dependencies in the unit, 0 without), and the unit's
+// rows re-sorted by (local level, position).  loc[q] = the local level at position
+// q.  Returns false when the kernel cannot take the order: a row with more than
+// kLvMaxDeps dependencies, or a wave that would meet its rows out of level order
+// (its quads hold slots 16 wv + 128 k + 0..15; the lowest level of its rows k must
+// lie above the highest of its rows k - 1).
+constexpr int kLvMaxDeps = 3;
+static bool level_units(int nb, const std::vector<int> &rp, const std::vector<int> &ci,
+                        const std::vector<int> &diag, bool upper, std::vector<int> &order, std::vector<int> &loc) {
+    const int nu = (nb + kLvUnit - 1) / kLvUnit;
+    std::vector<int> at(nb);
+    parallel_ranges(nb, [&](int lo, int hi) {
+        for (int q = lo; q < hi; ++q) at[order[q]] = q;
+    });
+    loc.assign(nb, 0);
+    std::atomic<bool> ok{true};
+    parallel_ranges(nu, [&](int w0, int w1) {
+        std::vector<int> lv(kLvUnit), cnt(kLvUnit + 1), sorted(kLvUnit);
+        for (int w = w0; w < w1 && ok.load(std::memory_order_relaxed); ++w) {
+            const int base = w * kLvUnit, n = std::min(kLvUnit, nb - base);
+            int nl = 0;
+            for (int s = 0; s < n; ++s) {  // the unit order is topological
+                const int i = order[base + s];
+                const int pb = upper ? diag[i] + 1 : rp[i], pe = upper ? rp[i + 1] : diag[i];
+                if (pe - pb > kLvMaxDeps) ok.store(false, std::memory_order_relaxed);
+                int l = 0;
+                for (int p = pb; p < pe; ++p) {
+                    const int qc = at[ci[p]];
+                    if (qc >= base && qc < base + n) l = std::max(l, lv[qc - base] + 1);
+                }
+                lv[s] = l;
+                nl = std::max(nl, l + 1);
+            }
+            std::fill(cnt.begin(), cnt.begin() + nl + 1, 0);
+            for (int s = 0; s < n; ++s) cnt[lv[s] + 1]++;
+            for (int l = 0; l < nl; ++l) cnt[l + 1] += cnt[l];
+            for (int s = 0; s < n; ++s) sorted[cnt[lv[s]]++] = s;
+            // apply the permutation (the rows of this unit only)
+            std::vector<int> rows(n);
+            for (int s = 0; s < n; ++s) rows[s] = order[base + sorted[s]];
+            for (int s = 0; s < n; ++s) {
+                loc[base + s] = lv[sorted[s]];
+                order[base + s] = rows[s];
+            }
+            for (int wv = 0; wv < kLvWaves; ++wv)
+                for (int k = 1; k < kLvRows; ++k) {
+                    const int first = kLvQuads * k + 16 * wv;
+                    if (first >= n) continue;
+                    const int prev_last = std::min(kLvQuads * (k - 1) + 16 * wv + 15, n - 1);
+                    if (loc[base + first] <= loc[base + prev_last]) ok.store(false, std::memory_order_relaxed);
+                }
+        }
+    });
+    return ok.load();
 }
 
 // f(lo, hi) over [0, n) split into contiguous ranges, one host thread each.
@@ -1134,14 +1410,17 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
     }
     // chunk < 0: the row schedule; -1 in unit order, -2 in level order, -3 in
     // unit order with the row kernel (the fallback of -1 when a unit has more than
-    // kUnitImpMax imports)
+    // kUnitImpMax imports), -4 level-synchronous units (falls back to -1 when the
+    // order does not suit bsr4_sptrsv_level_kernel)
+    ISAI_REQUIRE(chunk <= kPipeRows && chunk >= -4, "isai_bsr_sptrsv_analysis: chunk %d outside [-4, %d]", chunk,
+                 kPipeRows);
     bool rows = chunk < 0;
     const bool units = chunk != -2;
     bool unit_kernel = chunk == -1 || chunk == 0;
+    const bool level_kernel = chunk == -4;
+    const int usz = level_kernel ? kLvUnit : kRowsPerWG;  // rows per unit
     const bool automatic = chunk == 0;
     if (rows || automatic) chunk = kPipeRows;
-    ISAI_REQUIRE(chunk <= kPipeRows && chunk >= -3, "isai_bsr_sptrsv_analysis: chunk %d outside [-3, %d]", chunk,
-                 kPipeRows);
     hipStream_t st = (hipStream_t)stream;
     std::vector<int> rp(nb + 1), ci(nnzb), diag(nb, -1);
     ISAI_HIP(hipMemcpyAsync(rp.data(), d_row, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, st));
@@ -1190,12 +1469,12 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         if (rows && units) {
             // A structurally symmetric pattern makes U's dependency graph L's
             // reversed, so the reversed L order is a U order; when nb is a multiple
-            // of kRowsPerWG its units are L's units reversed, with the same
+            // of the unit size its units are L's units reversed, with the same
             // dependencies inside them.  Otherwise U's order is grown on a second
             // host thread.
             long long low = 0, up = 0;
             for (int i = 0; i < nb; ++i) low += diag[i] - rp[i], up += rp[i + 1] - diag[i] - 1;
-            std::atomic<bool> sym{low == up && nb % kRowsPerWG == 0};  // then L = U transposed iff L in U^T
+            std::atomic<bool> sym{low == up && nb % usz == 0};  // then L = U transposed iff L in U^T
             parallel_ranges(nb, [&](int lo, int hi) {
                 for (int i = lo; i < hi && sym.load(std::memory_order_relaxed); ++i)
                     for (int p = rp[i]; p < diag[i]; ++p)
@@ -1205,29 +1484,40 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
                         }
             });
             if (sym.load()) {
-                unit_order(nb, rp, ci, diag, false, lev[0], nl[0], order[0]);
+                unit_order(nb, rp, ci, diag, false, lev[0], nl[0], usz, order[0]);
                 order[1].assign(order[0].rbegin(), order[0].rend());
             } else {
-                std::thread tu([&] { unit_order(nb, rp, ci, diag, true, lev[1], nl[1], order[1]); });
-                unit_order(nb, rp, ci, diag, false, lev[0], nl[0], order[0]);
+                std::thread tu([&] { unit_order(nb, rp, ci, diag, true, lev[1], nl[1], usz, order[1]); });
+                unit_order(nb, rp, ci, diag, false, lev[0], nl[0], usz, order[0]);
                 tu.join();
             }
         } else if (rows) {
             for (int u = 0; u < 2; ++u) level_order(nb, lev[u], nl[u], u == 1, order[u]);
         }
     }
+    // Level-synchronous units: rows of each unit re-sorted by local level (loc);
+    // an order the level kernel cannot take falls back to the unit kernel.
+    std::vector<int> loc[2];
+    if (level_kernel) {
+        bool ok[2];
+        std::thread tu([&] { ok[1] = level_units(nb, rp, ci, diag, true, order[1], loc[1]); });
+        ok[0] = level_units(nb, rp, ci, diag, false, order[0], loc[0]);
+        tu.join();
+        if (!ok[0] || !ok[1]) return isai_bsr_sptrsv_analysis(nb, nnzb, bs, d_row, d_col, d_LU, -1, plan, stream);
+    }
     // Row schedules: the position of every row, and the per-plan column array cx.
-    // A dependency on a row of the same workgroup (64 consecutive positions) is
-    // ~slot, read from LDS.  With the unit kernel a dependency on another unit is
-    // ~(64 + k), k its import slot: the unit's imports are the distinct rows of
-    // other units its rows depend on, in order of first use.  One pass per
-    // triangle over the units, split over host threads (units are independent).
+    // A dependency on a row of the same workgroup (usz consecutive positions: 64,
+    // or kLvUnit for the level kernel) is ~slot, read from LDS.  With the unit and
+    // level kernels a dependency on another unit is ~(usz + k), k its import slot:
+    // the unit's imports are the distinct rows of other units its rows depend on,
+    // in order of first use.  One pass per triangle over the units, split over
+    // host threads (units are independent).
     std::vector<int> imp_ptr[2], imp_col[2], cx;
     int max_imp = 0;
     if (rows) {
         cx = ci;
-        const int nu = (nb + kRowsPerWG - 1) / kRowsPerWG;
-        const bool want_imports = units && unit_kernel;
+        const int nu = (nb + usz - 1) / usz;
+        const bool want_imports = units && (unit_kernel || level_kernel);
         for (int u = 0; u < 2; ++u) {
             std::vector<int> at(nb);
             parallel_ranges(nb, [&](int lo, int hi) {
@@ -1238,7 +1528,7 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
             if (want_imports) {
                 for (int w = 0; w < nu; ++w) {
                     int d = 0;
-                    for (int q = w * kRowsPerWG; q < std::min(nb, (w + 1) * kRowsPerWG); ++q) {
+                    for (int q = w * usz; q < std::min(nb, (w + 1) * usz); ++q) {
                         const int i = order[u][q];
                         d += u == 0 ? diag[i] - rp[i] : rp[i + 1] - diag[i] - 1;
                     }
@@ -1254,20 +1544,20 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
                 }
                 for (int w = w0; w < w1; ++w) {
                     int k = 0;
-                    for (int q = w * kRowsPerWG; q < std::min(nb, (w + 1) * kRowsPerWG); ++q) {
+                    for (int q = w * usz; q < std::min(nb, (w + 1) * usz); ++q) {
                         const int i = order[u][q];
                         const int pb = u == 0 ? rp[i] : diag[i] + 1, pe = u == 0 ? diag[i] : rp[i + 1];
                         for (int p = pb; p < pe; ++p) {
                             const int c = ci[p];
-                            if (at[c] / kRowsPerWG == w) {
-                                cx[p] = ~(at[c] % kRowsPerWG);
+                            if (at[c] / usz == w) {
+                                cx[p] = ~(at[c] % usz);
                             } else if (want_imports) {
                                 if (stamp[c] != w) {
                                     stamp[c] = w;
                                     slot[c] = k;
                                     buf[off[w] + k++] = c;
                                 }
-                                cx[p] = ~(kRowsPerWG + slot[c]);
+                                cx[p] = ~(usz + slot[c]);
                             }
                         }
                     }
@@ -1287,6 +1577,8 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
                 });
             }
         }
+        if (level_kernel && max_imp > kUnitImpMax)
+            return isai_bsr_sptrsv_analysis(nb, nnzb, bs, d_row, d_col, d_LU, -1, plan, stream);
         if (want_imports && max_imp > kUnitImpMax) {
             // too many imports for the unit kernel's LDS: the row kernel reads
             // dependencies on other units from the solution vector (global columns)
@@ -1298,12 +1590,18 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         }
     }
     unit_kernel = unit_kernel && rows && units;
-    const int row_lds = unit_kernel ? std::max(kUnitLds, (kRowsPerWG + max_imp) * 32) : units ? kUnitRowLds : kRowLds;
+    const int row_lds = level_kernel  ? (kLvUnit + max_imp) * 32
+                        : unit_kernel ? std::max(kUnitLds, (kRowsPerWG + max_imp) * 32)
+                        : units       ? kUnitRowLds
+                                      : kRowLds;
     // the smallest batch that holds the longest row of either triangle
     int max_deps = 0;
     for (int i = 0; i < nb; ++i) max_deps = std::max({max_deps, diag[i] - rp[i], rp[i + 1] - diag[i] - 1});
     const int unit_batch = max_deps <= 2 ? 2 : max_deps <= 4 ? 4 : 8;
-    if (unit_kernel) {
+    const int lv_batch = max_deps <= 2 ? 2 : 3;  // level kernel: at most kLvMaxDeps, checked above
+    if (level_kernel) {
+        ISAI_HIP(lv_batch == 2 ? level_attr<2>(row_lds) : level_attr<3>(row_lds));
+    } else if (unit_kernel) {
         ISAI_HIP(unit_batch == 2 ? unit_attr<2>(row_lds) : unit_batch == 4 ? unit_attr<4>(row_lds) : unit_attr<8>(row_lds));
     } else if (rows) {
         ISAI_HIP(hipFuncSetAttribute((const void *)bsr4_sptrsv_row_kernel<false>,
@@ -1320,6 +1618,8 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
     P->units = rows && units;
     P->unit_kernel = unit_kernel;
     P->unit_batch = unit_batch;
+    P->level_kernel = level_kernel;
+    P->lv_batch = lv_batch;
     P->row_lds = row_lds;
     P->nb = nb;
     P->nnzb = nnzb;
@@ -1344,7 +1644,7 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
     }
     for (int u = 0; u < 2 && ok && rows; ++u) ok = hipMalloc(&P->d_info[u], sizeof(int4) * nb) == hipSuccess;
     if (ok && rows) ok = hipMalloc(&P->d_cx, sizeof(int) * (size_t)nnzb) == hipSuccess;
-    for (int u = 0; u < 2 && ok && unit_kernel; ++u)
+    for (int u = 0; u < 2 && ok && (unit_kernel || level_kernel); ++u)
         ok = hipMalloc(&P->d_imp_ptr[u], sizeof(int) * imp_ptr[u].size()) == hipSuccess &&
              hipMalloc(&P->d_imp_col[u], sizeof(int) * std::max<size_t>(imp_col[u].size(), 1)) == hipSuccess;
     for (int u = 0; u < 2 && ok && !rows; ++u) {
@@ -1369,11 +1669,12 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
                 parallel_ranges(nb, [&](int lo, int hi) {
                     for (int q = lo; q < hi; ++q) {
                         const int i = order[u][q];
-                        info[q] = u == 0 ? make_int4(i, rp[i], diag[i], 1) : make_int4(i, diag[i] + 1, rp[i + 1], 1);
+                        const int w4 = level_kernel ? loc[u][q] : 1;  // local level (level kernel)
+                        info[q] = u == 0 ? make_int4(i, rp[i], diag[i], w4) : make_int4(i, diag[i] + 1, rp[i + 1], w4);
                     }
                 });
                 ISAI_HIP(hipMemcpy(P->d_info[u], info.data(), sizeof(int4) * nb, hipMemcpyHostToDevice));
-                if (unit_kernel) {
+                if (unit_kernel || level_kernel) {
                     ISAI_HIP(hipMemcpy(P->d_imp_ptr[u], imp_ptr[u].data(), sizeof(int) * imp_ptr[u].size(),
                                        hipMemcpyHostToDevice));
                     if (!imp_col[u].empty())
@@ -1454,6 +1755,12 @@ int isai_sptrsv_status(isai_sptrsv_plan_t P, isai_stream_t stream) {
         set_error("sptrsv: a dependency wait exceeded its spin limit (malformed pattern or missing diagonal)");
         return ISAI_ERR_TIMEOUT;
     }
+    return ISAI_OK;
+}
+
+int isai_bsr_sptrsv_schedule(isai_sptrsv_plan_t P, int *schedule) {
+    ISAI_REQUIRE(P && schedule, "isai_bsr_sptrsv_schedule: null argument");
+    *schedule = P->level_kernel ? -4 : P->unit_kernel ? -1 : P->rows ? (P->units ? -3 : -2) : P->chunk;
     return ISAI_OK;
 }
 

@@ -21,7 +21,7 @@ def _plan(rp, ci, d_rp, d_ci, d_lu, chunk):
 
 
 @pytest.mark.parametrize("problem,L", [(0, 16), (1, 8), (2, 5), (3, 12)])
-@pytest.mark.parametrize("chunk", [1, 5, 64, 128, -1, -2, -3])
+@pytest.mark.parametrize("chunk", [1, 5, 64, 128, -1, -2, -3, -4])
 def test_bsr_sptrsv_matches_oracle(gpu, problem, L, chunk):
     print(f"bsr sptrsv problem={problem} L={L} chunk={chunk}", flush=True)
     rp, ci, val = problems.generate(problem, L, seed=2)
@@ -43,6 +43,40 @@ def test_bsr_sptrsv_matches_oracle(gpu, problem, L, chunk):
         call("isai_bsr_sptrsv_solve", plan, d_lu.ptr, d_b.ptr, d_b.ptr, None)
         call("isai_sptrsv_status", plan, None)
         assert np.array_equal(d_b.to_numpy(), x)
+    finally:
+        call("isai_bsr_sptrsv_destroy", plan)
+
+
+def _schedule(plan):
+    s = ctypes.c_int()
+    call("isai_bsr_sptrsv_schedule", plan, ctypes.byref(s))
+    return s.value
+
+
+@pytest.mark.parametrize("problem,L,schedule", [(1, 16, -4), (1, 24, -4), (1, 20, -4), (1, 9, -4), (0, 64, -4),
+                                                (0, 45, -4), (2, 6, -1), (3, 20, -1)])
+def test_bsr_sptrsv_level_units(gpu, problem, L, schedule):
+    """Level-synchronous units (chunk = -4) against the oracle: 7-point grids of
+    16^3 and 24^3 (8 and 27 units of 8x8x8 rows), 20^3 and 9^3 (8,000 and 729 rows:
+    a partial last unit, and U's order grown by its own walk since the row count is
+    no multiple of 512), 2-D 5-point grids of 64^2 and 45^2 (two dependencies per
+    row); the 27-point and 9-point stencils (more than three dependencies per row)
+    fall back to the unit kernel.  Three solves in a row (ticket reset) and the
+    schedule the plan reports."""
+    print(f"bsr sptrsv level units problem={problem} L={L}", flush=True)
+    rp, ci, val = problems.generate(problem, L, seed=5)
+    d_rp, d_ci, d_lu = gpu_ilu(rp, ci, val)
+    n = (len(rp) - 1) * 4
+    b = RNG.standard_normal(n)
+    plan = _plan(rp, ci, d_rp, d_ci, d_lu, -4)
+    try:
+        assert _schedule(plan) == schedule
+        d_b, d_x = dev(b), DeviceArray((n,), np.float64)
+        ref = O.sptrsv_lu(rp, ci, d_lu.to_numpy(), b)
+        for rep in range(3):
+            call("isai_bsr_sptrsv_solve", plan, d_lu.ptr, d_b.ptr, d_x.ptr, None)
+            call("isai_sptrsv_status", plan, None)
+            assert rel_err(d_x.to_numpy(), ref) <= TOL, rep
     finally:
         call("isai_bsr_sptrsv_destroy", plan)
 
@@ -80,7 +114,7 @@ def test_csr_sptrsv_device_form(gpu, chunk):
     assert rel_err(d_x.to_numpy(), ref) <= TOL
 
 
-@pytest.mark.parametrize("chunk", [128, -1, -2, -3])
+@pytest.mark.parametrize("chunk", [128, -1, -2, -3, -4])
 def test_bsr_sptrsv_full_size_residual(gpu, chunk):
     """96^3 7-point: || L (U x) - b || / || b || with the GPU factor and solve, for
     the chunk schedule (128), the unit kernel (-1) and the row kernel in level (-2)
@@ -103,7 +137,7 @@ def test_bsr_sptrsv_full_size_residual(gpu, chunk):
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-13
 
 
-@pytest.mark.parametrize("chunk", [128, -1, -2])
+@pytest.mark.parametrize("chunk", [128, -1, -2, -4])
 def test_spin_limit_timeout_is_reported_and_cleared(gpu, chunk):
     """A wait budget of one poll makes dependent chunks (or rows) give up (bounded
     waits: every kernel still exits); the plan reports ISAI_ERR_TIMEOUT once, then
