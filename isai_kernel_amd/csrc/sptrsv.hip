@@ -538,24 +538,25 @@ __device__ __forceinline__ bool import_wave(int l, int nit, const int *__restric
     }
     bool timed_out = false;
     int stall = 0;
+    constexpr int kG = 8;  // items polled before any is compared
     while (__any(pend != 0)) {
         bool got = false;
 #pragma unroll
-        for (int g = 0; g < kUnitItemsPerLane / 8; ++g) {
-            const unsigned pm = (pend >> (8 * g)) & 0xffu;
+        for (int g = 0; g < kUnitItemsPerLane / kG; ++g) {
+            const unsigned pm = (pend >> (kG * g)) & ((1u << kG) - 1u);
             if (!__any(pm != 0)) continue;
-            unsigned long long v[8];
+            unsigned long long v[kG];
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
+            for (int j = 0; j < kG; ++j)
                 if (pm & (1u << j))
-                    v[j] = __hip_atomic_load((unsigned long long *)(out + at[8 * g + j]), __ATOMIC_RELAXED,
+                    v[j] = __hip_atomic_load((unsigned long long *)(out + at[kG * g + j]), __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
+            for (int j = 0; j < kG; ++j)
                 if ((pm & (1u << j)) && v[j] != kSentinel) {
-                    __hip_atomic_store(&slot[l + 64 * (8 * g + j)], v[j], __ATOMIC_RELAXED,
+                    __hip_atomic_store(&slot[l + 64 * (kG * g + j)], v[j], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                    pend &= ~(1u << (8 * g + j));
+                    pend &= ~(1u << (kG * g + j));
                     got = true;
                 }
         }
@@ -699,16 +700,17 @@ __global__ __launch_bounds__(kUnitWG) void bsr4_sptrsv_unit_kernel(
 // computation, where the unit kernel's per-row LDS polling cost about 0.8 us per
 // hop, and a dependency path of the 96^3 factor crosses 33 units instead of 69.
 //
-// Compute quad q (of 128) owns the rows at slots q + 128 k, k < 4, and loads all of
-// their inputs (block row m of each S_ik, inv(D_i) rhs_i) into registers at the
-// unit's start, so no global load lies on a level's path.  A wave's quads hold 16
-// consecutive slots of each k; the analysis checks that every wave meets its rows
-// in increasing level order (the highest level of its rows k - 1 below the lowest
-// of its rows k) and that no row has more than B dependencies, else the plan uses
-// the unit kernel.  The level barrier of the compute waves is an LDS counter (the
-// import wave keeps polling; s_barrier would tie it to the levels), preceded by a
-// wait for the wave's LDS stores only: the global stores that publish values are
-// not waited for.
+// Compute quad q (of 128) owns the rows at slots q + 128 k, k < 4, and loads their
+// inputs (block row m of each S_ik) into registers at the unit's start, and the
+// unit's values inv(D_i) rhs_i are formed in LDS then, so no global load lies on a
+// level's path.  A wave's quads hold 16 consecutive slots of each k; the analysis
+// checks that every wave meets its rows in increasing level order (the highest
+// level of its rows k - 1 below the lowest of its rows k), so one level loop per k
+// holds one row's code, and that no row has more than B (at most 3) dependencies,
+// else the plan uses the unit kernel.  The level barrier of the
+// compute waves is an LDS counter (the import wave keeps polling; s_barrier would
+// tie it to the levels), preceded by a wait for the wave's LDS stores only: the
+// global stores that publish values are not waited for.
 constexpr int kLvUnit = 512;
 constexpr int kLvQuads = 128;
 constexpr int kLvRows = kLvUnit / kLvQuads;  // rows per quad
@@ -720,11 +722,14 @@ template <bool UPPER, int B>
 __global__ __launch_bounds__(kLvWG) void bsr4_sptrsv_level_kernel(
     int nb, const int4 *__restrict__ info, const int *__restrict__ cxu, const int *__restrict__ imp_ptr,
     const int *__restrict__ imp_col, const double *__restrict__ S, const double *__restrict__ dinv,
-    const double *rhs, double *out, double *sentinel_fill, Counters *cnt, const int *__restrict__ skip) {
+    const double *rhs, double *out, double *sentinel_fill, Counters *cnt, const int *__restrict__ skip,
+    int imp_cap) {
     if (skip && *skip) return;
     __shared__ int s_t;
     __shared__ unsigned int s_bar;
-    extern __shared__ unsigned long long lv_x[];  // (kLvUnit + imports) x 4 values
+    // (kLvUnit + imp_cap) x 4 values: the unit's rows, then its imports; then the
+    // unit's kLvUnit x 4 values of c = inv(D_i) rhs_i
+    extern __shared__ unsigned long long lv_x[];
     unsigned int *ticket = UPPER ? &cnt->ticket_u : &cnt->ticket_l;
     unsigned int *donec = UPPER ? &cnt->done_u : &cnt->done_l;
     const int limit = cnt->spin_limit;
@@ -736,6 +741,8 @@ __global__ __launch_bounds__(kLvWG) void bsr4_sptrsv_level_kernel(
     const int w = s_t;
     const int ib = imp_ptr[w], nit = (imp_ptr[w + 1] - ib) * 4;
     for (int t = threadIdx.x; t < nit; t += kLvWG) lv_x[kLvUnit * 4 + t] = kSentinel;
+    const int zero_slot = (2 * kLvUnit + imp_cap) * 4;  // four zero words after the c values
+    if (threadIdx.x < 4) lv_x[zero_slot + threadIdx.x] = 0ull;
     __syncthreads();
     bool timed_out = false;
     if (threadIdx.x >= kLvCompute) {
@@ -744,54 +751,91 @@ __global__ __launch_bounds__(kLvWG) void bsr4_sptrsv_level_kernel(
         const int quad = threadIdx.x >> 2, m = threadIdx.x & 3, wv = threadIdx.x >> 6;
         const int base = w * kLvUnit;
         const int nrows = min(kLvUnit, nb - base);
-        // this quad's rows: level (INT_MAX: none), row, dependency count, LDS word of
-        // each dependency's component m, row m of each S_ik, and c = (inv(D_i) rhs_i)[m]
-        int lvl[kLvRows], irow[kLvRows], nd[kLvRows], hi[kLvRows];
-        int sl[kLvRows][B];
-        double c[kLvRows], s[kLvRows][B][4];
+        // The unit's inputs are loaded in two rounds of unconditional loads (clamped
+        // positions and blocks; a missing row reads valid memory and is ignored):
+        // first the position records, then every factor block, inverse diagonal
+        // block and right-hand side they point at.  Loads that wait on each other
+        // row by row took 14.6 us per unit, most of its time in the workgroup slot.
+        // Compute thread t forms c = inv(D_i) rhs_i of the unit's row at slot t in LDS;
+        // quad q keeps row m of each S_ik of its rows at slots q + 128 k, level
+        // (INT_MAX: none), row and the LDS word of each dependency's component m.
+        double *cbuf = (double *)(lv_x + (kLvUnit + imp_cap) * 4);
+        // unused inputs of a row read four zeros at zero_slot, with zero blocks in s,
+        // and subtract an exact zero
         const int nl = info[base + nrows - 1].w + 1;
+        const int4 ft = info[base + min((int)threadIdx.x, nrows - 1)];
+        int4 f[kLvRows];
+        int lvl[kLvRows], irow[kLvRows], nd[kLvRows], hi[kLvRows];
 #pragma unroll
         for (int k = 0; k < kLvRows; ++k) {
-            const int slot = quad + kLvQuads * k;
+            f[k] = info[base + min(quad + kLvQuads * k, nrows - 1)];
             // highest level among the wave's rows k (its slots are consecutive and
             // sorted by level), -1 when the wave has none
             const int first = kLvQuads * k + 16 * wv;
-            hi[k] = first < nrows ? info[base + min(first + 15, nrows - 1)].w : -1;
-            lvl[k] = INT_MAX;
-            irow[k] = 0;
-            nd[k] = 0;
-            c[k] = 0.0;
+            hi[k] = __builtin_amdgcn_readfirstlane(first < nrows ? info[base + min(first + 15, nrows - 1)].w : -1);
+        }
+        int sl[kLvRows][B];  // LDS word of each input's component 0 (packed in pairs below)
+        double s[kLvRows][B][4];
+#pragma unroll
+        for (int k = 0; k < kLvRows; ++k) {
+            const bool has = quad + kLvQuads * k < nrows;
+            lvl[k] = has ? f[k].w : INT_MAX;
+            irow[k] = f[k].x;
+            nd[k] = has ? f[k].z - f[k].y : 0;
 #pragma unroll
             for (int j = 0; j < B; ++j) {
-                sl[k][j] = m;
-                s[k][j][0] = s[k][j][1] = s[k][j][2] = s[k][j][3] = 0.0;
+                const int pj = j < nd[k] ? f[k].y + j : 0;
+                sl[k][j] = cxu[pj];
+                const double *Bk = S + (size_t)pj * 16 + m * 4;
+                s[k][j][0] = Bk[0], s[k][j][1] = Bk[1], s[k][j][2] = Bk[2], s[k][j][3] = Bk[3];
             }
-            if (slot < nrows) {
-                const int4 f = info[base + slot];
-                irow[k] = f.x;
-                nd[k] = f.z - f.y;
-                lvl[k] = f.w;
-                const double *D = dinv + (size_t)f.x * 16 + m * 4;
-                const double *b = rhs + (size_t)f.x * 4;
-                c[k] = D[0] * b[0] + D[1] * b[1] + D[2] * b[2] + D[3] * b[3];
+            // an unused input gets a zero block (whatever block 0 of S holds)
 #pragma unroll
-                for (int j = 0; j < B; ++j)
-                    if (j < nd[k]) {
-                        sl[k][j] = (~cxu[f.y + j]) * 4 + m;
-                        const double *Bk = S + (size_t)(f.y + j) * 16 + m * 4;
-                        s[k][j][0] = Bk[0], s[k][j][1] = Bk[1], s[k][j][2] = Bk[2], s[k][j][3] = Bk[3];
-                    }
-            }
+            for (int j = 0; j < B; ++j)
+                if (j >= nd[k]) s[k][j][0] = s[k][j][1] = s[k][j][2] = s[k][j][3] = 0.0;
         }
+        {
+            const double *D = dinv + (size_t)ft.x * 16;
+            const double *b = rhs + (size_t)ft.x * 4;
+            double Dv[16], bv[4];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) Dv[e] = D[e];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) bv[e] = b[e];
+            // every slot is written (slots past the unit's rows are never read), so
+            // the wait for these loads lies on every path into the level loop
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                cbuf[threadIdx.x * 4 + r] =
+                    Dv[r * 4] * bv[0] + Dv[r * 4 + 1] * bv[1] + Dv[r * 4 + 2] * bv[2] + Dv[r * 4 + 3] * bv[3];
+        }
+#pragma unroll
+        for (int k = 0; k < kLvRows; ++k)
+#pragma unroll
+            for (int j = 0; j < B; ++j) sl[k][j] = j < nd[k] ? (~sl[k][j]) * 4 : zero_slot;  // component 0
+        unsigned slp[kLvRows][(B + 1) / 2];
+#pragma unroll
+        for (int k = 0; k < kLvRows; ++k)
+#pragma unroll
+            for (int j = 0; j < B; j += 2)
+                slp[k][j / 2] = (unsigned)sl[k][j] | (j + 1 < B ? (unsigned)sl[k][j + 1] << 16 : 0u);
+        // every load of the set-up is complete here: the compiler's waits for them
+        // would otherwise fall inside the level loop, where each also waits for the
+        // stores that publish the previous levels
+        __builtin_amdgcn_s_waitcnt(0);
         // one row of this quad: read its inputs (own-unit inputs are final after the
         // previous level's barrier; imports are polled in LDS), compute, publish
         auto row = [&](int k) {
+            // component m of each input (unused inputs read a zero), then the quad
+            // exchanges components by DPP; an import still in flight reads the
+            // sentinel and is read again
             unsigned long long u[B];
             unsigned miss = 0;
 #pragma unroll
             for (int j = 0; j < B; ++j) {
-                u[j] = __hip_atomic_load(&lv_x[sl[k][j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (j < nd[k] && u[j] == kSentinel) miss |= 1u << j;
+                u[j] = __hip_atomic_load(&lv_x[((slp[k][j / 2] >> (16 * (j & 1))) & 0xffffu) + m], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (u[j] == kSentinel) miss |= 1u << j;
             }
             int stall = 0;
             long long t_stall = 0;
@@ -799,7 +843,8 @@ __global__ __launch_bounds__(kLvWG) void bsr4_sptrsv_level_kernel(
 #pragma unroll
                 for (int j = 0; j < B; ++j)
                     if (miss & (1u << j)) {
-                        u[j] = __hip_atomic_load(&lv_x[sl[k][j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        u[j] = __hip_atomic_load(&lv_x[((slp[k][j / 2] >> (16 * (j & 1))) & 0xffffu) + m],
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         if (u[j] != kSentinel) miss &= ~(1u << j);
                     }
                 // bounded by the clock (100 MHz): spin_limit microseconds
@@ -809,13 +854,13 @@ __global__ __launch_bounds__(kLvWG) void bsr4_sptrsv_level_kernel(
                     miss = 0;
                 }
             }
-            double z = c[k];
+            double z = cbuf[(quad + kLvQuads * k) * 4 + m];
 #pragma unroll
             for (int j = 0; j < B; ++j) {
                 const double xm = __longlong_as_double((long long)u[j]);
                 const double x0 = quad_bcast<0>(xm), x1 = quad_bcast<1>(xm);
                 const double x2 = quad_bcast<2>(xm), x3 = quad_bcast<3>(xm);
-                if (j < nd[k]) z -= s[k][j][0] * x0 + s[k][j][1] * x1 + s[k][j][2] * x2 + s[k][j][3] * x3;
+                z -= s[k][j][0] * x0 + s[k][j][1] * x1 + s[k][j][2] * x2 + s[k][j][3] * x3;
             }
             const int slot = quad + kLvQuads * k;
             __hip_atomic_store(&lv_x[slot * 4 + m], (unsigned long long)__double_as_longlong(z), __ATOMIC_RELAXED,
@@ -828,7 +873,7 @@ __global__ __launch_bounds__(kLvWG) void bsr4_sptrsv_level_kernel(
         auto level_barrier = [&](int l) {
             __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
             if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(&s_bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            const unsigned target = (unsigned)kLvWaves * (unsigned)(l + 1);
+            const unsigned target = (unsigned)kLvWaves * (unsigned)(l + 2);  // round 0: the set-up
             int stall = 0;
             long long t0 = 0;
             while (__hip_atomic_load(&s_bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target) {
@@ -840,6 +885,9 @@ __global__ __launch_bounds__(kLvWG) void bsr4_sptrsv_level_kernel(
             }
             __asm__ volatile("" ::: "memory");
         };
+        level_barrier(-1);  // the c values of every row are in LDS
+        // the wave meets its rows k = 0, 1, 2, 3 in increasing level order (checked
+        // by the analysis), so one level loop per k holds one row's code
         int l = 0;
 #pragma unroll
         for (int k = 0; k < kLvRows; ++k) {
@@ -1004,12 +1052,14 @@ static void unit_launch(isai_sptrsv_plan_t P, const double *d_b, double *d_x, hi
 template <int B>
 static void level_launch(isai_sptrsv_plan_t P, const double *d_b, double *d_x, hipStream_t st, const int *skip) {
     const unsigned g = (unsigned)((P->nb + kLvUnit - 1) / kLvUnit);
+    const int cap = P->row_lds / 32 - 2 * kLvUnit - 1;  // import slots of the LDS layout
     bsr4_sptrsv_level_kernel<false, B><<<g, kLvWG, P->row_lds, st>>>(P->nb, P->d_info[0], P->d_cx,
                                                                      P->d_imp_ptr[0], P->d_imp_col[0], P->d_S,
-                                                                     P->d_dinvL, d_b, P->d_y, d_x, P->d_cnt, skip);
+                                                                     P->d_dinvL, d_b, P->d_y, d_x, P->d_cnt, skip,
+                                                                     cap);
     bsr4_sptrsv_level_kernel<true, B><<<g, kLvWG, P->row_lds, st>>>(P->nb, P->d_info[1], P->d_cx, P->d_imp_ptr[1],
                                                                     P->d_imp_col[1], P->d_S, P->d_dinvU, P->d_y,
-                                                                    d_x, nullptr, P->d_cnt, skip);
+                                                                    d_x, nullptr, P->d_cnt, skip, cap);
 }
 
 template <int B>
@@ -1590,7 +1640,7 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         }
     }
     unit_kernel = unit_kernel && rows && units;
-    const int row_lds = level_kernel  ? (kLvUnit + max_imp) * 32
+    const int row_lds = level_kernel  ? (2 * kLvUnit + max_imp + 1) * 32
                         : unit_kernel ? std::max(kUnitLds, (kRowsPerWG + max_imp) * 32)
                         : units       ? kUnitRowLds
                                       : kRowLds;
