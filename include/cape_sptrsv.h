@@ -55,8 +55,9 @@ int isai_csr_sptrsv_lu_device(int m, int nnz, const int *d_rowptr, const int *d_
  * import wave (what -1 falls back to when a unit imports more than 512 rows);
  * -4: level-synchronous units (falls back to -1 when a row has more than 3
  * dependencies, a unit imports more than 512 rows or holds levels too wide for it);
- * chunk = 0: -1 when the levels of the triangles average at least 16 block rows,
- * else the chunk schedule with 128.  The analysis builds the tables from the
+ * chunk = 0: when the levels of the triangles average at least 16 block rows, -4 if
+ * its units' local levels average at least 16 rows too (3-D stencils), else -1;
+ * otherwise the chunk schedule with 128.  The analysis builds the tables from the
  * pattern and pre-scales the off-diagonal blocks by the inverted diagonal parts of
  * the packed factor; isai_bsr_sptrsv_update_values redoes the scaling for new
  * values on the same pattern. */

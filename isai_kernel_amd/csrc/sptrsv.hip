@@ -1449,8 +1449,15 @@ static void parallel_ranges(int n, F f) {
 
 extern "C" {
 
-int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const int *d_col, const double *d_LU,
-                             int chunk, isai_sptrsv_plan_t *plan, isai_stream_t stream) {
+// auto_level: the level kernel was picked by chunk = 0, and is kept only when the
+// units' levels are wide enough (kLvMinWidth rows on average): on a 2-D 5-point
+// grid a 512-row unit spans about 45 levels of about 11 rows, and the level kernel
+// lost to the unit kernel (256^2: 803 against 529 us; 1024^2: 4.49 against
+// 2.09 ms), while on 3-D 7-point grids (22 levels of 23 rows) it won at every
+// size (32^3: 130 against 174 us; 96^3: 507 against 827 us).
+constexpr int kLvMinWidth = 16;
+static int analysis_impl(int nb, int nnzb, int bs, const int *d_row, const int *d_col, const double *d_LU,
+                         int chunk, bool auto_level, isai_sptrsv_plan_t *plan, isai_stream_t stream) {
     ISAI_REQUIRE(plan, "isai_bsr_sptrsv_analysis: null output");
     *plan = nullptr;
     ISAI_REQUIRE(nb > 0 && nnzb >= nb && d_row && d_col && d_LU, "isai_bsr_sptrsv_analysis: bad argument");
@@ -1516,6 +1523,7 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         }
         nlev = std::max({nlev, nl[0], nl[1]});
         if (automatic) rows = nb / nlev >= kRowMinWidth;
+        if (automatic && rows) return analysis_impl(nb, nnzb, bs, d_row, d_col, d_LU, -4, true, plan, stream);
         if (rows && units) {
             // A structurally symmetric pattern makes U's dependency graph L's
             // reversed, so the reversed L order is a U order; when nb is a multiple
@@ -1553,7 +1561,16 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
         std::thread tu([&] { ok[1] = level_units(nb, rp, ci, diag, true, order[1], loc[1]); });
         ok[0] = level_units(nb, rp, ci, diag, false, order[0], loc[0]);
         tu.join();
-        if (!ok[0] || !ok[1]) return isai_bsr_sptrsv_analysis(nb, nnzb, bs, d_row, d_col, d_LU, -1, plan, stream);
+        if (!ok[0] || !ok[1]) return analysis_impl(nb, nnzb, bs, d_row, d_col, d_LU, -1, false, plan, stream);
+        if (auto_level) {
+            // rows per local level, averaged over the units of each triangle
+            for (int u = 0; u < 2; ++u) {
+                long long levels = 0;
+                for (int w = 0; w * kLvUnit < nb; ++w) levels += loc[u][std::min(nb, (w + 1) * kLvUnit) - 1] + 1;
+                if (nb < (long long)kLvMinWidth * levels)
+                    return analysis_impl(nb, nnzb, bs, d_row, d_col, d_LU, -1, false, plan, stream);
+            }
+        }
     }
     // Row schedules: the position of every row, and the per-plan column array cx.
     // A dependency on a row of the same workgroup (usz consecutive positions: 64,
@@ -1628,7 +1645,7 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
             }
         }
         if (level_kernel && max_imp > kUnitImpMax)
-            return isai_bsr_sptrsv_analysis(nb, nnzb, bs, d_row, d_col, d_LU, -1, plan, stream);
+            return analysis_impl(nb, nnzb, bs, d_row, d_col, d_LU, -1, false, plan, stream);
         if (want_imports && max_imp > kUnitImpMax) {
             // too many imports for the unit kernel's LDS: the row kernel reads
             // dependencies on other units from the solution vector (global columns)
@@ -1768,6 +1785,11 @@ int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const i
     }
     *plan = P;
     return ISAI_OK;
+}
+
+int isai_bsr_sptrsv_analysis(int nb, int nnzb, int bs, const int *d_row, const int *d_col, const double *d_LU,
+                             int chunk, isai_sptrsv_plan_t *plan, isai_stream_t stream) {
+    return analysis_impl(nb, nnzb, bs, d_row, d_col, d_LU, chunk, false, plan, stream);
 }
 
 int isai_bsr_sptrsv_update_values(isai_sptrsv_plan_t P, const double *d_LU, isai_stream_t stream) {

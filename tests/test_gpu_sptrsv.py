@@ -81,6 +81,28 @@ def test_bsr_sptrsv_level_units(gpu, problem, L, schedule):
         call("isai_bsr_sptrsv_destroy", plan)
 
 
+@pytest.mark.parametrize("problem,L,schedule", [(1, 16, -4), (0, 64, -1), (2, 16, -1)])
+def test_bsr_sptrsv_automatic_schedule(gpu, problem, L, schedule):
+    """chunk = 0 picks the level-synchronous units on a 3-D 7-point grid (23 rows per
+    local level), the unit kernel on a 2-D 5-point grid (512-row units of about 10
+    rows per level) and on the 27-point stencil (13 dependencies per row); every
+    choice solves to the oracle."""
+    print(f"bsr sptrsv automatic schedule problem={problem} L={L}", flush=True)
+    rp, ci, val = problems.generate(problem, L, seed=8)
+    d_rp, d_ci, d_lu = gpu_ilu(rp, ci, val)
+    n = (len(rp) - 1) * 4
+    b = RNG.standard_normal(n)
+    plan = _plan(rp, ci, d_rp, d_ci, d_lu, 0)
+    try:
+        assert _schedule(plan) == schedule
+        d_b, d_x = dev(b), DeviceArray((n,), np.float64)
+        call("isai_bsr_sptrsv_solve", plan, d_lu.ptr, d_b.ptr, d_x.ptr, None)
+        call("isai_sptrsv_status", plan, None)
+        assert rel_err(d_x.to_numpy(), O.sptrsv_lu(rp, ci, d_lu.to_numpy(), b)) <= TOL
+    finally:
+        call("isai_bsr_sptrsv_destroy", plan)
+
+
 def test_csr_sptrsv_host_form_matches_oracle(gpu):
     print("csr sptrsv host form", flush=True)
     rp, ci, val = problems.generate(0, 12, seed=3)
