@@ -1370,7 +1370,7 @@ static void unit_order(int nb, const std::vector<int> &rp, const std::vector<int
 
 // f(lo, hi) over [0, n) split into contiguous ranges, one host thread each.
 template <class F>
-static void parallel_ranges(int n, F f);
+static void parallel_ranges(int n, F f, int grain = 4096);
 
 // Level-synchronous units (bsr4_sptrsv_level_kernel): within each unit of kLvUnit
 // consecutive positions of a unit order, the local level of every row (1 + the
@@ -1426,15 +1426,16 @@ static bool level_units(int nb, const std::vector<int> &rp, const std::vector<in
                     if (loc[base + first] <= loc[base + prev_last]) ok.store(false, std::memory_order_relaxed);
                 }
         }
-    });
+    }, 8);
     return ok.load();
 }
 
-// f(lo, hi) over [0, n) split into contiguous ranges, one host thread each.
+// f(lo, hi) over [0, n) split into contiguous ranges, one host thread each, with at
+// least `grain` items per thread (rows: 4096; units, each hundreds of rows: fewer).
 template <class F>
-static void parallel_ranges(int n, F f) {
+static void parallel_ranges(int n, F f, int grain) {
     const int hw = (int)std::thread::hardware_concurrency();
-    const int T = std::max(1, std::min({16, hw > 0 ? hw : 1, (n + 4095) / 4096}));
+    const int T = std::max(1, std::min({16, hw > 0 ? hw : 1, (n + grain - 1) / grain}));
     if (T == 1) {
         f(0, n);
         return;
@@ -1474,8 +1475,8 @@ static int analysis_impl(int nb, int nnzb, int bs, const int *d_row, const int *
     bool rows = chunk < 0;
     const bool units = chunk != -2;
     bool unit_kernel = chunk == -1 || chunk == 0;
-    const bool level_kernel = chunk == -4;
-    const int usz = level_kernel ? kLvUnit : kRowsPerWG;  // rows per unit
+    bool level_kernel = chunk == -4;
+    int usz = level_kernel ? kLvUnit : kRowsPerWG;  // rows per unit
     const bool automatic = chunk == 0;
     if (rows || automatic) chunk = kPipeRows;
     hipStream_t st = (hipStream_t)stream;
@@ -1523,7 +1524,11 @@ static int analysis_impl(int nb, int nnzb, int bs, const int *d_row, const int *
         }
         nlev = std::max({nlev, nl[0], nl[1]});
         if (automatic) rows = nb / nlev >= kRowMinWidth;
-        if (automatic && rows) return analysis_impl(nb, nnzb, bs, d_row, d_col, d_LU, -4, true, plan, stream);
+        if (automatic && rows) {  // the level kernel, kept if its units' levels are wide enough
+            level_kernel = auto_level = true;
+            unit_kernel = false;
+            usz = kLvUnit;
+        }
         if (rows && units) {
             // A structurally symmetric pattern makes U's dependency graph L's
             // reversed, so the reversed L order is a U order; when nb is a multiple
@@ -1630,7 +1635,7 @@ static int analysis_impl(int nb, int nnzb, int bs, const int *d_row, const int *
                     }
                     cnt[w] = k;
                 }
-            });
+            }, std::max(1, 4096 / usz));
             if (want_imports) {
                 imp_ptr[u].assign(nu + 1, 0);
                 for (int w = 0; w < nu; ++w) {
@@ -1641,7 +1646,7 @@ static int analysis_impl(int nb, int nnzb, int bs, const int *d_row, const int *
                 parallel_ranges(nu, [&](int w0, int w1) {
                     for (int w = w0; w < w1; ++w)
                         std::copy(buf.begin() + off[w], buf.begin() + off[w] + cnt[w], imp_col[u].begin() + imp_ptr[u][w]);
-                });
+                }, std::max(1, 4096 / usz));
             }
         }
         if (level_kernel && max_imp > kUnitImpMax)
