@@ -40,11 +40,11 @@ sys.path.insert(0, str(ROOT))
 METRIC = "solve time to 1e-10 rel-res, mixed vs FP64 ISAI; BSR SpMV/SpTRSV HBM GB/s"
 HBM_PEAK_GBPS = 8000.0
 TOL = 1e-10
-# Bytes per launch of the FP32 A product (two dots) at 96^3 beyond the L2 (HBM and
-# Infinity Cache), from the FETCH_SIZE / WRITE_SIZE counter passes summarised in
-# profiles/hbm_traffic_96cube_7pt_pmc.md (FETCH_SIZE x2 by the copy-kernel
-# calibration): 233,667.1 KiB x 2 + 13,955.7 KiB.
-F32_SPMV_TRAFFIC_BYTES_96CUBE = 233667.1 * 2 * 1024 + 13955.7 * 1024
+# Bytes per launch of the FP32 A product (two dots, the handle's line-aligned FP32
+# copy of A) at 96^3 beyond the L2 (HBM and Infinity Cache), from the FETCH_SIZE /
+# WRITE_SIZE counter passes summarised in profiles/hbm_traffic_96cube_7pt_pmc.md
+# (FETCH_SIZE x2 by the copy-kernel calibration): 228,772.9 KiB x 2 + 13,952.4 KiB.
+F32_SPMV_TRAFFIC_BYTES_96CUBE = 228772.9 * 2 * 1024 + 13952.4 * 1024
 
 
 def log(msg: str) -> None:
