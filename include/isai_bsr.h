@@ -39,6 +39,15 @@ int isai_build_dinv(int nb, int bs, const int *d_Arow, const int *d_Acol, const 
 int isai_pairs_count(int nb, int Mnnz, const int *d_Arow, const int *d_Acol, const int *d_Mbrow,
                      const int *d_Mcol, int upper, int *d_PairPtr, int *total_pairs,
                      isai_stream_t stream);
+/* No counterpart in the reference: isai_pairs_count for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results; the scan's scratch lives in a fixed per-device array
+ * of the library (about 1 MiB, held until the process ends) instead of being
+ * allocated, so Mnnz + 1 is limited to 2^28 (ISAI_ERR_CAPACITY beyond).  Calls that
+ * overlap on one GPU share that array and race; serialize them. */
+int isai_pairs_count_noalloc(int nb, int Mnnz, const int *d_Arow, const int *d_Acol,
+                             const int *d_Mbrow, const int *d_Mcol, int upper, int *d_PairPtr,
+                             int *total_pairs, isai_stream_t stream);
 /* Replaces fill_pairs_lower_kernel_v2 (reference isai_bsr.h:1427-1476): PairPLocal
  * (index within S_j) and PairSrc (BSR position of A(t, S_j[p])).  Lower lists are
  * ascending in p_local, upper lists descending (the reference's ISAI(U) loop order,

@@ -89,6 +89,7 @@ SIGNATURES: dict[str, list] = {
     "isai_bcsc_pattern": [I, P, P, I, P, P],
     "isai_build_dinv": [I, I, P, P, P, I, P, P],
     "isai_pairs_count": [I, I, P, P, P, P, I, P, P, P],
+    "isai_pairs_count_noalloc": [I, I, P, P, P, P, I, P, P, P],
     "isai_pairs_fill": [I, P, P, P, P, I, P, P, P, P],
     "isai_build_lower": [I, I, P, P, P, P, P, P, P, P, I, P],
     "isai_build_upper": [I, I, P, P, P, P, P, P, P, P, I, P],

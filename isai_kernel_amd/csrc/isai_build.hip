@@ -15,6 +15,9 @@
 // reference's "cachedY" form); columns longer than the LDS budget read Y back from
 // Mval (the "globalY" form, cacheY_isai.h:159-341 picks between the two the same
 // way).  Every column of both factors is solved in a single launch.
+#include <algorithm>
+#include <mutex>
+
 #include "common.hpp"
 #include "internal.hpp"
 #include "isai_bsr.h"
@@ -300,6 +303,38 @@ int pairs_count_async(int nb, int Mnnz, const int *d_Arow, const int *d_Acol, co
     }
     return exclusive_scan_i32(d_PairPtr, d_PairPtr, (long long)Mnnz + 1, st);
 }
+
+// ---------------------------------------------------------------- allocation-free pair count
+// isai_pairs_count_noalloc keeps the scan's tile sums (every level) and its overflow
+// flag in this file-scope device array instead of allocating them.  It bounds
+// Mnnz + 1 by kNoAllocMaxEntries (2^28 entries): 2^17 + 2^6 + 1 tile sums, one
+// word for the flag, about 1 MiB per device, held until the process ends.  Every
+// call on a device shares that device's copy, so calls that overlap on one GPU race.
+constexpr long long kNoAllocMaxEntries = 1LL << 28;
+constexpr long long kNoAllocWords = 131168;  // >= scan_ws_longs(2^28) + 1, a multiple of 32
+static __device__ __align__(256) long long scratch[kNoAllocWords];
+
+constexpr int kNoAllocMaxDevices = 64;
+static long long *g_scratch_ptr[kNoAllocMaxDevices];
+static std::mutex g_scratch_mu;
+
+// Device address of the current device's copy of scratch, looked up once per
+// device with that device current.
+static int scratch_address(long long **out) {
+    int dev = 0;
+    ISAI_HIP(hipGetDevice(&dev));
+    ISAI_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, "isai_pairs_count_noalloc: device %d beyond %d", dev,
+                 kNoAllocMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        ISAI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ISAI_REQUIRE(p != nullptr, "isai_pairs_count_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<long long *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return ISAI_OK;
+}
 }  // namespace isai
 
 extern "C" {
@@ -334,6 +369,38 @@ int isai_pairs_count(int nb, int Mnnz, const int *d_Arow, const int *d_Acol, con
     ISAI_REQUIRE(d_PairPtr, "isai_pairs_count: null PairPtr");
     hipStream_t st = (hipStream_t)stream;
     int s = isai::pairs_count_async(nb, Mnnz, d_Arow, d_Acol, d_Mbrow, d_Mcol, upper, d_PairPtr, st);
+    if (s != ISAI_OK) return s;
+    ISAI_HIP(hipMemcpyAsync(total_pairs, d_PairPtr + Mnnz, sizeof(int), hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipStreamSynchronize(st));
+    return ISAI_OK;
+}
+
+int isai_pairs_count_noalloc(int nb, int Mnnz, const int *d_Arow, const int *d_Acol, const int *d_Mbrow,
+                             const int *d_Mcol, int upper, int *d_PairPtr, int *total_pairs,
+                             isai_stream_t stream) {
+    ISAI_REQUIRE(nb >= 0 && Mnnz >= 0 && total_pairs, "isai_pairs_count_noalloc: bad argument");
+    ISAI_REQUIRE(d_PairPtr, "isai_pairs_count_noalloc: null PairPtr");
+    const long long n = (long long)Mnnz + 1;
+    if (n > kNoAllocMaxEntries) {
+        set_error("isai_pairs_count_noalloc: Mnnz + 1 = %lld exceeds the scratch limit of %lld entries", n,
+                  kNoAllocMaxEntries);
+        return ISAI_ERR_CAPACITY;
+    }
+    ISAI_REQUIRE(scan_ws_longs(n) + 1 <= kNoAllocWords, "isai_pairs_count_noalloc: scratch too small");
+    long long *ws = nullptr;
+    int s = scratch_address(&ws);
+    if (s != ISAI_OK) return s;
+    int *overflow = reinterpret_cast<int *>(ws + kNoAllocWords - 1);
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned fill_grid = std::min(grid_for(n, 256), 1024u);
+    fill_i32_kernel<<<fill_grid, 256, 0, st>>>(d_PairPtr, n, 0);
+    ISAI_LAUNCH_CHECK();
+    if (nb > 0) {
+        pairs_kernel<false><<<grid_for(nb, 128), 128, 0, st>>>(nb, d_Arow, d_Acol, d_Mbrow, d_Mcol, upper,
+                                                              d_PairPtr, nullptr, nullptr);
+        ISAI_LAUNCH_CHECK();
+    }
+    s = exclusive_scan_i32_ws(d_PairPtr, d_PairPtr, n, ws, overflow, st);
     if (s != ISAI_OK) return s;
     ISAI_HIP(hipMemcpyAsync(total_pairs, d_PairPtr + Mnnz, sizeof(int), hipMemcpyDeviceToHost, st));
     ISAI_HIP(hipStreamSynchronize(st));

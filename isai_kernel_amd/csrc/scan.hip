@@ -128,6 +128,67 @@ static int scan_i64_inplace(long long *d, long long n, hipStream_t st) {
     return ISAI_OK;
 }
 
+// ------------------------------------------------ scan with caller-owned scratch
+// The same three passes with every tile-sum level in a workspace the caller owns
+// (scan_ws_longs(n) int64 words) and the overflow flag at *overflow: no allocation,
+// no memset (a fill kernel clears the flag and a one-tile level).
+__global__ void fill_i32_kernel(int *p, long long n, int v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void zero_i64_kernel(long long *p) { *p = 0; }
+
+long long scan_ws_longs(long long n) {
+    long long total = 0;
+    for (long long t = (n + kScanTile - 1) / kScanTile;; t = (t + kScanTile - 1) / kScanTile) {
+        total += t;
+        if (t <= 1) break;
+    }
+    return total;
+}
+
+static int scan_i64_inplace_ws(long long *d, long long n, long long *ws, hipStream_t st) {
+    if (n <= 1) {
+        if (n == 1) {
+            zero_i64_kernel<<<1, 1, 0, st>>>(d);
+            ISAI_LAUNCH_CHECK();
+        }
+        return ISAI_OK;
+    }
+    const long long tiles = (n + kScanTile - 1) / kScanTile;
+    long long *sums = tiles > 1 ? ws : nullptr;
+    tile_scan_i64_kernel<<<(unsigned)tiles, kScanThreads, 0, st>>>(d, n, sums);
+    ISAI_LAUNCH_CHECK();
+    if (tiles > 1) {
+        int s = scan_i64_inplace_ws(sums, tiles, ws + tiles, st);
+        if (s != ISAI_OK) return s;
+        add_tile_offsets_i64_kernel<<<(unsigned)tiles, kScanThreads, 0, st>>>(d, n, sums);
+        ISAI_LAUNCH_CHECK();
+    }
+    return ISAI_OK;
+}
+
+int exclusive_scan_i32_ws(const int *d_in, int *d_out, long long n, long long *ws, int *overflow, hipStream_t st) {
+    if (n <= 0) return ISAI_OK;
+    const long long tiles = (n + kScanTile - 1) / kScanTile;
+    fill_i32_kernel<<<1, 64, 0, st>>>(overflow, 1, 0);
+    ISAI_LAUNCH_CHECK();
+    tile_sum_kernel<<<(unsigned)tiles, kScanThreads, 0, st>>>(d_in, n, ws);
+    ISAI_LAUNCH_CHECK();
+    int s = scan_i64_inplace_ws(ws, tiles, ws + tiles, st);
+    if (s != ISAI_OK) return s;
+    final_scan_kernel<<<(unsigned)tiles, kScanThreads, 0, st>>>(d_in, d_out, n, ws, overflow);
+    ISAI_LAUNCH_CHECK();
+    int h_overflow = 0;
+    ISAI_HIP(hipMemcpyAsync(&h_overflow, overflow, sizeof(int), hipMemcpyDeviceToHost, st));
+    ISAI_HIP(hipStreamSynchronize(st));
+    if (h_overflow) {
+        set_error("exclusive scan: prefix exceeds int32");
+        return ISAI_ERR_CAPACITY;
+    }
+    return ISAI_OK;
+}
+
 int exclusive_scan_i32(const int *d_in, int *d_out, long long n, hipStream_t st) {
     if (n <= 0) return ISAI_OK;
     const long long tiles = (n + kScanTile - 1) / kScanTile;

@@ -27,10 +27,9 @@ def _value(out, tag):
 
 @pytest.mark.parametrize("args", [
     ("--problem", "0"),
-    # This case ended with a host segmentation fault (exit -11, no output) in a run
-    # whose library also held a 1 MB static device scratch array for an
-    # allocation-free entry point; that code was taken out, but no cause was shown,
-    # so the case stays skipped.
+    # This case ended with a host segmentation fault (exit -11, no output) in one run
+    # (its library held the static device scratch array of isai_pairs_count_noalloc,
+    # which this flow never calls); no cause was shown, so the case stays skipped.
     pytest.param(("--problem", "3", "--L", "40", "--precision", "bf16", "--solver", "fgmres"),
                  marks=pytest.mark.skip(reason="host segfault (-11) without a shown cause; kept off the GPU")),
 ])
