@@ -524,7 +524,8 @@ def main() -> int:
 
     if args.kernels_only:
         h.close()
-        print(json.dumps({"kernels": kernels, "copy_GBps": round(copy_gbps, 1)}), flush=True)
+        print(json.dumps({"kernels": kernels, "copy_GBps": round(copy_gbps, 1),
+                          "setup_ms": {"sptrsv_analysis": round(ms_sv, 3)}}), flush=True)
         return 0
 
     # ---- solves (the steps)

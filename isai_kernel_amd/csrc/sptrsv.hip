@@ -28,7 +28,6 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
-#include <queue>
 #include <thread>
 #include <vector>
 
@@ -1299,11 +1298,12 @@ static void level_order(int nb, const std::vector<int> &lev, int nlev, bool uppe
 // The walk moves along the level wavefront, which in row order spans the whole
 // matrix (a 3-D stencil's level planes cut through every row range), so it first
 // copies the triangle's dependency lists into level-major numbering, where the
-// rows it touches at any time sit close together; the order it produces is the
-// same (the keys keep the original level and position).
+// rows it touches at any time sit close together.  That numbering sorts rows by
+// (level, position), the walk's own tie-break, so the ready pool is a two-level
+// bitset over it (the shallowest ready row is its first set bit) and the rows that
+// became ready in the current unit sit in a small heap keyed (-inside, number).
 static void unit_order(int nb, const std::vector<int> &rp, const std::vector<int> &ci, const std::vector<int> &diag,
                        bool upper, const std::vector<int> &lev, int nlev, int usz, std::vector<int> &order) {
-    auto rank = [&](int i) { return upper ? nb - 1 - i : i; };
     // level-major numbering: perm[n] = row, inv[row] = n
     std::vector<int> cnt(nlev + 1, 0), perm(nb), inv(nb);
     for (int i = 0; i < nb; ++i) cnt[lev[i] + 1]++;
@@ -1334,24 +1334,38 @@ static void unit_order(int nb, const std::vector<int> &rp, const std::vector<int
         need[n] = rq[n + 1] - rq[n];
         for (int p = rq[n]; p < rq[n + 1]; ++p) cons[cur[cq[p]]++] = n;
     }
-    using Ready = std::pair<long long, int>;                  // (level << 32 | position rank, row)
-    using Local = std::pair<std::pair<int, long long>, int>;  // ((-inside, level key), row)
-    std::priority_queue<Ready, std::vector<Ready>, std::greater<Ready>> pool;
-    std::priority_queue<Local, std::vector<Local>, std::greater<Local>> local;
-    auto key = [&](int n) { return ((long long)lev[perm[n]] << 32) | (long long)rank(perm[n]); };
+    // ready pool: bit n of bits[n / 64], and bit w % 64 of sum[w / 64] when bits[w] != 0
+    const int nw = (nb + 63) / 64, ns = (nw + 63) / 64;
+    std::vector<uint64_t> bits(nw, 0), sum(ns, 0);
+    int low = ns;  // no summary word below it is non-zero
+    auto pool_add = [&](int n) {
+        bits[n >> 6] |= 1ull << (n & 63);
+        sum[n >> 12] |= 1ull << ((n >> 6) & 63);
+        low = std::min(low, n >> 12);
+    };
+    auto pool_pop = [&]() {
+        while (!sum[low]) ++low;
+        const int w = low * 64 + __builtin_ctzll(sum[low]);
+        const int n = w * 64 + __builtin_ctzll(bits[w]);
+        bits[w] &= bits[w] - 1;
+        if (!bits[w]) sum[low] &= sum[low] - 1;
+        return n;
+    };
+    std::vector<uint64_t> local;  // min-heap of (INT_MAX - inside) << 32 | n
+    local.reserve(1024);
     for (int n = 0; n < nb; ++n)
-        if (need[n] == 0) pool.push({key(n), n});
+        if (need[n] == 0) pool_add(n);
     order.clear();
     order.reserve(nb);
     for (int u = 0; (int)order.size() < nb; ++u) {
         for (int size = 0; size < usz && (int)order.size() < nb; ++size) {
             int n;
             if (!local.empty()) {
-                n = local.top().second;
-                local.pop();
+                std::pop_heap(local.begin(), local.end(), std::greater<uint64_t>());
+                n = (int)(local.back() & 0xffffffffu);
+                local.pop_back();
             } else {
-                n = pool.top().second;
-                pool.pop();
+                n = pool_pop();
             }
             unit[n] = u;
             order.push_back(n);
@@ -1360,10 +1374,12 @@ static void unit_order(int nb, const std::vector<int> &rp, const std::vector<int
                 if (--need[c] != 0) continue;
                 int inside = 0;
                 for (int p = rq[c]; p < rq[c + 1]; ++p) inside += unit[cq[p]] == u;
-                local.push({{-inside, key(c)}, c});
+                local.push_back(((uint64_t)(INT_MAX - inside) << 32) | (uint64_t)c);
+                std::push_heap(local.begin(), local.end(), std::greater<uint64_t>());
             }
         }
-        for (; !local.empty(); local.pop()) pool.push({key(local.top().second), local.top().second});
+        for (uint64_t e : local) pool_add((int)(e & 0xffffffffu));
+        local.clear();
     }
     for (int &n : order) n = perm[n];
 }
@@ -1608,14 +1624,23 @@ static int analysis_impl(int nb, int nnzb, int bs, const int *d_row, const int *
                 }
                 buf.resize(std::max(off[nu], 1));
             }
+            // a unit's imports are numbered through an open-addressing table of at
+            // least twice its dependency count, cleared after the unit at the
+            // entries it filled
+            int max_d = 0;
+            for (int w = 0; want_imports && w < nu; ++w) max_d = std::max(max_d, off[w + 1] - off[w]);
+            int hcap = 64;
+            while (hcap < 2 * max_d) hcap <<= 1;
             parallel_ranges(nu, [&](int w0, int w1) {
-                std::vector<int> stamp, slot;
+                std::vector<int> hkey, hval, used;
                 if (want_imports) {
-                    stamp.assign(nb, -1);
-                    slot.resize(nb);
+                    hkey.assign(hcap, -1);
+                    hval.resize(hcap);
                 }
                 for (int w = w0; w < w1; ++w) {
                     int k = 0;
+                    unsigned mask = 63;
+                    while ((int)mask + 1 < 2 * (off[w + 1] - off[w])) mask = mask * 2 + 1;
                     for (int q = w * usz; q < std::min(nb, (w + 1) * usz); ++q) {
                         const int i = order[u][q];
                         const int pb = u == 0 ? rp[i] : diag[i] + 1, pe = u == 0 ? diag[i] : rp[i + 1];
@@ -1624,15 +1649,20 @@ static int analysis_impl(int nb, int nnzb, int bs, const int *d_row, const int *
                             if (at[c] / usz == w) {
                                 cx[p] = ~(at[c] % usz);
                             } else if (want_imports) {
-                                if (stamp[c] != w) {
-                                    stamp[c] = w;
-                                    slot[c] = k;
+                                unsigned h = ((unsigned)c * 2654435761u >> 7) & mask;
+                                while (hkey[h] >= 0 && hkey[h] != c) h = (h + 1) & mask;
+                                if (hkey[h] < 0) {
+                                    hkey[h] = c;
+                                    hval[h] = k;
+                                    used.push_back((int)h);
                                     buf[off[w] + k++] = c;
                                 }
-                                cx[p] = ~(usz + slot[c]);
+                                cx[p] = ~(usz + hval[h]);
                             }
                         }
                     }
+                    for (int h : used) hkey[h] = -1;
+                    used.clear();
                     cnt[w] = k;
                 }
             }, std::max(1, 4096 / usz));
