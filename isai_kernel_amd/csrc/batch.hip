@@ -27,6 +27,9 @@ constexpr int kMaxRhs = 16;
 constexpr int kItersPerReplay = 4;  // iterations in one captured graph
 constexpr int kMaxRestarts = 8;
 constexpr int kSpmmMaxWg = 256;  // largest SpMM workgroup the autotuner tries
+// Whole-row mapping: right-hand sides per pass (measured on the 256-system 9-pt batch:
+// 2 per pass and a software-pipelined pass of 1 were no faster, at 3-4 waves per SIMD)
+constexpr int kWholeRowRhs = 1;
 
 // status: 0 running, 1 converged (recursive residual), 2 converged on s with the
 // x update pending in this iteration, 3 breakdown, 4 done (true residual met),
@@ -300,6 +303,98 @@ __device__ __forceinline__ void col_row(const RowCtx &q, const int *__restrict__
     }
 }
 
+// Whole-row mapping (FP64 values): the row-pair lane layout of f64_row, with the
+// row's column indices and value lines loaded once, up front -- lane s loads the
+// indices of blocks s and s + 8 (passed to the other lanes by shuffle) and every
+// value line of the row -- so a lane group holds a whole row of up to KB blocks in
+// flight instead of one block.  The right-hand sides are then taken C at a time:
+// all x gathers of a chunk are issued before its products, and the registers do
+// not grow with R (f64_row holds 4R accumulator registers per lane).  Per (row,
+// right-hand side) the products accumulate block by block into the same two
+// partial sums as f64_row, so the results equal the row-pair mapping's bit for
+// bit.  Rows longer than KB take the chunks block by block, reloading the values
+// per chunk.
+template <int KB, int C, class XT>
+__device__ __forceinline__ void f64_rowc(const RowCtx &q, int R, const int *__restrict__ ci,
+                                         const v2d *__restrict__ val, const XT *__restrict__ X,
+                                         double *__restrict__ Y) {
+    static_assert(KB <= 16, "two index loads per lane");
+    using XP = typename std::conditional<sizeof(XT) == 8, v2d, v2f>::type;
+    const int h = q.lane & 1;
+    const int len = q.re - q.rs;
+    // 32-bit element offsets from the uniform bases X and val (the launch checks
+    // that every multi-vector offset and value index fits), so the gathers take
+    // the scalar-base addressing form instead of a 64-bit address per load
+    const unsigned xo = (unsigned)q.v0 + 2u * h, st = (unsigned)q.step;
+    double *yr = Y + q.v0 + (size_t)(q.br - q.base) * 4 + (q.lane >> 1);
+    if (len <= KB) {
+        const int g0 = (int)(threadIdx.x & 63) & ~7;  // first lane of the group in the wave
+        const int c0 = q.lane < len ? ci[q.rs + q.lane] - q.base : 0;
+        const int c1 = (KB > 8 && q.lane + 8 < len) ? ci[q.rs + 8 + q.lane] - q.base : 0;
+        v2d a[KB];
+        const unsigned vo = (unsigned)q.rs * 8u + q.lane;
+#pragma unroll
+        for (int t = 0; t < KB; ++t)
+            a[t] = t < len ? __builtin_nontemporal_load(&val[vo + 8u * t]) : v2d{0.0, 0.0};
+        unsigned col[KB];
+#pragma unroll
+        for (int t = 0; t < KB; ++t) col[t] = xo + 4u * (unsigned)__shfl(t < 8 ? c0 : c1, g0 + (t & 7));
+        for (int j0 = 0; j0 < R; j0 += C) {
+            XP x[KB][C];  // held in storage precision until the products
+#pragma unroll
+            for (int t = 0; t < KB; ++t)
+#pragma unroll
+                for (int u = 0; u < C; ++u)
+                    x[t][u] = (t < len && j0 + u < R) ? *(const XP *)&X[col[t] + (unsigned)(j0 + u) * st] : XP{0, 0};
+            double acc0[C], acc1[C];
+#pragma unroll
+            for (int u = 0; u < C; ++u) acc0[u] = acc1[u] = 0.0;
+#pragma unroll
+            for (int t = 0; t < KB; ++t) {
+                if (t < len) {
+#pragma unroll
+                    for (int u = 0; u < C; ++u) {
+                        acc0[u] = fma(a[t].x, (double)x[t][u].x, acc0[u]);
+                        acc1[u] = fma(a[t].y, (double)x[t][u].y, acc1[u]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < C; ++u) {
+                double s = acc0[u] + acc1[u];
+                s += __shfl_xor(s, 1);
+                const int j = j0 + u;
+                if (j < R && (q.mask & (1u << j)) && h == 0) yr[(size_t)j * q.step] = s;
+            }
+        }
+        return;
+    }
+    for (int j0 = 0; j0 < R; j0 += C) {
+        double acc0[C], acc1[C];
+#pragma unroll
+        for (int u = 0; u < C; ++u) acc0[u] = acc1[u] = 0.0;
+        for (int k = q.rs; k < q.re; ++k) {
+            const unsigned c = xo + 4u * (unsigned)(ci[k] - q.base);
+            const v2d av = val[(unsigned)k * 8u + q.lane];
+            XP x[C];
+#pragma unroll
+            for (int u = 0; u < C; ++u) x[u] = j0 + u < R ? *(const XP *)&X[c + (unsigned)(j0 + u) * st] : XP{0, 0};
+#pragma unroll
+            for (int u = 0; u < C; ++u) {
+                acc0[u] = fma(av.x, (double)x[u].x, acc0[u]);
+                acc1[u] = fma(av.y, (double)x[u].y, acc1[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+            double s = acc0[u] + acc1[u];
+            s += __shfl_xor(s, 1);
+            const int j = j0 + u;
+            if (j < R && (q.mask & (1u << j)) && h == 0) yr[(size_t)j * q.step] = s;
+        }
+    }
+}
+
 // MFMA mapping.  v_mfma_f64_4x4x4f64 computes four independent 4x4x4 products; its
 // operands (measured on gfx950) sit at lane x + 4b + 16y with A[b][i=x][k=y],
 // B[b][k=y][j=x] and C[b][i=y][j=x].  One wave takes four block rows (b), every
@@ -541,6 +636,28 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64_kernel(int nb, const int
     const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     f64_unit<MAP, XT>(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3),
                       (int)(threadIdx.x & 7), all);
+}
+
+// MAP 3: whole-row mapping, KB blocks up front, C right-hand sides per chunk.
+template <int KB, int C, class XT>
+__device__ __forceinline__ void f64c_unit(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+                                          const v2d *__restrict__ val, const BatchDev &B, const XT *__restrict__ X,
+                                          double *__restrict__ Y, int br, int lane, int all) {
+    RowCtx q;
+    int r;
+    if (!row_context(B, rp, nb, br, lane, all, q, r)) return;
+    f64_rowc<KB, C, XT>(q, r, ci, val, X, Y);
+}
+
+template <int KB, int C, class XT>
+__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64c_kernel(int nb, const int *__restrict__ rp,
+                                                                const int *__restrict__ ci,
+                                                                const v2d *__restrict__ val, BatchDev B,
+                                                                const XT *__restrict__ X, double *__restrict__ Y,
+                                                                int remap, int all) {
+    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    f64c_unit<KB, C, XT>(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3),
+                         (int)(threadIdx.x & 7), all);
 }
 
 template <bool BF16, int MAP, class XT, class YT>
@@ -995,10 +1112,11 @@ struct PersistArgs {
 enum Operand { kOpA = 0, kOpML = 1, kOpMU = 2 };
 
 // Y = op X over the pairs still running (all = 1: every pair); MAP as the
-// autotuner's mapping (0 row-pair, 1 column-split, 2 MFMA), PREC the operand's
-// value precision.  Units: 8-lane groups (one block row) for MAP 0/1, waves of
-// kMfmaF64Rows / kMfmaLowpRows block rows for MAP 2.
-template <int MAP, int PREC, class XT, class YT>
+// autotuner's mapping (0 row-pair, 1 column-split, 2 MFMA, 3 whole-row with KB
+// blocks per row), PREC the operand's value precision.  Units: 8-lane groups (one
+// block row) for MAP 0/1/3, waves of kMfmaF64Rows / kMfmaLowpRows block rows for
+// MAP 2.
+template <int MAP, int PREC, int KB = 8, class XT, class YT>
 __device__ __forceinline__ void spmm_phase(int nb, const int *rp, const int *ci, const void *val,
                                            const BatchDev &B, const XT *X, YT *Y, int all) {
     const int lane = threadIdx.x & 63;
@@ -1015,7 +1133,9 @@ __device__ __forceinline__ void spmm_phase(int nb, const int *rp, const int *ci,
     } else {
         const int gpg = kPersistThreads / 8;
         for (int br = blockIdx.x * gpg + (int)(threadIdx.x >> 3); br < nb; br += gridDim.x * gpg) {
-            if constexpr (PREC == ISAI_PREC_FP64)
+            if constexpr (PREC == ISAI_PREC_FP64 && MAP == 3)
+                f64c_unit<KB, kWholeRowRhs, XT>(nb, rp, ci, (const v2d *)val, B, X, Y, br, lane & 7, all);
+            else if constexpr (PREC == ISAI_PREC_FP64)
                 f64_unit<MAP, XT>(nb, rp, ci, (const v2d *)val, B, X, Y, br, lane & 7, all);
             else
                 lowp_unit<PREC == ISAI_PREC_BF16, MAP, XT, YT>(nb, rp, ci, (const v4u *)val, B, X, Y, br, lane & 7,
@@ -1024,7 +1144,7 @@ __device__ __forceinline__ void spmm_phase(int nb, const int *rp, const int *ci,
     }
 }
 
-template <int MAPA, int MAPM, int PRECM>
+template <int MAPA, int MAPM, int PRECM, int KBA = 8>
 __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(PersistArgs a) {
     const BatchDev &B = a.B;
     const int sub = threadIdx.x / kRed, lane = threadIdx.x & 63;
@@ -1037,7 +1157,7 @@ __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(Persi
     // every group meets the same number of reduction barriers
     auto tiles_end = [&](int t0) { return t0 < B.ntiles; };
     auto A = [&](const auto *X, double *Y, int all) {
-        spmm_phase<MAPA, ISAI_PREC_FP64>(a.nb, a.rp, a.ci, a.val, B, X, Y, all);
+        spmm_phase<MAPA, ISAI_PREC_FP64, KBA>(a.nb, a.rp, a.ci, a.val, B, X, Y, all);
     };
     auto ML = [&](const double *X, IT *Y) { spmm_phase<MAPM, PRECM>(a.nb, a.Lrow, a.Lcol, a.Lval, B, X, Y, 0); };
     auto MU = [&](const IT *X, IT *Y) { spmm_phase<MAPM, PRECM>(a.nb, a.Urow, a.Ucol, a.Uval, B, X, Y, 0); };
@@ -1155,9 +1275,11 @@ struct isai_batch {
     static constexpr int kWork = 9;
     // launch shapes picked by the autotuner: [0] product with A, [1] with the ISAI factors
     int wg[2] = {256, 256}, remap[2] = {1, 1}, map[2] = {2, 2};
+    int kb[2] = {16, 16};  // whole-row mapping: blocks held per row (the operand's longest row, 8 / 10 / 12 / 16)
+    bool idx32 = false;    // whole-row mapping: value indices and multi-vector offsets fit 32 bits
     int variant = 0, ncand = 0;
     double tuned_us = 0;
-    double table[2][18] = {};  // autotuner: us per product, [operand][candidate]
+    double table[2][24] = {};  // autotuner: us per product, [operand][candidate]
     double ms_ilu = 0, ms_isai = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -1218,18 +1340,32 @@ int launch_spmm(isai_batch *b, Operand op, int prec, const int *rp, const int *c
     const BatchDev B = b->dev();
     const int slot = op == kOpA ? 0 : 1;
     const int threads = b->wg[slot];
-    const int rows = MAP < 2 ? threads / 8
-                             : (threads / 64) * (prec == ISAI_PREC_FP64 ? kMfmaF64Rows : kMfmaLowpRows);
+    const int rows = MAP != 2 ? threads / 8
+                              : (threads / 64) * (prec == ISAI_PREC_FP64 ? kMfmaF64Rows : kMfmaLowpRows);
     const unsigned grid = grid_for(b->nb, rows);
     const int remap = b->remap[slot];
     if (prec == ISAI_PREC_FP64) {
         if constexpr (std::is_same<YT, double>::value) {
-            if (MAP == 2)
+            if (MAP == 2) {
                 bspmm_mfma_f64_kernel<XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const double *)v, B, X, Y, remap,
                                                                     all);
-            else
-                bspmm_f64_kernel<MAP == 2 ? 0 : MAP, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v2d *)v, B,
-                                                                                  X, Y, remap, all);
+            } else if (MAP >= 3 && b->idx32) {
+                constexpr int C = kWholeRowRhs;
+                const int kb = b->kb[slot];
+                const v2d *v2 = (const v2d *)v;
+                if (kb <= 8)
+                    bspmm_f64c_kernel<8, C, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v2, B, X, Y, remap, all);
+                else if (kb <= 10)
+                    bspmm_f64c_kernel<10, C, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v2, B, X, Y, remap, all);
+                else if (kb <= 12)
+                    bspmm_f64c_kernel<12, C, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v2, B, X, Y, remap, all);
+                else
+                    bspmm_f64c_kernel<16, C, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v2, B, X, Y, remap, all);
+            } else {
+                // row-pair for MAP 0 and for the whole-row mappings beyond 32-bit offsets
+                bspmm_f64_kernel<MAP == 1 ? 1 : 0, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v2d *)v, B, X,
+                                                                                Y, remap, all);
+            }
         } else {
             set_error("batch spmm: an FP64 operand writes FP64 vectors");
             return ISAI_ERR_INVALID_ARG;
@@ -1240,10 +1376,10 @@ int launch_spmm(isai_batch *b, Operand op, int prec, const int *rp, const int *c
         else
             bspmm_mfma_lowp_kernel<true, XT, YT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v, B, X, Y, remap, all);
     } else if (prec == ISAI_PREC_FP32) {
-        bspmm_lowp_kernel<false, MAP == 2 ? 0 : MAP, XT, YT>
+        bspmm_lowp_kernel<false, MAP == 1 ? 1 : 0, XT, YT>
             <<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, remap, all);
     } else {
-        bspmm_lowp_kernel<true, MAP == 2 ? 0 : MAP, XT, YT>
+        bspmm_lowp_kernel<true, MAP == 1 ? 1 : 0, XT, YT>
             <<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, remap, all);
     }
     ISAI_LAUNCH_CHECK();
@@ -1267,6 +1403,7 @@ int spmm_typed(isai_batch *b, Operand op, const XT *X, YT *Y, int all, hipStream
     const int map = b->map[op == kOpA ? 0 : 1];
     if (map == 0) return launch_spmm<0>(b, op, prec, rp, ci, v, X, Y, all, st);
     if (map == 1) return launch_spmm<1>(b, op, prec, rp, ci, v, X, Y, all, st);
+    if (map == 3) return launch_spmm<3>(b, op, prec, rp, ci, v, X, Y, all, st);
     return launch_spmm<2>(b, op, prec, rp, ci, v, X, Y, all, st);
 }
 
@@ -1328,10 +1465,11 @@ int batch_iteration(isai_batch *b, double *x, hipStream_t st) {
 // Launch-shape autotuner: for the product with A (FP64) and with the ISAI factors
 // (storage precision) time every (lane mapping, workgroup size, XCD remap)
 // candidate on the batch itself, all pairs active, and keep the fastest of each.
-// Candidate c: mapping c / 6 (0 row-pair lanes, 1 column-split lanes, 2 MFMA),
-// workgroup kWgCand[(c / 2) % 3], remap c % 2.
+// Candidate c: mapping c / 6 (0 row-pair lanes, 1 column-split lanes, 2 MFMA,
+// 3 whole-row lanes), workgroup kWgCand[(c / 2) % 3], remap c % 2.  The factor
+// products have no whole-row form: their candidates 18..23 run the row-pair mapping.
 constexpr int kWgCand[3] = {64, 128, kSpmmMaxWg};
-constexpr int kCandPerOp = 18;
+constexpr int kCandPerOp = 24;
 
 int autotune(isai_batch *b, hipStream_t st) {
     double *X = b->work, *Y = b->work + b->len;
@@ -1407,17 +1545,25 @@ int capture_graph(isai_batch *b, double *d_X, hipStream_t st) {
 
 typedef void (*PersistFn)(PersistArgs);
 
-template <int MA, int MM>
+template <int MA, int MM, int KBA = 8>
 PersistFn persist_fn_prec(int prec) {
-    if (prec == ISAI_PREC_FP64) return batch_persistent_kernel<MA, MM, ISAI_PREC_FP64>;
-    if (prec == ISAI_PREC_FP32) return batch_persistent_kernel<MA, MM, ISAI_PREC_FP32>;
-    return batch_persistent_kernel<MA, MM, ISAI_PREC_BF16>;
+    if (prec == ISAI_PREC_FP64) return batch_persistent_kernel<MA, MM, ISAI_PREC_FP64, KBA>;
+    if (prec == ISAI_PREC_FP32) return batch_persistent_kernel<MA, MM, ISAI_PREC_FP32, KBA>;
+    return batch_persistent_kernel<MA, MM, ISAI_PREC_BF16, KBA>;
 }
 
-// The persistent kernel holds the row-pair (0) and MFMA (2) mappings; a
-// column-split choice (1) runs as row-pair there.
-PersistFn persist_fn(int map_a, int map_m, int prec) {
-    const bool ma = map_a == 2, mm = map_m == 2;
+// The persistent kernel holds the row-pair (0), MFMA (2) and whole-row (3, rows of
+// up to 10 blocks) mappings; a column-split choice (1), or a whole-row choice on
+// longer rows, runs as row-pair there.  The whole-row products equal the row-pair
+// ones bit for bit, and the whole-row A product fits the kernel's 128-register
+// budget without spilling (the row-pair one spills).
+PersistFn persist_fn(int map_a, int map_m, int prec, int kb_a, bool idx32) {
+    const bool mm = map_m == 2;
+    if (map_a == 3 && idx32 && kb_a <= 10) {
+        if (kb_a <= 8) return mm ? persist_fn_prec<3, 2, 8>(prec) : persist_fn_prec<3, 0, 8>(prec);
+        return mm ? persist_fn_prec<3, 2, 10>(prec) : persist_fn_prec<3, 0, 10>(prec);
+    }
+    const bool ma = map_a == 2;
     if (ma && mm) return persist_fn_prec<2, 2>(prec);
     if (ma) return persist_fn_prec<2, 0>(prec);
     if (mm) return persist_fn_prec<0, 2>(prec);
@@ -1457,7 +1603,7 @@ int persistent_solve(isai_batch *b, double tol, int maxit, const double *d_B, do
     a.tmp = w + 8 * n;
     a.tol = tol;
     a.relres = b->d_relres;
-    const PersistFn fn = persist_fn(b->map[0], b->map[1], f.precision);
+    const PersistFn fn = persist_fn(b->map[0], b->map[1], f.precision, b->kb[0], b->idx32);
     int occ = 0;
     ISAI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn, kPersistThreads, 0));
     if (occ < 1 || b->ncu < 1) {
@@ -1572,6 +1718,9 @@ int isai_batch_setup(int nsys, const int *sys_ptr, const int *nrhs, int nnzb, co
             set_error("isai_batch_setup: row pointer spans [%d, %d), expected [0, %d)", rp[0], rp[b->nb], nnzb);
             return fail(ISAI_ERR_INVALID_ARG);
         }
+        int maxlen = 0;
+        for (int i = 0; i < b->nb; ++i) maxlen = std::max(maxlen, rp[i + 1] - rp[i]);
+        b->kb[0] = maxlen <= 8 ? 8 : (maxlen <= 10 ? 10 : (maxlen <= 12 ? 12 : 16));
         for (int s = 0; s < nsys; ++s)
             for (int i = b->sys_ptr[s]; i < b->sys_ptr[s + 1]; ++i)
                 for (int q = rp[i]; q < rp[i + 1]; ++q)
@@ -1599,6 +1748,7 @@ int isai_batch_setup(int nsys, const int *sys_ptr, const int *nrhs, int nnzb, co
     }
     b->len = (size_t)off;
     b->ntiles = (int)tile_pair.size();
+    b->idx32 = 8LL * nnzb < (1LL << 32) && off < (1LL << 32);
     auto up = [](void **d, const void *h, size_t bytes) -> bool {
         if (hipMalloc(d, bytes) != hipSuccess) return false;
         return hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) == hipSuccess;

@@ -16,6 +16,7 @@ namespace isai {
 // non-temporal load builtins).
 typedef double v2d __attribute__((ext_vector_type(2)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 // Records the message of the last failure on this thread (isai_last_error()).

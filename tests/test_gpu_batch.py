@@ -66,9 +66,9 @@ def test_batch_layout_and_spmm(stream):
                 mask[o:o + 4 * int(sys_ptr[s + 1] - sys_ptr[s])] = False
         assert not Y[mask].any()
         v, us, nc = bt.tuning()
-        assert nc == 36 and 0 <= v < 18 and us > 0
+        assert nc == 48 and 0 <= v < 24 and us > 0
         tab = bt.tuning_table(0)
-        assert len(tab) == 18 and (tab > 0).all() and tab[v] == pytest.approx(us) and tab.min() == pytest.approx(us)
+        assert len(tab) == 24 and (tab > 0).all() and tab[v] == pytest.approx(us) and tab.min() == pytest.approx(us)
     finally:
         bt.close()
 
@@ -202,20 +202,27 @@ def test_batch_every_spmm_candidate(stream, precision):
         B = DeviceArray.from_numpy(bt.pack(_rhs(sys_ptr, nrhs, 5)))
         refs = [[O.spmv(*_system(rp, ci, val, sys_ptr, s), X[s][j]) for j in range(nrhs[s])] for s in range(len(nrhs))]
         its = []
-        for c in range(18):
+        row_pair = None
+        for c in range(24):
             bt.select(0, c)
             bt.select(1, c)
             bt.spmm(d_X, d_Y)
             call("isai_stream_synchronize", stream)
-            got = bt.unpack(d_Y.to_numpy())
+            y = d_Y.to_numpy()
+            got = bt.unpack(y)
             for s in range(len(nrhs)):
                 for j in range(nrhs[s]):
                     assert rel_err(got[s][j], refs[s][j]) <= 1e-14, (c, s, j)
+            # the whole-row mapping (3) accumulates in the row-pair order
+            if c == 0:
+                row_pair = y.copy()
+            elif c // 6 >= 3:
+                assert np.array_equal(y, row_pair), c
             d_x = DeviceArray.zeros((bt.length,), np.float64)
             st, iters, rr = bt.solve(B, d_x, tol=1e-10, maxit=500)
             assert st == 0 and (rr <= 1e-10).all(), c
             its.append(iters)
-            if c // 6 != 1:  # the persistent kernel holds the row-pair and MFMA mappings
+            if c // 6 != 1:  # the persistent kernel holds the row-pair, MFMA and whole-row mappings
                 d_p = DeviceArray.zeros((bt.length,), np.float64)
                 st, it_p, rr = bt.solve(B, d_p, tol=1e-10, maxit=500, use_graph=3)
                 assert st == 0 and np.array_equal(it_p, iters) and np.array_equal(d_p.to_numpy(), d_x.to_numpy()), c
@@ -224,6 +231,38 @@ def test_batch_every_spmm_candidate(stream, precision):
         if precision == "fp64":
             for it in its[1:]:
                 assert np.abs(it.astype(int) - its[0].astype(int)).max() <= 1
+    finally:
+        bt.close()
+
+
+@pytest.mark.parametrize("problem", [1, 2])
+def test_batch_whole_row_product_other_stencils(stream, problem):
+    """The whole-row mapping on 3-D 7-pt systems (rows of <= 7 blocks, all held at
+    once) and 27-pt systems (rows of 27 blocks, longer than the 16 a lane group holds:
+    the block-by-block path): equal to the row-pair product bit for bit, and to the
+    oracle's per-pair SpMV."""
+    print(f"batch whole-row product problem {problem}", flush=True)
+    rp, ci, val, sys_ptr, Ls, nrhs = problems.batch(3, 3, 6, seed=11, rhs_max=16, problem=problem)
+    bt = Batch(Matrix(rp, ci, val), sys_ptr, nrhs, precision="fp64", stream=stream)
+    try:
+        X = _rhs(sys_ptr, nrhs, 8)
+        d_X = DeviceArray.from_numpy(bt.pack(X))
+        d_Y = DeviceArray.zeros((bt.length,), np.float64)
+        refs = [[O.spmv(*_system(rp, ci, val, sys_ptr, s), X[s][j]) for j in range(nrhs[s])] for s in range(len(nrhs))]
+        row_pair = None
+        for c in [0] + list(range(18, 24)):
+            bt.select(0, c)
+            bt.spmm(d_X, d_Y)
+            call("isai_stream_synchronize", stream)
+            y = d_Y.to_numpy()
+            if c == 0:
+                row_pair = y.copy()
+                got = bt.unpack(y)
+                for s in range(len(nrhs)):
+                    for j in range(nrhs[s]):
+                        assert rel_err(got[s][j], refs[s][j]) <= 1e-14, (s, j)
+            else:
+                assert np.array_equal(y, row_pair), c
     finally:
         bt.close()
 
