@@ -649,13 +649,26 @@ __device__ __forceinline__ void f64c_unit(int nb, const int *__restrict__ rp, co
     f64_rowc<KB, C, XT>(q, r, ci, val, X, Y);
 }
 
+// Chunked XCD remap: the hardware deals consecutive workgroups round-robin over
+// the 8 XCDs; this hands each XCD runs of kXcdChunk consecutive logical workgroups
+// in turn (all XCDs still stream neighbouring regions at once, and the x entries a
+// run shares stay in one L2).  The tail past the last whole round keeps its order.
+constexpr int kXcdChunk = 64;
+__device__ __forceinline__ int xcd_chunk_remap(int bid, int nblocks) {
+    constexpr int kXcd = 8;
+    const int round = kXcd * kXcdChunk;
+    if (bid >= nblocks / round * round) return bid;
+    const int xcd = bid % kXcd, local = bid / kXcd;
+    return ((local / kXcdChunk) * kXcd + xcd) * kXcdChunk + local % kXcdChunk;
+}
+
 template <int KB, int C, class XT>
 __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64c_kernel(int nb, const int *__restrict__ rp,
                                                                 const int *__restrict__ ci,
                                                                 const v2d *__restrict__ val, BatchDev B,
                                                                 const XT *__restrict__ X, double *__restrict__ Y,
                                                                 int remap, int all) {
-    const int wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int wg = remap ? xcd_chunk_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     f64c_unit<KB, C, XT>(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3),
                          (int)(threadIdx.x & 7), all);
 }
@@ -818,9 +831,12 @@ __device__ __forceinline__ void brho_final_body(const BatchDev &B, int p, int la
     else S.beta = S.first ? 0.0 : (rho / S.rho_old) * (S.alpha / S.omega);
 }
 
-// p = r + beta (p - omega v)  (p = r on a pair's first iteration)
+// p = r + beta (p - omega v)  (p = r on a pair's first iteration); with FP32 /
+// BF16 factors also p32 = p rounded to FP32, the operand M_L reads (its products
+// round x to FP32 on load, so the copy gives the same results with half the
+// gathered bytes)
 __device__ __forceinline__ void bp_body(int t, const double *__restrict__ r, double *__restrict__ p,
-                                        const double *__restrict__ v, const BatchDev &B) {
+                                        const double *__restrict__ v, float *__restrict__ p32, const BatchDev &B) {
     if (!tile_live(B, t)) return;
     const PairState &S = B.ps[B.tile_pair[t]];
     const int tid = threadIdx.x & (kRed - 1);
@@ -838,8 +854,11 @@ __device__ __forceinline__ void bp_body(int t, const double *__restrict__ r, dou
         vv[k] = first ? 0.0 : v[i];
     }
 #pragma unroll
-    for (int k = 0; k < kTile / kRed; ++k)
-        p[base + k * kRed + tid] = first ? rr[k] : rr[k] + beta * (pp[k] - omega * vv[k]);
+    for (int k = 0; k < kTile / kRed; ++k) {
+        const double pk = first ? rr[k] : rr[k] + beta * (pp[k] - omega * vv[k]);
+        p[base + k * kRed + tid] = pk;
+        if (p32) p32[base + k * kRed + tid] = (float)pk;
+    }
 }
 
 __device__ __forceinline__ void balpha_final_body(const BatchDev &B, int p, int lane) {
@@ -853,9 +872,10 @@ __device__ __forceinline__ void balpha_final_body(const BatchDev &B, int p, int 
     S.first = 0;
 }
 
-// s = r - alpha v; partial ||s||^2
+// s = r - alpha v (and its FP32 copy s32 with FP32 / BF16 factors, as in bp_body);
+// partial ||s||^2
 __device__ __forceinline__ void bs_body(int t, int sub, const double *__restrict__ r, const double *__restrict__ v,
-                                        double *s, const BatchDev &B) {
+                                        double *s, float *__restrict__ s32, const BatchDev &B) {
     const bool live = tile_live(B, t);
     const int tid = threadIdx.x & (kRed - 1);
     const size_t base = (size_t)t * kTile;
@@ -867,6 +887,7 @@ __device__ __forceinline__ void bs_body(int t, int sub, const double *__restrict
             const size_t i = base + k * kRed + tid;
             const double si = r[i] - alpha * v[i];
             s[i] = si;
+            if (s32) s32[i] = (float)si;
             acc = fma(si, si, acc);
         }
     }
@@ -1015,12 +1036,12 @@ __global__ __launch_bounds__(kRed) void binit_kernel(const double *__restrict__ 
     binit_body(blockIdx.x, 0, b, r, rh, p, v, B);
 }
 __global__ __launch_bounds__(kRed) void bp_kernel(const double *__restrict__ r, double *__restrict__ p,
-                                                  const double *__restrict__ v, BatchDev B) {
-    bp_body(blockIdx.x, r, p, v, B);
+                                                  const double *__restrict__ v, float *p32, BatchDev B) {
+    bp_body(blockIdx.x, r, p, v, p32, B);
 }
 __global__ __launch_bounds__(kRed) void bs_kernel(const double *__restrict__ r, const double *__restrict__ v, double *s,
-                                                  BatchDev B) {
-    bs_body(blockIdx.x, 0, r, v, s, B);
+                                                  float *s32, BatchDev B) {
+    bs_body(blockIdx.x, 0, r, v, s, s32, B);
 }
 template <class PT>
 __global__ __launch_bounds__(kRed) void bxs_kernel(double *x, const PT *__restrict__ ph, BatchDev B) {
@@ -1105,6 +1126,7 @@ struct PersistArgs {
     double *x;                         // solutions (initial guesses in)
     double *r, *rh, *p, *v, *s, *t;
     void *ph, *sh, *tmp;               // FP32 with FP32 / BF16 factors (exact copies of their results)
+    size_t n;                          // multi-vector length
     double tol;
     double *relres;
 };
@@ -1153,13 +1175,22 @@ __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(Persi
     unsigned long long target = 0;
     using IT = typename std::conditional<PRECM == ISAI_PREC_FP64, double, float>::type;  // ISAI outputs
     IT *const ph = (IT *)a.ph, *const sh = (IT *)a.sh, *const tmp = (IT *)a.tmp;
+    // FP32 copy of p / s for M_L (second half of tmp's region), FP32 / BF16 factors only
+    float *const x32 = PRECM == ISAI_PREC_FP64 ? nullptr : (float *)a.tmp + a.n;
     // tile phases: the workgroup's tile groups take tiles t0 + sub together, so
     // every group meets the same number of reduction barriers
     auto tiles_end = [&](int t0) { return t0 < B.ntiles; };
     auto A = [&](const auto *X, double *Y, int all) {
         spmm_phase<MAPA, ISAI_PREC_FP64, KBA>(a.nb, a.rp, a.ci, a.val, B, X, Y, all);
     };
-    auto ML = [&](const double *X, IT *Y) { spmm_phase<MAPM, PRECM>(a.nb, a.Lrow, a.Lcol, a.Lval, B, X, Y, 0); };
+    auto ML = [&](const double *X, IT *Y) {
+        // FP32 / BF16 factors: M_L reads the FP32 copy of p or s (x32, written by
+        // the update that formed X), as the per-step launches do
+        if constexpr (PRECM == ISAI_PREC_FP64)
+            spmm_phase<MAPM, PRECM>(a.nb, a.Lrow, a.Lcol, a.Lval, B, X, Y, 0);
+        else
+            spmm_phase<MAPM, PRECM>(a.nb, a.Lrow, a.Lcol, a.Lval, B, (const float *)x32, Y, 0);
+    };
     auto MU = [&](const IT *X, IT *Y) { spmm_phase<MAPM, PRECM>(a.nb, a.Urow, a.Ucol, a.Uval, B, X, Y, 0); };
 #define ISAI_SYNC() \
     if (!grid_barrier(ctl, target)) return
@@ -1186,7 +1217,7 @@ __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(Persi
         int nactive = __hip_atomic_load(&ctl->nactive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // every pair stops within maxit iterations; the bound only guards the loop
         for (int it = 0; it <= ctl->maxit; ++it) {
-            ISAI_TILES(bp_body(t0 + sub, a.r, a.p, a.v, B));
+            ISAI_TILES(bp_body(t0 + sub, a.r, a.p, a.v, x32, B));
             ISAI_SYNC();
             ML(a.p, tmp);
             ISAI_SYNC();
@@ -1198,7 +1229,7 @@ __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(Persi
             ISAI_SYNC();
             ISAI_PAIRS(balpha_final_body(B, p, lane));
             ISAI_SYNC();
-            ISAI_TILES(bs_body(t0 + sub, sub, a.r, a.v, a.s, B));
+            ISAI_TILES(bs_body(t0 + sub, sub, a.r, a.v, a.s, x32, B));
             ISAI_SYNC();
             ISAI_PAIRS(bsnorm_final_body(B, p, lane));
             ISAI_SYNC();
@@ -1434,21 +1465,24 @@ int batch_iteration(isai_batch *b, double *x, hipStream_t st) {
     double *r = w, *rh = w + n, *p = w + 2 * n, *v = w + 3 * n, *s = w + 5 * n, *t = w + 7 * n;
     void *ph = w + 4 * n, *sh = w + 6 * n, *tmp = w + 8 * n;
     const bool f32 = b->precision != ISAI_PREC_FP64;
+    // FP32 / BF16 factors: tmp holds n floats, and the second half of its region the
+    // FP32 copy of p / s that M_L reads (the same values its loads would round to)
+    float *x32 = f32 ? (float *)tmp + n : nullptr;
     const BatchDev B = b->dev();
     const int np = b->npairs, T = b->ntiles;
     zero_active_kernel<<<1, 1, 0, st>>>(b->d_ctl);
     brho_final<<<np, 64, 0, st>>>(B);
-    bp_kernel<<<T, kRed, 0, st>>>(r, p, v, B);
-    BTRY(spmm(b, kOpML, p, false, tmp, f32, 0, st));
+    bp_kernel<<<T, kRed, 0, st>>>(r, p, v, x32, B);
+    BTRY(f32 ? spmm(b, kOpML, x32, true, tmp, true, 0, st) : spmm(b, kOpML, p, false, tmp, false, 0, st));
     BTRY(spmm(b, kOpMU, tmp, f32, ph, f32, 0, st));
     BTRY(spmm(b, kOpA, ph, f32, v, false, 0, st));
     tile_dot_kernel<<<T, kRed, 0, st>>>(rh, v, nullptr, B);
     balpha_final<<<np, 64, 0, st>>>(B);
-    bs_kernel<<<T, kRed, 0, st>>>(r, v, s, B);
+    bs_kernel<<<T, kRed, 0, st>>>(r, v, s, x32, B);
     bsnorm_final<<<np, 64, 0, st>>>(B);
     if (f32) bxs_kernel<float><<<T, kRed, 0, st>>>(x, (const float *)ph, B);
     else bxs_kernel<double><<<T, kRed, 0, st>>>(x, (const double *)ph, B);
-    BTRY(spmm(b, kOpML, s, false, tmp, f32, 0, st));
+    BTRY(f32 ? spmm(b, kOpML, x32, true, tmp, true, 0, st) : spmm(b, kOpML, s, false, tmp, false, 0, st));
     BTRY(spmm(b, kOpMU, tmp, f32, sh, f32, 0, st));
     BTRY(spmm(b, kOpA, sh, f32, t, false, 0, st));
     tile_dot_kernel<<<T, kRed, 0, st>>>(t, s, t, B);
@@ -1490,9 +1524,9 @@ int autotune(isai_batch *b, hipStream_t st) {
             b->wg[slot] = kWgCand[(c / 2) % 3];
             b->remap[slot] = c % 2;
             // each product as the iteration runs it: with FP32 / BF16 factors M_L reads
-            // FP64 x and writes FP32, and A reads the FP32 p-hat / s-hat
+            // the FP32 copy of p / s and writes FP32, and A reads the FP32 p-hat / s-hat
             const bool lp = b->precision != ISAI_PREC_FP64;
-            const bool xf = slot == 0 && lp, yf = slot == 1 && lp;
+            const bool xf = lp, yf = slot == 1 && lp;
             status = spmm(b, op, X, xf, Y, yf, 1, st);  // warm-up
             if (status != ISAI_OK) break;
             if (hipEventRecord(e0, st) != hipSuccess) status = ISAI_ERR_HIP;
@@ -1601,6 +1635,7 @@ int persistent_solve(isai_batch *b, double tol, int maxit, const double *d_B, do
     a.sh = w + 6 * n;
     a.t = w + 7 * n;
     a.tmp = w + 8 * n;
+    a.n = n;
     a.tol = tol;
     a.relres = b->d_relres;
     const PersistFn fn = persist_fn(b->map[0], b->map[1], f.precision, b->kb[0], b->idx32);
