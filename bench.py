@@ -253,7 +253,7 @@ def bench_batching(stream, steps: int) -> dict:
            "spmm_autotune": {"variant": variant, "us": round(tuned_us, 1), "candidates": ncand,
                              "table_us": table,
                              "candidate_key": "c: lane mapping c//6 (0 row-pair, 1 column-split, 2 MFMA, "
-                                              "3 whole-row; the factor products run 3 as row-pair), "
+                                              "3 whole-row: row-pair lanes for FP64 operands, MFMA lanes for FP32/BF16 factors), "
                                               "workgroup (64,128,256)[(c//2)%3], XCD remap c%2"},
            "generate_s": round(gen_s, 2)}
     log(f"batching form: {ms:.1f} ms (persistent kernel {ms_p:.1f} ms) for {bt.npairs} pairs "

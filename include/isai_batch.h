@@ -51,7 +51,8 @@ int isai_batch_pairs(isai_batch_t batch, int *npairs);
 int isai_batch_tuning(isai_batch_t batch, int *variant, double *us, int *ncandidates);
 /* The autotuner's table for one operand (0 = A, 1 = the ISAI factors): us[c] = time
  * per product of candidate c = mapping (c / 6: 0 row-pair lanes, 1 column-split
- * lanes, 2 MFMA, 3 whole-row lanes; the factor products run 3 as row-pair),
+ * lanes, 2 MFMA, 3 whole-row: row-pair lanes for FP64 operands, MFMA lanes for
+ * FP32 / BF16 factors; each equals mapping 0 or 2 bit for bit),
  * workgroup size {64, 128, 256}[(c / 2) % 3], XCD remap c % 2; up to capacity
  * entries, *count = number of candidates (24). */
 int isai_batch_tuning_table(isai_batch_t batch, int operand, double *us, int capacity, int *count);

@@ -31,6 +31,19 @@ constexpr int kSpmmMaxWg = 256;  // largest SpMM workgroup the autotuner tries
 // 2 per pass and a software-pipelined pass of 1 were no faster, at 3-4 waves per SIMD)
 constexpr int kWholeRowRhs = 1;
 
+// Chunked XCD remap: the hardware deals consecutive workgroups round-robin over
+// the 8 XCDs; this hands each XCD runs of kXcdChunk consecutive logical workgroups
+// in turn (all XCDs still stream neighbouring regions at once, and the x entries a
+// run shares stay in one L2).  The tail past the last whole round keeps its order.
+constexpr int kXcdChunk = 64;
+__device__ __forceinline__ int xcd_chunk_remap(int bid, int nblocks) {
+    constexpr int kXcd = 8;
+    const int round = kXcd * kXcdChunk;
+    if (bid >= nblocks / round * round) return bid;
+    const int xcd = bid % kXcd, local = bid / kXcd;
+    return ((local / kXcdChunk) * kXcd + xcd) * kXcdChunk + local % kXcdChunk;
+}
+
 // status: 0 running, 1 converged (recursive residual), 2 converged on s with the
 // x update pending in this iteration, 3 breakdown, 4 done (true residual met),
 // 5 iteration limit reached
@@ -592,6 +605,102 @@ __device__ __forceinline__ void mfma_lowp_unit(int nb, const int *__restrict__ r
     }
 }
 
+// Whole-row form of mfma_lowp_unit for rows of up to KB blocks (wave-wide): lane
+// (i, b) loads the column indices of blocks i and i + 4 of its row (passed to the
+// quad by shuffle) and row i of every block up front, then one pass per group of
+// four right-hand sides issues all of the group's x loads before its MFMAs.  The
+// MFMA chain of each group runs over the same steps in the same order (zero
+// operands past a row's end) as mfma_lowp_unit, so the results are equal bit for
+// bit; longer rows take mfma_lowp_unit.
+template <bool BF16, int KB, class XT, class YT>
+__device__ __forceinline__ void mfma_lowp_unit_wr(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                  const void *__restrict__ val, const BatchDev &B,
+                                                  const XT *__restrict__ X, YT *__restrict__ Y, int u, int lane,
+                                                  int all) {
+    static_assert(KB <= 8, "two index loads per lane");
+    const int q = lane & 3, b = lane >> 2;
+    MfmaRow m;
+    mfma_row(B, rp, nb, u * kMfmaLowpRows + b, all, m);
+    int L = m.len, G = m.len ? (m.R + 3) >> 2 : 0;
+#pragma unroll
+    for (int off = 4; off < 64; off <<= 1) {
+        L = max(L, __shfl_xor(L, off));
+        G = max(G, __shfl_xor(G, off));
+    }
+    if (L == 0) return;
+    if (L > KB) {
+        mfma_lowp_unit<BF16, XT, YT>(nb, rp, ci, val, B, X, Y, u, lane, all);
+        return;
+    }
+    const int c0 = q < m.len ? ci[m.rs + q] - m.base : 0;
+    const int c1 = (KB > 4 && q + 4 < m.len) ? ci[m.rs + 4 + q] - m.base : 0;
+    float a[KB][4];
+#pragma unroll
+    for (int t = 0; t < KB; ++t) {
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) a[t][kk] = 0.f;
+        if (t < m.len) {
+            const int k = m.rs + t;
+            if (BF16) {
+                const uint2 w = ((const uint2 *)val)[(size_t)k * 4 + q];
+                a[t][0] = __uint_as_float(w.x << 16);
+                a[t][1] = __uint_as_float(w.x & 0xffff0000u);
+                a[t][2] = __uint_as_float(w.y << 16);
+                a[t][3] = __uint_as_float(w.y & 0xffff0000u);
+            } else {
+                const v4f w = ((const v4f *)val)[(size_t)k * 4 + q];
+                a[t][0] = w.x;
+                a[t][1] = w.y;
+                a[t][2] = w.z;
+                a[t][3] = w.w;
+            }
+        }
+    }
+    int col[KB];
+#pragma unroll
+    for (int t = 0; t < KB; ++t) col[t] = __shfl(t < 4 ? c0 : c1, (lane & ~3) + (t & 3)) * 4;
+    v4f acc[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) acc[g] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        if (g >= G) break;
+        const int n = 4 * g + q;
+        const XT *xn = X + m.v0 + n * m.step;
+        v4f xv[KB];
+#pragma unroll
+        for (int t = 0; t < KB; ++t)
+            xv[t] = (t < m.len && n < m.R) ? load4f(xn + col[t]) : v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < KB; ++t) {
+            if (t < L) {
+                acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t][0], xv[t].x, acc[g], 0, 0, 0);
+                acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t][1], xv[t].y, acc[g], 0, 0, 0);
+                acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t][2], xv[t].z, acc[g], 0, 0, 0);
+                acc[g] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t][3], xv[t].w, acc[g], 0, 0, 0);
+            }
+        }
+    }
+    if (!m.len) return;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const int n = 4 * g + q;
+        if (g < G && n < m.R && (m.mask & (1u << n))) store4f(Y + m.v0 + n * m.step + (size_t)(m.br - m.base) * 4, acc[g]);
+    }
+}
+
+template <bool BF16, int KB, class XT, class YT>
+__global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_lowp_wr_kernel(int nb, const int *__restrict__ rp,
+                                                                        const int *__restrict__ ci,
+                                                                        const void *__restrict__ val, BatchDev B,
+                                                                        const XT *__restrict__ X,
+                                                                        YT *__restrict__ Y, int remap, int all) {
+    const int wg = remap ? xcd_chunk_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    mfma_lowp_unit_wr<BF16, KB, XT, YT>(nb, rp, ci, val, B, X, Y,
+                                        wg * (int)(blockDim.x / 64) + (int)(threadIdx.x >> 6), (int)(threadIdx.x & 63),
+                                        all);
+}
+
 template <bool BF16, class XT, class YT>
 __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_mfma_lowp_kernel(int nb, const int *__restrict__ rp,
                                                                      const int *__restrict__ ci,
@@ -647,19 +756,6 @@ __device__ __forceinline__ void f64c_unit(int nb, const int *__restrict__ rp, co
     int r;
     if (!row_context(B, rp, nb, br, lane, all, q, r)) return;
     f64_rowc<KB, C, XT>(q, r, ci, val, X, Y);
-}
-
-// Chunked XCD remap: the hardware deals consecutive workgroups round-robin over
-// the 8 XCDs; this hands each XCD runs of kXcdChunk consecutive logical workgroups
-// in turn (all XCDs still stream neighbouring regions at once, and the x entries a
-// run shares stay in one L2).  The tail past the last whole round keeps its order.
-constexpr int kXcdChunk = 64;
-__device__ __forceinline__ int xcd_chunk_remap(int bid, int nblocks) {
-    constexpr int kXcd = 8;
-    const int round = kXcd * kXcdChunk;
-    if (bid >= nblocks / round * round) return bid;
-    const int xcd = bid % kXcd, local = bid / kXcd;
-    return ((local / kXcdChunk) * kXcd + xcd) * kXcdChunk + local % kXcdChunk;
 }
 
 template <int KB, int C, class XT>
@@ -1306,7 +1402,9 @@ struct isai_batch {
     static constexpr int kWork = 9;
     // launch shapes picked by the autotuner: [0] product with A, [1] with the ISAI factors
     int wg[2] = {256, 256}, remap[2] = {1, 1}, map[2] = {2, 2};
-    int kb[2] = {16, 16};  // whole-row mapping: blocks held per row (the operand's longest row, 8 / 10 / 12 / 16)
+    // whole-row mappings: blocks held per row, from the operand's longest row (A: 8 /
+    // 10 / 12 / 16; the ISAI factors: 4 / 6 / 8, longer rows take the MFMA kernel)
+    int kb[2] = {16, 8};
     bool idx32 = false;    // whole-row mapping: value indices and multi-vector offsets fit 32 bits
     int variant = 0, ncand = 0;
     double tuned_us = 0;
@@ -1371,8 +1469,8 @@ int launch_spmm(isai_batch *b, Operand op, int prec, const int *rp, const int *c
     const BatchDev B = b->dev();
     const int slot = op == kOpA ? 0 : 1;
     const int threads = b->wg[slot];
-    const int rows = MAP != 2 ? threads / 8
-                              : (threads / 64) * (prec == ISAI_PREC_FP64 ? kMfmaF64Rows : kMfmaLowpRows);
+    const bool mfma = MAP == 2 || (MAP == 3 && prec != ISAI_PREC_FP64);  // factor whole-row form is MFMA
+    const int rows = !mfma ? threads / 8 : (threads / 64) * (prec == ISAI_PREC_FP64 ? kMfmaF64Rows : kMfmaLowpRows);
     const unsigned grid = grid_for(b->nb, rows);
     const int remap = b->remap[slot];
     if (prec == ISAI_PREC_FP64) {
@@ -1406,6 +1504,22 @@ int launch_spmm(isai_batch *b, Operand op, int prec, const int *rp, const int *c
             bspmm_mfma_lowp_kernel<false, XT, YT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v, B, X, Y, remap, all);
         else
             bspmm_mfma_lowp_kernel<true, XT, YT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v, B, X, Y, remap, all);
+    } else if (MAP == 3) {
+        const int kb = b->kb[1];
+        const bool bf = prec == ISAI_PREC_BF16;
+#define ISAI_WR_LAUNCH(KB)                                                                                         \
+    if (bf)                                                                                                        \
+        bspmm_mfma_lowp_wr_kernel<true, KB, XT, YT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v, B, X, Y, remap, all); \
+    else                                                                                                           \
+        bspmm_mfma_lowp_wr_kernel<false, KB, XT, YT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v, B, X, Y, remap, all);
+        if (kb <= 4) {
+            ISAI_WR_LAUNCH(4)
+        } else if (kb <= 6) {
+            ISAI_WR_LAUNCH(6)
+        } else {
+            ISAI_WR_LAUNCH(8)
+        }
+#undef ISAI_WR_LAUNCH
     } else if (prec == ISAI_PREC_FP32) {
         bspmm_lowp_kernel<false, MAP == 1 ? 1 : 0, XT, YT>
             <<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v4u *)v, B, X, Y, remap, all);
@@ -1500,8 +1614,8 @@ int batch_iteration(isai_batch *b, double *x, hipStream_t st) {
 // (storage precision) time every (lane mapping, workgroup size, XCD remap)
 // candidate on the batch itself, all pairs active, and keep the fastest of each.
 // Candidate c: mapping c / 6 (0 row-pair lanes, 1 column-split lanes, 2 MFMA,
-// 3 whole-row lanes), workgroup kWgCand[(c / 2) % 3], remap c % 2.  The factor
-// products have no whole-row form: their candidates 18..23 run the row-pair mapping.
+// 3 whole-row: row-pair lanes for A, MFMA lanes for FP32 / BF16 factors, row-pair
+// for FP64 factors), workgroup kWgCand[(c / 2) % 3], remap c % 2 (3: chunked remap).
 constexpr int kWgCand[3] = {64, 128, kSpmmMaxWg};
 constexpr int kCandPerOp = 24;
 
@@ -1587,12 +1701,13 @@ PersistFn persist_fn_prec(int prec) {
 }
 
 // The persistent kernel holds the row-pair (0), MFMA (2) and whole-row (3, rows of
-// up to 10 blocks) mappings; a column-split choice (1), or a whole-row choice on
-// longer rows, runs as row-pair there.  The whole-row products equal the row-pair
+// up to 10 blocks) mappings for A; a column-split choice (1), or a whole-row choice
+// on longer rows, runs as row-pair there, and the factors' whole-row choice as MFMA.  The whole-row products equal the row-pair
 // ones bit for bit, and the whole-row A product fits the kernel's 128-register
 // budget without spilling (the row-pair one spills).
 PersistFn persist_fn(int map_a, int map_m, int prec, int kb_a, bool idx32) {
-    const bool mm = map_m == 2;
+    // the factors' whole-row form (3) equals the MFMA one (2) bit for bit
+    const bool mm = map_m == 2 || (map_m == 3 && prec != ISAI_PREC_FP64);
     if (map_a == 3 && idx32 && kb_a <= 10) {
         if (kb_a <= 8) return mm ? persist_fn_prec<3, 2, 8>(prec) : persist_fn_prec<3, 0, 8>(prec);
         return mm ? persist_fn_prec<3, 2, 10>(prec) : persist_fn_prec<3, 0, 10>(prec);
@@ -1823,6 +1938,20 @@ int isai_batch_setup(int nsys, const int *sys_ptr, const int *nrhs, int nnzb, co
     double total = 0;
     s = isai_setup_times(b->pre, &b->ms_ilu, &b->ms_isai, &total);
     if (s != ISAI_OK) return fail(s);
+    {  // the factors' longest row (their stored, possibly padded, patterns)
+        HandleFactors f;
+        s = handle_factors_internal(b->pre, &f);
+        if (s != ISAI_OK) return fail(s);
+        std::vector<int> lr(b->nb + 1), ur(b->nb + 1);
+        if (hipMemcpy(lr.data(), f.Lrow, sizeof(int) * (b->nb + 1), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(ur.data(), f.Urow, sizeof(int) * (b->nb + 1), hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("isai_batch_setup: reading the factor patterns failed");
+            return fail(ISAI_ERR_HIP);
+        }
+        int maxlen = 0;
+        for (int i = 0; i < b->nb; ++i) maxlen = std::max({maxlen, lr[i + 1] - lr[i], ur[i + 1] - ur[i]});
+        b->kb[1] = maxlen <= 4 ? 4 : (maxlen <= 6 ? 6 : 8);
+    }
     s = autotune(b, st);
     if (s != ISAI_OK) return fail(s);
     int dev = 0;

@@ -265,6 +265,23 @@ def test_batch_whole_row_product_other_stencils(stream, problem):
                 assert np.array_equal(y, row_pair), c
     finally:
         bt.close()
+    # FP32 factors: the whole-row MFMA factor products (candidate 18; rows longer than
+    # 8 blocks fall back per wave) give the MFMA ones' iterates (candidate 12) bit for
+    # bit, with the same A product (candidate 18) in both
+    bt = Batch(Matrix(rp, ci, val), sys_ptr, nrhs, precision="fp32", stream=stream)
+    try:
+        B = DeviceArray.from_numpy(bt.pack(_rhs(sys_ptr, nrhs, 9)))
+        xs = []
+        for c in (12, 18):
+            bt.select(0, 18)
+            bt.select(1, c)
+            d_x = DeviceArray.zeros((bt.length,), np.float64)
+            st, iters, rr = bt.solve(B, d_x, tol=1e-10, maxit=500)
+            assert st == 0 and (rr <= 1e-10).all(), c
+            xs.append((iters, d_x.to_numpy()))
+        assert np.array_equal(xs[0][0], xs[1][0]) and np.array_equal(xs[0][1], xs[1][1])
+    finally:
+        bt.close()
 
 
 def test_batch_full_size_config(stream):
