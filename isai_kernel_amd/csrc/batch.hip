@@ -5,9 +5,9 @@
 // Layout: pair p = (system s, right-hand side j) owns a contiguous vector segment
 // of 4*nb_s doubles, padded to a multiple of kTile = 1024; the segments of one
 // system follow each other with a fixed stride, and tile t of the multi-vector
-// belongs to one pair (tile_pair[t]).  Reductions are per tile (fixed shape), then
-// per pair over its contiguous tile range in order, so every result is
-// deterministic and independent of how many iterations one replay holds.
+// belongs to one pair (tile_pair[t]).  Reductions are per tile (one wave, fixed
+// shape), then per pair over its contiguous tile range in order, so every result
+// is deterministic and independent of how many iterations one replay holds.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -801,22 +801,22 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_lowp_kernel(int nb, const in
 }
 
 // --------------------------------------------------------------- reductions
-constexpr int kRed = 256;               // threads per tile
-constexpr int kPersistThreads = 1024;   // persistent kernel: four tile groups of kRed threads
+constexpr int kRed = 64;               // threads per tile: one wave
+constexpr int kTileWg = 256;           // per-step launches: four tiles per workgroup
+constexpr int kPersistThreads = 1024;  // persistent kernel: sixteen one-wave tile groups
 
-// Sum of v over the kRed threads of tile group `sub` (threads sub*kRed .. +kRed-1),
-// in a fixed order.  Every thread of the workgroup calls it the same number of
-// times (it holds two workgroup barriers).
-__device__ __forceinline__ double wg_sum(double v, int sub) {
-    __shared__ double ws[kPersistThreads / 64];
+// Sum of v over the wave that holds one tile, by a fixed butterfly (every lane gets
+// the same total): a tile's partial sums need no workgroup barrier, so a workgroup's
+// waves take their tiles independently.
+__device__ __forceinline__ double tile_sum(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-    __syncthreads();
-    const double *w = ws + (kRed / 64) * sub;
-    return (w[0] + w[1]) + (w[2] + w[3]);
+    return v;
 }
+
+// Tile of a thread in the per-step launches.
+__device__ __forceinline__ int launch_tile() { return (int)(blockIdx.x * (kTileWg / kRed) + (threadIdx.x >> 6)); }
+inline unsigned tile_grid(int ntiles) { return (unsigned)((ntiles + kTileWg / kRed - 1) / (kTileWg / kRed)); }
 
 // The per-pair finals run one wave per pair: the wave sums the pair's tile partials
 // lane-strided, then by a fixed butterfly, so every lane holds the same,
@@ -836,14 +836,14 @@ __device__ __forceinline__ void pair_sums(const BatchDev &B, int p, int lane, do
     s1 = c;
 }
 
-// Tile t of a running pair (t may lie past the last tile in the persistent kernel,
-// whose tile groups step through the tiles together).
+// Tile t of a running pair (t may lie past the last tile: the per-step launches
+// round the tile count up to whole workgroups).
 __device__ __forceinline__ bool tile_live(const BatchDev &B, int t) {
     return t < B.ntiles && B.ps[B.tile_pair[t]].status == 0;
 }
 
 // part0[t] = a.b and part1[t] = a.c (c optional) over tile t, for running pairs.
-__device__ __forceinline__ void tile_dot_body(int t, int sub, const double *__restrict__ a,
+__device__ __forceinline__ void tile_dot_body(int t, const double *__restrict__ a,
                                               const double *__restrict__ b, const double *__restrict__ c,
                                               const BatchDev &B) {
     const bool live = tile_live(B, t);  // the pair's finals skip it too
@@ -858,8 +858,8 @@ __device__ __forceinline__ void tile_dot_body(int t, int sub, const double *__re
             if (c) s1 = fma(a[i], c[i], s1);
         }
     }
-    s0 = wg_sum(s0, sub);
-    if (c) s1 = wg_sum(s1, sub);
+    s0 = tile_sum(s0);
+    if (c) s1 = tile_sum(s1);
     if (live && tid == 0) {
         B.part0[t] = s0;
         B.part1[t] = s1;
@@ -871,8 +871,9 @@ __device__ __forceinline__ void tile_dot_body(int t, int sub, const double *__re
 // the same recurrences as the single-system solver, with one set of scalars per pair.
 
 // r = b - r (r holds A x), rhat = r, p = v = 0 for running pairs; partials ||b||^2, ||r||^2
-__device__ __forceinline__ void binit_body(int t, int sub, const double *__restrict__ b, double *r, double *rh,
-                                           double *p, double *v, const BatchDev &B) {
+__device__ __forceinline__ void binit_body(int t, const double *__restrict__ b, double *__restrict__ r,
+                                           double *__restrict__ rh, double *__restrict__ p, double *__restrict__ v,
+                                           const BatchDev &B) {
     const bool live = tile_live(B, t);
     const int tid = threadIdx.x & (kRed - 1);
     const size_t base = (size_t)t * kTile;
@@ -890,8 +891,8 @@ __device__ __forceinline__ void binit_body(int t, int sub, const double *__restr
             sr = fma(ri, ri, sr);
         }
     }
-    sb = wg_sum(sb, sub);
-    sr = wg_sum(sr, sub);
+    sb = tile_sum(sb);
+    sr = tile_sum(sr);
     if (live && tid == 0) {
         B.part0[t] = sb;
         B.part1[t] = sr;
@@ -939,21 +940,26 @@ __device__ __forceinline__ void bp_body(int t, const double *__restrict__ r, dou
     const size_t base = (size_t)t * kTile;
     const double beta = S.beta, omega = S.omega;
     const bool first = S.first;
-    // every load of the tile before any store (the stores to p otherwise kept the
+    // every load of a half tile before its stores (the stores to p otherwise kept the
     // next element's loads waiting)
-    double rr[kTile / kRed], pp[kTile / kRed], vv[kTile / kRed];
+    constexpr int kHalf = kTile / kRed / 2;
 #pragma unroll
-    for (int k = 0; k < kTile / kRed; ++k) {
-        const size_t i = base + k * kRed + tid;
-        rr[k] = r[i];
-        pp[k] = first ? 0.0 : p[i];
-        vv[k] = first ? 0.0 : v[i];
-    }
+    for (int hh = 0; hh < 2; ++hh) {
+        double rr[kHalf], pp[kHalf], vv[kHalf];
 #pragma unroll
-    for (int k = 0; k < kTile / kRed; ++k) {
-        const double pk = first ? rr[k] : rr[k] + beta * (pp[k] - omega * vv[k]);
-        p[base + k * kRed + tid] = pk;
-        if (p32) p32[base + k * kRed + tid] = (float)pk;
+        for (int k = 0; k < kHalf; ++k) {
+            const size_t i = base + (hh * kHalf + k) * kRed + tid;
+            rr[k] = r[i];
+            pp[k] = first ? 0.0 : p[i];
+            vv[k] = first ? 0.0 : v[i];
+        }
+#pragma unroll
+        for (int k = 0; k < kHalf; ++k) {
+            const size_t i = base + (hh * kHalf + k) * kRed + tid;
+            const double pk = first ? rr[k] : rr[k] + beta * (pp[k] - omega * vv[k]);
+            p[i] = pk;
+            if (p32) p32[i] = (float)pk;
+        }
     }
 }
 
@@ -970,8 +976,8 @@ __device__ __forceinline__ void balpha_final_body(const BatchDev &B, int p, int 
 
 // s = r - alpha v (and its FP32 copy s32 with FP32 / BF16 factors, as in bp_body);
 // partial ||s||^2
-__device__ __forceinline__ void bs_body(int t, int sub, const double *__restrict__ r, const double *__restrict__ v,
-                                        double *s, float *__restrict__ s32, const BatchDev &B) {
+__device__ __forceinline__ void bs_body(int t, const double *__restrict__ r, const double *__restrict__ v,
+                                        double *__restrict__ s, float *__restrict__ s32, const BatchDev &B) {
     const bool live = tile_live(B, t);
     const int tid = threadIdx.x & (kRed - 1);
     const size_t base = (size_t)t * kTile;
@@ -987,7 +993,7 @@ __device__ __forceinline__ void bs_body(int t, int sub, const double *__restrict
             acc = fma(si, si, acc);
         }
     }
-    acc = wg_sum(acc, sub);
+    acc = tile_sum(acc);
     if (live && tid == 0) B.part0[t] = acc;
 }
 
@@ -1001,7 +1007,7 @@ __device__ __forceinline__ void bsnorm_final_body(const BatchDev &B, int p, int 
 
 // pairs converged on s: x += alpha ph
 template <class PT>
-__device__ __forceinline__ void bxs_body(int t, double *x, const PT *__restrict__ ph, const BatchDev &B) {
+__device__ __forceinline__ void bxs_body(int t, double *__restrict__ x, const PT *__restrict__ ph, const BatchDev &B) {
     if (t >= B.ntiles) return;
     const PairState &S = B.ps[B.tile_pair[t]];
     if (S.status != 2) return;
@@ -1028,7 +1034,8 @@ __device__ __forceinline__ void bomega_final_body(const BatchDev &B, int p, int 
 // x += alpha ph + omega sh; r = s - omega t; partials ||r||^2 and rhat.r (the next
 // iteration's rho)
 template <class PT>
-__device__ __forceinline__ void bxr_body(int t, int sub, double *x, double *r, const double *__restrict__ rh,
+__device__ __forceinline__ void bxr_body(int t, double *__restrict__ x, double *__restrict__ r,
+                                         const double *__restrict__ rh,
                                          const PT *__restrict__ ph, const PT *__restrict__ sh,
                                          const double *__restrict__ s, const double *__restrict__ tv,
                                          const BatchDev &B) {
@@ -1049,8 +1056,8 @@ __device__ __forceinline__ void bxr_body(int t, int sub, double *x, double *r, c
             rho = fma(rh[i], ri, rho);
         }
     }
-    acc = wg_sum(acc, sub);
-    rho = wg_sum(rho, sub);
+    acc = tile_sum(acc);
+    rho = tile_sum(rho);
     if (live && tid == 0) {
         B.part0[t] = acc;
         B.part1[t] = rho;
@@ -1072,7 +1079,7 @@ __device__ __forceinline__ void brnorm_final_body(const BatchDev &B, int p, int 
 }
 
 // true residual of every pair: r = b - r (r holds A x); part0 = ||r||^2, part1 = ||b||^2
-__device__ __forceinline__ void bres_body(int t, int sub, const double *__restrict__ b, double *r,
+__device__ __forceinline__ void bres_body(int t, const double *__restrict__ b, double *__restrict__ r,
                                           const BatchDev &B) {
     const bool live = t < B.ntiles;
     const int tid = threadIdx.x & (kRed - 1);
@@ -1088,8 +1095,8 @@ __device__ __forceinline__ void bres_body(int t, int sub, const double *__restri
             sb = fma(bi, bi, sb);
         }
     }
-    sr = wg_sum(sr, sub);
-    sb = wg_sum(sb, sub);
+    sr = tile_sum(sr);
+    sb = tile_sum(sb);
     if (live && tid == 0) {
         B.part0[t] = sr;
         B.part1[t] = sb;
@@ -1123,35 +1130,35 @@ __device__ __forceinline__ void breset_body(const BatchDev &B, int p) {
 // ---- one launch per step (the eager and graph-replayed paths)
 __global__ void zero_active_kernel(Ctl *ctl) { ctl->nactive = 0; }
 
-__global__ __launch_bounds__(kRed) void tile_dot_kernel(const double *__restrict__ a, const double *__restrict__ b,
+__global__ __launch_bounds__(kTileWg) void tile_dot_kernel(const double *__restrict__ a, const double *__restrict__ b,
                                                         const double *__restrict__ c, BatchDev B) {
-    tile_dot_body(blockIdx.x, 0, a, b, c, B);
+    tile_dot_body(launch_tile(), a, b, c, B);
 }
-__global__ __launch_bounds__(kRed) void binit_kernel(const double *__restrict__ b, double *r, double *rh, double *p,
+__global__ __launch_bounds__(kTileWg) void binit_kernel(const double *__restrict__ b, double *r, double *rh, double *p,
                                                      double *v, BatchDev B) {
-    binit_body(blockIdx.x, 0, b, r, rh, p, v, B);
+    binit_body(launch_tile(), b, r, rh, p, v, B);
 }
-__global__ __launch_bounds__(kRed) void bp_kernel(const double *__restrict__ r, double *__restrict__ p,
+__global__ __launch_bounds__(kTileWg) void bp_kernel(const double *__restrict__ r, double *__restrict__ p,
                                                   const double *__restrict__ v, float *p32, BatchDev B) {
-    bp_body(blockIdx.x, r, p, v, p32, B);
+    bp_body(launch_tile(), r, p, v, p32, B);
 }
-__global__ __launch_bounds__(kRed) void bs_kernel(const double *__restrict__ r, const double *__restrict__ v, double *s,
+__global__ __launch_bounds__(kTileWg) void bs_kernel(const double *__restrict__ r, const double *__restrict__ v, double *s,
                                                   float *s32, BatchDev B) {
-    bs_body(blockIdx.x, 0, r, v, s, s32, B);
+    bs_body(launch_tile(), r, v, s, s32, B);
 }
 template <class PT>
-__global__ __launch_bounds__(kRed) void bxs_kernel(double *x, const PT *__restrict__ ph, BatchDev B) {
-    bxs_body(blockIdx.x, x, ph, B);
+__global__ __launch_bounds__(kTileWg) void bxs_kernel(double *x, const PT *__restrict__ ph, BatchDev B) {
+    bxs_body(launch_tile(), x, ph, B);
 }
 template <class PT>
-__global__ __launch_bounds__(kRed) void bxr_kernel(double *x, double *r, const double *__restrict__ rh,
+__global__ __launch_bounds__(kTileWg) void bxr_kernel(double *x, double *r, const double *__restrict__ rh,
                                                    const PT *__restrict__ ph, const PT *__restrict__ sh,
                                                    const double *__restrict__ s, const double *__restrict__ tv,
                                                    BatchDev B) {
-    bxr_body(blockIdx.x, 0, x, r, rh, ph, sh, s, tv, B);
+    bxr_body(launch_tile(), x, r, rh, ph, sh, s, tv, B);
 }
-__global__ __launch_bounds__(kRed) void bres_kernel(const double *__restrict__ b, double *r, BatchDev B) {
-    bres_body(blockIdx.x, 0, b, r, B);
+__global__ __launch_bounds__(kTileWg) void bres_kernel(const double *__restrict__ b, double *r, BatchDev B) {
+    bres_body(launch_tile(), b, r, B);
 }
 __global__ __launch_bounds__(64) void binit_final(BatchDev B, double tol) {
     binit_final_body(B, blockIdx.x, threadIdx.x, tol);
@@ -1273,9 +1280,9 @@ __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(Persi
     IT *const ph = (IT *)a.ph, *const sh = (IT *)a.sh, *const tmp = (IT *)a.tmp;
     // FP32 copy of p / s for M_L (second half of tmp's region), FP32 / BF16 factors only
     float *const x32 = PRECM == ISAI_PREC_FP64 ? nullptr : (float *)a.tmp + a.n;
-    // tile phases: the workgroup's tile groups take tiles t0 + sub together, so
-    // every group meets the same number of reduction barriers
-    auto tiles_end = [&](int t0) { return t0 < B.ntiles; };
+    // tile phases: wave `sub` of the workgroup takes tiles t0 + sub (a tile's sums
+    // stay inside its wave, so the waves need not step together)
+    auto tiles_end = [&](int t0) { return t0 + sub < B.ntiles; };
     auto A = [&](const auto *X, double *Y, int all) {
         spmm_phase<MAPA, ISAI_PREC_FP64, KBA>(a.nb, a.rp, a.ci, a.val, B, X, Y, all);
     };
@@ -1302,7 +1309,7 @@ __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(Persi
         // (re)start the running pairs from their current x
         A(a.x, a.r, 0);
         ISAI_SYNC();
-        ISAI_TILES(binit_body(t0 + sub, sub, a.b, a.r, a.rh, a.p, a.v, B));
+        ISAI_TILES(binit_body(t0 + sub, a.b, a.r, a.rh, a.p, a.v, B));
         ISAI_SYNC();
         ISAI_PAIRS({
             binit_final_body(B, p, lane, a.tol);
@@ -1321,11 +1328,11 @@ __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(Persi
             ISAI_SYNC();
             A(ph, a.v, 0);
             ISAI_SYNC();
-            ISAI_TILES(tile_dot_body(t0 + sub, sub, a.rh, a.v, nullptr, B));
+            ISAI_TILES(tile_dot_body(t0 + sub, a.rh, a.v, nullptr, B));
             ISAI_SYNC();
             ISAI_PAIRS(balpha_final_body(B, p, lane));
             ISAI_SYNC();
-            ISAI_TILES(bs_body(t0 + sub, sub, a.r, a.v, a.s, x32, B));
+            ISAI_TILES(bs_body(t0 + sub, a.r, a.v, a.s, x32, B));
             ISAI_SYNC();
             ISAI_PAIRS(bsnorm_final_body(B, p, lane));
             ISAI_SYNC();
@@ -1338,11 +1345,11 @@ __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(Persi
             ISAI_SYNC();
             A(sh, a.t, 0);
             ISAI_SYNC();
-            ISAI_TILES(tile_dot_body(t0 + sub, sub, a.t, a.s, a.t, B));
+            ISAI_TILES(tile_dot_body(t0 + sub, a.t, a.s, a.t, B));
             ISAI_SYNC();
             ISAI_PAIRS(bomega_final_body(B, p, lane));
             ISAI_SYNC();
-            ISAI_TILES(bxr_body(t0 + sub, sub, a.x, a.r, a.rh, ph, sh, a.s, a.t, B));
+            ISAI_TILES(bxr_body(t0 + sub, a.x, a.r, a.rh, ph, sh, a.s, a.t, B));
             ISAI_SYNC();
             // end of this iteration and rho / beta of the next
             ISAI_PAIRS({
@@ -1358,7 +1365,7 @@ __global__ __launch_bounds__(kPersistThreads) void batch_persistent_kernel(Persi
         // true FP64 residual of every pair; pairs that miss tol restart
         A(a.x, a.r, 1);
         ISAI_SYNC();
-        ISAI_TILES(bres_body(t0 + sub, sub, a.b, a.r, B));
+        ISAI_TILES(bres_body(t0 + sub, a.b, a.r, B));
         ISAI_SYNC();
         ISAI_PAIRS(bres_final_body(B, p, lane, a.tol, a.relres));
         ISAI_SYNC();
@@ -1586,25 +1593,25 @@ int batch_iteration(isai_batch *b, double *x, hipStream_t st) {
     const int np = b->npairs, T = b->ntiles;
     zero_active_kernel<<<1, 1, 0, st>>>(b->d_ctl);
     brho_final<<<np, 64, 0, st>>>(B);
-    bp_kernel<<<T, kRed, 0, st>>>(r, p, v, x32, B);
+    bp_kernel<<<tile_grid(T), kTileWg, 0, st>>>(r, p, v, x32, B);
     BTRY(f32 ? spmm(b, kOpML, x32, true, tmp, true, 0, st) : spmm(b, kOpML, p, false, tmp, false, 0, st));
     BTRY(spmm(b, kOpMU, tmp, f32, ph, f32, 0, st));
     BTRY(spmm(b, kOpA, ph, f32, v, false, 0, st));
-    tile_dot_kernel<<<T, kRed, 0, st>>>(rh, v, nullptr, B);
+    tile_dot_kernel<<<tile_grid(T), kTileWg, 0, st>>>(rh, v, nullptr, B);
     balpha_final<<<np, 64, 0, st>>>(B);
-    bs_kernel<<<T, kRed, 0, st>>>(r, v, s, x32, B);
+    bs_kernel<<<tile_grid(T), kTileWg, 0, st>>>(r, v, s, x32, B);
     bsnorm_final<<<np, 64, 0, st>>>(B);
-    if (f32) bxs_kernel<float><<<T, kRed, 0, st>>>(x, (const float *)ph, B);
-    else bxs_kernel<double><<<T, kRed, 0, st>>>(x, (const double *)ph, B);
+    if (f32) bxs_kernel<float><<<tile_grid(T), kTileWg, 0, st>>>(x, (const float *)ph, B);
+    else bxs_kernel<double><<<tile_grid(T), kTileWg, 0, st>>>(x, (const double *)ph, B);
     BTRY(f32 ? spmm(b, kOpML, x32, true, tmp, true, 0, st) : spmm(b, kOpML, s, false, tmp, false, 0, st));
     BTRY(spmm(b, kOpMU, tmp, f32, sh, f32, 0, st));
     BTRY(spmm(b, kOpA, sh, f32, t, false, 0, st));
-    tile_dot_kernel<<<T, kRed, 0, st>>>(t, s, t, B);
+    tile_dot_kernel<<<tile_grid(T), kTileWg, 0, st>>>(t, s, t, B);
     bomega_final<<<np, 64, 0, st>>>(B);
     if (f32)
-        bxr_kernel<float><<<T, kRed, 0, st>>>(x, r, rh, (const float *)ph, (const float *)sh, s, t, B);
+        bxr_kernel<float><<<tile_grid(T), kTileWg, 0, st>>>(x, r, rh, (const float *)ph, (const float *)sh, s, t, B);
     else
-        bxr_kernel<double><<<T, kRed, 0, st>>>(x, r, rh, (const double *)ph, (const double *)sh, s, t, B);
+        bxr_kernel<double><<<tile_grid(T), kTileWg, 0, st>>>(x, r, rh, (const double *)ph, (const double *)sh, s, t, B);
     brnorm_final<<<np, 64, 0, st>>>(B);
     ISAI_LAUNCH_CHECK();
     return ISAI_OK;
@@ -2063,7 +2070,7 @@ int isai_batch_solve(isai_batch_t b, double tol, int maxit, const double *d_B, d
     for (int round = 0; round <= kMaxRestarts; ++round) {
         // (re)start the running pairs from their current x
         BTRY(spmm(b, kOpA, d_X, r, 0, st));
-        binit_kernel<<<T, kRed, 0, st>>>(d_B, r, rh, p, v, B);
+        binit_kernel<<<tile_grid(T), kTileWg, 0, st>>>(d_B, r, rh, p, v, B);
         binit_final<<<np, 64, 0, st>>>(B, tol);
         ISAI_LAUNCH_CHECK();
         for (int rep = 0; rep < max_replays; ++rep) {
@@ -2078,7 +2085,7 @@ int isai_batch_solve(isai_batch_t b, double tol, int maxit, const double *d_B, d
         }
         // true FP64 residual of every pair; pairs that miss tol restart
         BTRY(spmm(b, kOpA, d_X, r, 1, st));
-        bres_kernel<<<T, kRed, 0, st>>>(d_B, r, B);
+        bres_kernel<<<tile_grid(T), kTileWg, 0, st>>>(d_B, r, B);
         bres_final<<<np, 64, 0, st>>>(B, tol, b->d_relres);
         ISAI_LAUNCH_CHECK();
         ISAI_HIP(hipMemcpyAsync(hs.data(), b->d_ps, sizeof(PairState) * np, hipMemcpyDeviceToHost, st));
