@@ -27,9 +27,6 @@ constexpr int kMaxRhs = 16;
 constexpr int kItersPerReplay = 4;  // iterations in one captured graph
 constexpr int kMaxRestarts = 8;
 constexpr int kSpmmMaxWg = 256;  // largest SpMM workgroup the autotuner tries
-// Whole-row mapping: right-hand sides per pass (measured on the 256-system 9-pt batch:
-// 2 per pass and a software-pipelined pass of 1 were no faster, at 3-4 waves per SIMD)
-constexpr int kWholeRowRhs = 1;
 
 // Chunked XCD remap: the hardware deals consecutive workgroups round-robin over
 // the 8 XCDs; this hands each XCD runs of kXcdChunk consecutive logical workgroups
@@ -320,14 +317,19 @@ __device__ __forceinline__ void col_row(const RowCtx &q, const int *__restrict__
 // row's column indices and value lines loaded once, up front -- lane s loads the
 // indices of blocks s and s + 8 (passed to the other lanes by shuffle) and every
 // value line of the row -- so a lane group holds a whole row of up to KB blocks in
-// flight instead of one block.  The right-hand sides are then taken C at a time:
-// all x gathers of a chunk are issued before its products, and the registers do
-// not grow with R (f64_row holds 4R accumulator registers per lane).  Per (row,
-// right-hand side) the products accumulate block by block into the same two
-// partial sums as f64_row, so the results equal the row-pair mapping's bit for
-// bit.  Rows longer than KB take the chunks block by block, reloading the values
-// per chunk.
-template <int KB, int C, class XT>
+// flight instead of one block.  The right-hand sides are then taken four per pass
+// with one gather per block: lane s (row s/2, columns 2h, 2h + 1 with h = s % 2)
+// loads its half of block column c for right-hand side j0 + s/2, and takes the
+// half of right-hand side j0 + u from lane 2u + h of its group by shuffle.  f64_row
+// issues one gather per block and right-hand side, which the texture address path
+// does not sustain (four of the eight lanes load the same 8 bytes); here each lane
+// loads distinct bytes and the gathers per block fall four-fold.  Registers do not
+// grow with R.  Per (row, right-hand side) the products accumulate block by block
+// into the same two partial sums as f64_row, so the results equal the row-pair
+// mapping's bit for bit.  X4 = false takes one right-hand side per pass instead
+// (fewer registers).  Rows longer than KB take the right-hand sides one at a time,
+// block by block, reloading the values per right-hand side.
+template <int KB, class XT, bool X4 = true>
 __device__ __forceinline__ void f64_rowc(const RowCtx &q, int R, const int *__restrict__ ci,
                                          const v2d *__restrict__ val, const XT *__restrict__ X,
                                          double *__restrict__ Y) {
@@ -352,59 +354,70 @@ __device__ __forceinline__ void f64_rowc(const RowCtx &q, int R, const int *__re
         unsigned col[KB];
 #pragma unroll
         for (int t = 0; t < KB; ++t) col[t] = xo + 4u * (unsigned)__shfl(t < 8 ? c0 : c1, g0 + (t & 7));
-        for (int j0 = 0; j0 < R; j0 += C) {
-            XP x[KB][C];  // held in storage precision until the products
+        if (!X4) {
+            // one right-hand side per pass, every gather of the pass issued before its
+            // products (fewer registers: the persistent kernel's form)
+            for (int j = 0; j < R; ++j) {
+                XP x[KB];
+#pragma unroll
+                for (int t = 0; t < KB; ++t) x[t] = t < len ? *(const XP *)&X[col[t] + (unsigned)j * st] : XP{0, 0};
+                double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+                for (int t = 0; t < KB; ++t) {
+                    if (t < len) {
+                        acc0 = fma(a[t].x, (double)x[t].x, acc0);
+                        acc1 = fma(a[t].y, (double)x[t].y, acc1);
+                    }
+                }
+                double sum = acc0 + acc1;
+                sum += __shfl_xor(sum, 1);
+                if ((q.mask & (1u << j)) && h == 0) yr[(size_t)j * q.step] = sum;
+            }
+            return;
+        }
+        for (int j0 = 0; j0 < R; j0 += 4) {
+            const int jl = j0 + (q.lane >> 1);  // the right-hand side this lane gathers
+            XP pc[KB];                          // held in storage precision until the products
 #pragma unroll
             for (int t = 0; t < KB; ++t)
+                pc[t] = (t < len && jl < R) ? *(const XP *)&X[col[t] + (unsigned)jl * st] : XP{0, 0};
+            double acc0[4], acc1[4];
 #pragma unroll
-                for (int u = 0; u < C; ++u)
-                    x[t][u] = (t < len && j0 + u < R) ? *(const XP *)&X[col[t] + (unsigned)(j0 + u) * st] : XP{0, 0};
-            double acc0[C], acc1[C];
-#pragma unroll
-            for (int u = 0; u < C; ++u) acc0[u] = acc1[u] = 0.0;
+            for (int u = 0; u < 4; ++u) acc0[u] = acc1[u] = 0.0;
 #pragma unroll
             for (int t = 0; t < KB; ++t) {
                 if (t < len) {
 #pragma unroll
-                    for (int u = 0; u < C; ++u) {
-                        acc0[u] = fma(a[t].x, (double)x[t][u].x, acc0[u]);
-                        acc1[u] = fma(a[t].y, (double)x[t][u].y, acc1[u]);
+                    for (int u = 0; u < 4; ++u) {
+                        const int src = g0 + 2 * u + h;
+                        const double vx = (double)__shfl(pc[t].x, src), vy = (double)__shfl(pc[t].y, src);
+                        acc0[u] = fma(a[t].x, vx, acc0[u]);
+                        acc1[u] = fma(a[t].y, vy, acc1[u]);
                     }
                 }
             }
 #pragma unroll
-            for (int u = 0; u < C; ++u) {
-                double s = acc0[u] + acc1[u];
-                s += __shfl_xor(s, 1);
+            for (int u = 0; u < 4; ++u) {
+                double sum = acc0[u] + acc1[u];
+                sum += __shfl_xor(sum, 1);
                 const int j = j0 + u;
-                if (j < R && (q.mask & (1u << j)) && h == 0) yr[(size_t)j * q.step] = s;
+                if (j < R && (q.mask & (1u << j)) && h == 0) yr[(size_t)j * q.step] = sum;
             }
         }
         return;
     }
-    for (int j0 = 0; j0 < R; j0 += C) {
-        double acc0[C], acc1[C];
-#pragma unroll
-        for (int u = 0; u < C; ++u) acc0[u] = acc1[u] = 0.0;
+    for (int j = 0; j < R; ++j) {
+        double acc0 = 0.0, acc1 = 0.0;
         for (int k = q.rs; k < q.re; ++k) {
             const unsigned c = xo + 4u * (unsigned)(ci[k] - q.base);
             const v2d av = val[(unsigned)k * 8u + q.lane];
-            XP x[C];
-#pragma unroll
-            for (int u = 0; u < C; ++u) x[u] = j0 + u < R ? *(const XP *)&X[c + (unsigned)(j0 + u) * st] : XP{0, 0};
-#pragma unroll
-            for (int u = 0; u < C; ++u) {
-                acc0[u] = fma(av.x, (double)x[u].x, acc0[u]);
-                acc1[u] = fma(av.y, (double)x[u].y, acc1[u]);
-            }
+            const XP x = *(const XP *)&X[c + (unsigned)j * st];
+            acc0 = fma(av.x, (double)x.x, acc0);
+            acc1 = fma(av.y, (double)x.y, acc1);
         }
-#pragma unroll
-        for (int u = 0; u < C; ++u) {
-            double s = acc0[u] + acc1[u];
-            s += __shfl_xor(s, 1);
-            const int j = j0 + u;
-            if (j < R && (q.mask & (1u << j)) && h == 0) yr[(size_t)j * q.step] = s;
-        }
+        double sum = acc0 + acc1;
+        sum += __shfl_xor(sum, 1);
+        if ((q.mask & (1u << j)) && h == 0) yr[(size_t)j * q.step] = sum;
     }
 }
 
@@ -747,25 +760,25 @@ __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64_kernel(int nb, const int
                       (int)(threadIdx.x & 7), all);
 }
 
-// MAP 3: whole-row mapping, KB blocks up front, C right-hand sides per chunk.
-template <int KB, int C, class XT>
+// MAP 3: whole-row mapping, KB blocks up front.
+template <int KB, class XT, bool X4 = true>
 __device__ __forceinline__ void f64c_unit(int nb, const int *__restrict__ rp, const int *__restrict__ ci,
                                           const v2d *__restrict__ val, const BatchDev &B, const XT *__restrict__ X,
                                           double *__restrict__ Y, int br, int lane, int all) {
     RowCtx q;
     int r;
     if (!row_context(B, rp, nb, br, lane, all, q, r)) return;
-    f64_rowc<KB, C, XT>(q, r, ci, val, X, Y);
+    f64_rowc<KB, XT, X4>(q, r, ci, val, X, Y);
 }
 
-template <int KB, int C, class XT>
+template <int KB, class XT>
 __global__ __launch_bounds__(kSpmmMaxWg) void bspmm_f64c_kernel(int nb, const int *__restrict__ rp,
                                                                 const int *__restrict__ ci,
                                                                 const v2d *__restrict__ val, BatchDev B,
                                                                 const XT *__restrict__ X, double *__restrict__ Y,
                                                                 int remap, int all) {
     const int wg = remap ? xcd_chunk_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    f64c_unit<KB, C, XT>(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3),
+    f64c_unit<KB, XT>(nb, rp, ci, val, B, X, Y, wg * (int)(blockDim.x / 8) + (int)(threadIdx.x >> 3),
                          (int)(threadIdx.x & 7), all);
 }
 
@@ -1259,7 +1272,7 @@ __device__ __forceinline__ void spmm_phase(int nb, const int *rp, const int *ci,
         const int gpg = kPersistThreads / 8;
         for (int br = blockIdx.x * gpg + (int)(threadIdx.x >> 3); br < nb; br += gridDim.x * gpg) {
             if constexpr (PREC == ISAI_PREC_FP64 && MAP == 3)
-                f64c_unit<KB, kWholeRowRhs, XT>(nb, rp, ci, (const v2d *)val, B, X, Y, br, lane & 7, all);
+                f64c_unit<KB, XT, false>(nb, rp, ci, (const v2d *)val, B, X, Y, br, lane & 7, all);
             else if constexpr (PREC == ISAI_PREC_FP64)
                 f64_unit<MAP, XT>(nb, rp, ci, (const v2d *)val, B, X, Y, br, lane & 7, all);
             else
@@ -1485,18 +1498,17 @@ int launch_spmm(isai_batch *b, Operand op, int prec, const int *rp, const int *c
             if (MAP == 2) {
                 bspmm_mfma_f64_kernel<XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const double *)v, B, X, Y, remap,
                                                                     all);
-            } else if (MAP >= 3 && b->idx32) {
-                constexpr int C = kWholeRowRhs;
+            } else if (MAP == 3 && b->idx32) {
                 const int kb = b->kb[slot];
                 const v2d *v2 = (const v2d *)v;
                 if (kb <= 8)
-                    bspmm_f64c_kernel<8, C, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v2, B, X, Y, remap, all);
+                    bspmm_f64c_kernel<8, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v2, B, X, Y, remap, all);
                 else if (kb <= 10)
-                    bspmm_f64c_kernel<10, C, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v2, B, X, Y, remap, all);
+                    bspmm_f64c_kernel<10, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v2, B, X, Y, remap, all);
                 else if (kb <= 12)
-                    bspmm_f64c_kernel<12, C, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v2, B, X, Y, remap, all);
+                    bspmm_f64c_kernel<12, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v2, B, X, Y, remap, all);
                 else
-                    bspmm_f64c_kernel<16, C, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v2, B, X, Y, remap, all);
+                    bspmm_f64c_kernel<16, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, v2, B, X, Y, remap, all);
             } else {
                 // row-pair for MAP 0 and for the whole-row mappings beyond 32-bit offsets
                 bspmm_f64_kernel<MAP == 1 ? 1 : 0, XT><<<grid, threads, 0, st>>>(b->nb, rp, ci, (const v2d *)v, B, X,
