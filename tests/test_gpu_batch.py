@@ -111,8 +111,9 @@ def test_batch_graph_replay_equals_eager(stream, precision):
     print(f"batch schedules 9 systems L 6..24 {precision}", flush=True)
     (rp, ci, val, sys_ptr, nrhs), bt = _make(9, 6, 24, 3, 16, precision, stream)
     try:
-        # the persistent kernel holds the row-pair and MFMA lane mappings (a
-        # column-split choice runs row-pair there): keep the fastest of those
+        # the persistent kernel holds the row-pair, MFMA and whole-row lane mappings
+        # or forms equal to them bit for bit (a column-split choice runs row-pair
+        # there): keep the fastest of those
         for op in (0, 1):
             tab = bt.tuning_table(op)
             bt.select(op, min((c for c in range(len(tab)) if c // 6 != 1), key=lambda c: tab[c]))
