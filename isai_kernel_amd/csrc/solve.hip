@@ -51,7 +51,8 @@ struct Scalars {
     int iters, maxit;  // BiCGSTAB iteration count and limit (device side)
     int jend;          // FGMRES: Arnoldi steps of the current cycle
     int reorth;        // FGMRES: 1 when the current step needs the second Gram-Schmidt pass
-    int pad[2];
+    int xzero;         // mixed FGMRES: 1 when the initial guess is zero everywhere
+    int pad;
     double wnorm2;     // FGMRES: ||w||^2 before the first Gram-Schmidt pass
     double partial[2][kRedBlocks];
     double mpart[(kMaxRestart + 1) * kRedBlocks];  // multi-dot partials, [vector][block]
@@ -377,11 +378,14 @@ __global__ __launch_bounds__(kRedThreads) void bicg_rnorm_final(Scalars *S) {
 
 // ------------------------------------------------------------- residual
 // r = b - r (r holds A x); partial ||r||^2, ||b||^2
+// r = b - r (r holds A x); with xz set (x was zero everywhere, so the product that
+// would have written r was skipped) r = b, the same bits b - A*0 gives
 __global__ __launch_bounds__(kRedThreads) void residual_kernel(size_t n, const double *__restrict__ b, double *r,
-                                                               Scalars *S) {
+                                                               Scalars *S, const int *xz = nullptr) {
+    const bool zero = xz && *xz;
     double sr = 0.0, sb = 0.0;
     for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (size_t)kRedBlocks * kRedThreads) {
-        const double bi = b[i], ri = bi - r[i];
+        const double bi = b[i], ri = zero ? bi : bi - r[i];
         r[i] = ri;
         sr += ri * ri;
         sb += bi * bi;
@@ -392,6 +396,15 @@ __global__ __launch_bounds__(kRedThreads) void residual_kernel(size_t n, const d
         S->partial[0][blockIdx.x] = sr;
         S->partial[1][blockIdx.x] = sb;
     }
+}
+
+// S->xzero = 1 when x is zero everywhere (-0 counts as zero), else 0: a one-thread
+// reset, then a scan in which any wave that meets a nonzero stores 0
+__global__ void xzero_reset_kernel(Scalars *S) { S->xzero = 1; }
+__global__ __launch_bounds__(256) void xzero_scan_kernel(size_t n, const double *__restrict__ x, Scalars *S) {
+    bool nz = false;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) nz |= x[i] != 0.0;
+    if (__ballot(nz) && (threadIdx.x & 63) == 0) S->xzero = 0;
 }
 
 __global__ __launch_bounds__(kRedThreads) void residual_final(Scalars *S, double *out2) {
@@ -1379,9 +1392,17 @@ int fgmres_mixed(isai_handle *h, int m, double tol, int maxit, const double *b, 
         launched_est = inner_total;
         return ISAI_OK;
     };
+    bool first_cycle = true;
     while (launched_est < maxit) {
-        TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, x, w, st, nullptr));
-        residual_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, b, w, h->S);
+        // the first cycle's product is skipped on the device when x is zero (r = b)
+        const int *xz = first_cycle ? &h->S->xzero : nullptr;
+        if (first_cycle) {
+            xzero_reset_kernel<<<1, 1, 0, st>>>(h->S);
+            xzero_scan_kernel<<<vec_grid(n), 256, 0, st>>>(n, x, h->S);
+        }
+        first_cycle = false;
+        TRY(spmv_launch(h->nb, h->nnzb, h->bs, ISAI_PREC_FP64, h->d_row, h->d_col, h->d_val, x, w, st, xz));
+        residual_kernel<<<kRedBlocks, kRedThreads, 0, st>>>(n, b, w, h->S, xz);
         fg_beta_final<<<1, kRedThreads, 0, st>>>(h->S, tol);
         fg_scale_kernel<<<vec_grid(n), 256, 0, st>>>(n, w, V, h->S, 0, 0, 1);
         ISAI_LAUNCH_CHECK();
